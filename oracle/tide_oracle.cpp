@@ -1,0 +1,953 @@
+// ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load this library, and only as the checker; the product never links it.
+//
+// A CPU restatement (scalar C++, no OpenMP) of the reference TideGBM histogram-GBDT path,
+// restricted to what the device path covers: numeric features, the MSE and within-date
+// pairwise objectives, TORB time buckets, GOSS, PAIR splits, best-first growth, leaf refit,
+// predict and update.  Every function cites the reference file:line it restates.
+// It is pinned bit-for-bit against the reference core itself (oracle/_ref, built from the
+// reference sources by oracle/Makefile.ref) on seeded panels; see tests/test_oracle_pin.py.
+//
+// Compiled with the reference's flags (-O3 -std=c++17 -fno-math-errno, x86-64 baseline: no FMA
+// contraction is possible) so that sequential FP64 sums and glibc exp match the reference.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace orc {
+
+using i64 = long long;
+
+struct Cfg {  // defaults: reference cpp/include/tidegbm/config.hpp:13-76, cpp/src/bindings.cpp:43-126
+    int n_estimators = 100;
+    double learning_rate = 0.1;
+    int max_depth = 6;
+    int max_leaves = 31;
+    int min_samples_leaf = 20;
+    double min_child_weight = 1e-3;
+    double l2 = 1.0;
+    double l1 = 0.0;
+    double max_delta_step = 0.0;
+    double min_split_gain = 0.0;
+    int max_bin = 255;
+    int n_time_buckets = 0;
+    i64 horizon = -1;
+    i64 embargo = 0;
+    bool iid = false;
+    bool pair_splits = true;
+    double pair_trigger = 0.25;
+    int pair_top_k = 6;
+    int max_pair_splits = 3;
+    double pair_l2 = 2.0;
+    int objective = 0;  // 0 MSE, 5 pairwise (types.hpp:26-35)
+    int random_state = 0;
+    int goss = -1;
+    double goss_top = 0.2;
+    double goss_other = 0.1;
+    int bin_sample = 262144;
+    bool finer_torb = true;
+    double torb_min_cover = 0.0;
+};
+
+struct Node {  // tree.hpp:11-25
+    int is_leaf = 1, split_type = 0, feature = -1, feature2 = -1, thr = -1, left = -1, right = -1,
+        mgl = 1, n_samples = 0, depth = 0;
+    double value = 0.0, gain = 0.0;
+};
+
+struct Tree {  // tree.hpp:32-41
+    std::vector<Node> nodes;
+    std::vector<double> values, prefix;
+    std::vector<int> cover;
+    int n_buckets = 1;
+};
+
+// ---------------------------------------------------------------- leaf formulas (tree.hpp:43-67)
+static double soft(double G, double l1) {
+    if (l1 <= 0) return G;
+    if (G > l1) return G - l1;
+    if (G < -l1) return G + l1;
+    return 0.0;
+}
+static double score_of(double G, double H, double lam, double l1) {  // histogram.cpp:126-129
+    const double t = soft(G, l1);
+    return (t * t) / (H + lam);
+}
+static double weight_of(double G, double H, double lam, double l1) {  // histogram.cpp:136-142
+    const double den = H + lam;
+    if (l1 <= 0) return -G / den;
+    if (G > l1) return -(G - l1) / den;
+    if (G < -l1) return -(G + l1) / den;
+    return 0.0;
+}
+static double leaf_val(double G, double H, double l2, double l1, double mds) {  // tree.hpp:53-67
+    double v;
+    if (l1 > 0) {
+        if (G > l1) v = -(G - l1) / (H + l2);
+        else if (G < -l1) v = -(G + l1) / (H + l2);
+        else v = 0.0;
+    } else {
+        v = -G / (H + l2);
+    }
+    if (mds > 0) v = std::min(std::max(v, -mds), mds);
+    return v;
+}
+
+// ---------------------------------------------------------------- binning (binning.cpp:14-128)
+// Quantile right-edges of one feature: sorted distinct sample values when they fit in
+// `nb` slots, else nb-1 linearly interpolated quantiles (binning.cpp:14-32).
+static std::vector<double> edges_of(std::vector<double>& v, int nb) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if (v.empty()) return {0.0};
+    if ((int)v.size() <= nb) return v;
+    std::vector<double> e;
+    const size_t top = v.size() - 1;
+    for (int i = 1; i < nb; ++i) {
+        const double pos = (double)i / nb * top;
+        const size_t lo = (size_t)pos;
+        const size_t hi = std::min(lo + 1, top);
+        const double fr = pos - lo;
+        e.push_back(v[lo] * (1 - fr) + v[hi] * fr);
+    }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    return e;
+}
+
+// Per-feature edges over non-NaN values; subsample with mt19937(seed + f*2654435761) and
+// std::shuffle when more than bin_sample values (binning.cpp:84-99).
+static void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
+                      std::vector<std::vector<double>>& edges) {
+    edges.assign(F, {});
+    for (int f = 0; f < F; ++f) {
+        std::vector<double> v;
+        v.reserve(n);
+        for (int i = 0; i < n; ++i) {
+            const double x = X[(size_t)i * F + f];
+            if (x == x) v.push_back(x);
+        }
+        if (bin_sample > 0 && (int)v.size() > bin_sample) {
+            std::mt19937 rng(seed + (uint32_t)f * 2654435761u);
+            std::shuffle(v.begin(), v.end(), rng);
+            v.resize((size_t)bin_sample);
+        }
+        edges[f] = edges_of(v, max_bin - 1);
+    }
+}
+
+// Bin of one value: 0 for NaN, else 1 + number of edges <= v (binning.cpp:110-123).
+static int bin_of(const std::vector<double>& e, double x) {
+    if (!(x == x)) return 0;
+    return 1 + (int)(std::upper_bound(e.begin(), e.end(), x) - e.begin());
+}
+
+// n_bins = max(max_bin, max over features of n_edges+2) (binning.cpp:99,130-134).
+static int total_bins(const std::vector<std::vector<double>>& edges, int max_bin) {
+    int m = 2;
+    for (auto& e : edges) m = std::max(m, (int)e.size() + 2);
+    return std::max(m, max_bin);
+}
+
+// ---------------------------------------------------------------- histograms
+struct Hist {  // histogram.hpp:14-46, planar
+    int F = 0, nb = 0;
+    std::vector<double> g, h;
+    std::vector<i64> c;
+    void init(int F_, int nb_) {
+        F = F_;
+        nb = nb_;
+        g.assign((size_t)F * nb, 0.0);
+        h.assign((size_t)F * nb, 0.0);
+        c.assign((size_t)F * nb, 0);
+    }
+};
+
+// Per feature, rows in ascending idx order: fold g, h, count into the row's bin
+// (histogram.cpp:44-76; row-major build_hist :24-42 gives identical sums).
+static void build(const uint16_t* xb, int F, const int* idx, int m, const double* g,
+                  const double* h, Hist& out, int nb) {
+    out.init(F, nb);
+    for (int f = 0; f < F; ++f) {
+        double* G = &out.g[(size_t)f * nb];
+        double* H = &out.h[(size_t)f * nb];
+        i64* C = &out.c[(size_t)f * nb];
+        for (int k = 0; k < m; ++k) {
+            const int i = idx[k];
+            const int b = xb[(size_t)i * F + f];
+            G[b] += g[i];
+            H[b] += h[i];
+            C[b] += 1;
+        }
+    }
+}
+
+static void subtract_into(Hist& parent, const Hist& child) {  // histogram.cpp:108-117
+    for (size_t i = 0; i < parent.g.size(); ++i) {
+        parent.g[i] -= child.g[i];
+        parent.h[i] -= child.h[i];
+        parent.c[i] -= child.c[i];
+    }
+}
+
+struct Cand {  // histogram.hpp:65-74
+    double gain = -1e300;
+    int feature = -1, feature2 = -1, thr = -1, type = 0, mgl = 1;
+};
+
+// Best numeric split of feature f with bin 0 = missing, thresholds b in [1, nb-2], empty bins
+// skipped, missing-right tried before missing-left, strict '>' keeps the first maximum
+// (histogram.cpp:144-193).
+static Cand scan_feature(const Hist& H, int f, double lam, double gamma, int min_leaf,
+                         double min_hess, double l1) {
+    Cand best;
+    const int nb = H.nb;
+    const double* G = &H.g[(size_t)f * nb];
+    const double* Hh = &H.h[(size_t)f * nb];
+    const i64* C = &H.c[(size_t)f * nb];
+    double tG = 0, tH = 0;
+    i64 tC = 0;
+    for (int b = 0; b < nb; ++b) {
+        tG += G[b];
+        tH += Hh[b];
+        tC += C[b];
+    }
+    const double parent = score_of(tG, tH, lam, l1);
+    double GL = 0, HL = 0;
+    i64 CL = 0;
+    for (int b = 1; b < nb - 1; ++b) {
+        GL += G[b];
+        HL += Hh[b];
+        CL += C[b];
+        if (C[b] == 0) continue;
+        for (int ml = 0; ml <= 1; ++ml) {
+            const double gl = GL + (ml ? G[0] : 0);
+            const double hl = HL + (ml ? Hh[0] : 0);
+            const i64 cl = CL + (ml ? C[0] : 0);
+            const double gr = tG - gl, hr = tH - hl;
+            const i64 cr = tC - cl;
+            if (cl < min_leaf || cr < min_leaf) continue;
+            if (hl < min_hess || hr < min_hess) continue;
+            (void)weight_of;  // monotone constraints are not on this path
+            const double gain =
+                0.5 * (score_of(gl, hl, lam, l1) + score_of(gr, hr, lam, l1) - parent) - gamma;
+            if (gain > best.gain) {
+                best.gain = gain;
+                best.feature = f;
+                best.thr = b;
+                best.type = 0;
+                best.mgl = ml;
+            }
+        }
+    }
+    return best;
+}
+
+// pair hash (histogram.hpp:81-85)
+static inline int pair_hash(int b1, int b2, int nb) {
+    return (int)(((uint32_t)b1 * 2654435761u + (uint32_t)b2 * 40503u) % (uint32_t)nb);
+}
+
+// Node routing of one bin row (tree.cpp:15-41 / 68-91): Axis: bin 0 -> missing direction,
+// else bin <= threshold; Pair: either bin 0 -> missing direction, else hash <= threshold.
+static bool goes_left(const Node& nd, const uint16_t* row, int nb) {
+    if (nd.split_type == 1) {
+        const int b1 = row[nd.feature], b2 = row[nd.feature2];
+        if (b1 == 0 || b2 == 0) return nd.mgl != 0;
+        return pair_hash(b1, b2, nb) <= nd.thr;
+    }
+    const int b = row[nd.feature];
+    if (b == 0) return nd.mgl != 0;
+    return b <= nd.thr;
+}
+
+static int leaf_of(const Tree& t, const uint16_t* row, int nb) {  // tree.cpp:15-41
+    int k = 0;
+    while (!t.nodes[k].is_leaf) k = goes_left(t.nodes[k], row, nb) ? t.nodes[k].left : t.nodes[k].right;
+    return k;
+}
+
+// ---------------------------------------------------------------- leaf candidate (tree.cpp:106-281)
+struct GrowCtx {
+    const uint16_t* xb;
+    int F, nb;
+    const double *g, *h;
+    const Cfg* cfg;
+};
+
+static Cand leaf_candidate(const GrowCtx& cx, const std::vector<int>& rows, const Hist& hist,
+                           int pair_budget, double energy) {
+    const Cfg& c = *cx.cfg;
+    const int F = cx.F;
+    std::vector<Cand> per_f(F);
+    for (int f = 0; f < F; ++f)
+        per_f[f] = scan_feature(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf,
+                                c.min_child_weight, c.l1);
+    Cand best;
+    for (int f = 0; f < F; ++f) {  // ascending feature order, strict '>' (tree.cpp:160-166)
+        if (per_f[f].feature < 0 || per_f[f].gain < c.min_split_gain) continue;
+        if (per_f[f].gain > best.gain) best = per_f[f];
+    }
+    // PAIR search on weak leaves (tree.cpp:215-279)
+    const bool weak =
+        (best.gain < c.pair_trigger * std::max(energy * 0.01, 1e-12)) || best.gain <= 0;
+    if (c.pair_splits && pair_budget > 0 && F >= 2 && weak) {
+        std::vector<int> top(F);
+        std::iota(top.begin(), top.end(), 0);
+        std::sort(top.begin(), top.end(),
+                  [&](int a, int b) { return per_f[a].gain > per_f[b].gain; });
+        const int k = std::min(c.pair_top_k, F);
+        std::vector<std::pair<int, int>> pairs;
+        for (int a = 0; a < k; ++a)
+            for (int b = a + 1; b < k; ++b) pairs.emplace_back(top[a], top[b]);
+        const int m = (int)rows.size();
+        for (auto& pr : pairs) {
+            const int f1 = pr.first, f2 = pr.second;
+            if (f1 < 0 || f2 < 0 || f1 >= F || f2 >= F || f1 == f2) continue;
+            Hist hz;
+            hz.init(1, cx.nb);
+            for (int q = 0; q < m; ++q) {
+                const int i = rows[q];
+                const uint16_t* row = cx.xb + (size_t)i * F;
+                const int z = pair_hash(row[f1], row[f2], cx.nb);
+                hz.g[z] += cx.g[i];
+                hz.h[z] += cx.h[i];
+                hz.c[z] += 1;
+            }
+            Cand pc = scan_feature(hz, 0, c.pair_l2, c.min_split_gain, c.min_samples_leaf,
+                                   c.min_child_weight, c.l1);
+            pc.type = 1;
+            pc.feature = f1;
+            pc.feature2 = f2;
+            if (pc.thr < 0) continue;  // no valid split
+            if (pc.gain > best.gain) best = pc;
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------- best-first growth (tree.cpp:283-472)
+static Tree grow(const GrowCtx& cx, std::vector<int> sample) {
+    const Cfg& c = *cx.cfg;
+    Tree t;
+    std::sort(sample.begin(), sample.end());
+    Node root;
+    root.n_samples = (int)sample.size();
+    double sg = 0, sh = 0;
+    for (int i : sample) {
+        sg += cx.g[i];
+        sh += cx.h[i];
+    }
+    root.value = leaf_val(sg, sh, c.l2, c.l1, c.max_delta_step);
+    t.nodes.push_back(root);
+    std::vector<std::vector<int>> rows(1, sample);
+    std::vector<Hist> hists(1);
+    build(cx.xb, cx.F, sample.data(), (int)sample.size(), cx.g, cx.h, hists[0], cx.nb);
+    std::vector<Cand> cache(1);
+    std::vector<char> cached(1, 0);
+    int pair_used = 0;
+    for (int it = 0; it < c.max_leaves - 1; ++it) {
+        int best_leaf = -1;
+        Cand best;
+        double best_score = c.min_split_gain;
+        for (int lid = 0; lid < (int)t.nodes.size(); ++lid) {
+            if (!t.nodes[lid].is_leaf || t.nodes[lid].depth >= c.max_depth) continue;
+            if ((int)rows[lid].size() < 2 * c.min_samples_leaf) continue;
+            if (!cached[lid]) {
+                double energy = 0;
+                for (int i : rows[lid]) energy += cx.g[i] * cx.g[i];
+                cache[lid] = leaf_candidate(cx, rows[lid], hists[lid],
+                                            c.max_pair_splits - pair_used, energy);
+                cached[lid] = 1;
+            }
+            const Cand& r = cache[lid];
+            if (r.feature >= 0 && r.gain >= c.min_split_gain && r.gain > best_score) {
+                best_score = r.gain;
+                best_leaf = lid;
+                best = r;
+            }
+        }
+        if (best_leaf < 0) break;
+        Node& p = t.nodes[best_leaf];
+        p.is_leaf = 0;
+        p.feature = best.feature;
+        p.feature2 = best.feature2;
+        p.thr = best.thr;
+        p.split_type = best.type;
+        p.mgl = best.mgl;
+        p.gain = best.gain;
+        if (best.type == 1) {  // a PAIR split consumes budget and invalidates every cache
+            ++pair_used;
+            std::fill(cached.begin(), cached.end(), 0);
+        }
+        cached[best_leaf] = 0;
+        std::vector<int> L, R;  // stable partition (tree.cpp:68-91)
+        for (int i : rows[best_leaf])
+            (goes_left(p, cx.xb + (size_t)i * cx.F, cx.nb) ? L : R).push_back(i);
+        Node ln, rn;
+        ln.depth = rn.depth = p.depth + 1;
+        ln.n_samples = (int)L.size();
+        rn.n_samples = (int)R.size();
+        for (auto pr : {std::make_pair(&ln, &L), std::make_pair(&rn, &R)}) {
+            double G = 0, H = 0;
+            for (int i : *pr.second) {
+                G += cx.g[i];
+                H += cx.h[i];
+            }
+            pr.first->value = leaf_val(G, H, c.l2, c.l1, c.max_delta_step);
+        }
+        const bool left_small = L.size() <= R.size();
+        Hist small;
+        const std::vector<int>& sm = left_small ? L : R;
+        build(cx.xb, cx.F, sm.data(), (int)sm.size(), cx.g, cx.h, small, cx.nb);
+        Hist big = std::move(hists[best_leaf]);
+        subtract_into(big, small);
+        const int lid = (int)t.nodes.size();
+        p.left = lid;
+        p.right = lid + 1;
+        t.nodes.push_back(ln);
+        t.nodes.push_back(rn);
+        rows.push_back(std::move(L));
+        rows.push_back(std::move(R));
+        hists.push_back(left_small ? std::move(small) : std::move(big));
+        hists.push_back(left_small ? std::move(big) : std::move(small));
+        hists[best_leaf] = Hist();
+        rows[best_leaf].clear();
+        cache.emplace_back();
+        cache.emplace_back();
+        cached.push_back(0);
+        cached.push_back(0);
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------- objectives (objective.cpp)
+static void grad_mse(const double* y, const double* p, const double* w, int n, double* g,
+                     double* h) {  // objective.cpp:105-119 (cost terms off)
+    for (int i = 0; i < n; ++i) {
+        g[i] = w[i] * (p[i] - y[i]);
+        h[i] = w[i];
+    }
+}
+
+// Within-group pairwise gradient (objective.cpp:179-219); groups are contiguous runs of equal
+// keys when sorted, else a stable sort by key (group_runs, objective.cpp:18-39).
+static void grad_pairwise(const double* y, const double* p, const double* w, const i64* grp,
+                          int n, double* g, double* h) {
+    std::fill(g, g + n, 0.0);
+    std::fill(h, h + n, 1e-6);
+    if (!grp) return;
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    bool sorted = true;
+    for (int i = 1; i < n; ++i)
+        if (grp[i] < grp[i - 1]) sorted = false;
+    if (!sorted)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return grp[a] < grp[b]; });
+    int s0 = 0;
+    while (s0 < n) {
+        int s1 = s0 + 1;
+        while (s1 < n && grp[ord[s1]] == grp[ord[s0]]) ++s1;
+        const int m = s1 - s0;
+        const int* ix = ord.data() + s0;
+        if (m >= 2) {
+            const int step = m > 48 ? std::max(1, m / 24) : 1;
+            for (int a = 0; a < m; ++a)
+                for (int b = a + step; b < m; b += step) {
+                    const int i = ix[a], j = ix[b];
+                    if (y[i] == y[j]) continue;
+                    const int hi = y[j] > y[i] ? j : i;
+                    const int lo = y[j] > y[i] ? i : j;
+                    const double s = p[hi] - p[lo];
+                    const double e = std::exp(std::min(std::max(s, -30.0), 30.0));
+                    const double dyw = std::max(std::abs(y[hi] - y[lo]), 1e-6);
+                    const double lam = -1.0 / (1.0 + e) * dyw;
+                    const double d2 = e / ((1 + e) * (1 + e)) * dyw;
+                    g[hi] += w[hi] * lam;
+                    g[lo] -= w[lo] * lam;
+                    h[hi] += w[hi] * d2;
+                    h[lo] += w[lo] * d2;
+                }
+        }
+        s0 = s1;
+    }
+}
+
+static double wmean(const double* y, const double* w, int n) {  // objective.cpp:41-49
+    double sw = 0, sy = 0;
+    for (int i = 0; i < n; ++i) {
+        sw += w[i];
+        sy += w[i] * y[i];
+    }
+    return sw > 0 ? sy / sw : 0;
+}
+
+// ---------------------------------------------------------------- model and fit (booster.cpp:231-742)
+struct Model {
+    Cfg cfg;
+    int F = 0, nb = 255, n_buckets = 1, best_iteration = 0;
+    std::vector<std::vector<double>> edges;
+    double init_score = 0.0;
+    std::vector<double> init_prefix;
+    std::vector<Tree> trees;
+    bool fitted = false;
+
+    void bins_of(const double* X, int n, std::vector<uint16_t>& xb) const {
+        xb.resize((size_t)n * F);
+        for (int i = 0; i < n; ++i)
+            for (int f = 0; f < F; ++f) xb[(size_t)i * F + f] = (uint16_t)bin_of(edges[f], X[(size_t)i * F + f]);
+    }
+
+    double init_of(const double* y, const double* w, int n) const {  // objective.cpp:80-100
+        if (n <= 0) return 0;
+        if (cfg.objective == 5) return 0;
+        return wmean(y, w, n);
+    }
+
+    void grad(const double* y, const double* p, const double* w, const i64* grp, int n, double* g,
+              double* h) const {
+        if (cfg.objective == 5) grad_pairwise(y, p, w, grp, n, g, h);
+        else grad_mse(y, p, w, n, g, h);
+    }
+
+    void fit(const double* X0, const double* y0, int n, int F_, const i64* ts0, const double* w0,
+             const i64* ent0, const i64* rg0) {
+        Cfg& c = cfg;
+        if (!c.iid && c.horizon < 0) throw std::runtime_error("horizon is required in time mode");
+        if (n >= 1000000) c.pair_splits = false;  // booster.cpp:237-240
+        F = F_;
+        std::vector<double> w(n, 1.0);
+        if (w0) std::copy(w0, w0 + n, w.begin());
+        // stable sort by (ts, entity, index) (booster.cpp:264-309)
+        std::vector<int> ord(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        if (!c.iid && ts0 && n > 1)
+            std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+                if (ts0[a] != ts0[b]) return ts0[a] < ts0[b];
+                if (ent0) return ent0[a] < ent0[b];
+                return a < b;
+            });
+        std::vector<double> X((size_t)n * F), y(n), ws(n);
+        std::vector<i64> ts(ts0 ? n : 0), rg(rg0 ? n : 0);
+        for (int i = 0; i < n; ++i) {
+            const int s = ord[i];
+            std::memcpy(&X[(size_t)i * F], X0 + (size_t)s * F, sizeof(double) * F);
+            y[i] = y0[s];
+            ws[i] = w[s];
+            if (ts0) ts[i] = ts0[s];
+            if (rg0) rg[i] = rg0[s];
+        }
+        // TORB buckets and causal cuts (booster.cpp:36-72, 311-364)
+        std::vector<int> bucket(n, 0);
+        n_buckets = 1;
+        std::vector<int> cut;
+        i64 embargo = c.embargo;
+        if (!c.iid) {
+            if (!ts0) throw std::runtime_error("timestamp is required unless iid_mode");
+            std::vector<i64> u(ts);
+            std::sort(u.begin(), u.end());
+            u.erase(std::unique(u.begin(), u.end()), u.end());
+            const int un = std::max(2, (int)u.size());
+            int want = c.n_time_buckets;
+            if (want <= 0) {
+                i64 step = 1;
+                if (u.size() >= 2) {
+                    std::vector<i64> d;
+                    for (size_t k = 1; k < u.size(); ++k)
+                        if (u[k] - u[k - 1] > 0) d.push_back(u[k] - u[k - 1]);
+                    if (!d.empty()) {
+                        std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
+                        step = std::max<i64>(d[d.size() / 2], 1);
+                    }
+                }
+                const int hs = (int)(std::max<i64>(c.horizon, 0) / step);
+                want = std::max(2, std::min(hs >= 8 ? 64 : 32, std::max(un, 2)));
+            }
+            n_buckets = c.finer_torb ? std::max(2, std::min(want, un)) : std::max(2, want);
+            // rank by stable timestamp order; rows are already sorted so rank == index
+            std::vector<int> o(n);
+            std::iota(o.begin(), o.end(), 0);
+            std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return ts[a] < ts[b]; });
+            for (int r = 0; r < n; ++r) {
+                int b = (int)((double)r / std::max(n, 1) * n_buckets);
+                if (b >= n_buckets) b = n_buckets - 1;
+                bucket[o[r]] = b;
+            }
+            embargo += std::max<i64>(c.horizon, 0);
+            cut.assign(n_buckets, 0);
+            std::vector<int> first(n_buckets, -1);
+            for (int i = 0; i < n; ++i)
+                if (first[bucket[i]] < 0) first[bucket[i]] = i;
+            int run = 0;
+            for (int b = 0; b < n_buckets; ++b) {
+                int k = 0;
+                if (first[b] >= 0)
+                    k = (int)(std::lower_bound(ts.begin(), ts.end(), ts[first[b]] - embargo) - ts.begin());
+                run = std::max(run, k);
+                cut[b] = run;
+            }
+        }
+        // bins (booster.cpp:378-390)
+        fit_edges(X.data(), n, F, c.max_bin, c.bin_sample, (uint32_t)c.random_state, edges);
+        nb = total_bins(edges, c.max_bin);
+        std::vector<uint16_t> xb;
+        bins_of(X.data(), n, xb);
+        // initial scores (booster.cpp:399-457)
+        std::vector<double> pred(n);
+        init_prefix.assign(n_buckets, 0.0);
+        if (c.iid) {
+            init_score = init_of(y.data(), ws.data(), n);
+            std::fill(pred.begin(), pred.end(), init_score);
+        } else {
+            for (int b = 0; b < n_buckets; ++b)
+                init_prefix[b] = cut[b] > 0 ? init_of(y.data(), ws.data(), cut[b]) : NAN;
+            init_score = init_of(y.data(), ws.data(), n);
+            for (int b = 0; b < n_buckets; ++b)
+                if (!std::isfinite(init_prefix[b])) init_prefix[b] = init_score;
+            for (int i = 0; i < n; ++i) pred[i] = init_prefix[bucket[i]];
+        }
+        trees.clear();
+        std::mt19937 rng((uint32_t)c.random_state);
+        std::vector<double> g(n), h(n);
+        const i64* grp = rg0 ? rg.data() : nullptr;
+        const bool use_goss = (c.goss > 0) || (c.goss < 0 && n >= 250000);
+        const double min_cover =
+            c.torb_min_cover > 0.0 ? c.torb_min_cover : (double)(2 * std::max(c.min_samples_leaf, 1));
+        for (int m = 0; m < c.n_estimators; ++m) {
+            grad(y.data(), pred.data(), ws.data(), grp, n, g.data(), h.data());
+            // grow_head (booster.cpp:519-665)
+            std::vector<int> sample(n);
+            std::iota(sample.begin(), sample.end(), 0);
+            std::vector<double> gg, hh;
+            const double* gu = g.data();
+            const double* hu = h.data();
+            if (use_goss && c.goss_top < 1.0 && sample.size() > 64) {  // booster.cpp:554-585
+                gg = g;
+                hh = h;
+                std::vector<int> o = sample;
+                std::sort(o.begin(), o.end(),
+                          [&](int a, int b) { return std::abs(g[a]) > std::abs(g[b]); });
+                const int S = (int)sample.size();
+                const int n_top = std::min(S, std::max(1, (int)std::ceil(c.goss_top * S)));
+                std::vector<char> keep(n, 0);
+                for (int i = 0; i < n_top; ++i) keep[o[i]] = 1;
+                std::vector<int> rest(o.begin() + n_top, o.end());
+                int n_other = std::min((int)std::ceil(c.goss_other * S), (int)rest.size());
+                std::shuffle(rest.begin(), rest.end(), rng);
+                const double sc = (1.0 - c.goss_top) / std::max(c.goss_other, 1e-12);
+                for (int i = 0; i < n_other; ++i) {
+                    keep[rest[i]] = 1;
+                    gg[rest[i]] *= sc;
+                    hh[rest[i]] *= sc;
+                }
+                sample.clear();
+                for (int i : o)
+                    if (keep[i]) sample.push_back(i);
+                std::sort(sample.begin(), sample.end());
+                gu = gg.data();
+                hu = hh.data();
+            }
+            GrowCtx cx{xb.data(), F, nb, gu, hu, &c};
+            (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample uses it
+            Tree t = grow(cx, sample);
+            const int nn = (int)t.nodes.size();
+            std::vector<int> nid(n);
+            for (int i = 0; i < n; ++i) nid[i] = leaf_of(t, &xb[(size_t)i * F], nb);
+            std::vector<double> Gs(nn, 0), Hs(nn, 0);  // refit over all rows (booster.cpp:589-600)
+            t.cover.assign(nn, 0);
+            for (int i = 0; i < n; ++i) {
+                Gs[nid[i]] += g[i];
+                Hs[nid[i]] += h[i];
+                t.cover[nid[i]] += 1;
+            }
+            t.values.resize(nn);
+            for (int k = 0; k < nn; ++k) t.values[k] = leaf_val(Gs[k], Hs[k], c.l2, c.l1, c.max_delta_step);
+            t.n_buckets = n_buckets;
+            t.prefix.assign((size_t)n_buckets * nn, 0.0);
+            if (c.iid) {
+                for (int b = 0; b < n_buckets; ++b)
+                    for (int k = 0; k < nn; ++k) t.prefix[(size_t)b * nn + k] = t.values[k];
+            } else {  // TORB prefix sweep (booster.cpp:609-648)
+                std::vector<double> pG(nn, 0), pH(nn, 0);
+                int swept = 0;
+                for (int b = 0; b < n_buckets; ++b) {
+                    for (; swept < cut[b]; ++swept) {
+                        pG[nid[swept]] += g[swept];
+                        pH[nid[swept]] += h[swept];
+                    }
+                    const double cover = (double)cut[b];
+                    const double alpha = cover <= 0.0 ? 0.0 : std::min(1.0, cover / std::max(min_cover, 1.0));
+                    for (int k = 0; k < nn; ++k) {
+                        const double full = t.values[k];
+                        const double pref =
+                            cover > 0.0 ? leaf_val(pG[k], pH[k], c.l2, c.l1, c.max_delta_step) : full;
+                        t.prefix[(size_t)b * nn + k] = alpha * pref + (1.0 - alpha) * full;
+                    }
+                }
+            }
+            trees.push_back(std::move(t));
+            const Tree& T = trees.back();
+            const double lr = c.learning_rate;
+            for (int i = 0; i < n; ++i) pred[i] += lr * T.prefix[(size_t)bucket[i] * nn + nid[i]];
+        }
+        best_iteration = c.n_estimators;
+        fitted = true;
+    }
+
+    // predict_raw_binned (booster.cpp:744-773): init_score + sum_t lr * values_t[leaf]
+    void predict(const double* X, int n, double* out) const {
+        if (!fitted) throw std::runtime_error("not fitted");
+        std::vector<uint16_t> xb;
+        bins_of(X, n, xb);
+        const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : cfg.n_estimators);
+        for (int i = 0; i < n; ++i) {
+            double s = init_score;
+            for (int t = 0; t < use; ++t) s += cfg.learning_rate * trees[t].values[leaf_of(trees[t], &xb[(size_t)i * F], nb)];
+            out[i] = s;
+        }
+    }
+
+    // update (booster.cpp:844-884): frozen bins, grow `extra` trees on all rows, no TORB.
+    void update(const double* X, const double* y, int n, const double* w0, int extra) {
+        if (!fitted) throw std::runtime_error("not fitted");
+        std::vector<uint16_t> xb;
+        bins_of(X, n, xb);
+        std::vector<double> pred(n), g(n), h(n), w(n, 1.0);
+        if (w0) std::copy(w0, w0 + n, w.begin());
+        predict(X, n, pred.data());
+        std::mt19937 rng((uint32_t)cfg.random_state + (uint32_t)trees.size());
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        for (int m = 0; m < extra; ++m) {
+            grad(y, pred.data(), w.data(), nullptr, n, g.data(), h.data());
+            GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg};
+            (void)rng();
+            Tree t = grow(cx, all);
+            const int nn = (int)t.nodes.size();
+            std::vector<int> nid(n);
+            for (int i = 0; i < n; ++i) nid[i] = leaf_of(t, &xb[(size_t)i * F], nb);
+            std::vector<double> Gs(nn, 0), Hs(nn, 0);
+            t.cover.assign(nn, 0);
+            for (int i = 0; i < n; ++i) {
+                Gs[nid[i]] += g[i];
+                Hs[nid[i]] += h[i];
+            }
+            t.values.resize(nn);
+            for (int k = 0; k < nn; ++k) t.values[k] = leaf_val(Gs[k], Hs[k], cfg.l2, cfg.l1, cfg.max_delta_step);
+            for (int k = 0; k < nn; ++k) t.cover[k] = t.nodes[k].n_samples;  // grow_tree's cover
+            t.n_buckets = 1;
+            t.prefix = t.values;
+            trees.push_back(std::move(t));
+            const Tree& T = trees.back();
+            for (int i = 0; i < n; ++i) pred[i] += cfg.learning_rate * T.values[nid[i]];
+        }
+        best_iteration = (int)trees.size();
+    }
+};
+
+}  // namespace orc
+
+// ================================================================= extern "C" checker API
+using orc::Model;
+static thread_local std::string g_orc_err;
+
+template <class Fn>
+static int orc_guard(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        g_orc_err = e.what();
+        return -1;
+    }
+}
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_orc_err.c_str(); }
+void* orc_create(void) { return new Model(); }
+void orc_free(void* h) { delete static_cast<Model*>(h); }
+
+int orc_set(void* hd, const char* key, double v) {
+    orc::Cfg& c = static_cast<Model*>(hd)->cfg;
+    const std::string k(key);
+    const int iv = (int)v;
+    if (k == "n_estimators") c.n_estimators = iv;
+    else if (k == "learning_rate") c.learning_rate = v;
+    else if (k == "max_depth") c.max_depth = iv;
+    else if (k == "max_leaves") c.max_leaves = iv;
+    else if (k == "min_samples_leaf") c.min_samples_leaf = iv;
+    else if (k == "min_child_weight") c.min_child_weight = v;
+    else if (k == "l2") c.l2 = v;
+    else if (k == "l1") c.l1 = v;
+    else if (k == "max_delta_step") c.max_delta_step = v;
+    else if (k == "min_split_gain") c.min_split_gain = v;
+    else if (k == "max_bin") c.max_bin = iv;
+    else if (k == "n_time_buckets") c.n_time_buckets = iv;
+    else if (k == "embargo") c.embargo = (long long)v;
+    else if (k == "pair_splits") c.pair_splits = iv != 0;
+    else if (k == "pair_trigger") c.pair_trigger = v;
+    else if (k == "pair_top_k") c.pair_top_k = iv;
+    else if (k == "max_pair_splits_per_tree") c.max_pair_splits = iv;
+    else if (k == "pair_l2") c.pair_l2 = v;
+    else if (k == "objective") c.objective = iv;
+    else if (k == "random_state") c.random_state = iv;
+    else if (k == "goss") c.goss = iv;
+    else if (k == "goss_top") c.goss_top = v;
+    else if (k == "goss_other") c.goss_other = v;
+    else if (k == "bin_sample") c.bin_sample = iv;
+    else if (k == "finer_torb") c.finer_torb = iv != 0;
+    else if (k == "torb_min_cover") c.torb_min_cover = v;
+    else if (k == "n_jobs") { /* scalar oracle */ }
+    else {
+        g_orc_err = "unknown key " + k;
+        return -1;
+    }
+    return 0;
+}
+
+int orc_fit(void* hd, const double* X, const double* y, int n, int F, const long long* ts,
+            const double* w, const long long* entity, const long long* rank_groups, int iid,
+            long long horizon) {
+    Model* m = static_cast<Model*>(hd);
+    return orc_guard([&] {
+        m->cfg.iid = iid != 0;
+        if (!m->cfg.iid) m->cfg.horizon = horizon;
+        m->fit(X, y, n, F, ts, w, entity, rank_groups);
+    });
+}
+
+int orc_update(void* hd, const double* X, const double* y, int n, int F, const double* w, int extra) {
+    Model* m = static_cast<Model*>(hd);
+    return orc_guard([&] {
+        if (F != m->F) throw std::runtime_error("n_features mismatch");
+        m->update(X, y, n, w, extra);
+    });
+}
+
+int orc_predict(void* hd, const double* X, int n, int F, double* out) {
+    Model* m = static_cast<Model*>(hd);
+    return orc_guard([&] {
+        if (F != m->F) throw std::runtime_error("n_features mismatch");
+        m->predict(X, n, out);
+    });
+}
+
+int orc_n_bins(void* hd) { return static_cast<Model*>(hd)->nb; }
+int orc_n_time_buckets(void* hd) { return static_cast<Model*>(hd)->n_buckets; }
+double orc_init_score(void* hd) { return static_cast<Model*>(hd)->init_score; }
+int orc_init_prefix(void* hd, double* out, int cap) {
+    const auto& v = static_cast<Model*>(hd)->init_prefix;
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
+int orc_edges(void* hd, int f, double* out, int cap) {
+    const auto& e = static_cast<Model*>(hd)->edges.at(f);
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
+int orc_n_trees(void* hd) { return (int)static_cast<Model*>(hd)->trees.size(); }
+int orc_tree_info(void* hd, int t, int* nn, int* nbk) {
+    const orc::Tree& tr = static_cast<Model*>(hd)->trees.at(t);
+    *nn = (int)tr.nodes.size();
+    *nbk = tr.n_buckets;
+    return 0;
+}
+int orc_tree_get(void* hd, int t, int* ints, double* dbl, double* values, int* cover, double* prefix) {
+    const orc::Tree& tr = static_cast<Model*>(hd)->trees.at(t);
+    for (size_t k = 0; k < tr.nodes.size(); ++k) {
+        const orc::Node& d = tr.nodes[k];
+        const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.thr, d.left, d.right,
+                           d.mgl, d.n_samples, d.depth};
+        std::memcpy(ints + 10 * k, v, sizeof v);
+        dbl[2 * k] = d.value;
+        dbl[2 * k + 1] = d.gain;
+        values[k] = tr.values[k];
+        cover[k] = tr.cover[k];
+    }
+    std::copy(tr.prefix.begin(), tr.prefix.end(), prefix);
+    return 0;
+}
+
+// ---- per-function checkers over caller buffers ----
+
+// Edges per feature into edges[f*256 + k], counts into n_edges[f]; returns n_bins.
+int orc_fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, unsigned seed,
+                  double* edges, int* n_edges) {
+    std::vector<std::vector<double>> e;
+    orc::fit_edges(X, n, F, max_bin, bin_sample, seed, e);
+    for (int f = 0; f < F; ++f) {
+        n_edges[f] = (int)e[f].size();
+        std::copy(e[f].begin(), e[f].end(), edges + (size_t)f * 256);
+    }
+    return orc::total_bins(e, max_bin);
+}
+
+// Row-major u8 bins from edges laid out as in orc_fit_edges.
+int orc_bin_transform(const double* X, int n, int F, const double* edges, const int* n_edges,
+                      unsigned char* out) {
+    for (int f = 0; f < F; ++f) {
+        std::vector<double> e(edges + (size_t)f * 256, edges + (size_t)f * 256 + n_edges[f]);
+        for (int i = 0; i < n; ++i) out[(size_t)i * F + f] = (unsigned char)orc::bin_of(e, X[(size_t)i * F + f]);
+    }
+    return 0;
+}
+
+// FP64 histogram over rows idx (ascending order of the caller's list) of row-major u8 bins.
+int orc_hist(const unsigned char* xb, int F, const int* idx, int m, const double* g,
+             const double* h, int nb, double* G, double* H, long long* C) {
+    std::vector<uint16_t> x16;  // widen to the reference's u16 storage
+    int mx = 0;
+    for (int k = 0; k < m; ++k) mx = std::max(mx, idx[k]);
+    x16.resize((size_t)(mx + 1) * F);
+    for (int k = 0; k < m; ++k)
+        for (int f = 0; f < F; ++f) x16[(size_t)idx[k] * F + f] = xb[(size_t)idx[k] * F + f];
+    orc::Hist out;
+    orc::build(x16.data(), F, idx, m, g, h, out, nb);
+    std::copy(out.g.begin(), out.g.end(), G);
+    std::copy(out.h.begin(), out.h.end(), H);
+    std::copy(out.c.begin(), out.c.end(), C);
+    return 0;
+}
+
+// Per-feature best split (gain/bin/missing-left per feature) and the leaf's argmax.
+int orc_scan(const double* G, const double* H, const long long* C, int F, int nb, double lam,
+             double gamma, int min_leaf, double min_hess, double l1, double* gain_f, int* bin_f,
+             int* ml_f, int* best_feature) {
+    orc::Hist hist;
+    hist.init(F, nb);
+    std::copy(G, G + (size_t)F * nb, hist.g.begin());
+    std::copy(H, H + (size_t)F * nb, hist.h.begin());
+    std::copy(C, C + (size_t)F * nb, hist.c.begin());
+    orc::Cand best;
+    for (int f = 0; f < F; ++f) {
+        orc::Cand c = orc::scan_feature(hist, f, lam, gamma, min_leaf, min_hess, l1);
+        gain_f[f] = c.gain;
+        bin_f[f] = c.thr;
+        ml_f[f] = c.mgl;
+        if (c.feature >= 0 && c.gain >= gamma && c.gain > best.gain) best = c;
+    }
+    *best_feature = best.feature;
+    return 0;
+}
+
+int orc_grad(int kind, const double* y, const double* p, const double* w, const long long* grp,
+             int n, double* g, double* h) {
+    std::vector<double> ones;
+    if (!w) {
+        ones.assign(n, 1.0);
+        w = ones.data();
+    }
+    if (kind == 5) orc::grad_pairwise(y, p, w, grp, n, g, h);
+    else orc::grad_mse(y, p, w, n, g, h);
+    return 0;
+}
+
+}  // extern "C"
