@@ -1,0 +1,131 @@
+/* Thin C ABI between the C++ host booster (libtidegbm_amd.so) and the gfx950 HIP kernels
+ * (libtgbm_hip.so).  Plain pointers, sizes and POD structs only; every call returns 0 on
+ * success and -1 on failure with the message in tgbm_last_error().
+ *
+ * It replaces the reference's only device seam, build_hist_gpu
+ * (reference cpp/include/tidegbm/histogram.hpp:56-58, cpp/src/histogram_cuda.cu:81-167), which
+ * re-uploaded g/h/row indices and downloaded a histogram on every node.  Here a session keeps
+ * the binned matrix, gradients, row partitions, histograms and tree state resident in HBM; the
+ * host only exchanges split decisions and finished trees.
+ *
+ * A session holds one shared binned matrix (n_rows x n_features, uint8 bins) and n_tasks
+ * independent fits ("tasks") over row prefixes [0, task_rows[t]) of it.  A plain fit is one task.
+ */
+#ifndef TGBM_DEVICE_H
+#define TGBM_DEVICE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tgbm_session tgbm_session;
+
+/* Per-node export layout shared by every tree accessor in this project:
+ * ints[10] = {is_leaf, split_type, feature, feature2, threshold_bin, left, right,
+ *             missing_go_left, n_samples, depth}; dbl[2] = {value, gain}.
+ * Mirrors reference Node (cpp/include/tidegbm/tree.hpp:11-25). */
+#define TGBM_NODE_INTS 10
+
+typedef struct tgbm_grow_params {
+    int max_depth;          /* reference Config::max_depth (config.hpp:15) */
+    int max_leaves;         /* Config::max_leaves, <= 128 on device */
+    int min_samples_leaf;   /* Config::min_samples_leaf */
+    double min_child_weight;
+    double l2, l1, max_delta_step, min_split_gain;
+    double pair_l2;         /* Config::pair_l2, used by PAIR scans */
+    int hist_mode;          /* 0 = auto, else forces a histogram kernel variant (autotuner) */
+    int pad;
+} tgbm_grow_params;
+
+/* One split decision made by the host (PAIR-capable driver). */
+typedef struct tgbm_decision {
+    int leaf;               /* node id to split, -1 = stop growing */
+    int type;               /* 0 axis, 1 pair */
+    int feature, feature2, threshold, missing_go_left;
+    double gain;
+} tgbm_decision;
+
+/* Leaf candidate as cached by the grower (reference SplitCand, histogram.hpp:65-74). */
+typedef struct tgbm_cand {
+    double gain;
+    int feature, feature2, threshold, type, missing_go_left, valid;
+} tgbm_cand;
+
+int tgbm_device_count(void);
+const char* tgbm_last_error(void);
+
+/* Session lifetime.  n_bins is the histogram width used by the PAIR hash (reference
+ * Booster::n_bins, binning.cpp:130-134).  h_const=1 declares h == 1.0 for every row of every
+ * task (unit-weight MSE without GOSS), which lets hessian sums be taken from counts. */
+int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bins, int n_tasks,
+                        const int* task_rows, int max_leaves, int h_const);
+void tgbm_session_destroy(tgbm_session* s);
+
+/* Upload raw features (host row-major f64 n_rows x n_features) and bin them on the device
+ * with per-feature right edges edges[f*256 + k], k < n_edges[f] (reference
+ * Binner::transform, binning.cpp:104-128): NaN -> 0, else 1 + #edges <= x. */
+int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges);
+/* Row-major uint8 bins back to the host (n_rows x n_features), for checking. */
+int tgbm_session_get_bins(tgbm_session* s, unsigned char* out);
+
+/* Per-task row data (host arrays of task_rows[t] entries).  w may be NULL (all 1).  bucket and
+ * sweep_seg drive TORB: sweep_seg[i] = number of bucket cuts <= i (0..n_buckets). */
+int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double* w,
+                       const double* pred0, const int* bucket, const unsigned char* sweep_seg,
+                       int n_buckets, const int* cuts, const long long* groups);
+
+/* Gradients for the current predictions: objective 0 = MSE (objective.cpp:105-119),
+ * 5 = within-group pairwise (objective.cpp:179-219). */
+int tgbm_task_grad(tgbm_session* s, int task, int objective);
+/* Copy g (and h) of a task back to the host (GOSS selection runs on the host). */
+int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h);
+/* Restrict the next tree to sorted sample rows; rows with scaled[i]!=0 use g*scale, h*scale
+ * (GOSS, booster.cpp:554-585).  sample == NULL restores "all rows". */
+int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sample,
+                         const unsigned char* scaled, double scale);
+
+/* Start a tree for every task: quantize g/h, build the root histogram, scan it. */
+int tgbm_tree_begin(tgbm_session* s, const tgbm_grow_params* p);
+/* Device-driven best-first growth of every task's tree to completion (no host round trips). */
+int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p);
+
+/* Host-driven growth (one task), used when PAIR splits are enabled. */
+int tgbm_leaf_scan_results(tgbm_session* s, int task, int node, double* gain_f, int* thr_f,
+                           int* ml_f, double* energy, int* n_rows);
+int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const int* f2,
+                     int n_pairs, const tgbm_grow_params* p, tgbm_cand* out);
+int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d,
+                     const tgbm_grow_params* p, int* left_id);
+
+/* Finish every task's tree: leaf ids for all rows, leaf refit and TORB prefix values
+ * (booster.cpp:589-648), pred += lr * prefix (booster.cpp:655-664).  Writes the tree of task
+ * t into the caller's buffers at offset t*max_nodes. */
+int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int iid,
+                  const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
+                  double* values, int* cover, double* prefix);
+
+int tgbm_task_get_pred(tgbm_session* s, int task, double* pred);
+
+/* Timers (HIP events on the session stream) summed per kernel family since the last reset:
+ * which = 0 histogram build, 1 partition, 2 scan, 3 tree end, 4 gradients. */
+int tgbm_timer_enable(tgbm_session* s, int on);
+int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches,
+                    double* bytes);
+
+/* ---------------- predict / serving: resident ensembles ---------------- */
+typedef struct tgbm_forest tgbm_forest;
+
+/* Flatten trees (node arrays as in the export layout) into a device-resident forest. */
+int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_trees,
+                       const int* tree_nodes, const int* node_ints, const double* values,
+                       const double* lr, double init_score, const double* edges,
+                       const int* n_edges);
+void tgbm_forest_destroy(tgbm_forest* f);
+/* Predict raw scores for host rows X (row-major f64) or for device-resident uint8 bins. */
+int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out);
+int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,82 @@
+/* Drop-in C API of the MI355X histogram-GBDT path (libtidegbm_amd.so).
+ *
+ * The first twelve entry points keep the exact signatures and conventions of the reference C API
+ * (reference cpp/include/tidegbm/c_api.h:9-28, implemented in cpp/src/c_api.cpp:18-158):
+ * return 0 on success and -1 on failure with the message in a thread-local string returned by
+ * tide_last_error(); handles are opaque and not re-entrant.  Behind them the fit, gradients,
+ * histograms, split search, row partition, TORB leaf statistics and predict run as gfx950 HIP
+ * kernels.  A handle's device defaults to "gpu"; there is no CPU fallback: with no GPU present
+ * tide_fit/tide_predict fail with a message that names the GPU (the reference's behaviour for
+ * device="gpu" without GPU support, config.hpp:89-94).
+ *
+ * The extensions below are additive; the reference C API has no way to set the objective,
+ * pair_splits, min_samples_leaf or device, nor to pass entity / rank_groups
+ * (c_api.cpp:32-86).
+ */
+#ifndef TIDEGBM_AMD_H
+#define TIDEGBM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* TideHandle;
+
+/* ---- reference C API (c_api.h:9-28) ---- */
+int tide_create(TideHandle* out);                                         /* c_api.h:11 */
+void tide_free(TideHandle h);                                             /* c_api.h:12 */
+int tide_set_int(TideHandle h, const char* key, int value);               /* c_api.h:15 */
+int tide_set_double(TideHandle h, const char* key, double value);         /* c_api.h:16 */
+int tide_fit(TideHandle h, const double* X, const double* y, int n, int n_features,
+             const long long* timestamps, const double* weight, const int* is_cat,
+             int iid, int horizon);                                       /* c_api.h:18-20 */
+int tide_predict(TideHandle h, const double* X, int n, int n_features, double* out);     /* :22 */
+int tide_predict_raw(TideHandle h, const double* X, int n, int n_features, double* out); /* :23 */
+int tide_predict_binned(TideHandle h, const unsigned short* xb, int n, double* out);     /* :24 */
+int tide_save_json(TideHandle h, const char* path);                      /* c_api.h:25 */
+int tide_save(TideHandle h, const char* path);                           /* c_api.h:27 */
+int tide_load(TideHandle* out, const char* path);                        /* c_api.h:28 */
+const char* tide_last_error(void);                                        /* c_api.h:30 */
+
+/* ---- extensions ---- */
+/* 1 when the gfx950 kernels are compiled in (the reference's cuda_compiled flag,
+ * bindings.cpp:324-328); the device may still be absent at run time. */
+int tide_device_compiled(void);
+/* Number of visible HIP devices (0 when none). */
+int tide_device_count(void);
+/* String keys: "device" ("gpu" | "cpu"; "cpu" is rejected at fit: this library has no CPU
+ * path) and "objective" ("mse" | "pairwise" and the reference's aliases, bindings.cpp:108-119). */
+int tide_set_string(TideHandle h, const char* key, const char* value);
+/* tide_fit plus the entity and rank_groups arrays Booster::fit accepts (booster.hpp:45-48);
+ * horizon in timestamp units as a 64-bit value.  Any pointer except X/y may be NULL. */
+int tide_fit_ex(TideHandle h, const double* X, const double* y, int n, int n_features,
+                const long long* timestamps, const double* weight, const int* is_cat,
+                const long long* entity, const long long* rank_groups, int iid,
+                long long horizon);
+/* Booster::update (booster.cpp:844-884): grow extra_trees more trees on new rows. */
+int tide_update(TideHandle h, const double* X, const double* y, int n, int n_features,
+                const double* weight, int extra_trees);
+/* Model inspection, node layout as in include/tgbm_device.h (10 ints + {value, gain}). */
+int tide_num_trees(TideHandle h);
+int tide_tree_info(TideHandle h, int tree, int* n_nodes, int* n_buckets);
+int tide_tree_get(TideHandle h, int tree, int* node_ints, double* node_dbl, double* values,
+                  int* cover, double* prefix_values);
+int tide_model_info(TideHandle h, int* n_features, int* n_bins, int* n_time_buckets,
+                    int* best_iteration, double* init_score);
+int tide_get_edges(TideHandle h, int feature, double* out, int cap);
+int tide_get_init_prefix(TideHandle h, double* out, int cap);
+/* Per-kernel-family device time of the last fit (ms, launches, algorithmic bytes); family 0 =
+ * root histograms, 5 = child histograms, 1 = partitions, 3 = tree end, 4 = gradients.
+ * Collected only when the int key "profile" was set to 1 before the fit. */
+int tide_profile(TideHandle h, int family, double* ms, long long* launches, double* bytes);
+/* Fit timing breakdown of the last fit: seconds for setup (sort, bins, uploads) and for the
+ * boosting loop. */
+int tide_fit_times(TideHandle h, double* setup_s, double* boost_s);
+/* Wall time (s since the boosting loop began) at the end of each tree of the last fit; each
+ * tree ends with a device synchronization.  Returns the number of trees. */
+int tide_tree_times(TideHandle h, double* out, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+"""ctypes front ends of the two CPU checkers (test infrastructure only).
+
+* ``oracle/liboracle.so``      — the scalar C++ restatement of the reference path (prefix orc_)
+* ``oracle/_ref/libtide_ref.so`` — the reference core built from its own sources (prefix tref_)
+
+Both expose the same shape of API, so tests can run one fit through either and compare trees
+field by field with the device path.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libtide_ref.so"
+
+_P, _I, _D, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class CpuModel:
+    """One fitted model of a CPU checker library."""
+
+    def __init__(self, lib: "CpuLib", **params):
+        self.lib = lib
+        self.h = _P(lib.fn("create")())
+        for k, v in params.items():
+            if lib.fn("set")(self.h, k.encode(), float(v)) != 0:
+                raise ValueError(lib.error())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.fn("free")(self.h)
+            self.h = None
+
+    def fit(self, X, y, ts=None, w=None, entity=None, rank_groups=None, iid=False, horizon=1):
+        X = np.ascontiguousarray(X, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        ts = None if ts is None else np.ascontiguousarray(ts, np.int64)
+        w = None if w is None else np.ascontiguousarray(w, np.float64)
+        entity = None if entity is None else np.ascontiguousarray(entity, np.int64)
+        rank_groups = None if rank_groups is None else np.ascontiguousarray(rank_groups, np.int64)
+        rc = self.lib.fn("fit")(self.h, _ptr(X), _ptr(y), X.shape[0], X.shape[1], _ptr(ts), _ptr(w),
+                                _ptr(entity), _ptr(rank_groups), 1 if iid else 0, int(horizon))
+        if rc != 0:
+            raise ValueError(self.lib.error())
+        return self
+
+    def update(self, X, y, extra, w=None):
+        X = np.ascontiguousarray(X, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        w = None if w is None else np.ascontiguousarray(w, np.float64)
+        if self.lib.fn("update")(self.h, _ptr(X), _ptr(y), X.shape[0], X.shape[1], _ptr(w), int(extra)) != 0:
+            raise ValueError(self.lib.error())
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float64)
+        out = np.empty(X.shape[0])
+        if self.lib.fn("predict")(self.h, _ptr(X), X.shape[0], X.shape[1], _ptr(out)) != 0:
+            raise ValueError(self.lib.error())
+        return out
+
+    def trees(self):
+        out = []
+        for t in range(self.lib.fn("n_trees")(self.h)):
+            nn, nb = _I(), _I()
+            self.lib.fn("tree_info")(self.h, t, ctypes.byref(nn), ctypes.byref(nb))
+            ints = np.empty((nn.value, 10), np.int32)
+            dbl = np.empty((nn.value, 2))
+            vals = np.empty(nn.value)
+            cov = np.empty(nn.value, np.int32)
+            pref = np.empty(nn.value * nb.value)
+            self.lib.fn("tree_get")(self.h, t, _ptr(ints), _ptr(dbl), _ptr(vals), _ptr(cov), _ptr(pref))
+            out.append(dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value))
+        return out
+
+    def edges(self, f):
+        buf = np.empty(256)
+        m = self.lib.fn("edges")(self.h, f, _ptr(buf), 256)
+        return buf[:m].copy()
+
+    def n_bins(self):
+        return self.lib.fn("n_bins")(self.h)
+
+    def init_prefix(self):
+        buf = np.empty(512)
+        m = self.lib.fn("init_prefix")(self.h, _ptr(buf), 512)
+        return buf[:m].copy()
+
+    def init_score(self):
+        return self.lib.fn("init_score")(self.h)
+
+
+class CpuLib:
+    def __init__(self, path: Path, prefix: str):
+        self.path = path
+        self.prefix = prefix
+        self.lib = ctypes.CDLL(str(path))
+        sig = {
+            "create": (_P, []), "free": (None, [_P]), "set": (_I, [_P, ctypes.c_char_p, _D]),
+            "fit": (_I, [_P, _P, _P, _I, _I, _P, _P, _P, _P, _I, _LL]),
+            "update": (_I, [_P, _P, _P, _I, _I, _P, _I]),
+            "predict": (_I, [_P, _P, _I, _I, _P]), "n_trees": (_I, [_P]),
+            "tree_info": (_I, [_P, _I, _P, _P]), "tree_get": (_I, [_P, _I, _P, _P, _P, _P, _P]),
+            "edges": (_I, [_P, _I, _P, _I]), "n_bins": (_I, [_P]), "init_prefix": (_I, [_P, _P, _I]),
+            "init_score": (_D, [_P]), "last_error": (ctypes.c_char_p, []),
+        }
+        for k, (res, args) in sig.items():
+            f = getattr(self.lib, f"{prefix}_{k}")
+            f.restype = res
+            f.argtypes = args
+
+    def fn(self, name):
+        return getattr(self.lib, f"{self.prefix}_{name}")
+
+    def error(self):
+        return self.fn("last_error")().decode(errors="replace")
+
+    def model(self, **params):
+        return CpuModel(self, **params)
+
+
+def oracle() -> CpuLib:
+    lib = CpuLib(ORACLE_SO, "orc")
+    L = lib.lib
+    L.orc_fit_edges.argtypes = [_P, _I, _I, _I, _I, ctypes.c_uint, _P, _P]
+    L.orc_fit_edges.restype = _I
+    L.orc_bin_transform.argtypes = [_P, _I, _I, _P, _P, _P]
+    L.orc_hist.argtypes = [_P, _I, _P, _I, _P, _P, _I, _P, _P, _P]
+    L.orc_scan.argtypes = [_P, _P, _P, _I, _I, _D, _D, _I, _D, _D, _P, _P, _P, _P]
+    L.orc_grad.argtypes = [_I, _P, _P, _P, _P, _I, _P, _P]
+    return lib
+
+
+def reference() -> CpuLib | None:
+    if not REF_SO.exists():
+        return None
+    lib = CpuLib(REF_SO, "tref")
+    L = lib.lib
+    L.tref_bins.argtypes = [_P, _P, _I, _I, _P]
+    L.tref_build_hist_col.argtypes = [_P, _I, _I, _P, _I, _P, _P, _I, _P, _P, _P]
+    L.tref_best_numeric.argtypes = [_P, _P, _P, _I, _I, _I, _D, _D, _I, _D, _I, _D, _P, _P, _P]
+    L.tref_grad.argtypes = [_I, _P, _P, _P, _P, _I, _P, _P]
+    L.tref_save.argtypes = [_P, ctypes.c_char_p, _I]
+    L.tref_load.restype = _P
+    L.tref_load.argtypes = [ctypes.c_char_p]
+    return lib
+
+
+STRUCT_COLS = ["is_leaf", "split_type", "feature", "feature2", "threshold", "left", "right",
+               "missing_go_left", "n_samples", "depth"]
+
+
+def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True):
+    """First structural difference between two tree lists, or None.
+
+    Integer node fields must match exactly; value/gain/values/prefix within tolerance.
+    """
+    if len(a) != len(b):
+        return f"tree count {len(a)} != {len(b)}"
+    for t, (x, y) in enumerate(zip(a, b)):
+        if x["ints"].shape != y["ints"].shape:
+            return f"tree {t}: node count {x['ints'].shape[0]} != {y['ints'].shape[0]}"
+        bad = np.argwhere(x["ints"] != y["ints"])
+        if bad.size:
+            k, c = bad[0]
+            return f"tree {t} node {k} {STRUCT_COLS[c]}: {x['ints'][k, c]} != {y['ints'][k, c]}"
+        if not np.array_equal(x["cover"], y["cover"]):
+            return f"tree {t}: cover differs"
+        if check_values:
+            for key in ("values", "prefix"):
+                if not np.allclose(x[key], y[key], rtol=rtol, atol=atol):
+                    d = np.max(np.abs(x[key] - y[key]))
+                    return f"tree {t}: {key} differ (max abs {d:g})"
+            if not np.allclose(x["dbl"], y["dbl"], rtol=rtol, atol=atol):
+                return f"tree {t}: node value/gain differ"
+    return None

@@ -1,0 +1,127 @@
+"""Whole-fit parity of the GPU path against the CPU oracle and the reference core.
+
+Bar (stated per test): tree structure (every integer node field, leaf cover) bit-exact; leaf and
+TORB prefix values within rtol 1e-7 (they come from exact fixed-point sums instead of sequential
+FP64 folds); predictions within rtol 1e-5.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import checkers
+from tidegbm_amd import Booster, synth
+
+pytestmark = pytest.mark.gpu
+
+VALUE_RTOL = 1e-7
+PRED_RTOL = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _fit_gpu(p, rg=None, iid=False, horizon=1, **params):
+    b = Booster(**params)
+    b.fit(p.X, p.y, timestamps=None if iid else p.ts, entity=None if iid else p.entity,
+          rank_groups=rg, iid=iid, horizon=horizon)
+    return b
+
+
+def _fit_cpu(lib, p, rg=None, iid=False, horizon=1, **params):
+    m = lib.model(**params)
+    m.fit(p.X, p.y, ts=None if iid else p.ts, entity=None if iid else p.entity, rank_groups=rg,
+          iid=iid, horizon=horizon)
+    return m
+
+
+def _pred_close(a, b):
+    scale = np.maximum(np.abs(b), 1e-3 * np.max(np.abs(b)) + 1e-12)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
+    _say(f"[case] {name} n={p.n} F={p.n_features} params={params}")
+    gpu = _fit_gpu(p, rg=rg, iid=iid, **params)
+    orc = _fit_cpu(checkers.oracle(), p, rg=rg, iid=iid, **params)
+    diff = checkers.compare_trees(gpu.trees(), orc.trees(), rtol=VALUE_RTOL, atol=1e-9)
+    assert diff is None, f"{name}: GPU vs oracle: {diff}"
+    Xp = p.X[:n_pred]
+    rel = _pred_close(gpu.predict(Xp), orc.predict(Xp))
+    assert rel < PRED_RTOL, f"{name}: prediction rel err {rel}"
+    ref = checkers.reference()
+    if ref is not None:
+        r = _fit_cpu(ref, p, rg=rg, iid=iid, **params)
+        diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9)
+        assert diff is None, f"{name}: GPU vs reference: {diff}"
+    return gpu, orc
+
+
+def test_small_panel_mse_default():
+    # the reference default path: PAIR search on (below 1M rows), host-driven growth
+    p = synth.panel_small(0)
+    _check("small-mse-default", p, n_estimators=40, max_depth=6, learning_rate=0.08, goss=0)
+
+
+@pytest.mark.parametrize("seed", [0, 11])
+def test_small_panel_pair_splits(seed):
+    p = synth.panel_small(seed)
+    rng = np.random.default_rng(seed)
+    y = np.sign(p.X[:, 0] * p.X[:, 1]) + 0.1 * rng.normal(size=p.n)
+    q = synth.Panel(p.X, y, p.ts, p.entity)
+    gpu, _ = _check(f"small-pair-{seed}", q, n_estimators=20, max_depth=6, learning_rate=0.08, goss=0)
+    assert sum(int((t["ints"][:, 1] == 1).sum()) for t in gpu.trees()) >= 5
+
+
+def test_small_panel_mse_device_driven():
+    p = synth.panel_small(1)
+    _check("small-mse-device", p, n_estimators=25, max_depth=6, learning_rate=0.08, goss=0, pair_splits=0)
+
+
+def test_small_panel_iid_mode():
+    p = synth.panel_small(2)
+    _check("small-iid", p, iid=True, n_estimators=15, max_depth=5, learning_rate=0.1, goss=0, pair_splits=0)
+
+
+def test_small_panel_pairwise():
+    p = synth.panel_small(3, n_dates=200, n_entities=60)
+    _check("small-pairwise", p, rg=p.ts, n_estimators=10, max_depth=6, learning_rate=0.08, goss=0,
+           pair_splits=0, objective=5)
+
+
+def test_small_panel_goss():
+    p = synth.panel_small(4)
+    _check("small-goss", p, n_estimators=10, max_depth=5, learning_rate=0.1, goss=1, pair_splits=0)
+
+
+def test_larger_panel_depth8_missing_values():
+    p = synth.panel_large(5, n_dates=250, n_entities=200, n_features=48)
+    _check("large-subset-mse", p, n_estimators=8, max_depth=8, learning_rate=0.05, goss=0, pair_splits=0)
+
+
+def test_refit_update_matches_oracle():
+    p = synth.panel_small(6)
+    _say("[case] update")
+    params = dict(n_estimators=6, max_depth=5, learning_rate=0.1, goss=0, pair_splits=0)
+    gpu = _fit_gpu(p, **params)
+    orc = _fit_cpu(checkers.oracle(), p, **params)
+    q = synth.panel_small(7, n_dates=100)
+    gpu.update(q.X, q.y, 5)
+    orc.update(q.X, q.y, 5)
+    diff = checkers.compare_trees(gpu.trees(), orc.trees(), rtol=VALUE_RTOL, atol=1e-9)
+    assert diff is None, diff
+    assert _pred_close(gpu.predict(q.X), orc.predict(q.X)) < PRED_RTOL
+
+
+def test_fit_is_deterministic():
+    p = synth.panel_small(8)
+    _say("[case] determinism")
+    params = dict(n_estimators=10, max_depth=6, learning_rate=0.08, goss=0)
+    a = _fit_gpu(p, **params)
+    b = _fit_gpu(p, **params)
+    for x, y in zip(a.trees(), b.trees()):
+        for k in ("ints", "dbl", "values", "prefix", "cover"):
+            assert np.array_equal(x[k], y[k]), k
+    assert np.array_equal(a.predict(p.X), b.predict(p.X))

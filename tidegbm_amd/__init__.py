@@ -1,0 +1,323 @@
+"""MI355X-native histogram-GBDT path of TideGBM, behind the reference's C API.
+
+The native library (lib/libtidegbm_amd.so, C++ host + gfx950 HIP kernels) is loaded with ctypes;
+there is no Python or CPU fallback: if the library is missing, importing this package raises.
+
+`TideRegressor` / `TideRanker` mirror the reference estimators' data-prep contract for this path
+(reference src/tidegbm/native.py:30-220): horizon/embargo converted to timestamp units
+(utils.py:47-59), rows ordered by (timestamp, entity code) (native.py:92-102), rank groups =
+timestamps for ranking objectives (native.py:134-137), native errors re-raised as ValueError
+(native.py:216-217), and the `device` keyword (api.py:79) whose "gpu" value selects this path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_LIBDIR = Path(__file__).resolve().parent / "lib"
+_LIBPATH = _LIBDIR / "libtidegbm_amd.so"
+
+if not _LIBPATH.exists():
+    raise ImportError(
+        f"{_LIBPATH} is missing: build the native libraries first (python tidegbm_amd/build.py)")
+
+_lib = ctypes.CDLL(str(_LIBPATH))
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_LL = ctypes.c_longlong
+
+
+def _sig(name, res, *args):
+    fn = getattr(_lib, name)
+    fn.restype = res
+    fn.argtypes = list(args)
+    return fn
+
+
+_sig("tide_create", _I, ctypes.POINTER(_P))
+_sig("tide_free", None, _P)
+_sig("tide_set_int", _I, _P, ctypes.c_char_p, _I)
+_sig("tide_set_double", _I, _P, ctypes.c_char_p, _D)
+_sig("tide_set_string", _I, _P, ctypes.c_char_p, ctypes.c_char_p)
+_sig("tide_fit", _I, _P, _P, _P, _I, _I, _P, _P, _P, _I, _I)
+_sig("tide_fit_ex", _I, _P, _P, _P, _I, _I, _P, _P, _P, _P, _P, _I, _LL)
+_sig("tide_update", _I, _P, _P, _P, _I, _I, _P, _I)
+_sig("tide_predict", _I, _P, _P, _I, _I, _P)
+_sig("tide_predict_raw", _I, _P, _P, _I, _I, _P)
+_sig("tide_predict_binned", _I, _P, _P, _I, _P)
+_sig("tide_save", _I, _P, ctypes.c_char_p)
+_sig("tide_save_json", _I, _P, ctypes.c_char_p)
+_sig("tide_load", _I, ctypes.POINTER(_P), ctypes.c_char_p)
+_sig("tide_last_error", ctypes.c_char_p)
+_sig("tide_device_compiled", _I)
+_sig("tide_device_count", _I)
+_sig("tide_num_trees", _I, _P)
+_sig("tide_tree_info", _I, _P, _I, _P, _P)
+_sig("tide_tree_get", _I, _P, _I, _P, _P, _P, _P, _P)
+_sig("tide_model_info", _I, _P, _P, _P, _P, _P, _P)
+_sig("tide_get_edges", _I, _P, _I, _P, _I)
+_sig("tide_get_init_prefix", _I, _P, _P, _I)
+_sig("tide_profile", _I, _P, _I, _P, _P, _P)
+_sig("tide_fit_times", _I, _P, _P, _P)
+_sig("tide_tree_times", _I, _P, _P, _I)
+
+#: True: the gfx950 kernels are compiled into the loaded library (reference: tidegbm.cuda_compiled).
+gpu_compiled: bool = bool(_lib.tide_device_compiled())
+cuda_compiled = gpu_compiled  # the reference's flag name
+
+OBJECTIVES = {"mse": 0, "auto": 0, "l2": 0, "regression": 0, "pairwise": 5, "lambdarank": 5}
+_INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_buckets", "random_state",
+             "n_jobs", "iid_mode", "halo", "verbose", "horizon", "embargo", "goss", "cat_min_count",
+             "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
+             "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode",
+             "force_host_driver"}
+_DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight",
+                "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",
+                "max_delta_step", "torb_min_cover", "time_decay", "tree_decay"}
+
+
+class TideError(ValueError):
+    pass
+
+
+def _err() -> str:
+    return _lib.tide_last_error().decode(errors="replace")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def device_count() -> int:
+    return int(_lib.tide_device_count())
+
+
+class Booster:
+    """Owner of one native handle (the reference's Booster behind its C API)."""
+
+    def __init__(self, handle=None, **params):
+        h = _P()
+        if handle is None:
+            if _lib.tide_create(ctypes.byref(h)) != 0:
+                raise TideError(_err())
+            self._h = h
+        else:
+            self._h = handle
+        self.set_params(**params)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.tide_free(h)
+            self._h = None
+
+    def set_params(self, **params):
+        for k, v in params.items():
+            if v is None:
+                continue
+            kb = k.encode()
+            if k in ("device",):
+                rc = _lib.tide_set_string(self._h, kb, str(v).encode())
+            elif k == "objective":
+                rc = _lib.tide_set_string(self._h, kb, str(v).encode())
+            elif k in _INT_KEYS:
+                rc = _lib.tide_set_int(self._h, kb, int(v))
+            elif k in _DOUBLE_KEYS:
+                rc = _lib.tide_set_double(self._h, kb, float(v))
+            else:
+                raise TideError(f"unknown parameter {k}")
+            if rc != 0:
+                raise TideError(_err())
+        return self
+
+    def fit(self, X, y, *, timestamps=None, weight=None, entity=None, rank_groups=None,
+            iid: bool = False, horizon: int = -1):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        n, F = X.shape
+        ts = None if timestamps is None else np.ascontiguousarray(timestamps, dtype=np.int64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        ent = None if entity is None else np.ascontiguousarray(entity, dtype=np.int64)
+        rg = None if rank_groups is None else np.ascontiguousarray(rank_groups, dtype=np.int64)
+        rc = _lib.tide_fit_ex(self._h, _ptr(X), _ptr(y), n, F, _ptr(ts), _ptr(w), None, _ptr(ent),
+                              _ptr(rg), 1 if iid else 0, int(horizon))
+        if rc != 0:
+            raise TideError(_err())
+        return self
+
+    def update(self, X, y, extra_trees: int, weight=None):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        if _lib.tide_update(self._h, _ptr(X), _ptr(y), X.shape[0], X.shape[1], _ptr(w), int(extra_trees)) != 0:
+            raise TideError(_err())
+        return self
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        out = np.empty(X.shape[0], dtype=np.float64)
+        if _lib.tide_predict(self._h, _ptr(X), X.shape[0], X.shape[1], _ptr(out)) != 0:
+            raise TideError(_err())
+        return out
+
+    def predict_binned(self, xb):
+        xb = np.ascontiguousarray(xb, dtype=np.uint16)
+        out = np.empty(xb.shape[0], dtype=np.float64)
+        if _lib.tide_predict_binned(self._h, _ptr(xb), xb.shape[0], _ptr(out)) != 0:
+            raise TideError(_err())
+        return out
+
+    def save(self, path, json: bool = False):
+        fn = _lib.tide_save_json if json else _lib.tide_save
+        if fn(self._h, str(path).encode()) != 0:
+            raise TideError(_err())
+
+    @classmethod
+    def load(cls, path):
+        h = _P()
+        if _lib.tide_load(ctypes.byref(h), str(path).encode()) != 0:
+            raise TideError(_err())
+        return cls(handle=h)
+
+    # ---- inspection
+    def model_info(self):
+        nf, nb, nt, bi = _I(), _I(), _I(), _I()
+        init = _D()
+        _lib.tide_model_info(self._h, ctypes.byref(nf), ctypes.byref(nb), ctypes.byref(nt), ctypes.byref(bi),
+                             ctypes.byref(init))
+        return dict(n_features=nf.value, n_bins=nb.value, n_time_buckets=nt.value,
+                    best_iteration=bi.value, init_score=init.value)
+
+    def trees(self):
+        return [tree_arrays(self._h, _lib.tide_tree_info, _lib.tide_tree_get, t)
+                for t in range(_lib.tide_num_trees(self._h))]
+
+    def edges(self, f):
+        buf = np.empty(256)
+        m = _lib.tide_get_edges(self._h, f, _ptr(buf), 256)
+        if m < 0:
+            raise TideError(_err())
+        return buf[:m].copy()
+
+    def init_prefix(self):
+        buf = np.empty(512)
+        m = _lib.tide_get_init_prefix(self._h, _ptr(buf), 512)
+        return buf[:m].copy()
+
+    def profile(self, family: int):
+        ms, n, b = _D(), _LL(), _D()
+        _lib.tide_profile(self._h, family, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
+        return ms.value, n.value, b.value
+
+    def tree_times(self):
+        buf = np.empty(1 << 16)
+        m = _lib.tide_tree_times(self._h, _ptr(buf), buf.size)
+        return buf[:m].copy()
+
+    def fit_times(self):
+        a, b = _D(), _D()
+        _lib.tide_fit_times(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+
+def tree_arrays(h, info_fn, get_fn, t):
+    """Trees as dicts of numpy arrays: ints (nn,10), dbl (nn,2), values, cover, prefix (nb*nn)."""
+    nn, nb = _I(), _I()
+    info_fn(h, t, ctypes.byref(nn), ctypes.byref(nb))
+    ints = np.empty((nn.value, 10), np.int32)
+    dbl = np.empty((nn.value, 2))
+    vals = np.empty(nn.value)
+    cov = np.empty(nn.value, np.int32)
+    pref = np.empty(nn.value * nb.value)
+    get_fn(h, t, _ptr(ints), _ptr(dbl), _ptr(vals), _ptr(cov), _ptr(pref))
+    return dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value)
+
+
+# ---------------------------------------------------------------- estimator mirror
+def _median_step(ts: np.ndarray) -> int:  # reference utils.py:62-70
+    u = np.unique(ts)
+    if u.size < 2:
+        return 1
+    d = np.diff(np.sort(u))
+    d = d[d > 0]
+    return 1 if d.size == 0 else int(np.median(d))
+
+
+def parse_horizon(value, ts: np.ndarray) -> int:
+    """Horizon/embargo in timestamp units (reference utils.py:47-59, integer timestamps)."""
+    if value is None:
+        return 0
+    if isinstance(value, (int, np.integer)):
+        return int(value) * max(_median_step(ts), 1)
+    if isinstance(value, float) and value.is_integer():
+        return parse_horizon(int(value), ts)
+    raise TideError("horizon must be an integer number of timestamp steps on this path")
+
+
+class TideRegressor:
+    """Estimator front end for the GPU path (reference TideEstimator, api.py:38-300)."""
+
+    _objective = "mse"
+
+    def __init__(self, n_estimators=100, learning_rate=0.1, max_depth=6, max_leaves=31,
+                 min_samples_leaf=20, l2=1.0, max_bin=255, horizon=None, embargo=None,
+                 iid_mode=False, pair_splits=True, goss=-1, random_state=0, device="gpu",
+                 objective=None, **kw):
+        self.params = dict(n_estimators=n_estimators, learning_rate=learning_rate, max_depth=max_depth,
+                           max_leaves=max_leaves, min_samples_leaf=min_samples_leaf, l2=l2,
+                           max_bin=max_bin, pair_splits=pair_splits, goss=goss,
+                           random_state=random_state, device=device,
+                           objective=objective or self._objective, **kw)
+        self.horizon = horizon
+        self.embargo = embargo
+        self.iid_mode = iid_mode
+        self.booster_: Booster | None = None
+
+    def fit(self, X, y, *, timestamp=None, entity=None, sample_weight=None):
+        X = np.asarray(X, dtype=np.float64)
+        y = np.asarray(y, dtype=np.float64).reshape(-1)
+        if not self.iid_mode and timestamp is None:
+            raise ValueError("timestamp is required. Pass timestamp=... or set iid_mode=True "
+                             "(iid_mode disables TORB and is unsafe for market data).")
+        w = np.ones(len(y)) if sample_weight is None else np.asarray(sample_weight, dtype=np.float64)
+        ts = ent = rg = None
+        horizon = -1
+        p = dict(self.params)
+        if not self.iid_mode:
+            ts = np.asarray(timestamp).astype(np.int64)
+            if self.horizon is None:
+                raise ValueError("horizon is required in time mode (label length in timestamp units).")
+            horizon = parse_horizon(self.horizon, ts)
+            p["embargo"] = parse_horizon(self.embargo, ts)
+            ent = np.zeros(len(y), dtype=np.int64)
+            if entity is not None:
+                _, ent = np.unique(np.asarray(entity), return_inverse=True)
+                ent = ent.astype(np.int64)
+            order = np.lexsort((ent, ts))
+            if not np.all(order == np.arange(order.size)):
+                X, y, ts, w, ent = X[order], y[order], ts[order], w[order], ent[order]
+            if OBJECTIVES.get(str(p["objective"])) == 5:
+                rg = ts
+        b = Booster(**p)
+        b.fit(X, y, timestamps=ts, weight=None if np.all(w == 1.0) else w, entity=ent,
+              rank_groups=rg, iid=self.iid_mode, horizon=horizon)
+        self.booster_ = b
+        return self
+
+    def predict(self, X):
+        if self.booster_ is None:
+            raise ValueError("not fitted")
+        return self.booster_.predict(X)
+
+
+class TideRanker(TideRegressor):
+    _objective = "pairwise"
+
+
+__all__ = ["Booster", "TideRegressor", "TideRanker", "TideError", "gpu_compiled", "cuda_compiled",
+           "device_count", "parse_horizon", "tree_arrays", "OBJECTIVES"]

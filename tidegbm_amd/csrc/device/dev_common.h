@@ -1,0 +1,209 @@
+// Shared device-side definitions for the gfx950 histogram-GBDT kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tgbm {
+
+// ---------------------------------------------------------------- sizes
+constexpr int NBIN = 256;          // histogram stride per feature (uint8 bins: 0 = missing)
+constexpr int MAX_NODES = 256;     // per tree (max_leaves <= 128)
+constexpr int MAX_FEATURES = 1024;
+constexpr int MAX_BUCKETS = 256;   // TORB buckets (+1 sweep segment for rows past the last cut)
+
+// ---------------------------------------------------------------- fixed point
+// Histogram sums are exact integer sums of per-row quantized values, so they are independent
+// of summation order (deterministic under any atomic schedule) and sibling subtraction is
+// exact.  g is quantized as q = rint(g * 2^s) with |q| < 2^QG_BITS; in LDS one 64-bit word per
+// (feature, bin) packs count << 48 | q-sum (48-bit two's complement), valid while a block folds
+// at most MAX_FLUSH_ROWS rows (|sum| < 2^47, count < 2^16).  h (when not identically 1) gets
+// its own unsigned 64-bit word with |q_h| < 2^QH_BITS.
+constexpr int CNT_SHIFT = 48;
+constexpr int QG_BITS = 31;
+constexpr int QH_BITS = 39;
+constexpr int QR_BITS = 39;        // refit / TORB sums over all rows (plain int64)
+constexpr int QE_BITS = 39;        // leaf energy sum of g^2
+constexpr int MAX_FLUSH_ROWS = 32768;
+constexpr unsigned long long CNT_ONE = 1ull << CNT_SHIFT;
+
+__host__ __device__ inline long long unpack_q(unsigned long long v) {
+    return (long long)(v << (64 - CNT_SHIFT)) >> (64 - CNT_SHIFT);
+}
+__host__ __device__ inline unsigned int unpack_c(unsigned long long v) {
+    const long long q = unpack_q(v);
+    return (unsigned int)((v - (unsigned long long)q) >> CNT_SHIFT);
+}
+
+// Exponent e with |x| < 2^e for the largest magnitude x (bits of a non-negative double).
+__host__ __device__ inline int mag_exponent(unsigned long long max_bits) {
+    const double m = __longlong_as_double((long long)max_bits);
+    if (!(m > 0.0)) return -1074;
+    int e;
+    frexp(m, &e);  // m = f * 2^e, f in [0.5, 1)  =>  m < 2^e
+    return e;
+}
+
+// ---------------------------------------------------------------- tree records
+// Node layout: v[] follows the export order {is_leaf, split_type, feature, feature2, thr,
+// left, right, missing_go_left, n_samples, depth} of reference Node (tree.hpp:11-25).
+struct DNode {
+    int v[10];
+    int seg_start, seg_count;   // rows of this node in the task's partitioned index array
+    int pad0, pad1;
+    double value, gain;
+};
+enum { N_LEAF = 0, N_TYPE, N_FEAT, N_FEAT2, N_THR, N_LEFT, N_RIGHT, N_MGL, N_NS, N_DEPTH };
+
+struct FCand {      // best split of one feature of one node
+    double gain;
+    int thr, ml;
+};
+
+struct DCand {      // cached leaf candidate
+    double gain;
+    int feature, feature2, thr, type, mgl, valid;
+};
+
+// Per-task state and pointers, resident in device memory (one entry per task).
+struct Task {
+    // ---- fixed at creation
+    int n_rows;            // this task uses rows [0, n_rows) of the shared matrix
+    int n_buckets;
+    int hconst;            // h == 1 for every row
+    int has_groups;
+    // ---- per tree (host resets at tree begin)
+    int n_sample;
+    int contiguous;        // sample == rows [0, n_sample)
+    int n_nodes;
+    int done;
+    int s_g, s_h;          // quantization exponents for histogram g/h
+    int s_r, s_e;          // refit/TORB (g) and energy exponents
+    int s_rh, pad_s;       // refit/TORB exponent for h
+    double goss_scale;
+    int goss;              // 1 when some rows use scaled g/h
+    int pending0, pending1; // nodes scanned in the last step (-1 = none) awaiting reduction
+    int pad_p;
+    // ---- current split
+    int cur_leaf, cur_type, cur_feat, cur_feat2, cur_thr, cur_mgl;
+    double cur_gain;
+    int n_left;
+    int pad_a;
+    // ---- reductions
+    unsigned long long max_g_bits, max_h_bits;
+    // ---- per-row arrays (length n_rows)
+    const double* y;
+    const double* w;       // may be null (all 1)
+    double* pred;
+    double* g;
+    double* h;
+    unsigned long long* pk;   // packed (1<<48) + q_g
+    unsigned long long* qh;   // q_h (only when !hconst)
+    const int* bucket;
+    const unsigned char* seg; // TORB sweep segment per row
+    const unsigned char* scaled; // GOSS flag per row (may be null)
+    int* nid;
+    int* idx;              // partitioned sample rows
+    int* idx_tmp;
+    const int* group_start; // pairwise groups: positions [gs[k], gs[k+1]) of group_rows
+    const int* group_rows;  // rows in stable group order (null: identity, groups contiguous)
+    int n_groups;
+    int pad_b;
+    // ---- partition scratch
+    int* chunk_cnt;
+    // ---- tree state
+    DNode* nodes;          // [MAX_NODES]
+    DCand* cands;          // [MAX_NODES]
+    FCand* fcand;          // [MAX_NODES][n_features]
+    long long* hg;         // [MAX_NODES][F][NBIN]
+    int* hc;               // [MAX_NODES][F][NBIN]
+    long long* hh;         // [MAX_NODES][F][NBIN] (only when !hconst)
+    long long* energy;     // [MAX_NODES] fixed-point sum of g^2 (host-driven mode)
+    // ---- TORB
+    const int* cuts;       // [n_buckets]
+    long long* tg;         // [(MAX_BUCKETS+1) * MAX_NODES]
+    long long* th;
+    int* tc;
+    double* values;        // [MAX_NODES]
+    double* prefix;        // [n_buckets * MAX_NODES]
+    int* cover;            // [MAX_NODES]
+};
+
+struct Shared {            // the session-wide matrix
+    int n_rows, n_features, fp, n_pad, n_bins;
+    const unsigned char* bins_row;   // [n_rows][fp]
+    const unsigned char* bins_col;   // [n_features][n_pad]
+};
+
+// A run of rows inside one TORB sweep segment (rows are time sorted, segment s = rows
+// [cut_{s-1}, cut_s), s = n_buckets for rows past the last cut).
+struct TorbChunk {
+    int r0, r1, s, pad;
+};
+
+// Grow parameters copied to kernels by value.
+struct GrowP {
+    int max_depth, max_leaves, min_leaf;
+    double min_hess, l2, l1, mds, gamma, pair_l2;
+};
+
+// ---------------------------------------------------------------- leaf formulas
+// Reference tree.hpp:43-67 and histogram.cpp:119-142 (no FMA: -ffp-contract=off).
+__device__ inline double soft_thr(double G, double l1) {
+    if (l1 <= 0) return G;
+    if (G > l1) return G - l1;
+    if (G < -l1) return G + l1;
+    return 0.0;
+}
+__device__ inline double leaf_score(double G, double H, double lam, double l1) {
+    const double t = soft_thr(G, l1);
+    return (t * t) / (H + lam);
+}
+__device__ inline double leaf_value(double G, double H, double l2, double l1, double mds) {
+    double v;
+    if (l1 > 0) {
+        if (G > l1) v = -(G - l1) / (H + l2);
+        else if (G < -l1) v = -(G + l1) / (H + l2);
+        else v = 0.0;
+    } else {
+        v = -G / (H + l2);
+    }
+    if (mds > 0) {
+        if (v > mds) v = mds;
+        if (v < -mds) v = -mds;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- wave helpers (wave64)
+__device__ inline long long wave_sum_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline int wave_sum_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Inclusive prefix over lanes.
+__device__ inline long long wave_incl_ll(long long v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+__device__ inline int wave_incl_i(int v) {
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Candidate ordering: larger gain first, then smaller key (feature/bin/missing order).
+__device__ inline bool cand_better(double ga, int ka, double gb, int kb) {
+    return ga > gb || (ga == gb && ka < kb);
+}
+
+}  // namespace tgbm

@@ -1,0 +1,241 @@
+// Ensemble predict on gfx950 (reference Booster::predict_raw_binned, booster.cpp:744-773):
+// raw score = init_score + sum over trees, in tree order, of lr_t * values_t[leaf(row)].
+// Built with -ffp-contract=off so the running sum rounds exactly like the reference loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/tgbm_device.h"
+#include "dev_common.h"
+#include "launch.h"
+
+using namespace tgbm;
+
+extern thread_local std::string g_dev_err;  // defined in session.hip
+
+#define FHIP(expr)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                     " at " #expr);                                          \
+    } while (0)
+
+// Packed node: x = feature | feature2 << 16, y = threshold, z = left, w = right (absolute).
+// Flags in a separate byte array: bit0 leaf, bits1-2 split type, bit3 missing_go_left.
+struct FNode {
+    int fx, thr, left, right;
+};
+
+struct tgbm_forest {
+    int F = 0, fp = 0, nb = 255, n_trees = 0, n_nodes = 0;
+    double init = 0.0;
+    FNode* nodes = nullptr;
+    unsigned char* flags = nullptr;
+    double* values = nullptr;
+    int* roots = nullptr;
+    double* lr = nullptr;
+    double* edges = nullptr;
+    int* n_edges = nullptr;
+    hipStream_t stream = nullptr;
+    // staging for requests
+    double* Xd = nullptr;
+    unsigned char* bins = nullptr;
+    unsigned char* bins_col_dummy = nullptr;
+    double* outd = nullptr;
+    int cap = 0;
+    ~tgbm_forest() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : {(void*)nodes, (void*)flags, (void*)values, (void*)roots, (void*)lr, (void*)edges,
+                        (void*)n_edges, (void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void reserve(int n) {
+        if (n <= cap) return;
+        for (void* p : {(void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
+            if (p) (void)hipFree(p);
+        cap = std::max(n, 1024);
+        FHIP(hipMalloc(&Xd, sizeof(double) * (size_t)cap * F));
+        FHIP(hipMalloc(&bins, (size_t)cap * fp + 16));
+        FHIP(hipMalloc(&bins_col_dummy, (size_t)F * (cap + 128)));
+        FHIP(hipMalloc(&outd, sizeof(double) * cap));
+    }
+};
+
+// One thread per row; the row's bins are re-read through L1/L2 at every visited node.
+__global__ __launch_bounds__(256) void k_forest(const FNode* __restrict__ nodes,
+                                                const unsigned char* __restrict__ flags,
+                                                const double* __restrict__ values,
+                                                const int* __restrict__ roots,
+                                                const double* __restrict__ lr, int n_trees,
+                                                double init, const unsigned char* __restrict__ bins,
+                                                int fp, int n, int nb, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned char* row = bins + (size_t)i * fp;
+    double s = init;
+    for (int t = 0; t < n_trees; ++t) {
+        int k = roots[t];
+        unsigned char fl = flags[k];
+        while (!(fl & 1)) {
+            const FNode nd = nodes[k];
+            const int b1 = row[nd.fx & 0xffff];
+            bool left;
+            if (((fl >> 1) & 3) == 1) {
+                const int b2 = row[nd.fx >> 16];
+                if (b1 == 0 || b2 == 0) left = (fl >> 3) & 1;
+                else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)nb) <= nd.thr;
+            } else {
+                left = b1 == 0 ? ((fl >> 3) & 1) : (b1 <= nd.thr);
+            }
+            k = left ? nd.left : nd.right;
+            fl = flags[k];
+        }
+        s += lr[t] * values[k];
+    }
+    out[i] = s;
+}
+
+__global__ void k_u16_to_u8(const unsigned short* __restrict__ in, int n, int F, unsigned char* __restrict__ out, int fp) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)n * F) return;
+    const size_t i = j / F, f = j % F;
+    const unsigned short v = in[j];
+    out[i * fp + f] = (unsigned char)(v > 255 ? 255 : v);
+}
+
+template <class Fn>
+static int fguard(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        g_dev_err = e.what();
+        return -1;
+    }
+}
+
+extern "C" {
+
+int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_trees, const int* tree_nodes,
+                       const int* node_ints, const double* values, const double* lr, double init_score,
+                       const double* edges, const int* n_edges) {
+    *out = nullptr;
+    return fguard([&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw std::runtime_error("GPU device not available: no HIP device found");
+        if (n_features <= 0 || n_trees < 0) throw std::runtime_error("bad forest shape");
+        auto* f = new tgbm_forest();
+        std::unique_ptr<tgbm_forest> guard(f);
+        f->F = n_features;
+        f->fp = (n_features + 15) / 16 * 16;
+        f->nb = n_bins;
+        f->n_trees = n_trees;
+        f->init = init_score;
+        int total = 0;
+        for (int t = 0; t < n_trees; ++t) total += tree_nodes[t];
+        f->n_nodes = total;
+        std::vector<FNode> nodes(std::max(total, 1));
+        std::vector<unsigned char> flags(std::max(total, 1));
+        std::vector<int> roots(std::max(n_trees, 1));
+        int base = 0;
+        for (int t = 0; t < n_trees; ++t) {
+            roots[t] = base;
+            for (int k = 0; k < tree_nodes[t]; ++k) {
+                const int* v = node_ints + (size_t)(base + k) * TGBM_NODE_INTS;
+                const bool leaf = v[0] != 0;
+                FNode nd{0, 0, 0, 0};
+                if (!leaf) {
+                    if (v[2] < 0 || v[2] >= n_features || v[5] < 0 || v[6] < 0 || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
+                        throw std::runtime_error("malformed tree node");
+                    if (v[1] != 0 && v[1] != 1) throw std::runtime_error("only axis and pair splits run on the device");
+                    const int f2 = v[1] == 1 ? v[3] : 0;
+                    if (f2 < 0 || f2 >= n_features) throw std::runtime_error("malformed pair node");
+                    nd.fx = v[2] | (f2 << 16);
+                    nd.thr = v[4];
+                    nd.left = base + v[5];
+                    nd.right = base + v[6];
+                }
+                nodes[base + k] = nd;
+                flags[base + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
+            }
+            base += tree_nodes[t];
+        }
+        FHIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+        FHIP(hipMalloc(&f->nodes, sizeof(FNode) * nodes.size()));
+        FHIP(hipMalloc(&f->flags, flags.size()));
+        FHIP(hipMalloc(&f->values, sizeof(double) * std::max(total, 1)));
+        FHIP(hipMalloc(&f->roots, sizeof(int) * roots.size()));
+        FHIP(hipMalloc(&f->lr, sizeof(double) * std::max(n_trees, 1)));
+        FHIP(hipMalloc(&f->edges, sizeof(double) * (size_t)n_features * NBIN));
+        FHIP(hipMalloc(&f->n_edges, sizeof(int) * n_features));
+        FHIP(hipMemcpy(f->nodes, nodes.data(), sizeof(FNode) * nodes.size(), hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
+        if (total) FHIP(hipMemcpy(f->values, values, sizeof(double) * total, hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->roots, roots.data(), sizeof(int) * roots.size(), hipMemcpyHostToDevice));
+        if (n_trees) FHIP(hipMemcpy(f->lr, lr, sizeof(double) * n_trees, hipMemcpyHostToDevice));
+        if (edges) {
+            FHIP(hipMemcpy(f->edges, edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyHostToDevice));
+            FHIP(hipMemcpy(f->n_edges, n_edges, sizeof(int) * n_features, hipMemcpyHostToDevice));
+        }
+        *out = guard.release();
+    });
+}
+
+void tgbm_forest_destroy(tgbm_forest* f) { delete f; }
+
+static void forest_run(tgbm_forest* f, int n, double* out) {
+    if (n > 0) {
+        hipLaunchKernelGGL(k_forest, dim3((n + 255) / 256), dim3(256), 0, f->stream, f->nodes, f->flags, f->values,
+                           f->roots, f->lr, f->n_trees, f->init, f->bins, f->fp, n, f->nb, f->outd);
+        FHIP(hipGetLastError());
+        FHIP(hipMemcpyAsync(out, f->outd, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
+    }
+    FHIP(hipStreamSynchronize(f->stream));
+}
+
+int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out) {
+    return fguard([&] {
+        if (n < 0) throw std::runtime_error("n < 0");
+        if (n == 0) return;
+        f->reserve(n);
+        FHIP(hipMemcpyAsync(f->Xd, X, sizeof(double) * (size_t)n * f->F, hipMemcpyHostToDevice, f->stream));
+        FHIP(hipMemsetAsync(f->bins, 0, (size_t)n * f->fp, f->stream));
+        launch_bin(f->stream, f->Xd, n, f->F, f->edges, f->n_edges, f->bins, f->fp, f->bins_col_dummy, n + 64);
+        FHIP(hipGetLastError());
+        forest_run(f, n, out);
+    });
+}
+
+int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out) {
+    return fguard([&] {
+        if (n < 0) throw std::runtime_error("n < 0");
+        if (n == 0) return;
+        f->reserve(n);
+        unsigned short* d = nullptr;
+        FHIP(hipMalloc(&d, sizeof(unsigned short) * (size_t)n * f->F));
+        try {
+            FHIP(hipMemcpyAsync(d, xb, sizeof(unsigned short) * (size_t)n * f->F, hipMemcpyHostToDevice, f->stream));
+            FHIP(hipMemsetAsync(f->bins, 0, (size_t)n * f->fp, f->stream));
+            const size_t tot = (size_t)n * f->F;
+            hipLaunchKernelGGL(k_u16_to_u8, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, f->stream, d, n, f->F,
+                               f->bins, f->fp);
+            FHIP(hipGetLastError());
+            forest_run(f, n, out);
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        (void)hipFree(d);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,29 @@
+// Host-side launch wrappers of the fit and predict kernels (defined next to the kernels).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include "dev_common.h"
+
+namespace tgbm {
+
+void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edges,
+                const int* n_edges, unsigned char* bins_row, int fp, unsigned char* bins_col,
+                int n_pad);
+void launch_grad(hipStream_t st, Task* tasks, int n_tasks, int objective, int max_rows,
+                 int max_groups);
+void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows);
+void launch_zero_slots(hipStream_t st, Task* tasks, int n_tasks, int F, int first, int count);
+void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
+                 bool contiguous_root, bool hconst, int max_count, int variant);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
+void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P, int mode);
+void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count);
+void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count);
+void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int node, int count,
+                 const int* pf, int n_pairs, long long* ph_g, int* ph_c, long long* ph_h,
+                 bool hconst, const GrowP& P, FCand* out);
+void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
+                     int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
+                     int max_chunks, const GrowP& P, double min_cover, int iid, double lr);
+
+}  // namespace tgbm

@@ -1,0 +1,699 @@
+// Thin C ABI (include/tgbm_device.h) over the gfx950 fit kernels: device memory ownership,
+// kernel sequencing for host-driven and device-driven best-first growth, and tree export.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <cstddef>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/tgbm_device.h"
+#include "dev_common.h"
+#include "launch.h"
+
+using namespace tgbm;
+
+thread_local std::string g_dev_err;  // shared by every entry point of this library
+
+#define HIP_OK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                     " at " #expr);                                          \
+    } while (0)
+
+template <class Fn>
+static int dev_guard(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        g_dev_err = e.what();
+        return -1;
+    }
+}
+
+template <class T>
+static T* dalloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    HIP_OK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+struct TaskHost {
+    int n_rows = 0;
+    int n_buckets = 1;
+    std::vector<TorbChunk> chunks;
+    bool has_sample = false;
+    int n_sample = 0;
+    bool goss = false;
+    int max_group = 0;
+};
+
+struct EventPair {
+    hipEvent_t a, b;
+    int family;
+    double bytes;
+};
+
+struct tgbm_session {
+    int n = 0, F = 0, fp = 0, n_pad = 0, nb = 255, T = 1, max_leaves = 31, max_nodes = 61;
+    bool hconst = true;
+    hipStream_t stream = nullptr;
+    Shared S{};
+    unsigned char* bins_row = nullptr;
+    unsigned char* bins_col = nullptr;
+    Task* tasks_d = nullptr;
+    std::vector<Task> tasks_h;      // pointer mirror (scalars are device-owned)
+    std::vector<TaskHost> th;
+    std::vector<void*> allocs;
+    // TORB chunk lists (device): array of per-task pointers
+    TorbChunk** chunk_ptrs_d = nullptr;
+    int* n_chunks_d = nullptr;
+    int max_chunks = 1;
+    // PAIR scratch
+    int* pair_f_d = nullptr;
+    long long* ph_g = nullptr;
+    int* ph_c = nullptr;
+    long long* ph_h = nullptr;
+    FCand* pair_out_d = nullptr;
+    // timers
+    bool timing = false;
+    std::vector<EventPair> events;
+    double t_ms[8] = {0};
+    long long t_n[8] = {0};
+    double t_bytes[8] = {0};
+
+    template <class T>
+    T* alloc(size_t n) {
+        T* p = dalloc<T>(n);
+        allocs.push_back(p);
+        return p;
+    }
+    ~tgbm_session() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : events) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void set_field_int(int t, size_t off, int v) {
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(tasks_d + t) + off, &v, sizeof v, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    void set_field_double(int t, size_t off, double v) {
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(tasks_d + t) + off, &v, sizeof v, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    void set_field_ptr(int t, size_t off, const void* v) {
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(tasks_d + t) + off, &v, sizeof v, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+    }
+    Task read_task(int t) {
+        Task out;
+        HIP_OK(hipMemcpyAsync(&out, tasks_d + t, sizeof(Task), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return out;
+    }
+    GrowP growp(const tgbm_grow_params* p) const {
+        GrowP g;
+        g.max_depth = p->max_depth;
+        g.max_leaves = p->max_leaves;
+        g.min_leaf = p->min_samples_leaf;
+        g.min_hess = p->min_child_weight;
+        g.l2 = p->l2;
+        g.l1 = p->l1;
+        g.mds = p->max_delta_step;
+        g.gamma = p->min_split_gain;
+        g.pair_l2 = p->pair_l2;
+        return g;
+    }
+    int max_rows() const {
+        int m = 0;
+        for (auto& t : th) m = std::max(m, t.n_rows);
+        return m;
+    }
+    int max_sample() const {
+        int m = 0;
+        for (auto& t : th) m = std::max(m, t.has_sample ? t.n_sample : t.n_rows);
+        return m;
+    }
+    bool all_contiguous() const {
+        for (auto& t : th)
+            if (t.has_sample) return false;
+        return true;
+    }
+    // -------- timers
+    void tic(int family, double bytes, EventPair& ep) {
+        ep.family = family;
+        ep.bytes = bytes;
+        HIP_OK(hipEventCreate(&ep.a));
+        HIP_OK(hipEventCreate(&ep.b));
+        HIP_OK(hipEventRecord(ep.a, stream));
+    }
+    void toc(EventPair& ep) {
+        HIP_OK(hipEventRecord(ep.b, stream));
+        events.push_back(ep);
+    }
+    void drain_events() {
+        if (events.empty()) return;
+        HIP_OK(hipStreamSynchronize(stream));
+        for (auto& e : events) {
+            float ms = 0;
+            HIP_OK(hipEventElapsedTime(&ms, e.a, e.b));
+            t_ms[e.family] += ms;
+            t_n[e.family] += 1;
+            t_bytes[e.family] += e.bytes;
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+        events.clear();
+    }
+};
+
+extern "C" {
+
+const char* tgbm_last_error(void) { return g_dev_err.c_str(); }
+
+int tgbm_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        g_dev_err = "no HIP GPU device available";
+        return 0;
+    }
+    return n;
+}
+
+int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bins, int n_tasks,
+                        const int* task_rows, int max_leaves, int h_const) {
+    *out = nullptr;
+    return dev_guard([&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw std::runtime_error("GPU device not available: no HIP device found");
+        if (n_rows <= 0 || n_features <= 0) throw std::runtime_error("empty input");
+        if (n_features > MAX_FEATURES) throw std::runtime_error("too many features for the device path");
+        if (n_bins < 2 || n_bins > NBIN) throw std::runtime_error("n_bins must be in [2, 256] on the device path");
+        if (max_leaves < 2 || 2 * max_leaves - 1 > MAX_NODES)
+            throw std::runtime_error("max_leaves must be in [2, 128] on the device path");
+        if (n_tasks < 1) throw std::runtime_error("n_tasks must be >= 1");
+        auto* s = new tgbm_session();
+        std::unique_ptr<tgbm_session> guard(s);
+        s->n = n_rows;
+        s->F = n_features;
+        s->fp = (n_features + 15) / 16 * 16;
+        s->n_pad = (n_rows + 63) / 64 * 64 + 64;
+        s->nb = n_bins;
+        s->T = n_tasks;
+        s->max_leaves = max_leaves;
+        s->max_nodes = 2 * max_leaves - 1;
+        s->hconst = h_const != 0;
+        HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->bins_row = s->alloc<unsigned char>((size_t)n_rows * s->fp);
+        s->bins_col = s->alloc<unsigned char>((size_t)n_features * s->n_pad);
+        HIP_OK(hipMemsetAsync(s->bins_row, 0, (size_t)n_rows * s->fp, s->stream));
+        HIP_OK(hipMemsetAsync(s->bins_col, 0, (size_t)n_features * s->n_pad, s->stream));
+        s->S.n_rows = n_rows;
+        s->S.n_features = n_features;
+        s->S.fp = s->fp;
+        s->S.n_pad = s->n_pad;
+        s->S.n_bins = n_bins;
+        s->S.bins_row = s->bins_row;
+        s->S.bins_col = s->bins_col;
+        s->tasks_h.assign(n_tasks, Task{});
+        s->th.assign(n_tasks, TaskHost{});
+        const size_t per_node = (size_t)n_features * NBIN;
+        for (int t = 0; t < n_tasks; ++t) {
+            const int m = task_rows ? task_rows[t] : n_rows;
+            if (m <= 0 || m > n_rows) throw std::runtime_error("task rows out of range");
+            Task& T = s->tasks_h[t];
+            std::memset(&T, 0, sizeof T);
+            s->th[t].n_rows = m;
+            T.n_rows = m;
+            T.n_buckets = 1;
+            T.hconst = s->hconst;
+            T.n_sample = m;
+            T.contiguous = 1;
+            T.goss_scale = 1.0;
+            T.y = s->alloc<double>(m);
+            T.w = nullptr;
+            T.pred = s->alloc<double>(m);
+            T.g = s->alloc<double>(m);
+            T.h = s->alloc<double>(m);
+            T.pk = s->alloc<unsigned long long>(m);
+            T.qh = s->hconst ? nullptr : s->alloc<unsigned long long>(m);
+            T.bucket = s->alloc<int>(m);
+            T.nid = s->alloc<int>(m);
+            T.idx = s->alloc<int>(m);
+            T.idx_tmp = s->alloc<int>(m);
+            T.chunk_cnt = s->alloc<int>(m / 4096 + 2);
+            T.nodes = s->alloc<DNode>(MAX_NODES);
+            T.cands = s->alloc<DCand>(MAX_NODES);
+            T.fcand = s->alloc<FCand>((size_t)MAX_NODES * n_features);
+            T.hg = s->alloc<long long>(per_node * s->max_nodes);
+            T.hc = s->alloc<int>(per_node * s->max_nodes);
+            T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * s->max_nodes);
+            T.energy = s->alloc<long long>(MAX_NODES);
+            T.cuts = s->alloc<int>(MAX_BUCKETS);
+            const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
+            T.tg = s->alloc<long long>(tw);
+            T.th = s->hconst ? nullptr : s->alloc<long long>(tw);
+            T.tc = s->alloc<int>(tw);
+            T.values = s->alloc<double>(MAX_NODES);
+            T.prefix = s->alloc<double>((size_t)MAX_BUCKETS * MAX_NODES);
+            T.cover = s->alloc<int>(MAX_NODES);
+            HIP_OK(hipMemsetAsync((void*)T.bucket, 0, sizeof(int) * m, s->stream));
+            HIP_OK(hipMemsetAsync((void*)T.cuts, 0, sizeof(int) * MAX_BUCKETS, s->stream));
+            s->th[t].chunks.push_back(TorbChunk{0, m, 1, 0});  // iid default: one segment past cut 0
+        }
+        s->tasks_d = s->alloc<Task>(n_tasks);
+        HIP_OK(hipMemcpyAsync(s->tasks_d, s->tasks_h.data(), sizeof(Task) * n_tasks, hipMemcpyHostToDevice, s->stream));
+        s->chunk_ptrs_d = s->alloc<TorbChunk*>(n_tasks);
+        s->n_chunks_d = s->alloc<int>(n_tasks);
+        s->pair_f_d = s->alloc<int>(2 * 64);
+        s->ph_g = s->alloc<long long>(64 * NBIN);
+        s->ph_c = s->alloc<int>(64 * NBIN);
+        s->ph_h = s->alloc<long long>(64 * NBIN);
+        s->pair_out_d = s->alloc<FCand>(64);
+        HIP_OK(hipStreamSynchronize(s->stream));
+        *out = guard.release();
+    });
+}
+
+void tgbm_session_destroy(tgbm_session* s) { delete s; }
+
+int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges) {
+    return dev_guard([&] {
+        for (int f = 0; f < s->F; ++f)
+            if (n_edges[f] < 0 || n_edges[f] > NBIN - 2)
+                throw std::runtime_error("a feature has more than 254 bin edges; the device path stores uint8 bins (max_bin <= 255)");
+        // stream X in row blocks to bound device memory
+        const int rows_blk = std::max(1, (int)std::min<size_t>((size_t)s->n, (512ull << 20) / (sizeof(double) * s->F)));
+        double* Xd = dalloc<double>((size_t)rows_blk * s->F);
+        double* Ed = dalloc<double>((size_t)s->F * NBIN);
+        int* Nd = dalloc<int>(s->F);
+        try {
+            HIP_OK(hipMemcpy(Ed, edges, sizeof(double) * s->F * NBIN, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(Nd, n_edges, sizeof(int) * s->F, hipMemcpyHostToDevice));
+            for (int r0 = 0; r0 < s->n; r0 += rows_blk) {
+                const int m = std::min(rows_blk, s->n - r0);
+                HIP_OK(hipMemcpyAsync(Xd, X + (size_t)r0 * s->F, sizeof(double) * m * s->F, hipMemcpyHostToDevice, s->stream));
+                launch_bin(s->stream, Xd, m, s->F, Ed, Nd, s->bins_row + (size_t)r0 * s->fp, s->fp,
+                           s->bins_col + r0, s->n_pad);
+                HIP_OK(hipGetLastError());
+                HIP_OK(hipStreamSynchronize(s->stream));
+            }
+        } catch (...) {
+            (void)hipFree(Xd);
+            (void)hipFree(Ed);
+            (void)hipFree(Nd);
+            throw;
+        }
+        (void)hipFree(Xd);
+        (void)hipFree(Ed);
+        (void)hipFree(Nd);
+    });
+}
+
+int tgbm_session_get_bins(tgbm_session* s, unsigned char* out) {
+    return dev_guard([&] {
+        std::vector<unsigned char> tmp((size_t)s->n * s->fp);
+        HIP_OK(hipMemcpy(tmp.data(), s->bins_row, tmp.size(), hipMemcpyDeviceToHost));
+        for (int i = 0; i < s->n; ++i) std::memcpy(out + (size_t)i * s->F, &tmp[(size_t)i * s->fp], s->F);
+    });
+}
+
+int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double* w, const double* pred0,
+                       const int* bucket, const unsigned char* sweep_seg, int n_buckets,
+                       const int* cuts, const long long* groups) {
+    (void)sweep_seg;
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        if (n_buckets < 1 || n_buckets > MAX_BUCKETS) throw std::runtime_error("n_buckets out of range for the device path");
+        Task& T = s->tasks_h[task];
+        TaskHost& H = s->th[task];
+        const int m = H.n_rows;
+        HIP_OK(hipMemcpy((void*)T.y, y, sizeof(double) * m, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(T.pred, pred0, sizeof(double) * m, hipMemcpyHostToDevice));
+        if (w) {
+            if (s->hconst) {
+                for (int i = 0; i < m; ++i)
+                    if (w[i] != 1.0) throw std::runtime_error("session declared h == 1 but weights are not all 1");
+            }
+            double* wd = s->alloc<double>(m);
+            HIP_OK(hipMemcpy(wd, w, sizeof(double) * m, hipMemcpyHostToDevice));
+            T.w = wd;
+            s->set_field_ptr(task, offsetof(Task, w), wd);
+        }
+        if (bucket) HIP_OK(hipMemcpy((void*)T.bucket, bucket, sizeof(int) * m, hipMemcpyHostToDevice));
+        H.n_buckets = n_buckets;
+        s->set_field_int(task, offsetof(Task, n_buckets), n_buckets);
+        std::vector<int> cut(n_buckets, 0);
+        if (cuts) std::copy(cuts, cuts + n_buckets, cut.begin());
+        HIP_OK(hipMemcpy((void*)T.cuts, cut.data(), sizeof(int) * n_buckets, hipMemcpyHostToDevice));
+        // sweep-segment chunk list: segment s covers [cut_{s-1}, cut_s), s = n_buckets past the last
+        H.chunks.clear();
+        const int CH = 8192;
+        int prev = 0;
+        for (int sg = 0; sg <= n_buckets; ++sg) {
+            const int end = sg < n_buckets ? std::min(std::max(cut[sg], prev), m) : m;
+            for (int r = prev; r < end; r += CH) H.chunks.push_back(TorbChunk{r, std::min(end, r + CH), sg, 0});
+            prev = std::max(prev, end);
+        }
+        // groups: runs of equal keys in stable key order (objective.cpp:18-39)
+        if (groups) {
+            std::vector<int> ord(m);
+            for (int i = 0; i < m; ++i) ord[i] = i;
+            bool sorted = true;
+            for (int i = 1; i < m; ++i)
+                if (groups[i] < groups[i - 1]) sorted = false;
+            if (!sorted) std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return groups[a] < groups[b]; });
+            std::vector<int> gs;
+            int i = 0, mx = 0;
+            while (i < m) {
+                int j = i + 1;
+                while (j < m && groups[ord[j]] == groups[ord[i]]) ++j;
+                gs.push_back(i);
+                mx = std::max(mx, j - i);
+                i = j;
+            }
+            gs.push_back(m);
+            int* gsd = s->alloc<int>(gs.size());
+            HIP_OK(hipMemcpy(gsd, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice));
+            s->set_field_ptr(task, offsetof(Task, group_start), gsd);
+            s->set_field_int(task, offsetof(Task, n_groups), (int)gs.size() - 1);
+            s->set_field_int(task, offsetof(Task, has_groups), 1);
+            if (!sorted) {
+                int* grd = s->alloc<int>(m);
+                HIP_OK(hipMemcpy(grd, ord.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+                s->set_field_ptr(task, offsetof(Task, group_rows), grd);
+            }
+            H.max_group = mx;
+        }
+        // upload the chunk lists of all tasks
+        std::vector<TorbChunk*> ptrs(s->T);
+        std::vector<int> counts(s->T);
+        s->max_chunks = 1;
+        for (int t = 0; t < s->T; ++t) {
+            auto& ch = s->th[t].chunks;
+            TorbChunk* d = s->alloc<TorbChunk>(ch.size());
+            HIP_OK(hipMemcpy(d, ch.data(), sizeof(TorbChunk) * ch.size(), hipMemcpyHostToDevice));
+            ptrs[t] = d;
+            counts[t] = (int)ch.size();
+            s->max_chunks = std::max(s->max_chunks, counts[t]);
+        }
+        HIP_OK(hipMemcpy(s->chunk_ptrs_d, ptrs.data(), sizeof(TorbChunk*) * s->T, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->n_chunks_d, counts.data(), sizeof(int) * s->T, hipMemcpyHostToDevice));
+    });
+}
+
+int tgbm_task_grad(tgbm_session* s, int task, int objective) {
+    return dev_guard([&] {
+        if (objective != 0 && objective != 5) throw std::runtime_error("objective not supported on the device path");
+        const int t0 = task < 0 ? 0 : task, nt = task < 0 ? s->T : 1;
+        int mg = 1;
+        for (int t = t0; t < t0 + nt; ++t) mg = std::max(mg, (int)std::max<size_t>(1, s->th[t].n_rows));
+        EventPair ep;
+        if (s->timing) s->tic(4, 0.0, ep);
+        launch_grad(s->stream, s->tasks_d + t0, nt, objective, s->max_rows(), mg);
+        if (s->timing) s->toc(ep);
+        HIP_OK(hipGetLastError());
+    });
+}
+
+int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h) {
+    return dev_guard([&] {
+        const Task& T = s->tasks_h[task];
+        const int m = s->th[task].n_rows;
+        HIP_OK(hipStreamSynchronize(s->stream));
+        if (g) HIP_OK(hipMemcpy(g, T.g, sizeof(double) * m, hipMemcpyDeviceToHost));
+        if (h) HIP_OK(hipMemcpy(h, T.h, sizeof(double) * m, hipMemcpyDeviceToHost));
+    });
+}
+
+int tgbm_task_get_pred(tgbm_session* s, int task, double* pred) {
+    return dev_guard([&] {
+        HIP_OK(hipStreamSynchronize(s->stream));
+        HIP_OK(hipMemcpy(pred, s->tasks_h[task].pred, sizeof(double) * s->th[task].n_rows, hipMemcpyDeviceToHost));
+    });
+}
+
+int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sample,
+                         const unsigned char* scaled, double scale) {
+    return dev_guard([&] {
+        Task& T = s->tasks_h[task];
+        TaskHost& H = s->th[task];
+        HIP_OK(hipStreamSynchronize(s->stream));
+        if (!sample) {
+            H.has_sample = false;
+            H.n_sample = H.n_rows;
+            H.goss = false;
+            s->set_field_int(task, offsetof(Task, n_sample), H.n_rows);
+            s->set_field_int(task, offsetof(Task, contiguous), 1);
+            s->set_field_int(task, offsetof(Task, goss), 0);
+            return;
+        }
+        if (n_sample < 0 || n_sample > H.n_rows) throw std::runtime_error("sample size out of range");
+        for (int k = 0; k < n_sample; ++k)
+            if (sample[k] < 0 || sample[k] >= H.n_rows || (k && sample[k] <= sample[k - 1]))
+                throw std::runtime_error("sample rows must be sorted, unique and in range");
+        HIP_OK(hipMemcpy(T.idx, sample, sizeof(int) * n_sample, hipMemcpyHostToDevice));
+        H.has_sample = true;
+        H.n_sample = n_sample;
+        s->set_field_int(task, offsetof(Task, n_sample), n_sample);
+        s->set_field_int(task, offsetof(Task, contiguous), 0);
+        if (scaled) {
+            if (s->hconst) throw std::runtime_error("scaled sample rows need a session without h == 1");
+            unsigned char* sd = const_cast<unsigned char*>(T.scaled);
+            if (!sd) {
+                sd = s->alloc<unsigned char>(H.n_rows);
+                T.scaled = sd;
+                s->set_field_ptr(task, offsetof(Task, scaled), sd);
+            }
+            HIP_OK(hipMemcpy(sd, scaled, H.n_rows, hipMemcpyHostToDevice));
+            H.goss = true;
+            s->set_field_int(task, offsetof(Task, goss), 1);
+            s->set_field_double(task, offsetof(Task, goss_scale), scale);
+        } else {
+            H.goss = false;
+            s->set_field_int(task, offsetof(Task, goss), 0);
+        }
+    });
+}
+
+// Root of every task's tree: quantize, build and scan the root histogram.
+static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy) {
+    const GrowP P = s->growp(p);
+    launch_tree_start(s->stream, s->S, s->tasks_d, s->T, s->max_rows());
+    launch_zero_slots(s->stream, s->tasks_d, s->T, s->F, 0, 1);
+    if (energy)
+        for (int t = 0; t < s->T; ++t) HIP_OK(hipMemsetAsync(s->tasks_h[t].energy, 0, sizeof(long long) * MAX_NODES, s->stream));
+    const int ms = s->max_sample();
+    EventPair ep;
+    if (s->timing) {
+        double bytes = 0;
+        for (auto& t : s->th) {
+            const double m = t.has_sample ? t.n_sample : t.n_rows;
+            bytes += m * s->F + m * (s->hconst ? 8.0 : 16.0) + (t.has_sample ? 4.0 * m : 0.0) + (double)s->F * s->nb * 24.0;
+        }
+        s->tic(0, bytes, ep);
+    }
+    launch_hist(s->stream, s->S, s->tasks_d, s->T, true, s->all_contiguous(), s->hconst, ms, p->hist_mode);
+    if (s->timing) s->toc(ep);
+    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 0);
+    if (energy)
+        for (int t = 0; t < s->T; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
+    HIP_OK(hipGetLastError());
+}
+
+// One split step of every task (device-decided when mode == 0).
+static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
+    const int ms = s->max_sample();
+    launch_select(s->stream, s->tasks_d, s->T, s->F, P, mode);
+    launch_zero_slots(s->stream, s->tasks_d, s->T, s->F, -1, 2);
+    EventPair e1, e2;
+    if (s->timing) s->tic(1, 0.0, e1);
+    launch_partition(s->stream, s->S, s->tasks_d, s->T, ms);
+    if (s->timing) s->toc(e1);
+    if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
+    launch_hist(s->stream, s->S, s->tasks_d, s->T, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
+    if (s->timing) s->toc(e2);
+    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 1);
+    HIP_OK(hipGetLastError());
+}
+
+int tgbm_tree_begin(tgbm_session* s, const tgbm_grow_params* p) {
+    return dev_guard([&] { tree_begin_impl(s, p, true); });
+}
+
+int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
+    return dev_guard([&] {
+        const GrowP P = s->growp(p);
+        for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
+    });
+}
+
+int tgbm_leaf_scan_results(tgbm_session* s, int task, int node, double* gain_f, int* thr_f, int* ml_f,
+                           double* energy, int* n_rows) {
+    return dev_guard([&] {
+        const Task& T = s->tasks_h[task];
+        std::vector<FCand> fc(s->F);
+        HIP_OK(hipStreamSynchronize(s->stream));
+        HIP_OK(hipMemcpy(fc.data(), T.fcand + (size_t)node * s->F, sizeof(FCand) * s->F, hipMemcpyDeviceToHost));
+        for (int f = 0; f < s->F; ++f) {
+            gain_f[f] = fc[f].thr >= 0 ? fc[f].gain : -1e300;
+            thr_f[f] = fc[f].thr;
+            ml_f[f] = fc[f].ml;
+        }
+        long long e = 0;
+        HIP_OK(hipMemcpy(&e, T.energy + node, sizeof e, hipMemcpyDeviceToHost));
+        const Task d = s->read_task(task);
+        DNode nd;
+        HIP_OK(hipMemcpy(&nd, T.nodes + node, sizeof nd, hipMemcpyDeviceToHost));
+        if (energy) *energy = std::ldexp((double)e, -d.s_e);
+        if (n_rows) *n_rows = nd.v[N_NS];
+    });
+}
+
+int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const int* f2, int n_pairs,
+                     const tgbm_grow_params* p, tgbm_cand* out) {
+    return dev_guard([&] {
+        if (n_pairs <= 0) return;
+        if (n_pairs > 32) throw std::runtime_error("too many PAIR candidates");
+        std::vector<int> pf(2 * n_pairs);
+        for (int k = 0; k < n_pairs; ++k) {
+            if (f1[k] < 0 || f1[k] >= s->F || f2[k] < 0 || f2[k] >= s->F)
+                throw std::runtime_error("PAIR feature out of range");
+            pf[2 * k] = f1[k];
+            pf[2 * k + 1] = f2[k];
+        }
+        HIP_OK(hipMemcpyAsync(s->pair_f_d, pf.data(), sizeof(int) * pf.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemsetAsync(s->ph_g, 0, sizeof(long long) * n_pairs * NBIN, s->stream));
+        HIP_OK(hipMemsetAsync(s->ph_c, 0, sizeof(int) * n_pairs * NBIN, s->stream));
+        HIP_OK(hipMemsetAsync(s->ph_h, 0, sizeof(long long) * n_pairs * NBIN, s->stream));
+        DNode nd;
+        HIP_OK(hipStreamSynchronize(s->stream));
+        HIP_OK(hipMemcpy(&nd, s->tasks_h[task].nodes + node, sizeof nd, hipMemcpyDeviceToHost));
+        launch_pair(s->stream, s->S, s->tasks_d, task, node, nd.seg_count, s->pair_f_d, n_pairs, s->ph_g,
+                    s->ph_c, s->ph_h, s->hconst, s->growp(p), s->pair_out_d);
+        HIP_OK(hipGetLastError());
+        std::vector<FCand> r(n_pairs);
+        HIP_OK(hipMemcpyAsync(r.data(), s->pair_out_d, sizeof(FCand) * n_pairs, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < n_pairs; ++k) {
+            out[k].gain = r[k].thr >= 0 ? r[k].gain : -1e300;
+            out[k].feature = f1[k];
+            out[k].feature2 = f2[k];
+            out[k].threshold = r[k].thr;
+            out[k].type = 1;
+            out[k].missing_go_left = r[k].ml;
+            out[k].valid = r[k].thr >= 0;
+        }
+    });
+}
+
+int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tgbm_grow_params* p,
+                     int* left_id) {
+    return dev_guard([&] {
+        if (s->T != 1) throw std::runtime_error("host-driven growth needs a single-task session");
+        struct {
+            int leaf, type, feat, feat2, thr, mgl;
+            double gain;
+        } cur{d->leaf, d->type, d->feature, d->feature2, d->threshold, d->missing_go_left, d->gain};
+        static_assert(offsetof(Task, cur_gain) - offsetof(Task, cur_leaf) == 24, "layout");
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(s->tasks_d + task) + offsetof(Task, cur_leaf), &cur,
+                              sizeof cur, hipMemcpyHostToDevice, s->stream));
+        const GrowP P = s->growp(p);
+        split_step(s, P, 1, p);
+        Task T = s->read_task(task);
+        if (T.done) {
+            *left_id = -1;
+            return;
+        }
+        const int L = T.n_nodes - 2;
+        DNode nd[2];
+        HIP_OK(hipMemcpy(nd, s->tasks_h[task].nodes + L, sizeof nd, hipMemcpyDeviceToHost));
+        launch_energy(s->stream, s->tasks_d, task, L, nd[0].seg_count);
+        launch_energy(s->stream, s->tasks_d, task, L + 1, nd[1].seg_count);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(s->stream));
+        *left_id = L;
+    });
+}
+
+int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int iid,
+                  const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
+                  double* values, int* cover, double* prefix) {
+    return dev_guard([&] {
+        const GrowP P = s->growp(p);
+        bool walk = false;
+        for (auto& t : s->th) walk |= t.has_sample;
+        EventPair ep;
+        if (s->timing) s->tic(3, 0.0, ep);
+        launch_tree_end(s->stream, s->S, s->tasks_d, s->T, walk, s->max_rows(),
+                        (const TorbChunk* const*)s->chunk_ptrs_d, s->n_chunks_d, s->max_chunks, P,
+                        min_cover, iid, learning_rate);
+        if (s->timing) s->toc(ep);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(s->stream));
+        std::vector<Task> tv(s->T);
+        HIP_OK(hipMemcpy(tv.data(), s->tasks_d, sizeof(Task) * s->T, hipMemcpyDeviceToHost));
+        const int MN = s->max_nodes;
+        std::vector<DNode> nodes(MAX_NODES);
+        for (int t = 0; t < s->T; ++t) {
+            const int nn = tv[t].n_nodes;
+            n_nodes[t] = nn;
+            const int B = s->th[t].n_buckets;
+            HIP_OK(hipMemcpy(nodes.data(), s->tasks_h[t].nodes, sizeof(DNode) * nn, hipMemcpyDeviceToHost));
+            for (int k = 0; k < nn; ++k) {
+                std::memcpy(node_ints + ((size_t)t * MN + k) * TGBM_NODE_INTS, nodes[k].v, sizeof(int) * 10);
+                node_dbl[((size_t)t * MN + k) * 2] = nodes[k].value;
+                node_dbl[((size_t)t * MN + k) * 2 + 1] = nodes[k].gain;
+            }
+            HIP_OK(hipMemcpy(values + (size_t)t * MN, s->tasks_h[t].values, sizeof(double) * nn, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(cover + (size_t)t * MN, s->tasks_h[t].cover, sizeof(int) * nn, hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(prefix + (size_t)t * MN * MAX_BUCKETS, s->tasks_h[t].prefix, sizeof(double) * nn * B,
+                             hipMemcpyDeviceToHost));
+            if (s->timing) {  // bytes of the child histogram builds of this tree
+                for (int k = 0; k < nn; ++k) {
+                    if (nodes[k].v[N_LEAF]) continue;
+                    const int L = nodes[k].v[N_LEFT], R = nodes[k].v[N_RIGHT];
+                    const double m = std::min(nodes[L].seg_count, nodes[R].seg_count);
+                    s->t_bytes[5] += m * s->F + m * (s->hconst ? 8.0 : 16.0) + 4.0 * m + (double)s->F * s->nb * 24.0;
+                }
+            }
+        }
+    });
+}
+
+int tgbm_timer_enable(tgbm_session* s, int on) {
+    return dev_guard([&] {
+        s->drain_events();
+        s->timing = on != 0;
+        for (int k = 0; k < 8; ++k) {
+            s->t_ms[k] = 0;
+            s->t_n[k] = 0;
+            s->t_bytes[k] = 0;
+        }
+    });
+}
+
+int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches, double* bytes) {
+    return dev_guard([&] {
+        if (which < 0 || which >= 8) throw std::runtime_error("timer index");
+        s->drain_events();
+        *ms = s->t_ms[which];
+        *launches = s->t_n[which];
+        *bytes = s->t_bytes[which];
+    });
+}
+
+}  // extern "C"

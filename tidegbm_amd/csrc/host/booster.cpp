@@ -1,0 +1,609 @@
+// Host orchestration of a fit.  Integer preparation (row order, TORB buckets and cuts, bin edges,
+// GOSS selection) is done here with the same library calls and call order as the reference so
+// that the rows, bins and samples handed to the device are identical; every per-row and
+// per-histogram computation runs on the GPU.
+#include "booster.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace tide {
+
+namespace {
+
+void dev_check(int rc) {
+    if (rc != 0) throw Error(tgbm_last_error());
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct SessionGuard {
+    tgbm_session* s = nullptr;
+    ~SessionGuard() {
+        if (s) tgbm_session_destroy(s);
+    }
+};
+
+// Quantile right-edges of one feature (reference quantile_edges, binning.cpp:14-32).
+std::vector<double> quantile_edges(std::vector<double>& v, int nb) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if (v.empty()) return {0.0};
+    if ((int)v.size() <= nb) return v;
+    std::vector<double> e;
+    e.reserve(nb - 1);
+    const size_t last = v.size() - 1;
+    for (int i = 1; i < nb; ++i) {
+        const double q = (double)i / nb;
+        const double pos = q * last;
+        const size_t lo = (size_t)pos;
+        const size_t hi = std::min(lo + 1, last);
+        const double fr = pos - lo;
+        e.push_back(v[lo] * (1 - fr) + v[hi] * fr);
+    }
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    return e;
+}
+
+// Per-feature edges over non-NaN values with the reference's sub-sampling
+// (Binner::fit numeric branch, binning.cpp:84-99); features run in parallel.
+void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
+               std::vector<std::vector<double>>& edges) {
+    edges.assign(F, {});
+#pragma omp parallel for schedule(dynamic) if (F > 1)
+    for (int f = 0; f < F; ++f) {
+        std::vector<double> v;
+        v.reserve(n);
+        for (int i = 0; i < n; ++i) {
+            const double x = X[(size_t)i * F + f];
+            if (x == x) v.push_back(x);
+        }
+        if (bin_sample > 0 && (int)v.size() > bin_sample) {
+            std::mt19937 rng(seed + (uint32_t)f * 2654435761u);
+            std::shuffle(v.begin(), v.end(), rng);
+            v.resize((size_t)bin_sample);
+        }
+        edges[f] = quantile_edges(v, max_bin - 1);
+    }
+}
+
+double leaf_value_host(double G, double H, const Config& c) {  // tree.hpp:53-67
+    double v;
+    if (c.l1 > 0) {
+        if (G > c.l1) v = -(G - c.l1) / (H + c.l2);
+        else if (G < -c.l1) v = -(G + c.l1) / (H + c.l2);
+        else v = 0.0;
+    } else {
+        v = -G / (H + c.l2);
+    }
+    if (c.max_delta_step > 0) v = std::min(std::max(v, -c.max_delta_step), c.max_delta_step);
+    return v;
+}
+
+tgbm_grow_params grow_params(const Config& c) {
+    tgbm_grow_params p;
+    std::memset(&p, 0, sizeof p);
+    p.max_depth = c.max_depth;
+    p.max_leaves = c.max_leaves;
+    p.min_samples_leaf = c.min_samples_leaf;
+    p.min_child_weight = c.min_child_weight;
+    p.l2 = c.l2;
+    p.l1 = c.l1;
+    p.max_delta_step = c.max_delta_step;
+    p.min_split_gain = c.min_split_gain;
+    p.pair_l2 = c.pair_l2;
+    p.hist_mode = c.hist_mode;
+    return p;
+}
+
+// Host mirror of one node during host-driven (PAIR-capable) growth.
+struct LeafInfo {
+    std::vector<double> gain;
+    std::vector<int> thr, ml;
+    double energy = 0.0;
+    int rows = 0, depth = 0;
+    bool leaf = true;
+    bool cached = false;
+    tgbm_cand cand{};
+};
+
+void fetch_leaf(tgbm_session* s, int node, int F, LeafInfo& li) {
+    li.gain.resize(F);
+    li.thr.resize(F);
+    li.ml.resize(F);
+    dev_check(tgbm_leaf_scan_results(s, 0, node, li.gain.data(), li.thr.data(), li.ml.data(), &li.energy, &li.rows));
+}
+
+// Candidate of one leaf (reference best_for_leaf, tree.cpp:106-281, with no categorical,
+// interaction-group, regime or HALO branches): axis argmax over ascending features, then the
+// PAIR search on weak leaves over the top-k features ordered by std::sort.
+tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& c, int F,
+                         int pair_budget, const tgbm_grow_params& gp) {
+    tgbm_cand best{};
+    best.gain = -1e300;
+    best.feature = -1;
+    best.feature2 = -1;
+    best.threshold = -1;
+    best.type = 0;
+    best.missing_go_left = 1;
+    for (int f = 0; f < F; ++f) {
+        if (li.thr[f] < 0 || li.gain[f] < c.min_split_gain) continue;
+        if (li.gain[f] > best.gain) {
+            best.gain = li.gain[f];
+            best.feature = f;
+            best.threshold = li.thr[f];
+            best.missing_go_left = li.ml[f];
+        }
+    }
+    const bool weak = (best.gain < c.pair_trigger * std::max(li.energy * 0.01, 1e-12)) || best.gain <= 0;
+    if (c.pair_splits && pair_budget > 0 && F >= 2 && weak) {
+        std::vector<int> top(F);
+        std::iota(top.begin(), top.end(), 0);
+        std::sort(top.begin(), top.end(), [&](int a, int b) { return li.gain[a] > li.gain[b]; });
+        const int k = std::min(c.pair_top_k, F);
+        std::vector<int> f1, f2;
+        for (int a = 0; a < k; ++a)
+            for (int b = a + 1; b < k; ++b) {
+                f1.push_back(top[a]);
+                f2.push_back(top[b]);
+            }
+        std::vector<tgbm_cand> pc(f1.size());
+        for (size_t off = 0; off < f1.size(); off += 32) {
+            const int m = (int)std::min<size_t>(32, f1.size() - off);
+            dev_check(tgbm_pair_search(s, 0, node, f1.data() + off, f2.data() + off, m, &gp, pc.data() + off));
+        }
+        for (auto& p : pc) {
+            if (!p.valid) continue;
+            if (p.gain > best.gain) best = p;
+        }
+    }
+    best.valid = best.feature >= 0;
+    return best;
+}
+
+// Host-driven best-first loop (reference grow_tree, tree.cpp:343-463).
+void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_params& gp) {
+    std::vector<LeafInfo> nodes(1);
+    fetch_leaf(s, 0, F, nodes[0]);
+    nodes[0].depth = 0;
+    int pair_used = 0;
+    for (int it = 0; it < c.max_leaves - 1; ++it) {
+        int best_leaf = -1;
+        tgbm_cand best{};
+        double best_score = c.min_split_gain;
+        for (int lid = 0; lid < (int)nodes.size(); ++lid) {
+            LeafInfo& li = nodes[lid];
+            if (!li.leaf || li.depth >= c.max_depth) continue;
+            if (li.rows < 2 * c.min_samples_leaf) continue;
+            if (!li.cached) {
+                li.cand = leaf_candidate(s, lid, li, c, F, c.max_pair_splits_per_tree - pair_used, gp);
+                li.cached = true;
+            }
+            const tgbm_cand& r = li.cand;
+            if (r.feature >= 0 && r.gain >= c.min_split_gain && r.gain > best_score) {
+                best_score = r.gain;
+                best_leaf = lid;
+                best = r;
+            }
+        }
+        if (best_leaf < 0) break;
+        if (best.type == 1) {
+            ++pair_used;
+            for (auto& li : nodes) li.cached = false;
+        }
+        nodes[best_leaf].cached = false;
+        nodes[best_leaf].leaf = false;
+        tgbm_decision d{best_leaf, best.type, best.feature, best.type == 1 ? best.feature2 : -1,
+                        best.threshold, best.missing_go_left, best.gain};
+        int left = -1;
+        dev_check(tgbm_apply_split(s, 0, &d, &gp, &left));
+        if (left < 0) break;
+        nodes.resize(left + 2);
+        for (int side = 0; side < 2; ++side) {
+            LeafInfo& ch = nodes[left + side];
+            fetch_leaf(s, left + side, F, ch);
+            ch.depth = nodes[best_leaf].depth + 1;
+            ch.leaf = true;
+            ch.cached = false;
+        }
+    }
+}
+
+// GOSS sample (reference booster.cpp:554-585): std::sort by |g| descending, keep the top
+// ceil(top*S), shuffle the rest with the fit's mt19937 and keep ceil(other*S) of them scaled.
+void goss_sample(const std::vector<double>& g, int n, const Config& c, std::mt19937& rng,
+                 std::vector<int>& sample, std::vector<unsigned char>& scaled, double& scale) {
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return std::abs(g[a]) > std::abs(g[b]); });
+    const int S = n;
+    const int n_top = std::min(S, std::max(1, (int)std::ceil(c.goss_top * S)));
+    std::vector<char> keep(n, 0);
+    for (int i = 0; i < n_top; ++i) keep[order[i]] = 1;
+    std::vector<int> rest(order.begin() + n_top, order.end());
+    int n_other = (int)std::ceil(c.goss_other * S);
+    n_other = std::min(n_other, (int)rest.size());
+    std::shuffle(rest.begin(), rest.end(), rng);
+    scale = (1.0 - c.goss_top) / std::max(c.goss_other, 1e-12);
+    scaled.assign(n, 0);
+    for (int i = 0; i < n_other; ++i) {
+        keep[rest[i]] = 1;
+        scaled[rest[i]] = 1;
+    }
+    sample.clear();
+    for (int i = 0; i < n; ++i)
+        if (keep[i]) sample.push_back(i);
+}
+
+}  // namespace
+
+void Config::validate() const {  // reference config.hpp:78-97
+    if (n_estimators < 1) throw Error("n_estimators must be >= 1");
+    if (learning_rate <= 0 || learning_rate > 1.5) throw Error("learning_rate out of range");
+    if (max_bin < 8 || max_bin > 256) throw Error("max_bin must be in [8, 256]");
+    if (!iid_mode && n_time_buckets == 1) throw Error("n_time_buckets must be 0 (auto) or >= 2");
+    if (!iid_mode && n_time_buckets < 0) throw Error("n_time_buckets must be 0 (auto) or >= 2");
+    if (!iid_mode && horizon < 0)
+        throw Error("horizon is required in time mode (label length in timestamp units). "
+                    "Overlapping returns without horizon leak into TORB.");
+    if (device != "cpu" && device != "gpu") throw Error("device must be cpu or gpu");
+    if (device == "cpu")
+        throw Error("device=cpu is not provided by this library: it implements the GPU path only");
+    // limits of the device path (each fails loudly rather than silently changing results)
+    if (max_bin > 255) throw Error("the GPU path stores uint8 bins: max_bin must be <= 255");
+    if (max_leaves < 2 || max_leaves > 128) throw Error("the GPU path supports max_leaves in [2, 128]");
+    if (objective != 0 && objective != 5) throw Error("the GPU path supports objective mse or pairwise");
+    if (halo) throw Error("halo is not supported on the GPU path");
+    if (subsample < 1.0) throw Error("subsample < 1 is not supported on the GPU path");
+    if (colsample < 1.0) throw Error("colsample < 1 is not supported on the GPU path");
+    if (time_decay < 1.0) throw Error("time_decay < 1 is not supported on the GPU path");
+    if (tree_decay < 1.0) throw Error("tree_decay < 1 is not supported on the GPU path");
+    if (min_samples_leaf < 0) throw Error("min_samples_leaf must be >= 0");
+}
+
+Booster::~Booster() { drop_forest(); }
+
+void Booster::drop_forest() {
+    if (forest_) tgbm_forest_destroy(forest_);
+    forest_ = nullptr;
+    forest_trees_ = -1;
+}
+
+void Booster::ensure_forest() {
+    const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : cfg.n_estimators);
+    if (forest_ && forest_trees_ == use) return;
+    drop_forest();
+    std::vector<int> tn(use);
+    std::vector<int> ints;
+    std::vector<double> vals, lrs(use, cfg.learning_rate);
+    for (int t = 0; t < use; ++t) {
+        const Tree& tr = trees[t];
+        tn[t] = (int)tr.nodes.size();
+        for (size_t k = 0; k < tr.nodes.size(); ++k) {
+            const Node& d = tr.nodes[k];
+            const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.threshold, d.left, d.right,
+                               d.missing_go_left, d.n_samples, d.depth};
+            ints.insert(ints.end(), v, v + 10);
+            vals.push_back(tr.values.empty() ? d.value : tr.values[k]);
+        }
+    }
+    std::vector<double> E((size_t)n_features * 256, 0.0);
+    std::vector<int> ne(n_features, 0);
+    for (int f = 0; f < n_features; ++f) {
+        ne[f] = (int)edges[f].size();
+        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+    }
+    dev_check(tgbm_forest_create(&forest_, n_features, n_bins, use, tn.data(), ints.data(), vals.data(),
+                                 lrs.data(), init_score, E.data(), ne.data()));
+    forest_trees_ = use;
+}
+
+void Booster::predict_raw(const double* X, int n, int nf, double* out) {
+    if (!fitted) throw Error("not fitted");
+    if (nf != n_features) throw Error("n_features mismatch");
+    if (n <= 0) return;
+    ensure_forest();
+    dev_check(tgbm_forest_predict(forest_, X, n, out));
+}
+
+void Booster::predict_binned_u16(const unsigned short* xb, int n, double* out) {
+    if (!fitted) throw Error("not fitted");
+    if (n <= 0) return;
+    for (size_t j = 0; j < (size_t)n * n_features; ++j)
+        if (xb[j] > 255) throw Error("bin index above 255 is not representable on the GPU path");
+    ensure_forest();
+    dev_check(tgbm_forest_predict_binned_u16(forest_, xb, n, out));
+}
+
+void Booster::fit(const double* X0, const double* y0, int n, int F, const long long* ts0, const double* w0,
+                  const int* is_cat, const long long* ent0, const long long* rg0) {
+    const double t0 = now_s();
+    cfg.validate();
+    if (n <= 0 || F <= 0) throw Error("empty input");
+    if (is_cat)
+        for (int f = 0; f < F; ++f)
+            if (is_cat[f]) throw Error("categorical features are not supported on the GPU path");
+    if (tgbm_device_count() < 1) throw Error("GPU device not available: no HIP device was found for device=gpu");
+    if (n >= 1000000) cfg.pair_splits = false;  // booster.cpp:237-240
+    drop_forest();
+    fitted = false;
+    n_features = F;
+    // ---- stable sort by (timestamp, entity, index) (booster.cpp:264-309)
+    std::vector<double> Xs, ys, ws(n, 1.0);
+    std::vector<long long> tss, rgs;
+    const double* X = X0;
+    const double* y = y0;
+    const long long* ts = ts0;
+    const long long* rg = rg0;
+    bool any_weight = false;
+    if (w0) {
+        std::copy(w0, w0 + n, ws.begin());
+        for (int i = 0; i < n; ++i) any_weight |= w0[i] != 1.0;
+    }
+    if (!cfg.iid_mode && ts0 && n > 1) {
+        std::vector<int> ord(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            if (ts0[a] != ts0[b]) return ts0[a] < ts0[b];
+            if (ent0) return ent0[a] < ent0[b];
+            return a < b;
+        });
+        bool perm = false;
+        for (int i = 0; i < n && !perm; ++i) perm = ord[i] != i;
+        if (perm) {
+            Xs.resize((size_t)n * F);
+            ys.resize(n);
+            tss.resize(n);
+            if (rg0) rgs.resize(n);
+            std::vector<double> w2(n);
+            for (int i = 0; i < n; ++i) {
+                const int s = ord[i];
+                std::memcpy(&Xs[(size_t)i * F], X0 + (size_t)s * F, sizeof(double) * F);
+                ys[i] = y0[s];
+                w2[i] = ws[s];
+                tss[i] = ts0[s];
+                if (rg0) rgs[i] = rg0[s];
+            }
+            ws.swap(w2);
+            X = Xs.data();
+            y = ys.data();
+            ts = tss.data();
+            if (rg0) rg = rgs.data();
+        }
+    }
+    // ---- TORB buckets and causal cuts (booster.cpp:36-72, 311-364)
+    std::vector<int> bucket(n, 0), cuts;
+    int n_buckets = 1;
+    long long embargo = cfg.embargo;
+    if (!cfg.iid_mode) {
+        if (!ts) throw Error("timestamp is required unless iid_mode");
+        std::vector<long long> u(ts, ts + n);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        const int uniq_n = std::max(2, (int)u.size());
+        int want = cfg.n_time_buckets;
+        if (want <= 0) {
+            long long step = 1;
+            if (u.size() >= 2) {
+                std::vector<long long> d;
+                for (size_t k = 1; k < u.size(); ++k)
+                    if (u[k] - u[k - 1] > 0) d.push_back(u[k] - u[k - 1]);
+                if (!d.empty()) {
+                    std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
+                    step = std::max<long long>(d[d.size() / 2], 1);
+                }
+            }
+            const int h_steps = (int)(std::max<long long>(cfg.horizon, 0) / step);
+            want = std::max(2, std::min(h_steps >= 8 ? 64 : 32, std::max(uniq_n, 2)));
+        }
+        n_buckets = cfg.finer_torb ? std::max(2, std::min(want, uniq_n)) : std::max(2, want);
+        if (n_buckets > 256) throw Error("the GPU path supports at most 256 time buckets");
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts[a] < ts[b]; });
+        for (int r = 0; r < n; ++r) {
+            int b = (int)((double)r / std::max(n, 1) * n_buckets);
+            if (b >= n_buckets) b = n_buckets - 1;
+            bucket[order[r]] = b;
+        }
+        embargo += std::max<long long>(cfg.horizon, 0);
+        bool sorted = true;
+        for (int i = 1; i < n && sorted; ++i) sorted = ts[i] >= ts[i - 1];
+        if (!sorted) throw Error("internal: rows not time-sorted");
+        std::vector<int> first(n_buckets, -1);
+        for (int i = 0; i < n; ++i)
+            if (first[bucket[i]] < 0) first[bucket[i]] = i;
+        cuts.assign(n_buckets, 0);
+        int run = 0;
+        for (int b = 0; b < n_buckets; ++b) {
+            int c = 0;
+            if (first[b] >= 0) c = (int)(std::lower_bound(ts, ts + n, ts[first[b]] - embargo) - ts);
+            run = std::max(run, c);
+            cuts[b] = run;
+        }
+    }
+    n_time_buckets = n_buckets;
+    // ---- bin edges (booster.cpp:378-380) and bins on the device
+    fit_edges(X, n, F, cfg.max_bin, cfg.bin_sample, (uint32_t)cfg.random_state, edges);
+    n_bins = cfg.max_bin;
+    for (auto& e : edges) n_bins = std::max(n_bins, (int)e.size() + 2);
+    // ---- initial scores (booster.cpp:399-457)
+    auto init_of = [&](int m) -> double {  // Objective::init_score (objective.cpp:80-100)
+        if (m <= 0) return 0;
+        if (cfg.objective == 5) return 0;
+        double sw = 0, sy = 0;
+        for (int i = 0; i < m; ++i) {
+            sw += ws[i];
+            sy += ws[i] * y[i];
+        }
+        return sw > 0 ? sy / sw : 0;
+    };
+    std::vector<double> pred(n);
+    init_prefix.assign(n_buckets, 0.0);
+    if (cfg.iid_mode) {
+        init_score = init_of(n);
+        std::fill(pred.begin(), pred.end(), init_score);
+    } else {
+        // running weighted sums snapshot at each cut: identical to refolding [0, cut) per bucket
+        std::vector<double> at(n_buckets, NAN);
+        if (cfg.objective == 5) {
+            for (int b = 0; b < n_buckets; ++b) at[b] = cuts[b] > 0 ? 0.0 : NAN;
+        } else {
+            double sw = 0, sy = 0;
+            int i = 0;
+            for (int b = 0; b < n_buckets; ++b) {
+                for (; i < cuts[b]; ++i) {
+                    sw += ws[i];
+                    sy += ws[i] * y[i];
+                }
+                if (cuts[b] > 0) at[b] = sw > 0 ? sy / sw : 0;
+            }
+        }
+        init_prefix = at;
+        init_score = init_of(n);
+        for (int b = 0; b < n_buckets; ++b)
+            if (!std::isfinite(init_prefix[b])) init_prefix[b] = init_score;
+        for (int i = 0; i < n; ++i) pred[i] = init_prefix[bucket[i]];
+    }
+    // ---- device session
+    const bool use_goss = (cfg.goss > 0) || (cfg.goss < 0 && n >= 250000);
+    const bool goss_active = use_goss && cfg.goss_top < 1.0 && n > 64;
+    const bool hconst = cfg.objective == 0 && !any_weight && !goss_active;
+    SessionGuard sg;
+    dev_check(tgbm_session_create(&sg.s, n, F, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
+    {
+        std::vector<double> E((size_t)F * 256, 0.0);
+        std::vector<int> ne(F);
+        for (int f = 0; f < F; ++f) {
+            ne[f] = (int)edges[f].size();
+            std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+        }
+        dev_check(tgbm_session_bin(sg.s, X, E.data(), ne.data()));
+    }
+    const long long* groups = (cfg.objective == 5) ? rg : nullptr;
+    dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, pred.data(), bucket.data(), nullptr,
+                                 n_buckets, cfg.iid_mode ? nullptr : cuts.data(), groups));
+    trees.clear();
+    if (cfg.profile) dev_check(tgbm_timer_enable(sg.s, 1));
+    t_setup = now_s() - t0;
+    const double t1 = now_s();
+    std::mt19937 rng((uint32_t)cfg.random_state);
+    grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, cuts, rng, goss_active, nullptr, false);
+    if (cfg.profile)
+        for (int k = 0; k < 8; ++k) dev_check(tgbm_timer_read(sg.s, k, &prof_ms[k], &prof_n[k], &prof_bytes[k]));
+    best_iteration = cfg.n_estimators;
+    fitted = true;
+    t_boost = now_s() - t1;
+}
+
+// The boosting loop over the device session (booster.cpp:667-703 with grow_head :519-665).
+void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, const std::vector<int>& cuts,
+                         std::mt19937& rng, bool goss, const double*, bool for_update) {
+    (void)cuts;
+    const tgbm_grow_params gp = grow_params(cfg);
+    const int F = n_features;
+    const bool host_driven = cfg.pair_splits || cfg.force_host_driver;
+    const double min_cover =
+        cfg.torb_min_cover > 0.0 ? cfg.torb_min_cover : (double)(2 * std::max(cfg.min_samples_leaf, 1));
+    const int MN = 2 * cfg.max_leaves - 1;
+    std::vector<int> ints((size_t)MN * TGBM_NODE_INTS);
+    std::vector<double> dbl((size_t)MN * 2), vals(MN), pref((size_t)MN * 256);
+    std::vector<int> cov(MN);
+    std::vector<double> g;
+    std::vector<int> sample;
+    std::vector<unsigned char> scaled;
+    const double t_loop = now_s();
+    if (!for_update) tree_end_times.clear();
+    for (int m = 0; m < n_trees; ++m) {
+        dev_check(tgbm_task_grad(s, -1, cfg.objective));
+        if (goss) {
+            g.resize(n);
+            dev_check(tgbm_task_get_grad(s, 0, g.data(), nullptr));
+            double scale = 1.0;
+            goss_sample(g, n, cfg, rng, sample, scaled, scale);
+            dev_check(tgbm_task_set_sample(s, 0, sample.data(), (int)sample.size(), scaled.data(), scale));
+        }
+        (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample would use it
+        dev_check(tgbm_tree_begin(s, &gp));
+        if (host_driven) grow_host_driven(s, cfg, F, gp);
+        else dev_check(tgbm_tree_grow_device(s, &gp));
+        int nn = 0;
+        dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
+                                ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
+        Tree t;
+        t.nodes.resize(nn);
+        for (int k = 0; k < nn; ++k) {
+            const int* v = &ints[(size_t)k * TGBM_NODE_INTS];
+            Node& d = t.nodes[k];
+            d.is_leaf = v[0];
+            d.split_type = v[1];
+            d.feature = v[2];
+            d.feature2 = v[3];
+            d.threshold = v[4];
+            d.left = v[5];
+            d.right = v[6];
+            d.missing_go_left = v[7];
+            d.n_samples = v[8];
+            d.depth = v[9];
+            d.value = dbl[2 * k];
+            d.gain = dbl[2 * k + 1];
+        }
+        t.values.assign(vals.begin(), vals.begin() + nn);
+        t.cover.assign(cov.begin(), cov.begin() + nn);
+        if (for_update) {
+            // Booster::update keeps grow_tree's cover (node sample counts) and one bucket
+            for (int k = 0; k < nn; ++k) t.cover[k] = t.nodes[k].n_samples;
+            t.n_buckets = 1;
+            t.prefix_values = t.values;
+        } else {
+            t.n_buckets = n_time_buckets;
+            t.prefix_values.assign(pref.begin(), pref.begin() + (size_t)nn * n_time_buckets);
+        }
+        trees.push_back(std::move(t));
+        if (!for_update) tree_end_times.push_back(now_s() - t_loop);  // tree_end synchronized
+    }
+}
+
+void Booster::update(const double* X, const double* y, int n, int nf, const double* w, int extra_trees) {
+    if (!fitted) throw Error("not fitted");
+    if (nf != n_features) throw Error("n_features mismatch");
+    if (n <= 0 || extra_trees <= 0) return;
+    bool any_weight = false;
+    if (w)
+        for (int i = 0; i < n; ++i) any_weight |= w[i] != 1.0;
+    std::vector<double> pred(n);
+    predict_raw(X, n, nf, pred.data());  // booster.cpp:859: current model, frozen binner
+    const bool hconst = cfg.objective == 0 && !any_weight;
+    SessionGuard sg;
+    dev_check(tgbm_session_create(&sg.s, n, nf, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
+    {
+        std::vector<double> E((size_t)nf * 256, 0.0);
+        std::vector<int> ne(nf);
+        for (int f = 0; f < nf; ++f) {
+            ne[f] = (int)edges[f].size();
+            std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+        }
+        dev_check(tgbm_session_bin(sg.s, X, E.data(), ne.data()));
+    }
+    std::vector<int> zeros(n, 0);
+    dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? w : nullptr, pred.data(), zeros.data(), nullptr, 1, nullptr,
+                                 nullptr));
+    std::mt19937 rng((uint32_t)cfg.random_state + (uint32_t)trees.size());
+    const int n_before = (int)trees.size();
+    drop_forest();
+    grow_trees(sg.s, n, extra_trees, false, {}, rng, false, nullptr, true);
+    (void)n_before;
+    best_iteration = (int)trees.size();
+}
+
+}  // namespace tide

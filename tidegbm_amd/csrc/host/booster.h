@@ -1,0 +1,125 @@
+// Host side of the MI355X histogram-GBDT path: the C++ booster that prepares data exactly like
+// the reference Booster::fit (reference cpp/src/booster.cpp:231-742) and drives the gfx950
+// kernels through the thin device ABI (include/tgbm_device.h).
+#pragma once
+
+#include <cstdint>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/tgbm_device.h"
+
+namespace tide {
+
+struct Error : std::runtime_error {
+    explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+// Configuration: names and defaults follow the reference Config (config.hpp:12-113) and the
+// kwargs defaults of bindings.cpp:43-126.  Fields the device path does not implement are kept
+// so that setting them to a non-default value fails loudly at fit time.
+struct Config {
+    int n_estimators = 100;
+    double learning_rate = 0.1;
+    int max_depth = 6;
+    int max_leaves = 31;
+    int min_samples_leaf = 20;
+    double min_child_weight = 1e-3;
+    double l2 = 1.0;
+    double l1 = 0.0;
+    double max_delta_step = 0.0;
+    double min_split_gain = 0.0;
+    int max_bin = 255;
+    double subsample = 1.0;
+    double colsample = 1.0;
+    int n_time_buckets = 0;
+    long long horizon = -1;
+    long long embargo = 0;
+    bool iid_mode = false;
+    bool halo = false;
+    bool pair_splits = true;
+    double pair_trigger = 0.25;
+    int pair_top_k = 6;
+    int max_pair_splits_per_tree = 3;
+    double pair_l2 = 2.0;
+    double time_decay = 1.0;
+    double tree_decay = 1.0;
+    int objective = 0;  // ObjectiveKind: 0 MSE, 5 RankPairwise (types.hpp:26-35)
+    int n_jobs = 0;
+    int random_state = 0;
+    int verbose = 0;
+    int goss = -1;
+    double goss_top = 0.2;
+    double goss_other = 0.1;
+    int bin_sample = 262144;
+    bool finer_torb = true;
+    double torb_min_cover = 0.0;
+    int cat_min_count = 5;
+    bool col_major_hist = true;
+    std::string device = "gpu";
+    int profile = 0;
+    int hist_mode = 0;
+    int force_host_driver = 0;
+
+    void validate() const;  // config.hpp:78-97 plus the device path's limits
+};
+
+struct Node {  // reference Node (tree.hpp:11-25)
+    int is_leaf = 1, split_type = 0, feature = -1, feature2 = -1, threshold = -1, left = -1,
+        right = -1, missing_go_left = 1, n_samples = 0, depth = 0;
+    double value = 0.0, gain = 0.0;
+};
+
+struct Tree {  // reference Tree (tree.hpp:32-41)
+    std::vector<Node> nodes;
+    std::vector<double> values;
+    std::vector<int> cover;
+    std::vector<double> prefix_values;
+    int n_buckets = 1;
+};
+
+class Booster {
+public:
+    Config cfg;
+    int n_features = 0;
+    int n_bins = 255;
+    int n_time_buckets = 1;
+    int best_iteration = 0;
+    double init_score = 0.0;
+    std::vector<double> init_prefix;
+    std::vector<std::vector<double>> edges;
+    std::vector<Tree> trees;
+    bool fitted = false;
+    // last fit diagnostics
+    double t_setup = 0.0, t_boost = 0.0;
+    std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
+    double prof_ms[8] = {0}, prof_bytes[8] = {0};
+    long long prof_n[8] = {0};
+
+    ~Booster();
+    Booster() = default;
+    Booster(const Booster&) = delete;
+    Booster& operator=(const Booster&) = delete;
+
+    void fit(const double* X, const double* y, int n, int nf, const long long* ts, const double* w,
+             const int* is_cat, const long long* entity, const long long* rank_groups);
+    void update(const double* X, const double* y, int n, int nf, const double* w, int extra_trees);
+    void predict_raw(const double* X, int n, int nf, double* out);
+    void predict_binned_u16(const unsigned short* xb, int n, double* out);
+
+    std::string save_binary() const;
+    std::string save_json() const;
+    static Booster* load_binary(const std::string& blob);
+
+private:
+    tgbm_forest* forest_ = nullptr;
+    int forest_trees_ = -1;
+    void ensure_forest();
+    void drop_forest();
+    void grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, const std::vector<int>& cuts,
+                    std::mt19937& rng, bool goss, const double* gsample_unused, bool for_update);
+};
+
+}  // namespace tide

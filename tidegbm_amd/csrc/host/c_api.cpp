@@ -1,0 +1,276 @@
+// The drop-in C API (include/tidegbm_amd.h).  Conventions follow the reference
+// cpp/src/c_api.cpp:12-158: 0 / -1 returns, a thread-local error message, opaque handles,
+// "unknown int key" / "unknown double key" on bad setter keys.
+#include <cstdio>
+#include <fstream>
+#include <iterator>
+#include <string>
+
+#include "../../../include/tidegbm_amd.h"
+#include "booster.h"
+
+using tide::Booster;
+
+static thread_local std::string g_err;
+
+template <class Fn>
+static int guard(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+extern "C" {
+
+const char* tide_last_error(void) { return g_err.c_str(); }
+
+int tide_create(TideHandle* out) {
+    return guard([&] { *out = new Booster(); });
+}
+
+void tide_free(TideHandle h) { delete static_cast<Booster*>(h); }
+
+// Keys of the reference setter (c_api.cpp:35-49) plus the extensions listed in the header.
+int tide_set_int(TideHandle h, const char* key, int v) {
+    tide::Config& c = static_cast<Booster*>(h)->cfg;
+    const std::string k(key);
+    if (k == "n_estimators") c.n_estimators = v;
+    else if (k == "max_depth") c.max_depth = v;
+    else if (k == "max_leaves") c.max_leaves = v;
+    else if (k == "max_bin") c.max_bin = v;
+    else if (k == "n_time_buckets") c.n_time_buckets = v;
+    else if (k == "random_state") c.random_state = v;
+    else if (k == "n_jobs") c.n_jobs = v;
+    else if (k == "iid_mode") c.iid_mode = v != 0;
+    else if (k == "halo") c.halo = v != 0;
+    else if (k == "verbose") c.verbose = v;
+    else if (k == "horizon") c.horizon = v;
+    else if (k == "embargo") c.embargo = v;
+    else if (k == "goss") c.goss = v;
+    else if (k == "cat_min_count") c.cat_min_count = v;
+    else if (k == "col_major_hist") c.col_major_hist = v != 0;
+    // extensions
+    else if (k == "min_samples_leaf") c.min_samples_leaf = v;
+    else if (k == "pair_splits") c.pair_splits = v != 0;
+    else if (k == "pair_top_k") c.pair_top_k = v;
+    else if (k == "max_pair_splits_per_tree") c.max_pair_splits_per_tree = v;
+    else if (k == "bin_sample") c.bin_sample = v;
+    else if (k == "finer_torb") c.finer_torb = v != 0;
+    else if (k == "objective") c.objective = v;
+    else if (k == "profile") c.profile = v;
+    else if (k == "hist_mode") c.hist_mode = v;
+    else if (k == "force_host_driver") c.force_host_driver = v;
+    else {
+        g_err = "unknown int key";
+        return -1;
+    }
+    return 0;
+}
+
+int tide_set_double(TideHandle h, const char* key, double v) {
+    tide::Config& c = static_cast<Booster*>(h)->cfg;
+    const std::string k(key);
+    if (k == "learning_rate") c.learning_rate = v;
+    else if (k == "l2") c.l2 = v;
+    else if (k == "l1") c.l1 = v;
+    else if (k == "subsample") c.subsample = v;
+    else if (k == "colsample") c.colsample = v;
+    // extensions
+    else if (k == "min_child_weight") c.min_child_weight = v;
+    else if (k == "min_split_gain") c.min_split_gain = v;
+    else if (k == "pair_trigger") c.pair_trigger = v;
+    else if (k == "pair_l2") c.pair_l2 = v;
+    else if (k == "goss_top") c.goss_top = v;
+    else if (k == "goss_other") c.goss_other = v;
+    else if (k == "max_delta_step") c.max_delta_step = v;
+    else if (k == "torb_min_cover") c.torb_min_cover = v;
+    else if (k == "time_decay") c.time_decay = v;
+    else if (k == "tree_decay") c.tree_decay = v;
+    else {
+        g_err = "unknown double key";
+        return -1;
+    }
+    return 0;
+}
+
+int tide_set_string(TideHandle h, const char* key, const char* value) {
+    tide::Config& c = static_cast<Booster*>(h)->cfg;
+    const std::string k(key), v(value ? value : "");
+    if (k == "device") {
+        c.device = v;
+        return 0;
+    }
+    if (k == "objective") {  // names of bindings.cpp:108-119
+        if (v == "mse" || v == "auto" || v == "l2" || v == "regression") c.objective = 0;
+        else if (v == "pairwise" || v == "lambdarank") c.objective = 5;
+        else if (v == "huber" || v == "quantile" || v == "pinball" || v == "logloss" || v == "binary" ||
+                 v == "classification" || v == "rank" || v == "listnet" || v == "multiclass" || v == "custom") {
+            g_err = "objective " + v + " is not supported on the GPU path";
+            return -1;
+        } else {
+            g_err = "unknown objective: " + v;
+            return -1;
+        }
+        return 0;
+    }
+    g_err = "unknown string key";
+    return -1;
+}
+
+int tide_fit(TideHandle h, const double* X, const double* y, int n, int n_features, const long long* timestamps,
+             const double* weight, const int* is_cat, int iid, int horizon) {
+    return tide_fit_ex(h, X, y, n, n_features, timestamps, weight, is_cat, nullptr, nullptr, iid, horizon);
+}
+
+int tide_fit_ex(TideHandle h, const double* X, const double* y, int n, int n_features, const long long* timestamps,
+                const double* weight, const int* is_cat, const long long* entity, const long long* rank_groups,
+                int iid, long long horizon) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        b->cfg.iid_mode = iid != 0;
+        if (!b->cfg.iid_mode) b->cfg.horizon = horizon;
+        b->fit(X, y, n, n_features, timestamps, weight, is_cat, entity, rank_groups);
+    });
+}
+
+int tide_update(TideHandle h, const double* X, const double* y, int n, int n_features, const double* weight,
+                int extra_trees) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] { b->update(X, y, n, n_features, weight, extra_trees); });
+}
+
+int tide_predict(TideHandle h, const double* X, int n, int n_features, double* out) {
+    // MSE and pairwise finalize as the identity (objective.cpp:331-341)
+    return tide_predict_raw(h, X, n, n_features, out);
+}
+
+int tide_predict_raw(TideHandle h, const double* X, int n, int n_features, double* out) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] { b->predict_raw(X, n, n_features, out); });
+}
+
+int tide_predict_binned(TideHandle h, const unsigned short* xb, int n, double* out) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] { b->predict_binned_u16(xb, n, out); });
+}
+
+int tide_save_json(TideHandle h, const char* path) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        std::ofstream f(path);
+        if (!f) throw tide::Error("cannot open file");
+        f << b->save_json();
+    });
+}
+
+int tide_save(TideHandle h, const char* path) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        const std::string blob = b->save_binary();
+        std::ofstream f(path, std::ios::binary);
+        if (!f) throw tide::Error("cannot open file");
+        f.write(blob.data(), (std::streamsize)blob.size());
+    });
+}
+
+int tide_load(TideHandle* out, const char* path) {
+    return guard([&] {
+        std::ifstream f(path, std::ios::binary);
+        if (!f) throw tide::Error("cannot open file");
+        std::string blob((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        *out = Booster::load_binary(blob);
+    });
+}
+
+int tide_device_compiled(void) { return 1; }
+int tide_device_count(void) { return tgbm_device_count(); }
+
+int tide_num_trees(TideHandle h) { return (int)static_cast<Booster*>(h)->trees.size(); }
+
+int tide_tree_info(TideHandle h, int t, int* n_nodes, int* n_buckets) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        const tide::Tree& tr = b->trees.at(t);
+        *n_nodes = (int)tr.nodes.size();
+        *n_buckets = tr.n_buckets;
+    });
+}
+
+int tide_tree_get(TideHandle h, int t, int* ints, double* dbl, double* values, int* cover, double* prefix) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        const tide::Tree& tr = b->trees.at(t);
+        for (size_t k = 0; k < tr.nodes.size(); ++k) {
+            const tide::Node& d = tr.nodes[k];
+            const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.threshold, d.left, d.right,
+                               d.missing_go_left, d.n_samples, d.depth};
+            for (int j = 0; j < 10; ++j) ints[10 * k + j] = v[j];
+            dbl[2 * k] = d.value;
+            dbl[2 * k + 1] = d.gain;
+            values[k] = tr.values[k];
+            cover[k] = tr.cover[k];
+        }
+        for (size_t i = 0; i < tr.prefix_values.size(); ++i) prefix[i] = tr.prefix_values[i];
+    });
+}
+
+int tide_model_info(TideHandle h, int* n_features, int* n_bins, int* n_time_buckets, int* best_iteration,
+                    double* init_score) {
+    auto* b = static_cast<Booster*>(h);
+    *n_features = b->n_features;
+    *n_bins = b->n_bins;
+    *n_time_buckets = b->n_time_buckets;
+    *best_iteration = b->best_iteration;
+    *init_score = b->init_score;
+    return 0;
+}
+
+int tide_get_edges(TideHandle h, int f, double* out, int cap) {
+    auto* b = static_cast<Booster*>(h);
+    if (f < 0 || f >= (int)b->edges.size()) {
+        g_err = "feature out of range";
+        return -1;
+    }
+    const auto& e = b->edges[f];
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
+
+int tide_get_init_prefix(TideHandle h, double* out, int cap) {
+    auto* b = static_cast<Booster*>(h);
+    const auto& v = b->init_prefix;
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
+
+int tide_profile(TideHandle h, int family, double* ms, long long* launches, double* bytes) {
+    auto* b = static_cast<Booster*>(h);
+    if (family < 0 || family >= 8) {
+        g_err = "family out of range";
+        return -1;
+    }
+    *ms = b->prof_ms[family];
+    *launches = b->prof_n[family];
+    *bytes = b->prof_bytes[family];
+    return 0;
+}
+
+int tide_tree_times(TideHandle h, double* out, int cap) {
+    auto* b = static_cast<Booster*>(h);
+    const int m = (int)b->tree_end_times.size();
+    for (int i = 0; i < m && i < cap; ++i) out[i] = b->tree_end_times[i];
+    return m;
+}
+
+int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
+    auto* b = static_cast<Booster*>(h);
+    *setup_s = b->t_setup;
+    *boost_s = b->t_boost;
+    return 0;
+}
+
+}  // extern "C"
