@@ -1,0 +1,81 @@
+"""Write the golden fixtures of tests/golden/ from the reference core (oracle/_ref).
+
+Inputs are regenerated from the seeded panels in tidegbm_amd.synth, so only outputs (and an input
+checksum that detects generator drift) are stored.  Run from the repository root after
+`python tidegbm_amd/build.py` on a machine where the reference tree was present:
+
+    python tests/golden/make_golden.py
+"""
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import checkers  # noqa: E402
+from tidegbm_amd import synth  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+
+
+def cases():
+    """(name, panel, rank_groups, iid, horizon, params) of every fixture."""
+    small = synth.panel_small(0, n_dates=120)
+    rng = np.random.default_rng(0)
+    xor = synth.Panel(small.X, np.sign(small.X[:, 0] * small.X[:, 1]) + 0.1 * rng.normal(size=small.n),
+                      small.ts, small.entity)
+    rank = synth.panel_small(3, n_dates=60, n_entities=70)
+    nan = synth.panel_large(5, n_dates=60, n_entities=50, n_features=10, nan_frac=0.05)
+    return [
+        ("mse_time", small, None, False, 1, dict(n_estimators=8, max_depth=6, learning_rate=0.08, goss=0)),
+        ("mse_pair", xor, None, False, 1, dict(n_estimators=8, max_depth=6, learning_rate=0.08, goss=0)),
+        ("mse_iid", small, None, True, 1, dict(n_estimators=6, max_depth=5, learning_rate=0.1, goss=0)),
+        ("pairwise", rank, rank.ts, False, 1, dict(n_estimators=5, max_depth=6, learning_rate=0.08, goss=0,
+                                                    objective=5)),
+        ("goss", small, None, False, 1, dict(n_estimators=5, max_depth=5, learning_rate=0.1, goss=1)),
+        ("nan_h10", nan, None, False, 10, dict(n_estimators=5, max_depth=6, learning_rate=0.1, goss=0,
+                                               bin_sample=500, pair_splits=0)),
+    ]
+
+
+def fit(lib, name, p, rg, iid, horizon, params):
+    m = lib.model(**params)
+    m.fit(p.X, p.y, ts=None if iid else p.ts, entity=None if iid else p.entity, rank_groups=rg, iid=iid,
+          horizon=horizon)
+    return m
+
+
+def pack(m, p):
+    trees = m.trees()
+    nn = np.array([t["ints"].shape[0] for t in trees], np.int32)
+    nb = np.array([t["n_buckets"] for t in trees], np.int32)
+    edges = [m.edges(f) for f in range(p.n_features)]
+    return dict(
+        input_checksum=np.array([np.nansum(p.X), np.sum(p.y)]),
+        n_nodes=nn, n_buckets=nb,
+        ints=np.concatenate([t["ints"] for t in trees]),
+        dbl=np.concatenate([t["dbl"] for t in trees]),
+        values=np.concatenate([t["values"] for t in trees]),
+        cover=np.concatenate([t["cover"] for t in trees]),
+        prefix=np.concatenate([t["prefix"] for t in trees]),
+        n_edges=np.array([e.size for e in edges], np.int32),
+        edges=np.concatenate(edges),
+        init_prefix=m.init_prefix(),
+        pred=m.predict(p.X[:400]),
+    )
+
+
+def main():
+    ref = checkers.reference()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libtide_ref.so is not built")
+    for name, p, rg, iid, h, params in cases():
+        np.savez_compressed(OUT / f"{name}.npz", **pack(fit(ref, name, p, rg, iid, h, params), p))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -124,7 +124,10 @@ class Booster:
             if k in ("device",):
                 rc = _lib.tide_set_string(self._h, kb, str(v).encode())
             elif k == "objective":
-                rc = _lib.tide_set_string(self._h, kb, str(v).encode())
+                if isinstance(v, (int, np.integer)):
+                    rc = _lib.tide_set_int(self._h, kb, int(v))
+                else:
+                    rc = _lib.tide_set_string(self._h, kb, str(v).encode())
             elif k in _INT_KEYS:
                 rc = _lib.tide_set_int(self._h, kb, int(v))
             elif k in _DOUBLE_KEYS:
