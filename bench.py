@@ -73,7 +73,7 @@ def run_plain(args):
     p = synth.panel_large(1)
     W, K = args.warmup, args.steps
     params = dict(max_depth=8, learning_rate=0.05, max_bin=255, goss=0)
-    b = t.Booster(n_estimators=W + K, profile=1, **params)
+    b = t.Booster(n_estimators=W + K, **params)
     log(f"[bench] fit {W}+{K} trees")
     t0 = time.perf_counter()
     b.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, horizon=1)
@@ -83,8 +83,17 @@ def run_plain(args):
     timed = float(tt[W + K - 1] - t_start)
     trees_per_s = K / timed
     setup_s, boost_s = b.fit_times()
-    ms0, n0, by0 = b.profile(0)   # root histogram builds
-    ms5, n5, by5 = b.profile(5)   # child histogram builds
+    # Kernel durations: a second fit of the same panel with HIP events around every histogram
+    # launch on the session stream (eager launches; the timed fit above replays graphs).
+    n_prof = max(4, min(W + K, args.profile_trees))
+    bp = t.Booster(n_estimators=n_prof, profile=1, **params)
+    log(f"[bench] profiled fit {n_prof} trees")
+    bp.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, horizon=1)
+    for k_a, k_b in zip(b.trees()[:n_prof], bp.trees()):
+        if not (k_a["ints"] == k_b["ints"]).all():
+            raise SystemExit("profiled fit diverged from the timed fit")
+    ms0, n0, by0 = bp.profile(0)   # root histogram builds
+    ms5, n5, by5 = bp.profile(5)   # child histogram builds
     root_ms = ms0 / max(n0, 1)
     root_bytes = by0 / max(n0, 1)
     achieved = root_bytes / (root_ms * 1e-3) / 1e9
@@ -120,6 +129,7 @@ def run_plain(args):
             "avg_launch_ms": round(root_ms, 5),
         },
         "hist_tree_gbs": round(tree_gbs, 1),
+        "hist_root_launches": n0, "hist_child_launches": n5,
         "hist_child_gbs": None if child_gbs is None else round(child_gbs, 1),
         "fit_setup_s": round(setup_s, 3),
         "fit_wall_s": round(wall, 3),
@@ -138,6 +148,7 @@ def main():
     ap.add_argument("--steps", type=int, default=190)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-trees", type=int, default=6)
+    ap.add_argument("--profile-trees", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

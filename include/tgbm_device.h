@@ -34,7 +34,7 @@ typedef struct tgbm_grow_params {
     double l2, l1, max_delta_step, min_split_gain;
     double pair_l2;         /* Config::pair_l2, used by PAIR scans */
     int hist_mode;          /* 0 = auto, else forces a histogram kernel variant (autotuner) */
-    int pad;
+    int host_driven;        /* 1: the host decides splits (tgbm_apply_split); 0: the device does */
 } tgbm_grow_params;
 
 /* One split decision made by the host (PAIR-capable driver). */

@@ -10,6 +10,7 @@ constexpr int NBIN = 256;          // histogram stride per feature (uint8 bins: 
 constexpr int MAX_NODES = 256;     // per tree (max_leaves <= 128)
 constexpr int MAX_FEATURES = 1024;
 constexpr int MAX_BUCKETS = 256;   // TORB buckets (+1 sweep segment for rows past the last cut)
+constexpr int MAX_RED_BLOCKS = 2048;
 
 // ---------------------------------------------------------------- fixed point
 // Histogram sums are exact integer sums of per-row quantized values, so they are independent
@@ -49,7 +50,8 @@ __host__ __device__ inline int mag_exponent(unsigned long long max_bits) {
 struct DNode {
     int v[10];
     int seg_start, seg_count;   // rows of this node in the task's partitioned index array
-    int pad0, pad1;
+    int buf;                    // which of the two index buffers holds the node's rows
+    int pad1;
     double value, gain;
 };
 enum { N_LEAF = 0, N_TYPE, N_FEAT, N_FEAT2, N_THR, N_LEFT, N_RIGHT, N_MGL, N_NS, N_DEPTH };
@@ -88,8 +90,19 @@ struct Task {
     double cur_gain;
     int n_left;
     int pad_a;
+    // ---- step descriptor: the split leaf's rows (set with the decision) and the children
+    // (set by the partition), so kernels read one cache line instead of chasing nodes[]
+    int cur_start, cur_count, cur_buf, cur_left;
+    int small_node, big_node, small_start, small_count, small_buf, pad_c;
+    int part_lcnt, part_rcnt;  // rows placed left / right by the running partition
     // ---- reductions
     unsigned long long max_g_bits, max_h_bits;
+    unsigned long long* red;   // per-block maxima of |g| and |h| [2][MAX_RED_BLOCKS]
+    int n_red;                 // blocks that wrote red[] in the last gradient launch
+    int scan_ticket;           // completion counter of the scan launch (last block selects)
+    int n_leaves_tab;          // entries of leaf_tab
+    int pad_t;
+    int* leaf_tab;             // [MAX_NODES][4] {seg_start, seg_count, node, buf} sorted by start
     // ---- per-row arrays (length n_rows)
     const double* y;
     const double* w;       // may be null (all 1)
@@ -102,8 +115,8 @@ struct Task {
     const unsigned char* seg; // TORB sweep segment per row
     const unsigned char* scaled; // GOSS flag per row (may be null)
     int* nid;
-    int* idx;              // partitioned sample rows
-    int* idx_tmp;
+    int* idx;              // partitioned sample rows (buffer 0)
+    int* idx_tmp;          // buffer 1: a split moves the parent's rows into the other buffer
     const int* group_start; // pairwise groups: positions [gs[k], gs[k+1]) of group_rows
     const int* group_rows;  // rows in stable group order (null: identity, groups contiguous)
     int n_groups;
@@ -204,6 +217,10 @@ __device__ inline int wave_incl_i(int v) {
 // Candidate ordering: larger gain first, then smaller key (feature/bin/missing order).
 __device__ inline bool cand_better(double ga, int ka, double gb, int kb) {
     return ga > gb || (ga == gb && ka < kb);
+}
+
+__device__ inline const int* node_rows(const Task& T, const DNode& n) {
+    return n.buf ? T.idx_tmp : T.idx;
 }
 
 }  // namespace tgbm

@@ -64,7 +64,6 @@ struct tgbm_forest {
         cap = std::max(n, 1024);
         FHIP(hipMalloc(&Xd, sizeof(double) * (size_t)cap * F));
         FHIP(hipMalloc(&bins, (size_t)cap * fp + 16));
-        FHIP(hipMalloc(&bins_col_dummy, (size_t)F * (cap + 128)));
         FHIP(hipMalloc(&outd, sizeof(double) * cap));
     }
 };
@@ -209,7 +208,7 @@ int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out) {
         f->reserve(n);
         FHIP(hipMemcpyAsync(f->Xd, X, sizeof(double) * (size_t)n * f->F, hipMemcpyHostToDevice, f->stream));
         FHIP(hipMemsetAsync(f->bins, 0, (size_t)n * f->fp, f->stream));
-        launch_bin(f->stream, f->Xd, n, f->F, f->edges, f->n_edges, f->bins, f->fp, f->bins_col_dummy, n + 64);
+        launch_bin(f->stream, f->Xd, n, f->F, f->edges, f->n_edges, f->bins, f->fp, nullptr, n + 64);
         FHIP(hipGetLastError());
         forest_run(f, n, out);
     });

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ X, int n
             b = 1 + lo;
         }
         bins_row[(size_t)i * fp + f] = (unsigned char)b;
-        bins_col[(size_t)f * n_pad + i] = (unsigned char)b;
+        if (bins_col) bins_col[(size_t)f * n_pad + i] = (unsigned char)b;
     }
 }
 
@@ -62,15 +62,37 @@ void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edg
                        bins_col, n_pad, rpb);
 }
 
-// =================================================================== gradients
-__device__ inline void block_max_u64(unsigned long long v, unsigned long long* dst) {
+// =================================================================== reductions
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned long long t = __shfl_xor(v, o, 64);
         v = t > v ? t : v;
     }
-    if ((threadIdx.x & 63) == 0 && v) atomicMax(dst, v);
+    return v;
 }
 
+// Block maximum of two values, written to the task's per-block slots (no global atomics).
+__device__ inline void block_max2_store(unsigned long long a, unsigned long long b, Task& T) {
+    __shared__ unsigned long long sa[16], sb[16];
+    a = wave_max_u64(a);
+    b = wave_max_u64(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            a = sa[k] > a ? sa[k] : a;
+            b = sb[k] > b ? sb[k] : b;
+        }
+        T.red[blockIdx.x] = a;
+        T.red[MAX_RED_BLOCKS + blockIdx.x] = b;
+    }
+}
+
+// =================================================================== gradients
 // MSE (objective.cpp:105-119, cost terms off): g = w (pred - y), h = w.
 __global__ __launch_bounds__(256) void k_grad_mse(Task* tasks) {
     Task& T = tasks[blockIdx.y];
@@ -85,15 +107,54 @@ __global__ __launch_bounds__(256) void k_grad_mse(Task* tasks) {
         mg = bg > mg ? bg : mg;
         mh = bh > mh ? bh : mh;
     }
-    block_max_u64(mg, &T.max_g_bits);
-    block_max_u64(mh, &T.max_h_bits);
+    block_max2_store(mg, mh, T);
+    if (blockIdx.x == 0 && threadIdx.x == 0) T.n_red = gridDim.x;
 }
 
 // Within-group pairwise gradient (objective.cpp:179-219).  One block per group; each row
 // replays, in the reference's order, the pairs that touch it: first as the later element b of
 // pairs (a, b) for ascending a, then as the earlier element a for ascending b.  Row values of
-// the group are staged in LDS.
+// the group are staged in LDS (groups larger than the LDS stage read global memory).
 constexpr int PAIR_MAX_GROUP = 4096;
+
+template <bool LDS>
+__device__ inline void pairwise_rows(Task& T, int g0, int m, const int* gr, const double* sy,
+                                     const double* sp, unsigned long long& mg, unsigned long long& mh) {
+    const int step = m > 48 ? max(1, m / 24) : 1;
+    auto Y = [&](int q) { return LDS ? sy[q] : T.y[gr ? gr[q] : g0 + q]; };
+    auto Pd = [&](int q) { return LDS ? sp[q] : T.pred[gr ? gr[q] : g0 + q]; };
+    for (int p = threadIdx.x; p < m; p += blockDim.x) {
+        const int rp = gr ? gr[p] : g0 + p;
+        const double wp = T.w ? T.w[rp] : 1.0;
+        double g = 0.0, h = 1e-6;
+        if (m >= 2) {
+            auto pair = [&](int a, int b) {  // a < b, positions within the group
+                const double ya = Y(a), yb = Y(b);
+                if (ya == yb) return;
+                const bool b_hi = yb > ya;
+                const int hi = b_hi ? b : a, lo = b_hi ? a : b;
+                const double s = Pd(hi) - Pd(lo);
+                const double sc = s < -30.0 ? -30.0 : (s > 30.0 ? 30.0 : s);
+                const double e = exp(sc);
+                const double dyw = fmax(fabs(Y(hi) - Y(lo)), 1e-6);
+                const double lam = -1.0 / (1.0 + e) * dyw;
+                const double d2 = e / ((1 + e) * (1 + e)) * dyw;
+                if (p == hi) g += wp * lam;
+                else g -= wp * lam;
+                h += wp * d2;
+            };
+            for (int a = p % step; a + step <= p; a += step) pair(a, p);
+            for (int b = p + step; b < m; b += step) pair(p, b);
+        }
+        T.g[rp] = g;
+        T.h[rp] = h;
+        const unsigned long long bg = (unsigned long long)__double_as_longlong(fabs(g));
+        const unsigned long long bh = (unsigned long long)__double_as_longlong(fabs(h));
+        mg = bg > mg ? bg : mg;
+        mh = bh > mh ? bh : mh;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_grad_pairwise(Task* tasks) {
     Task& T = tasks[blockIdx.y];
     __shared__ double sy[PAIR_MAX_GROUP], sp[PAIR_MAX_GROUP];
@@ -101,82 +162,24 @@ __global__ __launch_bounds__(256) void k_grad_pairwise(Task* tasks) {
     for (int grp = blockIdx.x; grp < T.n_groups; grp += gridDim.x) {
         const int g0 = T.group_start[grp], m = T.group_start[grp + 1] - g0;
         const int* gr = T.group_rows ? T.group_rows + g0 : nullptr;  // null: rows g0.. in order
+        if (m > PAIR_MAX_GROUP) {
+            pairwise_rows<false>(T, g0, m, gr, sy, sp, mg, mh);
+            continue;
+        }
         __syncthreads();
-        for (int k = threadIdx.x; k < m && k < PAIR_MAX_GROUP; k += blockDim.x) {
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
             const int r = gr ? gr[k] : g0 + k;
             sy[k] = T.y[r];
             sp[k] = T.pred[r];
         }
         __syncthreads();
-        (void)sy;
-        const int step = m > 48 ? max(1, m / 24) : 1;
-        if (m > PAIR_MAX_GROUP) {  // oversized group: read the row values from global memory
-            for (int p = threadIdx.x; p < m; p += blockDim.x) {
-                const int rp = gr ? gr[p] : g0 + p;
-                const double wp = T.w ? T.w[rp] : 1.0;
-                double g = 0.0, h = 1e-6;
-                const int step = max(1, m / 24);
-                auto val_y = [&](int q) { return T.y[gr ? gr[q] : g0 + q]; };
-                auto val_p = [&](int q) { return T.pred[gr ? gr[q] : g0 + q]; };
-                auto pair = [&](int a, int b) {
-                    const double ya = val_y(a), yb = val_y(b);
-                    if (ya == yb) return;
-                    const bool b_hi = yb > ya;
-                    const int hi = b_hi ? b : a, lo = b_hi ? a : b;
-                    const double s = val_p(hi) - val_p(lo);
-                    const double sc = s < -30.0 ? -30.0 : (s > 30.0 ? 30.0 : s);
-                    const double e = exp(sc);
-                    const double dyw = fmax(fabs(val_y(hi) - val_y(lo)), 1e-6);
-                    const double lam = -1.0 / (1.0 + e) * dyw;
-                    const double d2 = e / ((1 + e) * (1 + e)) * dyw;
-                    if (p == hi) g += wp * lam;
-                    else g -= wp * lam;
-                    h += wp * d2;
-                };
-                for (int a = p % step; a + step <= p; a += step) pair(a, p);
-                for (int b = p + step; b < m; b += step) pair(p, b);
-                T.g[rp] = g;
-                T.h[rp] = h;
-                const unsigned long long bg = (unsigned long long)__double_as_longlong(fabs(g));
-                const unsigned long long bh = (unsigned long long)__double_as_longlong(fabs(h));
-                mg = bg > mg ? bg : mg;
-                mh = bh > mh ? bh : mh;
-            }
-            continue;
-        }
-        for (int p = threadIdx.x; p < m; p += blockDim.x) {
-            const int rp = gr ? gr[p] : g0 + p;
-            const double wp = T.w ? T.w[rp] : 1.0;
-            double g = 0.0, h = 1e-6;
-            if (m >= 2) {
-                auto pair = [&](int a, int b) {  // a < b, positions within the group
-                    const double ya = sy[a], yb = sy[b];
-                    if (ya == yb) return;
-                    const bool b_hi = yb > ya;
-                    const int hi = b_hi ? b : a, lo = b_hi ? a : b;
-                    const double s = sp[hi] - sp[lo];
-                    const double sc = s < -30.0 ? -30.0 : (s > 30.0 ? 30.0 : s);
-                    const double e = exp(sc);
-                    const double dyw = fmax(fabs(sy[hi] - sy[lo]), 1e-6);
-                    const double lam = -1.0 / (1.0 + e) * dyw;
-                    const double d2 = e / ((1 + e) * (1 + e)) * dyw;
-                    if (p == hi) g += wp * lam;
-                    else g -= wp * lam;
-                    h += wp * d2;
-                };
-                for (int a = p % step; a + step <= p; a += step) pair(a, p);
-                for (int b = p + step; b < m; b += step) pair(p, b);
-            }
-            T.g[rp] = g;
-            T.h[rp] = h;
-            const unsigned long long bg = (unsigned long long)__double_as_longlong(fabs(g));
-            const unsigned long long bh = (unsigned long long)__double_as_longlong(fabs(h));
-            mg = bg > mg ? bg : mg;
-            mh = bh > mh ? bh : mh;
-        }
+        pairwise_rows<true>(T, g0, m, gr, sy, sp, mg, mh);
     }
-    block_max_u64(mg, &T.max_g_bits);
-    block_max_u64(mh, &T.max_h_bits);
+    // rows outside every group keep g = 0, h = 1e-6 (set by k_grad_fill): include 1e-6 in max h
+    const unsigned long long fill_h = (unsigned long long)__double_as_longlong(1e-6);
+    mh = mh > fill_h ? mh : fill_h;
+    block_max2_store(mg, mh, T);
+    if (blockIdx.x == 0 && threadIdx.x == 0) T.n_red = gridDim.x;
 }
 
 // Rows outside any group keep g = 0, h = 1e-6 (objective.cpp:180-181).
@@ -188,49 +191,62 @@ __global__ void k_grad_fill(Task* tasks) {
     }
 }
 
-__global__ void k_reset_max(Task* tasks) {
-    if (threadIdx.x == 0) {
-        tasks[blockIdx.x].max_g_bits = 0;
-        tasks[blockIdx.x].max_h_bits = 0;
-    }
-}
-
 void launch_grad(hipStream_t st, Task* tasks, int n_tasks, int objective, int max_rows,
                  int max_groups) {
-    hipLaunchKernelGGL(k_reset_max, dim3(n_tasks), dim3(64), 0, st, tasks);
     if (objective == 5) {
-        hipLaunchKernelGGL(k_grad_fill, dim3(std::max(1, std::min(2048, (max_rows + 255) / 256)), n_tasks),
+        hipLaunchKernelGGL(k_grad_fill, dim3(std::max(1, std::min(1024, (max_rows + 1023) / 1024)), n_tasks),
                            dim3(256), 0, st, tasks);
-        hipLaunchKernelGGL(k_grad_pairwise, dim3(std::max(1, std::min(max_groups, 8192)), n_tasks),
+        hipLaunchKernelGGL(k_grad_pairwise, dim3(std::max(1, std::min(max_groups, MAX_RED_BLOCKS)), n_tasks),
                            dim3(256), 0, st, tasks);
     } else {
-        hipLaunchKernelGGL(k_grad_mse, dim3(std::max(1, std::min(2048, (max_rows + 255) / 256)), n_tasks),
+        hipLaunchKernelGGL(k_grad_mse, dim3(std::max(1, std::min(1024, (max_rows + 1023) / 1024)), n_tasks),
                            dim3(256), 0, st, tasks);
     }
 }
 
 // =================================================================== tree start
 // Quantization exponents from the gradient maxima, root node over the sample.
-__global__ void k_tree_init(Task* tasks, int n_features) {
+__global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     Task& T = tasks[blockIdx.x];
+    unsigned long long a = 0, b = 0;
+    for (int k = threadIdx.x; k < T.n_red; k += blockDim.x) {
+        a = T.red[k] > a ? T.red[k] : a;
+        b = T.red[MAX_RED_BLOCKS + k] > b ? T.red[MAX_RED_BLOCKS + k] : b;
+    }
+    __shared__ unsigned long long sa[4], sb[4];
+    a = wave_max_u64(a);
+    b = wave_max_u64(b);
+    if ((threadIdx.x & 63) == 0) {
+        sa[threadIdx.x >> 6] = a;
+        sb[threadIdx.x >> 6] = b;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    int eg = mag_exponent(T.max_g_bits);
+    for (int k = 1; k < 4; ++k) {
+        a = sa[k] > a ? sa[k] : a;
+        b = sb[k] > b ? sb[k] : b;
+    }
+    T.max_g_bits = a;
+    T.max_h_bits = b;
+    int eg = mag_exponent(a);
+    int eh = mag_exponent(b);
     if (T.goss && T.goss_scale > 1.0) {
         int es;
         frexp(T.goss_scale, &es);
         eg += es;
+        eh += es;
     }
-    const int eh = mag_exponent(T.max_h_bits) + ((T.goss && T.goss_scale > 1.0) ? 4 : 0);
     T.s_g = QG_BITS - eg;
     T.s_h = QH_BITS - eh;
-    T.s_r = QR_BITS - mag_exponent(T.max_g_bits);
-    T.s_rh = QR_BITS - mag_exponent(T.max_h_bits);
+    T.s_r = QR_BITS - mag_exponent(a);
+    T.s_rh = QR_BITS - mag_exponent(b);
     T.s_e = QE_BITS - 2 * eg;
     T.n_nodes = 1;
     T.done = 0;
     T.pending0 = 0;
     T.pending1 = -1;
     T.cur_leaf = -1;
+    T.scan_ticket = 0;
     DNode& r = T.nodes[0];
     for (int k = 0; k < 10; ++k) r.v[k] = -1;
     r.v[N_LEAF] = 1;
@@ -240,9 +256,9 @@ __global__ void k_tree_init(Task* tasks, int n_features) {
     r.v[N_DEPTH] = 0;
     r.seg_start = 0;
     r.seg_count = T.n_sample;
+    r.buf = 0;
     r.value = 0.0;
     r.gain = 0.0;
-    (void)n_features;
 }
 
 // Packed per-row histogram operands: pk = 2^48 + rint(g_use 2^s_g), qh = rint(h_use 2^s_h).
@@ -260,52 +276,68 @@ __global__ __launch_bounds__(256) void k_pack(Task* tasks) {
         const long long q = __double2ll_rn(ldexp(g, sg));
         T.pk[i] = CNT_ONE + (unsigned long long)q;
         if (!T.hconst) T.qh[i] = (unsigned long long)__double2ll_rn(ldexp(h, sh));
+        if (T.contiguous && i < T.n_sample) T.idx[i] = i;
     }
 }
 
-// Identity partition for "all rows" samples.
-__global__ void k_iota(Task* tasks) {
+// Zero the histogram slots of every node of the tree (once per tree).
+__global__ __launch_bounds__(256) void k_zero_slots(Task* tasks, size_t words) {
     Task& T = tasks[blockIdx.y];
-    if (!T.contiguous) return;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n_sample; i += gridDim.x * blockDim.x)
-        T.idx[i] = i;
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < words; j += (size_t)gridDim.x * blockDim.x) {
+        T.hg[j] = 0;
+        T.hc[j] = 0;
+        if (!T.hconst) T.hh[j] = 0;
+    }
 }
 
-// Zero the histogram slots of `count` nodes starting at node id first (first < 0: the two
-// children of the current split, i.e. ids n_nodes-2, n_nodes-1 after the split was applied).
-__global__ __launch_bounds__(256) void k_zero_slots(Task* tasks, int F, int first, int count) {
-    Task& T = tasks[blockIdx.y];
-    int n0 = first;
-    if (first < 0) {
-        if (T.done) return;
-        n0 = T.n_nodes - 2;
-    }
-    const size_t per = (size_t)F * NBIN;
-    const size_t total = per * count;
-    const size_t base = (size_t)n0 * per;
-    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < total; j += (size_t)gridDim.x * blockDim.x) {
-        T.hg[base + j] = 0;
-        T.hc[base + j] = 0;
-        if (!T.hconst) T.hh[base + j] = 0;
-    }
+void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows,
+                       int max_nodes) {
+    const int rb = std::max(1, std::min(2048, (max_rows + 1023) / 1024));
+    hipLaunchKernelGGL(k_tree_init, dim3(n_tasks), dim3(256), 0, st, tasks);
+    hipLaunchKernelGGL(k_pack, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
+    const size_t words = (size_t)max_nodes * S.n_features * NBIN;
+    const int zb = (int)std::min<size_t>(2048, (words + 1023) / 1024);
+    hipLaunchKernelGGL(k_zero_slots, dim3(zb, n_tasks), dim3(256), 0, st, tasks, words);
 }
 
 // =================================================================== histogram build
-// Which node a histogram launch fills: the root (node 0, rows [0, n_sample) of the partition)
-// or the smaller child of the current split (reference tree.cpp:418-428).
-__device__ inline bool hist_target(const Task& T, bool root, int& node, int& start, int& count) {
+// Which node a histogram launch fills: the root (node 0) or the smaller child of the current
+// split (reference tree.cpp:418-428: left when n_left <= n_right), with its row range.  The
+// children's ranges follow from the partition's left count; block (0,0) of the child launch
+// records them in the node table and the step descriptor for the scan and the tree end.
+__device__ inline bool hist_target(const Task& T, Task* Tw, bool root, int& node, int& start,
+                                   int& count, int& buf) {
     if (root) {
         node = 0;
         start = 0;
         count = T.n_sample;
+        buf = 0;
         return true;
     }
     if (T.done) return false;
-    const int L = T.nodes[T.cur_leaf].v[N_LEFT], R = T.nodes[T.cur_leaf].v[N_RIGHT];
-    const int cl = T.nodes[L].seg_count, cr = T.nodes[R].seg_count;
-    node = (cl <= cr) ? L : R;
-    start = T.nodes[node].seg_start;
-    count = T.nodes[node].seg_count;
+    const int nl = T.part_lcnt, cnt = T.cur_count, st = T.cur_start;
+    const int L = T.cur_left, R = T.cur_left + 1;
+    const bool left_small = nl <= cnt - nl;
+    node = left_small ? L : R;
+    start = left_small ? st : st + nl;
+    count = left_small ? nl : cnt - nl;
+    buf = 1 - T.cur_buf;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+        DNode& l = T.nodes[L];
+        DNode& r = T.nodes[R];
+        l.seg_start = st;
+        l.seg_count = nl;
+        l.v[N_NS] = nl;
+        r.seg_start = st + nl;
+        r.seg_count = cnt - nl;
+        r.v[N_NS] = cnt - nl;
+        Tw->n_left = nl;
+        Tw->small_node = node;
+        Tw->big_node = left_small ? R : L;
+        Tw->small_start = start;
+        Tw->small_count = count;
+        Tw->small_buf = buf;
+    }
     return true;
 }
 
@@ -326,41 +358,65 @@ __device__ inline void hist_flush(const Task& T, const unsigned long long* lA,
     }
 }
 
-// Row-major kernel: each lane loads 16 consecutive feature bins of one row (128-bit load);
-// LPR lanes cover the block's FB = 16*LPR features of a row.  Works on the partition index
-// (gathered rows) or on contiguous rows.  One LDS word per (feature, bin) packs count and the
-// fixed-point g sum; a second word holds q_h when h is not constant.
+// Row kernel (partitioned rows): each lane loads 16 consecutive feature bins of one row
+// (128-bit load); LPR lanes cover the block's FB = 16*LPR features of a row.  A lane issues the
+// index loads of RPL rows, then their bin and gradient loads, then 16*RPL LDS atomics, so one
+// block pass costs two memory round trips.  The block's row range is sized from the node's
+// actual row count, so small nodes still spread over many blocks.
 template <int LPR, bool HV>
-__global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int root, int rows_per_block) {
+__global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int root, int min_rows) {
     constexpr int FB = 16 * LPR;
+    constexpr int RPL = 8;
     extern __shared__ unsigned long long lds[];
     unsigned long long* lA = lds;
     unsigned long long* lB = lds + FB * NBIN;
-    const Task& T = tasks[blockIdx.z];
-    int node, start, count;
-    if (!hist_target(T, root != 0, node, start, count)) return;
-    const int r0 = blockIdx.x * rows_per_block;
+    const Task T = tasks[blockIdx.z];
+    int node, start, count, buf;
+    if (!hist_target(T, tasks + blockIdx.z, root != 0, node, start, count, buf)) return;
+    int per = (count + gridDim.x - 1) / gridDim.x;
+    per = max(per, min_rows);
+    per = min((per + 63) & ~63, MAX_FLUSH_ROWS);
+    const int r0 = blockIdx.x * per;
     if (r0 >= count) return;
-    const int r1 = min(count, r0 + rows_per_block);
+    const int r1 = min(count, r0 + per);
     const int f0 = blockIdx.y * FB;
     for (int j = threadIdx.x; j < FB * NBIN * (HV ? 2 : 1); j += blockDim.x) lds[j] = 0;
-    __syncthreads();
     const int sub = threadIdx.x % LPR;
     const int fcol = f0 + sub * 16;
+    const int* __restrict__ rows = (buf ? T.idx_tmp : T.idx) + start;
+    const unsigned long long* __restrict__ pk = T.pk;
+    const unsigned long long* __restrict__ qhp = T.qh;
     const bool contig = T.contiguous && root;
-    for (int k = r0 + (int)threadIdx.x / LPR; k < r1; k += blockDim.x / LPR) {
-        const int i = contig ? k : T.idx[start + k];
-        if (fcol < S.fp) {
-            const uint4 b = *(const uint4*)(S.bins_row + (size_t)i * S.fp + fcol);
-            const unsigned long long p = T.pk[i];
-            const unsigned long long q = HV ? T.qh[i] : 0ull;
-            const unsigned int w4[4] = {b.x, b.y, b.z, b.w};
+    const int stride = blockDim.x / LPR;
+    const bool active = fcol < S.fp;
+    __syncthreads();
+    for (int k = r0 + (int)threadIdx.x / LPR; k < r1; k += stride * RPL) {
+        int i[RPL];
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int kk = k + u * stride;
+            i[u] = kk < r1 ? (contig ? kk : rows[kk]) : -1;
+        }
+        uint4 b[RPL];
+        unsigned long long p[RPL], q[RPL];
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            const int ii = i[u] < 0 ? 0 : i[u];
+            b[u] = active ? *(const uint4*)(S.bins_row + (size_t)ii * S.fp + fcol) : make_uint4(0, 0, 0, 0);
+            p[u] = pk[ii];
+            q[u] = HV ? qhp[ii] : 0ull;
+        }
+        if (!active) continue;
+#pragma unroll
+        for (int u = 0; u < RPL; ++u) {
+            if (i[u] < 0) continue;
+            const unsigned int w4[4] = {b[u].x, b[u].y, b[u].z, b[u].w};
 #pragma unroll
             for (int t = 0; t < 16; ++t) {
                 const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
                 const int slot = (sub * 16 + t) * NBIN + bin;
-                atomicAdd(&lA[slot], p);
-                if (HV) atomicAdd(&lB[slot], q);
+                atomicAdd(&lA[slot], p[u]);
+                if (HV) atomicAdd(&lB[slot], q[u]);
             }
         }
     }
@@ -368,19 +424,21 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
     hist_flush<HV>(T, lA, lB, FB, f0, S.n_features, node);
 }
 
-// Column-major kernel for contiguous rows: each lane loads 16 consecutive rows of one
-// feature column (128-bit load); the block covers FB features and a long run of rows, so the
-// per-block partial histograms stay small next to the bytes read.
+// Column kernel for the contiguous root: each lane loads 16 consecutive rows of each of the
+// block's FB feature columns (128-bit loads) plus their 16 packed gradients, then issues the
+// FB*16 LDS atomics back to back.  Blocks cover long row runs so per-block partial histograms
+// stay small next to the bytes read.
 template <int FB, bool HV>
 __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int rows_per_block) {
     __shared__ unsigned long long lA[FB * NBIN];
     __shared__ unsigned long long lB[HV ? FB * NBIN : 1];
-    const Task& T = tasks[blockIdx.z];
+    const Task T = tasks[blockIdx.z];
     const int count = T.n_sample;
     const int r0 = blockIdx.x * rows_per_block;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + rows_per_block);
     const int f0 = blockIdx.y * FB;
+    const int nf = min(FB, S.n_features - f0);
     for (int j = threadIdx.x; j < FB * NBIN; j += blockDim.x) {
         lA[j] = 0;
         if (HV) lB[j] = 0;
@@ -389,25 +447,48 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     for (int base = r0 + wave * 1024; base < r1; base += nwave * 1024) {
         const int row = base + lane * 16;
-        if (row >= r1) continue;
-        unsigned long long p[16], q[16];
+        if (base + 1024 <= r1 && nf == FB) {
+            // full tile: 16 rows per lane, no per-row checks
+            const uint4* pp = (const uint4*)(T.pk + row);
+            uint4 qv[8];
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const bool ok = row + t < r1;
-            p[t] = ok ? T.pk[row + t] : 0ull;
-            q[t] = (HV && ok) ? T.qh[row + t] : 0ull;
-        }
-        for (int fl = 0; fl < FB; ++fl) {
-            const int f = f0 + fl;
-            if (f >= S.n_features) break;
-            const uint4 b = *(const uint4*)(S.bins_col + (size_t)f * S.n_pad + row);
-            const unsigned int w4[4] = {b.x, b.y, b.z, b.w};
+            for (int t = 0; t < 8; ++t) qv[t] = pp[t];
+            uint4 hv[HV ? 8 : 1];
+            if (HV) {
+                const uint4* qq = (const uint4*)(T.qh + row);
 #pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                if (!p[t]) continue;
-                const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
-                atomicAdd(&lA[fl * NBIN + bin], p[t]);
-                if (HV) atomicAdd(&lB[fl * NBIN + bin], q[t]);
+                for (int t = 0; t < 8; ++t) hv[t] = qq[t];
+            }
+            uint4 b[FB];
+#pragma unroll
+            for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(S.bins_col + (size_t)(f0 + fl) * S.n_pad + row);
+#pragma unroll
+            for (int fl = 0; fl < FB; ++fl) {
+                const unsigned w4[4] = {b[fl].x, b[fl].y, b[fl].z, b[fl].w};
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const uint4 v = qv[t >> 1];
+                    const unsigned long long p = (t & 1) ? ((unsigned long long)v.w << 32 | v.z)
+                                                         : ((unsigned long long)v.y << 32 | v.x);
+                    const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
+                    atomicAdd(&lA[fl * NBIN + bin], p);
+                    if (HV) {
+                        const uint4 hw = hv[t >> 1];
+                        const unsigned long long q = (t & 1) ? ((unsigned long long)hw.w << 32 | hw.z)
+                                                             : ((unsigned long long)hw.y << 32 | hw.x);
+                        atomicAdd(&lB[fl * NBIN + bin], q);
+                    }
+                }
+            }
+        } else if (row < r1) {
+            // tail tile: per-row checks
+            for (int fl = 0; fl < nf; ++fl) {
+                const unsigned char* col = S.bins_col + (size_t)(f0 + fl) * S.n_pad;
+                for (int t = 0; t < 16 && row + t < r1; ++t) {
+                    const int bin = col[row + t];
+                    atomicAdd(&lA[fl * NBIN + bin], T.pk[row + t]);
+                    if (HV) atomicAdd(&lB[fl * NBIN + bin], T.qh[row + t]);
+                }
             }
         }
     }
@@ -420,25 +501,25 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     if (max_count <= 0) return;
     if (root && contiguous_root && variant != 2) {
         constexpr int FB = 8;
-        const int rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, ((max_count / 64) + 1023) / 1024 * 1024));
-        dim3 grid((max_count + rpb - 1) / rpb, (S.n_features + FB - 1) / FB, n_tasks);
+        // ~2 blocks per CU over the whole matrix; rows per block a multiple of the 4-wave tile
+        const int fbs = (S.n_features + FB - 1) / FB;
+        int rpb = (int)(((long long)max_count * fbs / 512 + 4095) / 4096 * 4096);
+        rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, rpb));
+        dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
         if (hconst) hipLaunchKernelGGL((k_hist_col<FB, false>), grid, dim3(256), 0, st, S, tasks, rpb);
         else hipLaunchKernelGGL((k_hist_col<FB, true>), grid, dim3(256), 0, st, S, tasks, rpb);
         return;
     }
-    // Row-major: LPR = 2 (32 features per block).
+    // Row kernel: LPR = 2 (32 features per block), 512 threads.
     constexpr int LPR = 2, FB = 32;
-    const int rpb = std::min(MAX_FLUSH_ROWS, std::max(1024, ((max_count / 128) + 255) / 256 * 256));
-    dim3 grid((max_count + rpb - 1) / rpb, (S.fp + FB - 1) / FB, n_tasks);
+    const int fbs = (S.fp + FB - 1) / FB;
+    const int min_rows = 2048;
+    int gx = std::max(1, std::min(512 / fbs, (max_count + min_rows - 1) / min_rows));
+    gx = std::max(gx, (max_count + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS);
+    dim3 grid(gx, fbs, n_tasks);
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
-    static bool lds_attr = false;
-    if (!lds_attr) {  // dynamic LDS above 64 KiB needs the per-kernel opt-in
-        (void)hipFuncSetAttribute((const void*)k_hist_row<LPR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        (void)hipFuncSetAttribute((const void*)k_hist_row<LPR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        lds_attr = true;
-    }
-    if (hconst) hipLaunchKernelGGL((k_hist_row<LPR, false>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, rpb);
-    else hipLaunchKernelGGL((k_hist_row<LPR, true>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, rpb);
+    if (hconst) hipLaunchKernelGGL((k_hist_row<LPR, false>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, min_rows);
+    else hipLaunchKernelGGL((k_hist_row<LPR, true>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, min_rows);
 }
 
 // =================================================================== split scan
@@ -446,10 +527,7 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
 // histogram.cpp:144-193): totals over all bins, bin 0 = missing, thresholds b in [1, nb-2],
 // empty bins skipped, missing-right (ml=0) before missing-left, first maximum kept.  One wave per
 // feature, 4 bins per lane; prefix sums are exact int64, so the scan order cannot change them.
-// For the larger child the histogram is first formed as parent - smaller (exact).
-constexpr int SCAN_FPB = 8;   // features per block (2 per wave)
-
-// Lane j holds bins 4j..4j+3 (qg/c/qh preloaded; bins >= nb must be zero).
+// Lane j holds bins 4j..4j+3 (bins >= nb must be zero).
 __device__ inline void scan_feature_wave(const long long* qg, const int* c, const long long* qh,
                                          bool hconst, double inv_g, double inv_h, int nb,
                                          double lam, double gamma, int min_leaf, double min_hess,
@@ -525,26 +603,171 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
     out_ml = (bkey == 0x7fffffff) ? 1 : (bkey & 1);
 }
 
-// mode 0: scan node 0 (root); mode 1: scan both children of the current split.
-__global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode) {
-    Task& T = tasks[blockIdx.z];
+// Best-first selection of the next split (tree.cpp:343-392, 445-456), run by one block: reduce
+// the freshly scanned nodes' per-feature results to cached leaf candidates (tree.cpp:160-166:
+// ascending features, skip invalid or gain < min_split_gain, first maximum kept), then pick the
+// leaf with the best candidate (lowest node id on ties; gain must exceed min_split_gain).
+__device__ void select_block(Task& T, int F, const GrowP& P) {
+    __shared__ double sg[16];
+    __shared__ int sk[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int pi = 0; pi < 2; ++pi) {
+        const int nd = pi == 0 ? T.pending0 : T.pending1;
+        if (nd < 0) continue;
+        double best = -INFINITY;
+        int bk = 0x7fffffff;
+        for (int f = threadIdx.x; f < F; f += blockDim.x) {
+            const FCand c = T.fcand[(size_t)nd * F + f];
+            if (c.thr < 0 || c.gain < P.gamma) continue;
+            if (cand_better(c.gain, f, best, bk)) {
+                best = c.gain;
+                bk = f;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double og = __shfl_xor(best, o, 64);
+            const int ok = __shfl_xor(bk, o, 64);
+            if (cand_better(og, ok, best, bk)) {
+                best = og;
+                bk = ok;
+            }
+        }
+        if (lane == 0) {
+            sg[wave] = best;
+            sk[wave] = bk;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < nw; ++w)
+                if (cand_better(sg[w], sk[w], best, bk)) {
+                    best = sg[w];
+                    bk = sk[w];
+                }
+            DCand& c = T.cands[nd];
+            const DNode& n = T.nodes[nd];
+            const bool elig = n.v[N_DEPTH] < P.max_depth && n.v[N_NS] >= 2 * P.min_leaf;
+            c.valid = (bk != 0x7fffffff) && elig;
+            c.gain = best;
+            c.feature = bk == 0x7fffffff ? -1 : bk;
+            c.feature2 = -1;
+            c.type = 0;
+            if (bk != 0x7fffffff) {
+                const FCand fc = T.fcand[(size_t)nd * F + bk];
+                c.thr = fc.thr;
+                c.mgl = fc.ml;
+            } else {
+                c.thr = -1;
+                c.mgl = 1;
+            }
+        }
+        __syncthreads();
+    }
+    // leaf choice: strict '>' over ascending node ids == (gain desc, id asc)
+    double best = -INFINITY;
+    int bid = 0x7fffffff;
+    const int nn = T.n_nodes;
+    for (int lid = threadIdx.x; lid < nn; lid += blockDim.x) {
+        const DCand c = T.cands[lid];
+        if (!T.nodes[lid].v[N_LEAF] || !c.valid || c.feature < 0 || !(c.gain >= P.gamma)) continue;
+        if (!(c.gain > P.gamma)) continue;
+        if (cand_better(c.gain, lid, best, bid)) {
+            best = c.gain;
+            bid = lid;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double og = __shfl_xor(best, o, 64);
+        const int ok = __shfl_xor(bid, o, 64);
+        if (cand_better(og, ok, best, bid)) {
+            best = og;
+            bid = ok;
+        }
+    }
+    __syncthreads();
+    if (lane == 0) {
+        sg[wave] = best;
+        sk[wave] = bid;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < nw; ++w)
+        if (cand_better(sg[w], sk[w], best, bid)) {
+            best = sg[w];
+            bid = sk[w];
+        }
+    if (T.n_nodes + 2 > MAX_NODES || T.n_nodes + 2 > 2 * P.max_leaves - 1 || bid == 0x7fffffff) {
+        T.done = 1;
+        return;
+    }
+    const DCand c = T.cands[bid];
+    T.cur_leaf = bid;
+    T.cur_type = c.type;
+    T.cur_feat = c.feature;
+    T.cur_feat2 = c.feature2;
+    T.cur_thr = c.thr;
+    T.cur_mgl = c.mgl;
+    T.cur_gain = c.gain;
+}
+
+// Split bookkeeping for the decision in T.cur_* (tree.cpp:385-392, 445-456); one thread.
+__device__ void apply_decision(Task& T) {
+    if (T.done) return;
+    const int leaf = T.cur_leaf;
+    DNode& p = T.nodes[leaf];
+    const int L = T.n_nodes, R = T.n_nodes + 1;
+    p.v[N_LEAF] = 0;
+    p.v[N_TYPE] = T.cur_type;
+    p.v[N_FEAT] = T.cur_feat;
+    p.v[N_FEAT2] = T.cur_feat2;
+    p.v[N_THR] = T.cur_thr;
+    p.v[N_MGL] = T.cur_mgl;
+    p.v[N_LEFT] = L;
+    p.v[N_RIGHT] = R;
+    p.gain = T.cur_gain;
+    T.cur_start = p.seg_start;
+    T.cur_count = p.seg_count;
+    T.cur_buf = p.buf;
+    T.cur_left = L;
+    T.part_lcnt = 0;
+    T.part_rcnt = 0;
+    for (int side = 0; side < 2; ++side) {
+        DNode& c = T.nodes[side == 0 ? L : R];
+        for (int k = 0; k < 10; ++k) c.v[k] = -1;
+        c.v[N_LEAF] = 1;
+        c.v[N_TYPE] = 0;
+        c.v[N_MGL] = 1;
+        c.v[N_NS] = 0;
+        c.v[N_DEPTH] = p.v[N_DEPTH] + 1;
+        c.seg_start = p.seg_start;
+        c.seg_count = 0;
+        c.buf = 1 - p.buf;
+        c.value = 0.0;
+        c.gain = 0.0;
+        T.cands[side == 0 ? L : R].valid = 0;
+    }
+    T.n_nodes += 2;
+}
+
+// mode 0: scan node 0 (root); mode 1: scan both children of the current split.  With
+// fuse_select, the last block to finish (completion ticket) selects and applies the next split.
+constexpr int SCAN_FPB = 4;   // features per block (one per wave)
+__global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode, int fuse_select) {
+    const Task T = tasks[blockIdx.z];
+    Task& Tw = tasks[blockIdx.z];
+    if (T.done) return;
     int node, small = -1, parent = -1;
     if (mode == 0) {
-        if (blockIdx.y != 0) return;
         node = 0;
     } else {
-        if (T.done) return;
         parent = T.cur_leaf;
-        const int L = T.nodes[parent].v[N_LEFT], R = T.nodes[parent].v[N_RIGHT];
-        small = (T.nodes[L].seg_count <= T.nodes[R].seg_count) ? L : R;
-        node = blockIdx.y == 0 ? L : R;
+        small = T.small_node;
+        node = T.cur_left + (int)blockIdx.y;
     }
     const int F = S.n_features;
     const size_t per = (size_t)F * NBIN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int fl = wave; fl < SCAN_FPB; fl += 4) {
-        const int f = blockIdx.x * SCAN_FPB + fl;
-        if (f >= F) break;
+    const int f = blockIdx.x * SCAN_FPB + wave;
+    if (f < F) {
         const size_t off = (size_t)node * per + (size_t)f * NBIN;
         long long qg[4], qh[4];
         int c[4];
@@ -592,166 +815,80 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
             }
         }
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && blockIdx.y == 0) {
-        if (mode == 0) {
-            T.pending0 = 0;
-            T.pending1 = -1;
-        } else {
-            T.pending0 = T.nodes[parent].v[N_LEFT];
-            T.pending1 = T.nodes[parent].v[N_RIGHT];
+    if (!fuse_select) {
+        if (blockIdx.x == 0 && threadIdx.x == 0 && blockIdx.y == 0) {
+            Tw.pending0 = mode == 0 ? 0 : T.cur_left;
+            Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
         }
+        return;
     }
+    // completion ticket: the last block of this task selects the next split
+    __shared__ int s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = gridDim.x * gridDim.y;
+        const int t = atomicAdd(&Tw.scan_ticket, 1);
+        s_last = (t == total - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        Tw.scan_ticket = 0;
+        Tw.pending0 = mode == 0 ? 0 : T.cur_left;
+        Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
+    }
+    __syncthreads();
+    select_block(Tw, F, P);
+    if (threadIdx.x == 0) apply_decision(Tw);
 }
 
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
+                 int fuse_select) {
     dim3 grid((S.n_features + SCAN_FPB - 1) / SCAN_FPB, mode == 0 ? 1 : 2, n_tasks);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, S, tasks, P, mode);
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, S, tasks, P, mode, fuse_select);
 }
 
-// =================================================================== best-first selection
-// Reduce the per-feature results of freshly scanned nodes to one cached candidate each
-// (tree.cpp:160-166: ascending features, skip invalid or gain < min_split_gain, strict '>'),
-// then pick the leaf to split (tree.cpp:343-383: lowest node id among equal scores) and apply
-// the split bookkeeping (tree.cpp:385-392, 445-456).  mode 1 applies a host-made decision.
-__global__ __launch_bounds__(256) void k_select(Task* tasks, int F, GrowP P, int mode) {
+// Device-made decision: reduce the freshly scanned nodes, choose the next split, apply it.
+__global__ __launch_bounds__(256) void k_select(Task* tasks, int F, GrowP P) {
     Task& T = tasks[blockIdx.x];
     if (T.done) return;
-    __shared__ double sg[4];
-    __shared__ int sk[4];
-    __shared__ int s_leaf;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (mode == 0) {
-        for (int pi = 0; pi < 2; ++pi) {
-            const int nd = pi == 0 ? T.pending0 : T.pending1;
-            if (nd < 0) continue;
-            double best = -INFINITY;
-            int bk = 0x7fffffff;
-            for (int f = threadIdx.x; f < F; f += blockDim.x) {
-                const FCand c = T.fcand[(size_t)nd * F + f];
-                if (c.thr < 0 || c.gain < P.gamma) continue;
-                if (cand_better(c.gain, f, best, bk)) {
-                    best = c.gain;
-                    bk = f;
-                }
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                const double og = __shfl_xor(best, o, 64);
-                const int ok = __shfl_xor(bk, o, 64);
-                if (cand_better(og, ok, best, bk)) {
-                    best = og;
-                    bk = ok;
-                }
-            }
-            if (lane == 0) {
-                sg[wave] = best;
-                sk[wave] = bk;
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
-                    if (cand_better(sg[w], sk[w], best, bk)) {
-                        best = sg[w];
-                        bk = sk[w];
-                    }
-                DCand& c = T.cands[nd];
-                const DNode& n = T.nodes[nd];
-                const bool elig = n.v[N_DEPTH] < P.max_depth && n.v[N_NS] >= 2 * P.min_leaf;
-                c.valid = (bk != 0x7fffffff) && elig;
-                c.gain = best;
-                c.feature = bk == 0x7fffffff ? -1 : bk;
-                c.feature2 = -1;
-                c.type = 0;
-                if (bk != 0x7fffffff) {
-                    const FCand fc = T.fcand[(size_t)nd * F + bk];
-                    c.thr = fc.thr;
-                    c.mgl = fc.ml;
-                } else {
-                    c.thr = -1;
-                    c.mgl = 1;
-                }
-            }
-            __syncthreads();
-        }
-    }
-    if (threadIdx.x != 0) return;
-    if (T.n_nodes + 2 > MAX_NODES || T.n_nodes + 2 > 2 * P.max_leaves - 1) {
+    select_block(T, F, P);
+    if (threadIdx.x == 0) apply_decision(T);
+}
+
+void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P) {
+    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, tasks, F, P);
+}
+
+// Host-made decision (PAIR-capable driver): validate and apply.
+__global__ void k_apply(Task* tasks, int F, GrowP P) {
+    Task& T = tasks[blockIdx.x];
+    if (threadIdx.x != 0 || T.done) return;
+    const int leaf = T.cur_leaf;
+    const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
+                     T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
+                     (T.cur_type != 0 && T.cur_type != 1) || T.n_nodes + 2 > MAX_NODES ||
+                     T.n_nodes + 2 > 2 * P.max_leaves - 1;
+    if (bad) {  // a decision that does not name a splittable leaf stops the tree
         T.done = 1;
         return;
     }
-    int leaf = -1;
-    if (mode == 0) {
-        double best_score = P.gamma;
-        for (int lid = 0; lid < T.n_nodes; ++lid) {
-            const DNode& n = T.nodes[lid];
-            if (!n.v[N_LEAF]) continue;
-            const DCand& c = T.cands[lid];
-            if (!c.valid || c.feature < 0 || !(c.gain >= P.gamma)) continue;
-            if (c.gain > best_score) {
-                best_score = c.gain;
-                leaf = lid;
-            }
-        }
-        if (leaf < 0) {
-            T.done = 1;
-            return;
-        }
-        const DCand& c = T.cands[leaf];
-        T.cur_leaf = leaf;
-        T.cur_type = c.type;
-        T.cur_feat = c.feature;
-        T.cur_feat2 = c.feature2;
-        T.cur_thr = c.thr;
-        T.cur_mgl = c.mgl;
-        T.cur_gain = c.gain;
-    } else {
-        leaf = T.cur_leaf;
-        const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
-                         T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
-                         (T.cur_type != 0 && T.cur_type != 1);
-        if (bad) {  // a host decision that does not name a splittable leaf stops the tree
-            T.done = 1;
-            return;
-        }
-    }
-    (void)s_leaf;
-    DNode& p = T.nodes[leaf];
-    const int L = T.n_nodes, R = T.n_nodes + 1;
-    p.v[N_LEAF] = 0;
-    p.v[N_TYPE] = T.cur_type;
-    p.v[N_FEAT] = T.cur_feat;
-    p.v[N_FEAT2] = T.cur_feat2;
-    p.v[N_THR] = T.cur_thr;
-    p.v[N_MGL] = T.cur_mgl;
-    p.v[N_LEFT] = L;
-    p.v[N_RIGHT] = R;
-    p.gain = T.cur_gain;
-    for (int side = 0; side < 2; ++side) {
-        DNode& c = T.nodes[side == 0 ? L : R];
-        for (int k = 0; k < 10; ++k) c.v[k] = -1;
-        c.v[N_LEAF] = 1;
-        c.v[N_TYPE] = 0;
-        c.v[N_MGL] = 1;
-        c.v[N_NS] = 0;
-        c.v[N_DEPTH] = p.v[N_DEPTH] + 1;
-        c.seg_start = p.seg_start;
-        c.seg_count = 0;
-        c.value = 0.0;
-        c.gain = 0.0;
-        T.cands[side == 0 ? L : R].valid = 0;
-    }
-    T.n_nodes += 2;
+    apply_decision(T);
 }
 
-void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P, int mode) {
-    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, tasks, F, P, mode);
+void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P) {
+    hipLaunchKernelGGL(k_apply, dim3(n_tasks), dim3(64), 0, st, tasks, F, P);
 }
 
 // =================================================================== stable partition
 // Reference split_indices (tree.cpp:68-91): rows of the leaf, in ascending order, go left when
 // Axis: bin 0 -> missing_go_left else bin <= threshold; Pair: either bin 0 -> missing_go_left
-// else pair_hash <= threshold.  Three passes: per-chunk left counts, stable scatter into the
-// scratch index, copy back (block 0 records the children's row ranges).
-constexpr int PART_CHUNK = 4096;   // rows per block (16 per thread)
+// else pair_hash <= threshold.  Two passes: per-chunk left counts, then a stable scatter into
+// the other index buffer (the children's rows live there; block 0 records their ranges).
+constexpr int PART_CHUNK = 2048;   // rows per block (8 per thread)
+constexpr int PART_RPT = 8;
 
 __device__ inline bool row_goes_left(const Shared& S, const Task& T, int i) {
     const int b1 = S.bins_col[(size_t)T.cur_feat * S.n_pad + i];
@@ -776,99 +913,73 @@ __device__ inline int block_sum_int(int v, int* red) {
     return s;
 }
 
-__global__ __launch_bounds__(256) void k_part_count(Shared S, Task* tasks) {
-    Task& T = tasks[blockIdx.y];
+// One pass: each block places its rows with one atomic reservation per side; lefts fill the
+// leaf's range from the front, rights from the back.  The order of rows inside a child's range
+// therefore depends on block timing, but nothing downstream depends on it: every histogram,
+// leaf and TORB sum is an exact integer sum over the set of rows.
+__global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks) {
+    const Task T = tasks[blockIdx.y];
+    Task& Tw = tasks[blockIdx.y];
     if (T.done) return;
-    const DNode& p = T.nodes[T.cur_leaf];
-    const int c0 = blockIdx.x * PART_CHUNK;
-    if (c0 >= p.seg_count) return;
-    __shared__ int red[4];
-    int left = 0;
-    const int k0 = c0 + threadIdx.x * 16;
-    for (int k = k0; k < k0 + 16 && k < p.seg_count; ++k) left += row_goes_left(S, T, T.idx[p.seg_start + k]);
-    const int tot = block_sum_int(left, red);
-    if (threadIdx.x == 0) T.chunk_cnt[blockIdx.x] = tot;
-}
-
-__global__ __launch_bounds__(256) void k_part_scatter(Shared S, Task* tasks) {
-    Task& T = tasks[blockIdx.y];
-    if (T.done) return;
-    const DNode& p = T.nodes[T.cur_leaf];
-    const int cnt = p.seg_count, st = p.seg_start;
+    const int cnt = T.cur_count, st = T.cur_start;
     const int c0 = blockIdx.x * PART_CHUNK;
     if (c0 >= cnt) return;
-    __shared__ int red[4];
-    __shared__ int wsum[4];
-    const int nch = (cnt + PART_CHUNK - 1) / PART_CHUNK;
-    int before = 0, total = 0;
-    for (int c = threadIdx.x; c < nch; c += blockDim.x) {
-        const int v = T.chunk_cnt[c];
-        total += v;
-        if (c < (int)blockIdx.x) before += v;
-    }
-    before = block_sum_int(before, red);
-    total = block_sum_int(total, red);
-    // thread-local flags for 16 consecutive rows
-    const int k0 = c0 + threadIdx.x * 16;
+    __shared__ int wsum[4], wsumr[4];
+    __shared__ int baseL, baseR;
+    const int* __restrict__ src = (T.cur_buf ? T.idx_tmp : T.idx) + st;
+    int* __restrict__ dst = (T.cur_buf ? T.idx : T.idx_tmp) + st;
+    const int k0 = c0 + threadIdx.x * PART_RPT;
     unsigned flags = 0;
-    int rows[16];
-    int nl = 0;
-    for (int t = 0; t < 16; ++t) {
-        const int k = k0 + t;
-        rows[t] = -1;
-        if (k < cnt) {
-            rows[t] = T.idx[st + k];
-            if (row_goes_left(S, T, rows[t])) {
-                flags |= 1u << t;
-                ++nl;
-            }
+    int rows[PART_RPT];
+    int nl = 0, nr = 0;
+#pragma unroll
+    for (int t = 0; t < PART_RPT; ++t) rows[t] = (k0 + t < cnt) ? src[k0 + t] : -1;
+#pragma unroll
+    for (int t = 0; t < PART_RPT; ++t) {
+        if (rows[t] < 0) continue;
+        if (row_goes_left(S, T, rows[t])) {
+            flags |= 1u << t;
+            ++nl;
+        } else {
+            ++nr;
         }
     }
-    // block-wide exclusive scan of nl
-    const int incl = wave_incl_i(nl);
+    const int incl = wave_incl_i(nl), inclr = wave_incl_i(nr);
     const int wave = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 63) wsum[wave] = incl;
-    __syncthreads();
-    int woff = 0;
-    for (int w = 0; w < wave; ++w) woff += wsum[w];
-    int lrank = before + woff + incl - nl;                       // lefts before my first row
-    int rrank = (k0 - c0) + (c0 - before) - (woff + incl - nl);  // rights before my first row
-    // (k0 - c0) rows of this chunk precede me; (c0 - before) rights precede this chunk
-    for (int t = 0; t < 16; ++t) {
-        if (rows[t] < 0) break;
-        if (flags & (1u << t)) T.idx_tmp[st + lrank++] = rows[t];
-        else T.idx_tmp[st + total + rrank++] = rows[t];
+    if ((threadIdx.x & 63) == 63) {
+        wsum[wave] = incl;
+        wsumr[wave] = inclr;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) T.n_left = total;
-}
-
-__global__ __launch_bounds__(256) void k_part_copy(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
-    if (T.done) return;
-    const DNode& p = T.nodes[T.cur_leaf];
-    const int cnt = p.seg_count, st = p.seg_start;
-    const int c0 = blockIdx.x * PART_CHUNK;
-    if (c0 >= cnt) return;
-    for (int k = c0 + threadIdx.x; k < c0 + PART_CHUNK && k < cnt; k += blockDim.x) T.idx[st + k] = T.idx_tmp[st + k];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int nl = T.n_left;
-        DNode& L = T.nodes[p.v[N_LEFT]];
-        DNode& R = T.nodes[p.v[N_RIGHT]];
-        L.seg_start = st;
-        L.seg_count = nl;
-        L.v[N_NS] = nl;
-        R.seg_start = st + nl;
-        R.seg_count = cnt - nl;
-        R.v[N_NS] = cnt - nl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tl = 0, tr = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+            tl += wsum[w];
+            tr += wsumr[w];
+        }
+        baseL = tl ? atomicAdd(&Tw.part_lcnt, tl) : 0;
+        baseR = tr ? atomicAdd(&Tw.part_rcnt, tr) : 0;
+    }
+    __syncthreads();
+    int woff = 0, woffr = 0;
+    for (int w = 0; w < wave; ++w) {
+        woff += wsum[w];
+        woffr += wsumr[w];
+    }
+    int lpos = baseL + woff + incl - nl;
+    int rpos = baseR + woffr + inclr - nr;
+#pragma unroll
+    for (int t = 0; t < PART_RPT; ++t) {
+        if (rows[t] < 0) break;
+        if (flags & (1u << t)) dst[lpos++] = rows[t];
+        else dst[cnt - 1 - rpos++] = rows[t];
     }
 }
 
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count) {
     const int nch = std::max(1, (max_count + PART_CHUNK - 1) / PART_CHUNK);
     dim3 grid(nch, n_tasks);
-    hipLaunchKernelGGL(k_part_count, grid, dim3(256), 0, st, S, tasks);
-    hipLaunchKernelGGL(k_part_scatter, grid, dim3(256), 0, st, S, tasks);
-    hipLaunchKernelGGL(k_part_copy, grid, dim3(256), 0, st, tasks);
+    hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, S, tasks);
 }
 
 // =================================================================== host-driven helpers
@@ -877,26 +988,32 @@ void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks,
 __global__ __launch_bounds__(256) void k_energy(Task* tasks, int task, int node) {
     Task& T = tasks[task];
     const DNode& n = T.nodes[node];
+    const int* rows = node_rows(T, n) + n.seg_start;
     long long acc = 0;
     const double sc = T.goss_scale;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n.seg_count; k += gridDim.x * blockDim.x) {
-        const int i = T.idx[n.seg_start + k];
+        const int i = rows[k];
         double g = T.g[i];
         if (T.goss && T.scaled && T.scaled[i]) g *= sc;
         acc += __double2ll_rn(ldexp(g * g, T.s_e));
     }
     acc = wave_sum_ll(acc);
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd((unsigned long long*)&T.energy[node], (unsigned long long)acc);
+    __shared__ long long sacc[4];
+    if ((threadIdx.x & 63) == 0) sacc[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long s = sacc[0] + sacc[1] + sacc[2] + sacc[3];
+        if (s) atomicAdd((unsigned long long*)&T.energy[node], (unsigned long long)s);
+    }
 }
 
 void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count) {
-    const int blocks = std::max(1, std::min(1024, (count + 255) / 256));
+    const int blocks = std::max(1, std::min(256, (count + 1023) / 1024));
     hipLaunchKernelGGL(k_energy, dim3(blocks), dim3(256), 0, st, tasks, task, node);
 }
 
 // PAIR histograms: one hashed histogram per feature pair over a node's rows (the fused
 // hash+accumulate pass of tree.cpp:243-267), then one scan per pair with lambda = pair_l2.
-constexpr int MAX_PAIRS = 32;
 __global__ __launch_bounds__(256) void k_pair_hist(Shared S, Task* tasks, int task, int node,
                                                    const int* __restrict__ pf, int n_pairs,
                                                    long long* ph_g, int* ph_c, long long* ph_h,
@@ -910,10 +1027,11 @@ __global__ __launch_bounds__(256) void k_pair_hist(Shared S, Task* tasks, int ta
     const int r0 = blockIdx.x * rows_per_block;
     if (r0 >= n.seg_count) return;
     const int r1 = min(n.seg_count, r0 + rows_per_block);
+    const int* rows = node_rows(T, n) + n.seg_start;
     for (int j = threadIdx.x; j < n_pairs * NBIN * (hv ? 2 : 1); j += blockDim.x) lds[j] = 0;
     __syncthreads();
     for (int k = r0 + threadIdx.x; k < r1; k += blockDim.x) {
-        const int i = T.idx[n.seg_start + k];
+        const int i = rows[k];
         const unsigned char* row = S.bins_row + (size_t)i * S.fp;
         const unsigned long long p = T.pk[i];
         const unsigned long long q = hv ? T.qh[i] : 0ull;
@@ -969,11 +1087,6 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
     const int rpb = 4096;
     const int blocks = std::max(1, (count + rpb - 1) / rpb);
     const size_t lds = (size_t)n_pairs * NBIN * 8 * (hconst ? 1 : 2);
-    static bool lds_attr = false;
-    if (!lds_attr) {
-        (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        lds_attr = true;
-    }
     hipLaunchKernelGGL(k_pair_hist, dim3(blocks), dim3(256), lds, st, S, tasks, task, node, pf,
                        n_pairs, ph_g, ph_c, ph_h, rpb);
     hipLaunchKernelGGL(k_pair_scan, dim3(n_pairs), dim3(64), 0, st, S, tasks, task, ph_g, ph_c,
@@ -981,42 +1094,58 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
 }
 
 // =================================================================== tree end
-// Leaf id of every sampled row from the final partition (all-rows samples), or by walking the
-// tree over the bins (GOSS samples) — reference apply_all (tree.cpp:43-55).
-__global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
-    __shared__ int st[MAX_NODES], cn[MAX_NODES], id[MAX_NODES];
+// Leaves sorted by the start of their row range (one block per task).
+__global__ __launch_bounds__(256) void k_leaf_table(Task* tasks) {
+    Task& T = tasks[blockIdx.x];
     __shared__ int nl;
+    __shared__ int st[MAX_NODES];  // segment start of non-empty leaves, -1 otherwise
     if (threadIdx.x == 0) nl = 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < T.n_nodes; k += blockDim.x) {
-        const DNode& n = T.nodes[k];
-        if (n.v[N_LEAF] && n.seg_count > 0) {
-            // rank by segment start (starts of non-empty leaves are distinct)
-            int r = 0;
-            for (int j = 0; j < T.n_nodes; ++j) {
-                const DNode& m = T.nodes[j];
-                if (m.v[N_LEAF] && m.seg_count > 0 && m.seg_start < n.seg_start) ++r;
-            }
-            st[r] = n.seg_start;
-            cn[r] = n.seg_count;
-            id[r] = k;
-            atomicAdd(&nl, 1);
+    const int nn = T.n_nodes;
+    for (int k = threadIdx.x; k < MAX_NODES; k += blockDim.x) {
+        st[k] = -1;
+        if (k < nn) {
+            const DNode& n = T.nodes[k];
+            if (n.v[N_LEAF] && n.seg_count > 0) st[k] = n.seg_start;
         }
     }
     __syncthreads();
-    const int L = nl;
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        const int s0 = st[k];
+        if (s0 < 0) continue;
+        int r = 0;  // starts of non-empty leaves are distinct
+        for (int j = 0; j < nn; ++j) r += (st[j] >= 0 && st[j] < s0);
+        const DNode& n = T.nodes[k];
+        T.leaf_tab[4 * r] = s0;
+        T.leaf_tab[4 * r + 1] = n.seg_count;
+        T.leaf_tab[4 * r + 2] = k;
+        T.leaf_tab[4 * r + 3] = n.buf;
+        atomicAdd(&nl, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) T.n_leaves_tab = nl;
+}
+
+// Leaf id of every sampled row from the final partition (all-rows samples) — reference
+// apply_all (tree.cpp:43-55) restricted to the rows the tree was grown on.
+__global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
+    Task& T = tasks[blockIdx.y];
+    __shared__ int tab[MAX_NODES * 4];
+    const int L = T.n_leaves_tab;
+    for (int k = threadIdx.x; k < 4 * L; k += blockDim.x) tab[k] = T.leaf_tab[k];
+    __syncthreads();
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < T.n_sample; k += gridDim.x * blockDim.x) {
         int lo = 0, hi = L - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
-            if (st[mid] <= k) lo = mid;
+            if (tab[4 * mid] <= k) lo = mid;
             else hi = mid - 1;
         }
-        T.nid[T.idx[k]] = id[lo];
+        const int* rows = tab[4 * lo + 3] ? T.idx_tmp : T.idx;
+        T.nid[rows[k]] = tab[4 * lo + 2];
     }
 }
 
+// Tree walk over the bins for every row (GOSS samples).
 __global__ __launch_bounds__(256) void k_nid_walk(Shared S, Task* tasks) {
     Task& T = tasks[blockIdx.y];
     __shared__ int nf[MAX_NODES], nf2[MAX_NODES], nt[MAX_NODES], nlft[MAX_NODES], nrgt[MAX_NODES], nflag[MAX_NODES];
@@ -1053,7 +1182,6 @@ __global__ __launch_bounds__(256) void k_nid_walk(Shared S, Task* tasks) {
 // TORB sweep sums per (sweep segment, node) over ALL rows (booster.cpp:589-637): rows are time
 // sorted, so segment s is the row range [cut_{s-1}, cut_s) (s = n_buckets: rows past the last
 // cut, which only enter the full-train refit).  Each block folds a run of one segment into LDS.
-
 __global__ __launch_bounds__(256) void k_torb_accum(Task* tasks, const TorbChunk* const* chunks,
                                                     const int* n_chunks) {
     Task& T = tasks[blockIdx.y];
@@ -1084,40 +1212,58 @@ __global__ __launch_bounds__(256) void k_torb_accum(Task* tasks, const TorbChunk
     }
 }
 
-// Leaf refit and TORB prefix values (booster.cpp:591-648); one thread per node.
-__global__ __launch_bounds__(256) void k_torb_final(Task* tasks, GrowP P, double min_cover, int iid) {
-    Task& T = tasks[blockIdx.x];
+// Leaf refit and TORB prefix values (booster.cpp:591-648).  One wave per node (block x = node):
+// lanes hold sweep segments, an exact int64 wave scan gives the prefix sums at every bucket cut.
+__global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double min_cover, int iid) {
+    const Task T = tasks[blockIdx.y];
     const int nn = T.n_nodes;
-    const int k = threadIdx.x;
-    if (k >= nn) return;
     const int B = T.n_buckets;
+    const int lane = threadIdx.x & 63;
     const double inv = ldexp(1.0, -T.s_r), invh = ldexp(1.0, -T.s_rh);
-    long long fg = 0, fh = 0, fc = 0;
-    for (int s = 0; s <= B; ++s) {
-        const size_t o = (size_t)s * MAX_NODES + k;
-        fg += T.tg[o];
-        fh += T.hconst ? 0 : T.th[o];
-        fc += T.tc[o];
-    }
-    const double Hf = T.hconst ? (double)fc : (double)fh * invh;
-    const double full = leaf_value((double)fg * inv, Hf, P.l2, P.l1, P.mds);
-    T.values[k] = full;
-    T.cover[k] = (int)fc;
-    if (iid) {
-        for (int b = 0; b < B; ++b) T.prefix[(size_t)b * nn + k] = full;
-        return;
-    }
-    long long pg = 0, ph = 0, pc = 0;
-    for (int b = 0; b < B; ++b) {
-        const size_t o = (size_t)b * MAX_NODES + k;
-        pg += T.tg[o];
-        ph += T.hconst ? 0 : T.th[o];
-        pc += T.tc[o];
-        const double cover = (double)T.cuts[b];
-        const double alpha = cover <= 0.0 ? 0.0 : fmin(1.0, cover / fmax(min_cover, 1.0));
-        const double Hp = T.hconst ? (double)pc : (double)ph * invh;
-        const double pref = cover > 0.0 ? leaf_value((double)pg * inv, Hp, P.l2, P.l1, P.mds) : full;
-        T.prefix[(size_t)b * nn + k] = alpha * pref + (1.0 - alpha) * full;
+    for (int k = blockIdx.x; k < nn; k += gridDim.x) {
+        // full sums over segments 0..B
+        long long fg = 0, fh = 0, fc = 0;
+        for (int s = lane; s <= B; s += 64) {
+            const size_t o = (size_t)s * MAX_NODES + k;
+            fg += T.tg[o];
+            fh += T.hconst ? 0 : T.th[o];
+            fc += T.tc[o];
+        }
+        fg = wave_sum_ll(fg);
+        fh = wave_sum_ll(fh);
+        fc = wave_sum_ll(fc);
+        const double Hf = T.hconst ? (double)fc : (double)fh * invh;
+        const double full = leaf_value((double)fg * inv, Hf, P.l2, P.l1, P.mds);
+        if (lane == 0) {
+            T.values[k] = full;
+            T.cover[k] = (int)fc;
+        }
+        if (iid) {
+            for (int b = lane; b < B; b += 64) T.prefix[(size_t)b * nn + k] = full;
+            continue;
+        }
+        long long cg = 0, ch = 0, cc = 0;  // carry of the prefix across 64-segment chunks
+        for (int s0 = 0; s0 < B; s0 += 64) {
+            const int b = s0 + lane;
+            long long xg = 0, xh = 0, xc = 0;
+            if (b < B) {
+                const size_t o = (size_t)b * MAX_NODES + k;
+                xg = T.tg[o];
+                xh = T.hconst ? 0 : T.th[o];
+                xc = T.tc[o];
+            }
+            const long long pg = cg + wave_incl_ll(xg), ph = ch + wave_incl_ll(xh), pc = cc + wave_incl_ll(xc);
+            if (b < B) {
+                const double cover = (double)T.cuts[b];
+                const double alpha = cover <= 0.0 ? 0.0 : fmin(1.0, cover / fmax(min_cover, 1.0));
+                const double Hp = T.hconst ? (double)pc : (double)ph * invh;
+                const double pref = cover > 0.0 ? leaf_value((double)pg * inv, Hp, P.l2, P.l1, P.mds) : full;
+                T.prefix[(size_t)b * nn + k] = alpha * pref + (1.0 - alpha) * full;
+            }
+            cg = __shfl(pg, 63, 64);
+            ch = __shfl(ph, 63, 64);
+            cc = __shfl(pc, 63, 64);
+        }
     }
 }
 
@@ -1141,28 +1287,29 @@ __global__ void k_zero_torb(Task* tasks, int words) {
 void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
                      int max_chunks, const GrowP& P, double min_cover, int iid, double lr) {
-    const int rb = std::max(1, std::min(4096, (max_rows + 255) / 256));
-    if (walk) hipLaunchKernelGGL(k_nid_walk, dim3(rb, n_tasks), dim3(256), 0, st, S, tasks);
-    else hipLaunchKernelGGL(k_nid_segments, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
+    const int rb = std::max(1, std::min(2048, (max_rows + 1023) / 1024));
+    if (walk) {
+        hipLaunchKernelGGL(k_nid_walk, dim3(rb, n_tasks), dim3(256), 0, st, S, tasks);
+    } else {
+        hipLaunchKernelGGL(k_leaf_table, dim3(n_tasks), dim3(256), 0, st, tasks);
+        hipLaunchKernelGGL(k_nid_segments, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
+    }
     const int words = (MAX_BUCKETS + 1) * MAX_NODES;
-    hipLaunchKernelGGL(k_zero_torb, dim3((words + 255) / 256, n_tasks), dim3(256), 0, st, tasks, words);
+    hipLaunchKernelGGL(k_zero_torb, dim3((words + 1023) / 1024, n_tasks), dim3(256), 0, st, tasks, words);
     hipLaunchKernelGGL(k_torb_accum, dim3(std::max(1, max_chunks), n_tasks), dim3(256), 0, st, tasks,
                        chunks, n_chunks);
-    hipLaunchKernelGGL(k_torb_final, dim3(n_tasks), dim3(256), 0, st, tasks, P, min_cover, iid);
+    hipLaunchKernelGGL(k_torb_final, dim3(MAX_NODES, n_tasks), dim3(64), 0, st, tasks, P, min_cover, iid);
     hipLaunchKernelGGL(k_pred_update, dim3(rb, n_tasks), dim3(256), 0, st, tasks, lr);
 }
 
-void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows) {
-    const int rb = std::max(1, std::min(4096, (max_rows + 255) / 256));
-    hipLaunchKernelGGL(k_tree_init, dim3(n_tasks), dim3(64), 0, st, tasks, S.n_features);
-    hipLaunchKernelGGL(k_pack, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
-    hipLaunchKernelGGL(k_iota, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
-}
-
-void launch_zero_slots(hipStream_t st, Task* tasks, int n_tasks, int F, int first, int count) {
-    const size_t total = (size_t)F * NBIN * count;
-    const int blocks = (int)std::min<size_t>(1024, (total + 255) / 256);
-    hipLaunchKernelGGL(k_zero_slots, dim3(blocks, n_tasks), dim3(256), 0, st, tasks, F, first, count);
+// Dynamic LDS above 64 KiB needs a per-kernel opt-in; done once, outside any stream capture.
+void init_kernel_attributes() {
+    static bool done = false;
+    if (done) return;
+    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    done = true;
 }
 
 }  // namespace tgbm

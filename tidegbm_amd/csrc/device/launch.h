@@ -6,17 +6,20 @@
 
 namespace tgbm {
 
+void init_kernel_attributes();
 void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edges,
                 const int* n_edges, unsigned char* bins_row, int fp, unsigned char* bins_col,
                 int n_pad);
 void launch_grad(hipStream_t st, Task* tasks, int n_tasks, int objective, int max_rows,
                  int max_groups);
-void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows);
-void launch_zero_slots(hipStream_t st, Task* tasks, int n_tasks, int F, int first, int count);
+void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows,
+                       int max_nodes);
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant);
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
-void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P, int mode);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
+                 int fuse_select);
+void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P);
+void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P);
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count);
 void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count);
 void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int node, int count,

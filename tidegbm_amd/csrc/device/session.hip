@@ -6,6 +6,7 @@
 #include <cmath>
 #include <memory>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -83,6 +84,11 @@ struct tgbm_session {
     int* ph_c = nullptr;
     long long* ph_h = nullptr;
     FCand* pair_out_d = nullptr;
+    // captured device-driven split loop (one graph launch per tree)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key = -1;   // sample size / leaf budget / kernel variant the graph was captured for
+    bool use_graphs = true;
     // timers
     bool timing = false;
     std::vector<EventPair> events;
@@ -98,6 +104,8 @@ struct tgbm_session {
     }
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
         for (auto& e : events) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
@@ -216,6 +224,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->max_leaves = max_leaves;
         s->max_nodes = 2 * max_leaves - 1;
         s->hconst = h_const != 0;
+        if (const char* e = std::getenv("TGBM_NO_GRAPH")) s->use_graphs = !(e[0] == '1');
+        init_kernel_attributes();
         HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->bins_row = s->alloc<unsigned char>((size_t)n_rows * s->fp);
         s->bins_col = s->alloc<unsigned char>((size_t)n_features * s->n_pad);
@@ -254,7 +264,9 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.nid = s->alloc<int>(m);
             T.idx = s->alloc<int>(m);
             T.idx_tmp = s->alloc<int>(m);
-            T.chunk_cnt = s->alloc<int>(m / 4096 + 2);
+            T.chunk_cnt = s->alloc<int>(m / 2048 + 2);
+            T.red = s->alloc<unsigned long long>(2 * MAX_RED_BLOCKS);
+            T.leaf_tab = s->alloc<int>(4 * MAX_NODES);
             T.nodes = s->alloc<DNode>(MAX_NODES);
             T.cands = s->alloc<DCand>(MAX_NODES);
             T.fcand = s->alloc<FCand>((size_t)MAX_NODES * n_features);
@@ -361,7 +373,7 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
         HIP_OK(hipMemcpy((void*)T.cuts, cut.data(), sizeof(int) * n_buckets, hipMemcpyHostToDevice));
         // sweep-segment chunk list: segment s covers [cut_{s-1}, cut_s), s = n_buckets past the last
         H.chunks.clear();
-        const int CH = 8192;
+        const int CH = 2048;
         int prev = 0;
         for (int sg = 0; sg <= n_buckets; ++sg) {
             const int end = sg < n_buckets ? std::min(std::max(cut[sg], prev), m) : m;
@@ -489,11 +501,11 @@ int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sam
     });
 }
 
-// Root of every task's tree: quantize, build and scan the root histogram.
-static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy) {
+// Root of every task's tree: quantize, zero the node histograms, build and scan the root.  In
+// device-driven mode the scan's last block also selects the first split.
+static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     const GrowP P = s->growp(p);
-    launch_tree_start(s->stream, s->S, s->tasks_d, s->T, s->max_rows());
-    launch_zero_slots(s->stream, s->tasks_d, s->T, s->F, 0, 1);
+    launch_tree_start(s->stream, s->S, s->tasks_d, s->T, s->max_rows(), s->max_nodes);
     if (energy)
         for (int t = 0; t < s->T; ++t) HIP_OK(hipMemsetAsync(s->tasks_h[t].energy, 0, sizeof(long long) * MAX_NODES, s->stream));
     const int ms = s->max_sample();
@@ -508,17 +520,19 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
     }
     launch_hist(s->stream, s->S, s->tasks_d, s->T, true, s->all_contiguous(), s->hconst, ms, p->hist_mode);
     if (s->timing) s->toc(ep);
-    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 0);
+    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 0, 0);
+    if (device_mode) launch_select(s->stream, s->tasks_d, s->T, s->F, P);
     if (energy)
         for (int t = 0; t < s->T; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
     HIP_OK(hipGetLastError());
 }
 
-// One split step of every task (device-decided when mode == 0).
+// One split step of every task: partition the decided leaf, build the smaller child, scan both
+// children (device mode: the scan's last block selects the next split; host mode: the host
+// applied its own decision first).
 static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
     const int ms = s->max_sample();
-    launch_select(s->stream, s->tasks_d, s->T, s->F, P, mode);
-    launch_zero_slots(s->stream, s->tasks_d, s->T, s->F, -1, 2);
+    if (mode == 1) launch_apply(s->stream, s->tasks_d, s->T, s->F, P);
     EventPair e1, e2;
     if (s->timing) s->tic(1, 0.0, e1);
     launch_partition(s->stream, s->S, s->tasks_d, s->T, ms);
@@ -526,18 +540,37 @@ static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_gro
     if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
     launch_hist(s->stream, s->S, s->tasks_d, s->T, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
     if (s->timing) s->toc(e2);
-    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 1);
+    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 1, 0);
+    if (mode == 0) launch_select(s->stream, s->tasks_d, s->T, s->F, P);
     HIP_OK(hipGetLastError());
 }
 
 int tgbm_tree_begin(tgbm_session* s, const tgbm_grow_params* p) {
-    return dev_guard([&] { tree_begin_impl(s, p, true); });
+    return dev_guard([&] { tree_begin_impl(s, p, p->host_driven != 0, p->host_driven == 0); });
 }
 
 int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
     return dev_guard([&] {
         const GrowP P = s->growp(p);
-        for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
+        if (!s->use_graphs || s->timing) {
+            for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
+            return;
+        }
+        // The loop's kernels read every decision from device memory, so one capture serves
+        // every tree with the same sample size and leaf budget.
+        const int key = s->max_sample() * 1024 + p->max_leaves + 1000000 * p->hist_mode;
+        if (!s->graph_exec || s->graph_key != key) {
+            if (s->graph_exec) HIP_OK(hipGraphExecDestroy(s->graph_exec));
+            if (s->graph) HIP_OK(hipGraphDestroy(s->graph));
+            s->graph_exec = nullptr;
+            s->graph = nullptr;
+            HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
+            HIP_OK(hipStreamEndCapture(s->stream, &s->graph));
+            HIP_OK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+            s->graph_key = key;
+        }
+        HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
     });
 }
 

@@ -103,6 +103,7 @@ tgbm_grow_params grow_params(const Config& c) {
     p.min_split_gain = c.min_split_gain;
     p.pair_l2 = c.pair_l2;
     p.hist_mode = c.hist_mode;
+    p.host_driven = (c.pair_splits || c.force_host_driver) ? 1 : 0;
     return p;
 }
 
