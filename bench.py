@@ -63,14 +63,9 @@ def cpu_baseline(p, params, cpu_trees):
             "setup_s": round(times[1] - per_tree, 3)}
 
 
-def run_plain(args):
+def run_plain(args, p):
     import tidegbm_amd as t
-    from tidegbm_amd import synth
 
-    if t.device_count() < 1:
-        raise SystemExit("bench.py needs a HIP device")
-    log("[bench] generating the 1M x 128 panel")
-    p = synth.panel_large(1)
     W, K = args.warmup, args.steps
     params = dict(max_depth=8, learning_rate=0.05, max_bin=255, goss=0)
     b = t.Booster(n_estimators=W + K, **params)
@@ -142,6 +137,58 @@ def run_plain(args):
     return out
 
 
+def copy_probe(mib):
+    """A plain device copy of known size: the reference point that scales rocprofv3's FETCH_SIZE
+    and WRITE_SIZE counters to bytes (each dispatch reads and writes exactly mib MiB)."""
+    import torch
+    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_mib": mib, "dispatches": 5}), flush=True)
+
+
+def run_walkforward(args, p):
+    """64 walk-forward fits (16 expanding folds x horizons 1, 3, 5, 10) over the shared binned
+    1M x 128 matrix: batched (every kernel launch covers all fits) against the same fits run one
+    after another over the same resident bins.  Both schedules must give identical trees."""
+    import numpy as np
+    import tidegbm_amd as t
+    from tidegbm_amd import synth
+
+    tasks = synth.walk_forward_tasks(p, n_folds=16, horizons=(1, 3, 5, 10), first_dates=250)
+    params = dict(n_estimators=args.wf_trees, max_depth=6, learning_rate=0.05, max_bin=255, goss=0,
+                  pair_splits=0)
+    kw = dict(entity=p.entity, horizons=[k.horizon for k in tasks], **params)
+    rows = [k.n_train for k in tasks]
+    ys = [k.y for k in tasks]
+    log(f"[walk-forward] {len(tasks)} fits, rows {min(rows)}..{max(rows)}, {args.wf_trees} trees each: batched")
+    mb = t.fit_batch(p.X, p.ts, rows, ys, schedule="batched", **kw)
+    sb, bb = mb[0].fit_times()
+    log(f"[walk-forward] batched boost {bb:.3f}s (setup {sb:.3f}s); sequential")
+    ms = t.fit_batch(p.X, p.ts, rows, ys, schedule="sequential", **kw)
+    ss, bs = ms[0].fit_times()
+    log(f"[walk-forward] sequential boost {bs:.3f}s")
+    same = True
+    for a, b in zip(mb, ms):
+        for x, y in zip(a.trees(), b.trees()):
+            same &= all(np.array_equal(x[k], y[k]) for k in ("ints", "dbl", "values", "prefix"))
+    if not same:
+        raise SystemExit("batched and sequential walk-forward fits differ")
+    n_trees = len(tasks) * args.wf_trees
+    return {
+        "workload": "64 walk-forward fits of the 1M x 128 panel: 16 expanding folds (train 250..2500 "
+                    "dates minus the purge gap) x horizons 1,3,5,10; mse, depth 6, max_leaves 31, lr 0.05, "
+                    f"{args.wf_trees} trees each, goss=0, pair_splits=0, shared bins",
+        "fits": len(tasks), "rows_min": min(rows), "rows_max": max(rows), "rows_total": int(sum(rows)),
+        "batched_s": round(bb, 3), "sequential_s": round(bs, 3),
+        "speedup": round(bs / bb, 3),
+        "batched_trees_per_s": round(n_trees / bb, 1), "sequential_trees_per_s": round(n_trees / bs, 1),
+        "setup_s": round(sb, 3), "identical_trees": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,14 +197,32 @@ def main():
     ap.add_argument("--cpu-trees", type=int, default=6)
     ap.add_argument("--profile-trees", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["all", "plain", "walkforward"], default="all")
+    ap.add_argument("--wf-trees", type=int, default=100, help="trees per walk-forward fit")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the fit shards by row "
               "ranges across GPUs only in design (see DESIGN.md), not in this build.", file=sys.stderr)
         raise SystemExit(2)
     if args.warmup < 1:
         args.warmup = 1
-    out = run_plain(args)
+    import tidegbm_amd as t
+    from tidegbm_amd import synth
+    if t.device_count() < 1:
+        raise SystemExit("bench.py needs a HIP device")
+    log("[bench] generating the 1M x 128 panel")
+    p = synth.panel_large(1)
+    if args.workload == "walkforward":
+        print(json.dumps({"walk_forward": run_walkforward(args, p)}), flush=True)
+        return
+    out = run_plain(args, p)
+    if args.workload == "all":
+        out["walk_forward"] = run_walkforward(args, p)
     print(json.dumps(out), flush=True)
 
 

@@ -65,6 +65,11 @@ void tgbm_session_destroy(tgbm_session* s);
  * with per-feature right edges edges[f*256 + k], k < n_edges[f] (reference
  * Binner::transform, binning.cpp:104-128): NaN -> 0, else 1 + #edges <= x. */
 int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges);
+/* Restrict every later gradient, tree_begin/grow/end launch to tasks [first, first + count)
+ * (all tasks after create).  Batched fits keep every task active; the one-fit-at-a-time
+ * schedule activates one task at a time over the same resident bins.  The results of a task
+ * do not depend on which other tasks share its launches. */
+int tgbm_session_set_active(tgbm_session* s, int first, int count);
 /* Row-major uint8 bins back to the host (n_rows x n_features), for checking. */
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out);
 
@@ -97,9 +102,9 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
 int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d,
                      const tgbm_grow_params* p, int* left_id);
 
-/* Finish every task's tree: leaf ids for all rows, leaf refit and TORB prefix values
- * (booster.cpp:589-648), pred += lr * prefix (booster.cpp:655-664).  Writes the tree of task
- * t into the caller's buffers at offset t*max_nodes. */
+/* Finish every active task's tree: leaf ids for all rows, leaf refit and TORB prefix values
+ * (booster.cpp:589-648), pred += lr * prefix (booster.cpp:655-664).  Writes the tree of the
+ * i-th active task into the caller's buffers at offset i*max_nodes (prefix: i*max_nodes*256). */
 int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int iid,
                   const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
                   double* values, int* cover, double* prefix);

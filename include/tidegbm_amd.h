@@ -76,6 +76,21 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s);
  * tree ends with a device synchronization.  Returns the number of trees. */
 int tide_tree_times(TideHandle h, double* out, int cap);
 
+/* Batched walk-forward fits (no counterpart in the reference C API; the reference refits every
+ * fold x horizon of cv.py:45-53 with its own Booster::fit).  n_tasks fits share one binned
+ * matrix: rows of X (n x n_features, row-major) must be sorted by (timestamps, entity); task t
+ * trains on rows [0, task_rows[t]) with target y[t] (task_rows[t] values) and horizon
+ * horizons[t] (NULL: the proto's horizon), every other setting taken from proto.  Bin edges
+ * are fitted once over the largest task's rows.  schedule 0 grows all tasks' trees together
+ * (one launch per kernel covers every task); schedule 1 fits the tasks one after another on the
+ * same resident bins.  Both give identical trees.  out[t] receives a new fitted handle (free
+ * each with tide_free).  Requires pair_splits=0 and goss off (goss=0, or every task under the
+ * 250k-row GOSS threshold with goss=-1). */
+int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features,
+                   const long long* timestamps, const long long* entity, int n_tasks,
+                   const int* task_rows, const double* const* y, const long long* horizons,
+                   int schedule, TideHandle* out);
+
 #ifdef __cplusplus
 }
 #endif

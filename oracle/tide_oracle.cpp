@@ -498,6 +498,9 @@ struct Model {
     std::vector<double> init_prefix;
     std::vector<Tree> trees;
     bool fitted = false;
+    // Caller-supplied bin edges replace Binner::fit (checker mode for fits that share a binned
+    // matrix, e.g. walk-forward folds binned once); empty = fit edges as the reference does.
+    std::vector<std::vector<double>> fixed_edges;
 
     void bins_of(const double* X, int n, std::vector<uint16_t>& xb) const {
         xb.resize((size_t)n * F);
@@ -595,7 +598,12 @@ struct Model {
             }
         }
         // bins (booster.cpp:378-390)
-        fit_edges(X.data(), n, F, c.max_bin, c.bin_sample, (uint32_t)c.random_state, edges);
+        if (!fixed_edges.empty()) {
+            if ((int)fixed_edges.size() != F) throw std::runtime_error("fixed edges: feature count mismatch");
+            edges = fixed_edges;
+        } else {
+            fit_edges(X.data(), n, F, c.max_bin, c.bin_sample, (uint32_t)c.random_state, edges);
+        }
         nb = total_bins(edges, c.max_bin);
         std::vector<uint16_t> xb;
         bins_of(X.data(), n, xb);
@@ -821,6 +829,18 @@ int orc_fit(void* hd, const double* X, const double* y, int n, int F, const long
         m->cfg.iid = iid != 0;
         if (!m->cfg.iid) m->cfg.horizon = horizon;
         m->fit(X, y, n, F, ts, w, entity, rank_groups);
+    });
+}
+
+// Fix the bin edges of later fits (edges[f*256 + k], k < n_edges[f]); n_features = 0 clears.
+int orc_set_edges(void* hd, const double* edges, const int* n_edges, int n_features) {
+    Model* m = static_cast<Model*>(hd);
+    return orc_guard([&] {
+        m->fixed_edges.assign(n_features, {});
+        for (int f = 0; f < n_features; ++f) {
+            if (n_edges[f] < 0 || n_edges[f] > 255) throw std::runtime_error("edge count out of range");
+            m->fixed_edges[f].assign(edges + (size_t)f * 256, edges + (size_t)f * 256 + n_edges[f]);
+        }
     });
 }
 

@@ -52,6 +52,18 @@ class CpuModel:
             raise ValueError(self.lib.error())
         return self
 
+    def set_edges(self, edges):
+        """Oracle only: fix the bin edges of later fits (list of per-feature edge arrays)."""
+        F = len(edges)
+        E = np.zeros((F, 256))
+        ne = np.zeros(F, np.int32)
+        for f, e in enumerate(edges):
+            E[f, :len(e)] = e
+            ne[f] = len(e)
+        if self.lib.lib.orc_set_edges(self.h, _ptr(E), _ptr(ne), F) != 0:
+            raise ValueError(self.lib.error())
+        return self
+
     def update(self, X, y, extra, w=None):
         X = np.ascontiguousarray(X, np.float64)
         y = np.ascontiguousarray(y, np.float64)
@@ -135,6 +147,7 @@ def oracle() -> CpuLib:
     L.orc_hist.argtypes = [_P, _I, _P, _I, _P, _P, _I, _P, _P, _P]
     L.orc_scan.argtypes = [_P, _P, _P, _I, _I, _D, _D, _I, _D, _D, _P, _P, _P, _P]
     L.orc_grad.argtypes = [_I, _P, _P, _P, _P, _I, _P, _P]
+    L.orc_set_edges.argtypes = [_P, _P, _P, _I]
     return lib
 
 
@@ -179,5 +192,6 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True):
                     d = np.max(np.abs(x[key] - y[key]))
                     return f"tree {t}: {key} differ (max abs {d:g})"
             if not np.allclose(x["dbl"], y["dbl"], rtol=rtol, atol=atol):
-                return f"tree {t}: node value/gain differ"
+                d = np.max(np.abs(x["dbl"] - y["dbl"]))
+                return f"tree {t}: node value/gain differ (max abs {d:g})"
     return None

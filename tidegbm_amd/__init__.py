@@ -65,6 +65,7 @@ _sig("tide_get_init_prefix", _I, _P, _P, _I)
 _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
 _sig("tide_tree_times", _I, _P, _P, _I)
+_sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
 
 #: True: the gfx950 kernels are compiled into the loaded library (reference: tidegbm.cuda_compiled).
 gpu_compiled: bool = bool(_lib.tide_device_compiled())
@@ -226,6 +227,43 @@ class Booster:
         a, b = _D(), _D()
         _lib.tide_fit_times(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+
+def fit_batch(X, timestamps, task_rows, targets, *, entity=None, horizons=None,
+              schedule: str = "batched", **params):
+    """Walk-forward fits over row prefixes of one shared binned matrix.
+
+    Rows of X must be sorted by (timestamps, entity); task t trains on rows [0, task_rows[t])
+    with target targets[t] and horizon horizons[t].  schedule "batched" grows every task's trees
+    together; "sequential" fits the tasks one after another on the same resident bins (the
+    plain one-fit-per-call pattern).  Returns one fitted Booster per task.
+    """
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, F = X.shape
+    ts = None if timestamps is None else np.ascontiguousarray(timestamps, dtype=np.int64)
+    ent = None if entity is None else np.ascontiguousarray(entity, dtype=np.int64)
+    rows = np.ascontiguousarray(task_rows, dtype=np.int32)
+    T = len(rows)
+    ys = [np.ascontiguousarray(y, dtype=np.float64) for y in targets]
+    if len(ys) != T:
+        raise TideError("one target per task is required")
+    for t in range(T):
+        if ys[t].shape[0] != rows[t]:
+            raise TideError(f"target {t} has {ys[t].shape[0]} values for {rows[t]} rows")
+    yp = (ctypes.c_void_p * T)(*[y.ctypes.data for y in ys])
+    hz = None if horizons is None else np.ascontiguousarray(horizons, dtype=np.int64)
+    if hz is not None and hz.shape[0] != T:
+        raise TideError("one horizon per task is required")
+    sched = {"batched": 0, "sequential": 1}.get(schedule)
+    if sched is None:
+        raise TideError(f"unknown schedule {schedule!r}")
+    proto = Booster(**params)
+    out = (_P * T)()
+    rc = _lib.tide_fit_batch(proto._h, _ptr(X), n, F, _ptr(ts), _ptr(ent), T, _ptr(rows), yp, _ptr(hz),
+                             sched, out)
+    if rc != 0:
+        raise TideError(_err())
+    return [Booster(handle=_P(out[t])) for t in range(T)]
 
 
 def tree_arrays(h, info_fn, get_fn, t):

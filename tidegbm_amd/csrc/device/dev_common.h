@@ -66,6 +66,21 @@ struct DCand {      // cached leaf candidate
     int feature, feature2, thr, type, mgl, valid;
 };
 
+// Byte offsets of one task's finished tree in the export staging buffer.
+struct ExportLayout {
+    size_t off_nodes, off_values, off_cover, off_prefix, stride;
+};
+__host__ inline ExportLayout export_layout(int max_nodes, int max_buckets) {
+    ExportLayout L;
+    auto up = [](size_t x, size_t a) { return (x + a - 1) / a * a; };
+    L.off_nodes = 16;
+    L.off_values = up(L.off_nodes + sizeof(DNode) * max_nodes, 8);
+    L.off_cover = L.off_values + sizeof(double) * max_nodes;
+    L.off_prefix = up(L.off_cover + sizeof(int) * max_nodes, 8);
+    L.stride = up(L.off_prefix + sizeof(double) * max_nodes * max_buckets, 256);
+    return L;
+}
+
 // Per-task state and pointers, resident in device memory (one entry per task).
 struct Task {
     // ---- fixed at creation

@@ -1302,6 +1302,33 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
     hipLaunchKernelGGL(k_pred_update, dim3(rb, n_tasks), dim3(256), 0, st, tasks, lr);
 }
 
+// Finished trees of all active tasks packed into one staging buffer (one copy to the host):
+// task i at out + i*L.stride holds n_nodes, nodes[0..nn), values[0..nn), cover[0..nn) and
+// prefix[0..nn*n_buckets) at the offsets of L.
+__global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, char* out) {
+    const Task& T = tasks[blockIdx.x];
+    char* base = out + (size_t)blockIdx.x * L.stride;
+    const int nn = T.n_nodes;
+    if (threadIdx.x == 0) *(int*)base = nn;
+    const int* src = (const int*)T.nodes;
+    int* dst = (int*)(base + L.off_nodes);
+    const int words = nn * (int)(sizeof(DNode) / sizeof(int));
+    for (int j = threadIdx.x; j < words; j += blockDim.x) dst[j] = src[j];
+    double* vals = (double*)(base + L.off_values);
+    int* cov = (int*)(base + L.off_cover);
+    for (int j = threadIdx.x; j < nn; j += blockDim.x) {
+        vals[j] = T.values[j];
+        cov[j] = T.cover[j];
+    }
+    double* pre = (double*)(base + L.off_prefix);
+    const int np = nn * T.n_buckets;
+    for (int j = threadIdx.x; j < np; j += blockDim.x) pre[j] = T.prefix[j];
+}
+
+void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out) {
+    hipLaunchKernelGGL(k_export, dim3(n_tasks), dim3(256), 0, st, tasks, L, out);
+}
+
 // Dynamic LDS above 64 KiB needs a per-kernel opt-in; done once, outside any stream capture.
 void init_kernel_attributes() {
     static bool done = false;

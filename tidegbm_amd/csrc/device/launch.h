@@ -28,5 +28,6 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
 void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
                      int max_chunks, const GrowP& P, double min_cover, int iid, double lr);
+void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out);
 
 }  // namespace tgbm

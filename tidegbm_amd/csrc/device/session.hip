@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -65,6 +66,7 @@ struct EventPair {
 
 struct tgbm_session {
     int n = 0, F = 0, fp = 0, n_pad = 0, nb = 255, T = 1, max_leaves = 31, max_nodes = 61;
+    int a0 = 0, an = 1;     // active tasks [a0, a0 + an): every fit launch covers exactly these
     bool hconst = true;
     hipStream_t stream = nullptr;
     Shared S{};
@@ -84,11 +86,14 @@ struct tgbm_session {
     int* ph_c = nullptr;
     long long* ph_h = nullptr;
     FCand* pair_out_d = nullptr;
-    // captured device-driven split loop (one graph launch per tree)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_key = -1;   // sample size / leaf budget / kernel variant the graph was captured for
+    // captured device-driven split loops (one graph launch per tree), keyed by what the capture
+    // baked in: largest sample, leaf budget, histogram variant and the active task range
+    std::map<std::vector<long long>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     bool use_graphs = true;
+    // finished-tree staging (device + pinned host), sized for every task at the largest bucket count
+    char* export_d = nullptr;
+    char* export_h = nullptr;
+    size_t export_bytes = 0;
     // timers
     bool timing = false;
     std::vector<EventPair> events;
@@ -104,13 +109,16 @@ struct tgbm_session {
     }
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& kv : graphs) {
+            (void)hipGraphExecDestroy(kv.second.second);
+            (void)hipGraphDestroy(kv.second.first);
+        }
         for (auto& e : events) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
         for (void* p : allocs) (void)hipFree(p);
+        if (export_h) (void)hipHostFree(export_h);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void set_field_int(int t, size_t off, int v) {
@@ -144,19 +152,20 @@ struct tgbm_session {
         g.pair_l2 = p->pair_l2;
         return g;
     }
+    Task* td() const { return tasks_d + a0; }   // first active task (device)
     int max_rows() const {
         int m = 0;
-        for (auto& t : th) m = std::max(m, t.n_rows);
+        for (int t = a0; t < a0 + an; ++t) m = std::max(m, th[t].n_rows);
         return m;
     }
     int max_sample() const {
         int m = 0;
-        for (auto& t : th) m = std::max(m, t.has_sample ? t.n_sample : t.n_rows);
+        for (int t = a0; t < a0 + an; ++t) m = std::max(m, th[t].has_sample ? th[t].n_sample : th[t].n_rows);
         return m;
     }
     bool all_contiguous() const {
-        for (auto& t : th)
-            if (t.has_sample) return false;
+        for (int t = a0; t < a0 + an; ++t)
+            if (th[t].has_sample) return false;
         return true;
     }
     // -------- timers
@@ -221,6 +230,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->n_pad = (n_rows + 63) / 64 * 64 + 64;
         s->nb = n_bins;
         s->T = n_tasks;
+        s->a0 = 0;
+        s->an = n_tasks;
         s->max_leaves = max_leaves;
         s->max_nodes = 2 * max_leaves - 1;
         s->hconst = h_const != 0;
@@ -335,6 +346,15 @@ int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, cons
     });
 }
 
+int tgbm_session_set_active(tgbm_session* s, int first, int count) {
+    return dev_guard([&] {
+        if (first < 0 || count < 1 || first + count > s->T) throw std::runtime_error("active task range out of range");
+        HIP_OK(hipStreamSynchronize(s->stream));
+        s->a0 = first;
+        s->an = count;
+    });
+}
+
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out) {
     return dev_guard([&] {
         std::vector<unsigned char> tmp((size_t)s->n * s->fp);
@@ -430,12 +450,13 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
 int tgbm_task_grad(tgbm_session* s, int task, int objective) {
     return dev_guard([&] {
         if (objective != 0 && objective != 5) throw std::runtime_error("objective not supported on the device path");
-        const int t0 = task < 0 ? 0 : task, nt = task < 0 ? s->T : 1;
+        if (task >= s->T) throw std::runtime_error("task out of range");
+        const int t0 = task < 0 ? s->a0 : task, nt = task < 0 ? s->an : 1;
         int mg = 1;
         for (int t = t0; t < t0 + nt; ++t) mg = std::max(mg, (int)std::max<size_t>(1, s->th[t].n_rows));
         EventPair ep;
         if (s->timing) s->tic(4, 0.0, ep);
-        launch_grad(s->stream, s->tasks_d + t0, nt, objective, s->max_rows(), mg);
+        launch_grad(s->stream, s->tasks_d + t0, nt, objective, mg, mg);
         if (s->timing) s->toc(ep);
         HIP_OK(hipGetLastError());
     });
@@ -505,25 +526,26 @@ int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sam
 // device-driven mode the scan's last block also selects the first split.
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     const GrowP P = s->growp(p);
-    launch_tree_start(s->stream, s->S, s->tasks_d, s->T, s->max_rows(), s->max_nodes);
+    launch_tree_start(s->stream, s->S, s->td(), s->an, s->max_rows(), s->max_nodes);
     if (energy)
-        for (int t = 0; t < s->T; ++t) HIP_OK(hipMemsetAsync(s->tasks_h[t].energy, 0, sizeof(long long) * MAX_NODES, s->stream));
+        for (int t = s->a0; t < s->a0 + s->an; ++t) HIP_OK(hipMemsetAsync(s->tasks_h[t].energy, 0, sizeof(long long) * MAX_NODES, s->stream));
     const int ms = s->max_sample();
     EventPair ep;
     if (s->timing) {
         double bytes = 0;
-        for (auto& t : s->th) {
+        for (int k = s->a0; k < s->a0 + s->an; ++k) {
+            const TaskHost& t = s->th[k];
             const double m = t.has_sample ? t.n_sample : t.n_rows;
             bytes += m * s->F + m * (s->hconst ? 8.0 : 16.0) + (t.has_sample ? 4.0 * m : 0.0) + (double)s->F * s->nb * 24.0;
         }
         s->tic(0, bytes, ep);
     }
-    launch_hist(s->stream, s->S, s->tasks_d, s->T, true, s->all_contiguous(), s->hconst, ms, p->hist_mode);
+    launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous(), s->hconst, ms, p->hist_mode);
     if (s->timing) s->toc(ep);
-    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 0, 0);
-    if (device_mode) launch_select(s->stream, s->tasks_d, s->T, s->F, P);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 0, 0);
+    if (device_mode) launch_select(s->stream, s->td(), s->an, s->F, P);
     if (energy)
-        for (int t = 0; t < s->T; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
+        for (int t = s->a0; t < s->a0 + s->an; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
     HIP_OK(hipGetLastError());
 }
 
@@ -532,16 +554,16 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
 // applied its own decision first).
 static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
     const int ms = s->max_sample();
-    if (mode == 1) launch_apply(s->stream, s->tasks_d, s->T, s->F, P);
+    if (mode == 1) launch_apply(s->stream, s->td(), s->an, s->F, P);
     EventPair e1, e2;
     if (s->timing) s->tic(1, 0.0, e1);
-    launch_partition(s->stream, s->S, s->tasks_d, s->T, ms);
+    launch_partition(s->stream, s->S, s->td(), s->an, ms);
     if (s->timing) s->toc(e1);
     if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
-    launch_hist(s->stream, s->S, s->tasks_d, s->T, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
+    launch_hist(s->stream, s->S, s->td(), s->an, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
     if (s->timing) s->toc(e2);
-    launch_scan(s->stream, s->S, s->tasks_d, s->T, P, 1, 0);
-    if (mode == 0) launch_select(s->stream, s->tasks_d, s->T, s->F, P);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 1, 0);
+    if (mode == 0) launch_select(s->stream, s->td(), s->an, s->F, P);
     HIP_OK(hipGetLastError());
 }
 
@@ -558,19 +580,29 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
         }
         // The loop's kernels read every decision from device memory, so one capture serves
         // every tree with the same sample size and leaf budget.
-        const int key = s->max_sample() * 1024 + p->max_leaves + 1000000 * p->hist_mode;
-        if (!s->graph_exec || s->graph_key != key) {
-            if (s->graph_exec) HIP_OK(hipGraphExecDestroy(s->graph_exec));
-            if (s->graph) HIP_OK(hipGraphDestroy(s->graph));
-            s->graph_exec = nullptr;
-            s->graph = nullptr;
+        const std::vector<long long> key{s->max_sample(), p->max_leaves, p->hist_mode, s->a0, s->an};
+        auto it = s->graphs.find(key);
+        if (it == s->graphs.end()) {
+            if (s->graphs.size() >= 256) {  // bound the cache; captures are cheap to redo
+                for (auto& kv : s->graphs) {
+                    HIP_OK(hipGraphExecDestroy(kv.second.second));
+                    HIP_OK(hipGraphDestroy(kv.second.first));
+                }
+                s->graphs.clear();
+            }
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
             HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-            for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
-            HIP_OK(hipStreamEndCapture(s->stream, &s->graph));
-            HIP_OK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
-            s->graph_key = key;
+            for (int k = 0; k < p->max_leaves - 1; ++k) split_step(s, P, 0, p);
+            HIP_OK(hipStreamEndCapture(s->stream, &g));
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                HIP_OK(e);
+            }
+            it = s->graphs.emplace(key, std::make_pair(g, ge)).first;
         }
-        HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
+        HIP_OK(hipGraphLaunch(it->second.second, s->stream));
     });
 }
 
@@ -636,7 +668,8 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
 int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tgbm_grow_params* p,
                      int* left_id) {
     return dev_guard([&] {
-        if (s->T != 1) throw std::runtime_error("host-driven growth needs a single-task session");
+        if (s->an != 1 || task != s->a0)
+            throw std::runtime_error("host-driven growth needs exactly one active task, the one named");
         struct {
             int leaf, type, feat, feat2, thr, mgl;
             double gain;
@@ -667,39 +700,51 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
                   double* values, int* cover, double* prefix) {
     return dev_guard([&] {
         const GrowP P = s->growp(p);
-        bool walk = false;
-        for (auto& t : s->th) walk |= t.has_sample;
+        bool walk = !s->all_contiguous();
         EventPair ep;
         if (s->timing) s->tic(3, 0.0, ep);
-        launch_tree_end(s->stream, s->S, s->tasks_d, s->T, walk, s->max_rows(),
-                        (const TorbChunk* const*)s->chunk_ptrs_d, s->n_chunks_d, s->max_chunks, P,
+        launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
+                        (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
                         min_cover, iid, learning_rate);
         if (s->timing) s->toc(ep);
         HIP_OK(hipGetLastError());
+        int bmax = 1;
+        for (int t = 0; t < s->T; ++t) bmax = std::max(bmax, s->th[t].n_buckets);
+        const ExportLayout L = export_layout(s->max_nodes, bmax);
+        const size_t need = L.stride * s->T;
+        if (need > s->export_bytes) {
+            if (s->export_h) HIP_OK(hipHostFree(s->export_h));
+            s->export_h = nullptr;
+            s->export_d = s->alloc<char>(need);
+            HIP_OK(hipHostMalloc((void**)&s->export_h, need, hipHostMallocDefault));
+            s->export_bytes = need;
+        }
+        launch_export(s->stream, s->td(), s->an, L, s->export_d);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(s->export_h, s->export_d, L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
-        std::vector<Task> tv(s->T);
-        HIP_OK(hipMemcpy(tv.data(), s->tasks_d, sizeof(Task) * s->T, hipMemcpyDeviceToHost));
         const int MN = s->max_nodes;
-        std::vector<DNode> nodes(MAX_NODES);
-        for (int t = 0; t < s->T; ++t) {
-            const int nn = tv[t].n_nodes;
-            n_nodes[t] = nn;
+        for (int i = 0; i < s->an; ++i) {
+            const int t = s->a0 + i;
+            const char* base = s->export_h + L.stride * i;
+            const int nn = *(const int*)base;
+            if (nn < 1 || nn > MN) throw std::runtime_error("internal: exported tree size out of range");
+            n_nodes[i] = nn;
             const int B = s->th[t].n_buckets;
-            HIP_OK(hipMemcpy(nodes.data(), s->tasks_h[t].nodes, sizeof(DNode) * nn, hipMemcpyDeviceToHost));
+            const DNode* nodes = (const DNode*)(base + L.off_nodes);
             for (int k = 0; k < nn; ++k) {
-                std::memcpy(node_ints + ((size_t)t * MN + k) * TGBM_NODE_INTS, nodes[k].v, sizeof(int) * 10);
-                node_dbl[((size_t)t * MN + k) * 2] = nodes[k].value;
-                node_dbl[((size_t)t * MN + k) * 2 + 1] = nodes[k].gain;
+                std::memcpy(node_ints + ((size_t)i * MN + k) * TGBM_NODE_INTS, nodes[k].v, sizeof(int) * 10);
+                node_dbl[((size_t)i * MN + k) * 2] = nodes[k].value;
+                node_dbl[((size_t)i * MN + k) * 2 + 1] = nodes[k].gain;
             }
-            HIP_OK(hipMemcpy(values + (size_t)t * MN, s->tasks_h[t].values, sizeof(double) * nn, hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(cover + (size_t)t * MN, s->tasks_h[t].cover, sizeof(int) * nn, hipMemcpyDeviceToHost));
-            HIP_OK(hipMemcpy(prefix + (size_t)t * MN * MAX_BUCKETS, s->tasks_h[t].prefix, sizeof(double) * nn * B,
-                             hipMemcpyDeviceToHost));
+            std::memcpy(values + (size_t)i * MN, base + L.off_values, sizeof(double) * nn);
+            std::memcpy(cover + (size_t)i * MN, base + L.off_cover, sizeof(int) * nn);
+            std::memcpy(prefix + (size_t)i * MN * MAX_BUCKETS, base + L.off_prefix, sizeof(double) * nn * B);
             if (s->timing) {  // bytes of the child histogram builds of this tree
                 for (int k = 0; k < nn; ++k) {
                     if (nodes[k].v[N_LEAF]) continue;
-                    const int L = nodes[k].v[N_LEFT], R = nodes[k].v[N_RIGHT];
-                    const double m = std::min(nodes[L].seg_count, nodes[R].seg_count);
+                    const int Lc = nodes[k].v[N_LEFT], Rc = nodes[k].v[N_RIGHT];
+                    const double m = std::min(nodes[Lc].seg_count, nodes[Rc].seg_count);
                     s->t_bytes[5] += m * s->F + m * (s->hconst ? 8.0 : 16.0) + 4.0 * m + (double)s->F * s->nb * 24.0;
                 }
             }

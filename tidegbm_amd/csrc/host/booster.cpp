@@ -16,8 +16,6 @@
 
 namespace tide {
 
-namespace {
-
 void dev_check(int rc) {
     if (rc != 0) throw Error(tgbm_last_error());
 }
@@ -25,6 +23,8 @@ void dev_check(int rc) {
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+
+namespace {
 
 struct SessionGuard {
     tgbm_session* s = nullptr;
@@ -55,6 +55,8 @@ std::vector<double> quantile_edges(std::vector<double>& v, int nb) {
     return e;
 }
 
+}  // namespace
+
 // Per-feature edges over non-NaN values with the reference's sub-sampling
 // (Binner::fit numeric branch, binning.cpp:84-99); features run in parallel.
 void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
@@ -77,6 +79,8 @@ void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint3
     }
 }
 
+namespace {
+
 double leaf_value_host(double G, double H, const Config& c) {  // tree.hpp:53-67
     double v;
     if (c.l1 > 0) {
@@ -89,6 +93,8 @@ double leaf_value_host(double G, double H, const Config& c) {  // tree.hpp:53-67
     if (c.max_delta_step > 0) v = std::min(std::max(v, -c.max_delta_step), c.max_delta_step);
     return v;
 }
+
+}  // namespace
 
 tgbm_grow_params grow_params(const Config& c) {
     tgbm_grow_params p;
@@ -108,6 +114,8 @@ tgbm_grow_params grow_params(const Config& c) {
 }
 
 // Host mirror of one node during host-driven (PAIR-capable) growth.
+namespace {
+
 struct LeafInfo {
     std::vector<double> gain;
     std::vector<int> thr, ml;
@@ -248,6 +256,142 @@ void goss_sample(const std::vector<double>& g, int n, const Config& c, std::mt19
 
 }  // namespace
 
+// TORB time buckets, causal cuts and initial scores of one time-sorted fit
+// (booster.cpp:36-72 buckets, :311-364 cuts, :399-457 init_prefix / init pred).
+TimeLayout time_layout(const Config& cfg, const long long* ts, const double* y, const double* ws, int n) {
+    TimeLayout tl;
+    std::vector<int>& bucket = tl.bucket;
+    std::vector<int>& cuts = tl.cuts;
+    bucket.assign(n, 0);
+    int n_buckets = 1;
+    long long embargo = cfg.embargo;
+    if (!cfg.iid_mode) {
+        if (!ts) throw Error("timestamp is required unless iid_mode");
+        std::vector<long long> u(ts, ts + n);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        const int uniq_n = std::max(2, (int)u.size());
+        int want = cfg.n_time_buckets;
+        if (want <= 0) {
+            long long step = 1;
+            if (u.size() >= 2) {
+                std::vector<long long> d;
+                for (size_t k = 1; k < u.size(); ++k)
+                    if (u[k] - u[k - 1] > 0) d.push_back(u[k] - u[k - 1]);
+                if (!d.empty()) {
+                    std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
+                    step = std::max<long long>(d[d.size() / 2], 1);
+                }
+            }
+            const int h_steps = (int)(std::max<long long>(cfg.horizon, 0) / step);
+            want = std::max(2, std::min(h_steps >= 8 ? 64 : 32, std::max(uniq_n, 2)));
+        }
+        n_buckets = cfg.finer_torb ? std::max(2, std::min(want, uniq_n)) : std::max(2, want);
+        if (n_buckets > 256) throw Error("the GPU path supports at most 256 time buckets");
+        std::vector<int> order(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts[a] < ts[b]; });
+        for (int r = 0; r < n; ++r) {
+            int b = (int)((double)r / std::max(n, 1) * n_buckets);
+            if (b >= n_buckets) b = n_buckets - 1;
+            bucket[order[r]] = b;
+        }
+        embargo += std::max<long long>(cfg.horizon, 0);
+        bool sorted = true;
+        for (int i = 1; i < n && sorted; ++i) sorted = ts[i] >= ts[i - 1];
+        if (!sorted) throw Error("internal: rows not time-sorted");
+        std::vector<int> first(n_buckets, -1);
+        for (int i = 0; i < n; ++i)
+            if (first[bucket[i]] < 0) first[bucket[i]] = i;
+        cuts.assign(n_buckets, 0);
+        int run = 0;
+        for (int b = 0; b < n_buckets; ++b) {
+            int c = 0;
+            if (first[b] >= 0) c = (int)(std::lower_bound(ts, ts + n, ts[first[b]] - embargo) - ts);
+            run = std::max(run, c);
+            cuts[b] = run;
+        }
+    }
+    tl.n_buckets = n_buckets;
+    auto init_of = [&](int m) -> double {  // Objective::init_score (objective.cpp:80-100)
+        if (m <= 0) return 0;
+        if (cfg.objective == 5) return 0;
+        double sw = 0, sy = 0;
+        for (int i = 0; i < m; ++i) {
+            sw += ws[i];
+            sy += ws[i] * y[i];
+        }
+        return sw > 0 ? sy / sw : 0;
+    };
+    std::vector<double>& pred = tl.pred;
+    pred.assign(n, 0.0);
+    tl.init_prefix.assign(n_buckets, 0.0);
+    if (cfg.iid_mode) {
+        tl.init_score = init_of(n);
+        std::fill(pred.begin(), pred.end(), tl.init_score);
+    } else {
+        // running weighted sums snapshot at each cut: identical to refolding [0, cut) per bucket
+        std::vector<double> at(n_buckets, NAN);
+        if (cfg.objective == 5) {
+            for (int b = 0; b < n_buckets; ++b) at[b] = cuts[b] > 0 ? 0.0 : NAN;
+        } else {
+            double sw = 0, sy = 0;
+            int i = 0;
+            for (int b = 0; b < n_buckets; ++b) {
+                for (; i < cuts[b]; ++i) {
+                    sw += ws[i];
+                    sy += ws[i] * y[i];
+                }
+                if (cuts[b] > 0) at[b] = sw > 0 ? sy / sw : 0;
+            }
+        }
+        tl.init_prefix = at;
+        tl.init_score = init_of(n);
+        for (int b = 0; b < n_buckets; ++b)
+            if (!std::isfinite(tl.init_prefix[b])) tl.init_prefix[b] = tl.init_score;
+        for (int i = 0; i < n; ++i) pred[i] = tl.init_prefix[bucket[i]];
+    }
+    return tl;
+}
+
+void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges) {
+    const int F = (int)edges.size();
+    std::vector<double> E((size_t)F * 256, 0.0);
+    std::vector<int> ne(F);
+    for (int f = 0; f < F; ++f) {
+        ne[f] = (int)edges[f].size();
+        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+    }
+    dev_check(tgbm_session_bin(s, X, E.data(), ne.data()));
+}
+
+Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov, const double* pref,
+                 int n_buckets) {
+    Tree t;
+    t.nodes.resize(nn);
+    for (int k = 0; k < nn; ++k) {
+        const int* v = &ints[(size_t)k * TGBM_NODE_INTS];
+        Node& d = t.nodes[k];
+        d.is_leaf = v[0];
+        d.split_type = v[1];
+        d.feature = v[2];
+        d.feature2 = v[3];
+        d.threshold = v[4];
+        d.left = v[5];
+        d.right = v[6];
+        d.missing_go_left = v[7];
+        d.n_samples = v[8];
+        d.depth = v[9];
+        d.value = dbl[2 * k];
+        d.gain = dbl[2 * k + 1];
+    }
+    t.values.assign(vals, vals + nn);
+    t.cover.assign(cov, cov + nn);
+    t.n_buckets = n_buckets;
+    t.prefix_values.assign(pref, pref + (size_t)nn * n_buckets);
+    return t;
+}
+
 void Config::validate() const {  // reference config.hpp:78-97
     if (n_estimators < 1) throw Error("n_estimators must be >= 1");
     if (learning_rate <= 0 || learning_rate > 1.5) throw Error("learning_rate out of range");
@@ -382,124 +526,31 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
             if (rg0) rg = rgs.data();
         }
     }
-    // ---- TORB buckets and causal cuts (booster.cpp:36-72, 311-364)
-    std::vector<int> bucket(n, 0), cuts;
-    int n_buckets = 1;
-    long long embargo = cfg.embargo;
-    if (!cfg.iid_mode) {
-        if (!ts) throw Error("timestamp is required unless iid_mode");
-        std::vector<long long> u(ts, ts + n);
-        std::sort(u.begin(), u.end());
-        u.erase(std::unique(u.begin(), u.end()), u.end());
-        const int uniq_n = std::max(2, (int)u.size());
-        int want = cfg.n_time_buckets;
-        if (want <= 0) {
-            long long step = 1;
-            if (u.size() >= 2) {
-                std::vector<long long> d;
-                for (size_t k = 1; k < u.size(); ++k)
-                    if (u[k] - u[k - 1] > 0) d.push_back(u[k] - u[k - 1]);
-                if (!d.empty()) {
-                    std::nth_element(d.begin(), d.begin() + d.size() / 2, d.end());
-                    step = std::max<long long>(d[d.size() / 2], 1);
-                }
-            }
-            const int h_steps = (int)(std::max<long long>(cfg.horizon, 0) / step);
-            want = std::max(2, std::min(h_steps >= 8 ? 64 : 32, std::max(uniq_n, 2)));
-        }
-        n_buckets = cfg.finer_torb ? std::max(2, std::min(want, uniq_n)) : std::max(2, want);
-        if (n_buckets > 256) throw Error("the GPU path supports at most 256 time buckets");
-        std::vector<int> order(n);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ts[a] < ts[b]; });
-        for (int r = 0; r < n; ++r) {
-            int b = (int)((double)r / std::max(n, 1) * n_buckets);
-            if (b >= n_buckets) b = n_buckets - 1;
-            bucket[order[r]] = b;
-        }
-        embargo += std::max<long long>(cfg.horizon, 0);
-        bool sorted = true;
-        for (int i = 1; i < n && sorted; ++i) sorted = ts[i] >= ts[i - 1];
-        if (!sorted) throw Error("internal: rows not time-sorted");
-        std::vector<int> first(n_buckets, -1);
-        for (int i = 0; i < n; ++i)
-            if (first[bucket[i]] < 0) first[bucket[i]] = i;
-        cuts.assign(n_buckets, 0);
-        int run = 0;
-        for (int b = 0; b < n_buckets; ++b) {
-            int c = 0;
-            if (first[b] >= 0) c = (int)(std::lower_bound(ts, ts + n, ts[first[b]] - embargo) - ts);
-            run = std::max(run, c);
-            cuts[b] = run;
-        }
-    }
-    n_time_buckets = n_buckets;
+    // ---- TORB buckets, causal cuts and initial scores
+    TimeLayout tl = time_layout(cfg, ts, y, ws.data(), n);
+    n_time_buckets = tl.n_buckets;
+    init_prefix = tl.init_prefix;
+    init_score = tl.init_score;
     // ---- bin edges (booster.cpp:378-380) and bins on the device
     fit_edges(X, n, F, cfg.max_bin, cfg.bin_sample, (uint32_t)cfg.random_state, edges);
     n_bins = cfg.max_bin;
     for (auto& e : edges) n_bins = std::max(n_bins, (int)e.size() + 2);
-    // ---- initial scores (booster.cpp:399-457)
-    auto init_of = [&](int m) -> double {  // Objective::init_score (objective.cpp:80-100)
-        if (m <= 0) return 0;
-        if (cfg.objective == 5) return 0;
-        double sw = 0, sy = 0;
-        for (int i = 0; i < m; ++i) {
-            sw += ws[i];
-            sy += ws[i] * y[i];
-        }
-        return sw > 0 ? sy / sw : 0;
-    };
-    std::vector<double> pred(n);
-    init_prefix.assign(n_buckets, 0.0);
-    if (cfg.iid_mode) {
-        init_score = init_of(n);
-        std::fill(pred.begin(), pred.end(), init_score);
-    } else {
-        // running weighted sums snapshot at each cut: identical to refolding [0, cut) per bucket
-        std::vector<double> at(n_buckets, NAN);
-        if (cfg.objective == 5) {
-            for (int b = 0; b < n_buckets; ++b) at[b] = cuts[b] > 0 ? 0.0 : NAN;
-        } else {
-            double sw = 0, sy = 0;
-            int i = 0;
-            for (int b = 0; b < n_buckets; ++b) {
-                for (; i < cuts[b]; ++i) {
-                    sw += ws[i];
-                    sy += ws[i] * y[i];
-                }
-                if (cuts[b] > 0) at[b] = sw > 0 ? sy / sw : 0;
-            }
-        }
-        init_prefix = at;
-        init_score = init_of(n);
-        for (int b = 0; b < n_buckets; ++b)
-            if (!std::isfinite(init_prefix[b])) init_prefix[b] = init_score;
-        for (int i = 0; i < n; ++i) pred[i] = init_prefix[bucket[i]];
-    }
     // ---- device session
     const bool use_goss = (cfg.goss > 0) || (cfg.goss < 0 && n >= 250000);
     const bool goss_active = use_goss && cfg.goss_top < 1.0 && n > 64;
     const bool hconst = cfg.objective == 0 && !any_weight && !goss_active;
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, F, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
-    {
-        std::vector<double> E((size_t)F * 256, 0.0);
-        std::vector<int> ne(F);
-        for (int f = 0; f < F; ++f) {
-            ne[f] = (int)edges[f].size();
-            std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
-        }
-        dev_check(tgbm_session_bin(sg.s, X, E.data(), ne.data()));
-    }
+    bin_on_device(sg.s, X, edges);
     const long long* groups = (cfg.objective == 5) ? rg : nullptr;
-    dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, pred.data(), bucket.data(), nullptr,
-                                 n_buckets, cfg.iid_mode ? nullptr : cuts.data(), groups));
+    dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, tl.pred.data(), tl.bucket.data(),
+                                 nullptr, tl.n_buckets, cfg.iid_mode ? nullptr : tl.cuts.data(), groups));
     trees.clear();
     if (cfg.profile) dev_check(tgbm_timer_enable(sg.s, 1));
     t_setup = now_s() - t0;
     const double t1 = now_s();
     std::mt19937 rng((uint32_t)cfg.random_state);
-    grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, cuts, rng, goss_active, nullptr, false);
+    grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, tl.cuts, rng, goss_active, nullptr, false);
     if (cfg.profile)
         for (int k = 0; k < 8; ++k) dev_check(tgbm_timer_read(sg.s, k, &prof_ms[k], &prof_n[k], &prof_bytes[k]));
     best_iteration = cfg.n_estimators;
@@ -541,34 +592,12 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         int nn = 0;
         dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
                                 ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
-        Tree t;
-        t.nodes.resize(nn);
-        for (int k = 0; k < nn; ++k) {
-            const int* v = &ints[(size_t)k * TGBM_NODE_INTS];
-            Node& d = t.nodes[k];
-            d.is_leaf = v[0];
-            d.split_type = v[1];
-            d.feature = v[2];
-            d.feature2 = v[3];
-            d.threshold = v[4];
-            d.left = v[5];
-            d.right = v[6];
-            d.missing_go_left = v[7];
-            d.n_samples = v[8];
-            d.depth = v[9];
-            d.value = dbl[2 * k];
-            d.gain = dbl[2 * k + 1];
-        }
-        t.values.assign(vals.begin(), vals.begin() + nn);
-        t.cover.assign(cov.begin(), cov.begin() + nn);
+        Tree t = unpack_tree(nn, ints.data(), dbl.data(), vals.data(), cov.data(), pref.data(), n_time_buckets);
         if (for_update) {
             // Booster::update keeps grow_tree's cover (node sample counts) and one bucket
             for (int k = 0; k < nn; ++k) t.cover[k] = t.nodes[k].n_samples;
             t.n_buckets = 1;
             t.prefix_values = t.values;
-        } else {
-            t.n_buckets = n_time_buckets;
-            t.prefix_values.assign(pref.begin(), pref.begin() + (size_t)nn * n_time_buckets);
         }
         trees.push_back(std::move(t));
         if (!for_update) tree_end_times.push_back(now_s() - t_loop);  // tree_end synchronized
@@ -587,15 +616,7 @@ void Booster::update(const double* X, const double* y, int n, int nf, const doub
     const bool hconst = cfg.objective == 0 && !any_weight;
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, nf, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
-    {
-        std::vector<double> E((size_t)nf * 256, 0.0);
-        std::vector<int> ne(nf);
-        for (int f = 0; f < nf; ++f) {
-            ne[f] = (int)edges[f].size();
-            std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
-        }
-        dev_check(tgbm_session_bin(sg.s, X, E.data(), ne.data()));
-    }
+    bin_on_device(sg.s, X, edges);
     std::vector<int> zeros(n, 0);
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? w : nullptr, pred.data(), zeros.data(), nullptr, 1, nullptr,
                                  nullptr));

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -80,6 +81,24 @@ struct Tree {  // reference Tree (tree.hpp:32-41)
     int n_buckets = 1;
 };
 
+// TORB layout and initial scores of one time-sorted fit.
+struct TimeLayout {
+    int n_buckets = 1;
+    std::vector<int> bucket, cuts;       // per row / per bucket
+    std::vector<double> init_prefix;     // per bucket
+    double init_score = 0.0;
+    std::vector<double> pred;            // initial prediction per row
+};
+TimeLayout time_layout(const Config& cfg, const long long* ts, const double* y, const double* w, int n);
+void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
+               std::vector<std::vector<double>>& edges);
+void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges);
+Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov,
+                 const double* pref, int n_buckets);
+tgbm_grow_params grow_params(const Config& c);
+void dev_check(int rc);
+double now_s();
+
 class Booster {
 public:
     Config cfg;
@@ -121,5 +140,18 @@ private:
     void grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, const std::vector<int>& cuts,
                     std::mt19937& rng, bool goss, const double* gsample_unused, bool for_update);
 };
+
+struct BatchStats {
+    double setup_s = 0.0, boost_s = 0.0;
+    double prof_ms[8] = {0}, prof_bytes[8] = {0};
+    long long prof_n[8] = {0};
+};
+
+// Walk-forward batch: n_tasks fits over row prefixes [0, task_rows[t]) of one matrix whose rows
+// are sorted by (timestamp, entity), with per-task targets ys[t] and horizons; schedule 0 grows
+// all tasks' trees together, 1 one task after another.  Fills out[t] with fitted boosters.
+BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const long long* ts,
+                     const long long* entity, int n_tasks, const int* task_rows, const double* const* ys,
+                     const long long* horizons, int schedule, std::vector<std::unique_ptr<Booster>>& out);
 
 }  // namespace tide

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <fstream>
 #include <iterator>
+#include <memory>
 #include <string>
 
 #include "../../../include/tidegbm_amd.h"
@@ -271,6 +272,18 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
     *setup_s = b->t_setup;
     *boost_s = b->t_boost;
     return 0;
+}
+
+int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features, const long long* timestamps,
+                   const long long* entity, int n_tasks, const int* task_rows, const double* const* y,
+                   const long long* horizons, int schedule, TideHandle* out) {
+    auto* p = static_cast<Booster*>(proto);
+    return guard([&] {
+        if (!p || !X || !task_rows || !y || !out) throw tide::Error("null argument");
+        std::vector<std::unique_ptr<Booster>> res;
+        tide::fit_batch(p->cfg, X, n, n_features, timestamps, entity, n_tasks, task_rows, y, horizons, schedule, res);
+        for (int t = 0; t < n_tasks; ++t) out[t] = res[t].release();
+    });
 }
 
 }  // extern "C"
