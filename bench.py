@@ -189,6 +189,101 @@ def run_walkforward(args, p):
     }
 
 
+def _serve_stream(p, seconds, rate, n_models, seed):
+    """Seeded request stream: Poisson arrivals; 90% predicts of log-uniform 1..65,536 panel rows
+    (a contiguous block at a random start), 10% refits of 20k..200k rows x 50 trees."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            break
+        m = int(rng.integers(n_models))
+        if rng.random() < 0.9:
+            n = int(min(65536, max(1, np.floor(np.exp(rng.uniform(0.0, np.log(65536.0)))))))
+            out.append((t, "predict", m, int(rng.integers(0, p.n - n + 1)), n))
+        else:
+            n = int(rng.integers(20000, 200001))
+            out.append((t, "refit", m, int(rng.integers(0, p.n - n + 1)), n))
+    return out
+
+
+def _serve_run(t, p, models, stream, batched, slots):
+    import numpy as np
+    srv = t.Server(models, p.X, predict_budget=2 << 30, refit_slots=slots, refit_max_rows=200000,
+                   batched=batched)
+    tickets = []
+    t0 = time.perf_counter()
+    for (at, kind, m, r0, n) in stream:
+        while True:
+            d = t0 + at - time.perf_counter()
+            if d <= 0:
+                break
+            time.sleep(min(d, 0.0005))
+        if kind == "predict":
+            tickets.append((kind, n, srv.predict(m, np.arange(r0, r0 + n, dtype=np.int32))))
+        else:
+            tickets.append((kind, n, srv.refit(m, r0, p.y[r0:r0 + n], extra_trees=50)))
+    t_sub = time.perf_counter() - t0
+    lat = {"predict": [], "refit": []}
+    small = []
+    rows = 0
+    for kind, n, tk in tickets:
+        _, _, l = srv.wait(tk)
+        lat[kind].append(l)
+        if kind == "predict":
+            rows += n
+            if n <= 4096:
+                small.append(l)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    pct = lambda a, q: round(1e3 * float(np.percentile(a, q)), 3) if len(a) else None
+    return {
+        "requests": len(stream), "predicts": len(lat["predict"]), "refits": len(lat["refit"]),
+        "submit_s": round(t_sub, 3), "drain_s": round(wall, 3),
+        "predict_p50_ms": pct(lat["predict"], 50), "predict_p99_ms": pct(lat["predict"], 99),
+        "predict_le4096_p50_ms": pct(small, 50), "predict_le4096_p99_ms": pct(small, 99),
+        "refit_p50_ms": pct(lat["refit"], 50), "refit_p99_ms": pct(lat["refit"], 99),
+        "predict_rows_per_s": round(rows / wall, 1), "predict_launches": st["predict_launches"],
+        "refit_rounds": st["refit_rounds"],
+    }
+
+
+def run_serve(args, p):
+    """Serving loop: 8 resident ensembles (500 trees, depth 8, max_leaves 31, 128 features) over the
+    resident 1M x 128 binned panel; a seeded Poisson stream at 2,000 requests/s, served batched
+    (coalesced predict launches, refits grown together in slots) and unbatched (one request at a
+    time, arrival order)."""
+    import tidegbm_amd as t
+    from tidegbm_amd import synth
+
+    tasks = synth.walk_forward_tasks(p, n_folds=2, horizons=(1, 3, 5, 10), first_dates=2200)
+    params = dict(n_estimators=args.serve_trees, max_depth=8, learning_rate=0.05, max_bin=255, goss=0,
+                  pair_splits=0)
+    log(f"[serve] training {len(tasks)} ensembles x {args.serve_trees} trees (batched fit)")
+    t_fit = time.perf_counter()
+    models = t.fit_batch(p.X, p.ts, [k.n_train for k in tasks], [k.y for k in tasks], entity=p.entity,
+                         horizons=[k.horizon for k in tasks], **params)
+    t_fit = time.perf_counter() - t_fit
+    stream = _serve_stream(p, args.serve_seconds, 2000.0, len(models), seed=7)
+    log(f"[serve] {len(stream)} requests over {args.serve_seconds}s: batched")
+    b = _serve_run(t, p, models, stream, True, args.serve_slots)
+    log(f"[serve] batched: {b}")
+    log("[serve] unbatched")
+    u = _serve_run(t, p, models, stream, False, 1)
+    log(f"[serve] unbatched: {u}")
+    return {
+        "workload": f"8 resident ensembles ({args.serve_trees} trees, depth 8, max_leaves 31, 128 features) over "
+                    "the resident 1M x 128 binned panel; Poisson arrivals at 2,000 req/s for "
+                    f"{args.serve_seconds}s; 90% predicts of log-uniform 1..65,536 panel rows, 10% refits of "
+                    "20k..200k rows x 50 trees; predict staging budget 2 GiB, "
+                    f"{args.serve_slots} refit slots of up to 200,000 rows",
+        "ensembles_fit_s": round(t_fit, 3), "batched": b, "unbatched": u,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,7 +292,10 @@ def main():
     ap.add_argument("--cpu-trees", type=int, default=6)
     ap.add_argument("--profile-trees", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "plain", "walkforward"], default="all")
+    ap.add_argument("--workload", choices=["all", "plain", "walkforward", "serve"], default="all")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving request stream")
+    ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
+    ap.add_argument("--serve-slots", type=int, default=32, help="concurrent refits when batched")
     ap.add_argument("--wf-trees", type=int, default=100, help="trees per walk-forward fit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
@@ -220,9 +318,13 @@ def main():
     if args.workload == "walkforward":
         print(json.dumps({"walk_forward": run_walkforward(args, p)}), flush=True)
         return
+    if args.workload == "serve":
+        print(json.dumps({"serve": run_serve(args, p)}), flush=True)
+        return
     out = run_plain(args, p)
     if args.workload == "all":
         out["walk_forward"] = run_walkforward(args, p)
+        out["serve"] = run_serve(args, p)
     print(json.dumps(out), flush=True)
 
 

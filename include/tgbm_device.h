@@ -14,6 +14,8 @@
 #ifndef TGBM_DEVICE_H
 #define TGBM_DEVICE_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -70,6 +72,13 @@ int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, cons
  * schedule activates one task at a time over the same resident bins.  The results of a task
  * do not depend on which other tasks share its launches. */
 int tgbm_session_set_active(tgbm_session* s, int first, int count);
+/* Rebind task `task` to rows [row0, row0 + n_rows) of the shared matrix (n_rows at most the
+ * task's row count at creation); resets its sample, TORB layout (one bucket) and groups, so
+ * tgbm_task_set_rows must follow.  Used to recycle task slots for refits of row windows. */
+int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows);
+/* Exchange the state of tasks a and b (row windows, per-row arrays, trees in progress), so
+ * the tasks a caller wants to grow can be kept in one contiguous active range. */
+int tgbm_task_swap(tgbm_session* s, int a, int b);
 /* Row-major uint8 bins back to the host (n_rows x n_features), for checking. */
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out);
 
@@ -129,6 +138,27 @@ void tgbm_forest_destroy(tgbm_forest* f);
 /* Predict raw scores for host rows X (row-major f64) or for device-resident uint8 bins. */
 int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out);
 int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out);
+
+/* ---- batched predict over many resident ensembles (serving) ----
+ * A predictor owns a stream and pinned/device staging memory.  tgbm_predict_batch scores all
+ * items in one kernel launch: each item names an ensemble and either rows of a session's
+ * resident binned panel (rows != NULL, ids into the panel) or raw feature rows X (row-major
+ * f64, binned with the ensemble's edges).  Scores are bit-identical to tgbm_forest_predict
+ * (the reference's predict_raw_binned order, booster.cpp:744-773) whatever the batch holds. */
+typedef struct tgbm_predictor tgbm_predictor;
+typedef struct tgbm_predict_item {
+    const tgbm_forest* forest;
+    const int* rows;        /* panel row ids, or NULL */
+    const double* X;        /* n x n_features raw rows when rows == NULL */
+    int n;
+    double* out;            /* n scores (host) */
+} tgbm_predict_item;
+int tgbm_predictor_create(tgbm_predictor** out, size_t staging_bytes);
+void tgbm_predictor_destroy(tgbm_predictor* p);
+/* Staging bytes one item needs (admission accounting); a batch needs their sum plus 256 per item. */
+size_t tgbm_predict_bytes(const tgbm_forest* f, int from_x, int n);
+int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_predict_item* items,
+                       int n_items);
 
 #ifdef __cplusplus
 }

@@ -91,6 +91,39 @@ int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features,
                    const int* task_rows, const double* const* y, const long long* horizons,
                    int schedule, TideHandle* out);
 
+/* ---- serving loop (no counterpart in the reference, whose request-level calls are
+ * tide_predict and Booster::update, booster.cpp:801-884) ----
+ * A server keeps ensembles and a binned panel resident on the GPU and serves asynchronous
+ * predict and refit requests.  Batched mode coalesces every queued predict that fits the
+ * predict_budget staging bytes into one launch and grows up to refit_slots refits together;
+ * unbatched mode serves one request at a time in arrival order (baseline).  The ensembles must
+ * be fitted MSE models with pair_splits=0 that share bin edges and growth settings (e.g. the
+ * outputs of one tide_fit_batch); panel_X (n_rows x n_features, row-major) is binned once with
+ * those edges.  Submissions return a ticket (>0) or -1 with tide_last_error() set. */
+typedef void* TideServer;
+int tide_server_create(TideServer* out, const TideHandle* models, int n_models,
+                       const double* panel_X, int n_rows, int n_features,
+                       long long predict_budget_bytes, int refit_slots, int refit_max_rows,
+                       int batched);
+void tide_server_free(TideServer s);
+/* Score panel rows (ids into panel_X) with ensemble `model`. */
+long long tide_server_predict(TideServer s, int model, const int* rows, int n);
+/* Score raw feature rows (n x n_features); binned on the GPU with the shared edges. */
+long long tide_server_predict_x(TideServer s, int model, const double* X, int n);
+/* Booster::update on panel rows [row0, row0 + n) with targets y: extra_trees more trees,
+ * published as the model's next version when done. */
+long long tide_server_refit(TideServer s, int model, int row0, int n, const double* y,
+                           int extra_trees);
+/* Wait for a ticket: predict scores into out (cap values), the version used (predict) or
+ * published (refit), submit-to-done seconds.  Returns the request's row count, -1 on error. */
+int tide_server_wait(TideServer s, long long ticket, double* out, int cap, int* version,
+                     double* latency_s);
+/* New handle holding a version of a model (-1 = current) and that version's parent. */
+int tide_server_model(TideServer s, int model, int version, TideHandle* out, int* parent);
+/* Counters: predict launches, predict requests, predict rows, refit rounds, refits done,
+ * seconds the predict and refit threads spent on the GPU. */
+int tide_server_stats(TideServer s, long long* counts5, double* busy2);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,14 @@ _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
+_sig("tide_server_create", _I, ctypes.POINTER(_P), _P, _I, _P, _I, _I, _LL, _I, _I, _I)
+_sig("tide_server_free", None, _P)
+_sig("tide_server_predict", _LL, _P, _I, _P, _I)
+_sig("tide_server_predict_x", _LL, _P, _I, _P, _I)
+_sig("tide_server_refit", _LL, _P, _I, _I, _I, _P, _I)
+_sig("tide_server_wait", _I, _P, _LL, _P, _I, _P, _P)
+_sig("tide_server_model", _I, _P, _I, _I, ctypes.POINTER(_P), _P)
+_sig("tide_server_stats", _I, _P, _P, _P)
 
 #: True: the gfx950 kernels are compiled into the loaded library (reference: tidegbm.cuda_compiled).
 gpu_compiled: bool = bool(_lib.tide_device_compiled())
@@ -264,6 +272,84 @@ def fit_batch(X, timestamps, task_rows, targets, *, entity=None, horizons=None,
     if rc != 0:
         raise TideError(_err())
     return [Booster(handle=_P(out[t])) for t in range(T)]
+
+
+class Server:
+    """Serving loop over resident ensembles and a resident binned panel (see tide_server_create).
+
+    predict / predict_x / refit return tickets at once; wait(ticket) blocks for the result:
+    (scores or None, version, latency seconds).
+    """
+
+    def __init__(self, models, panel_X, *, predict_budget=1 << 30, refit_slots=16,
+                 refit_max_rows=262144, batched=True):
+        X = np.ascontiguousarray(panel_X, dtype=np.float64)
+        self._X_rows, self._F = X.shape
+        hs = (_P * len(models))(*[m._h for m in models])
+        self._models = list(models)   # keep the handles alive while the server copies them
+        h = _P()
+        rc = _lib.tide_server_create(ctypes.byref(h), hs, len(models), _ptr(X), X.shape[0], X.shape[1],
+                                     int(predict_budget), int(refit_slots), int(refit_max_rows),
+                                     1 if batched else 0)
+        if rc != 0:
+            raise TideError(_err())
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.tide_server_free(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _ticket(self, t, n, kind):
+        if t < 0:
+            raise TideError(_err())
+        self._n[t] = (n, kind)
+        return t
+
+    def predict(self, model: int, rows):
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        return self._ticket(_lib.tide_server_predict(self._h, int(model), _ptr(r), r.shape[0]), r.shape[0], 0)
+
+    def predict_x(self, model: int, X):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim != 2 or X.shape[1] != self._F:
+            raise TideError("X must be n x n_features")
+        return self._ticket(_lib.tide_server_predict_x(self._h, int(model), _ptr(X), X.shape[0]), X.shape[0], 0)
+
+    def refit(self, model: int, row0: int, y, extra_trees: int = 50):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        t = _lib.tide_server_refit(self._h, int(model), int(row0), y.shape[0], _ptr(y), int(extra_trees))
+        return self._ticket(t, y.shape[0], 2)
+
+    def wait(self, ticket):
+        n, kind = self._n.pop(ticket)
+        out = np.empty(n) if kind != 2 else None
+        ver, lat = _I(), _D()
+        rc = _lib.tide_server_wait(self._h, ticket, _ptr(out), n if out is not None else 0, ctypes.byref(ver),
+                                   ctypes.byref(lat))
+        if rc < 0:
+            raise TideError(_err())
+        return out, ver.value, lat.value
+
+    def model(self, model: int, version: int = -1):
+        """(Booster snapshot of a version, parent version)."""
+        h, parent = _P(), _I()
+        if _lib.tide_server_model(self._h, int(model), int(version), ctypes.byref(h), ctypes.byref(parent)) != 0:
+            raise TideError(_err())
+        return Booster(handle=h), parent.value
+
+    def stats(self):
+        c = np.zeros(5, np.int64)
+        b = np.zeros(2)
+        _lib.tide_server_stats(self._h, _ptr(c), _ptr(b))
+        return dict(predict_launches=int(c[0]), predict_requests=int(c[1]), predict_rows=int(c[2]),
+                    refit_rounds=int(c[3]), refits=int(c[4]), predict_busy_s=float(b[0]),
+                    refit_busy_s=float(b[1]))
 
 
 def tree_arrays(h, info_fn, get_fn, t):

@@ -83,8 +83,11 @@ __host__ inline ExportLayout export_layout(int max_nodes, int max_buckets) {
 
 // Per-task state and pointers, resident in device memory (one entry per task).
 struct Task {
-    // ---- fixed at creation
-    int n_rows;            // this task uses rows [0, n_rows) of the shared matrix
+    // ---- row window (set at creation or by a later bind)
+    int n_rows;            // this task uses rows [row0, row0 + n_rows) of the shared matrix
+    int row0;              // local row i of every per-row array is shared-matrix row row0 + i
+    int cap_rows;          // capacity of the per-row arrays
+    int pad_r;
     int n_buckets;
     int hconst;            // h == 1 for every row
     int has_groups;

@@ -13,6 +13,7 @@
 
 #include "../../../include/tgbm_device.h"
 #include "dev_common.h"
+#include "forest.h"
 #include "launch.h"
 
 using namespace tgbm;
@@ -27,46 +28,20 @@ extern thread_local std::string g_dev_err;  // defined in session.hip
                                      " at " #expr);                                          \
     } while (0)
 
-// Packed node: x = feature | feature2 << 16, y = threshold, z = left, w = right (absolute).
-// Flags in a separate byte array: bit0 leaf, bits1-2 split type, bit3 missing_go_left.
-struct FNode {
-    int fx, thr, left, right;
-};
-
-struct tgbm_forest {
-    int F = 0, fp = 0, nb = 255, n_trees = 0, n_nodes = 0;
-    double init = 0.0;
-    FNode* nodes = nullptr;
-    unsigned char* flags = nullptr;
-    double* values = nullptr;
-    int* roots = nullptr;
-    double* lr = nullptr;
-    double* edges = nullptr;
-    int* n_edges = nullptr;
-    hipStream_t stream = nullptr;
-    // staging for requests
-    double* Xd = nullptr;
-    unsigned char* bins = nullptr;
-    unsigned char* bins_col_dummy = nullptr;
-    double* outd = nullptr;
-    int cap = 0;
-    ~tgbm_forest() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {(void*)nodes, (void*)flags, (void*)values, (void*)roots, (void*)lr, (void*)edges,
-                        (void*)n_edges, (void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
-            if (p) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-    void reserve(int n) {
-        if (n <= cap) return;
-        for (void* p : {(void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
-            if (p) (void)hipFree(p);
-        cap = std::max(n, 1024);
-        FHIP(hipMalloc(&Xd, sizeof(double) * (size_t)cap * F));
-        FHIP(hipMalloc(&bins, (size_t)cap * fp + 16));
-        FHIP(hipMalloc(&outd, sizeof(double) * cap));
-    }
-};
+void tgbm_forest::reserve(int n) {
+    if (n <= cap) return;
+    for (void* p : {(void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
+        if (p) (void)hipFree(p);
+    Xd = nullptr;
+    bins = nullptr;
+    outd = nullptr;
+    cap = 0;
+    const int c = std::max(n, 1024);
+    FHIP(hipMalloc(&Xd, sizeof(double) * (size_t)c * F));
+    FHIP(hipMalloc(&bins, (size_t)c * fp + 16));
+    FHIP(hipMalloc(&outd, sizeof(double) * c));
+    cap = c;
+}
 
 // One thread per row; the row's bins are re-read through L1/L2 at every visited node.
 __global__ __launch_bounds__(256) void k_forest(const FNode* __restrict__ nodes,
@@ -143,11 +118,14 @@ int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_tree
         for (int t = 0; t < n_trees; ++t) total += tree_nodes[t];
         f->n_nodes = total;
         std::vector<FNode> nodes(std::max(total, 1));
+        std::vector<SNode> snodes(std::max(total, 1));
         std::vector<unsigned char> flags(std::max(total, 1));
-        std::vector<int> roots(std::max(n_trees, 1));
+        std::vector<int> roots(std::max(n_trees, 1)), toff(n_trees + 1);
         int base = 0;
         for (int t = 0; t < n_trees; ++t) {
+            if (tree_nodes[t] < 1 || tree_nodes[t] > 255) throw std::runtime_error("trees must have 1..255 nodes");
             roots[t] = base;
+            toff[t] = base;
             for (int k = 0; k < tree_nodes[t]; ++k) {
                 const int* v = node_ints + (size_t)(base + k) * TGBM_NODE_INTS;
                 const bool leaf = v[0] != 0;
@@ -165,12 +143,30 @@ int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_tree
                 }
                 nodes[base + k] = nd;
                 flags[base + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
+                SNode sn{0, 0, 0, 0, 0, flags[base + k]};
+                if (!leaf) {
+                    if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
+                    sn.f = (unsigned short)v[2];
+                    sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
+                    sn.thr = (unsigned char)v[4];
+                    sn.left = (unsigned char)v[5];
+                    sn.right = (unsigned char)v[6];
+                }
+                snodes[base + k] = sn;
             }
             base += tree_nodes[t];
+        }
+        toff[n_trees] = base;
+        f->max_chunk_nodes = 1;
+        for (int t0 = 0; t0 < n_trees; t0 += SERVE_TREES_PER_CHUNK) {
+            const int t1 = std::min(n_trees, t0 + SERVE_TREES_PER_CHUNK);
+            f->max_chunk_nodes = std::max(f->max_chunk_nodes, toff[t1] - toff[t0]);
         }
         FHIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
         FHIP(hipMalloc(&f->nodes, sizeof(FNode) * nodes.size()));
         FHIP(hipMalloc(&f->flags, flags.size()));
+        FHIP(hipMalloc(&f->snodes, sizeof(SNode) * snodes.size()));
+        FHIP(hipMalloc(&f->tree_off, sizeof(int) * toff.size()));
         FHIP(hipMalloc(&f->values, sizeof(double) * std::max(total, 1)));
         FHIP(hipMalloc(&f->roots, sizeof(int) * roots.size()));
         FHIP(hipMalloc(&f->lr, sizeof(double) * std::max(n_trees, 1)));
@@ -178,6 +174,8 @@ int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_tree
         FHIP(hipMalloc(&f->n_edges, sizeof(int) * n_features));
         FHIP(hipMemcpy(f->nodes, nodes.data(), sizeof(FNode) * nodes.size(), hipMemcpyHostToDevice));
         FHIP(hipMemcpy(f->flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->snodes, snodes.data(), sizeof(SNode) * snodes.size(), hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->tree_off, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
         if (total) FHIP(hipMemcpy(f->values, values, sizeof(double) * total, hipMemcpyHostToDevice));
         FHIP(hipMemcpy(f->roots, roots.data(), sizeof(int) * roots.size(), hipMemcpyHostToDevice));
         if (n_trees) FHIP(hipMemcpy(f->lr, lr, sizeof(double) * n_trees, hipMemcpyHostToDevice));

@@ -402,7 +402,7 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
             const int ii = i[u] < 0 ? 0 : i[u];
-            b[u] = active ? *(const uint4*)(S.bins_row + (size_t)ii * S.fp + fcol) : make_uint4(0, 0, 0, 0);
+            b[u] = active ? *(const uint4*)(S.bins_row + (size_t)(T.row0 + ii) * S.fp + fcol) : make_uint4(0, 0, 0, 0);
             p[u] = pk[ii];
             q[u] = HV ? qhp[ii] : 0ull;
         }
@@ -445,6 +445,7 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const unsigned char* colb = S.bins_col + T.row0;  // row0 is a multiple of 16 here (launch_hist)
     for (int base = r0 + wave * 1024; base < r1; base += nwave * 1024) {
         const int row = base + lane * 16;
         if (base + 1024 <= r1 && nf == FB) {
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
             }
             uint4 b[FB];
 #pragma unroll
-            for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(S.bins_col + (size_t)(f0 + fl) * S.n_pad + row);
+            for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(colb + (size_t)(f0 + fl) * S.n_pad + row);
 #pragma unroll
             for (int fl = 0; fl < FB; ++fl) {
                 const unsigned w4[4] = {b[fl].x, b[fl].y, b[fl].z, b[fl].w};
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
         } else if (row < r1) {
             // tail tile: per-row checks
             for (int fl = 0; fl < nf; ++fl) {
-                const unsigned char* col = S.bins_col + (size_t)(f0 + fl) * S.n_pad;
+                const unsigned char* col = colb + (size_t)(f0 + fl) * S.n_pad;
                 for (int t = 0; t < 16 && row + t < r1; ++t) {
                     const int bin = col[row + t];
                     atomicAdd(&lA[fl * NBIN + bin], T.pk[row + t]);
@@ -882,18 +883,19 @@ void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& 
     hipLaunchKernelGGL(k_apply, dim3(n_tasks), dim3(64), 0, st, tasks, F, P);
 }
 
-// =================================================================== stable partition
-// Reference split_indices (tree.cpp:68-91): rows of the leaf, in ascending order, go left when
-// Axis: bin 0 -> missing_go_left else bin <= threshold; Pair: either bin 0 -> missing_go_left
-// else pair_hash <= threshold.  Two passes: per-chunk left counts, then a stable scatter into
-// the other index buffer (the children's rows live there; block 0 records their ranges).
+// =================================================================== partition
+// Reference split_indices (tree.cpp:68-91) decides which rows of the leaf go left: Axis: bin 0
+// -> missing_go_left else bin <= threshold; Pair: either bin 0 -> missing_go_left else
+// pair_hash <= threshold.  The rows move into the other index buffer (lefts at the front of the
+// leaf's range, rights at the back); the reference keeps them in ascending order, this kernel
+// does not need to (see k_partition).
 constexpr int PART_CHUNK = 2048;   // rows per block (8 per thread)
 constexpr int PART_RPT = 8;
 
 __device__ inline bool row_goes_left(const Shared& S, const Task& T, int i) {
-    const int b1 = S.bins_col[(size_t)T.cur_feat * S.n_pad + i];
+    const int b1 = S.bins_col[(size_t)T.cur_feat * S.n_pad + T.row0 + i];
     if (T.cur_type == 1) {
-        const int b2 = S.bins_col[(size_t)T.cur_feat2 * S.n_pad + i];
+        const int b2 = S.bins_col[(size_t)T.cur_feat2 * S.n_pad + T.row0 + i];
         if (b1 == 0 || b2 == 0) return T.cur_mgl != 0;
         const unsigned z = ((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)S.n_bins;
         return (int)z <= T.cur_thr;
@@ -1032,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_pair_hist(Shared S, Task* tasks, int ta
     __syncthreads();
     for (int k = r0 + threadIdx.x; k < r1; k += blockDim.x) {
         const int i = rows[k];
-        const unsigned char* row = S.bins_row + (size_t)i * S.fp;
+        const unsigned char* row = S.bins_row + (size_t)(T.row0 + i) * S.fp;
         const unsigned long long p = T.pk[i];
         const unsigned long long q = hv ? T.qh[i] : 0ull;
         for (int pp = 0; pp < n_pairs; ++pp) {
@@ -1160,7 +1162,7 @@ __global__ __launch_bounds__(256) void k_nid_walk(Shared S, Task* tasks) {
     }
     __syncthreads();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n_rows; i += gridDim.x * blockDim.x) {
-        const unsigned char* row = S.bins_row + (size_t)i * S.fp;
+        const unsigned char* row = S.bins_row + (size_t)(T.row0 + i) * S.fp;
         int k = 0;
         while (!(nflag[k] & 1)) {
             const int fl = nflag[k];

@@ -4,7 +4,12 @@
 #include <algorithm>
 #include "dev_common.h"
 
+struct tgbm_session;
+
 namespace tgbm {
+
+// Resident binned panel of a session (row-major bins, fp bytes per row, n rows, F features).
+void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, int* fp, int* n, int* F);
 
 void init_kernel_attributes();
 void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edges,

@@ -1,0 +1,293 @@
+// Batched ensemble predict for the serving loop: one launch scores every request of a batch,
+// each request naming its own ensemble and its rows (rows of the session's resident binned
+// panel, or raw feature rows binned here with the ensemble's edges).
+//
+// Per request the result is the reference's predict_raw_binned (booster.cpp:744-773):
+// init_score + sum over trees, in tree order, of lr_t * values_t[leaf(row)], with the same
+// rounding (sequential FP64 adds, no FMA contraction), so a row's score does not depend on which
+// other requests shared its launch.
+//
+// Kernel layout: a block scores SERVE_ROWS rows of one request.  It stages the rows' bins in
+// LDS, then walks the ensemble SERVE_TREES_PER_CHUNK trees at a time: the chunk's packed nodes
+// are copied to LDS, SERVE_LANES threads per row traverse different trees of the chunk
+// (LDS-only dependent loads) and record leaf ids, and one thread per row then adds the chunk's
+// leaf values in tree order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/tgbm_device.h"
+#include "dev_common.h"
+#include "forest.h"
+#include "launch.h"
+
+using namespace tgbm;
+
+extern thread_local std::string g_dev_err;  // defined in session.hip
+
+#define SHIP(expr)                                                                           \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) +   \
+                                     " at " #expr);                                          \
+    } while (0)
+
+namespace {
+
+constexpr int SERVE_ROWS = 64;
+constexpr int SERVE_LANES = 4;   // threads per row (256-thread blocks)
+constexpr int SERVE_TC = SERVE_TREES_PER_CHUNK;
+constexpr size_t SERVE_MAX_LDS = 152 * 1024;   // dynamic LDS opt-in (below 160 KiB: the kernel has static LDS too)
+
+struct SReq {
+    const SNode* nodes;
+    const int* tree_off;
+    const double* values;
+    const double* lr;
+    const unsigned char* bins;   // row-major, fp bytes per row
+    const int* rows;             // row ids into bins (null: rows 0..n-1)
+    double* out;
+    double init;
+    int n_trees, nb, fp, n, blk0, pad;
+};
+
+__host__ __device__ constexpr size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// LDS bytes of k_serve for rows of fp bytes and chunks of at most chunk_nodes nodes.
+size_t serve_lds(int fp, int chunk_nodes) {
+    size_t b = (size_t)SERVE_ROWS * fp;                 // bins
+    b += (size_t)SERVE_TC * SERVE_ROWS;                  // leaf ids
+    b = align_up(b + sizeof(int) * (SERVE_TC + 1), 8);   // chunk tree offsets
+    return b + sizeof(SNode) * (size_t)chunk_nodes;
+}
+
+__global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, int n_req, int max_fp) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_req;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = n_req - 1;   // last request whose first block is <= blockIdx.x
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].blk0 <= (int)blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        s_req = lo;
+    }
+    __syncthreads();
+    const SReq R = reqs[s_req];
+    const int r0 = ((int)blockIdx.x - R.blk0) * SERVE_ROWS;
+    const int nr = min(SERVE_ROWS, R.n - r0);
+    unsigned char* lbins = smem;
+    unsigned char* lleaf = smem + (size_t)SERVE_ROWS * max_fp;
+    int* loff = (int*)(lleaf + SERVE_TC * SERVE_ROWS);
+    SNode* lnodes = (SNode*)(smem + align_up((size_t)SERVE_ROWS * max_fp + SERVE_TC * SERVE_ROWS +
+                                                 sizeof(int) * (SERVE_TC + 1), 8));
+    const int q = R.fp / 16;   // 16-byte words per row
+    for (int j = threadIdx.x; j < nr * q; j += blockDim.x) {
+        const int r = j / q, c = j % q;
+        const int row = R.rows ? R.rows[r0 + r] : r0 + r;
+        ((uint4*)lbins)[r * q + c] = ((const uint4*)(R.bins + (size_t)row * R.fp))[c];
+    }
+    const int r = threadIdx.x % SERVE_ROWS, g = threadIdx.x / SERVE_ROWS;
+    const unsigned char* row = lbins + r * R.fp;
+    double s = R.init;
+    for (int t0 = 0; t0 < R.n_trees; t0 += SERVE_TC) {
+        const int nt = min(SERVE_TC, R.n_trees - t0);
+        const int base = R.tree_off[t0];
+        const int nn = R.tree_off[t0 + nt] - base;
+        __syncthreads();   // previous chunk fully consumed
+        for (int j = threadIdx.x; j <= nt; j += blockDim.x) loff[j] = R.tree_off[t0 + j] - base;
+        for (int j = threadIdx.x; j < nn; j += blockDim.x)
+            ((uint2*)lnodes)[j] = ((const uint2*)(R.nodes + base))[j];
+        __syncthreads();
+        if (r < nr) {
+            for (int t = g; t < nt; t += SERVE_LANES) {
+                const SNode* tn = lnodes + loff[t];
+                int k = 0;
+                SNode nd = tn[0];
+                while (!(nd.flags & 1)) {
+                    const int b1 = row[nd.f];
+                    bool left;
+                    if (((nd.flags >> 1) & 3) == 1) {
+                        const int b2 = row[nd.f2];
+                        if (b1 == 0 || b2 == 0) left = (nd.flags >> 3) & 1;
+                        else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)R.nb) <= (int)nd.thr;
+                    } else {
+                        left = b1 == 0 ? ((nd.flags >> 3) & 1) : (b1 <= (int)nd.thr);
+                    }
+                    k = left ? nd.left : nd.right;
+                    nd = tn[k];
+                }
+                lleaf[t * SERVE_ROWS + r] = (unsigned char)k;
+            }
+        }
+        __syncthreads();
+        if (g == 0 && r < nr) {
+            for (int t = 0; t < nt; ++t) s += R.lr[t0 + t] * R.values[base + loff[t] + lleaf[t * SERVE_ROWS + r]];
+        }
+    }
+    if (g == 0 && r < nr) R.out[r0 + r] = s;
+}
+
+template <class Fn>
+int sguard(Fn&& fn) {
+    try {
+        fn();
+        return 0;
+    } catch (const std::exception& e) {
+        g_dev_err = e.what();
+        return -1;
+    }
+}
+
+size_t item_bytes(const tgbm_forest* f, int from_x, int n) {
+    size_t b = align_up(sizeof(double) * (size_t)n, 256);                 // scores
+    if (from_x) b += align_up(sizeof(double) * (size_t)n * f->F, 256) + align_up((size_t)n * f->fp, 256);
+    else b += align_up(sizeof(int) * (size_t)n, 256);                     // row ids
+    return b;
+}
+
+}  // namespace
+
+struct tgbm_predictor {
+    hipStream_t stream = nullptr;
+    char* dev = nullptr;
+    char* host = nullptr;   // pinned
+    size_t cap = 0;
+    int lds_ok = 0;
+    ~tgbm_predictor() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (dev) (void)hipFree(dev);
+        if (host) (void)hipHostFree(host);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+size_t tgbm_predict_bytes(const tgbm_forest* f, int from_x, int n) {
+    return item_bytes(f, from_x, n) + align_up(sizeof(SReq), 256);
+}
+
+int tgbm_predictor_create(tgbm_predictor** out, size_t staging_bytes) {
+    *out = nullptr;
+    return sguard([&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw std::runtime_error("GPU device not available: no HIP device found");
+        auto* p = new tgbm_predictor();
+        std::unique_ptr<tgbm_predictor> guard(p);
+        p->cap = std::max<size_t>(staging_bytes, 1 << 20);
+        SHIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        SHIP(hipMalloc(&p->dev, p->cap));
+        SHIP(hipHostMalloc((void**)&p->host, p->cap, hipHostMallocDefault));
+        SHIP(hipFuncSetAttribute((const void*)k_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SERVE_MAX_LDS));
+        *out = guard.release();
+    });
+}
+
+void tgbm_predictor_destroy(tgbm_predictor* p) { delete p; }
+
+int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_predict_item* items, int n_items) {
+    return sguard([&] {
+        if (n_items <= 0) return;
+        const unsigned char* pbins = nullptr;
+        int pfp = 0, pn = 0, pF = 0;
+        if (panel) session_panel_info(panel, &pbins, &pfp, &pn, &pF);
+        // ---- layout: request table, every item's scores, then every item's inputs (row ids, or
+        // raw rows and their bins), so the scores come back in one contiguous copy
+        const size_t table = align_up(sizeof(SReq) * (size_t)n_items, 256);
+        size_t total = table;
+        std::vector<size_t> off(n_items), in_off(n_items);
+        for (int i = 0; i < n_items; ++i) {
+            off[i] = total;
+            total += align_up(sizeof(double) * (size_t)std::max(items[i].n, 0), 256);
+        }
+        const size_t scores_end = total;
+        int max_fp = 16, max_chunk = 1;
+        for (int i = 0; i < n_items; ++i) {
+            const tgbm_predict_item& it = items[i];
+            const tgbm_forest* f = it.forest;
+            if (!f) throw std::runtime_error("predict item without an ensemble");
+            if (it.n < 0) throw std::runtime_error("predict item with n < 0");
+            const int from_x = it.rows ? 0 : 1;
+            if (!from_x) {
+                if (!panel) throw std::runtime_error("row ids need a resident panel");
+                if (f->F != pF) throw std::runtime_error("ensemble and panel feature counts differ");
+                for (int k = 0; k < it.n; ++k)
+                    if (it.rows[k] < 0 || it.rows[k] >= pn) throw std::runtime_error("panel row id out of range");
+            } else if (!it.X && it.n > 0) {
+                throw std::runtime_error("predict item needs row ids or feature rows");
+            }
+            in_off[i] = total;
+            total += item_bytes(f, from_x, it.n) - align_up(sizeof(double) * (size_t)it.n, 256);
+            max_fp = std::max(max_fp, f->fp);
+            max_chunk = std::max(max_chunk, f->max_chunk_nodes);
+        }
+        if (total > p->cap) throw std::runtime_error("predict batch exceeds the predictor's staging memory");
+        const size_t lds = serve_lds(max_fp, max_chunk);
+        if (lds > SERVE_MAX_LDS) throw std::runtime_error("ensemble chunk too large for LDS");
+        // ---- pack inputs on the host (pinned), one upload
+        std::vector<SReq> reqs(n_items);
+        int blocks = 0;
+        size_t upload_end = table;
+        for (int i = 0; i < n_items; ++i) {
+            const tgbm_predict_item& it = items[i];
+            const tgbm_forest* f = it.forest;
+            SReq& R = reqs[i];
+            R.nodes = f->snodes;
+            R.tree_off = f->tree_off;
+            R.values = f->values;
+            R.lr = f->lr;
+            R.init = f->init;
+            R.n_trees = f->n_trees;
+            R.nb = f->nb;
+            R.fp = f->fp;
+            R.n = it.n;
+            R.blk0 = blocks;
+            R.pad = 0;
+            blocks += (it.n + SERVE_ROWS - 1) / SERVE_ROWS;
+            R.out = (double*)(p->dev + off[i]);
+            size_t o = in_off[i];
+            if (it.rows) {
+                std::memcpy(p->host + o, it.rows, sizeof(int) * (size_t)it.n);
+                R.rows = (const int*)(p->dev + o);
+                R.bins = pbins;
+                o += align_up(sizeof(int) * (size_t)it.n, 256);
+            } else {
+                std::memcpy(p->host + o, it.X, sizeof(double) * (size_t)it.n * f->F);
+                o += align_up(sizeof(double) * (size_t)it.n * f->F, 256);
+                R.rows = nullptr;
+                R.bins = (const unsigned char*)(p->dev + o);
+                o += align_up((size_t)it.n * f->fp, 256);
+            }
+            upload_end = std::max(upload_end, o);
+        }
+        std::memcpy(p->host, reqs.data(), sizeof(SReq) * n_items);
+        SHIP(hipMemcpyAsync(p->dev, p->host, upload_end, hipMemcpyHostToDevice, p->stream));
+        for (int i = 0; i < n_items; ++i) {
+            const tgbm_predict_item& it = items[i];
+            if (it.rows || it.n == 0) continue;
+            const tgbm_forest* f = it.forest;
+            const double* Xd = (const double*)(p->dev + in_off[i]);
+            launch_bin(p->stream, Xd, it.n, f->F, f->edges, f->n_edges, (unsigned char*)reqs[i].bins, f->fp, nullptr,
+                       it.n + 64);
+        }
+        if (blocks > 0)
+            hipLaunchKernelGGL(k_serve, dim3(blocks), dim3(256), lds, p->stream, (const SReq*)p->dev, n_items, max_fp);
+        SHIP(hipGetLastError());
+        // ---- scores back: one copy over the span of all score regions
+        SHIP(hipMemcpyAsync(p->host + table, p->dev + table, scores_end - table, hipMemcpyDeviceToHost, p->stream));
+        SHIP(hipStreamSynchronize(p->stream));
+        for (int i = 0; i < n_items; ++i)
+            if (items[i].n > 0) std::memcpy(items[i].out, p->host + off[i], sizeof(double) * (size_t)items[i].n);
+    });
+}
+
+}  // extern "C"

@@ -50,8 +50,15 @@ static T* dalloc(size_t n) {
 
 struct TaskHost {
     int n_rows = 0;
+    int row0 = 0;
+    int cap_rows = 0;
     int n_buckets = 1;
     std::vector<TorbChunk> chunks;
+    TorbChunk* chunks_d = nullptr;   // device copy of chunks (capacity chunks_cap)
+    int chunks_cap = 0;
+    double* w_d = nullptr;           // weights (allocated on first use, cap_rows)
+    int* gstart_d = nullptr;         // pairwise groups (allocated on first use)
+    int* grows_d = nullptr;
     bool has_sample = false;
     int n_sample = 0;
     bool goss = false;
@@ -158,6 +165,14 @@ struct tgbm_session {
         for (int t = a0; t < a0 + an; ++t) m = std::max(m, th[t].n_rows);
         return m;
     }
+    // Grid sizing row count of the split loop: the largest active sample rounded up to a power of
+    // two, so captured loops are shared by every task mix of similar size (surplus blocks exit).
+    int launch_rows() const {
+        const int m = max_sample();
+        int r = 1;
+        while (r < m && r < (1 << 30)) r <<= 1;
+        return r;
+    }
     int max_sample() const {
         int m = 0;
         for (int t = a0; t < a0 + an; ++t) m = std::max(m, th[t].has_sample ? th[t].n_sample : th[t].n_rows);
@@ -167,6 +182,27 @@ struct tgbm_session {
         for (int t = a0; t < a0 + an; ++t)
             if (th[t].has_sample) return false;
         return true;
+    }
+    // the column root kernel loads 16 rows per 128-bit access: every window must start aligned
+    bool all_aligned() const {
+        for (int t = a0; t < a0 + an; ++t)
+            if (th[t].row0 % 16) return false;
+        return true;
+    }
+    // Upload task t's TORB chunk list and its entry in the per-task pointer table.
+    void upload_chunks(int t) {
+        TaskHost& H = th[t];
+        const int m = std::max<int>(1, (int)H.chunks.size());
+        if (m > H.chunks_cap) {
+            H.chunks_d = alloc<TorbChunk>(m);
+            H.chunks_cap = m;
+        }
+        HIP_OK(hipMemcpy(H.chunks_d, H.chunks.data(), sizeof(TorbChunk) * H.chunks.size(), hipMemcpyHostToDevice));
+        const int cnt = (int)H.chunks.size();
+        HIP_OK(hipMemcpy(chunk_ptrs_d + t, &H.chunks_d, sizeof(TorbChunk*), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(n_chunks_d + t, &cnt, sizeof(int), hipMemcpyHostToDevice));
+        max_chunks = 1;
+        for (auto& x : th) max_chunks = std::max(max_chunks, (int)x.chunks.size());
     }
     // -------- timers
     void tic(int family, double bytes, EventPair& ep) {
@@ -195,6 +231,15 @@ struct tgbm_session {
         events.clear();
     }
 };
+
+namespace tgbm {
+void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, int* fp, int* n, int* F) {
+    *bins_row = s->bins_row;
+    *fp = s->fp;
+    *n = s->n;
+    *F = s->F;
+}
+}  // namespace tgbm
 
 extern "C" {
 
@@ -258,7 +303,10 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             Task& T = s->tasks_h[t];
             std::memset(&T, 0, sizeof T);
             s->th[t].n_rows = m;
+            s->th[t].cap_rows = m;
             T.n_rows = m;
+            T.row0 = 0;
+            T.cap_rows = m;
             T.n_buckets = 1;
             T.hconst = s->hconst;
             T.n_sample = m;
@@ -301,6 +349,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         HIP_OK(hipMemcpyAsync(s->tasks_d, s->tasks_h.data(), sizeof(Task) * n_tasks, hipMemcpyHostToDevice, s->stream));
         s->chunk_ptrs_d = s->alloc<TorbChunk*>(n_tasks);
         s->n_chunks_d = s->alloc<int>(n_tasks);
+        for (int t = 0; t < n_tasks; ++t) s->upload_chunks(t);
         s->pair_f_d = s->alloc<int>(2 * 64);
         s->ph_g = s->alloc<long long>(64 * NBIN);
         s->ph_c = s->alloc<int>(64 * NBIN);
@@ -355,6 +404,53 @@ int tgbm_session_set_active(tgbm_session* s, int first, int count) {
     });
 }
 
+int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows) {
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        TaskHost& H = s->th[task];
+        if (n_rows < 1 || n_rows > H.cap_rows) throw std::runtime_error("task rows exceed the task's capacity");
+        if (row0 < 0 || (long long)row0 + n_rows > s->n) throw std::runtime_error("task row window outside the shared matrix");
+        HIP_OK(hipStreamSynchronize(s->stream));
+        H.n_rows = n_rows;
+        H.row0 = row0;
+        H.has_sample = false;
+        H.n_sample = n_rows;
+        H.goss = false;
+        H.n_buckets = 1;
+        H.chunks.assign(1, TorbChunk{0, n_rows, 1, 0});
+        Task& T = s->tasks_h[task];
+        T.n_rows = n_rows;
+        T.row0 = row0;
+        struct {
+            int n_rows, row0;
+        } win{n_rows, row0};
+        static_assert(offsetof(Task, row0) - offsetof(Task, n_rows) == 4, "layout");
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(s->tasks_d + task) + offsetof(Task, n_rows), &win, sizeof win,
+                              hipMemcpyHostToDevice, s->stream));
+        s->set_field_int(task, offsetof(Task, n_sample), n_rows);
+        s->set_field_int(task, offsetof(Task, contiguous), 1);
+        s->set_field_int(task, offsetof(Task, goss), 0);
+        s->set_field_int(task, offsetof(Task, n_buckets), 1);
+        HIP_OK(hipMemsetAsync((void*)T.bucket, 0, sizeof(int) * n_rows, s->stream));
+        s->upload_chunks(task);
+    });
+}
+
+int tgbm_task_swap(tgbm_session* s, int a, int b) {
+    return dev_guard([&] {
+        if (a < 0 || a >= s->T || b < 0 || b >= s->T) throw std::runtime_error("task out of range");
+        if (a == b) return;
+        HIP_OK(hipStreamSynchronize(s->stream));
+        Task ta = s->read_task(a), tb = s->read_task(b);
+        HIP_OK(hipMemcpy(s->tasks_d + a, &tb, sizeof(Task), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->tasks_d + b, &ta, sizeof(Task), hipMemcpyHostToDevice));
+        std::swap(s->tasks_h[a], s->tasks_h[b]);
+        std::swap(s->th[a], s->th[b]);
+        s->upload_chunks(a);
+        s->upload_chunks(b);
+    });
+}
+
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out) {
     return dev_guard([&] {
         std::vector<unsigned char> tmp((size_t)s->n * s->fp);
@@ -380,10 +476,13 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
                 for (int i = 0; i < m; ++i)
                     if (w[i] != 1.0) throw std::runtime_error("session declared h == 1 but weights are not all 1");
             }
-            double* wd = s->alloc<double>(m);
-            HIP_OK(hipMemcpy(wd, w, sizeof(double) * m, hipMemcpyHostToDevice));
-            T.w = wd;
-            s->set_field_ptr(task, offsetof(Task, w), wd);
+            if (!H.w_d) H.w_d = s->alloc<double>(H.cap_rows);
+            HIP_OK(hipMemcpy(H.w_d, w, sizeof(double) * m, hipMemcpyHostToDevice));
+            T.w = H.w_d;
+            s->set_field_ptr(task, offsetof(Task, w), H.w_d);
+        } else if (T.w) {
+            T.w = nullptr;
+            s->set_field_ptr(task, offsetof(Task, w), nullptr);
         }
         if (bucket) HIP_OK(hipMemcpy((void*)T.bucket, bucket, sizeof(int) * m, hipMemcpyHostToDevice));
         H.n_buckets = n_buckets;
@@ -418,32 +517,24 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
                 i = j;
             }
             gs.push_back(m);
-            int* gsd = s->alloc<int>(gs.size());
+            if (!H.gstart_d) H.gstart_d = s->alloc<int>((size_t)H.cap_rows + 1);
+            int* gsd = H.gstart_d;
             HIP_OK(hipMemcpy(gsd, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice));
             s->set_field_ptr(task, offsetof(Task, group_start), gsd);
             s->set_field_int(task, offsetof(Task, n_groups), (int)gs.size() - 1);
             s->set_field_int(task, offsetof(Task, has_groups), 1);
             if (!sorted) {
-                int* grd = s->alloc<int>(m);
-                HIP_OK(hipMemcpy(grd, ord.data(), sizeof(int) * m, hipMemcpyHostToDevice));
-                s->set_field_ptr(task, offsetof(Task, group_rows), grd);
+                if (!H.grows_d) H.grows_d = s->alloc<int>(H.cap_rows);
+                HIP_OK(hipMemcpy(H.grows_d, ord.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+                s->set_field_ptr(task, offsetof(Task, group_rows), H.grows_d);
+            } else {
+                s->set_field_ptr(task, offsetof(Task, group_rows), nullptr);
             }
             H.max_group = mx;
+        } else {
+            s->set_field_int(task, offsetof(Task, has_groups), 0);
         }
-        // upload the chunk lists of all tasks
-        std::vector<TorbChunk*> ptrs(s->T);
-        std::vector<int> counts(s->T);
-        s->max_chunks = 1;
-        for (int t = 0; t < s->T; ++t) {
-            auto& ch = s->th[t].chunks;
-            TorbChunk* d = s->alloc<TorbChunk>(ch.size());
-            HIP_OK(hipMemcpy(d, ch.data(), sizeof(TorbChunk) * ch.size(), hipMemcpyHostToDevice));
-            ptrs[t] = d;
-            counts[t] = (int)ch.size();
-            s->max_chunks = std::max(s->max_chunks, counts[t]);
-        }
-        HIP_OK(hipMemcpy(s->chunk_ptrs_d, ptrs.data(), sizeof(TorbChunk*) * s->T, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(s->n_chunks_d, counts.data(), sizeof(int) * s->T, hipMemcpyHostToDevice));
+        s->upload_chunks(task);
     });
 }
 
@@ -540,7 +631,8 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
         }
         s->tic(0, bytes, ep);
     }
-    launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous(), s->hconst, ms, p->hist_mode);
+    launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous() && s->all_aligned(), s->hconst, ms,
+                p->hist_mode);
     if (s->timing) s->toc(ep);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 0, 0);
     if (device_mode) launch_select(s->stream, s->td(), s->an, s->F, P);
@@ -553,7 +645,7 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
 // children (device mode: the scan's last block selects the next split; host mode: the host
 // applied its own decision first).
 static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
-    const int ms = s->max_sample();
+    const int ms = s->launch_rows();
     if (mode == 1) launch_apply(s->stream, s->td(), s->an, s->F, P);
     EventPair e1, e2;
     if (s->timing) s->tic(1, 0.0, e1);
@@ -580,7 +672,7 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
         }
         // The loop's kernels read every decision from device memory, so one capture serves
         // every tree with the same sample size and leaf budget.
-        const std::vector<long long> key{s->max_sample(), p->max_leaves, p->hist_mode, s->a0, s->an};
+        const std::vector<long long> key{s->launch_rows(), p->max_leaves, p->hist_mode, s->a0, s->an};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
             if (s->graphs.size() >= 256) {  // bound the cache; captures are cheap to redo

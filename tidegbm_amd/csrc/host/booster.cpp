@@ -424,15 +424,14 @@ void Booster::drop_forest() {
     forest_trees_ = -1;
 }
 
-void Booster::ensure_forest() {
-    const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : cfg.n_estimators);
-    if (forest_ && forest_trees_ == use) return;
-    drop_forest();
+tgbm_forest* make_forest(const Booster& b) {
+    const int use = std::min((int)b.trees.size(), b.best_iteration > 0 ? b.best_iteration : b.cfg.n_estimators);
+    const int F = b.n_features;
     std::vector<int> tn(use);
     std::vector<int> ints;
-    std::vector<double> vals, lrs(use, cfg.learning_rate);
+    std::vector<double> vals, lrs(use, b.cfg.learning_rate);
     for (int t = 0; t < use; ++t) {
-        const Tree& tr = trees[t];
+        const Tree& tr = b.trees[t];
         tn[t] = (int)tr.nodes.size();
         for (size_t k = 0; k < tr.nodes.size(); ++k) {
             const Node& d = tr.nodes[k];
@@ -442,14 +441,38 @@ void Booster::ensure_forest() {
             vals.push_back(tr.values.empty() ? d.value : tr.values[k]);
         }
     }
-    std::vector<double> E((size_t)n_features * 256, 0.0);
-    std::vector<int> ne(n_features, 0);
-    for (int f = 0; f < n_features; ++f) {
-        ne[f] = (int)edges[f].size();
-        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+    std::vector<double> E((size_t)F * 256, 0.0);
+    std::vector<int> ne(F, 0);
+    for (int f = 0; f < F; ++f) {
+        ne[f] = (int)b.edges[f].size();
+        std::copy(b.edges[f].begin(), b.edges[f].end(), E.begin() + (size_t)f * 256);
     }
-    dev_check(tgbm_forest_create(&forest_, n_features, n_bins, use, tn.data(), ints.data(), vals.data(),
-                                 lrs.data(), init_score, E.data(), ne.data()));
+    tgbm_forest* out = nullptr;
+    dev_check(tgbm_forest_create(&out, F, b.n_bins, use, tn.data(), ints.data(), vals.data(), lrs.data(), b.init_score,
+                                 E.data(), ne.data()));
+    return out;
+}
+
+std::shared_ptr<Booster> Booster::clone_model() const {
+    auto m = std::make_shared<Booster>();
+    m->cfg = cfg;
+    m->n_features = n_features;
+    m->n_bins = n_bins;
+    m->n_time_buckets = n_time_buckets;
+    m->best_iteration = best_iteration;
+    m->init_score = init_score;
+    m->init_prefix = init_prefix;
+    m->edges = edges;
+    m->trees = trees;
+    m->fitted = fitted;
+    return m;
+}
+
+void Booster::ensure_forest() {
+    const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : cfg.n_estimators);
+    if (forest_ && forest_trees_ == use) return;
+    drop_forest();
+    forest_ = make_forest(*this);
     forest_trees_ = use;
 }
 

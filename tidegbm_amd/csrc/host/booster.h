@@ -96,6 +96,9 @@ void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vect
 Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov,
                  const double* pref, int n_buckets);
 tgbm_grow_params grow_params(const Config& c);
+class Booster;
+// Device-resident forest of a fitted model's first best_iteration trees (lr per tree = cfg lr).
+tgbm_forest* make_forest(const Booster& b);
 void dev_check(int rc);
 double now_s();
 
@@ -127,6 +130,9 @@ public:
     void update(const double* X, const double* y, int n, int nf, const double* w, int extra_trees);
     void predict_raw(const double* X, int n, int nf, double* out);
     void predict_binned_u16(const unsigned short* xb, int n, double* out);
+
+    // Copy of the fitted model (configuration, edges, trees, initial scores); no device state.
+    std::shared_ptr<Booster> clone_model() const;
 
     std::string save_binary() const;
     std::string save_json() const;

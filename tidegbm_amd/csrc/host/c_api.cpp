@@ -9,6 +9,7 @@
 
 #include "../../../include/tidegbm_amd.h"
 #include "booster.h"
+#include "server.h"
 
 using tide::Booster;
 
@@ -283,6 +284,78 @@ int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features, con
         std::vector<std::unique_ptr<Booster>> res;
         tide::fit_batch(p->cfg, X, n, n_features, timestamps, entity, n_tasks, task_rows, y, horizons, schedule, res);
         for (int t = 0; t < n_tasks; ++t) out[t] = res[t].release();
+    });
+}
+
+int tide_server_create(TideServer* out, const TideHandle* models, int n_models, const double* panel_X, int n_rows,
+                       int n_features, long long predict_budget_bytes, int refit_slots, int refit_max_rows,
+                       int batched) {
+    return guard([&] {
+        *out = nullptr;
+        if (!models || n_models < 1) throw tide::Error("a server needs at least one ensemble");
+        if (predict_budget_bytes <= 0) throw tide::Error("predict_budget_bytes must be positive");
+        std::vector<const Booster*> ms(n_models);
+        for (int i = 0; i < n_models; ++i) ms[i] = static_cast<const Booster*>(models[i]);
+        *out = new tide::Server(ms, panel_X, n_rows, n_features, (size_t)predict_budget_bytes, refit_slots,
+                                refit_max_rows, batched != 0);
+    });
+}
+
+void tide_server_free(TideServer s) { delete static_cast<tide::Server*>(s); }
+
+long long tide_server_predict(TideServer s, int model, const int* rows, int n) {
+    long long id = -1;
+    if (guard([&] { id = static_cast<tide::Server*>(s)->submit_predict(model, rows, n); }) != 0) return -1;
+    return id;
+}
+
+long long tide_server_predict_x(TideServer s, int model, const double* X, int n) {
+    long long id = -1;
+    if (guard([&] { id = static_cast<tide::Server*>(s)->submit_predict_x(model, X, n); }) != 0) return -1;
+    return id;
+}
+
+long long tide_server_refit(TideServer s, int model, int row0, int n, const double* y, int extra_trees) {
+    long long id = -1;
+    if (guard([&] { id = static_cast<tide::Server*>(s)->submit_refit(model, row0, n, y, extra_trees); }) != 0)
+        return -1;
+    return id;
+}
+
+int tide_server_wait(TideServer s, long long ticket, double* out, int cap, int* version, double* latency_s) {
+    int n = -1;
+    if (guard([&] { n = static_cast<tide::Server*>(s)->wait(ticket, out, cap, version, latency_s); }) != 0) return -1;
+    return n;
+}
+
+int tide_server_model(TideServer s, int model, int version, TideHandle* out, int* parent) {
+    return guard([&] {
+        auto m = static_cast<tide::Server*>(s)->model_version(model, version, parent);
+        Booster* b = new Booster();
+        *out = b;
+        b->cfg = m->cfg;
+        b->n_features = m->n_features;
+        b->n_bins = m->n_bins;
+        b->n_time_buckets = m->n_time_buckets;
+        b->best_iteration = m->best_iteration;
+        b->init_score = m->init_score;
+        b->init_prefix = m->init_prefix;
+        b->edges = m->edges;
+        b->trees = m->trees;
+        b->fitted = m->fitted;
+    });
+}
+
+int tide_server_stats(TideServer s, long long* counts5, double* busy2) {
+    return guard([&] {
+        const tide::ServerStats st = static_cast<tide::Server*>(s)->stats();
+        counts5[0] = st.predict_batches;
+        counts5[1] = st.predict_requests;
+        counts5[2] = st.predict_rows;
+        counts5[3] = st.refit_rounds;
+        counts5[4] = st.refit_requests;
+        busy2[0] = st.predict_busy_s;
+        busy2[1] = st.refit_busy_s;
     });
 }
 
