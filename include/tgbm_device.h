@@ -125,6 +125,10 @@ int tgbm_task_get_pred(tgbm_session* s, int task, double* pred);
 int tgbm_timer_enable(tgbm_session* s, int on);
 int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches,
                     double* bytes);
+/* Per-launch log of a family (launch order): milliseconds and the rows the launch covered
+ * (root: the tree's sample; child histograms: the smaller child, 0 once the tree stopped
+ * growing; -1 where unknown).  Returns the number of launches logged (at most cap copied). */
+int tgbm_timer_log(tgbm_session* s, int which, double* ms, double* rows, int cap);
 
 /* ---------------- predict / serving: resident ensembles ---------------- */
 typedef struct tgbm_forest tgbm_forest;

@@ -69,6 +69,9 @@ int tide_get_init_prefix(TideHandle h, double* out, int cap);
  * root histograms, 5 = child histograms, 1 = partitions, 3 = tree end, 4 = gradients.
  * Collected only when the int key "profile" was set to 1 before the fit. */
 int tide_profile(TideHandle h, int family, double* ms, long long* launches, double* bytes);
+/* Per-launch log of a profiled fit's kernel family: milliseconds and rows per launch, in launch
+ * order.  Returns the number of launches (at most cap copied). */
+int tide_profile_log(TideHandle h, int family, double* ms, double* rows, int cap);
 /* Fit timing breakdown of the last fit: seconds for setup (sort, bins, uploads) and for the
  * boosting loop. */
 int tide_fit_times(TideHandle h, double* setup_s, double* boost_s);

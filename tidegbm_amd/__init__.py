@@ -64,6 +64,7 @@ _sig("tide_get_edges", _I, _P, _I, _P, _I)
 _sig("tide_get_init_prefix", _I, _P, _P, _I)
 _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
+_sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
 _sig("tide_server_create", _I, ctypes.POINTER(_P), _P, _I, _P, _I, _I, _LL, _I, _I, _I)
@@ -225,6 +226,15 @@ class Booster:
         ms, n, b = _D(), _LL(), _D()
         _lib.tide_profile(self._h, family, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(b))
         return ms.value, n.value, b.value
+
+    def profile_log(self, family: int):
+        """(ms per launch, rows per launch) of a profiled fit's kernel family, launch order."""
+        m = _lib.tide_profile_log(self._h, family, None, None, 0)
+        if m < 0:
+            raise TideError(_err())
+        ms, rows = np.empty(m), np.empty(m)
+        _lib.tide_profile_log(self._h, family, _ptr(ms), _ptr(rows), m)
+        return ms, rows
 
     def tree_times(self):
         buf = np.empty(1 << 16)

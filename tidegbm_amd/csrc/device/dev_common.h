@@ -113,6 +113,8 @@ struct Task {
     int cur_start, cur_count, cur_buf, cur_left;
     int small_node, big_node, small_start, small_count, small_buf, pad_c;
     int part_lcnt, part_rcnt;  // rows placed left / right by the running partition
+    int hist_parts;            // partial histograms written by the last histogram launch
+    int pad_h;
     // ---- reductions
     unsigned long long max_g_bits, max_h_bits;
     unsigned long long* red;   // per-block maxima of |g| and |h| [2][MAX_RED_BLOCKS]
@@ -149,6 +151,8 @@ struct Task {
     int* hc;               // [MAX_NODES][F][NBIN]
     long long* hh;         // [MAX_NODES][F][NBIN] (only when !hconst)
     long long* energy;     // [MAX_NODES] fixed-point sum of g^2 (host-driven mode)
+    unsigned long long* part;    // [part_cap][F][NBIN] per-block partial histograms (packed)
+    unsigned long long* part_h;  // [part_cap][F][NBIN] partial h sums (only when !hconst)
     // ---- TORB
     const int* cuts;       // [n_buckets]
     long long* tg;         // [(MAX_BUCKETS+1) * MAX_NODES]
@@ -161,6 +165,7 @@ struct Task {
 
 struct Shared {            // the session-wide matrix
     int n_rows, n_features, fp, n_pad, n_bins;
+    int part_cap;          // per-task capacity of per-block partial histograms
     const unsigned char* bins_row;   // [n_rows][fp]
     const unsigned char* bins_col;   // [n_features][n_pad]
 };

@@ -247,6 +247,8 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     T.pending1 = -1;
     T.cur_leaf = -1;
     T.scan_ticket = 0;
+    T.part_lcnt = 0;
+    T.part_rcnt = 0;
     DNode& r = T.nodes[0];
     for (int k = 0; k < 10; ++k) r.v[k] = -1;
     r.v[N_LEAF] = 1;
@@ -280,32 +282,23 @@ __global__ __launch_bounds__(256) void k_pack(Task* tasks) {
     }
 }
 
-// Zero the histogram slots of every node of the tree (once per tree).
-__global__ __launch_bounds__(256) void k_zero_slots(Task* tasks, size_t words) {
-    Task& T = tasks[blockIdx.y];
-    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < words; j += (size_t)gridDim.x * blockDim.x) {
-        T.hg[j] = 0;
-        T.hc[j] = 0;
-        if (!T.hconst) T.hh[j] = 0;
-    }
-}
-
 void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows,
                        int max_nodes) {
     const int rb = std::max(1, std::min(2048, (max_rows + 1023) / 1024));
     hipLaunchKernelGGL(k_tree_init, dim3(n_tasks), dim3(256), 0, st, tasks);
     hipLaunchKernelGGL(k_pack, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
-    const size_t words = (size_t)max_nodes * S.n_features * NBIN;
-    const int zb = (int)std::min<size_t>(2048, (words + 1023) / 1024);
-    hipLaunchKernelGGL(k_zero_slots, dim3(zb, n_tasks), dim3(256), 0, st, tasks, words);
+    (void)S;
+    (void)max_nodes;
 }
 
 // =================================================================== histogram build
+__device__ void apply_nodes(const Task& T, Task* Tw, int nl);
+
 // Which node a histogram launch fills: the root (node 0) or the smaller child of the current
 // split (reference tree.cpp:418-428: left when n_left <= n_right), with its row range.  The
 // children's ranges follow from the partition's left count; block (0,0) of the child launch
 // records them in the node table and the step descriptor for the scan and the tree end.
-__device__ inline bool hist_target(const Task& T, Task* Tw, bool root, int& node, int& start,
+__device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool leader, int& node, int& start,
                                    int& count, int& buf) {
     if (root) {
         node = 0;
@@ -322,15 +315,8 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, int& node
     start = left_small ? st : st + nl;
     count = left_small ? nl : cnt - nl;
     buf = 1 - T.cur_buf;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-        DNode& l = T.nodes[L];
-        DNode& r = T.nodes[R];
-        l.seg_start = st;
-        l.seg_count = nl;
-        l.v[N_NS] = nl;
-        r.seg_start = st + nl;
-        r.seg_count = cnt - nl;
-        r.v[N_NS] = cnt - nl;
+    if (leader && threadIdx.x == 0) {
+        apply_nodes(T, Tw, nl);
         Tw->n_left = nl;
         Tw->small_node = node;
         Tw->big_node = left_small ? R : L;
@@ -341,20 +327,16 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, int& node
     return true;
 }
 
-// Fold a block's LDS histogram into the node's global histogram (exact integer atomics).
+// Store a block's LDS histogram as partial number `slot` of the launch (plain stores; the scan
+// sums the partials exactly, so no global atomics are needed and nothing has to be zeroed).
 template <bool HV>
-__device__ inline void hist_flush(const Task& T, const unsigned long long* lA,
-                                  const unsigned long long* lB, int FB, int f0, int F, int node) {
-    const size_t base = (size_t)node * F * NBIN;
-    for (int j = threadIdx.x; j < FB * NBIN; j += blockDim.x) {
-        const int f = f0 + j / NBIN;
-        if (f >= F) break;
-        const unsigned long long v = lA[j];
-        if (v == 0) continue;
-        const size_t o = base + (size_t)f * NBIN + (j % NBIN);
-        atomicAdd((unsigned long long*)&T.hg[o], (unsigned long long)unpack_q(v));
-        atomicAdd(&T.hc[o], (int)unpack_c(v));
-        if (HV) atomicAdd((unsigned long long*)&T.hh[o], lB[j]);
+__device__ inline void hist_store_partial(const Task& T, const unsigned long long* lA,
+                                          const unsigned long long* lB, int FB, int f0, int F, int slot) {
+    const size_t base = (size_t)slot * F * NBIN + (size_t)f0 * NBIN;
+    const int words = min(FB, F - f0) * NBIN;
+    for (int j = threadIdx.x; j < words; j += blockDim.x) {
+        T.part[base + j] = lA[j];
+        if (HV) T.part_h[base + j] = lB[j];
     }
 }
 
@@ -372,11 +354,13 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
     unsigned long long* lB = lds + FB * NBIN;
     const Task T = tasks[blockIdx.z];
     int node, start, count, buf;
-    if (!hist_target(T, tasks + blockIdx.z, root != 0, node, start, count, buf)) return;
+    if (!hist_target(T, tasks + blockIdx.z, root != 0, blockIdx.x == 0 && blockIdx.y == 0, node, start, count, buf))
+        return;
     int per = (count + gridDim.x - 1) / gridDim.x;
     per = max(per, min_rows);
     per = min((per + 63) & ~63, MAX_FLUSH_ROWS);
     const int r0 = blockIdx.x * per;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) tasks[blockIdx.z].hist_parts = (count + per - 1) / per;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + per);
     const int f0 = blockIdx.y * FB;
@@ -421,7 +405,8 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
         }
     }
     __syncthreads();
-    hist_flush<HV>(T, lA, lB, FB, f0, S.n_features, node);
+    (void)node;
+    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
 }
 
 // Column kernel for the contiguous root: each lane loads 16 consecutive rows of each of the
@@ -435,6 +420,8 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
     const Task T = tasks[blockIdx.z];
     const int count = T.n_sample;
     const int r0 = blockIdx.x * rows_per_block;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+        tasks[blockIdx.z].hist_parts = (count + rows_per_block - 1) / rows_per_block;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + rows_per_block);
     const int f0 = blockIdx.y * FB;
@@ -494,7 +481,7 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
         }
     }
     __syncthreads();
-    hist_flush<HV>(T, lA, lB, FB, f0, S.n_features, 0);
+    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
 }
 
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
@@ -506,21 +493,41 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         const int fbs = (S.n_features + FB - 1) / FB;
         int rpb = (int)(((long long)max_count * fbs / 512 + 4095) / 4096 * 4096);
         rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, rpb));
+        // at most part_cap partials (part_cap >= rows / MAX_FLUSH_ROWS by construction)
+        const int need = (int)(((long long)max_count + S.part_cap - 1) / S.part_cap);
+        if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
         dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
         if (hconst) hipLaunchKernelGGL((k_hist_col<FB, false>), grid, dim3(256), 0, st, S, tasks, rpb);
         else hipLaunchKernelGGL((k_hist_col<FB, true>), grid, dim3(256), 0, st, S, tasks, rpb);
         return;
     }
-    // Row kernel: LPR = 2 (32 features per block), 512 threads.
-    constexpr int LPR = 2, FB = 32;
+    // Row kernel: LPR lanes per row (16 features each), 512 threads.  Default LPR = 1 (16
+    // features per block: more blocks per node at the same partial count) and 1024-row minimum
+    // tiles, the fastest setting over the node sizes of a 1M x 128, 31-leaf fit (per-size sweep:
+    // 11.8 us at 2-8K rows, 13.5 us at 8-32K, 17.1 us at 32-128K, 28.6 us above).  Variants >= 16
+    // pick LPR = 1 << ((v - 16) / 8) and min_rows = 1024 << ((v - 16) % 8).
+    int lpr = 1, min_rows = 1024;
+    if (variant >= 16) {
+        lpr = 1 << std::min(2, (variant - 16) / 8);
+        min_rows = 1024 << std::min(5, (variant - 16) % 8);
+    }
+    if (!hconst && lpr > 2) lpr = 2;   // two LDS words per bin: 64 features would not fit
+    const int FB = 16 * lpr;
     const int fbs = (S.fp + FB - 1) / FB;
-    const int min_rows = 2048;
-    int gx = std::max(1, std::min(512 / fbs, (max_count + min_rows - 1) / min_rows));
+    int gx = std::max(1, std::min(std::max(1, 512 / fbs), (max_count + min_rows - 1) / min_rows));
+    gx = std::min(gx, S.part_cap);
     gx = std::max(gx, (max_count + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS);
     dim3 grid(gx, fbs, n_tasks);
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
-    if (hconst) hipLaunchKernelGGL((k_hist_row<LPR, false>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, min_rows);
-    else hipLaunchKernelGGL((k_hist_row<LPR, true>), grid, dim3(512), lds, st, S, tasks, root ? 1 : 0, min_rows);
+    const int rt = root ? 1 : 0;
+    if (hconst) {
+        if (lpr == 1) hipLaunchKernelGGL((k_hist_row<1, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+        else if (lpr == 2) hipLaunchKernelGGL((k_hist_row<2, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+        else hipLaunchKernelGGL((k_hist_row<4, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+    } else {
+        if (lpr == 1) hipLaunchKernelGGL((k_hist_row<1, true>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+        else hipLaunchKernelGGL((k_hist_row<2, true>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+    }
 }
 
 // =================================================================== split scan
@@ -604,118 +611,164 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
     out_ml = (bkey == 0x7fffffff) ? 1 : (bkey & 1);
 }
 
-// Best-first selection of the next split (tree.cpp:343-392, 445-456), run by one block: reduce
-// the freshly scanned nodes' per-feature results to cached leaf candidates (tree.cpp:160-166:
-// ascending features, skip invalid or gain < min_split_gain, first maximum kept), then pick the
-// leaf with the best candidate (lowest node id on ties; gain must exceed min_split_gain).
-__device__ void select_block(Task& T, int F, const GrowP& P) {
-    __shared__ double sg[16];
-    __shared__ int sk[16];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+// One split decision (the node to split and its predicate).
+struct Decision {
+    int leaf;              // -1: the tree stops growing
+    int type, feat, feat2, thr, mgl;
+    double gain;
+};
+
+// Best-first selection of the next split (tree.cpp:343-392, 445-456), computed by a whole
+// 256-thread block: reduce the last scan's per-feature results of the two pending nodes to their
+// leaf candidates (tree.cpp:160-166: ascending features, skip invalid or gain < min_split_gain,
+// first maximum kept), then pick the leaf with the best candidate (lowest node id on ties; the
+// gain must exceed min_split_gain).  All inputs are loaded up front so their latencies overlap.
+// Every block that calls it reaches the same decision; only a block with `writer` set stores the
+// pending candidates in the candidate cache.
+__device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer, Decision& out) {
+    __shared__ double sg[2][4];
+    __shared__ int sk[2][4], st[2][4], sm[2][4];
+    __shared__ DCand pc[2];
+    __shared__ double lg[4];
+    __shared__ int lk[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pend[2] = {T.pending0, T.pending1};
+    const int nn = T.n_nodes;
+    // inputs of the leaf choice that do not depend on this step's scan
+    const int lid = threadIdx.x;
+    DCand lc;
+    lc.valid = 0;
+    bool lleaf = false;
+    if (lid < nn) {
+        lc = T.cands[lid];
+        lleaf = T.nodes[lid].v[N_LEAF] != 0;
+    }
+    int ndepth[2] = {0, 0}, nns[2] = {0, 0};
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi)
+            if (pend[pi] >= 0) {
+                ndepth[pi] = T.nodes[pend[pi]].v[N_DEPTH];
+                nns[pi] = T.nodes[pend[pi]].v[N_NS];
+            }
+    }
+    double best[2] = {-INFINITY, -INFINITY};
+    int bk[2] = {0x7fffffff, 0x7fffffff}, bt[2] = {-1, -1}, bm[2] = {1, 1};
+#pragma unroll
     for (int pi = 0; pi < 2; ++pi) {
-        const int nd = pi == 0 ? T.pending0 : T.pending1;
+        const int nd = pend[pi];
         if (nd < 0) continue;
-        double best = -INFINITY;
-        int bk = 0x7fffffff;
         for (int f = threadIdx.x; f < F; f += blockDim.x) {
             const FCand c = T.fcand[(size_t)nd * F + f];
             if (c.thr < 0 || c.gain < P.gamma) continue;
-            if (cand_better(c.gain, f, best, bk)) {
-                best = c.gain;
-                bk = f;
+            if (cand_better(c.gain, f, best[pi], bk[pi])) {
+                best[pi] = c.gain;
+                bk[pi] = f;
+                bt[pi] = c.thr;
+                bm[pi] = c.ml;
             }
         }
+    }
+#pragma unroll
+    for (int pi = 0; pi < 2; ++pi) {
         for (int o = 32; o > 0; o >>= 1) {
-            const double og = __shfl_xor(best, o, 64);
-            const int ok = __shfl_xor(bk, o, 64);
-            if (cand_better(og, ok, best, bk)) {
-                best = og;
-                bk = ok;
+            const double og = __shfl_xor(best[pi], o, 64);
+            const int ok = __shfl_xor(bk[pi], o, 64);
+            const int ot = __shfl_xor(bt[pi], o, 64);
+            const int om = __shfl_xor(bm[pi], o, 64);
+            if (cand_better(og, ok, best[pi], bk[pi])) {
+                best[pi] = og;
+                bk[pi] = ok;
+                bt[pi] = ot;
+                bm[pi] = om;
             }
         }
         if (lane == 0) {
-            sg[wave] = best;
-            sk[wave] = bk;
+            sg[pi][wave] = best[pi];
+            sk[pi][wave] = bk[pi];
+            st[pi][wave] = bt[pi];
+            sm[pi][wave] = bm[pi];
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < nw; ++w)
-                if (cand_better(sg[w], sk[w], best, bk)) {
-                    best = sg[w];
-                    bk = sk[w];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int pi = 0; pi < 2; ++pi) {
+            const int nd = pend[pi];
+            if (nd < 0) continue;
+            double b = sg[pi][0];
+            int k = sk[pi][0], t = st[pi][0], m = sm[pi][0];
+            for (int w = 1; w < 4; ++w)
+                if (cand_better(sg[pi][w], sk[pi][w], b, k)) {
+                    b = sg[pi][w];
+                    k = sk[pi][w];
+                    t = st[pi][w];
+                    m = sm[pi][w];
                 }
-            DCand& c = T.cands[nd];
-            const DNode& n = T.nodes[nd];
-            const bool elig = n.v[N_DEPTH] < P.max_depth && n.v[N_NS] >= 2 * P.min_leaf;
-            c.valid = (bk != 0x7fffffff) && elig;
-            c.gain = best;
-            c.feature = bk == 0x7fffffff ? -1 : bk;
+            DCand c;
+            const bool elig = ndepth[pi] < P.max_depth && nns[pi] >= 2 * P.min_leaf;
+            c.valid = (k != 0x7fffffff) && elig;
+            c.gain = b;
+            c.feature = k == 0x7fffffff ? -1 : k;
             c.feature2 = -1;
             c.type = 0;
-            if (bk != 0x7fffffff) {
-                const FCand fc = T.fcand[(size_t)nd * F + bk];
-                c.thr = fc.thr;
-                c.mgl = fc.ml;
-            } else {
-                c.thr = -1;
-                c.mgl = 1;
-            }
+            c.thr = k == 0x7fffffff ? -1 : t;
+            c.mgl = k == 0x7fffffff ? 1 : m;
+            pc[pi] = c;
+            if (writer) T.cands[nd] = c;
         }
-        __syncthreads();
     }
+    __syncthreads();
     // leaf choice: strict '>' over ascending node ids == (gain desc, id asc)
-    double best = -INFINITY;
-    int bid = 0x7fffffff;
-    const int nn = T.n_nodes;
-    for (int lid = threadIdx.x; lid < nn; lid += blockDim.x) {
-        const DCand c = T.cands[lid];
-        if (!T.nodes[lid].v[N_LEAF] || !c.valid || c.feature < 0 || !(c.gain >= P.gamma)) continue;
-        if (!(c.gain > P.gamma)) continue;
-        if (cand_better(c.gain, lid, best, bid)) {
-            best = c.gain;
-            bid = lid;
+    double lb = -INFINITY;
+    int lbid = 0x7fffffff;
+    if (lid < nn && lleaf) {
+        const DCand c = lid == pend[0] ? pc[0] : (lid == pend[1] ? pc[1] : lc);
+        if (c.valid && c.feature >= 0 && c.gain > P.gamma) {
+            lb = c.gain;
+            lbid = lid;
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
-        const double og = __shfl_xor(best, o, 64);
-        const int ok = __shfl_xor(bid, o, 64);
-        if (cand_better(og, ok, best, bid)) {
-            best = og;
-            bid = ok;
+        const double og = __shfl_xor(lb, o, 64);
+        const int ok = __shfl_xor(lbid, o, 64);
+        if (cand_better(og, ok, lb, lbid)) {
+            lb = og;
+            lbid = ok;
         }
     }
-    __syncthreads();
     if (lane == 0) {
-        sg[wave] = best;
-        sk[wave] = bid;
+        lg[wave] = lb;
+        lk[wave] = lbid;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    for (int w = 1; w < nw; ++w)
-        if (cand_better(sg[w], sk[w], best, bid)) {
-            best = sg[w];
-            bid = sk[w];
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w)
+            if (cand_better(lg[w], lk[w], lb, lbid)) {
+                lb = lg[w];
+                lbid = lk[w];
+            }
+        if (T.n_nodes + 2 > MAX_NODES || T.n_nodes + 2 > 2 * P.max_leaves - 1 || lbid == 0x7fffffff) {
+            out.leaf = -1;
+        } else {
+            const DCand c = lbid == pend[0] ? pc[0] : (lbid == pend[1] ? pc[1] : T.cands[lbid]);
+            out.leaf = lbid;
+            out.type = c.type;
+            out.feat = c.feature;
+            out.feat2 = c.feature2;
+            out.thr = c.thr;
+            out.mgl = c.mgl;
+            out.gain = c.gain;
         }
-    if (T.n_nodes + 2 > MAX_NODES || T.n_nodes + 2 > 2 * P.max_leaves - 1 || bid == 0x7fffffff) {
-        T.done = 1;
-        return;
     }
-    const DCand c = T.cands[bid];
-    T.cur_leaf = bid;
-    T.cur_type = c.type;
-    T.cur_feat = c.feature;
-    T.cur_feat2 = c.feature2;
-    T.cur_thr = c.thr;
-    T.cur_mgl = c.mgl;
-    T.cur_gain = c.gain;
+    __syncthreads();
 }
 
-// Split bookkeeping for the decision in T.cur_* (tree.cpp:385-392, 445-456); one thread.
-__device__ void apply_decision(Task& T) {
-    if (T.done) return;
-    const int leaf = T.cur_leaf;
-    DNode& p = T.nodes[leaf];
-    const int L = T.n_nodes, R = T.n_nodes + 1;
+// Node bookkeeping of the split in T.cur_* (tree.cpp:385-392, 445-456): the split node becomes
+// internal, its children get ids L = cur_left and L + 1 with their row ranges; one thread.
+__device__ void apply_nodes(const Task& T, Task* Tw, int nl) {
+    const int L = T.cur_left, R = L + 1;
+    DNode& p = T.nodes[T.cur_leaf];
     p.v[N_LEAF] = 0;
     p.v[N_TYPE] = T.cur_type;
     p.v[N_FEAT] = T.cur_feat;
@@ -725,162 +778,178 @@ __device__ void apply_decision(Task& T) {
     p.v[N_LEFT] = L;
     p.v[N_RIGHT] = R;
     p.gain = T.cur_gain;
-    T.cur_start = p.seg_start;
-    T.cur_count = p.seg_count;
-    T.cur_buf = p.buf;
-    T.cur_left = L;
-    T.part_lcnt = 0;
-    T.part_rcnt = 0;
     for (int side = 0; side < 2; ++side) {
         DNode& c = T.nodes[side == 0 ? L : R];
         for (int k = 0; k < 10; ++k) c.v[k] = -1;
         c.v[N_LEAF] = 1;
         c.v[N_TYPE] = 0;
         c.v[N_MGL] = 1;
-        c.v[N_NS] = 0;
+        c.v[N_NS] = side == 0 ? nl : T.cur_count - nl;
         c.v[N_DEPTH] = p.v[N_DEPTH] + 1;
-        c.seg_start = p.seg_start;
-        c.seg_count = 0;
-        c.buf = 1 - p.buf;
+        c.seg_start = side == 0 ? T.cur_start : T.cur_start + nl;
+        c.seg_count = side == 0 ? nl : T.cur_count - nl;
+        c.buf = 1 - T.cur_buf;
         c.value = 0.0;
         c.gain = 0.0;
         T.cands[side == 0 ? L : R].valid = 0;
     }
-    T.n_nodes += 2;
+    Tw->n_nodes = L + 2;
 }
 
-// mode 0: scan node 0 (root); mode 1: scan both children of the current split.  With
-// fuse_select, the last block to finish (completion ticket) selects and applies the next split.
-constexpr int SCAN_FPB = 4;   // features per block (one per wave)
-__global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode, int fuse_select) {
+// Exact sums, over partials k = k0, k0 + dk, ... < np of the last histogram launch, of bins
+// lane*4 .. lane*4+3 of feature f.  Four partials are loaded per round so their latencies overlap.
+__device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk, int np, long long sq[4],
+                                    long long sc[4], long long sh[4]) {
+    const int lane = threadIdx.x & 63;
+    const size_t stride = (size_t)F * NBIN;
+    const size_t off = (size_t)f * NBIN + lane * 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        sq[t] = 0;
+        sc[t] = 0;
+        sh[t] = 0;
+    }
+    int k = k0;
+    for (; k + 3 * dk < np; k += 4 * dk) {
+        uint4 v[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
+            v[2 * u] = a[0];
+            v[2 * u + 1] = a[1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const unsigned long long w0 = (unsigned long long)v[u].y << 32 | v[u].x;
+            const unsigned long long w1 = (unsigned long long)v[u].w << 32 | v[u].z;
+            const int t = (u & 1) * 2;
+            sq[t] += unpack_q(w0);
+            sc[t] += unpack_c(w0);
+            sq[t + 1] += unpack_q(w1);
+            sc[t + 1] += unpack_c(w1);
+        }
+    }
+    for (; k < np; k += dk) {
+        const unsigned long long* w = T.part + (size_t)k * stride + off;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            sq[t] += unpack_q(w[t]);
+            sc[t] += unpack_c(w[t]);
+        }
+    }
+    if (!T.hconst) {
+        for (int j = k0; j < np; j += dk) {
+            const unsigned long long* w = T.part_h + (size_t)j * stride + off;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) sh[t] += (long long)w[t];
+        }
+    }
+}
+
+// Scan one feature of one node from its sums in registers; lane 0 stores the feature's best
+// split (and, for feature 0, the node value from the node's own sums, tree.cpp:323-328, 407-416).
+__device__ inline void scan_store(const Shared& S, const Task& T, const GrowP& P, int node, int f,
+                                  long long qg[4], int c[4], long long qh[4]) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        if (lane * 4 + t >= S.n_bins) {
+            qg[t] = 0;
+            c[t] = 0;
+            qh[t] = 0;
+        }
+    double gain;
+    int thr, ml;
+    long long tq, tqh, tc;
+    const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
+    scan_feature_wave(qg, c, qh, T.hconst, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf, P.min_hess, P.l1,
+                      gain, thr, ml, tq, tqh, tc);
+    if (lane == 0) {
+        FCand& o = T.fcand[(size_t)node * S.n_features + f];
+        o.gain = gain;
+        o.thr = thr;
+        o.ml = ml;
+        if (f == 0) {
+            const double G = (double)tq * inv_g;
+            const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
+            T.nodes[node].value = leaf_value(G, H, P.l2, P.l1, P.mds);
+        }
+    }
+}
+
+// mode 0: the root (node 0) from the root launch's partials.  mode 1: both children of the
+// current split: the smaller from the child launch's partials, the larger as parent - smaller
+// (exact in fixed point; reference subtract_hist_inplace, histogram.cpp:108-117).  A block owns
+// one feature: its four waves sum interleaved quarters of the partials, then wave 0 stores and
+// scans the smaller child while wave 1 stores and scans the larger.  Both children's histograms
+// are kept for later use as parents.
+__global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode) {
     const Task T = tasks[blockIdx.z];
     Task& Tw = tasks[blockIdx.z];
     if (T.done) return;
-    int node, small = -1, parent = -1;
-    if (mode == 0) {
-        node = 0;
-    } else {
-        parent = T.cur_leaf;
-        small = T.small_node;
-        node = T.cur_left + (int)blockIdx.y;
-    }
     const int F = S.n_features;
     const size_t per = (size_t)F * NBIN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int f = blockIdx.x * SCAN_FPB + wave;
-    if (f < F) {
-        const size_t off = (size_t)node * per + (size_t)f * NBIN;
-        long long qg[4], qh[4];
-        int c[4];
-        const bool sub = (mode == 1 && node != small);
-        const size_t poff = sub ? (size_t)parent * per + (size_t)f * NBIN : 0;
-        const size_t soff = sub ? (size_t)small * per + (size_t)f * NBIN : 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int b = lane * 4 + t;
-            if (sub) {
-                // larger child = parent - smaller, exact in fixed point (reference
-                // subtract_hist_inplace, histogram.cpp:108-117); stored for later use
-                qg[t] = T.hg[poff + b] - T.hg[soff + b];
-                c[t] = T.hc[poff + b] - T.hc[soff + b];
-                qh[t] = T.hconst ? 0 : T.hh[poff + b] - T.hh[soff + b];
-                T.hg[off + b] = qg[t];
-                T.hc[off + b] = c[t];
-                if (!T.hconst) T.hh[off + b] = qh[t];
-            } else {
-                qg[t] = T.hg[off + b];
-                c[t] = T.hc[off + b];
-                qh[t] = T.hconst ? 0 : T.hh[off + b];
-            }
-            if (b >= S.n_bins) {
-                qg[t] = 0;
-                c[t] = 0;
-                qh[t] = 0;
-            }
-        }
-        double gain;
-        int thr, ml;
-        long long tq, tqh, tc;
-        const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
-        scan_feature_wave(qg, c, qh, T.hconst, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf,
-                          P.min_hess, P.l1, gain, thr, ml, tq, tqh, tc);
-        if (lane == 0) {
-            FCand& o = T.fcand[(size_t)node * F + f];
-            o.gain = gain;
-            o.thr = thr;
-            o.ml = ml;
-            if (f == 0) {  // node value from its own sums (tree.cpp:323-328, 407-416)
-                const double G = (double)tq * inv_g;
-                const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
-                T.nodes[node].value = leaf_value(G, H, P.l2, P.l1, P.mds);
-            }
-        }
-    }
-    if (!fuse_select) {
-        if (blockIdx.x == 0 && threadIdx.x == 0 && blockIdx.y == 0) {
-            Tw.pending0 = mode == 0 ? 0 : T.cur_left;
-            Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
-        }
-        return;
-    }
-    // completion ticket: the last block of this task selects the next split
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int total = gridDim.x * gridDim.y;
-        const int t = atomicAdd(&Tw.scan_ticket, 1);
-        s_last = (t == total - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        Tw.scan_ticket = 0;
+    const int f = blockIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         Tw.pending0 = mode == 0 ? 0 : T.cur_left;
         Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
+        Tw.part_lcnt = 0;   // the histogram launch has consumed them; the next partition counts anew
+        Tw.part_rcnt = 0;
+    }
+    __shared__ long long rq[4][NBIN], rh[4][NBIN];
+    __shared__ int rc[4][NBIN];
+    {
+        long long sq[4], sc[4], sh[4];
+        sum_partials(T, F, f, wave, 4, T.hist_parts, sq, sc, sh);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            rq[wave][lane * 4 + t] = sq[t];
+            rc[wave][lane * 4 + t] = (int)sc[t];
+            rh[wave][lane * 4 + t] = sh[t];
+        }
     }
     __syncthreads();
-    select_block(Tw, F, P);
-    if (threadIdx.x == 0) apply_decision(Tw);
-}
-
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
-                 int fuse_select) {
-    dim3 grid((S.n_features + SCAN_FPB - 1) / SCAN_FPB, mode == 0 ? 1 : 2, n_tasks);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, S, tasks, P, mode, fuse_select);
-}
-
-// Device-made decision: reduce the freshly scanned nodes, choose the next split, apply it.
-__global__ __launch_bounds__(256) void k_select(Task* tasks, int F, GrowP P) {
-    Task& T = tasks[blockIdx.x];
-    if (T.done) return;
-    select_block(T, F, P);
-    if (threadIdx.x == 0) apply_decision(T);
-}
-
-void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P) {
-    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, tasks, F, P);
-}
-
-// Host-made decision (PAIR-capable driver): validate and apply.
-__global__ void k_apply(Task* tasks, int F, GrowP P) {
-    Task& T = tasks[blockIdx.x];
-    if (threadIdx.x != 0 || T.done) return;
-    const int leaf = T.cur_leaf;
-    const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
-                     T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
-                     (T.cur_type != 0 && T.cur_type != 1) || T.n_nodes + 2 > MAX_NODES ||
-                     T.n_nodes + 2 > 2 * P.max_leaves - 1;
-    if (bad) {  // a decision that does not name a splittable leaf stops the tree
-        T.done = 1;
-        return;
+    if (wave > 1 || (mode == 0 && wave == 1)) return;
+    long long qg[4], qh[4];
+    int c[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int b = lane * 4 + t;
+        qg[t] = rq[0][b] + rq[1][b] + rq[2][b] + rq[3][b];
+        c[t] = rc[0][b] + rc[1][b] + rc[2][b] + rc[3][b];
+        qh[t] = rh[0][b] + rh[1][b] + rh[2][b] + rh[3][b];
     }
-    apply_decision(T);
+    if (wave == 0) {
+        const int small = mode == 0 ? 0 : T.small_node;
+        const size_t so = (size_t)small * per + (size_t)f * NBIN + lane * 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            T.hg[so + t] = qg[t];
+            T.hc[so + t] = c[t];
+            if (!T.hconst) T.hh[so + t] = qh[t];
+        }
+        scan_store(S, T, P, small, f, qg, c, qh);
+    } else {
+        const int big = T.big_node;
+        const size_t po = (size_t)T.cur_leaf * per + (size_t)f * NBIN + lane * 4;
+        const size_t bo = (size_t)big * per + (size_t)f * NBIN + lane * 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            qg[t] = T.hg[po + t] - qg[t];
+            c[t] = T.hc[po + t] - c[t];
+            qh[t] = T.hconst ? 0 : T.hh[po + t] - qh[t];
+            T.hg[bo + t] = qg[t];
+            T.hc[bo + t] = c[t];
+            if (!T.hconst) T.hh[bo + t] = qh[t];
+        }
+        scan_store(S, T, P, big, f, qg, c, qh);
+    }
 }
 
-void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P) {
-    hipLaunchKernelGGL(k_apply, dim3(n_tasks), dim3(64), 0, st, tasks, F, P);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
+    dim3 grid(S.n_features, 1, n_tasks);
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, S, tasks, P, mode);
 }
 
 // =================================================================== partition
@@ -892,44 +961,78 @@ void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& 
 constexpr int PART_CHUNK = 2048;   // rows per block (8 per thread)
 constexpr int PART_RPT = 8;
 
-__device__ inline bool row_goes_left(const Shared& S, const Task& T, int i) {
-    const int b1 = S.bins_col[(size_t)T.cur_feat * S.n_pad + T.row0 + i];
-    if (T.cur_type == 1) {
-        const int b2 = S.bins_col[(size_t)T.cur_feat2 * S.n_pad + T.row0 + i];
-        if (b1 == 0 || b2 == 0) return T.cur_mgl != 0;
+__device__ inline bool row_goes_left(const Shared& S, const Task& T, const Decision& d, int i) {
+    const int b1 = S.bins_col[(size_t)d.feat * S.n_pad + T.row0 + i];
+    if (d.type == 1) {
+        const int b2 = S.bins_col[(size_t)d.feat2 * S.n_pad + T.row0 + i];
+        if (b1 == 0 || b2 == 0) return d.mgl != 0;
         const unsigned z = ((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)S.n_bins;
-        return (int)z <= T.cur_thr;
+        return (int)z <= d.thr;
     }
-    if (b1 == 0) return T.cur_mgl != 0;
-    return b1 <= T.cur_thr;
+    if (b1 == 0) return d.mgl != 0;
+    return b1 <= d.thr;
 }
 
-__device__ inline int block_sum_int(int v, int* red) {
-    v = wave_sum_i(v);
-    const int wave = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) red[wave] = v;
-    __syncthreads();
-    int s = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-    return s;
-}
-
-// One pass: each block places its rows with one atomic reservation per side; lefts fill the
-// leaf's range from the front, rights from the back.  The order of rows inside a child's range
-// therefore depends on block timing, but nothing downstream depends on it: every histogram,
-// leaf and TORB sum is an exact integer sum over the set of rows.
-__global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks) {
+// Decide the split and partition the split leaf's rows.  mode 0: every block runs the same
+// best-first selection (select_decide), so no separate selection launch is needed; mode 1: the
+// host wrote the decision into T.cur_* (validated here).  Block 0 records the decision and the
+// leaf's row range in the step descriptor; the node table itself is updated by the next
+// (histogram) launch, so no block of this launch can observe a half-applied split.
+//
+// Each block places its rows with one atomic reservation per side: lefts fill the leaf's range
+// from the front, rights from the back.  The order of rows inside a child's range therefore
+// depends on block timing, but nothing downstream depends on it: every histogram, leaf and TORB
+// sum is an exact integer sum over the set of rows.
+__global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP P, int mode) {
     const Task T = tasks[blockIdx.y];
     Task& Tw = tasks[blockIdx.y];
     if (T.done) return;
-    const int cnt = T.cur_count, st = T.cur_start;
+    __shared__ Decision sd;
+    if (mode == 0) {
+        select_decide(T, S.n_features, P, blockIdx.x == 0, sd);
+    } else {
+        if (threadIdx.x == 0) {
+            const int leaf = T.cur_leaf, F = S.n_features;
+            const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
+                             T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
+                             (T.cur_type != 0 && T.cur_type != 1) || T.n_nodes + 2 > MAX_NODES ||
+                             T.n_nodes + 2 > 2 * P.max_leaves - 1;
+            sd.leaf = bad ? -1 : leaf;   // a decision that does not name a splittable leaf stops the tree
+            sd.type = T.cur_type;
+            sd.feat = T.cur_feat;
+            sd.feat2 = T.cur_feat2;
+            sd.thr = T.cur_thr;
+            sd.mgl = T.cur_mgl;
+            sd.gain = T.cur_gain;
+        }
+        __syncthreads();
+    }
+    const Decision d = sd;
+    if (d.leaf < 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) Tw.done = 1;
+        return;
+    }
+    const DNode& lf = T.nodes[d.leaf];
+    const int cnt = lf.seg_count, st = lf.seg_start, lbuf = lf.buf;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Tw.cur_leaf = d.leaf;
+        Tw.cur_type = d.type;
+        Tw.cur_feat = d.feat;
+        Tw.cur_feat2 = d.feat2;
+        Tw.cur_thr = d.thr;
+        Tw.cur_mgl = d.mgl;
+        Tw.cur_gain = d.gain;
+        Tw.cur_start = st;
+        Tw.cur_count = cnt;
+        Tw.cur_buf = lbuf;
+        Tw.cur_left = T.n_nodes;
+    }
     const int c0 = blockIdx.x * PART_CHUNK;
     if (c0 >= cnt) return;
     __shared__ int wsum[4], wsumr[4];
     __shared__ int baseL, baseR;
-    const int* __restrict__ src = (T.cur_buf ? T.idx_tmp : T.idx) + st;
-    int* __restrict__ dst = (T.cur_buf ? T.idx : T.idx_tmp) + st;
+    const int* __restrict__ src = (lbuf ? T.idx_tmp : T.idx) + st;
+    int* __restrict__ dst = (lbuf ? T.idx : T.idx_tmp) + st;
     const int k0 = c0 + threadIdx.x * PART_RPT;
     unsigned flags = 0;
     int rows[PART_RPT];
@@ -939,7 +1042,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks) {
 #pragma unroll
     for (int t = 0; t < PART_RPT; ++t) {
         if (rows[t] < 0) continue;
-        if (row_goes_left(S, T, rows[t])) {
+        if (row_goes_left(S, T, d, rows[t])) {
             flags |= 1u << t;
             ++nl;
         } else {
@@ -978,10 +1081,11 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks) {
     }
 }
 
-void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count) {
+void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
+                      int mode) {
     const int nch = std::max(1, (max_count + PART_CHUNK - 1) / PART_CHUNK);
     dim3 grid(nch, n_tasks);
-    hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, S, tasks);
+    hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, S, tasks, P, mode);
 }
 
 // =================================================================== host-driven helpers
@@ -1335,8 +1439,11 @@ void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout&
 void init_kernel_attributes() {
     static bool done = false;
     if (done) return;
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_hist_row<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_hist_row<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     done = true;
 }

@@ -21,11 +21,10 @@ void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks
                        int max_nodes);
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant);
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
-                 int fuse_select);
-void launch_apply(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P);
-void launch_select(hipStream_t st, Task* tasks, int n_tasks, int F, const GrowP& P);
-void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
+// mode 0: the partition launch selects the split itself; mode 1: the host wrote it in T.cur_*.
+void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
+                      int mode);
 void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count);
 void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int node, int count,
                  const int* pf, int n_pairs, long long* ph_g, int* ph_c, long long* ph_h,

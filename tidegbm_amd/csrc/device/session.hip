@@ -107,6 +107,7 @@ struct tgbm_session {
     double t_ms[8] = {0};
     long long t_n[8] = {0};
     double t_bytes[8] = {0};
+    std::vector<double> log_ms[8], log_rows[8];   // per launch, in launch order
 
     template <class T>
     T* alloc(size_t n) {
@@ -225,6 +226,7 @@ struct tgbm_session {
             t_ms[e.family] += ms;
             t_n[e.family] += 1;
             t_bytes[e.family] += e.bytes;
+            log_ms[e.family].push_back(ms);
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
@@ -294,6 +296,9 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->S.n_bins = n_bins;
         s->S.bins_row = s->bins_row;
         s->S.bins_col = s->bins_col;
+        int max_task_rows = 1;
+        for (int t = 0; t < n_tasks; ++t) max_task_rows = std::max(max_task_rows, task_rows ? task_rows[t] : n_rows);
+        s->S.part_cap = std::max(64, (max_task_rows + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS + 1);
         s->tasks_h.assign(n_tasks, Task{});
         s->th.assign(n_tasks, TaskHost{});
         const size_t per_node = (size_t)n_features * NBIN;
@@ -333,6 +338,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.hc = s->alloc<int>(per_node * s->max_nodes);
             T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * s->max_nodes);
             T.energy = s->alloc<long long>(MAX_NODES);
+            T.part = s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
+            T.part_h = s->hconst ? nullptr : s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
             T.cuts = s->alloc<int>(MAX_BUCKETS);
             const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
             T.tg = s->alloc<long long>(tw);
@@ -634,8 +641,8 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
     launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous() && s->all_aligned(), s->hconst, ms,
                 p->hist_mode);
     if (s->timing) s->toc(ep);
-    launch_scan(s->stream, s->S, s->td(), s->an, P, 0, 0);
-    if (device_mode) launch_select(s->stream, s->td(), s->an, s->F, P);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 0);
+    (void)device_mode;   // the first split is selected by the first partition launch
     if (energy)
         for (int t = s->a0; t < s->a0 + s->an; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
     HIP_OK(hipGetLastError());
@@ -646,16 +653,14 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
 // applied its own decision first).
 static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
     const int ms = s->launch_rows();
-    if (mode == 1) launch_apply(s->stream, s->td(), s->an, s->F, P);
     EventPair e1, e2;
     if (s->timing) s->tic(1, 0.0, e1);
-    launch_partition(s->stream, s->S, s->td(), s->an, ms);
+    launch_partition(s->stream, s->S, s->td(), s->an, ms, P, mode);
     if (s->timing) s->toc(e1);
     if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
     launch_hist(s->stream, s->S, s->td(), s->an, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
     if (s->timing) s->toc(e2);
-    launch_scan(s->stream, s->S, s->td(), s->an, P, 1, 0);
-    if (mode == 0) launch_select(s->stream, s->td(), s->an, s->F, P);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 1);
     HIP_OK(hipGetLastError());
 }
 
@@ -832,7 +837,14 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
             std::memcpy(values + (size_t)i * MN, base + L.off_values, sizeof(double) * nn);
             std::memcpy(cover + (size_t)i * MN, base + L.off_cover, sizeof(int) * nn);
             std::memcpy(prefix + (size_t)i * MN * MAX_BUCKETS, base + L.off_prefix, sizeof(double) * nn * B);
-            if (s->timing) {  // bytes of the child histogram builds of this tree
+            if (s->timing) {  // rows of the root and of every child histogram build, in launch order
+                const int steps = p->max_leaves - 1;
+                s->log_rows[0].push_back(nodes[0].seg_count);
+                for (int k = 0; k < steps; ++k) {
+                    const bool grew = 2 * k + 2 < nn;
+                    s->log_rows[5].push_back(grew ? std::min(nodes[2 * k + 1].seg_count, nodes[2 * k + 2].seg_count) : 0);
+                    s->log_rows[1].push_back(grew ? nodes[2 * k + 1].seg_count + nodes[2 * k + 2].seg_count : 0);
+                }
                 for (int k = 0; k < nn; ++k) {
                     if (nodes[k].v[N_LEAF]) continue;
                     const int Lc = nodes[k].v[N_LEFT], Rc = nodes[k].v[N_RIGHT];
@@ -852,8 +864,25 @@ int tgbm_timer_enable(tgbm_session* s, int on) {
             s->t_ms[k] = 0;
             s->t_n[k] = 0;
             s->t_bytes[k] = 0;
+            s->log_ms[k].clear();
+            s->log_rows[k].clear();
         }
     });
+}
+
+int tgbm_timer_log(tgbm_session* s, int which, double* ms, double* rows, int cap) {
+    int n = -1;
+    if (dev_guard([&] {
+            if (which < 0 || which >= 8) throw std::runtime_error("timer index");
+            s->drain_events();
+            n = (int)s->log_ms[which].size();
+            for (int i = 0; i < n && i < cap; ++i) {
+                ms[i] = s->log_ms[which][i];
+                rows[i] = i < (int)s->log_rows[which].size() ? s->log_rows[which][i] : -1.0;
+            }
+        }) != 0)
+        return -1;
+    return n;
 }
 
 int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches, double* bytes) {

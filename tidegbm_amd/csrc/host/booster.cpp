@@ -574,8 +574,16 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     const double t1 = now_s();
     std::mt19937 rng((uint32_t)cfg.random_state);
     grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, tl.cuts, rng, goss_active, nullptr, false);
-    if (cfg.profile)
-        for (int k = 0; k < 8; ++k) dev_check(tgbm_timer_read(sg.s, k, &prof_ms[k], &prof_n[k], &prof_bytes[k]));
+    if (cfg.profile) {
+        for (int k = 0; k < 8; ++k) {
+            dev_check(tgbm_timer_read(sg.s, k, &prof_ms[k], &prof_n[k], &prof_bytes[k]));
+            const int m = tgbm_timer_log(sg.s, k, nullptr, nullptr, 0);
+            if (m < 0) dev_check(-1);
+            prof_log_ms[k].resize(m);
+            prof_log_rows[k].resize(m);
+            if (m > 0) tgbm_timer_log(sg.s, k, prof_log_ms[k].data(), prof_log_rows[k].data(), m);
+        }
+    }
     best_iteration = cfg.n_estimators;
     fitted = true;
     t_boost = now_s() - t1;

@@ -119,6 +119,7 @@ public:
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
     double prof_ms[8] = {0}, prof_bytes[8] = {0};
     long long prof_n[8] = {0};
+    std::vector<double> prof_log_ms[8], prof_log_rows[8];
 
     ~Booster();
     Booster() = default;

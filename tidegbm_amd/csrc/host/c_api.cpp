@@ -268,6 +268,20 @@ int tide_tree_times(TideHandle h, double* out, int cap) {
     return m;
 }
 
+int tide_profile_log(TideHandle h, int family, double* ms, double* rows, int cap) {
+    auto* b = static_cast<Booster*>(h);
+    if (family < 0 || family >= 8) {
+        g_err = "profile family out of range";
+        return -1;
+    }
+    const int m = (int)b->prof_log_ms[family].size();
+    for (int i = 0; i < m && i < cap; ++i) {
+        ms[i] = b->prof_log_ms[family][i];
+        rows[i] = b->prof_log_rows[family][i];
+    }
+    return m;
+}
+
 int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
     auto* b = static_cast<Booster*>(h);
     *setup_s = b->t_setup;
