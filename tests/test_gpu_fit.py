@@ -46,7 +46,10 @@ def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
     _say(f"[case] {name} n={p.n} F={p.n_features} params={params}")
     gpu = _fit_gpu(p, rg=rg, iid=iid, **params)
     orc = _fit_cpu(checkers.oracle(), p, rg=rg, iid=iid, **params)
-    diff = checkers.compare_trees(gpu.trees(), orc.trees(), rtol=VALUE_RTOL, atol=1e-9)
+    # training-row bins: lets the comparison accept the one characterized flag difference
+    xb = checkers.bins_of(p.X, [gpu.edges(f) for f in range(p.n_features)])
+    nb = gpu.model_info()["n_bins"]
+    diff = checkers.compare_trees(gpu.trees(), orc.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb, n_bins=nb)
     assert diff is None, f"{name}: GPU vs oracle: {diff}"
     Xp = p.X[:n_pred]
     rel = _pred_close(gpu.predict(Xp), orc.predict(Xp))
@@ -54,7 +57,7 @@ def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
     ref = checkers.reference()
     if ref is not None:
         r = _fit_cpu(ref, p, rg=rg, iid=iid, **params)
-        diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9)
+        diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb, n_bins=nb)
         assert diff is None, f"{name}: GPU vs reference: {diff}"
     return gpu, orc
 
@@ -142,3 +145,18 @@ def test_graph_replays_match_eager_launches(monkeypatch):
         for k in ("ints", "dbl", "values", "prefix", "cover"):
             assert np.array_equal(x[k], y[k]), f"tree {t} {k}"
     assert np.array_equal(eager.predict(p.X), graph.predict(p.X))
+
+
+def test_missing_values_panel_matches_oracle():
+    # A panel with 1% NaN where a split's node receives no row with its feature missing: the
+    # comparison accepts only the characterized missing_go_left difference (see checkers).
+    p = synth.panel_large(4, n_dates=150, n_entities=120, n_features=20)
+    params = dict(n_estimators=10, max_depth=7, learning_rate=0.1, goss=0, pair_splits=0)
+    _say("[case] NaN panel, seed 4")
+    gpu = _fit_gpu(p, **params)
+    orc = _fit_cpu(checkers.oracle(), p, **params)
+    xb = checkers.bins_of(p.X, [gpu.edges(f) for f in range(p.n_features)])
+    nb = gpu.model_info()["n_bins"]
+    diff = checkers.compare_trees(gpu.trees(), orc.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb, n_bins=nb)
+    assert diff is None, diff
+    assert _pred_close(gpu.predict(p.X), orc.predict(p.X)) < PRED_RTOL

@@ -59,11 +59,13 @@ enum { N_LEAF = 0, N_TYPE, N_FEAT, N_FEAT2, N_THR, N_LEFT, N_RIGHT, N_MGL, N_NS,
 struct FCand {      // best split of one feature of one node
     double gain;
     int thr, ml;
+    int cl, pad;    // rows the split sends left (exact, from the node's histogram counts)
 };
 
 struct DCand {      // cached leaf candidate
     double gain;
     int feature, feature2, thr, type, mgl, valid;
+    int cl, pad;    // rows the split sends left
 };
 
 // Byte offsets of one task's finished tree in the export staging buffer.
@@ -112,7 +114,9 @@ struct Task {
     // (set by the partition), so kernels read one cache line instead of chasing nodes[]
     int cur_start, cur_count, cur_buf, cur_left;
     int small_node, big_node, small_start, small_count, small_buf, pad_c;
-    int part_lcnt, part_rcnt;  // rows placed left / right by the running partition
+    unsigned long long part_lr; // rows placed by the running partition: right << 32 | left
+    int cur_nl;                 // rows the current split sends left (-1: not known in advance)
+    int err;                    // nonzero: an internal consistency check failed in this tree
     int hist_parts;            // partial histograms written by the last histogram launch
     int pad_h;
     // ---- reductions

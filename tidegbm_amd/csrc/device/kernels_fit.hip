@@ -243,12 +243,13 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     T.s_e = QE_BITS - 2 * eg;
     T.n_nodes = 1;
     T.done = 0;
+    T.err = 0;
+    T.cur_nl = -1;
     T.pending0 = 0;
     T.pending1 = -1;
     T.cur_leaf = -1;
     T.scan_ticket = 0;
-    T.part_lcnt = 0;
-    T.part_rcnt = 0;
+    T.part_lr = 0;
     DNode& r = T.nodes[0];
     for (int k = 0; k < 10; ++k) r.v[k] = -1;
     r.v[N_LEAF] = 1;
@@ -308,7 +309,7 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
         return true;
     }
     if (T.done) return false;
-    const int nl = T.part_lcnt, cnt = T.cur_count, st = T.cur_start;
+    const int nl = (int)(T.part_lr & 0xffffffffull), cnt = T.cur_count, st = T.cur_start;
     const int L = T.cur_left, R = T.cur_left + 1;
     const bool left_small = nl <= cnt - nl;
     node = left_small ? L : R;
@@ -316,6 +317,8 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
     count = left_small ? nl : cnt - nl;
     buf = 1 - T.cur_buf;
     if (leader && threadIdx.x == 0) {
+        // the partition must send exactly the rows the chosen split's histogram counts left
+        if (T.cur_nl >= 0 && T.cur_nl != nl) Tw->err = 1;
         apply_nodes(T, Tw, nl);
         Tw->n_left = nl;
         Tw->small_node = node;
@@ -413,18 +416,31 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
 // block's FB feature columns (128-bit loads) plus their 16 packed gradients, then issues the
 // FB*16 LDS atomics back to back.  Blocks cover long row runs so per-block partial histograms
 // stay small next to the bytes read.
+//
+// With xcd_major set, the launch's blocks are renumbered so that the feature blocks of one row
+// chunk run on one XCD (workgroups are dealt to the 8 XCDs round-robin by linear id): the chunk's
+// packed gradients are then fetched into that XCD's L2 once instead of once per feature block.
 template <int FB, bool HV>
-__global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int rows_per_block) {
+__global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int rows_per_block, int xcd_major) {
     __shared__ unsigned long long lA[FB * NBIN];
     __shared__ unsigned long long lB[HV ? FB * NBIN : 1];
     const Task T = tasks[blockIdx.z];
     const int count = T.n_sample;
-    const int r0 = blockIdx.x * rows_per_block;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (xcd_major) {
+        const int nb = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * gridDim.x;
+        const int x = id % 8, l = id / 8;
+        const int base = nb / 8, rem = nb % 8;   // XCD x' holds base + (x' < rem) of the ids
+        const int j = x * base + min(x, rem) + l;
+        bx = j / gridDim.y;
+        by = j % gridDim.y;
+    }
+    const int r0 = bx * rows_per_block;
+    if (bx == 0 && by == 0 && threadIdx.x == 0)
         tasks[blockIdx.z].hist_parts = (count + rows_per_block - 1) / rows_per_block;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + rows_per_block);
-    const int f0 = blockIdx.y * FB;
+    const int f0 = by * FB;
     const int nf = min(FB, S.n_features - f0);
     for (int j = threadIdx.x; j < FB * NBIN; j += blockDim.x) {
         lA[j] = 0;
@@ -481,14 +497,19 @@ __global__ __launch_bounds__(256) void k_hist_col(Shared S, Task* tasks, int row
         }
     }
     __syncthreads();
-    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
+    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, bx);
 }
 
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant) {
     if (max_count <= 0) return;
-    if (root && contiguous_root && variant != 2) {
-        constexpr int FB = 8;
+    if (root && contiguous_root && (variant & 0xff) != 2) {
+        // Root variants (variant >> 8): bit 0 selects 16 features per block (else 8), bit 1 a
+        // 512-thread block (else 256), bit 3 XCD-major block placement.
+        const int rv = variant >> 8;
+        const int FB = (rv & 1) ? 16 : 8;
+        const int threads = 256 << std::min(1, (rv >> 1) & 3);
+        const int xcd = (rv >> 3) & 1;
         // ~2 blocks per CU over the whole matrix; rows per block a multiple of the 4-wave tile
         const int fbs = (S.n_features + FB - 1) / FB;
         int rpb = (int)(((long long)max_count * fbs / 512 + 4095) / 4096 * 4096);
@@ -497,8 +518,13 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         const int need = (int)(((long long)max_count + S.part_cap - 1) / S.part_cap);
         if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
         dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
-        if (hconst) hipLaunchKernelGGL((k_hist_col<FB, false>), grid, dim3(256), 0, st, S, tasks, rpb);
-        else hipLaunchKernelGGL((k_hist_col<FB, true>), grid, dim3(256), 0, st, S, tasks, rpb);
+        if (FB == 8) {
+            if (hconst) hipLaunchKernelGGL((k_hist_col<8, false>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+            else hipLaunchKernelGGL((k_hist_col<8, true>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+        } else {
+            if (hconst) hipLaunchKernelGGL((k_hist_col<16, false>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+            else hipLaunchKernelGGL((k_hist_col<16, true>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+        }
         return;
     }
     // Row kernel: LPR lanes per row (16 features each), 512 threads.  Default LPR = 1 (16
@@ -507,16 +533,19 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     // 11.8 us at 2-8K rows, 13.5 us at 8-32K, 17.1 us at 32-128K, 28.6 us above).  Variants >= 16
     // pick LPR = 1 << ((v - 16) / 8) and min_rows = 1024 << ((v - 16) % 8).
     int lpr = 1, min_rows = 1024;
-    if (variant >= 16) {
-        lpr = 1 << std::min(2, (variant - 16) / 8);
-        min_rows = 1024 << std::min(5, (variant - 16) % 8);
+    const int cv = variant & 0xff;
+    if (cv >= 16) {
+        lpr = 1 << std::min(2, (cv - 16) / 8);
+        min_rows = 1024 << std::min(5, (cv - 16) % 8);
     }
     if (!hconst && lpr > 2) lpr = 2;   // two LDS words per bin: 64 features would not fit
     const int FB = 16 * lpr;
     const int fbs = (S.fp + FB - 1) / FB;
+    // Blocks per feature block: no block folds more than MAX_FLUSH_ROWS rows, and there are at most
+    // part_cap partials (part_cap >= 2 * rows / MAX_FLUSH_ROWS + 1 > max_count / MAX_FLUSH_ROWS).
     int gx = std::max(1, std::min(std::max(1, 512 / fbs), (max_count + min_rows - 1) / min_rows));
-    gx = std::min(gx, S.part_cap);
     gx = std::max(gx, (max_count + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS);
+    gx = std::min(gx, S.part_cap);
     dim3 grid(gx, fbs, n_tasks);
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
     const int rt = root ? 1 : 0;
@@ -540,7 +569,7 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
                                          bool hconst, double inv_g, double inv_h, int nb,
                                          double lam, double gamma, int min_leaf, double min_hess,
                                          double l1, double& out_gain, int& out_thr, int& out_ml,
-                                         long long& totQ, long long& totQh, long long& totC) {
+                                         int& out_cl, long long& totQ, long long& totQh, long long& totC) {
     const int lane = threadIdx.x & 63;
     long long lg = 0, lh = 0;
     long long lc = 0;
@@ -576,6 +605,7 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
     const double parent = leaf_score(tG, tH, lam, l1);
     double best = -INFINITY;
     int bkey = 0x7fffffff;
+    int bcl = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int b = lane * 4 + t;
@@ -595,26 +625,31 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
             if (cand_better(gain, key, best, bkey)) {
                 best = gain;
                 bkey = key;
+                bcl = (int)cl;
             }
         }
     }
     for (int o = 32; o > 0; o >>= 1) {
         const double og = __shfl_xor(best, o, 64);
         const int ok = __shfl_xor(bkey, o, 64);
+        const int oc = __shfl_xor(bcl, o, 64);
         if (cand_better(og, ok, best, bkey)) {
             best = og;
             bkey = ok;
+            bcl = oc;
         }
     }
     out_gain = best;
     out_thr = (bkey == 0x7fffffff) ? -1 : bkey >> 1;
     out_ml = (bkey == 0x7fffffff) ? 1 : (bkey & 1);
+    out_cl = (bkey == 0x7fffffff) ? 0 : bcl;
 }
 
 // One split decision (the node to split and its predicate).
 struct Decision {
     int leaf;              // -1: the tree stops growing
     int type, feat, feat2, thr, mgl;
+    int nl;                // rows the split sends left (from the node's histogram; -1: unknown)
     double gain;
 };
 
@@ -626,12 +661,12 @@ struct Decision {
 // Every block that calls it reaches the same decision; only a block with `writer` set stores the
 // pending candidates in the candidate cache.
 __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer, Decision& out) {
-    __shared__ double sg[2][4];
-    __shared__ int sk[2][4], st[2][4], sm[2][4];
+    __shared__ double sg[2][16];
+    __shared__ int sk[2][16], st[2][16], sm[2][16], scl[2][16];
     __shared__ DCand pc[2];
-    __shared__ double lg[4];
-    __shared__ int lk[4];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double lg[16];
+    __shared__ int lk[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int pend[2] = {T.pending0, T.pending1};
     const int nn = T.n_nodes;
     // inputs of the leaf choice that do not depend on this step's scan
@@ -653,7 +688,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             }
     }
     double best[2] = {-INFINITY, -INFINITY};
-    int bk[2] = {0x7fffffff, 0x7fffffff}, bt[2] = {-1, -1}, bm[2] = {1, 1};
+    int bk[2] = {0x7fffffff, 0x7fffffff}, bt[2] = {-1, -1}, bm[2] = {1, 1}, bcl[2] = {0, 0};
 #pragma unroll
     for (int pi = 0; pi < 2; ++pi) {
         const int nd = pend[pi];
@@ -666,6 +701,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
                 bk[pi] = f;
                 bt[pi] = c.thr;
                 bm[pi] = c.ml;
+                bcl[pi] = c.cl;
             }
         }
     }
@@ -676,11 +712,13 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             const int ok = __shfl_xor(bk[pi], o, 64);
             const int ot = __shfl_xor(bt[pi], o, 64);
             const int om = __shfl_xor(bm[pi], o, 64);
+            const int oc = __shfl_xor(bcl[pi], o, 64);
             if (cand_better(og, ok, best[pi], bk[pi])) {
                 best[pi] = og;
                 bk[pi] = ok;
                 bt[pi] = ot;
                 bm[pi] = om;
+                bcl[pi] = oc;
             }
         }
         if (lane == 0) {
@@ -688,6 +726,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             sk[pi][wave] = bk[pi];
             st[pi][wave] = bt[pi];
             sm[pi][wave] = bm[pi];
+            scl[pi][wave] = bcl[pi];
         }
     }
     __syncthreads();
@@ -697,13 +736,14 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             const int nd = pend[pi];
             if (nd < 0) continue;
             double b = sg[pi][0];
-            int k = sk[pi][0], t = st[pi][0], m = sm[pi][0];
-            for (int w = 1; w < 4; ++w)
+            int k = sk[pi][0], t = st[pi][0], m = sm[pi][0], cl = scl[pi][0];
+            for (int w = 1; w < nw; ++w)
                 if (cand_better(sg[pi][w], sk[pi][w], b, k)) {
                     b = sg[pi][w];
                     k = sk[pi][w];
                     t = st[pi][w];
                     m = sm[pi][w];
+                    cl = scl[pi][w];
                 }
             DCand c;
             const bool elig = ndepth[pi] < P.max_depth && nns[pi] >= 2 * P.min_leaf;
@@ -714,6 +754,8 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             c.type = 0;
             c.thr = k == 0x7fffffff ? -1 : t;
             c.mgl = k == 0x7fffffff ? 1 : m;
+            c.cl = k == 0x7fffffff ? 0 : cl;
+            c.pad = 0;
             pc[pi] = c;
             if (writer) T.cands[nd] = c;
         }
@@ -743,7 +785,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w)
+        for (int w = 1; w < nw; ++w)
             if (cand_better(lg[w], lk[w], lb, lbid)) {
                 lb = lg[w];
                 lbid = lk[w];
@@ -758,6 +800,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             out.feat2 = c.feature2;
             out.thr = c.thr;
             out.mgl = c.mgl;
+            out.nl = c.cl;
             out.gain = c.gain;
         }
     }
@@ -766,34 +809,40 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
 
 // Node bookkeeping of the split in T.cur_* (tree.cpp:385-392, 445-456): the split node becomes
 // internal, its children get ids L = cur_left and L + 1 with their row ranges; one thread.
-__device__ void apply_nodes(const Task& T, Task* Tw, int nl) {
-    const int L = T.cur_left, R = L + 1;
-    DNode& p = T.nodes[T.cur_leaf];
+__device__ void apply_nodes_d(const Task& T, Task* Tw, int leaf, int type, int feat, int feat2, int thr, int mgl,
+                              double gain, int L, int start, int count, int buf, int nl) {
+    const int R = L + 1;
+    DNode& p = T.nodes[leaf];
     p.v[N_LEAF] = 0;
-    p.v[N_TYPE] = T.cur_type;
-    p.v[N_FEAT] = T.cur_feat;
-    p.v[N_FEAT2] = T.cur_feat2;
-    p.v[N_THR] = T.cur_thr;
-    p.v[N_MGL] = T.cur_mgl;
+    p.v[N_TYPE] = type;
+    p.v[N_FEAT] = feat;
+    p.v[N_FEAT2] = feat2;
+    p.v[N_THR] = thr;
+    p.v[N_MGL] = mgl;
     p.v[N_LEFT] = L;
     p.v[N_RIGHT] = R;
-    p.gain = T.cur_gain;
+    p.gain = gain;
     for (int side = 0; side < 2; ++side) {
         DNode& c = T.nodes[side == 0 ? L : R];
         for (int k = 0; k < 10; ++k) c.v[k] = -1;
         c.v[N_LEAF] = 1;
         c.v[N_TYPE] = 0;
         c.v[N_MGL] = 1;
-        c.v[N_NS] = side == 0 ? nl : T.cur_count - nl;
+        c.v[N_NS] = side == 0 ? nl : count - nl;
         c.v[N_DEPTH] = p.v[N_DEPTH] + 1;
-        c.seg_start = side == 0 ? T.cur_start : T.cur_start + nl;
-        c.seg_count = side == 0 ? nl : T.cur_count - nl;
-        c.buf = 1 - T.cur_buf;
+        c.seg_start = side == 0 ? start : start + nl;
+        c.seg_count = side == 0 ? nl : count - nl;
+        c.buf = 1 - buf;
         c.value = 0.0;
         c.gain = 0.0;
         T.cands[side == 0 ? L : R].valid = 0;
     }
     Tw->n_nodes = L + 2;
+}
+
+__device__ void apply_nodes(const Task& T, Task* Tw, int nl) {
+    apply_nodes_d(T, Tw, T.cur_leaf, T.cur_type, T.cur_feat, T.cur_feat2, T.cur_thr, T.cur_mgl, T.cur_gain,
+                  T.cur_left, T.cur_start, T.cur_count, T.cur_buf, nl);
 }
 
 // Exact sums, over partials k = k0, k0 + dk, ... < np of the last histogram launch, of bins
@@ -859,16 +908,17 @@ __device__ inline void scan_store(const Shared& S, const Task& T, const GrowP& P
             qh[t] = 0;
         }
     double gain;
-    int thr, ml;
+    int thr, ml, cl;
     long long tq, tqh, tc;
     const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
     scan_feature_wave(qg, c, qh, T.hconst, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf, P.min_hess, P.l1,
-                      gain, thr, ml, tq, tqh, tc);
+                      gain, thr, ml, cl, tq, tqh, tc);
     if (lane == 0) {
         FCand& o = T.fcand[(size_t)node * S.n_features + f];
         o.gain = gain;
         o.thr = thr;
         o.ml = ml;
+        o.cl = cl;
         if (f == 0) {
             const double G = (double)tq * inv_g;
             const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
@@ -894,8 +944,7 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Tw.pending0 = mode == 0 ? 0 : T.cur_left;
         Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
-        Tw.part_lcnt = 0;   // the histogram launch has consumed them; the next partition counts anew
-        Tw.part_rcnt = 0;
+        Tw.part_lr = 0;   // the histogram launch has consumed it; the next partition counts anew
     }
     __shared__ long long rq[4][NBIN], rh[4][NBIN];
     __shared__ int rc[4][NBIN];
@@ -1003,6 +1052,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
             sd.feat2 = T.cur_feat2;
             sd.thr = T.cur_thr;
             sd.mgl = T.cur_mgl;
+            sd.nl = -1;
             sd.gain = T.cur_gain;
         }
         __syncthreads();
@@ -1026,6 +1076,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
         Tw.cur_count = cnt;
         Tw.cur_buf = lbuf;
         Tw.cur_left = T.n_nodes;
+        Tw.cur_nl = d.nl;
     }
     const int c0 = blockIdx.x * PART_CHUNK;
     if (c0 >= cnt) return;
@@ -1062,8 +1113,10 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
             tl += wsum[w];
             tr += wsumr[w];
         }
-        baseL = tl ? atomicAdd(&Tw.part_lcnt, tl) : 0;
-        baseR = tr ? atomicAdd(&Tw.part_rcnt, tr) : 0;
+        // one reservation per block for both sides: same-address device atomics serialize
+        const unsigned long long old = atomicAdd(&Tw.part_lr, (unsigned long long)tr << 32 | (unsigned)tl);
+        baseL = (int)(old & 0xffffffffull);
+        baseR = (int)(old >> 32);
     }
     __syncthreads();
     int woff = 0, woffr = 0;
@@ -1164,7 +1217,7 @@ __global__ __launch_bounds__(64) void k_pair_scan(Shared S, Task* tasks, int tas
     const Task& T = tasks[task];
     const int pp = blockIdx.x;
     double gain;
-    int thr, ml;
+    int thr, ml, cl;
     long long tq, tqh, tc;
     const int lane = threadIdx.x & 63;
     long long qg[4], qh[4];
@@ -1179,11 +1232,12 @@ __global__ __launch_bounds__(64) void k_pair_scan(Shared S, Task* tasks, int tas
     }
     scan_feature_wave(qg, c, qh, T.hconst, ldexp(1.0, -T.s_g),
                       ldexp(1.0, -T.s_h), S.n_bins, P.pair_l2, P.gamma, P.min_leaf, P.min_hess,
-                      P.l1, gain, thr, ml, tq, tqh, tc);
+                      P.l1, gain, thr, ml, cl, tq, tqh, tc);
     if (threadIdx.x == 0) {
         out[pp].gain = gain;
         out[pp].thr = thr;
         out[pp].ml = ml;
+        out[pp].cl = cl;
     }
 }
 
@@ -1415,7 +1469,10 @@ __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, cha
     const Task& T = tasks[blockIdx.x];
     char* base = out + (size_t)blockIdx.x * L.stride;
     const int nn = T.n_nodes;
-    if (threadIdx.x == 0) *(int*)base = nn;
+    if (threadIdx.x == 0) {
+        *(int*)base = nn;
+        *((int*)base + 1) = T.err;
+    }
     const int* src = (const int*)T.nodes;
     int* dst = (int*)(base + L.off_nodes);
     const int words = nn * (int)(sizeof(DNode) / sizeof(int));

@@ -298,7 +298,9 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->S.bins_col = s->bins_col;
         int max_task_rows = 1;
         for (int t = 0; t < n_tasks; ++t) max_task_rows = std::max(max_task_rows, task_rows ? task_rows[t] : n_rows);
-        s->S.part_cap = std::max(64, (max_task_rows + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS + 1);
+        // Partials per launch: launches of the split loop are sized for the largest sample rounded
+        // up to a power of two (< 2x the rows), and no block may fold more than MAX_FLUSH_ROWS rows.
+        s->S.part_cap = std::max(64, (int)((2LL * max_task_rows + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS) + 1);
         s->tasks_h.assign(n_tasks, Task{});
         s->th.assign(n_tasks, TaskHost{});
         const size_t per_node = (size_t)n_features * NBIN;
@@ -826,6 +828,8 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
             const char* base = s->export_h + L.stride * i;
             const int nn = *(const int*)base;
             if (nn < 1 || nn > MN) throw std::runtime_error("internal: exported tree size out of range");
+            if (*((const int*)base + 1) != 0)
+                throw std::runtime_error("internal: a partition placed a different number of rows left than its split's histogram counts");
             n_nodes[i] = nn;
             const int B = s->th[t].n_buckets;
             const DNode* nodes = (const DNode*)(base + L.off_nodes);
