@@ -8,9 +8,12 @@ A step is one boosting iteration (one tree).  `--warmup W` trees are grown untim
 `--steps K` trees are timed from the per-tree wall clock of the native loop (every tree ends with
 a device synchronization).  Inputs are resident in HBM before the timed trees start.
 
-Roofline: the dominant kernel is the histogram build.  Each root build reads B = n*F bin bytes +
-n*8 packed-gradient bytes and writes F*n_bins*24 histogram bytes (the algorithmic bytes); HIP
-events around every root-histogram launch on the session stream give its average duration.
+Roofline: the dominant kernel by device time is the child histogram build (k_hist_row, one per
+split).  A build over m rows reads m*F bin bytes + m*8 packed-gradient bytes + m*4 row-index
+bytes and writes F*n_bins*24 histogram bytes (the algorithmic bytes, with the reference's 24-byte
+HBin); HIP events around every launch on the session stream give the average duration.  The
+root build (k_hist_col, m = n, no row index) is reported beside it.  `traffic` comes from the
+FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
 
 CPU baseline: the reference core (oracle/_ref, built from the reference's own C++ sources) on
 the host's cores, same panel, timed as (fit of 1+C trees) - (fit of 1 tree) over C trees.
@@ -63,6 +66,16 @@ def cpu_baseline(p, params, cpu_trees):
             "setup_s": round(times[1] - per_tree, 3)}
 
 
+def _traffic():
+    """Per-launch HBM bytes of the histogram kernels from the committed counter summary."""
+    f = ROOT / "profiles" / "hist_traffic.json"
+    try:
+        d = json.loads(f.read_text())
+        return {k: v["bytes_per_launch"] for k, v in d["kernels"].items()}
+    except Exception:
+        return {}
+
+
 def run_plain(args, p):
     import tidegbm_amd as t
 
@@ -89,11 +102,15 @@ def run_plain(args, p):
             raise SystemExit("profiled fit diverged from the timed fit")
     ms0, n0, by0 = bp.profile(0)   # root histogram builds
     ms5, n5, by5 = bp.profile(5)   # child histogram builds
+    ms1, n1, _ = bp.profile(1)     # partitions (with the split selection)
     root_ms = ms0 / max(n0, 1)
     root_bytes = by0 / max(n0, 1)
-    achieved = root_bytes / (root_ms * 1e-3) / 1e9
-    child_gbs = by5 / (ms5 * 1e-3) / 1e9 if ms5 > 0 else None
+    root_gbs = root_bytes / (root_ms * 1e-3) / 1e9
+    child_ms = ms5 / max(n5, 1)
+    child_bytes = by5 / max(n5, 1)
+    achieved = child_bytes / (child_ms * 1e-3) / 1e9
     tree_gbs = (by0 + by5) / ((ms0 + ms5) * 1e-3) / 1e9
+    traffic = _traffic()
     out = {
         "metric": METRIC,
         "value": round(trees_per_s, 3),
@@ -118,14 +135,22 @@ def run_plain(args, p):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic.get("k_hist_row"),
+            "kernel": "child histogram build (k_hist_row), smaller child of every split, 128 features",
+            "bytes_per_launch": round(child_bytes),
+            "avg_launch_ms": round(child_ms, 5),
+            "launches": n5,
+        },
+        "roofline_root": {
+            "bound": "hbm", "achieved": round(root_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(root_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_hist_col"),
             "kernel": "root histogram build (k_hist_col), 1,000,000 x 128",
-            "bytes_per_launch": root_bytes,
-            "avg_launch_ms": round(root_ms, 5),
+            "bytes_per_launch": root_bytes, "avg_launch_ms": round(root_ms, 5), "launches": n0,
         },
         "hist_tree_gbs": round(tree_gbs, 1),
-        "hist_root_launches": n0, "hist_child_launches": n5,
-        "hist_child_gbs": None if child_gbs is None else round(child_gbs, 1),
+        "device_ms_per_tree": {"root_hist": round(ms0 / max(n0, 1), 4),
+                               "child_hist": round(ms5 / max(n0, 1), 4),
+                               "partition_select": round(ms1 / max(n0, 1), 4)},
         "fit_setup_s": round(setup_s, 3),
         "fit_wall_s": round(wall, 3),
         "trees_per_s_end_to_end": round((W + K) / (setup_s + boost_s), 3),

@@ -121,7 +121,8 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
 int tgbm_task_get_pred(tgbm_session* s, int task, double* pred);
 
 /* Timers (HIP events on the session stream) summed per kernel family since the last reset:
- * which = 0 histogram build, 1 partition, 2 scan, 3 tree end, 4 gradients. */
+ * which = 0 root histogram, 1 partition (with the split selection), 2 split scan, 3 tree end,
+ * 4 gradients, 5 child histograms. */
 int tgbm_timer_enable(tgbm_session* s, int on);
 int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches,
                     double* bytes);

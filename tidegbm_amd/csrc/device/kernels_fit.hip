@@ -529,21 +529,27 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     }
     // Row kernel: LPR lanes per row (16 features each), 512 threads.  Default LPR = 1 (16
     // features per block: more blocks per node at the same partial count) and 1024-row minimum
-    // tiles, the fastest setting over the node sizes of a 1M x 128, 31-leaf fit (per-size sweep:
-    // 11.8 us at 2-8K rows, 13.5 us at 8-32K, 17.1 us at 32-128K, 28.6 us above).  Variants >= 16
-    // pick LPR = 1 << ((v - 16) / 8) and min_rows = 1024 << ((v - 16) % 8).
+    // tiles, the fastest setting over the node sizes of a 1M x 128, 31-leaf fit.  Variants >= 16
+    // (low byte) pick LPR = 1 << ((v - 16) / 8) and min_rows = 256 << ((v - 16) % 8).
     int lpr = 1, min_rows = 1024;
     const int cv = variant & 0xff;
     if (cv >= 16) {
         lpr = 1 << std::min(2, (cv - 16) / 8);
-        min_rows = 1024 << std::min(5, (cv - 16) % 8);
+        min_rows = 256 << std::min(7, (cv - 16) % 8);
     }
     if (!hconst && lpr > 2) lpr = 2;   // two LDS words per bin: 64 features would not fit
     const int FB = 16 * lpr;
     const int fbs = (S.fp + FB - 1) / FB;
     // Blocks per feature block: no block folds more than MAX_FLUSH_ROWS rows, and there are at most
     // part_cap partials (part_cap >= 2 * rows / MAX_FLUSH_ROWS + 1 > max_count / MAX_FLUSH_ROWS).
-    int gx = std::max(1, std::min(std::max(1, 512 / fbs), (max_count + min_rows - 1) / min_rows));
+    // At most 32 partials by default: more blocks speed the build up less than the extra
+    // partials slow the scan's sums down (whole-tree sweep at 1M x 128: 32 -> 836 trees/s, 64 ->
+    // 787, 16 -> 745).  Bits 12-13 of the variant pick 32, 64, 16 or 8.
+    // (Fewer blocks per task when many tasks share the launch was slower too: batched
+    // walk-forward 2.27 s at 4096 blocks per launch vs 1.88 s at 32 partials per task.)
+    constexpr int pcaps[4] = {32, 64, 16, 8};
+    const int pcap = pcaps[(variant >> 12) & 3];
+    int gx = std::max(1, std::min(pcap, (max_count + min_rows - 1) / min_rows));
     gx = std::max(gx, (max_count + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS);
     gx = std::min(gx, S.part_cap);
     dim3 grid(gx, fbs, n_tasks);
@@ -1052,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
             sd.feat2 = T.cur_feat2;
             sd.thr = T.cur_thr;
             sd.mgl = T.cur_mgl;
-            sd.nl = -1;
+            sd.nl = T.cur_nl;   // known for decisions of k_select, -1 for the host's
             sd.gain = T.cur_gain;
         }
         __syncthreads();
@@ -1132,6 +1138,33 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
         if (flags & (1u << t)) dst[lpos++] = rows[t];
         else dst[cnt - 1 - rpos++] = rows[t];
     }
+}
+
+// Standalone selection (one block per task) for launches that grow many tasks at once: the
+// partition then reads the decision instead of every partition block recomputing it.
+__global__ __launch_bounds__(256) void k_select(Shared S, Task* tasks, GrowP P) {
+    const Task T = tasks[blockIdx.x];
+    Task& Tw = tasks[blockIdx.x];
+    if (T.done) return;
+    __shared__ Decision sd;
+    select_decide(T, S.n_features, P, true, sd);
+    if (threadIdx.x != 0) return;
+    if (sd.leaf < 0) {
+        Tw.done = 1;
+        return;
+    }
+    Tw.cur_leaf = sd.leaf;
+    Tw.cur_type = sd.type;
+    Tw.cur_feat = sd.feat;
+    Tw.cur_feat2 = sd.feat2;
+    Tw.cur_thr = sd.thr;
+    Tw.cur_mgl = sd.mgl;
+    Tw.cur_gain = sd.gain;
+    Tw.cur_nl = sd.nl;
+}
+
+void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P) {
+    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, S, tasks, P);
 }
 
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,

@@ -22,7 +22,9 @@ void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant);
 void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
-// mode 0: the partition launch selects the split itself; mode 1: the host wrote it in T.cur_*.
+// One block per task selects the next split into T.cur_* (for launches over many tasks).
+void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
+// mode 0: the partition launch selects the split itself; mode 1: the decision is in T.cur_*.
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
                       int mode);
 void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count);

@@ -657,12 +657,21 @@ static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_gro
     const int ms = s->launch_rows();
     EventPair e1, e2;
     if (s->timing) s->tic(1, 0.0, e1);
-    launch_partition(s->stream, s->S, s->td(), s->an, ms, P, mode);
+    if (mode == 0 && s->an > 1) {
+        // many tasks: one selection block per task instead of one per partition block
+        launch_select(s->stream, s->S, s->td(), s->an, P);
+        launch_partition(s->stream, s->S, s->td(), s->an, ms, P, 1);
+    } else {
+        launch_partition(s->stream, s->S, s->td(), s->an, ms, P, mode);
+    }
     if (s->timing) s->toc(e1);
     if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
     launch_hist(s->stream, s->S, s->td(), s->an, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
     if (s->timing) s->toc(e2);
+    EventPair e3;
+    if (s->timing) s->tic(2, 0.0, e3);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 1);
+    if (s->timing) s->toc(e3);
     HIP_OK(hipGetLastError());
 }
 
