@@ -4,6 +4,8 @@
 //
 // Built with -ffp-contract=off: every floating-point expression that mirrors a reference
 // formula must round like the reference's x86-64 build (no fused multiply-add).
+#include <hip/hip_ext.h>
+
 #include "dev_common.h"
 #include "launch.h"
 
@@ -500,8 +502,17 @@ __global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int row
     hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, bx);
 }
 
+// Launch with optional start/stop events recorded at the kernel's own boundaries (AMD extended
+// launch), so timed runs measure the kernel and not the dispatch gap around it.
+#define TIMED_LAUNCH(kernel, grid, block, lds, st, e0, e1, ...)                                      \
+    do {                                                                                            \
+        if (e0) hipExtLaunchKernelGGL(kernel, grid, block, lds, st, e0, e1, 0, __VA_ARGS__);        \
+        else hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);                         \
+    } while (0)
+
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
-                 bool contiguous_root, bool hconst, int max_count, int variant) {
+                 bool contiguous_root, bool hconst, int max_count, int variant, hipEvent_t e0,
+                 hipEvent_t e1) {
     if (max_count <= 0) return;
     if (root && contiguous_root && (variant & 0xff) != 2) {
         // Root variants (variant >> 8): bit 0 selects 16 features per block (else 8), bit 1 a
@@ -519,11 +530,11 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
         dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
         if (FB == 8) {
-            if (hconst) hipLaunchKernelGGL((k_hist_col<8, false>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
-            else hipLaunchKernelGGL((k_hist_col<8, true>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+            if (hconst) TIMED_LAUNCH((k_hist_col<8, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            else TIMED_LAUNCH((k_hist_col<8, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
         } else {
-            if (hconst) hipLaunchKernelGGL((k_hist_col<16, false>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
-            else hipLaunchKernelGGL((k_hist_col<16, true>), grid, dim3(threads), 0, st, S, tasks, rpb, xcd);
+            if (hconst) TIMED_LAUNCH((k_hist_col<16, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            else TIMED_LAUNCH((k_hist_col<16, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
         }
         return;
     }
@@ -556,12 +567,12 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
     const int rt = root ? 1 : 0;
     if (hconst) {
-        if (lpr == 1) hipLaunchKernelGGL((k_hist_row<1, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
-        else if (lpr == 2) hipLaunchKernelGGL((k_hist_row<2, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
-        else hipLaunchKernelGGL((k_hist_row<4, false>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
+        else if (lpr == 2) TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
+        else TIMED_LAUNCH((k_hist_row<4, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
     } else {
-        if (lpr == 1) hipLaunchKernelGGL((k_hist_row<1, true>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
-        else hipLaunchKernelGGL((k_hist_row<2, true>), grid, dim3(512), lds, st, S, tasks, rt, min_rows);
+        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
+        else TIMED_LAUNCH((k_hist_row<2, true>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
     }
 }
 

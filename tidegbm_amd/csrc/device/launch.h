@@ -19,8 +19,10 @@ void launch_grad(hipStream_t st, Task* tasks, int n_tasks, int objective, int ma
                  int max_groups);
 void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_rows,
                        int max_nodes);
+// e0/e1 (optional): events recorded at the kernel's start and end.
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
-                 bool contiguous_root, bool hconst, int max_count, int variant);
+                 bool contiguous_root, bool hconst, int max_count, int variant,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
 // One block per task selects the next split into T.cur_* (for launches over many tasks).
 void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);

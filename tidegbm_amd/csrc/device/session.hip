@@ -217,6 +217,13 @@ struct tgbm_session {
         HIP_OK(hipEventRecord(ep.b, stream));
         events.push_back(ep);
     }
+    // events that an extended launch records at the kernel's own start and end
+    void kernel_events(int family, double bytes, EventPair& ep) {
+        ep.family = family;
+        ep.bytes = bytes;
+        HIP_OK(hipEventCreate(&ep.a));
+        HIP_OK(hipEventCreate(&ep.b));
+    }
     void drain_events() {
         if (events.empty()) return;
         HIP_OK(hipStreamSynchronize(stream));
@@ -638,11 +645,11 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
             const double m = t.has_sample ? t.n_sample : t.n_rows;
             bytes += m * s->F + m * (s->hconst ? 8.0 : 16.0) + (t.has_sample ? 4.0 * m : 0.0) + (double)s->F * s->nb * 24.0;
         }
-        s->tic(0, bytes, ep);
+        s->kernel_events(0, bytes, ep);
     }
     launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous() && s->all_aligned(), s->hconst, ms,
-                p->hist_mode);
-    if (s->timing) s->toc(ep);
+                p->hist_mode, s->timing ? ep.a : nullptr, s->timing ? ep.b : nullptr);
+    if (s->timing) s->events.push_back(ep);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 0);
     (void)device_mode;   // the first split is selected by the first partition launch
     if (energy)
@@ -665,9 +672,10 @@ static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_gro
         launch_partition(s->stream, s->S, s->td(), s->an, ms, P, mode);
     }
     if (s->timing) s->toc(e1);
-    if (s->timing) s->tic(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
-    launch_hist(s->stream, s->S, s->td(), s->an, false, false, s->hconst, ms / 2 + 1, p->hist_mode);
-    if (s->timing) s->toc(e2);
+    if (s->timing) s->kernel_events(5, 0.0, e2);  // child histogram builds (bytes filled at tree end)
+    launch_hist(s->stream, s->S, s->td(), s->an, false, false, s->hconst, ms / 2 + 1, p->hist_mode,
+                s->timing ? e2.a : nullptr, s->timing ? e2.b : nullptr);
+    if (s->timing) s->events.push_back(e2);
     EventPair e3;
     if (s->timing) s->tic(2, 0.0, e3);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 1);
