@@ -272,7 +272,8 @@ def _serve_run(t, p, models, stream, batched, slots):
         "predict_le4096_p50_ms": pct(small, 50), "predict_le4096_p99_ms": pct(small, 99),
         "refit_p50_ms": pct(lat["refit"], 50), "refit_p99_ms": pct(lat["refit"], 99),
         "predict_rows_per_s": round(rows / wall, 1), "predict_launches": st["predict_launches"],
-        "refit_rounds": st["refit_rounds"],
+        "refit_rounds": st["refit_rounds"], "refit_admit_s": round(st["refit_admit_s"], 3),
+        "refit_round_s": round(st["refit_round_s"], 3), "refit_publish_s": round(st["refit_publish_s"], 3),
     }
 
 

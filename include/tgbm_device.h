@@ -82,8 +82,10 @@ int tgbm_task_swap(tgbm_session* s, int a, int b);
 /* Row-major uint8 bins back to the host (n_rows x n_features), for checking. */
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out);
 
-/* Per-task row data (host arrays of task_rows[t] entries).  w may be NULL (all 1).  bucket and
- * sweep_seg drive TORB: sweep_seg[i] = number of bucket cuts <= i (0..n_buckets). */
+/* Per-task row data (host arrays of task_rows[t] entries).  w may be NULL (all 1); pred0 may be
+ * NULL (keep the task's device predictions, e.g. from tgbm_task_predict_forest); bucket may be
+ * NULL (keep the device buckets).  bucket and sweep_seg drive TORB: sweep_seg[i] = number of
+ * bucket cuts <= i (0..n_buckets). */
 int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double* w,
                        const double* pred0, const int* bucket, const unsigned char* sweep_seg,
                        int n_buckets, const int* cuts, const long long* groups);
@@ -140,6 +142,12 @@ int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_tree
                        const double* lr, double init_score, const double* edges,
                        const int* n_edges);
 void tgbm_forest_destroy(tgbm_forest* f);
+/* New forest = the first keep_trees trees of `base` (copied on the device) followed by n_trees
+ * new trees (same node layout as tgbm_forest_create); features, bins, edges and init score are
+ * base's.  Lets a refit publish a version without re-uploading its parent's trees. */
+int tgbm_forest_append(tgbm_forest** out, const tgbm_forest* base, int keep_trees, int n_trees,
+                       const int* tree_nodes, const int* node_ints, const double* values,
+                       const double* lr);
 /* Predict raw scores for host rows X (row-major f64) or for device-resident uint8 bins. */
 int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out);
 int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out);
@@ -164,6 +172,10 @@ void tgbm_predictor_destroy(tgbm_predictor* p);
 size_t tgbm_predict_bytes(const tgbm_forest* f, int from_x, int n);
 int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_predict_item* items,
                        int n_items);
+/* Overwrite task `task`'s device predictions with forest f's raw scores of the task's panel row
+ * window (the same sums as tgbm_predict_batch), on the session stream: a refit's starting
+ * predictions without a host round trip.  Call after tgbm_task_bind. */
+int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f);
 
 #ifdef __cplusplus
 }

@@ -123,9 +123,10 @@ int tide_server_wait(TideServer s, long long ticket, double* out, int cap, int* 
                      double* latency_s);
 /* New handle holding a version of a model (-1 = current) and that version's parent. */
 int tide_server_model(TideServer s, int model, int version, TideHandle* out, int* parent);
-/* Counters: predict launches, predict requests, predict rows, refit rounds, refits done,
- * seconds the predict and refit threads spent on the GPU. */
-int tide_server_stats(TideServer s, long long* counts5, double* busy2);
+/* Counters: predict launches, predict requests, predict rows, refit rounds, refits done; and
+ * seconds: predict thread busy, refit thread busy, and the refit thread's admitting (base
+ * predictions, slot binding, uploads), tree rounds and version publishing. */
+int tide_server_stats(TideServer s, long long* counts5, double* seconds5);
 
 #ifdef __cplusplus
 }

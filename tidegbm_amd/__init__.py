@@ -355,11 +355,12 @@ class Server:
 
     def stats(self):
         c = np.zeros(5, np.int64)
-        b = np.zeros(2)
+        b = np.zeros(5)
         _lib.tide_server_stats(self._h, _ptr(c), _ptr(b))
         return dict(predict_launches=int(c[0]), predict_requests=int(c[1]), predict_rows=int(c[2]),
                     refit_rounds=int(c[3]), refits=int(c[4]), predict_busy_s=float(b[0]),
-                    refit_busy_s=float(b[1]))
+                    refit_busy_s=float(b[1]), refit_admit_s=float(b[2]), refit_round_s=float(b[3]),
+                    refit_publish_s=float(b[4]))
 
 
 def tree_arrays(h, info_fn, get_fn, t):

@@ -29,6 +29,7 @@ extern thread_local std::string g_dev_err;  // defined in session.hip
     } while (0)
 
 void tgbm_forest::reserve(int n) {
+    if (!stream) FHIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));   // first single-call predict
     if (n <= cap) return;
     for (void* p : {(void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
         if (p) (void)hipFree(p);
@@ -96,6 +97,117 @@ static int fguard(Fn&& fn) {
     }
 }
 
+// Host-side node tables of n_trees trees (export layout) whose first node gets absolute id `base`.
+struct HostNodes {
+    std::vector<FNode> nodes;
+    std::vector<SNode> snodes;
+    std::vector<unsigned char> flags;
+    std::vector<int> tree_nodes;   // per tree
+    int total = 0;
+};
+
+static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node_ints, int n_features, int base) {
+    HostNodes h;
+    for (int t = 0; t < n_trees; ++t) h.total += tree_nodes[t];
+    h.nodes.resize(std::max(h.total, 1));
+    h.snodes.resize(std::max(h.total, 1));
+    h.flags.resize(std::max(h.total, 1));
+    h.tree_nodes.assign(tree_nodes, tree_nodes + n_trees);
+    int off = 0;
+    for (int t = 0; t < n_trees; ++t) {
+        if (tree_nodes[t] < 1 || tree_nodes[t] > 255) throw std::runtime_error("trees must have 1..255 nodes");
+        for (int k = 0; k < tree_nodes[t]; ++k) {
+            const int* v = node_ints + (size_t)(off + k) * TGBM_NODE_INTS;
+            const bool leaf = v[0] != 0;
+            FNode nd{0, 0, 0, 0};
+            if (!leaf) {
+                if (v[2] < 0 || v[2] >= n_features || v[5] < 0 || v[6] < 0 || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
+                    throw std::runtime_error("malformed tree node");
+                if (v[1] != 0 && v[1] != 1) throw std::runtime_error("only axis and pair splits run on the device");
+                const int f2 = v[1] == 1 ? v[3] : 0;
+                if (f2 < 0 || f2 >= n_features) throw std::runtime_error("malformed pair node");
+                nd.fx = v[2] | (f2 << 16);
+                nd.thr = v[4];
+                nd.left = base + off + v[5];
+                nd.right = base + off + v[6];
+            }
+            h.nodes[off + k] = nd;
+            h.flags[off + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
+            SNode sn{0, 0, 0, 0, 0, h.flags[off + k]};
+            if (!leaf) {
+                if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
+                sn.f = (unsigned short)v[2];
+                sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
+                sn.thr = (unsigned char)v[4];
+                sn.left = (unsigned char)v[5];
+                sn.right = (unsigned char)v[6];
+            }
+            h.snodes[off + k] = sn;
+        }
+        off += tree_nodes[t];
+    }
+    return h;
+}
+
+// Allocate a forest of n_trees / total nodes whose first `keep_trees` trees (keep_nodes nodes)
+// are copied from `base` on the device, followed by the trees of `h`.
+static tgbm_forest* assemble(const tgbm_forest* base, int keep_trees, int keep_nodes, const HostNodes& h,
+                             const double* values, const double* lr, int n_features, int n_bins, double init_score,
+                             const double* edges, const int* n_edges) {
+    auto* f = new tgbm_forest();
+    std::unique_ptr<tgbm_forest> guard(f);
+    const int n_new = (int)h.tree_nodes.size();
+    f->F = n_features;
+    f->fp = (n_features + 15) / 16 * 16;
+    f->nb = n_bins;
+    f->n_trees = keep_trees + n_new;
+    f->n_nodes = keep_nodes + h.total;
+    f->init = init_score;
+    f->toff_h.assign(base ? base->toff_h.begin() : f->toff_h.begin(), base ? base->toff_h.begin() + keep_trees + 1
+                                                                            : f->toff_h.begin());
+    if (f->toff_h.empty()) f->toff_h.push_back(0);
+    for (int t = 0; t < n_new; ++t) f->toff_h.push_back(f->toff_h.back() + h.tree_nodes[t]);
+    f->max_chunk_nodes = 1;
+    for (int t0 = 0; t0 < f->n_trees; t0 += SERVE_TREES_PER_CHUNK) {
+        const int t1 = std::min(f->n_trees, t0 + SERVE_TREES_PER_CHUNK);
+        f->max_chunk_nodes = std::max(f->max_chunk_nodes, f->toff_h[t1] - f->toff_h[t0]);
+    }
+    const size_t NN = std::max(f->n_nodes, 1), NT = std::max(f->n_trees, 1);
+    FHIP(hipMalloc(&f->nodes, sizeof(FNode) * NN));
+    FHIP(hipMalloc(&f->flags, NN));
+    FHIP(hipMalloc(&f->snodes, sizeof(SNode) * NN));
+    FHIP(hipMalloc(&f->tree_off, sizeof(int) * (f->n_trees + 1)));
+    FHIP(hipMalloc(&f->values, sizeof(double) * NN));
+    FHIP(hipMalloc(&f->roots, sizeof(int) * NT));
+    FHIP(hipMalloc(&f->lr, sizeof(double) * NT));
+    FHIP(hipMalloc(&f->edges, sizeof(double) * (size_t)n_features * NBIN));
+    FHIP(hipMalloc(&f->n_edges, sizeof(int) * n_features));
+    if (base && keep_trees > 0) {
+        FHIP(hipMemcpy(f->nodes, base->nodes, sizeof(FNode) * keep_nodes, hipMemcpyDeviceToDevice));
+        FHIP(hipMemcpy(f->flags, base->flags, keep_nodes, hipMemcpyDeviceToDevice));
+        FHIP(hipMemcpy(f->snodes, base->snodes, sizeof(SNode) * keep_nodes, hipMemcpyDeviceToDevice));
+        FHIP(hipMemcpy(f->values, base->values, sizeof(double) * keep_nodes, hipMemcpyDeviceToDevice));
+        FHIP(hipMemcpy(f->lr, base->lr, sizeof(double) * keep_trees, hipMemcpyDeviceToDevice));
+    }
+    if (h.total) {
+        FHIP(hipMemcpy(f->nodes + keep_nodes, h.nodes.data(), sizeof(FNode) * h.total, hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->flags + keep_nodes, h.flags.data(), h.total, hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->snodes + keep_nodes, h.snodes.data(), sizeof(SNode) * h.total, hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->values + keep_nodes, values, sizeof(double) * h.total, hipMemcpyHostToDevice));
+    }
+    if (n_new) FHIP(hipMemcpy(f->lr + keep_trees, lr, sizeof(double) * n_new, hipMemcpyHostToDevice));
+    FHIP(hipMemcpy(f->tree_off, f->toff_h.data(), sizeof(int) * f->toff_h.size(), hipMemcpyHostToDevice));
+    FHIP(hipMemcpy(f->roots, f->toff_h.data(), sizeof(int) * std::max(f->n_trees, 1), hipMemcpyHostToDevice));
+    if (base) {
+        FHIP(hipMemcpy(f->edges, base->edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyDeviceToDevice));
+        FHIP(hipMemcpy(f->n_edges, base->n_edges, sizeof(int) * n_features, hipMemcpyDeviceToDevice));
+    } else if (edges) {
+        FHIP(hipMemcpy(f->edges, edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyHostToDevice));
+        FHIP(hipMemcpy(f->n_edges, n_edges, sizeof(int) * n_features, hipMemcpyHostToDevice));
+    }
+    return guard.release();
+}
+
 extern "C" {
 
 int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_trees, const int* tree_nodes,
@@ -107,83 +219,20 @@ int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_tree
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             throw std::runtime_error("GPU device not available: no HIP device found");
         if (n_features <= 0 || n_trees < 0) throw std::runtime_error("bad forest shape");
-        auto* f = new tgbm_forest();
-        std::unique_ptr<tgbm_forest> guard(f);
-        f->F = n_features;
-        f->fp = (n_features + 15) / 16 * 16;
-        f->nb = n_bins;
-        f->n_trees = n_trees;
-        f->init = init_score;
-        int total = 0;
-        for (int t = 0; t < n_trees; ++t) total += tree_nodes[t];
-        f->n_nodes = total;
-        std::vector<FNode> nodes(std::max(total, 1));
-        std::vector<SNode> snodes(std::max(total, 1));
-        std::vector<unsigned char> flags(std::max(total, 1));
-        std::vector<int> roots(std::max(n_trees, 1)), toff(n_trees + 1);
-        int base = 0;
-        for (int t = 0; t < n_trees; ++t) {
-            if (tree_nodes[t] < 1 || tree_nodes[t] > 255) throw std::runtime_error("trees must have 1..255 nodes");
-            roots[t] = base;
-            toff[t] = base;
-            for (int k = 0; k < tree_nodes[t]; ++k) {
-                const int* v = node_ints + (size_t)(base + k) * TGBM_NODE_INTS;
-                const bool leaf = v[0] != 0;
-                FNode nd{0, 0, 0, 0};
-                if (!leaf) {
-                    if (v[2] < 0 || v[2] >= n_features || v[5] < 0 || v[6] < 0 || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
-                        throw std::runtime_error("malformed tree node");
-                    if (v[1] != 0 && v[1] != 1) throw std::runtime_error("only axis and pair splits run on the device");
-                    const int f2 = v[1] == 1 ? v[3] : 0;
-                    if (f2 < 0 || f2 >= n_features) throw std::runtime_error("malformed pair node");
-                    nd.fx = v[2] | (f2 << 16);
-                    nd.thr = v[4];
-                    nd.left = base + v[5];
-                    nd.right = base + v[6];
-                }
-                nodes[base + k] = nd;
-                flags[base + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
-                SNode sn{0, 0, 0, 0, 0, flags[base + k]};
-                if (!leaf) {
-                    if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
-                    sn.f = (unsigned short)v[2];
-                    sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
-                    sn.thr = (unsigned char)v[4];
-                    sn.left = (unsigned char)v[5];
-                    sn.right = (unsigned char)v[6];
-                }
-                snodes[base + k] = sn;
-            }
-            base += tree_nodes[t];
-        }
-        toff[n_trees] = base;
-        f->max_chunk_nodes = 1;
-        for (int t0 = 0; t0 < n_trees; t0 += SERVE_TREES_PER_CHUNK) {
-            const int t1 = std::min(n_trees, t0 + SERVE_TREES_PER_CHUNK);
-            f->max_chunk_nodes = std::max(f->max_chunk_nodes, toff[t1] - toff[t0]);
-        }
-        FHIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
-        FHIP(hipMalloc(&f->nodes, sizeof(FNode) * nodes.size()));
-        FHIP(hipMalloc(&f->flags, flags.size()));
-        FHIP(hipMalloc(&f->snodes, sizeof(SNode) * snodes.size()));
-        FHIP(hipMalloc(&f->tree_off, sizeof(int) * toff.size()));
-        FHIP(hipMalloc(&f->values, sizeof(double) * std::max(total, 1)));
-        FHIP(hipMalloc(&f->roots, sizeof(int) * roots.size()));
-        FHIP(hipMalloc(&f->lr, sizeof(double) * std::max(n_trees, 1)));
-        FHIP(hipMalloc(&f->edges, sizeof(double) * (size_t)n_features * NBIN));
-        FHIP(hipMalloc(&f->n_edges, sizeof(int) * n_features));
-        FHIP(hipMemcpy(f->nodes, nodes.data(), sizeof(FNode) * nodes.size(), hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->flags, flags.data(), flags.size(), hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->snodes, snodes.data(), sizeof(SNode) * snodes.size(), hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->tree_off, toff.data(), sizeof(int) * toff.size(), hipMemcpyHostToDevice));
-        if (total) FHIP(hipMemcpy(f->values, values, sizeof(double) * total, hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->roots, roots.data(), sizeof(int) * roots.size(), hipMemcpyHostToDevice));
-        if (n_trees) FHIP(hipMemcpy(f->lr, lr, sizeof(double) * n_trees, hipMemcpyHostToDevice));
-        if (edges) {
-            FHIP(hipMemcpy(f->edges, edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyHostToDevice));
-            FHIP(hipMemcpy(f->n_edges, n_edges, sizeof(int) * n_features, hipMemcpyHostToDevice));
-        }
-        *out = guard.release();
+        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, n_features, 0);
+        *out = assemble(nullptr, 0, 0, h, values, lr, n_features, n_bins, init_score, edges, n_edges);
+    });
+}
+
+int tgbm_forest_append(tgbm_forest** out, const tgbm_forest* base, int keep_trees, int n_trees, const int* tree_nodes,
+                       const int* node_ints, const double* values, const double* lr) {
+    *out = nullptr;
+    return fguard([&] {
+        if (!base) throw std::runtime_error("forest append needs a base forest");
+        if (keep_trees < 0 || keep_trees > base->n_trees || n_trees < 0) throw std::runtime_error("bad forest append shape");
+        const int keep_nodes = base->toff_h[keep_trees];
+        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, base->F, keep_nodes);
+        *out = assemble(base, keep_trees, keep_nodes, h, values, lr, base->F, base->nb, base->init, nullptr, nullptr);
     });
 }
 

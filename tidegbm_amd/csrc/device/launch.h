@@ -10,6 +10,10 @@ namespace tgbm {
 
 // Resident binned panel of a session (row-major bins, fp bytes per row, n rows, F features).
 void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, int* fp, int* n, int* F);
+// A task's device predictions, row window and the session stream, plus a device scratch of at
+// least scratch_bytes owned by the session.
+void session_task_window(tgbm_session* s, int task, double** pred, int* row0, int* n_rows, hipStream_t* stream,
+                         void** scratch, size_t scratch_bytes);
 
 void init_kernel_attributes();
 void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edges,

@@ -290,4 +290,47 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
     });
 }
 
+int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f) {
+    return sguard([&] {
+        if (!f) throw std::runtime_error("no ensemble");
+        const unsigned char* pbins = nullptr;
+        int pfp = 0, pn = 0, pF = 0;
+        session_panel_info(s, &pbins, &pfp, &pn, &pF);
+        if (f->F != pF) throw std::runtime_error("ensemble and panel feature counts differ");
+        double* pred = nullptr;
+        int row0 = 0, n = 0;
+        hipStream_t st = nullptr;
+        void* scratch = nullptr;
+        session_task_window(s, task, &pred, &row0, &n, &st, &scratch, sizeof(SReq));
+        const size_t lds = serve_lds(f->fp, f->max_chunk_nodes);
+        if (lds > SERVE_MAX_LDS) throw std::runtime_error("ensemble chunk too large for LDS");
+        static bool attr = false;
+        if (!attr) {
+            SHIP(hipFuncSetAttribute((const void*)k_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SERVE_MAX_LDS));
+            attr = true;
+        }
+        SReq R;
+        R.nodes = f->snodes;
+        R.tree_off = f->tree_off;
+        R.values = f->values;
+        R.lr = f->lr;
+        R.bins = pbins + (size_t)row0 * pfp;   // the task's rows are a contiguous window
+        R.rows = nullptr;
+        R.out = pred;
+        R.init = f->init;
+        R.n_trees = f->n_trees;
+        R.nb = f->nb;
+        R.fp = f->fp;
+        R.n = n;
+        R.blk0 = 0;
+        R.pad = 0;
+        SHIP(hipMemcpyAsync(scratch, &R, sizeof R, hipMemcpyHostToDevice, st));
+        const int blocks = (n + SERVE_ROWS - 1) / SERVE_ROWS;
+        if (blocks > 0)
+            hipLaunchKernelGGL(k_serve, dim3(blocks), dim3(256), lds, st, (const SReq*)scratch, 1, f->fp);
+        SHIP(hipGetLastError());
+        SHIP(hipStreamSynchronize(st));   // the descriptor lives on this call's stack
+    });
+}
+
 }  // extern "C"

@@ -97,6 +97,9 @@ struct tgbm_session {
     // baked in: largest sample, leaf budget, histogram variant and the active task range
     std::map<std::vector<long long>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
     bool use_graphs = true;
+    // small device scratch for descriptor tables of launches made on behalf of other modules
+    char* small_scratch = nullptr;
+    size_t small_scratch_bytes = 0;
     // finished-tree staging (device + pinned host), sized for every task at the largest bucket count
     char* export_d = nullptr;
     char* export_h = nullptr;
@@ -247,6 +250,20 @@ void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, i
     *fp = s->fp;
     *n = s->n;
     *F = s->F;
+}
+
+void session_task_window(tgbm_session* s, int task, double** pred, int* row0, int* n_rows, hipStream_t* stream,
+                         void** scratch, size_t scratch_bytes) {
+    if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+    *pred = s->tasks_h[task].pred;
+    *row0 = s->th[task].row0;
+    *n_rows = s->th[task].n_rows;
+    *stream = s->stream;
+    if (scratch_bytes > s->small_scratch_bytes) {
+        s->small_scratch = s->alloc<char>(scratch_bytes);
+        s->small_scratch_bytes = scratch_bytes;
+    }
+    *scratch = s->small_scratch;
 }
 }  // namespace tgbm
 
@@ -485,8 +502,9 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
         Task& T = s->tasks_h[task];
         TaskHost& H = s->th[task];
         const int m = H.n_rows;
+        HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy((void*)T.y, y, sizeof(double) * m, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(T.pred, pred0, sizeof(double) * m, hipMemcpyHostToDevice));
+        if (pred0) HIP_OK(hipMemcpy(T.pred, pred0, sizeof(double) * m, hipMemcpyHostToDevice));
         if (w) {
             if (s->hconst) {
                 for (int i = 0; i < m; ++i)

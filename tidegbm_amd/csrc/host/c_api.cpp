@@ -370,6 +370,9 @@ int tide_server_stats(TideServer s, long long* counts5, double* busy2) {
         counts5[4] = st.refit_requests;
         busy2[0] = st.predict_busy_s;
         busy2[1] = st.refit_busy_s;
+        busy2[2] = st.refit_admit_s;
+        busy2[3] = st.refit_round_s;
+        busy2[4] = st.refit_publish_s;
     });
 }
 

@@ -40,8 +40,7 @@ using Clock = std::chrono::steady_clock;
 
 double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-std::shared_ptr<tgbm_forest> forest_of(const Booster& b) {
-    tgbm_forest* f = make_forest(b);
+std::shared_ptr<tgbm_forest> own(tgbm_forest* f) {
     return std::shared_ptr<tgbm_forest>(f, [](tgbm_forest* p) { tgbm_forest_destroy(p); });
 }
 
@@ -49,8 +48,9 @@ std::shared_ptr<tgbm_forest> forest_of(const Booster& b) {
 
 struct Server::Version {
     int version = 0, parent = -1;
-    std::shared_ptr<Booster> model;
-    std::shared_ptr<tgbm_forest> forest;
+    std::shared_ptr<const Snap> snap;
+    std::shared_ptr<tgbm_forest> forest;   // the first forest_trees trees of snap
+    int forest_trees = 0;
 };
 
 struct Server::Req {
@@ -110,14 +110,19 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     bin_on_device(sess_, X, m0.edges);
     dev_check(tgbm_predictor_create(&pred_, predict_budget));
     predict_cap_ = std::max<size_t>(predict_budget, 1 << 20);
-    dev_check(tgbm_predictor_create(&refit_pred_, (size_t)refit_max_rows * 16 + (4 << 20)));
     for (const Booster* m : models) {
+        auto snap = std::make_shared<Snap>();
+        auto proto = m->clone_model();
+        for (auto& t : proto->trees) snap->trees.push_back(std::make_shared<const Tree>(std::move(t)));
+        proto->trees.clear();
+        snap->proto = proto;
         auto v = std::make_shared<Version>();
         v->version = 0;
-        v->model = m->clone_model();
-        v->forest = forest_of(*v->model);
+        v->snap = snap;
+        v->forest = own(make_forest(*m));
+        v->forest_trees = std::min((int)m->trees.size(), m->best_iteration > 0 ? m->best_iteration : m->cfg.n_estimators);
         ModelState ms;
-        ms.history.push_back(HistEntry{0, -1, v->model});
+        ms.history.push_back(HistEntry{0, -1, snap});
         ms.current = v;
         ms.shape = v->forest;
         models_.push_back(std::move(ms));
@@ -147,7 +152,6 @@ Server::~Server() {
     }
     cv_done_.notify_all();
     if (pred_) tgbm_predictor_destroy(pred_);
-    if (refit_pred_) tgbm_predictor_destroy(refit_pred_);
     if (sess_) tgbm_session_destroy(sess_);
 }
 
@@ -226,16 +230,24 @@ int Server::wait(long long id, double* out, int cap, int* version, double* laten
 }
 
 std::shared_ptr<Booster> Server::model_version(int model, int version, int* parent) {
-    std::lock_guard<std::mutex> lk(mu_);
-    if (model < 0 || model >= (int)models_.size()) throw Error("model index out of range");
-    auto& h = models_[model].history;
-    if (version < 0) version = models_[model].current->version;
-    for (auto& v : h)
-        if (v.version == version) {
-            if (parent) *parent = v.parent;
-            return v.model;
-        }
-    throw Error("unknown model version (only the most recent versions are kept)");
+    std::shared_ptr<const Snap> snap;
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (model < 0 || model >= (int)models_.size()) throw Error("model index out of range");
+        auto& h = models_[model].history;
+        if (version < 0) version = models_[model].current->version;
+        for (auto& v : h)
+            if (v.version == version) {
+                if (parent) *parent = v.parent;
+                snap = v.snap;
+            }
+    }
+    if (!snap) throw Error("unknown model version (only the most recent versions are kept)");
+    auto b = snap->proto->clone_model();
+    b->trees.reserve(snap->trees.size());
+    for (auto& t : snap->trees) b->trees.push_back(*t);
+    if (version > 0) b->best_iteration = (int)b->trees.size();
+    return b;
 }
 
 ServerStats Server::stats() {
@@ -311,15 +323,9 @@ void Server::predict_loop() {
 // Bind a free slot to a refit request: the window's rows, its targets and the base version's
 // predictions (Booster::update: current model on the new rows, booster.cpp:859).
 void Server::start_refit(int slot, const std::shared_ptr<Req>& r) {
-    std::vector<double> pred0(r->n);
-    std::vector<int> rows(r->n);
-    for (int k = 0; k < r->n; ++k) rows[k] = r->row0 + k;
-    tgbm_predict_item it{r->base->forest.get(), rows.data(), nullptr, r->n, pred0.data()};
-    dev_check(tgbm_predict_batch(refit_pred_, sess_, &it, 1));
-    dev_check(tgbm_task_bind(sess_, slot, r->row0, r->n));
-    std::vector<int> zeros(r->n, 0);
-    dev_check(tgbm_task_set_rows(sess_, slot, r->y.data(), nullptr, pred0.data(), zeros.data(), nullptr, 1, nullptr,
-                                 nullptr));
+    dev_check(tgbm_task_bind(sess_, slot, r->row0, r->n));   // also clears the slot's buckets
+    dev_check(tgbm_task_predict_forest(sess_, slot, r->base->forest.get()));
+    dev_check(tgbm_task_set_rows(sess_, slot, r->y.data(), nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr));
     slots_[slot].req = r;
     slots_[slot].trees.clear();
     std::vector<double>().swap(r->y);   // inputs are on the device now
@@ -329,17 +335,38 @@ void Server::start_refit(int slot, const std::shared_ptr<Req>& r) {
 // trees and sets best_iteration to the new count, booster.cpp:880-883).
 void Server::publish_refit(Slot& sl) {
     Req& r = *sl.req;
-    auto m = r.base->model->clone_model();
-    for (auto& t : sl.trees) m->trees.push_back(std::move(t));
-    m->best_iteration = (int)m->trees.size();
+    const Version& base = *r.base;
+    auto snap = std::make_shared<Snap>();
+    snap->proto = base.snap->proto;
+    snap->trees = base.snap->trees;   // shared tree storage
+    for (auto& t : sl.trees) snap->trees.push_back(std::make_shared<const Tree>(std::move(t)));
+    // device forest: the parent's forest plus every tree after it (the new ones)
+    const int keep = base.forest_trees;
+    const int add = (int)snap->trees.size() - keep;
+    std::vector<int> tn(add), ints;
+    std::vector<double> vals, lrs(add, snap->proto->cfg.learning_rate);
+    for (int k = 0; k < add; ++k) {
+        const Tree& tr = *snap->trees[keep + k];
+        tn[k] = (int)tr.nodes.size();
+        for (size_t j = 0; j < tr.nodes.size(); ++j) {
+            const Node& d = tr.nodes[j];
+            const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.threshold, d.left, d.right,
+                               d.missing_go_left, d.n_samples, d.depth};
+            ints.insert(ints.end(), v, v + 10);
+            vals.push_back(tr.values.empty() ? d.value : tr.values[j]);
+        }
+    }
+    tgbm_forest* f = nullptr;
+    dev_check(tgbm_forest_append(&f, base.forest.get(), keep, add, tn.data(), ints.data(), vals.data(), lrs.data()));
     auto v = std::make_shared<Version>();
-    v->model = m;
-    v->forest = forest_of(*m);
-    v->parent = r.base->version;
+    v->snap = snap;
+    v->forest = own(f);
+    v->forest_trees = (int)snap->trees.size();
+    v->parent = base.version;
     std::lock_guard<std::mutex> lk(mu_);
     ModelState& ms = models_[r.model];
     v->version = ++ms.last_version;
-    ms.history.push_back(HistEntry{v->version, v->parent, v->model});
+    ms.history.push_back(HistEntry{v->version, v->parent, snap});
     while (ms.history.size() > kKeepVersions) ms.history.pop_front();
     ms.current = v;
     r.result_version = v->version;
@@ -399,14 +426,20 @@ void Server::refit_loop() {
                 failed.push_back(r);
             }
         }
+        const auto t_admit = Clock::now();
         if (!failed.empty()) finish(failed, err);
         if (n_active == 0) continue;
         std::vector<std::shared_ptr<Req>> done;
+        double t_pub = 0.0;
+        auto t_round = Clock::now();
         try {
             refit_round(n_active);
+            t_round = Clock::now();
             for (int i = n_active - 1; i >= 0; --i) {   // retire finished refits, keep slots packed
                 if ((int)slots_[i].trees.size() < slots_[i].req->extra_trees) continue;
+                const auto tp = Clock::now();
                 publish_refit(slots_[i]);
+                t_pub += secs(tp, Clock::now());
                 done.push_back(slots_[i].req);
                 const int last = n_active - 1;
                 if (i != last) {
@@ -428,6 +461,9 @@ void Server::refit_loop() {
             st_.refit_rounds += 1;
             st_.refit_requests += (long long)done.size();
             st_.refit_busy_s += secs(t, Clock::now());
+            st_.refit_admit_s += secs(t, t_admit);
+            st_.refit_round_s += secs(t_admit, t_round);
+            st_.refit_publish_s += t_pub;
         }
         if (!done.empty()) finish(done, err);
     }

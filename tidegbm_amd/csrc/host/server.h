@@ -19,6 +19,9 @@ struct ServerStats {
     long long predict_batches = 0, predict_requests = 0, predict_rows = 0;
     long long refit_rounds = 0, refit_requests = 0;
     double predict_busy_s = 0.0, refit_busy_s = 0.0;
+    // refit thread breakdown: admitting (base predictions, slot binding, row upload), tree
+    // rounds, publishing versions (model copy + device forest)
+    double refit_admit_s = 0.0, refit_round_s = 0.0, refit_publish_s = 0.0;
 };
 
 class Server {
@@ -48,9 +51,15 @@ private:
     struct Version;
     struct Req;
     struct Slot;
+    // Host side of a version: configuration / edges / init scores, and its trees, shared with the
+    // versions it was derived from (publishing a refit copies pointers, not trees).
+    struct Snap {
+        std::shared_ptr<const Booster> proto;   // fitted model fields without trees
+        std::vector<std::shared_ptr<const Tree>> trees;
+    };
     struct HistEntry {
         int version, parent;
-        std::shared_ptr<Booster> model;
+        std::shared_ptr<const Snap> snap;
     };
     // A version's device forest lives while it is current or an admitted request holds it; the
     // host snapshots of the last kKeepVersions versions stay queryable.
@@ -80,7 +89,6 @@ private:
     size_t predict_cap_ = 0;
     tgbm_session* sess_ = nullptr;
     tgbm_predictor* pred_ = nullptr;
-    tgbm_predictor* refit_pred_ = nullptr;
     std::vector<ModelState> models_;
     std::vector<Slot> slots_;
     std::mutex mu_;
