@@ -12,7 +12,7 @@ Roofline: the dominant kernel by device time is the child histogram build (k_his
 split).  A build over m rows reads m*F bin bytes + m*8 packed-gradient bytes + m*4 row-index
 bytes and writes F*n_bins*24 histogram bytes (the algorithmic bytes, with the reference's 24-byte
 HBin); HIP events around every launch on the session stream give the average duration.  The
-root build (k_hist_col, m = n, no row index) is reported beside it.  `traffic` comes from the
+root build (k_hist_root, m = n, no row index) is reported beside it.  `traffic` comes from the
 FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
 
 CPU baseline: the reference core (oracle/_ref, built from the reference's own C++ sources) on
@@ -143,8 +143,8 @@ def run_plain(args, p):
         },
         "roofline_root": {
             "bound": "hbm", "achieved": round(root_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(root_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_hist_col"),
-            "kernel": "root histogram build (k_hist_col), 1,000,000 x 128",
+            "frac": round(root_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_hist_root"),
+            "kernel": "root histogram build (k_hist_root), 1,000,000 x 128",
             "bytes_per_launch": root_bytes, "avg_launch_ms": round(root_ms, 5), "launches": n0,
         },
         "hist_tree_gbs": round(tree_gbs, 1),
@@ -310,6 +310,34 @@ def run_serve(args, p):
     }
 
 
+def run_ranking(args):
+    """Pairwise ranking fit at full size: 5,000 dates with U(200, 1000) entities (~3M rows) x 64
+    AR(1) features, within-date pairwise objective, 100 trees, depth 6, lr 0.08, reference
+    defaults (GOSS auto-on at >= 250k rows; PAIR splits auto-off at >= 1M rows)."""
+    import tidegbm_amd as t
+    from tidegbm_amd import synth
+
+    log("[ranking] generating the ranking panel")
+    p = synth.panel_ranking(2)
+    b = t.Booster(n_estimators=args.rank_trees, max_depth=6, learning_rate=0.08, objective=5, profile=0)
+    t0 = time.perf_counter()
+    b.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, rank_groups=p.ts, horizon=1)
+    wall = time.perf_counter() - t0
+    setup_s, boost_s = b.fit_times()
+    tt = b.tree_times()
+    host_trees, wait_s, sample_s = b.goss_stats()
+    return {
+        "workload": f"ranking panel: 5,000 dates x U(200,1000) entities = {p.n:,} rows x 64 features, "
+                    f"pairwise within dates, {args.rank_trees} trees, depth 6, lr 0.08, GOSS auto (on), "
+                    "PAIR auto (off at >= 1M rows)",
+        "rows": p.n, "trees": args.rank_trees, "fit_wall_s": round(wall, 3), "setup_s": round(setup_s, 3),
+        "boost_s": round(boost_s, 3), "trees_per_s": round(args.rank_trees / boost_s, 2),
+        "ms_per_tree_steady": round(1e3 * float(tt[-1] - tt[min(4, len(tt) - 1)]) / max(1, len(tt) - 5), 3),
+        "goss_host_trees": host_trees, "goss_wait_ms_per_tree": round(1e3 * wait_s / args.rank_trees, 3),
+        "goss_sample_ms_per_tree": round(1e3 * sample_s / args.rank_trees, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,7 +346,8 @@ def main():
     ap.add_argument("--cpu-trees", type=int, default=6)
     ap.add_argument("--profile-trees", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "plain", "walkforward", "serve"], default="all")
+    ap.add_argument("--workload", choices=["all", "plain", "walkforward", "serve", "ranking"], default="all")
+    ap.add_argument("--rank-trees", type=int, default=100, help="trees of the ranking workload")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving request stream")
     ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
     ap.add_argument("--serve-slots", type=int, default=32, help="concurrent refits when batched")
@@ -339,6 +368,9 @@ def main():
     from tidegbm_amd import synth
     if t.device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
+    if args.workload == "ranking":
+        print(json.dumps({"ranking": run_ranking(args)}), flush=True)
+        return
     log("[bench] generating the 1M x 128 panel")
     p = synth.panel_large(1)
     if args.workload == "walkforward":

@@ -100,6 +100,15 @@ int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h);
 int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sample,
                          const unsigned char* scaled, double scale);
 
+/* GOSS on the device (booster.cpp:554-585) for the next tree of `task`, from the gradients of
+ * the last tgbm_task_grad: keep the ceil(top*S) rows of largest |g|, draw the n_other =
+ * ceil(other*S) rows rest[perm[k]] from the remaining rows in |g|-descending order, scale their
+ * g and h by (1-top)/other.  perm is the first n_other entries of the permutation std::shuffle
+ * applies to the rest (it depends on the rest's size and the generator only).  When two rows
+ * share a |g| value, *tie is set and nothing changes (order such rows with std::sort on the host
+ * and call tgbm_task_set_sample). */
+int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other, const int* perm,
+                   int n_perm, int* tie);
 /* Start a tree for every task: quantize g/h, build the root histogram, scan it. */
 int tgbm_tree_begin(tgbm_session* s, const tgbm_grow_params* p);
 /* Device-driven best-first growth of every task's tree to completion (no host round trips). */

@@ -75,6 +75,11 @@ int tide_profile_log(TideHandle h, int family, double* ms, double* rows, int cap
 /* Fit timing breakdown of the last fit: seconds for setup (sort, bins, uploads) and for the
  * boosting loop. */
 int tide_fit_times(TideHandle h, double* setup_s, double* boost_s);
+/* GOSS diagnostics of the last fit: trees whose sample was drawn on the host (the device sample
+ * needs distinct |g| values past the kept top rows; a tree with equal ones is ordered by
+ * std::sort on the host, as the reference orders it, booster.cpp:561-562), seconds the loop
+ * waited for the generator draws, and seconds spent in the sample step (device or host). */
+int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sample_s);
 /* Wall time (s since the boosting loop began) at the end of each tree of the last fit; each
  * tree ends with a device synchronization.  Returns the number of trees. */
 int tide_tree_times(TideHandle h, double* out, int cap);

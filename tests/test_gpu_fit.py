@@ -4,6 +4,7 @@ Bar (stated per test): tree structure (every integer node field, leaf cover) bit
 TORB prefix values within rtol 1e-7 (they come from exact fixed-point sums instead of sequential
 FP64 folds); predictions within rtol 1e-5.
 """
+import dataclasses
 import sys
 
 import numpy as np
@@ -94,9 +95,36 @@ def test_small_panel_pairwise():
            pair_splits=0, objective=5)
 
 
+def test_ranking_panel_pairwise_goss_auto():
+    # The ranking configuration's shape at a tenth of its dates: uneven dates (200..1000 rows),
+    # 64 features, within-date pairwise objective, reference defaults (GOSS auto-on at >= 250k
+    # rows, PAIR splits allowed below 1M rows), checked against the oracle and the reference core.
+    p = synth.panel_ranking(3, n_dates=500)
+    assert p.n >= 250000
+    gpu, _ = _check("ranking-pairwise-goss-auto", p, rg=p.ts, n_pred=20000, n_estimators=5, max_depth=6,
+                    learning_rate=0.08, objective=5)
+    _say(f"  GOSS trees sampled on the host: {gpu.goss_stats()[0]} of 5")
+
+
 def test_small_panel_goss():
+    # distinct |g| values: the sample is taken on the device
     p = synth.panel_small(4)
-    _check("small-goss", p, n_estimators=10, max_depth=5, learning_rate=0.1, goss=1, pair_splits=0)
+    gpu, _ = _check("small-goss", p, n_estimators=10, max_depth=5, learning_rate=0.1, goss=1, pair_splits=0)
+    host_trees, _, _ = gpu.goss_stats()
+    _say(f"  GOSS trees sampled on the host: {host_trees} of 10")
+    assert host_trees < 10, "no tree took the device GOSS sample"
+
+
+def test_small_panel_goss_equal_gradients():
+    # targets on a 0.5 grid: rows of one leaf share |g|, the order of equal |g| decides the draw,
+    # and those trees are sampled on the host with std::sort as the reference samples them
+    p = synth.panel_small(6)
+    p = dataclasses.replace(p, y=np.round(p.y * 2.0) / 2.0)
+    gpu, _ = _check("small-goss-ties", p, n_estimators=8, max_depth=5, learning_rate=0.1, goss=1,
+                    pair_splits=0)
+    host_trees, _, _ = gpu.goss_stats()
+    _say(f"  GOSS trees sampled on the host: {host_trees} of 8")
+    assert host_trees > 0, "equal |g| values did not reach the host GOSS sample"
 
 
 def test_larger_panel_depth8_missing_values():

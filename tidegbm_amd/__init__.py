@@ -64,6 +64,7 @@ _sig("tide_get_edges", _I, _P, _I, _P, _I)
 _sig("tide_get_init_prefix", _I, _P, _P, _I)
 _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
+_sig("tide_goss_stats", _I, _P, _P, _P, _P)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
@@ -245,6 +246,13 @@ class Booster:
         a, b = _D(), _D()
         _lib.tide_fit_times(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def goss_stats(self):
+        """(trees sampled on the host because of equal |g|, seconds waiting for the generator
+        draws, seconds in the sample step) of the last GOSS fit."""
+        n, w, d = ctypes.c_int(), _D(), _D()
+        _lib.tide_goss_stats(self._h, ctypes.byref(n), ctypes.byref(w), ctypes.byref(d))
+        return n.value, w.value, d.value
 
 
 def fit_batch(X, timestamps, task_rows, targets, *, entity=None, horizons=None,

@@ -422,10 +422,14 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
 // With xcd_major set, the launch's blocks are renumbered so that the feature blocks of one row
 // chunk run on one XCD (workgroups are dealt to the 8 XCDs round-robin by linear id): the chunk's
 // packed gradients are then fetched into that XCD's L2 once instead of once per feature block.
-template <int FB, bool HV>
-__global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int rows_per_block, int xcd_major) {
-    __shared__ unsigned long long lA[FB * NBIN];
-    __shared__ unsigned long long lB[HV ? FB * NBIN : 1];
+// K copies per bin (K > 1): lane l adds into copy l % K and the K copies of a bin sit in
+// consecutive words, so for K = 32 the 64 lanes of a wave spread over all LDS banks whatever their
+// bins; the copies are summed (exactly: integer adds) before the partial is stored.
+template <int FB, int K, bool HV>
+__global__ __launch_bounds__(K > 1 ? 1024 : 512) void k_hist_col(Shared S, Task* tasks, int rows_per_block,
+                                                                 int xcd_major) {
+    __shared__ unsigned long long lA[FB * NBIN * K];
+    __shared__ unsigned long long lB[HV ? FB * NBIN * K : 1];
     const Task T = tasks[blockIdx.z];
     const int count = T.n_sample;
     int bx = blockIdx.x, by = blockIdx.y;
@@ -444,12 +448,13 @@ __global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int row
     const int r1 = min(count, r0 + rows_per_block);
     const int f0 = by * FB;
     const int nf = min(FB, S.n_features - f0);
-    for (int j = threadIdx.x; j < FB * NBIN; j += blockDim.x) {
+    for (int j = threadIdx.x; j < FB * NBIN * K; j += blockDim.x) {
         lA[j] = 0;
         if (HV) lB[j] = 0;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int rep = lane % K;
     const unsigned char* colb = S.bins_col + T.row0;  // row0 is a multiple of 16 here (launch_hist)
     for (int base = r0 + wave * 1024; base < r1; base += nwave * 1024) {
         const int row = base + lane * 16;
@@ -477,12 +482,12 @@ __global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int row
                     const unsigned long long p = (t & 1) ? ((unsigned long long)v.w << 32 | v.z)
                                                          : ((unsigned long long)v.y << 32 | v.x);
                     const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
-                    atomicAdd(&lA[fl * NBIN + bin], p);
+                    atomicAdd(&lA[(fl * NBIN + bin) * K + rep], p);
                     if (HV) {
                         const uint4 hw = hv[t >> 1];
                         const unsigned long long q = (t & 1) ? ((unsigned long long)hw.w << 32 | hw.z)
                                                              : ((unsigned long long)hw.y << 32 | hw.x);
-                        atomicAdd(&lB[fl * NBIN + bin], q);
+                        atomicAdd(&lB[(fl * NBIN + bin) * K + rep], q);
                     }
                 }
             }
@@ -492,14 +497,190 @@ __global__ __launch_bounds__(512) void k_hist_col(Shared S, Task* tasks, int row
                 const unsigned char* col = colb + (size_t)(f0 + fl) * S.n_pad;
                 for (int t = 0; t < 16 && row + t < r1; ++t) {
                     const int bin = col[row + t];
-                    atomicAdd(&lA[fl * NBIN + bin], T.pk[row + t]);
-                    if (HV) atomicAdd(&lB[fl * NBIN + bin], T.qh[row + t]);
+                    atomicAdd(&lA[(fl * NBIN + bin) * K + rep], T.pk[row + t]);
+                    if (HV) atomicAdd(&lB[(fl * NBIN + bin) * K + rep], T.qh[row + t]);
                 }
             }
         }
     }
     __syncthreads();
-    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, bx);
+    if (K == 1) {
+        hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, bx);
+        return;
+    }
+    // Sum the copies of each bin.  Lane j starts at copy j / (32 / K): the 32 lanes j..j+31 of a
+    // step then read 32 different bank pairs.
+    const size_t pbase = (size_t)bx * S.n_features * NBIN + (size_t)f0 * NBIN;
+    for (int j = threadIdx.x; j < nf * NBIN; j += blockDim.x) {
+        const int k0 = (j / (32 / K > 0 ? 32 / K : 1)) % K;
+        unsigned long long a = 0, c = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int r = (k0 + k) % K;
+            a += lA[j * K + r];
+            if (HV) c += lB[j * K + r];
+        }
+        T.part[pbase + j] = a;
+        if (HV) T.part_h[pbase + j] = c;
+    }
+}
+
+// Root build over the contiguous rows, one block per 16384 rows x a range of features.  Each
+// lane keeps the packed gradients of its 16 rows in registers and walks the block's features in
+// passes of FB = 64 / K features (32 / K when h varies): per pass it adds into a K-copy LDS
+// histogram (lane l updates copy l % K, and the K copies of a bin sit in consecutive words, so a
+// wave's adds spread over the LDS banks whatever their bins) while the next pass's bin columns
+// load.  After a pass the threads sum the copies of each bin.  The copies are never cleared: they
+// keep accumulating over the passes, and a thread stores the difference from its previous sum
+// (exact modulo 2^64, the packed words' own arithmetic; one pass adds at most 16384 rows, so the
+// packed count cannot carry).  Every lane takes the same path through the passes, so the next
+// pass's loads stay in flight during this pass's adds.
+constexpr int ROOT_RPL = 16;
+constexpr int ROOT_THREADS = 1024;
+constexpr int ROOT_ROWS = ROOT_THREADS * ROOT_RPL;
+
+// Bins of 16 rows starting at `row` for features f .. f+FB-1, clamped to the block's last feature
+// fl1 - 1 (so every call issues FB loads; the passes' load counts match and the adds of one pass
+// need not wait for the next pass's loads).  `row` is always a readable row: the bin columns hold
+// at least 64 rows past the last row.
+template <int FB>
+__device__ inline void root_load_bins(uint4 (&b)[FB], const unsigned char* colb, size_t n_pad, int f, int fl1,
+                                      int row) {
+#pragma unroll
+    for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(colb + (size_t)min(f + fl, fl1 - 1) * n_pad + row);
+}
+
+template <int K, bool HV>
+__global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* tasks, int feats_per_block,
+                                                             int xcd_major) {
+    static_assert(K == 16, "copies per bin: the sums' bank spread and the register budget assume 16");
+    constexpr int FB = (HV ? 32 : 64) / K;        // 128 KB of LDS
+    constexpr int WORDS = FB * NBIN;              // bins of one pass (per sum)
+    constexpr int SPT = WORDS * (HV ? 2 : 1) / ROOT_THREADS;   // bins summed per thread
+    __shared__ unsigned long long lds[WORDS * K * (HV ? 2 : 1)];
+    unsigned long long* lA = lds;
+    unsigned long long* lB = lds + WORDS * K;     // h sums (when h varies)
+    const Task T = tasks[blockIdx.z];
+    const int count = T.n_sample;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (xcd_major) {
+        // Blocks go out round-robin over the 8 XCDs (id % 8).  Each XCD gets a contiguous run of
+        // (row range, feature range) pairs, feature range fastest, so the blocks that read one
+        // row range's gradients share that XCD's L2.
+        const int nb = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * gridDim.x;
+        const int x = id % 8, l = id / 8;
+        const int base = nb / 8, rem = nb % 8;
+        const int j = x * base + min(x, rem) + l;
+        bx = j / gridDim.y;
+        by = j % gridDim.y;
+    }
+    if (bx == 0 && by == 0 && threadIdx.x == 0)
+        tasks[blockIdx.z].hist_parts = (count + ROOT_ROWS - 1) / ROOT_ROWS;
+    const int r0 = bx * ROOT_ROWS;
+    const int fr0 = by * feats_per_block;
+    const int fr1 = min(S.n_features, fr0 + feats_per_block);
+    if (r0 >= count || fr0 >= fr1) return;
+    for (int j = threadIdx.x; j < WORDS * K * (HV ? 2 : 1); j += ROOT_THREADS) lds[j] = 0;
+    const int lane = threadIdx.x & 63;
+    const int rep = lane % K;
+    const int row = r0 + threadIdx.x * ROOT_RPL;
+    const bool full = row + ROOT_RPL <= count;
+    // lanes wholly past the end read the last row group's bins and add zeros; a lane across the
+    // end reads past it (inside the column padding) and adds zeros for those rows
+    const int ld_row = row < count ? row : (count - 1) & ~(ROOT_RPL - 1);
+    const unsigned char* colb = S.bins_col + T.row0;  // row0 is a multiple of 16 here (launch_hist)
+    unsigned long long pg[ROOT_RPL], ph[HV ? ROOT_RPL : 1];
+    if (full) {
+        const uint4* pp = (const uint4*)(T.pk + row);
+#pragma unroll
+        for (int t = 0; t < ROOT_RPL / 2; ++t) {
+            const uint4 v = pp[t];
+            pg[2 * t] = (unsigned long long)v.y << 32 | v.x;
+            pg[2 * t + 1] = (unsigned long long)v.w << 32 | v.z;
+        }
+        if (HV) {
+            const uint4* qq = (const uint4*)(T.qh + row);
+#pragma unroll
+            for (int t = 0; t < ROOT_RPL / 2; ++t) {
+                const uint4 v = qq[t];
+                ph[2 * t] = (unsigned long long)v.y << 32 | v.x;
+                ph[2 * t + 1] = (unsigned long long)v.w << 32 | v.z;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < ROOT_RPL; ++t) {
+            pg[t] = row + t < count ? T.pk[row + t] : 0ull;
+            if (HV) ph[t] = row + t < count ? T.qh[row + t] : 0ull;
+        }
+    }
+    uint4 bc[FB], bn[FB];
+    root_load_bins<FB>(bc, colb, S.n_pad, fr0, fr1, ld_row);
+    // Sums: thread t owns bins w = t + i * ROOT_THREADS (i < SPT) of the pass, w < WORDS in lA and
+    // w >= WORDS in lB; it starts at copy (w / (32 / K)) % K, so 32 lanes read 32 bank pairs.
+    // Stores go through a global-segment pointer, so they count only against the vector memory
+    // counter.  A pass's sums are stored at the start of the next pass, ahead of that pass's
+    // loads: the wait for those loads then never waits on a store issued after them.
+    typedef __attribute__((address_space(1))) unsigned long long global_u64;
+    global_u64* outA = (global_u64*)T.part;
+    global_u64* outB = (global_u64*)T.part_h;
+    const size_t pbase = (size_t)bx * S.n_features * NBIN;
+    unsigned long long prev[SPT], pend[SPT];
+#pragma unroll
+    for (int i = 0; i < SPT; ++i) prev[i] = pend[i] = 0;
+    int pend_f = -1;
+    __syncthreads();
+    auto flush = [&]() {
+        if (pend_f < 0) return;
+        const int nw = min(FB, fr1 - pend_f) * NBIN;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int w = (int)threadIdx.x + i * ROOT_THREADS;
+            const int ws = w < WORDS ? w : w - WORDS;   // bin of the pass
+            if (ws < nw) (w < WORDS ? outA : outB)[pbase + (size_t)pend_f * NBIN + ws] = pend[i];
+        }
+    };
+    auto pass = [&](const uint4(&b)[FB], int f) {
+        const int nf = min(FB, fr1 - f);
+#pragma unroll
+        for (int fl = 0; fl < FB; ++fl) {
+            if (fl >= nf) continue;
+            const unsigned w4[4] = {b[fl].x, b[fl].y, b[fl].z, b[fl].w};
+#pragma unroll
+            for (int t = 0; t < ROOT_RPL; ++t) {
+                const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
+                atomicAdd(&lA[(fl * NBIN + bin) * K + rep], pg[t]);
+                if (HV) atomicAdd(&lB[(fl * NBIN + bin) * K + rep], ph[t]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int w = (int)threadIdx.x + i * ROOT_THREADS;
+            const int k0 = (w / (32 / K)) % K;
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) sum += lds[w * K + (k0 + k) % K];
+            pend[i] = sum - prev[i];
+            prev[i] = sum;
+        }
+        pend_f = f;
+        __syncthreads();
+    };
+    // the first pass's bins have arrived before the loop (vmcnt 0): inside it the adds then wait
+    // only for loads issued in the previous pass
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    int f = fr0;
+    for (; f + FB < fr1; f += FB) {
+        flush();
+        root_load_bins<FB>(bn, colb, S.n_pad, f + FB, fr1, ld_row);
+        pass(bc, f);
+#pragma unroll
+        for (int fl = 0; fl < FB; ++fl) bc[fl] = bn[fl];
+    }
+    flush();
+    pass(bc, f);
+    flush();
 }
 
 // Launch with optional start/stop events recorded at the kernel's own boundaries (AMD extended
@@ -515,9 +696,56 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                  hipEvent_t e1) {
     if (max_count <= 0) return;
     if (root && contiguous_root && (variant & 0xff) != 2) {
-        // Root variants (variant >> 8): bit 0 selects 16 features per block (else 8), bit 1 a
-        // 512-thread block (else 256), bit 3 XCD-major block placement.
+        // Root variants (variant >> 8).  0: k_hist_root with XCD-major placement and the
+        // features per block sized so the grid about fills the GPU once.  Bits 4-6 == 6:
+        // k_hist_root with fpb = 16 << bits 7-8 and bit 3 XCD-major placement.  Otherwise the
+        // single-pass k_hist_col: bit 0 selects 16 features per block (else 8), bits 1-2 a 512-
+        // or 1024-thread block (else 256; 1024 only with copies), bit 3 XCD-major placement, bits
+        // 4-6 log2 of the copies per bin K (0, or 2..5: K = 4..32 with 64 / K features per block,
+        // 32 / K when h varies; 128 KB of LDS).
         const int rv = variant >> 8;
+        const int lk = (rv >> 4) & 7;
+        if (rv == 0 || lk == 6) {
+            // k_hist_root: 16384 rows x fpb features per block
+            int fpb = 16 << ((rv >> 7) & 3);
+            int xcd = (rv >> 3) & 1;
+            if (rv == 0) {
+                // smallest power of two in [4, 64] with at most ~one block per CU (256 CUs)
+                const long long want = (long long)((max_count + ROOT_ROWS - 1) / ROOT_ROWS) * n_tasks * S.n_features;
+                fpb = 4;
+                while (fpb < 64 && want > 256LL * fpb) fpb *= 2;
+                xcd = 1;
+            }
+            dim3 grid((max_count + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
+            if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, S, tasks, fpb, xcd);
+            else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, S, tasks, fpb, xcd);
+            return;
+        }
+        if (lk >= 2 && lk <= 5) {
+            const int K = 1 << lk;
+            const int FB = (hconst ? 64 : 32) / K;
+            const int threads = 256 << std::min(2, (rv >> 1) & 3);
+            const int xcd = (rv >> 3) & 1;
+            const int fbs = (S.n_features + FB - 1) / FB;
+            int rpb = (int)(((long long)max_count * fbs / 256 + 4095) / 4096 * 4096);
+            rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, rpb));
+            const int need = (int)(((long long)max_count + S.part_cap - 1) / S.part_cap);
+            if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
+            dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
+#define ROOT_REP(KK)                                                                                          \
+    do {                                                                                                      \
+        if (hconst) TIMED_LAUNCH((k_hist_col<64 / KK, KK, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd); \
+        else TIMED_LAUNCH((k_hist_col<32 / KK, KK, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);      \
+    } while (0)
+            switch (K) {
+                case 4: ROOT_REP(4); break;
+                case 8: ROOT_REP(8); break;
+                case 16: ROOT_REP(16); break;
+                default: ROOT_REP(32); break;
+            }
+#undef ROOT_REP
+            return;
+        }
         const int FB = (rv & 1) ? 16 : 8;
         const int threads = 256 << std::min(1, (rv >> 1) & 3);
         const int xcd = (rv >> 3) & 1;
@@ -530,11 +758,11 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
         dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
         if (FB == 8) {
-            if (hconst) TIMED_LAUNCH((k_hist_col<8, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-            else TIMED_LAUNCH((k_hist_col<8, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            if (hconst) TIMED_LAUNCH((k_hist_col<8, 1, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            else TIMED_LAUNCH((k_hist_col<8, 1, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
         } else {
-            if (hconst) TIMED_LAUNCH((k_hist_col<16, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-            else TIMED_LAUNCH((k_hist_col<16, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            if (hconst) TIMED_LAUNCH((k_hist_col<16, 1, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
+            else TIMED_LAUNCH((k_hist_col<16, 1, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
         }
         return;
     }

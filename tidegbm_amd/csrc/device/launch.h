@@ -41,5 +41,11 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
                      int max_chunks, const GrowP& P, double min_cover, int iid, double lr);
 void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out);
+// GOSS selection (see goss.hip): temp storage bytes for n rows, and the launch sequence.
+size_t goss_temp_bytes(int n);
+hipError_t launch_goss(hipStream_t st, const double* g, int n, int n_top, int n_other, const int* perm,
+                       unsigned long long* k0, unsigned long long* k1, int* v0, int* v1, unsigned char* keep,
+                       unsigned char* scaled, int* sample_out, int* n_sample_out, int* tie, void* tmp,
+                       size_t tmp_bytes);
 
 }  // namespace tgbm

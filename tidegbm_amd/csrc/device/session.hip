@@ -57,6 +57,16 @@ struct TaskHost {
     TorbChunk* chunks_d = nullptr;   // device copy of chunks (capacity chunks_cap)
     int chunks_cap = 0;
     double* w_d = nullptr;           // weights (allocated on first use, cap_rows)
+    // device GOSS buffers (allocated on first use, cap_rows)
+    unsigned long long* gk0 = nullptr;
+    unsigned long long* gk1 = nullptr;
+    int* gv0 = nullptr;
+    int* gv1 = nullptr;
+    unsigned char* gkeep = nullptr;
+    int* gperm = nullptr;
+    int* gsmall = nullptr;           // [0] selected count, [1] tie flag
+    void* gtmp = nullptr;
+    size_t gtmp_bytes = 0;
     int* gstart_d = nullptr;         // pairwise groups (allocated on first use)
     int* grows_d = nullptr;
     bool has_sample = false;
@@ -644,6 +654,59 @@ int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sam
             H.goss = false;
             s->set_field_int(task, offsetof(Task, goss), 0);
         }
+    });
+}
+
+int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other, const int* perm, int n_perm,
+                   int* tie) {
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        if (s->hconst) throw std::runtime_error("GOSS scales h: the session must not declare h == 1");
+        Task& T = s->tasks_h[task];
+        TaskHost& H = s->th[task];
+        const int S = H.n_rows;
+        const int n_top = std::min(S, std::max(1, (int)std::ceil(goss_top * S)));
+        const int R = S - n_top;
+        const int n_other = std::min((int)std::ceil(goss_other * S), R);
+        if (n_perm != n_other) throw std::runtime_error("GOSS draw size does not match the rows");
+        for (int k = 0; k < n_perm; ++k)
+            if (perm[k] < 0 || perm[k] >= R) throw std::runtime_error("GOSS draw index out of range");
+        if (!H.gk0) {
+            const size_t c = (size_t)H.cap_rows;
+            H.gk0 = s->alloc<unsigned long long>(c);
+            H.gk1 = s->alloc<unsigned long long>(c);
+            H.gv0 = s->alloc<int>(c);
+            H.gv1 = s->alloc<int>(c);
+            H.gkeep = s->alloc<unsigned char>(c);
+            H.gperm = s->alloc<int>(c);
+            H.gsmall = s->alloc<int>(2);
+            H.gtmp_bytes = goss_temp_bytes(H.cap_rows);
+            H.gtmp = s->alloc<char>(H.gtmp_bytes);
+        }
+        unsigned char* sd = const_cast<unsigned char*>(T.scaled);
+        if (!sd) {
+            sd = s->alloc<unsigned char>(H.cap_rows);
+            T.scaled = sd;
+            s->set_field_ptr(task, offsetof(Task, scaled), sd);
+        }
+        if (n_other > 0)
+            HIP_OK(hipMemcpyAsync(H.gperm, perm, sizeof(int) * n_other, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemsetAsync(H.gsmall, 0, sizeof(int) * 2, s->stream));
+        HIP_OK(launch_goss(s->stream, T.g, S, n_top, n_other, H.gperm, H.gk0, H.gk1, H.gv0, H.gv1, H.gkeep, sd,
+                           T.idx, H.gsmall, H.gsmall + 1, H.gtmp, H.gtmp_bytes));
+        int res[2] = {0, 0};
+        HIP_OK(hipMemcpyAsync(res, H.gsmall, sizeof res, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        *tie = res[1] != 0;
+        if (*tie) return;   // equal |g| values: the caller orders them with std::sort on the host
+        if (res[0] != n_top + n_other) throw std::runtime_error("internal: GOSS selected an unexpected row count");
+        H.has_sample = true;
+        H.n_sample = res[0];
+        H.goss = true;
+        s->set_field_int(task, offsetof(Task, n_sample), res[0]);
+        s->set_field_int(task, offsetof(Task, contiguous), 0);
+        s->set_field_int(task, offsetof(Task, goss), 1);
+        s->set_field_double(task, offsetof(Task, goss_scale), (1.0 - goss_top) / std::max(goss_other, 1e-12));
     });
 }
 

@@ -8,7 +8,11 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -228,27 +232,98 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
     }
 }
 
-// GOSS sample (reference booster.cpp:554-585): std::sort by |g| descending, keep the top
-// ceil(top*S), shuffle the rest with the fit's mt19937 and keep ceil(other*S) of them scaled.
-void goss_sample(const std::vector<double>& g, int n, const Config& c, std::mt19937& rng,
-                 std::vector<int>& sample, std::vector<unsigned char>& scaled, double& scale) {
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return std::abs(g[a]) > std::abs(g[b]); });
-    const int S = n;
-    const int n_top = std::min(S, std::max(1, (int)std::ceil(c.goss_top * S)));
-    std::vector<char> keep(n, 0);
-    for (int i = 0; i < n_top; ++i) keep[order[i]] = 1;
-    std::vector<int> rest(order.begin() + n_top, order.end());
-    int n_other = (int)std::ceil(c.goss_other * S);
-    n_other = std::min(n_other, (int)rest.size());
-    std::shuffle(rest.begin(), rest.end(), rng);
-    scale = (1.0 - c.goss_top) / std::max(c.goss_other, 1e-12);
-    scaled.assign(n, 0);
-    for (int i = 0; i < n_other; ++i) {
-        keep[rest[i]] = 1;
-        scaled[rest[i]] = 1;
+// GOSS row counts of one tree (reference booster.cpp:564-570).
+struct GossCounts {
+    int n_top, n_rest, n_other;
+};
+GossCounts goss_counts(int S, const Config& c) {
+    GossCounts k;
+    k.n_top = std::min(S, std::max(1, (int)std::ceil(c.goss_top * S)));
+    k.n_rest = S - k.n_top;
+    k.n_other = std::min((int)std::ceil(c.goss_other * S), k.n_rest);
+    return k;
+}
+
+// The generator draws of a GOSS fit, produced ahead of the trees.  Per tree the reference makes
+// one std::shuffle of the rest (booster.cpp:571) and one tree seed draw (:587).  The shuffle's
+// swaps depend on the generator and the rest's size only, never on the values shuffled, so the
+// rest after the shuffle is rest[P[i]] with P the same call applied to 0..n_rest-1.  A thread
+// computes P for each tree in order and queues its first n_other entries (the drawn positions);
+// the generator is that thread's alone until the object is destroyed.
+class GossDraws {
+  public:
+    GossDraws(std::mt19937& rng, int n_rest, int n_other, int n_trees)
+        : rng_(rng), n_rest_(n_rest), n_other_(n_other), n_trees_(n_trees) {
+        th_ = std::thread([this] { run(); });
     }
+    ~GossDraws() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    // The drawn positions of the next tree (valid until the following call).
+    const std::vector<int>& next() {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !q_.empty(); });
+        cur_ = std::move(q_.front());
+        q_.pop_front();
+        cv_.notify_all();
+        return cur_;
+    }
+
+  private:
+    static constexpr size_t kAhead = 4;
+    void run() {
+        std::vector<int> p(n_rest_);
+        for (int m = 0; m < n_trees_; ++m) {
+            std::iota(p.begin(), p.end(), 0);
+            std::shuffle(p.begin(), p.end(), rng_);
+            (void)rng_();  // the tree's own seed; only colsample would use it
+            std::vector<int> head(p.begin(), p.begin() + n_other_);
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return stop_ || q_.size() < kAhead; });
+            if (stop_) return;
+            q_.push_back(std::move(head));
+            cv_.notify_all();
+        }
+    }
+    std::mt19937& rng_;
+    const int n_rest_, n_other_, n_trees_;
+    std::thread th_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<std::vector<int>> q_;
+    std::vector<int> cur_;
+    bool stop_ = false;
+};
+
+// GOSS sample on the host (reference booster.cpp:557-583) with the drawn rest positions of the
+// tree.  std::sort with the reference's comparator orders equal |g| in an unspecified but fixed
+// way; sorting (|g|, row) records by |g| performs the same comparisons and moves as sorting row
+// indices through g, so it yields the same order, with contiguous reads.
+void goss_sample_host(const std::vector<double>& g, int n, const Config& c, const std::vector<int>& drawn,
+                      std::vector<int>& sample, std::vector<unsigned char>& scaled, double& scale) {
+    struct Rec {
+        double a;
+        int row;
+    };
+    std::vector<Rec> order(n);
+    for (int i = 0; i < n; ++i) order[i] = Rec{std::abs(g[i]), i};
+    std::sort(order.begin(), order.end(), [](const Rec& x, const Rec& y) { return x.a > y.a; });
+    const GossCounts k = goss_counts(n, c);
+    if ((int)drawn.size() != k.n_other) throw Error("internal: GOSS draw size");
+    std::vector<char> keep(n, 0);
+    scaled.assign(n, 0);
+    for (int i = 0; i < k.n_top; ++i) keep[order[i].row] = 1;
+    for (int i = 0; i < k.n_other; ++i) {
+        const int row = order[k.n_top + drawn[i]].row;
+        keep[row] = 1;
+        scaled[row] = 1;
+    }
+    scale = (1.0 - c.goss_top) / std::max(c.goss_other, 1e-12);
     sample.clear();
     for (int i = 0; i < n; ++i)
         if (keep[i]) sample.push_back(i);
@@ -605,18 +680,36 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
     std::vector<double> g;
     std::vector<int> sample;
     std::vector<unsigned char> scaled;
+    // GOSS (booster.cpp:557-583) runs on the device; the generator's draws come from a thread
+    // that works ahead, and a tree whose |g| has ties is sampled on the host with std::sort.
+    std::unique_ptr<GossDraws> draws;
+    const GossCounts gk = goss_counts(n, cfg);
+    if (goss) draws.reset(new GossDraws(rng, gk.n_rest, gk.n_other, n_trees));
+    goss_host_trees = 0;
+    goss_wait_s = goss_sample_s = 0.0;
     const double t_loop = now_s();
     if (!for_update) tree_end_times.clear();
     for (int m = 0; m < n_trees; ++m) {
         dev_check(tgbm_task_grad(s, -1, cfg.objective));
         if (goss) {
-            g.resize(n);
-            dev_check(tgbm_task_get_grad(s, 0, g.data(), nullptr));
-            double scale = 1.0;
-            goss_sample(g, n, cfg, rng, sample, scaled, scale);
-            dev_check(tgbm_task_set_sample(s, 0, sample.data(), (int)sample.size(), scaled.data(), scale));
+            const double t0 = now_s();
+            const std::vector<int>& drawn = draws->next();
+            const double t1 = now_s();
+            goss_wait_s += t1 - t0;
+            int tie = 0;
+            dev_check(tgbm_task_goss(s, 0, cfg.goss_top, cfg.goss_other, drawn.data(), (int)drawn.size(), &tie));
+            if (tie) {
+                ++goss_host_trees;
+                g.resize(n);
+                dev_check(tgbm_task_get_grad(s, 0, g.data(), nullptr));
+                double scale = 1.0;
+                goss_sample_host(g, n, cfg, drawn, sample, scaled, scale);
+                dev_check(tgbm_task_set_sample(s, 0, sample.data(), (int)sample.size(), scaled.data(), scale));
+            }
+            goss_sample_s += now_s() - t1;
+        } else {
+            (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample would use it
         }
-        (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample would use it
         dev_check(tgbm_tree_begin(s, &gp));
         if (host_driven) grow_host_driven(s, cfg, F, gp);
         else dev_check(tgbm_tree_grow_device(s, &gp));

@@ -117,6 +117,8 @@ public:
     // last fit diagnostics
     double t_setup = 0.0, t_boost = 0.0;
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
+    int goss_host_trees = 0;             // GOSS trees of the last fit sampled on the host (equal |g|)
+    double goss_wait_s = 0.0, goss_sample_s = 0.0;  // GOSS: waiting for draws, the sample step
     double prof_ms[8] = {0}, prof_bytes[8] = {0};
     long long prof_n[8] = {0};
     std::vector<double> prof_log_ms[8], prof_log_rows[8];

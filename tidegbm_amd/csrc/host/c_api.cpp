@@ -289,6 +289,14 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
     return 0;
 }
 
+int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sample_s) {
+    auto* b = static_cast<Booster*>(h);
+    *host_trees = b->goss_host_trees;
+    *wait_s = b->goss_wait_s;
+    *sample_s = b->goss_sample_s;
+    return 0;
+}
+
 int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features, const long long* timestamps,
                    const long long* entity, int n_tasks, const int* task_rows, const double* const* y,
                    const long long* horizons, int schedule, TideHandle* out) {
