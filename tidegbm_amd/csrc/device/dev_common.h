@@ -215,30 +215,117 @@ __device__ inline double leaf_value(double G, double H, double l2, double l1, do
 }
 
 // ---------------------------------------------------------------- wave helpers (wave64)
-__device__ inline long long wave_sum_ll(long long v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// Lane moves through DPP (data-parallel primitives of the VALU, a few cycles each) instead of
+// LDS-routed shuffles.  Every helper must be called by all 64 lanes of the wave.  Controls
+// (GFX9 encoding): row_shr:n = 0x110 + n within rows of 16 lanes; row_bcast:15 = 0x142 (lane 15
+// of each row into the next row); row_bcast:31 = 0x143 (lane 31 into rows 2 and 3).  A lane
+// whose source is outside its row, or whose row is masked off, receives `id`.
+template <int CTRL, int ROW_MASK>
+__device__ inline int dpp_i(int v, int id) {
+    return __builtin_amdgcn_update_dpp(id, v, CTRL, ROW_MASK, 0xf, false);
 }
-__device__ inline int wave_sum_i(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+template <int CTRL, int ROW_MASK>
+__device__ inline unsigned long long dpp_u64(unsigned long long v, unsigned long long id) {
+    const unsigned lo = (unsigned)dpp_i<CTRL, ROW_MASK>((int)(unsigned)v, (int)(unsigned)id);
+    const unsigned hi = (unsigned)dpp_i<CTRL, ROW_MASK>((int)(unsigned)(v >> 32), (int)(unsigned)(id >> 32));
+    return (unsigned long long)hi << 32 | lo;
 }
-// Inclusive prefix over lanes.
+__device__ inline unsigned long long read_lane_u64(unsigned long long v, int lane) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return (unsigned long long)hi << 32 | lo;
+}
+// Inclusive scan over the lanes with an associative operation (Hillis-Steele in each row, then
+// the row totals): lane 63 ends with the whole wave's result.
+template <class Op>
+__device__ inline unsigned long long wave_scan_u64(unsigned long long x, unsigned long long id, Op op) {
+    x = op(x, dpp_u64<0x111, 0xf>(x, id));
+    x = op(x, dpp_u64<0x112, 0xf>(x, id));
+    x = op(x, dpp_u64<0x114, 0xf>(x, id));
+    x = op(x, dpp_u64<0x118, 0xf>(x, id));
+    x = op(x, dpp_u64<0x142, 0xa>(x, id));
+    x = op(x, dpp_u64<0x143, 0xc>(x, id));
+    return x;
+}
+template <class Op>
+__device__ inline int wave_scan_i(int x, int id, Op op) {
+    x = op(x, dpp_i<0x111, 0xf>(x, id));
+    x = op(x, dpp_i<0x112, 0xf>(x, id));
+    x = op(x, dpp_i<0x114, 0xf>(x, id));
+    x = op(x, dpp_i<0x118, 0xf>(x, id));
+    x = op(x, dpp_i<0x142, 0xa>(x, id));
+    x = op(x, dpp_i<0x143, 0xc>(x, id));
+    return x;
+}
+struct OpAdd64 {
+    __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a + b; }
+};
+struct OpMax64 {
+    __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a > b ? a : b; }
+};
+struct OpAddI {
+    __device__ int operator()(int a, int b) const { return a + b; }
+};
+struct OpMinI {
+    __device__ int operator()(int a, int b) const { return a < b ? a : b; }
+};
+// Inclusive prefix over lanes (two's complement sums: signed and unsigned adds agree).
 __device__ inline long long wave_incl_ll(long long v) {
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
+    return (long long)wave_scan_u64((unsigned long long)v, 0ull, OpAdd64());
+}
+__device__ inline int wave_incl_i(int v) { return wave_scan_i(v, 0, OpAddI()); }
+// Totals, the same in every lane.
+__device__ inline long long wave_sum_ll(long long v) { return (long long)read_lane_u64((unsigned long long)wave_incl_ll(v), 63); }
+__device__ inline int wave_sum_i(int v) { return __builtin_amdgcn_readlane(wave_incl_i(v), 63); }
+__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
+    return read_lane_u64(wave_scan_u64(v, 0ull, OpMax64()), 63);
+}
+__device__ inline int wave_min_i(int v) {
+    return __builtin_amdgcn_readlane(wave_scan_i(v, 0x7fffffff, OpMinI()), 63);
+}
+// Order-preserving 64-bit key of a double (larger double, larger key; -0 and +0 equal); NaN
+// maps to 0, below every number.
+__device__ inline unsigned long long gain_key(double d) {
+    d = d + 0.0;
+    if (d != d) return 0ull;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+// The lane holding the wave's best candidate: largest gain, then smallest key.
+__device__ inline int wave_best_lane(double gain, int key) {
+    const unsigned long long gk = gain_key(gain);
+    const unsigned long long m = wave_max_u64(gk);
+    const int kmin = wave_min_i(gk == m ? key : 0x7fffffff);
+    return (int)__builtin_ctzll(__ballot(gk == m && key == kmin));
+}
+
+// Global-segment view of a device pointer.  Pointers read from memory (the Task descriptors)
+// are generic to the compiler, and generic accesses also count against the LDS/scalar counter,
+// so every LDS wait would wait for them as well; hot kernels access through this view.
+template <class T>
+__device__ inline __attribute__((address_space(1))) T* gbl(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
+// Load / store of a whole struct or vector through the global segment (4-byte pieces, which the
+// compiler merges into wide accesses when the type's alignment allows).
+template <class T>
+__device__ inline T ldg(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "4-byte pieces");
+    T v;
+    const auto s = (const __attribute__((address_space(1))) unsigned*)p;
+    unsigned* d = (unsigned*)&v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = s[i];
     return v;
 }
-__device__ inline int wave_incl_i(int v) {
-    const int lane = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
+template <class T>
+__device__ inline void stg(T* p, const T& v) {
+    static_assert(sizeof(T) % 4 == 0, "4-byte pieces");
+    const auto d = (__attribute__((address_space(1))) unsigned*)p;
+    const unsigned* s = (const unsigned*)&v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = s[i];
 }
 
 // Candidate ordering: larger gain first, then smaller key (feature/bin/missing order).

@@ -65,14 +65,6 @@ void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edg
 }
 
 // =================================================================== reductions
-__device__ inline unsigned long long wave_max_u64(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned long long t = __shfl_xor(v, o, 64);
-        v = t > v ? t : v;
-    }
-    return v;
-}
-
 // Block maximum of two values, written to the task's per-block slots (no global atomics).
 __device__ inline void block_max2_store(unsigned long long a, unsigned long long b, Task& T) {
     __shared__ unsigned long long sa[16], sb[16];
@@ -339,9 +331,11 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
                                           const unsigned long long* lB, int FB, int f0, int F, int slot) {
     const size_t base = (size_t)slot * F * NBIN + (size_t)f0 * NBIN;
     const int words = min(FB, F - f0) * NBIN;
+    auto part = gbl(T.part);
+    auto part_h = gbl(T.part_h);
     for (int j = threadIdx.x; j < words; j += blockDim.x) {
-        T.part[base + j] = lA[j];
-        if (HV) T.part_h[base + j] = lB[j];
+        part[base + j] = lA[j];
+        if (HV) part_h[base + j] = lB[j];
     }
 }
 
@@ -372,9 +366,9 @@ __global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int roo
     for (int j = threadIdx.x; j < FB * NBIN * (HV ? 2 : 1); j += blockDim.x) lds[j] = 0;
     const int sub = threadIdx.x % LPR;
     const int fcol = f0 + sub * 16;
-    const int* __restrict__ rows = (buf ? T.idx_tmp : T.idx) + start;
-    const unsigned long long* __restrict__ pk = T.pk;
-    const unsigned long long* __restrict__ qhp = T.qh;
+    const auto rows = gbl((const int*)(buf ? T.idx_tmp : T.idx) + start);
+    const auto pk = gbl((const unsigned long long*)T.pk);
+    const auto qhp = gbl((const unsigned long long*)T.qh);
     const bool contig = T.contiguous && root;
     const int stride = blockDim.x / LPR;
     const bool active = fcol < S.fp;
@@ -594,7 +588,7 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* task
         const uint4* pp = (const uint4*)(T.pk + row);
 #pragma unroll
         for (int t = 0; t < ROOT_RPL / 2; ++t) {
-            const uint4 v = pp[t];
+            const uint4 v = ldg(pp + t);
             pg[2 * t] = (unsigned long long)v.y << 32 | v.x;
             pg[2 * t + 1] = (unsigned long long)v.w << 32 | v.z;
         }
@@ -602,7 +596,7 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* task
             const uint4* qq = (const uint4*)(T.qh + row);
 #pragma unroll
             for (int t = 0; t < ROOT_RPL / 2; ++t) {
-                const uint4 v = qq[t];
+                const uint4 v = ldg(qq + t);
                 ph[2 * t] = (unsigned long long)v.y << 32 | v.x;
                 ph[2 * t + 1] = (unsigned long long)v.w << 32 | v.z;
             }
@@ -610,8 +604,8 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* task
     } else {
 #pragma unroll
         for (int t = 0; t < ROOT_RPL; ++t) {
-            pg[t] = row + t < count ? T.pk[row + t] : 0ull;
-            if (HV) ph[t] = row + t < count ? T.qh[row + t] : 0ull;
+            pg[t] = row + t < count ? gbl(T.pk)[row + t] : 0ull;
+            if (HV) ph[t] = row + t < count ? gbl(T.qh)[row + t] : 0ull;
         }
     }
     uint4 bc[FB], bn[FB];
@@ -621,9 +615,8 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* task
     // Stores go through a global-segment pointer, so they count only against the vector memory
     // counter.  A pass's sums are stored at the start of the next pass, ahead of that pass's
     // loads: the wait for those loads then never waits on a store issued after them.
-    typedef __attribute__((address_space(1))) unsigned long long global_u64;
-    global_u64* outA = (global_u64*)T.part;
-    global_u64* outB = (global_u64*)T.part_h;
+    const auto outA = gbl(T.part);
+    const auto outB = gbl(T.part_h);
     const size_t pbase = (size_t)bx * S.n_features * NBIN;
     unsigned long long prev[SPT], pend[SPT];
 #pragma unroll
@@ -816,19 +809,10 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
                                          double l1, double& out_gain, int& out_thr, int& out_ml,
                                          int& out_cl, long long& totQ, long long& totQh, long long& totC) {
     const int lane = threadIdx.x & 63;
-    long long lg = 0, lh = 0;
-    long long lc = 0;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        lg += qg[t];
-        lh += qh[t];
-        lc += c[t];
-    }
-    totQ = wave_sum_ll(lg);
-    totQh = wave_sum_ll(lh);
-    totC = wave_sum_ll(lc);
-    const long long missQ = __shfl(qg[0], 0, 64), missQh = __shfl(qh[0], 0, 64);
-    const long long missC = __shfl((long long)c[0], 0, 64);
+    // bin 0 (missing) sits in lane 0
+    const long long missQ = (long long)read_lane_u64((unsigned long long)qg[0], 0);
+    const long long missQh = hconst ? 0 : (long long)read_lane_u64((unsigned long long)qh[0], 0);
+    const long long missC = __builtin_amdgcn_readlane(c[0], 0);
     // prefix over bins 1..b (bin 0 excluded)
     long long pg[4], ph[4], pc[4];
     long long rg = 0, rh = 0, rc = 0;
@@ -844,7 +828,12 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
         ph[t] = rh;
         pc[t] = rc;
     }
-    const long long eg = wave_incl_ll(rg) - rg, eh = wave_incl_ll(rh) - rh, ec = wave_incl_ll(rc) - rc;
+    const long long ig = wave_incl_ll(rg), ic = wave_incl_ll(rc), ih = hconst ? 0 : wave_incl_ll(rh);
+    const long long eg = ig - rg, eh = ih - (hconst ? 0 : rh), ec = ic - rc;
+    // totals over all bins: bins 1.. from lane 63's inclusive prefix, plus the missing bin
+    totQ = (long long)read_lane_u64((unsigned long long)ig, 63) + missQ;
+    totQh = hconst ? 0 : (long long)read_lane_u64((unsigned long long)ih, 63) + missQh;
+    totC = (long long)read_lane_u64((unsigned long long)ic, 63) + missC;
     const double tG = (double)totQ * inv_g;
     const double tH = hconst ? (double)totC : (double)totQh * inv_h;
     const double parent = leaf_score(tG, tH, lam, l1);
@@ -874,16 +863,11 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
             }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const double og = __shfl_xor(best, o, 64);
-        const int ok = __shfl_xor(bkey, o, 64);
-        const int oc = __shfl_xor(bcl, o, 64);
-        if (cand_better(og, ok, best, bkey)) {
-            best = og;
-            bkey = ok;
-            bcl = oc;
-        }
-    }
+    // lanes hold increasing keys, so the wave's first maximum is the best lane's best
+    const int wl = wave_best_lane(best, bkey);
+    best = __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(best), wl));
+    bkey = __builtin_amdgcn_readlane(bkey, wl);
+    bcl = __builtin_amdgcn_readlane(bcl, wl);
     out_gain = best;
     out_thr = (bkey == 0x7fffffff) ? -1 : bkey >> 1;
     out_ml = (bkey == 0x7fffffff) ? 1 : (bkey & 1);
@@ -895,6 +879,7 @@ struct Decision {
     int leaf;              // -1: the tree stops growing
     int type, feat, feat2, thr, mgl;
     int nl;                // rows the split sends left (from the node's histogram; -1: unknown)
+    int start, count, buf; // the leaf's rows: range in the index buffer `buf`
     double gain;
 };
 
@@ -911,17 +896,26 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
     __shared__ DCand pc[2];
     __shared__ double lg[16];
     __shared__ int lk[16];
+    // every node's cached candidate and row range, so the final choice reads no global memory
+    __shared__ DCand ncand[MAX_NODES];
+    __shared__ int nstart[MAX_NODES], ncount[MAX_NODES], nbuf[MAX_NODES];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = blockDim.x >> 6;
     const int pend[2] = {T.pending0, T.pending1};
     const int nn = T.n_nodes;
-    // inputs of the leaf choice that do not depend on this step's scan
+    // inputs of the leaf choice that do not depend on this step's scan (one node per thread;
+    // the block has at least MAX_NODES threads)
     const int lid = threadIdx.x;
     DCand lc;
     lc.valid = 0;
     bool lleaf = false;
     if (lid < nn) {
-        lc = T.cands[lid];
-        lleaf = T.nodes[lid].v[N_LEAF] != 0;
+        lc = ldg(T.cands + lid);
+        const DNode nd = ldg(T.nodes + lid);
+        lleaf = nd.v[N_LEAF] != 0;
+        ncand[lid] = lc;
+        nstart[lid] = nd.seg_start;
+        ncount[lid] = nd.seg_count;
+        nbuf[lid] = nd.buf;
     }
     int ndepth[2] = {0, 0}, nns[2] = {0, 0};
     if (threadIdx.x == 0) {
@@ -939,7 +933,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
         const int nd = pend[pi];
         if (nd < 0) continue;
         for (int f = threadIdx.x; f < F; f += blockDim.x) {
-            const FCand c = T.fcand[(size_t)nd * F + f];
+            const FCand c = ldg(T.fcand + (size_t)nd * F + f);
             if (c.thr < 0 || c.gain < P.gamma) continue;
             if (cand_better(c.gain, f, best[pi], bk[pi])) {
                 best[pi] = c.gain;
@@ -952,20 +946,12 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
     }
 #pragma unroll
     for (int pi = 0; pi < 2; ++pi) {
-        for (int o = 32; o > 0; o >>= 1) {
-            const double og = __shfl_xor(best[pi], o, 64);
-            const int ok = __shfl_xor(bk[pi], o, 64);
-            const int ot = __shfl_xor(bt[pi], o, 64);
-            const int om = __shfl_xor(bm[pi], o, 64);
-            const int oc = __shfl_xor(bcl[pi], o, 64);
-            if (cand_better(og, ok, best[pi], bk[pi])) {
-                best[pi] = og;
-                bk[pi] = ok;
-                bt[pi] = ot;
-                bm[pi] = om;
-                bcl[pi] = oc;
-            }
-        }
+        const int wl = wave_best_lane(best[pi], bk[pi]);
+        best[pi] = __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(best[pi]), wl));
+        bk[pi] = __builtin_amdgcn_readlane(bk[pi], wl);
+        bt[pi] = __builtin_amdgcn_readlane(bt[pi], wl);
+        bm[pi] = __builtin_amdgcn_readlane(bm[pi], wl);
+        bcl[pi] = __builtin_amdgcn_readlane(bcl[pi], wl);
         if (lane == 0) {
             sg[pi][wave] = best[pi];
             sk[pi][wave] = bk[pi];
@@ -1016,13 +1002,10 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
             lbid = lid;
         }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const double og = __shfl_xor(lb, o, 64);
-        const int ok = __shfl_xor(lbid, o, 64);
-        if (cand_better(og, ok, lb, lbid)) {
-            lb = og;
-            lbid = ok;
-        }
+    {
+        const int wl = wave_best_lane(lb, lbid);
+        lb = __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(lb), wl));
+        lbid = __builtin_amdgcn_readlane(lbid, wl);
     }
     if (lane == 0) {
         lg[wave] = lb;
@@ -1038,8 +1021,11 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
         if (T.n_nodes + 2 > MAX_NODES || T.n_nodes + 2 > 2 * P.max_leaves - 1 || lbid == 0x7fffffff) {
             out.leaf = -1;
         } else {
-            const DCand c = lbid == pend[0] ? pc[0] : (lbid == pend[1] ? pc[1] : T.cands[lbid]);
+            const DCand c = lbid == pend[0] ? pc[0] : (lbid == pend[1] ? pc[1] : ncand[lbid]);
             out.leaf = lbid;
+            out.start = nstart[lbid];
+            out.count = ncount[lbid];
+            out.buf = nbuf[lbid];
             out.type = c.type;
             out.feat = c.feature;
             out.feat2 = c.feature2;
@@ -1109,8 +1095,8 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
-            v[2 * u] = a[0];
-            v[2 * u + 1] = a[1];
+            v[2 * u] = ldg(a);
+            v[2 * u + 1] = ldg(a + 1);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -1124,7 +1110,7 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
         }
     }
     for (; k < np; k += dk) {
-        const unsigned long long* w = T.part + (size_t)k * stride + off;
+        const auto w = gbl((const unsigned long long*)T.part + (size_t)k * stride + off);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             sq[t] += unpack_q(w[t]);
@@ -1133,7 +1119,7 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
     }
     if (!T.hconst) {
         for (int j = k0; j < np; j += dk) {
-            const unsigned long long* w = T.part_h + (size_t)j * stride + off;
+            const auto w = gbl((const unsigned long long*)T.part_h + (size_t)j * stride + off);
 #pragma unroll
             for (int t = 0; t < 4; ++t) sh[t] += (long long)w[t];
         }
@@ -1159,11 +1145,13 @@ __device__ inline void scan_store(const Shared& S, const Task& T, const GrowP& P
     scan_feature_wave(qg, c, qh, T.hconst, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf, P.min_hess, P.l1,
                       gain, thr, ml, cl, tq, tqh, tc);
     if (lane == 0) {
-        FCand& o = T.fcand[(size_t)node * S.n_features + f];
+        FCand o;
         o.gain = gain;
         o.thr = thr;
         o.ml = ml;
         o.cl = cl;
+        o.pad = 0;
+        stg(T.fcand + (size_t)node * S.n_features + f, o);
         if (f == 0) {
             const double G = (double)tq * inv_g;
             const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
@@ -1193,6 +1181,19 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
     }
     __shared__ long long rq[4][NBIN], rh[4][NBIN];
     __shared__ int rc[4][NBIN];
+    // wave 1 of a child scan derives the larger child from the parent's sums: load them now, so
+    // their latency overlaps the partial loads
+    long long par_q[4] = {0, 0, 0, 0}, par_h[4] = {0, 0, 0, 0};
+    int par_c[4] = {0, 0, 0, 0};
+    if (mode == 1 && wave == 1) {
+        const size_t po = (size_t)T.cur_leaf * per + (size_t)f * NBIN + lane * 4;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            par_q[t] = gbl(T.hg)[po + t];
+            par_c[t] = gbl(T.hc)[po + t];
+            if (!T.hconst) par_h[t] = gbl(T.hh)[po + t];
+        }
+    }
     {
         long long sq[4], sc[4], sh[4];
         sum_partials(T, F, f, wave, 4, T.hist_parts, sq, sc, sh);
@@ -1219,23 +1220,22 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
         const size_t so = (size_t)small * per + (size_t)f * NBIN + lane * 4;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            T.hg[so + t] = qg[t];
-            T.hc[so + t] = c[t];
-            if (!T.hconst) T.hh[so + t] = qh[t];
+            gbl(T.hg)[so + t] = qg[t];
+            gbl(T.hc)[so + t] = c[t];
+            if (!T.hconst) gbl(T.hh)[so + t] = qh[t];
         }
         scan_store(S, T, P, small, f, qg, c, qh);
     } else {
         const int big = T.big_node;
-        const size_t po = (size_t)T.cur_leaf * per + (size_t)f * NBIN + lane * 4;
         const size_t bo = (size_t)big * per + (size_t)f * NBIN + lane * 4;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            qg[t] = T.hg[po + t] - qg[t];
-            c[t] = T.hc[po + t] - c[t];
-            qh[t] = T.hconst ? 0 : T.hh[po + t] - qh[t];
-            T.hg[bo + t] = qg[t];
-            T.hc[bo + t] = c[t];
-            if (!T.hconst) T.hh[bo + t] = qh[t];
+            qg[t] = par_q[t] - qg[t];
+            c[t] = par_c[t] - c[t];
+            qh[t] = T.hconst ? 0 : par_h[t] - qh[t];
+            gbl(T.hg)[bo + t] = qg[t];
+            gbl(T.hc)[bo + t] = c[t];
+            if (!T.hconst) gbl(T.hh)[bo + t] = qh[t];
         }
         scan_store(S, T, P, big, f, qg, c, qh);
     }
@@ -1299,6 +1299,12 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
             sd.mgl = T.cur_mgl;
             sd.nl = T.cur_nl;   // known for decisions of k_select, -1 for the host's
             sd.gain = T.cur_gain;
+            if (!bad) {
+                const DNode nd = ldg(T.nodes + leaf);
+                sd.start = nd.seg_start;
+                sd.count = nd.seg_count;
+                sd.buf = nd.buf;
+            }
         }
         __syncthreads();
     }
@@ -1307,8 +1313,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
         if (blockIdx.x == 0 && threadIdx.x == 0) Tw.done = 1;
         return;
     }
-    const DNode& lf = T.nodes[d.leaf];
-    const int cnt = lf.seg_count, st = lf.seg_start, lbuf = lf.buf;
+    const int cnt = d.count, st = d.start, lbuf = d.buf;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Tw.cur_leaf = d.leaf;
         Tw.cur_type = d.type;
@@ -1327,8 +1332,8 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
     if (c0 >= cnt) return;
     __shared__ int wsum[4], wsumr[4];
     __shared__ int baseL, baseR;
-    const int* __restrict__ src = (lbuf ? T.idx_tmp : T.idx) + st;
-    int* __restrict__ dst = (lbuf ? T.idx : T.idx_tmp) + st;
+    const auto src = gbl((const int*)(lbuf ? T.idx_tmp : T.idx) + st);
+    const auto dst = gbl((lbuf ? T.idx : T.idx_tmp) + st);
     const int k0 = c0 + threadIdx.x * PART_RPT;
     unsigned flags = 0;
     int rows[PART_RPT];
