@@ -189,10 +189,9 @@ struct GrowP {
 // ---------------------------------------------------------------- leaf formulas
 // Reference tree.hpp:43-67 and histogram.cpp:119-142 (no FMA: -ffp-contract=off).
 __device__ inline double soft_thr(double G, double l1) {
-    if (l1 <= 0) return G;
-    if (G > l1) return G - l1;
-    if (G < -l1) return G + l1;
-    return 0.0;
+    // selects rather than branches, so candidate evaluations stay straight-line code
+    const double t = G > l1 ? G - l1 : (G < -l1 ? G + l1 : 0.0);
+    return l1 > 0 ? t : G;
 }
 __device__ inline double leaf_score(double G, double H, double lam, double l1) {
     const double t = soft_thr(G, l1);

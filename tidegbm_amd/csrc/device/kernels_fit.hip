@@ -803,11 +803,12 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
 // empty bins skipped, missing-right (ml=0) before missing-left, first maximum kept.  One wave per
 // feature, 4 bins per lane; prefix sums are exact int64, so the scan order cannot change them.
 // Lane j holds bins 4j..4j+3 (bins >= nb must be zero).
-__device__ inline void scan_feature_wave(const long long* qg, const int* c, const long long* qh,
-                                         bool hconst, double inv_g, double inv_h, int nb,
-                                         double lam, double gamma, int min_leaf, double min_hess,
-                                         double l1, double& out_gain, int& out_thr, int& out_ml,
-                                         int& out_cl, long long& totQ, long long& totQh, long long& totC) {
+template <bool hconst>
+__device__ inline void scan_feature_wave_t(const long long* qg, const int* c, const long long* qh,
+                                           double inv_g, double inv_h, int nb, double lam, double gamma,
+                                           int min_leaf, double min_hess, double l1, double& out_gain,
+                                           int& out_thr, int& out_ml, int& out_cl, long long& totQ,
+                                           long long& totQh, long long& totC) {
     const int lane = threadIdx.x & 63;
     // bin 0 (missing) sits in lane 0
     const long long missQ = (long long)read_lane_u64((unsigned long long)qg[0], 0);
@@ -840,28 +841,37 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
     double best = -INFINITY;
     int bkey = 0x7fffffff;
     int bcl = 0;
+    // The lane's 8 candidates are evaluated as straight-line code (an invalid one is computed
+    // and then ignored), so their divisions are independent and overlap; they are then taken in
+    // key order with the same strict comparison, so the result equals a sequential scan's.
+    double cg[8];
+    int cclv[8];
+    bool cok[8];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int b = lane * 4 + t;
-        if (b < 1 || b > nb - 2 || c[t] == 0) continue;
+        const bool bin_ok = (b >= 1) & (b <= nb - 2) & (c[t] != 0);
         const long long QL = eg + pg[t], QhL = eh + ph[t], CL = ec + pc[t];
+#pragma unroll
         for (int ml = 0; ml <= 1; ++ml) {
             const long long ql = QL + (ml ? missQ : 0), qhl = QhL + (ml ? missQh : 0);
             const long long cl = CL + (ml ? missC : 0);
             const long long cr = totC - cl;
-            if (cl < min_leaf || cr < min_leaf) continue;
             const double gl = (double)ql * inv_g, gr = (double)(totQ - ql) * inv_g;
             const double hl = hconst ? (double)cl : (double)qhl * inv_h;
             const double hr = hconst ? (double)cr : (double)(totQh - qhl) * inv_h;
-            if (hl < min_hess || hr < min_hess) continue;
-            const double gain = 0.5 * (leaf_score(gl, hl, lam, l1) + leaf_score(gr, hr, lam, l1) - parent) - gamma;
-            const int key = b * 2 + ml;
-            if (cand_better(gain, key, best, bkey)) {
-                best = gain;
-                bkey = key;
-                bcl = (int)cl;
-            }
+            cok[t * 2 + ml] = bin_ok & (cl >= min_leaf) & (cr >= min_leaf) & (hl >= min_hess) & (hr >= min_hess);
+            cg[t * 2 + ml] = 0.5 * (leaf_score(gl, hl, lam, l1) + leaf_score(gr, hr, lam, l1) - parent) - gamma;
+            cclv[t * 2 + ml] = (int)cl;
         }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int key = (lane * 4 + (j >> 1)) * 2 + (j & 1);
+        const bool take = cok[j] & ((cg[j] > best) | ((cg[j] == best) & (key < bkey)));
+        best = take ? cg[j] : best;
+        bkey = take ? key : bkey;
+        bcl = take ? cclv[j] : bcl;
     }
     // lanes hold increasing keys, so the wave's first maximum is the best lane's best
     const int wl = wave_best_lane(best, bkey);
@@ -872,6 +882,19 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
     out_thr = (bkey == 0x7fffffff) ? -1 : bkey >> 1;
     out_ml = (bkey == 0x7fffffff) ? 1 : (bkey & 1);
     out_cl = (bkey == 0x7fffffff) ? 0 : bcl;
+}
+
+__device__ inline void scan_feature_wave(const long long* qg, const int* c, const long long* qh, bool hconst,
+                                         double inv_g, double inv_h, int nb, double lam, double gamma,
+                                         int min_leaf, double min_hess, double l1, double& out_gain,
+                                         int& out_thr, int& out_ml, int& out_cl, long long& totQ,
+                                         long long& totQh, long long& totC) {
+    if (hconst)
+        scan_feature_wave_t<true>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
+                                  out_thr, out_ml, out_cl, totQ, totQh, totC);
+    else
+        scan_feature_wave_t<false>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
+                                   out_thr, out_ml, out_cl, totQ, totQh, totC);
 }
 
 // One split decision (the node to split and its predicate).
@@ -961,21 +984,26 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (wave == 0) {
+        // the waves' results side by side: lanes 0-15 for pending node 0, lanes 16-31 for node 1
+        const int pl = lane >> 4, wv = lane & 15;
+        const bool have = pl < 2 && wv < nw;
+        const double gv = have ? sg[pl][wv] : -INFINITY;
+        const int kv = have ? sk[pl][wv] : 0x7fffffff;
+        const int tv = have ? st[pl][wv] : -1, mv = have ? sm[pl][wv] : 1, cv = have ? scl[pl][wv] : 0;
 #pragma unroll
         for (int pi = 0; pi < 2; ++pi) {
+            const bool mine = pl == pi;
+            const double gm = mine ? gv : -INFINITY;
+            const int km = mine ? kv : 0x7fffffff;
+            const int wl = wave_best_lane(gm, km);
+            const double b = __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(gm), wl));
+            const int k = __builtin_amdgcn_readlane(km, wl);
+            const int t = __builtin_amdgcn_readlane(mine ? tv : -1, wl);
+            const int m = __builtin_amdgcn_readlane(mine ? mv : 1, wl);
+            const int cl = __builtin_amdgcn_readlane(mine ? cv : 0, wl);
             const int nd = pend[pi];
-            if (nd < 0) continue;
-            double b = sg[pi][0];
-            int k = sk[pi][0], t = st[pi][0], m = sm[pi][0], cl = scl[pi][0];
-            for (int w = 1; w < nw; ++w)
-                if (cand_better(sg[pi][w], sk[pi][w], b, k)) {
-                    b = sg[pi][w];
-                    k = sk[pi][w];
-                    t = st[pi][w];
-                    m = sm[pi][w];
-                    cl = scl[pi][w];
-                }
+            if (lane != 0 || nd < 0) continue;
             DCand c;
             const bool elig = ndepth[pi] < P.max_depth && nns[pi] >= 2 * P.min_leaf;
             c.valid = (k != 0x7fffffff) && elig;
