@@ -803,7 +803,7 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
 // empty bins skipped, missing-right (ml=0) before missing-left, first maximum kept.  One wave per
 // feature, 4 bins per lane; prefix sums are exact int64, so the scan order cannot change them.
 // Lane j holds bins 4j..4j+3 (bins >= nb must be zero).
-template <bool hconst>
+template <bool hconst, int T0, int T1>
 __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, const long long* qh,
                                            double inv_g, double inv_h, int nb, double lam, double gamma,
                                            int min_leaf, double min_hess, double l1, double& out_gain,
@@ -844,11 +844,12 @@ __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, co
     // The lane's 8 candidates are evaluated as straight-line code (an invalid one is computed
     // and then ignored), so their divisions are independent and overlap; they are then taken in
     // key order with the same strict comparison, so the result equals a sequential scan's.
+    // only bins lane*4+t with T0 <= t < T1 are candidates of this call
     double cg[8];
     int cclv[8];
     bool cok[8];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = T0; t < T1; ++t) {
         const int b = lane * 4 + t;
         const bool bin_ok = (b >= 1) & (b <= nb - 2) & (c[t] != 0);
         const long long QL = eg + pg[t], QhL = eh + ph[t], CL = ec + pc[t];
@@ -866,7 +867,7 @@ __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, co
         }
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = T0 * 2; j < T1 * 2; ++j) {
         const int key = (lane * 4 + (j >> 1)) * 2 + (j & 1);
         const bool take = cok[j] & ((cg[j] > best) | ((cg[j] == best) & (key < bkey)));
         best = take ? cg[j] : best;
@@ -890,11 +891,34 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
                                          int& out_thr, int& out_ml, int& out_cl, long long& totQ,
                                          long long& totQh, long long& totC) {
     if (hconst)
-        scan_feature_wave_t<true>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
-                                  out_thr, out_ml, out_cl, totQ, totQh, totC);
+        scan_feature_wave_t<true, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
+                                        out_thr, out_ml, out_cl, totQ, totQh, totC);
     else
-        scan_feature_wave_t<false>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
-                                   out_thr, out_ml, out_cl, totQ, totQh, totC);
+        scan_feature_wave_t<false, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
+                                         out_thr, out_ml, out_cl, totQ, totQh, totC);
+}
+// Half of the candidates (bins lane*4 + {0,1} for half 0, lane*4 + {2,3} for half 1): two waves
+// holding the same sums split one feature's scan.
+__device__ inline void scan_feature_half(const long long* qg, const int* c, const long long* qh, bool hconst,
+                                         int half, double inv_g, double inv_h, int nb, double lam, double gamma,
+                                         int min_leaf, double min_hess, double l1, double& out_gain, int& out_thr,
+                                         int& out_ml, int& out_cl, long long& totQ, long long& totQh,
+                                         long long& totC) {
+    if (hconst) {
+        if (half == 0)
+            scan_feature_wave_t<true, 0, 2>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
+                                            out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+        else
+            scan_feature_wave_t<true, 2, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
+                                            out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+    } else {
+        if (half == 0)
+            scan_feature_wave_t<false, 0, 2>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
+                                             out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+        else
+            scan_feature_wave_t<false, 2, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
+                                             out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+    }
 }
 
 // One split decision (the node to split and its predicate).
@@ -1154,46 +1178,12 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
     }
 }
 
-// Scan one feature of one node from its sums in registers; lane 0 stores the feature's best
-// split (and, for feature 0, the node value from the node's own sums, tree.cpp:323-328, 407-416).
-__device__ inline void scan_store(const Shared& S, const Task& T, const GrowP& P, int node, int f,
-                                  long long qg[4], int c[4], long long qh[4]) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int t = 0; t < 4; ++t)
-        if (lane * 4 + t >= S.n_bins) {
-            qg[t] = 0;
-            c[t] = 0;
-            qh[t] = 0;
-        }
-    double gain;
-    int thr, ml, cl;
-    long long tq, tqh, tc;
-    const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
-    scan_feature_wave(qg, c, qh, T.hconst, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf, P.min_hess, P.l1,
-                      gain, thr, ml, cl, tq, tqh, tc);
-    if (lane == 0) {
-        FCand o;
-        o.gain = gain;
-        o.thr = thr;
-        o.ml = ml;
-        o.cl = cl;
-        o.pad = 0;
-        stg(T.fcand + (size_t)node * S.n_features + f, o);
-        if (f == 0) {
-            const double G = (double)tq * inv_g;
-            const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
-            T.nodes[node].value = leaf_value(G, H, P.l2, P.l1, P.mds);
-        }
-    }
-}
-
 // mode 0: the root (node 0) from the root launch's partials.  mode 1: both children of the
 // current split: the smaller from the child launch's partials, the larger as parent - smaller
 // (exact in fixed point; reference subtract_hist_inplace, histogram.cpp:108-117).  A block owns
-// one feature: its four waves sum interleaved quarters of the partials, then wave 0 stores and
-// scans the smaller child while wave 1 stores and scans the larger.  Both children's histograms
-// are kept for later use as parents.
+// one feature: its four waves sum interleaved quarters of the partials, then waves 0 and 2 scan
+// the smaller child and waves 1 and 3 the larger (half of the candidates each); wave 0 / 1 stores
+// the child's histogram, kept for later use as a parent, and its best split of the feature.
 __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode) {
     const Task T = tasks[blockIdx.z];
     Task& Tw = tasks[blockIdx.z];
@@ -1209,11 +1199,11 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
     }
     __shared__ long long rq[4][NBIN], rh[4][NBIN];
     __shared__ int rc[4][NBIN];
-    // wave 1 of a child scan derives the larger child from the parent's sums: load them now, so
-    // their latency overlaps the partial loads
+    // waves 1 and 3 of a child scan derive the larger child from the parent's sums: load them
+    // now, so their latency overlaps the partial loads
     long long par_q[4] = {0, 0, 0, 0}, par_h[4] = {0, 0, 0, 0};
     int par_c[4] = {0, 0, 0, 0};
-    if (mode == 1 && wave == 1) {
+    if (mode == 1 && (wave & 1)) {
         const size_t po = (size_t)T.cur_leaf * per + (size_t)f * NBIN + lane * 4;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
@@ -1233,7 +1223,10 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
         }
     }
     __syncthreads();
-    if (wave > 1 || (mode == 0 && wave == 1)) return;
+    // waves 0 and 2 scan the smaller child (the root in mode 0), waves 1 and 3 the larger; the
+    // two waves of a node hold the same sums and each evaluates half of its candidates
+    const int side = wave & 1, half = wave >> 1;
+    const bool active = mode == 1 || side == 0;
     long long qg[4], qh[4];
     int c[4];
 #pragma unroll
@@ -1243,29 +1236,75 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
         c[t] = rc[0][b] + rc[1][b] + rc[2][b] + rc[3][b];
         qh[t] = rh[0][b] + rh[1][b] + rh[2][b] + rh[3][b];
     }
-    if (wave == 0) {
-        const int small = mode == 0 ? 0 : T.small_node;
-        const size_t so = (size_t)small * per + (size_t)f * NBIN + lane * 4;
+    int node = -1;
+    double gain = -INFINITY;
+    int thr = -1, ml = 1, cl = 0;
+    long long tq = 0, tqh = 0, tc = 0;
+    const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
+    if (active) {
+        if (side == 0) {
+            node = mode == 0 ? 0 : T.small_node;
+        } else {
+            node = T.big_node;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            gbl(T.hg)[so + t] = qg[t];
-            gbl(T.hc)[so + t] = c[t];
-            if (!T.hconst) gbl(T.hh)[so + t] = qh[t];
+            for (int t = 0; t < 4; ++t) {
+                qg[t] = par_q[t] - qg[t];
+                c[t] = par_c[t] - c[t];
+                qh[t] = T.hconst ? 0 : par_h[t] - qh[t];
+            }
         }
-        scan_store(S, T, P, small, f, qg, c, qh);
-    } else {
-        const int big = T.big_node;
-        const size_t bo = (size_t)big * per + (size_t)f * NBIN + lane * 4;
+        if (half == 0) {
+            // both children's histograms are kept for later use as parents
+            const size_t o = (size_t)node * per + (size_t)f * NBIN + lane * 4;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            qg[t] = par_q[t] - qg[t];
-            c[t] = par_c[t] - c[t];
-            qh[t] = T.hconst ? 0 : par_h[t] - qh[t];
-            gbl(T.hg)[bo + t] = qg[t];
-            gbl(T.hc)[bo + t] = c[t];
-            if (!T.hconst) gbl(T.hh)[bo + t] = qh[t];
+            for (int t = 0; t < 4; ++t) {
+                gbl(T.hg)[o + t] = qg[t];
+                gbl(T.hc)[o + t] = c[t];
+                if (!T.hconst) gbl(T.hh)[o + t] = qh[t];
+            }
         }
-        scan_store(S, T, P, big, f, qg, c, qh);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (lane * 4 + t >= S.n_bins) {
+                qg[t] = 0;
+                c[t] = 0;
+                qh[t] = 0;
+            }
+        scan_feature_half(qg, c, qh, T.hconst, half, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf,
+                          P.min_hess, P.l1, gain, thr, ml, cl, tq, tqh, tc);
+    }
+    __shared__ double xg[4];
+    __shared__ int xt[4], xm[4], xc[4];
+    if (lane == 0) {
+        xg[wave] = gain;
+        xt[wave] = thr;
+        xm[wave] = ml;
+        xc[wave] = cl;
+    }
+    __syncthreads();
+    if (!active || half != 0 || lane != 0) return;
+    // the other half's best (keys interleave across the halves: key = bin * 2 + missing-left)
+    const int o = wave + 2;
+    const int key = thr < 0 ? 0x7fffffff : thr * 2 + ml;
+    const int okey = xt[o] < 0 ? 0x7fffffff : xt[o] * 2 + xm[o];
+    if (cand_better(xg[o], okey, gain, key)) {
+        gain = xg[o];
+        thr = xt[o];
+        ml = xm[o];
+        cl = xc[o];
+    }
+    FCand fc;
+    fc.gain = gain;
+    fc.thr = thr;
+    fc.ml = ml;
+    fc.cl = cl;
+    fc.pad = 0;
+    stg(T.fcand + (size_t)node * S.n_features + f, fc);
+    if (f == 0) {
+        // node value from the node's own sums (tree.cpp:323-328, 407-416)
+        const double G = (double)tq * inv_g;
+        const double H = T.hconst ? (double)tc : (double)tqh * inv_h;
+        T.nodes[node].value = leaf_value(G, H, P.l2, P.l1, P.mds);
     }
 }
 
