@@ -154,6 +154,7 @@ def run_plain(args, p):
         "fit_setup_s": round(setup_s, 3),
         "fit_wall_s": round(wall, 3),
         "trees_per_s_end_to_end": round((W + K) / (setup_s + boost_s), 3),
+        "hist_variant": b.hist_variant(),
     }
     if not args.no_cpu:
         cb = cpu_baseline(p, params, args.cpu_trees)
@@ -211,6 +212,7 @@ def run_walkforward(args, p):
         "speedup": round(bs / bb, 3),
         "batched_trees_per_s": round(n_trees / bb, 1), "sequential_trees_per_s": round(n_trees / bs, 1),
         "setup_s": round(sb, 3), "identical_trees": bool(same),
+        "hist_variant_batched": mb[0].hist_variant(),
     }
 
 

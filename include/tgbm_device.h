@@ -35,7 +35,11 @@ typedef struct tgbm_grow_params {
     double min_child_weight;
     double l2, l1, max_delta_step, min_split_gain;
     double pair_l2;         /* Config::pair_l2, used by PAIR scans */
-    int hist_mode;          /* 0 = auto, else forces a histogram kernel variant (autotuner) */
+    int hist_mode;          /* histogram kernel variant, 0 = defaults (all variants give the same
+                               sums): bits 0-7 child tile (>= 16: 16 << (v-16)/8 features and
+                               256 << (v-16)%8 minimum rows per block; 2: the row kernel also for
+                               the root), bits 12-13 child partial cap (32, 64, 16, 8), bits
+                               20-29 root kernel variant; chosen per fit by the host tuner */
     int host_driven;        /* 1: the host decides splits (tgbm_apply_split); 0: the device does */
 } tgbm_grow_params;
 

@@ -80,6 +80,10 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s);
  * std::sort on the host, as the reference orders it, booster.cpp:561-562), seconds the loop
  * waited for the generator draws, and seconds spent in the sample step (device or host). */
 int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sample_s);
+/* Histogram kernel variant the last fit ended with: the tuner's choice (tide_set_int "autotune",
+ * on by default for fits of 54 trees or more; the variants give identical trees), or the forced
+ * "hist_mode". */
+int tide_hist_variant(TideHandle h);
 /* Wall time (s since the boosting loop began) at the end of each tree of the last fit; each
  * tree ends with a device synchronization.  Returns the number of trees. */
 int tide_tree_times(TideHandle h, double* out, int cap);

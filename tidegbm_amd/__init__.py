@@ -65,6 +65,7 @@ _sig("tide_get_init_prefix", _I, _P, _P, _I)
 _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
 _sig("tide_goss_stats", _I, _P, _P, _P, _P)
+_sig("tide_hist_variant", _I, _P)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
@@ -85,7 +86,7 @@ OBJECTIVES = {"mse": 0, "auto": 0, "l2": 0, "regression": 0, "pairwise": 5, "lam
 _INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_buckets", "random_state",
              "n_jobs", "iid_mode", "halo", "verbose", "horizon", "embargo", "goss", "cat_min_count",
              "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
-             "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode",
+             "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune",
              "force_host_driver"}
 _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight",
                 "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",
@@ -246,6 +247,10 @@ class Booster:
         a, b = _D(), _D()
         _lib.tide_fit_times(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def hist_variant(self) -> int:
+        """Histogram kernel variant the last fit ended with (tuned or forced; same trees)."""
+        return int(_lib.tide_hist_variant(self._h))
 
     def goss_stats(self):
         """(trees sampled on the host because of equal |g|, seconds waiting for the generator

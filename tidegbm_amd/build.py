@@ -23,7 +23,7 @@ ROOT = Path(__file__).resolve().parent.parent
 PKG = ROOT / "tidegbm_amd"
 LIB = PKG / "lib"
 DEV_SRC = ["kernels_fit.hip", "session.hip", "forest.hip", "serve.hip", "goss.hip"]
-HOST_SRC = ["booster.cpp", "serialize.cpp", "c_api.cpp", "batch.cpp", "server.cpp"]
+HOST_SRC = ["booster.cpp", "serialize.cpp", "c_api.cpp", "batch.cpp", "server.cpp", "tuner.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: reference formulas must round like its x86-64 build (no FMA).  The
 # packed-fp32 feature is disabled for every HIP object (hipcc's host pass warns that it does

@@ -689,14 +689,15 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                  hipEvent_t e1) {
     if (max_count <= 0) return;
     if (root && contiguous_root && (variant & 0xff) != 2) {
-        // Root variants (variant >> 8).  0: k_hist_root with XCD-major placement and the
-        // features per block sized so the grid about fills the GPU once.  Bits 4-6 == 6:
+        // Root variants (variant >> 20; the child variants use bits 0-15).  0: k_hist_root with
+        // XCD-major placement and the features per block sized so the grid about fills the GPU
+        // once.  Bits 4-6 == 6:
         // k_hist_root with fpb = 16 << bits 7-8 and bit 3 XCD-major placement.  Otherwise the
         // single-pass k_hist_col: bit 0 selects 16 features per block (else 8), bits 1-2 a 512-
         // or 1024-thread block (else 256; 1024 only with copies), bit 3 XCD-major placement, bits
         // 4-6 log2 of the copies per bin K (0, or 2..5: K = 4..32 with 64 / K features per block,
         // 32 / K when h varies; 128 KB of LDS).
-        const int rv = variant >> 8;
+        const int rv = (variant >> 20) & 0x3ff;
         const int lk = (rv >> 4) & 7;
         if (rv == 0 || lk == 6) {
             // k_hist_root: 16384 rows x fpb features per block

@@ -14,6 +14,7 @@
 //   sequential : one fold active at a time, all of its trees, then the next fold (the plain
 //                one-fit-per-call pattern, over the same resident bins)
 #include "booster.h"
+#include "tuner.h"
 
 #include <algorithm>
 #include <cmath>
@@ -97,7 +98,7 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
     st.setup_s = now_s() - t0;
     // ---- boosting
     const double t1 = now_s();
-    const tgbm_grow_params gp = grow_params(proto);
+    const tgbm_grow_params gp0 = grow_params(proto);
     const double min_cover =
         proto.torb_min_cover > 0.0 ? proto.torb_min_cover : (double)(2 * std::max(proto.min_samples_leaf, 1));
     const int MN = 2 * proto.max_leaves - 1;
@@ -106,14 +107,27 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
         dev_check(tgbm_session_set_active(so.s, first, count));
         std::vector<int> nn(count), ints((size_t)count * MN * TGBM_NODE_INTS), cov((size_t)count * MN);
         std::vector<double> dbl((size_t)count * MN * 2), vals((size_t)count * MN), pref((size_t)count * MN * 256);
+        // histogram kernel variant: forced, or tuned over the first trees (identical trees either
+        // way); shape: the active tasks and their largest launch size
+        int max_rows = 0;
+        for (int t = first; t < first + count; ++t) max_rows = std::max(max_rows, task_rows[t]);
+        const std::string shape = "batch:" + std::to_string(count) + ":" + std::to_string(launch_rows_of(max_rows)) +
+                                  ":" + std::to_string(F) + ":" + std::to_string(proto.objective) + ":" +
+                                  std::to_string(proto.max_leaves);
+        HistTuner tuner(shape, proto.n_estimators, proto.autotune != 0 && proto.hist_mode == 0);
+        tgbm_grow_params gp = gp0;
         for (int m = 0; m < proto.n_estimators; ++m) {
+            if (proto.hist_mode == 0) gp.hist_mode = tuner.mode(m);
             dev_check(tgbm_task_grad(so.s, -1, proto.objective));
+            const double t_tree = now_s();
             dev_check(tgbm_tree_begin(so.s, &gp));
             dev_check(tgbm_tree_grow_device(so.s, &gp));
             dev_check(tgbm_tree_end(so.s, proto.learning_rate, min_cover, iid, &gp, nn.data(), ints.data(), dbl.data(),
                                     vals.data(), cov.data(), pref.data()));
+            tuner.record(m, now_s() - t_tree);   // tree_end synchronized
             for (int i = 0; i < count; ++i) {
                 Booster& b = *out[first + i];
+                b.hist_variant = gp.hist_mode;
                 const size_t o = (size_t)i * MN;
                 b.trees.push_back(unpack_tree(nn[i], &ints[o * TGBM_NODE_INTS], &dbl[o * 2], &vals[o], &cov[o],
                                               &pref[o * 256], b.n_time_buckets));

@@ -3,6 +3,7 @@
 // that the rows, bins and samples handed to the device are identical; every per-row and
 // per-histogram computation runs on the GPU.
 #include "booster.h"
+#include "tuner.h"
 
 #include <algorithm>
 #include <chrono>
@@ -99,6 +100,13 @@ double leaf_value_host(double G, double H, const Config& c) {  // tree.hpp:53-67
 }
 
 }  // namespace
+
+// The power-of-two row count the device launches are sized for (workload shape key of the tuner).
+long long launch_rows_of(long long n) {
+    long long r = 1;
+    while (r < n) r <<= 1;
+    return r;
+}
 
 tgbm_grow_params grow_params(const Config& c) {
     tgbm_grow_params p;
@@ -668,9 +676,14 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
 void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, const std::vector<int>& cuts,
                          std::mt19937& rng, bool goss, const double*, bool for_update) {
     (void)cuts;
-    const tgbm_grow_params gp = grow_params(cfg);
+    tgbm_grow_params gp = grow_params(cfg);
     const int F = n_features;
     const bool host_driven = cfg.pair_splits || cfg.force_host_driver;
+    // the histogram kernel variant: forced, or chosen over the first trees (all give the same trees)
+    const std::string shape = "fit:" + std::to_string(launch_rows_of(n)) + ":" + std::to_string(F) + ":" +
+                              std::to_string(cfg.objective) + ":" + std::to_string(goss) + ":" +
+                              std::to_string(cfg.max_leaves) + ":" + std::to_string(host_driven);
+    HistTuner tuner(shape, n_trees, cfg.autotune != 0 && cfg.hist_mode == 0 && !for_update);
     const double min_cover =
         cfg.torb_min_cover > 0.0 ? cfg.torb_min_cover : (double)(2 * std::max(cfg.min_samples_leaf, 1));
     const int MN = 2 * cfg.max_leaves - 1;
@@ -710,12 +723,15 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         } else {
             (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample would use it
         }
+        if (cfg.hist_mode == 0) gp.hist_mode = tuner.mode(m);
+        const double t_tree = now_s();
         dev_check(tgbm_tree_begin(s, &gp));
         if (host_driven) grow_host_driven(s, cfg, F, gp);
         else dev_check(tgbm_tree_grow_device(s, &gp));
         int nn = 0;
         dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
                                 ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
+        tuner.record(m, now_s() - t_tree);   // tree_end synchronized
         Tree t = unpack_tree(nn, ints.data(), dbl.data(), vals.data(), cov.data(), pref.data(), n_time_buckets);
         if (for_update) {
             // Booster::update keeps grow_tree's cover (node sample counts) and one bucket
@@ -726,6 +742,7 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         trees.push_back(std::move(t));
         if (!for_update) tree_end_times.push_back(now_s() - t_loop);  // tree_end synchronized
     }
+    hist_variant = gp.hist_mode;
 }
 
 void Booster::update(const double* X, const double* y, int n, int nf, const double* w, int extra_trees) {

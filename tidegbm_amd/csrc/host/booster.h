@@ -61,7 +61,8 @@ struct Config {
     bool col_major_hist = true;
     std::string device = "gpu";
     int profile = 0;
-    int hist_mode = 0;
+    int hist_mode = 0;       // histogram kernel variant; 0: defaults, tuned per fit when autotune
+    int autotune = 1;        // choose the variant online over the first trees (tuner.h)
     int force_host_driver = 0;
 
     void validate() const;  // config.hpp:78-97 plus the device path's limits
@@ -96,6 +97,7 @@ void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vect
 Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov,
                  const double* pref, int n_buckets);
 tgbm_grow_params grow_params(const Config& c);
+long long launch_rows_of(long long n);
 class Booster;
 // Device-resident forest of a fitted model's first best_iteration trees (lr per tree = cfg lr).
 tgbm_forest* make_forest(const Booster& b);
@@ -118,6 +120,7 @@ public:
     double t_setup = 0.0, t_boost = 0.0;
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
     int goss_host_trees = 0;             // GOSS trees of the last fit sampled on the host (equal |g|)
+    int hist_variant = 0;                // histogram kernel variant the last fit ended with
     double goss_wait_s = 0.0, goss_sample_s = 0.0;  // GOSS: waiting for draws, the sample step
     double prof_ms[8] = {0}, prof_bytes[8] = {0};
     long long prof_n[8] = {0};

@@ -65,6 +65,7 @@ int tide_set_int(TideHandle h, const char* key, int v) {
     else if (k == "objective") c.objective = v;
     else if (k == "profile") c.profile = v;
     else if (k == "hist_mode") c.hist_mode = v;
+    else if (k == "autotune") c.autotune = v;
     else if (k == "force_host_driver") c.force_host_driver = v;
     else {
         g_err = "unknown int key";
@@ -288,6 +289,8 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
     *boost_s = b->t_boost;
     return 0;
 }
+
+int tide_hist_variant(TideHandle h) { return static_cast<Booster*>(h)->hist_variant; }
 
 int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sample_s) {
     auto* b = static_cast<Booster*>(h);
