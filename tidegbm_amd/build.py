@@ -29,7 +29,10 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed-fp32 feature is disabled for every HIP object (hipcc's host pass warns that it does
 # not know the feature; harmless).
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # the first kernel arguments arrive in SGPRs at wave launch (no dependent load of the
+             # task-descriptor pointer before each kernel's first read)
+             "-mllvm", "-amdgpu-kernarg-preload-count=16"]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fopenmp", "-Wall", "-Wno-unused-function"]
 
 

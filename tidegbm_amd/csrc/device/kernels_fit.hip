@@ -345,7 +345,7 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
 // block pass costs two memory round trips.  The block's row range is sized from the node's
 // actual row count, so small nodes still spread over many blocks.
 template <int LPR, bool HV>
-__global__ __launch_bounds__(512) void k_hist_row(Shared S, Task* tasks, int root, int min_rows) {
+__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows) {
     constexpr int FB = 16 * LPR;
     constexpr int RPL = 8;
     extern __shared__ unsigned long long lds[];
@@ -545,7 +545,7 @@ __device__ inline void root_load_bins(uint4 (&b)[FB], const unsigned char* colb,
 }
 
 template <int K, bool HV>
-__global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Shared S, Task* tasks, int feats_per_block,
+__global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared S, int feats_per_block,
                                                              int xcd_major) {
     static_assert(K == 16, "copies per bin: the sums' bank spread and the register budget assume 16");
     constexpr int FB = (HV ? 32 : 64) / K;        // 128 KB of LDS
@@ -711,8 +711,8 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                 xcd = 1;
             }
             dim3 grid((max_count + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
-            if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, S, tasks, fpb, xcd);
-            else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, S, tasks, fpb, xcd);
+            if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
+            else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
             return;
         }
         if (lk >= 2 && lk <= 5) {
@@ -789,12 +789,12 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
     const int rt = root ? 1 : 0;
     if (hconst) {
-        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
-        else if (lpr == 2) TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
-        else TIMED_LAUNCH((k_hist_row<4, false>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
+        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+        else if (lpr == 2) TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+        else TIMED_LAUNCH((k_hist_row<4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
     } else {
-        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
-        else TIMED_LAUNCH((k_hist_row<2, true>), grid, dim3(512), lds, st, e0, e1, S, tasks, rt, min_rows);
+        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+        else TIMED_LAUNCH((k_hist_row<2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
     }
 }
 
@@ -1185,7 +1185,7 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
 // one feature: its four waves sum interleaved quarters of the partials, then waves 0 and 2 scan
 // the smaller child and waves 1 and 3 the larger (half of the candidates each); wave 0 / 1 stores
 // the child's histogram, kept for later use as a parent, and its best split of the feature.
-__global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, int mode) {
+__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode) {
     const Task T = tasks[blockIdx.z];
     Task& Tw = tasks[blockIdx.z];
     if (T.done) return;
@@ -1311,7 +1311,7 @@ __global__ __launch_bounds__(256) void k_scan(Shared S, Task* tasks, GrowP P, in
 
 void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
     dim3 grid(S.n_features, 1, n_tasks);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, S, tasks, P, mode);
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode);
 }
 
 // =================================================================== partition
@@ -1345,7 +1345,7 @@ __device__ inline bool row_goes_left(const Shared& S, const Task& T, const Decis
 // from the front, rights from the back.  The order of rows inside a child's range therefore
 // depends on block timing, but nothing downstream depends on it: every histogram, leaf and TORB
 // sum is an exact integer sum over the set of rows.
-__global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP P, int mode) {
+__global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP P, int mode) {
     const Task T = tasks[blockIdx.y];
     Task& Tw = tasks[blockIdx.y];
     if (T.done) return;
@@ -1454,7 +1454,7 @@ __global__ __launch_bounds__(256) void k_partition(Shared S, Task* tasks, GrowP 
 
 // Standalone selection (one block per task) for launches that grow many tasks at once: the
 // partition then reads the decision instead of every partition block recomputing it.
-__global__ __launch_bounds__(256) void k_select(Shared S, Task* tasks, GrowP P) {
+__global__ __launch_bounds__(256) void k_select(Task* tasks, Shared S, GrowP P) {
     const Task T = tasks[blockIdx.x];
     Task& Tw = tasks[blockIdx.x];
     if (T.done) return;
@@ -1476,14 +1476,14 @@ __global__ __launch_bounds__(256) void k_select(Shared S, Task* tasks, GrowP P) 
 }
 
 void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P) {
-    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, S, tasks, P);
+    hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, tasks, S, P);
 }
 
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
                       int mode) {
     const int nch = std::max(1, (max_count + PART_CHUNK - 1) / PART_CHUNK);
     dim3 grid(nch, n_tasks);
-    hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, S, tasks, P, mode);
+    hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, tasks, S, P, mode);
 }
 
 // =================================================================== host-driven helpers
