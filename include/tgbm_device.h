@@ -134,6 +134,11 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
                   double* values, int* cover, double* prefix);
 
 int tgbm_task_get_pred(tgbm_session* s, int task, double* pred);
+/* Training loss of `task` after its last finished tree (reference Objective::loss,
+ * objective.cpp:262-329, recorded at booster.cpp:704-706): weighted mean squared error, or for
+ * the pairwise objective minus the mean within-group Pearson correlation.  Computed on the device
+ * from the updated predictions and exported with the tree. */
+int tgbm_task_train_loss(tgbm_session* s, int task, double* loss);
 
 /* Timers (HIP events on the session stream) summed per kernel family since the last reset:
  * which = 0 root histogram, 1 partition (with the split selection), 2 split scan, 3 tree end,

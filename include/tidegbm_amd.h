@@ -84,6 +84,9 @@ int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sampl
  * on by default for fits of 54 trees or more; the variants give identical trees), or the forced
  * "hist_mode". */
 int tide_hist_variant(TideHandle h);
+/* Training loss after each tree of the last fit, the reference's evals_result()["train"]
+ * (booster.cpp:704-706, bindings.cpp:318-321).  Returns the number of trees (at most cap copied). */
+int tide_evals_train(TideHandle h, double* out, int cap);
 /* Wall time (s since the boosting loop began) at the end of each tree of the last fit; each
  * tree ends with a device synchronization.  Returns the number of trees. */
 int tide_tree_times(TideHandle h, double* out, int cap);

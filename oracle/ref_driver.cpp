@@ -118,6 +118,11 @@ int tref_n_bins(void* h) { return static_cast<Booster*>(h)->n_bins; }
 int tref_n_time_buckets(void* h) { return static_cast<Booster*>(h)->n_time_buckets; }
 double tref_init_score(void* h) { return static_cast<Booster*>(h)->init_score; }
 int tref_best_iteration(void* h) { return static_cast<Booster*>(h)->best_iteration; }
+int tref_evals_train(void* h, double* out, int cap) {
+    const auto& e = static_cast<Booster*>(h)->evals.train;
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
 
 int tref_init_prefix(void* h, double* out, int cap) {
     const auto& v = static_cast<Booster*>(h)->init_prefix;

@@ -480,6 +480,59 @@ static void grad_pairwise(const double* y, const double* p, const double* w, con
     }
 }
 
+// Training loss reported after every tree (objective.cpp:262-329, recorded at booster.cpp:704-706):
+// the pairwise objective with groups reports minus the mean within-group Pearson correlation
+// over groups of at least 3 rows whose variances exceed 1e-18; every other objective here the
+// weighted mean squared error.  Sequential sums in row / group order.
+static double loss_of(int objective, const double* y, const double* p, const double* w, const i64* grp,
+                      int n) {
+    if (objective == 5 && grp) {
+        std::vector<int> ord(n);
+        std::iota(ord.begin(), ord.end(), 0);
+        bool sorted = true;
+        for (int i = 1; i < n; ++i)
+            if (grp[i] < grp[i - 1]) sorted = false;
+        if (!sorted) std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return grp[a] < grp[b]; });
+        double acc = 0;
+        int ng = 0;
+        int s0 = 0;
+        while (s0 < n) {
+            int s1 = s0 + 1;
+            while (s1 < n && grp[ord[s1]] == grp[ord[s0]]) ++s1;
+            const int m = s1 - s0;
+            const int* ix = ord.data() + s0;
+            s0 = s1;
+            if (m < 3) continue;
+            double my = 0, mp = 0;
+            for (int k = 0; k < m; ++k) {
+                my += y[ix[k]];
+                mp += p[ix[k]];
+            }
+            my /= m;
+            mp /= m;
+            double vy = 0, vp = 0, c = 0;
+            for (int k = 0; k < m; ++k) {
+                const double dy = y[ix[k]] - my, dp = p[ix[k]] - mp;
+                vy += dy * dy;
+                vp += dp * dp;
+                c += dy * dp;
+            }
+            if (vy > 1e-18 && vp > 1e-18) {
+                acc += c / std::sqrt(vy * vp);
+                ng += 1;
+            }
+        }
+        return ng > 0 ? -acc / ng : 0.0;
+    }
+    double sum = 0, sw = 0;
+    for (int i = 0; i < n; ++i) {
+        const double d = p[i] - y[i];
+        sum += w[i] * d * d;
+        sw += w[i];
+    }
+    return sw > 0 ? sum / sw : 0;
+}
+
 static double wmean(const double* y, const double* w, int n) {  // objective.cpp:41-49
     double sw = 0, sy = 0;
     for (int i = 0; i < n; ++i) {
@@ -497,6 +550,7 @@ struct Model {
     double init_score = 0.0;
     std::vector<double> init_prefix;
     std::vector<Tree> trees;
+    std::vector<double> evals_train;   // training loss after each tree of the last fit
     bool fitted = false;
     // Caller-supplied bin edges replace Binner::fit (checker mode for fits that share a binned
     // matrix, e.g. walk-forward folds binned once); empty = fit edges as the reference does.
@@ -622,6 +676,7 @@ struct Model {
             for (int i = 0; i < n; ++i) pred[i] = init_prefix[bucket[i]];
         }
         trees.clear();
+        evals_train.clear();
         std::mt19937 rng((uint32_t)c.random_state);
         std::vector<double> g(n), h(n);
         const i64* grp = rg0 ? rg.data() : nullptr;
@@ -704,6 +759,7 @@ struct Model {
             const Tree& T = trees.back();
             const double lr = c.learning_rate;
             for (int i = 0; i < n; ++i) pred[i] += lr * T.prefix[(size_t)bucket[i] * nn + nid[i]];
+            evals_train.push_back(loss_of(c.objective, y.data(), pred.data(), ws.data(), grp, n));
         }
         best_iteration = c.n_estimators;
         fitted = true;
@@ -874,6 +930,11 @@ int orc_edges(void* hd, int f, double* out, int cap) {
     return (int)e.size();
 }
 int orc_n_trees(void* hd) { return (int)static_cast<Model*>(hd)->trees.size(); }
+int orc_evals_train(void* hd, double* out, int cap) {
+    const auto& e = static_cast<Model*>(hd)->evals_train;
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
 int orc_tree_info(void* hd, int t, int* nn, int* nbk) {
     const orc::Tree& tr = static_cast<Model*>(hd)->trees.at(t);
     *nn = (int)tr.nodes.size();

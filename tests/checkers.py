@@ -108,6 +108,13 @@ class CpuModel:
     def init_score(self):
         return self.lib.fn("init_score")(self.h)
 
+    def evals_train(self):
+        """Training loss after each tree of the last fit (reference evals.train)."""
+        n = self.lib.fn("evals_train")(self.h, None, 0)
+        out = np.empty(n)
+        self.lib.fn("evals_train")(self.h, _ptr(out), n)
+        return out
+
 
 class CpuLib:
     def __init__(self, path: Path, prefix: str):
@@ -122,6 +129,7 @@ class CpuLib:
             "tree_info": (_I, [_P, _I, _P, _P]), "tree_get": (_I, [_P, _I, _P, _P, _P, _P, _P]),
             "edges": (_I, [_P, _I, _P, _I]), "n_bins": (_I, [_P]), "init_prefix": (_I, [_P, _P, _I]),
             "init_score": (_D, [_P]), "last_error": (ctypes.c_char_p, []),
+            "evals_train": (_I, [_P, _P, _I]),
         }
         for k, (res, args) in sig.items():
             f = getattr(self.lib, f"{prefix}_{k}")

@@ -63,3 +63,4 @@ def test_batched_matches_oracle_with_shared_edges():
         assert err is None, f"task {k} (rows {n}, h {task.horizon}): {err}"
         pg, po = m.predict(p.X), o.predict(p.X)
         assert np.allclose(pg, po, rtol=1e-5, atol=1e-9), f"task {k}: predictions"
+        assert np.allclose(m.evals_result()["train"], o.evals_train(), rtol=1e-10, atol=0), f"task {k}: loss"

@@ -2,7 +2,7 @@
 
 Bar (stated per test): tree structure (every integer node field, leaf cover) bit-exact; leaf and
 TORB prefix values within rtol 1e-7 (they come from exact fixed-point sums instead of sequential
-FP64 folds); predictions within rtol 1e-5.
+FP64 folds); predictions within rtol 1e-5; the per-tree training loss within rtol 1e-10.
 """
 import dataclasses
 import sys
@@ -17,6 +17,7 @@ pytestmark = pytest.mark.gpu
 
 VALUE_RTOL = 1e-7
 PRED_RTOL = 1e-5
+LOSS_RTOL = 1e-10
 
 
 def _say(msg):
@@ -55,6 +56,11 @@ def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
     Xp = p.X[:n_pred]
     rel = _pred_close(gpu.predict(Xp), orc.predict(Xp))
     assert rel < PRED_RTOL, f"{name}: prediction rel err {rel}"
+    # evals.train: the device reduces the weighted squared errors in a fixed parallel order (the
+    # reference sums sequentially) and the pairwise groups with the reference's sequential sums
+    ev, eo = gpu.evals_result()["train"], orc.evals_train()
+    assert len(ev) == len(eo) == params["n_estimators"], (len(ev), len(eo))
+    assert np.allclose(ev, eo, rtol=LOSS_RTOL, atol=0), f"{name}: training loss {ev} vs {eo}"
     ref = checkers.reference()
     if ref is not None:
         r = _fit_cpu(ref, p, rg=rg, iid=iid, **params)

@@ -104,6 +104,24 @@ def test_unsorted_rows_and_update():
     _assert_identical(o, r, extra.X)
 
 
+@pytest.mark.parametrize("objective", [0, 5])
+def test_training_loss_per_tree(objective):
+    # evals.train (objective.cpp:262-329, booster.cpp:704-706): weighted MSE, or minus the mean
+    # within-date Pearson correlation for the pairwise objective; bit-exact sequential sums
+    p = synth.panel_small(12)
+    w = np.random.default_rng(1).uniform(0.5, 2.0, p.n)
+    rg = p.ts if objective == 5 else None
+    out = []
+    for lib in (checkers.oracle(), REF):
+        m = lib.model(n_estimators=6, max_depth=5, learning_rate=0.1, goss=0, pair_splits=0, objective=objective)
+        m.fit(p.X, p.y, ts=p.ts, w=w if objective == 0 else None, entity=p.entity, rank_groups=rg, horizon=1)
+        out.append(m.evals_train())
+    assert len(out[0]) == 6
+    assert np.array_equal(out[0], out[1]), (out[0], out[1])
+    if objective == 0:
+        assert np.all(np.diff(out[0]) < 0)      # the training MSE falls tree by tree
+
+
 def test_histogram_and_scan_functions():
     # per-function pins: the restated histogram / split scan / gradients vs the reference's own
     rng = np.random.default_rng(3)

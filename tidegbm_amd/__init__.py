@@ -66,6 +66,7 @@ _sig("tide_profile", _I, _P, _I, _P, _P, _P)
 _sig("tide_fit_times", _I, _P, _P, _P)
 _sig("tide_goss_stats", _I, _P, _P, _P, _P)
 _sig("tide_hist_variant", _I, _P)
+_sig("tide_evals_train", _I, _P, _P, _I)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
@@ -247,6 +248,14 @@ class Booster:
         a, b = _D(), _D()
         _lib.tide_fit_times(self._h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def evals_result(self) -> dict:
+        """The reference's evals_result(): training loss per tree (no eval sets on this path)."""
+        n = int(_lib.tide_evals_train(self._h, None, 0))
+        out = np.empty(n)
+        if n:
+            _lib.tide_evals_train(self._h, _ptr(out), n)
+        return {"train": out, "valid": np.empty(0)}
 
     def hist_variant(self) -> int:
         """Histogram kernel variant the last fit ended with (tuned or forced; same trees)."""

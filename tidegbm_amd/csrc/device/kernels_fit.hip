@@ -1809,6 +1809,112 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
     hipLaunchKernelGGL(k_pred_update, dim3(rb, n_tasks), dim3(256), 0, st, tasks, lr);
 }
 
+// =================================================================== training loss
+// The loss recorded after every tree (objective.cpp:262-329 via booster.cpp:704-706), from the
+// updated predictions of all rows.  Weighted mean squared error: fixed-order two-level sums
+// (deterministic; equal to the reference's sequential sum up to rounding).  Pairwise objective:
+// one thread per group computes the group's Pearson correlation with the reference's
+// sequential sums, and one thread averages the groups in order (equal to the reference bit for
+// bit when the group values are).
+__global__ __launch_bounds__(256) void k_loss_mse(Task* tasks) {
+    const Task& T = tasks[blockIdx.y];
+    const int n = T.n_rows;
+    const auto y = gbl(T.y);
+    const auto pr = gbl((const double*)T.pred);
+    const double* w = T.w;
+    double s = 0.0, sw = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double wi = w ? gbl(w)[i] : 1.0;
+        const double d = pr[i] - y[i];
+        s += wi * d * d;
+        sw += wi;
+    }
+    __shared__ double ss[8], sws[8];
+    // wave sums (fixed lane order), then the waves in order
+    double ws = s, wsw = sw;
+    for (int o = 32; o > 0; o >>= 1) {
+        ws += __shfl_down(ws, o, 64);
+        wsw += __shfl_down(wsw, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        ss[wave] = ws;
+        sws[wave] = wsw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+            a += ss[k];
+            b += sws[k];
+        }
+        T.loss_part[2 * blockIdx.x] = a;
+        T.loss_part[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_loss_groups(Task* tasks) {
+    const Task& T = tasks[blockIdx.y];
+    const int grp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (grp >= T.n_groups) return;
+    const int g0 = T.group_start[grp], m = T.group_start[grp + 1] - g0;
+    const int* gr = T.group_rows ? T.group_rows + g0 : nullptr;
+    double val = __longlong_as_double(0x7ff8000000000000ll);   // NaN: the group does not count
+    if (m >= 3) {
+        double my = 0.0, mp = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const int r = gr ? gr[k] : g0 + k;
+            my += T.y[r];
+            mp += T.pred[r];
+        }
+        my /= m;
+        mp /= m;
+        double vy = 0.0, vp = 0.0, c = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const int r = gr ? gr[k] : g0 + k;
+            const double dy = T.y[r] - my, dp = T.pred[r] - mp;
+            vy += dy * dy;
+            vp += dp * dp;
+            c += dy * dp;
+        }
+        if (vy > 1e-18 && vp > 1e-18) val = c / sqrt(vy * vp);
+    }
+    T.loss_part[2 * LOSS_BLOCKS + grp] = val;
+}
+
+__global__ __launch_bounds__(64) void k_loss_final(Task* tasks, int pairwise) {
+    Task& T = tasks[blockIdx.x];
+    if (threadIdx.x != 0) return;
+    if (pairwise && T.has_groups) {
+        double acc = 0.0;
+        int ng = 0;
+        for (int g = 0; g < T.n_groups; ++g) {
+            const double v = T.loss_part[2 * LOSS_BLOCKS + g];
+            if (v == v) {
+                acc += v;
+                ng += 1;
+            }
+        }
+        T.train_loss = ng > 0 ? -acc / ng : 0.0;
+        return;
+    }
+    double s = 0.0, sw = 0.0;
+    for (int b = 0; b < LOSS_BLOCKS; ++b) {
+        s += T.loss_part[2 * b];
+        sw += T.loss_part[2 * b + 1];
+    }
+    T.train_loss = sw > 0 ? s / sw : 0.0;
+}
+
+void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int pairwise, int max_groups) {
+    if (pairwise)
+        hipLaunchKernelGGL(k_loss_groups, dim3(std::max(1, (max_groups + 255) / 256), n_tasks), dim3(256), 0, st,
+                           tasks);
+    else
+        hipLaunchKernelGGL(k_loss_mse, dim3(LOSS_BLOCKS, n_tasks), dim3(256), 0, st, tasks);
+    hipLaunchKernelGGL(k_loss_final, dim3(n_tasks), dim3(64), 0, st, tasks, pairwise);
+}
+
 // Finished trees of all active tasks packed into one staging buffer (one copy to the host):
 // task i at out + i*L.stride holds n_nodes, nodes[0..nn), values[0..nn), cover[0..nn) and
 // prefix[0..nn*n_buckets) at the offsets of L.
@@ -1819,6 +1925,7 @@ __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, cha
     if (threadIdx.x == 0) {
         *(int*)base = nn;
         *((int*)base + 1) = T.err;
+        *((double*)base + 1) = T.train_loss;   // header bytes 8..15
     }
     const int* src = (const int*)T.nodes;
     int* dst = (int*)(base + L.off_nodes);

@@ -69,6 +69,10 @@ struct TaskHost {
     size_t gtmp_bytes = 0;
     int* gstart_d = nullptr;         // pairwise groups (allocated on first use)
     int* grows_d = nullptr;
+    int n_groups = 0;
+    double* loss_d = nullptr;        // loss partial sums (allocated on first use, loss_cap)
+    int loss_cap = 0;
+    double last_loss = 0.0;          // training loss after the last exported tree
     bool has_sample = false;
     int n_sample = 0;
     bool goss = false;
@@ -85,6 +89,7 @@ struct tgbm_session {
     int n = 0, F = 0, fp = 0, n_pad = 0, nb = 255, T = 1, max_leaves = 31, max_nodes = 61;
     int a0 = 0, an = 1;     // active tasks [a0, a0 + an): every fit launch covers exactly these
     bool hconst = true;
+    int objective = 0;      // objective of the last gradient launch (the tree end's loss)
     hipStream_t stream = nullptr;
     Shared S{};
     unsigned char* bins_row = nullptr;
@@ -566,6 +571,7 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
             HIP_OK(hipMemcpy(gsd, gs.data(), sizeof(int) * gs.size(), hipMemcpyHostToDevice));
             s->set_field_ptr(task, offsetof(Task, group_start), gsd);
             s->set_field_int(task, offsetof(Task, n_groups), (int)gs.size() - 1);
+            H.n_groups = (int)gs.size() - 1;
             s->set_field_int(task, offsetof(Task, has_groups), 1);
             if (!sorted) {
                 if (!H.grows_d) H.grows_d = s->alloc<int>(H.cap_rows);
@@ -592,6 +598,7 @@ int tgbm_task_grad(tgbm_session* s, int task, int objective) {
         EventPair ep;
         if (s->timing) s->tic(4, 0.0, ep);
         launch_grad(s->stream, s->tasks_d + t0, nt, objective, mg, mg);
+        s->objective = objective;
         if (s->timing) s->toc(ep);
         HIP_OK(hipGetLastError());
     });
@@ -903,6 +910,21 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
         launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
                         (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
                         min_cover, iid, learning_rate);
+        // training loss from the updated predictions (exported with the tree)
+        const bool pairwise = s->objective == 5;
+        int max_groups = 1;
+        for (int t = s->a0; t < s->a0 + s->an; ++t) {
+            TaskHost& H = s->th[t];
+            const int need = 2 * LOSS_BLOCKS + H.n_groups;
+            if (need > H.loss_cap) {
+                H.loss_d = s->alloc<double>((size_t)need);
+                H.loss_cap = need;
+                s->tasks_h[t].loss_part = H.loss_d;
+                s->set_field_ptr(t, offsetof(Task, loss_part), H.loss_d);
+            }
+            max_groups = std::max(max_groups, H.n_groups);
+        }
+        launch_loss(s->stream, s->td(), s->an, pairwise ? 1 : 0, max_groups);
         if (s->timing) s->toc(ep);
         HIP_OK(hipGetLastError());
         int bmax = 1;
@@ -929,6 +951,7 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
             if (*((const int*)base + 1) != 0)
                 throw std::runtime_error("internal: a partition placed a different number of rows left than its split's histogram counts");
             n_nodes[i] = nn;
+            s->th[t].last_loss = *((const double*)base + 1);
             const int B = s->th[t].n_buckets;
             const DNode* nodes = (const DNode*)(base + L.off_nodes);
             for (int k = 0; k < nn; ++k) {
@@ -955,6 +978,13 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
                 }
             }
         }
+    });
+}
+
+int tgbm_task_train_loss(tgbm_session* s, int task, double* loss) {
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        *loss = s->th[task].last_loss;
     });
 }
 

@@ -128,6 +128,9 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
             for (int i = 0; i < count; ++i) {
                 Booster& b = *out[first + i];
                 b.hist_variant = gp.hist_mode;
+                double loss = 0.0;
+                dev_check(tgbm_task_train_loss(so.s, first + i, &loss));
+                b.evals_train.push_back(loss);
                 const size_t o = (size_t)i * MN;
                 b.trees.push_back(unpack_tree(nn[i], &ints[o * TGBM_NODE_INTS], &dbl[o * 2], &vals[o], &cov[o],
                                               &pref[o * 256], b.n_time_buckets));

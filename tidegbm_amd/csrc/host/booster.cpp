@@ -701,7 +701,10 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
     goss_host_trees = 0;
     goss_wait_s = goss_sample_s = 0.0;
     const double t_loop = now_s();
-    if (!for_update) tree_end_times.clear();
+    if (!for_update) {
+        tree_end_times.clear();
+        evals_train.clear();   // booster.cpp:481: a fit starts a new eval record
+    }
     for (int m = 0; m < n_trees; ++m) {
         dev_check(tgbm_task_grad(s, -1, cfg.objective));
         if (goss) {
@@ -732,6 +735,11 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
                                 ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
         tuner.record(m, now_s() - t_tree);   // tree_end synchronized
+        if (!for_update) {
+            double loss = 0.0;
+            dev_check(tgbm_task_train_loss(s, 0, &loss));
+            evals_train.push_back(loss);   // booster.cpp:704-706
+        }
         Tree t = unpack_tree(nn, ints.data(), dbl.data(), vals.data(), cov.data(), pref.data(), n_time_buckets);
         if (for_update) {
             // Booster::update keeps grow_tree's cover (node sample counts) and one bucket

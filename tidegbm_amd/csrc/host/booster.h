@@ -119,6 +119,7 @@ public:
     // last fit diagnostics
     double t_setup = 0.0, t_boost = 0.0;
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
+    std::vector<double> evals_train;     // training loss after each tree (reference evals.train)
     int goss_host_trees = 0;             // GOSS trees of the last fit sampled on the host (equal |g|)
     int hist_variant = 0;                // histogram kernel variant the last fit ended with
     double goss_wait_s = 0.0, goss_sample_s = 0.0;  // GOSS: waiting for draws, the sample step

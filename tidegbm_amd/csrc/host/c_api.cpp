@@ -292,6 +292,12 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
 
 int tide_hist_variant(TideHandle h) { return static_cast<Booster*>(h)->hist_variant; }
 
+int tide_evals_train(TideHandle h, double* out, int cap) {
+    const auto& e = static_cast<Booster*>(h)->evals_train;
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
+
 int tide_goss_stats(TideHandle h, int* host_trees, double* wait_s, double* sample_s) {
     auto* b = static_cast<Booster*>(h);
     *host_trees = b->goss_host_trees;
