@@ -212,16 +212,68 @@ def rows_reaching(ints, xb, node, n_bins=256):
     return reach
 
 
+def leaves_of(ints, xb, n_bins=256):
+    """Leaf node of every row (reference apply_one, tree.cpp:15-55)."""
+    leaf = np.zeros(xb.shape[0], np.int64)
+    stack = [(0, np.ones(xb.shape[0], bool))]
+    while stack:
+        k, m = stack.pop()
+        if not m.any():
+            continue
+        if ints[k, 0]:
+            leaf[m] = k
+            continue
+        f, f2, thr, ml = ints[k, 2], ints[k, 3], ints[k, 4], ints[k, 7]
+        b = xb[:, f]
+        if ints[k, 1] == 1:
+            b2 = xb[:, f2]
+            z = ((b.astype(np.uint64) * 2654435761 + b2.astype(np.uint64) * 40503) & 0xFFFFFFFF) % n_bins
+            left = np.where((b == 0) | (b2 == 0), ml == 1, z.astype(np.int64) <= thr)
+        else:
+            left = np.where(b == 0, ml == 1, b <= thr)
+        stack.append((ints[k, 5], m & left))
+        stack.append((ints[k, 6], m & ~left))
+    return leaf
+
+
+def tie_split(x, y, xb, k, n_bins=256):
+    """Node k is an axis split in both trees and the two splits cut the rows reaching it into
+    the same two sets (equal, or with the sides swapped): their gains are equal in exact
+    arithmetic, and only rounding in the reference's FP64 sums can order them."""
+    xi, yi = x["ints"], y["ints"]
+    if xi[k, 0] or yi[k, 0] or xi[k, 1] != 0 or yi[k, 1] != 0:
+        return False
+    m = rows_reaching(xi, xb, k, n_bins)
+    if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins)):
+        return False
+
+    def goes_left(ints):
+        b = xb[:, ints[k, 2]]
+        return m & np.where(b == 0, ints[k, 7] == 1, b <= ints[k, 4])
+
+    lx, ly = goes_left(xi), goes_left(yi)
+    return bool(np.array_equal(lx, ly) or np.array_equal(lx, m & ~ly))
+
+
 def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins=256):
     """First structural difference between two tree lists, or None.
 
     Integer node fields must match exactly; value/gain/values/prefix within tolerance.
 
-    With the training rows' bins `xb`, one exception is accepted: missing_go_left of an axis
-    split whose node received no training row with the split feature missing.  There the
-    reference's flag is set by the rounding residue that its FP64 parent - sibling histogram
-    subtraction leaves in the (empty) missing bin; exact sums leave no residue and keep
-    missing-right, the first maximum.  No training row is routed by such a flag.
+    With the training rows' bins `xb`, two exceptions are accepted, both verified on the rows:
+    - missing_go_left of an axis split whose node received no training row with the split
+      feature missing.  There the reference's flag is set by the rounding residue that its FP64
+      parent - sibling histogram subtraction leaves in the (empty) missing bin; exact sums leave
+      no residue and keep missing-right, the first maximum.  No training row is routed by such a
+      flag.
+    - a different feature (with its threshold and missing direction) at an axis split when both
+      splits send exactly the same training rows of the node left.  Their gains are then equal
+      in exact arithmetic, and exact sums pick the lower feature index; the reference's
+      sequential FP64 sums group the same rows by each feature's bins and differ by a few ulps,
+      which decide its choice.  Everything below such a node sees the same rows.
+    When the two splits send the same sets to opposite sides, the subtrees are swapped and node
+    numbers below differ; such a tree is accepted when every training row reaches leaves of equal
+    value and cover in both trees (the next trees then see the same predictions and must match).
     """
     if len(a) != len(b):
         return f"tree count {len(a)} != {len(b)}"
@@ -230,14 +282,37 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
             return f"tree {t}: node count {x['ints'].shape[0]} != {y['ints'].shape[0]}"
         bad = np.argwhere(x["ints"] != y["ints"])
         if bad.size and xb is not None:
+            xi, yi = x["ints"], y["ints"]
+            tied = set()
+            for k in {int(k) for k, c in bad if c == 2}:
+                if xi[k, 0] or yi[k, 0] or xi[k, 1] != 0 or yi[k, 1] != 0:
+                    continue
+                m = rows_reaching(xi, xb, k, n_bins)
+                if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins)):
+                    continue
+                def goes_left(ints):
+                    b = xb[:, ints[k, 2]]
+                    return np.where(b == 0, ints[k, 7] == 1, b <= ints[k, 4])
+                if np.array_equal(m & goes_left(xi), m & goes_left(yi)):
+                    tied.add(k)
             keep = []
             for k, c in bad:
-                if c == 7 and x["ints"][k, 1] == 0 and x["ints"][k, 0] == 0:
-                    m = rows_reaching(x["ints"], xb, k, n_bins)
-                    if not np.any(xb[m, x["ints"][k, 2]] == 0):
+                if k in tied and c in (2, 4, 7):
+                    continue
+                if c == 7 and xi[k, 1] == 0 and xi[k, 0] == 0:
+                    m = rows_reaching(xi, xb, k, n_bins)
+                    if not np.any(xb[m, xi[k, 2]] == 0):
                         continue
                 keep.append((k, c))
             bad = np.array(keep).reshape(-1, 2)
+        if bad.size and xb is not None:
+            # a split decided by an exact-arithmetic tie: the same function on the training rows
+            k = int(bad[:, 0].min())
+            if tie_split(x, y, xb, k, n_bins):
+                lx, ly = leaves_of(x["ints"], xb, n_bins), leaves_of(y["ints"], xb, n_bins)
+                if (np.allclose(x["values"][lx], y["values"][ly], rtol=rtol, atol=atol)
+                        and np.array_equal(x["cover"][lx], y["cover"][ly])):
+                    continue
         if bad.size:
             k, c = bad[0]
             return f"tree {t} node {k} {STRUCT_COLS[c]}: {x['ints'][k, c]} != {y['ints'][k, c]}"

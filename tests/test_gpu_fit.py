@@ -138,6 +138,20 @@ def test_larger_panel_depth8_missing_values():
     _check("large-subset-mse", p, n_estimators=8, max_depth=8, learning_rate=0.05, goss=0, pair_splits=0)
 
 
+def test_wide_panel_300_features():
+    # more features than a 256-thread selection block and than 16-feature tiles divide evenly
+    p = synth.panel_small(13, n_dates=300, n_entities=40, n_features=300)
+    _check("wide-300", p, n_estimators=5, max_depth=6, learning_rate=0.1, goss=0, pair_splits=0)
+
+
+@pytest.mark.parametrize("n_dates,n_entities", [(20, 5), (7, 3)])
+def test_tiny_panels(n_dates, n_entities):
+    # 100 and 21 rows: nodes below min_samples_leaf, tails inside the first tile of every kernel
+    p = synth.panel_small(14, n_dates=n_dates, n_entities=n_entities, n_features=5)
+    _check(f"tiny-{p.n}", p, n_estimators=6, max_depth=4, learning_rate=0.1, goss=0, pair_splits=0,
+           min_samples_leaf=2)
+
+
 def test_refit_update_matches_oracle():
     p = synth.panel_small(6)
     _say("[case] update")
