@@ -25,16 +25,16 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _fit_gpu(p, rg=None, iid=False, horizon=1, **params):
+def _fit_gpu(p, rg=None, iid=False, horizon=1, w=None, **params):
     b = Booster(**params)
-    b.fit(p.X, p.y, timestamps=None if iid else p.ts, entity=None if iid else p.entity,
+    b.fit(p.X, p.y, timestamps=None if iid else p.ts, weight=w, entity=None if iid else p.entity,
           rank_groups=rg, iid=iid, horizon=horizon)
     return b
 
 
-def _fit_cpu(lib, p, rg=None, iid=False, horizon=1, **params):
+def _fit_cpu(lib, p, rg=None, iid=False, horizon=1, w=None, **params):
     m = lib.model(**params)
-    m.fit(p.X, p.y, ts=None if iid else p.ts, entity=None if iid else p.entity, rank_groups=rg,
+    m.fit(p.X, p.y, ts=None if iid else p.ts, w=w, entity=None if iid else p.entity, rank_groups=rg,
           iid=iid, horizon=horizon)
     return m
 
@@ -44,10 +44,10 @@ def _pred_close(a, b):
     return float(np.max(np.abs(a - b) / scale))
 
 
-def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
+def _check(name, p, rg=None, iid=False, n_pred=2000, w=None, horizon=1, **params):
     _say(f"[case] {name} n={p.n} F={p.n_features} params={params}")
-    gpu = _fit_gpu(p, rg=rg, iid=iid, **params)
-    orc = _fit_cpu(checkers.oracle(), p, rg=rg, iid=iid, **params)
+    gpu = _fit_gpu(p, rg=rg, iid=iid, w=w, horizon=horizon, **params)
+    orc = _fit_cpu(checkers.oracle(), p, rg=rg, iid=iid, w=w, horizon=horizon, **params)
     # training-row bins: lets the comparison accept the one characterized flag difference
     xb = checkers.bins_of(p.X, [gpu.edges(f) for f in range(p.n_features)])
     nb = gpu.model_info()["n_bins"]
@@ -63,7 +63,7 @@ def _check(name, p, rg=None, iid=False, n_pred=2000, **params):
     assert np.allclose(ev, eo, rtol=LOSS_RTOL, atol=0), f"{name}: training loss {ev} vs {eo}"
     ref = checkers.reference()
     if ref is not None:
-        r = _fit_cpu(ref, p, rg=rg, iid=iid, **params)
+        r = _fit_cpu(ref, p, rg=rg, iid=iid, w=w, horizon=horizon, **params)
         diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb, n_bins=nb)
         assert diff is None, f"{name}: GPU vs reference: {diff}"
     return gpu, orc
@@ -150,6 +150,36 @@ def test_tiny_panels(n_dates, n_entities):
     p = synth.panel_small(14, n_dates=n_dates, n_entities=n_entities, n_features=5)
     _check(f"tiny-{p.n}", p, n_estimators=6, max_depth=4, learning_rate=0.1, goss=0, pair_splits=0,
            min_samples_leaf=2)
+
+
+# Growth and leaf parameters over their ranges, one small panel each (seeded), against the oracle
+# and the reference core: depth / leaf budget, minimum leaf rows and weight, L1 / L2, minimum gain,
+# delta clipping, learning rate, coarse bins, time buckets, embargo, horizon, iid mode, weights.
+SWEEP = [
+    ("leaves-63-depth-9", dict(max_depth=9, max_leaves=63, min_samples_leaf=5), {}),
+    ("depth-3", dict(max_depth=3, min_samples_leaf=50), {}),
+    ("l1-l2", dict(l1=0.5, l2=5.0), {}),
+    ("min-gain-and-weight", dict(min_split_gain=0.05, min_child_weight=8.0), {}),
+    ("delta-clip", dict(max_delta_step=0.05, learning_rate=0.3), {}),
+    ("coarse-bins", dict(max_bin=15), {}),
+    ("bins-63-buckets-8", dict(max_bin=63, n_time_buckets=8), {}),
+    ("embargo-horizon-5", dict(embargo=2), dict(horizon=5)),
+    ("no-finer-torb", dict(finer_torb=0, torb_min_cover=100.0), {}),
+    ("iid-weighted", dict(), dict(iid=True, weighted=True)),
+    ("time-weighted", dict(min_child_weight=2.0), dict(weighted=True)),
+    ("pairwise-l1", dict(objective=5, l1=0.1, max_depth=5), dict(groups=True)),
+]
+
+
+@pytest.mark.parametrize("name,params,opts", SWEEP, ids=[c[0] for c in SWEEP])
+def test_parameter_sweep_matches_oracle(name, params, opts):
+    seed = 100 + [c[0] for c in SWEEP].index(name)
+    p = synth.panel_small(seed, n_dates=200, n_entities=40, n_features=12)
+    w = np.random.default_rng(seed).uniform(0.2, 3.0, p.n) if opts.get("weighted") else None
+    base = dict(n_estimators=8, max_depth=6, learning_rate=0.1, goss=0, pair_splits=0)
+    base.update(params)
+    _check(name, p, rg=p.ts if opts.get("groups") else None, iid=opts.get("iid", False), w=w,
+           horizon=opts.get("horizon", 1), **base)
 
 
 def test_refit_update_matches_oracle():
