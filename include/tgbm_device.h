@@ -195,6 +195,29 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
  * predictions without a host round trip.  Call after tgbm_task_bind. */
 int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f);
 
+/* ---- validation set scored tree by tree during a fit (reference Booster::fit with X_eval,
+ * booster.cpp:459-478 and :707-732) ----
+ * The rows are binned once with the fit's edges (edges[f*256 + k], k < n_edges[f]).  A device
+ * prediction per row accumulates init + lr * values[leaf] tree by tree, the running sum of
+ * predict_multi_binned (booster.cpp:775-799).  For the per-date correlation metric, order lists
+ * the rows of every date of at least 3 rows, date after date, and date g is
+ * order[group_start[g] .. group_start[g+1]) (grouped_ic, booster.cpp:155-174); n_groups may be 0. */
+typedef struct tgbm_evalset tgbm_evalset;
+int tgbm_eval_create(tgbm_evalset** out, int n_features, int n_bins, const double* edges,
+                     const int* n_edges, const double* X, const double* y, int n,
+                     const int* order, const int* group_start, int n_groups);
+void tgbm_eval_destroy(tgbm_evalset* e);
+/* Every row's prediction = init_score. */
+int tgbm_eval_reset(tgbm_evalset* e, double init_score);
+/* Add one tree (export node layout, TGBM_NODE_INTS ints per node; values per node) times lr. */
+int tgbm_eval_add_tree(tgbm_evalset* e, int n_nodes, const int* node_ints, const double* values,
+                       double lr);
+/* metric 0: mean squared error, unit weights, rows summed in order (objective.cpp:323-328);
+ * 1: mean over the listed dates of the Pearson correlation of y and prediction (0 without dates). */
+int tgbm_eval_metric(tgbm_evalset* e, int metric, double* out);
+/* The current predictions (first cap rows). */
+int tgbm_eval_predictions(tgbm_evalset* e, double* out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

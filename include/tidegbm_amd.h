@@ -87,6 +87,16 @@ int tide_hist_variant(TideHandle h);
 /* Training loss after each tree of the last fit, the reference's evals_result()["train"]
  * (booster.cpp:704-706, bindings.cpp:318-321).  Returns the number of trees (at most cap copied). */
 int tide_evals_train(TideHandle h, double* out, int cap);
+/* Eval set of the following fits: the X_eval, y_eval, ts_eval, n_eval arguments of the reference's
+ * Booster::fit (booster.hpp:45-48, booster.cpp:231-234), which its C API cannot pass.  X_eval is
+ * n_eval x n_features row-major; ts_eval may be NULL in iid mode; n_eval = 0 clears the set.
+ * Each tree is then scored on these rows on the GPU ("eval_metric" auto | loss | ic, tide_set_string)
+ * and "early_stopping_rounds" (tide_set_int) applies as in booster.cpp:707-732. */
+int tide_set_eval(TideHandle h, const double* X_eval, const double* y_eval, const long long* ts_eval,
+                  int n_eval, int n_features);
+/* Validation metric after each tree of the last fit, the reference's evals_result()["valid"].
+ * Returns the number of values (at most cap copied). */
+int tide_evals_valid(TideHandle h, double* out, int cap);
 /* Wall time (s since the boosting loop began) at the end of each tree of the last fit; each
  * tree ends with a device synchronization.  Returns the number of trees. */
 int tide_tree_times(TideHandle h, double* out, int cap);

@@ -15,12 +15,23 @@
 #include "tidegbm/tree.hpp"
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace tidegbm;
 
 static thread_local std::string g_ref_err;
+
+// Eval sets handed to Booster::fit (X_eval, y_eval, ts_eval, n_eval), kept per handle.
+struct EvalSet {
+    std::vector<double> X, y;
+    std::vector<long long> ts;
+    int n = 0;
+};
+static std::map<void*, EvalSet> g_eval;
+static std::mutex g_eval_mu;
 
 template <class F>
 static int guarded(F&& fn) {
@@ -38,7 +49,25 @@ extern "C" {
 const char* tref_last_error(void) { return g_ref_err.c_str(); }
 
 void* tref_create(void) { return new Booster(); }
-void tref_free(void* h) { delete static_cast<Booster*>(h); }
+void tref_free(void* h) {
+    {
+        std::lock_guard<std::mutex> lk(g_eval_mu);
+        g_eval.erase(h);
+    }
+    delete static_cast<Booster*>(h);
+}
+
+// Eval set of later fits: X (n x nf, row-major), y, ts (may be NULL); n = 0 clears.
+int tref_set_eval(void* h, const double* X, const double* y, const long long* ts, int n, int nf) {
+    std::lock_guard<std::mutex> lk(g_eval_mu);
+    EvalSet& e = g_eval[h];
+    e.n = n > 0 ? n : 0;
+    e.X.assign(X, X + (size_t)e.n * (nf > 0 ? nf : 0));
+    e.y.assign(y, y + e.n);
+    if (ts) e.ts.assign(ts, ts + e.n);
+    else e.ts.clear();
+    return 0;
+}
 
 // Numeric config setter; keys are the reference Config field names (config.hpp:12-76).
 int tref_set(void* h, const char* key, double v) {
@@ -74,6 +103,8 @@ int tref_set(void* h, const char* key, double v) {
     else if (k == "max_delta_step") c.max_delta_step = v;
     else if (k == "col_major_hist") c.col_major_hist = iv != 0;
     else if (k == "ooc_bytes") c.ooc_bytes = static_cast<size_t>(v);
+    else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
+    else if (k == "eval_metric") c.eval_metric = iv == 1 ? "loss" : (iv == 2 ? "ic" : "auto");
     else {
         g_ref_err = "unknown key " + k;
         return -1;
@@ -85,11 +116,19 @@ int tref_fit(void* h, const double* X, const double* y, int n, int nf, const lon
              const double* w, const long long* entity, const long long* rank_groups, int iid,
              long long horizon) {
     auto* b = static_cast<Booster*>(h);
+    EvalSet ev;
+    {
+        std::lock_guard<std::mutex> lk(g_eval_mu);
+        auto it = g_eval.find(h);
+        if (it != g_eval.end()) ev = it->second;
+    }
     return guarded([&] {
         b->cfg.iid_mode = iid != 0;
         if (!b->cfg.iid_mode) b->cfg.horizon = horizon;
         b->fit(X, y, n, nf, reinterpret_cast<const i64*>(ts), w, nullptr, nullptr, nullptr,
-               reinterpret_cast<const i64*>(rank_groups), nullptr, nullptr, nullptr, 0,
+               reinterpret_cast<const i64*>(rank_groups), ev.n > 0 ? ev.X.data() : nullptr,
+               ev.n > 0 ? ev.y.data() : nullptr,
+               ev.n > 0 && !ev.ts.empty() ? reinterpret_cast<const i64*>(ev.ts.data()) : nullptr, ev.n,
                reinterpret_cast<const i64*>(entity), nullptr);
     });
 }
@@ -118,6 +157,11 @@ int tref_n_bins(void* h) { return static_cast<Booster*>(h)->n_bins; }
 int tref_n_time_buckets(void* h) { return static_cast<Booster*>(h)->n_time_buckets; }
 double tref_init_score(void* h) { return static_cast<Booster*>(h)->init_score; }
 int tref_best_iteration(void* h) { return static_cast<Booster*>(h)->best_iteration; }
+int tref_evals_valid(void* h, double* out, int cap) {
+    const auto& e = static_cast<Booster*>(h)->evals.valid;
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
 int tref_evals_train(void* h, double* out, int cap) {
     const auto& e = static_cast<Booster*>(h)->evals.train;
     for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];

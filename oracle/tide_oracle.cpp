@@ -54,6 +54,8 @@ struct Cfg {  // defaults: reference cpp/include/tidegbm/config.hpp:13-76, cpp/s
     int bin_sample = 262144;
     bool finer_torb = true;
     double torb_min_cover = 0.0;
+    int early_stopping_rounds = 0;  // config.hpp early_stopping_rounds (api.py:23-24: None -> 0)
+    int eval_metric = 0;            // config.hpp:76 "auto" | "loss" | "ic" as 0 | 1 | 2
 };
 
 struct Node {  // tree.hpp:11-25
@@ -533,6 +535,52 @@ static double loss_of(int objective, const double* y, const double* p, const dou
     return sw > 0 ? sum / sw : 0;
 }
 
+// Validation metric helpers (booster.cpp:132-174).  pearson_ix: sequential means and sums over
+// the listed rows; a group of fewer than 3 rows or with a variance below 1e-18 scores 0.
+static double pearson_ix(const double* y, const double* p, const int* ix, int m) {
+    if (m < 3) return 0.0;
+    double my = 0, mp = 0;
+    for (int k = 0; k < m; ++k) {
+        my += y[ix[k]];
+        mp += p[ix[k]];
+    }
+    my /= m;
+    mp /= m;
+    double vy = 0, vp = 0, c = 0;
+    for (int k = 0; k < m; ++k) {
+        const double dy = y[ix[k]] - my;
+        const double dp = p[ix[k]] - mp;
+        vy += dy * dy;
+        vp += dp * dp;
+        c += dy * dp;
+    }
+    if (vy < 1e-18 || vp < 1e-18) return 0.0;
+    return c / std::sqrt(vy * vp);
+}
+
+// grouped_ic (booster.cpp:155-174): rows ordered by timestamp with std::sort (not stable: the
+// order inside a date is the library's, and it feeds the sequential sums), the mean of the
+// per-date correlations over dates of at least 3 rows; no timestamps = one correlation.
+static double grouped_ic(const double* y, const double* pred, const i64* ts, int n) {
+    if (n < 3) return 0.0;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    if (!ts) return pearson_ix(y, pred, order.data(), n);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return ts[a] < ts[b]; });
+    double s = 0;
+    int ng = 0;
+    for (int i = 0; i < n;) {
+        int j = i + 1;
+        while (j < n && ts[order[j]] == ts[order[i]]) ++j;
+        if (j - i >= 3) {
+            s += pearson_ix(y, pred, order.data() + i, j - i);
+            ng += 1;
+        }
+        i = j;
+    }
+    return ng > 0 ? s / ng : 0.0;
+}
+
 static double wmean(const double* y, const double* w, int n) {  // objective.cpp:41-49
     double sw = 0, sy = 0;
     for (int i = 0; i < n; ++i) {
@@ -551,6 +599,11 @@ struct Model {
     std::vector<double> init_prefix;
     std::vector<Tree> trees;
     std::vector<double> evals_train;   // training loss after each tree of the last fit
+    std::vector<double> evals_valid;   // validation metric after each tree of the last fit
+    // Eval set of later fits (Booster::fit's X_eval, y_eval, ts_eval, n_eval, booster.cpp:231-234).
+    std::vector<double> Xe, ye;
+    std::vector<i64> te;
+    int n_eval = 0;
     bool fitted = false;
     // Caller-supplied bin edges replace Binner::fit (checker mode for fits that share a binned
     // matrix, e.g. walk-forward folds binned once); empty = fit edges as the reference does.
@@ -675,6 +728,29 @@ struct Model {
                 if (!std::isfinite(init_prefix[b])) init_prefix[b] = init_score;
             for (int i = 0; i < n; ++i) pred[i] = init_prefix[bucket[i]];
         }
+        // eval set (booster.cpp:459-478): later than the training rows (minus the embargo), binned
+        // with the training edges
+        std::vector<uint16_t> xbe;
+        if (n_eval > 0) {
+            if (Xe.size() != (size_t)n_eval * F) throw std::runtime_error("eval set: feature count mismatch");
+            if (!c.iid) {
+                if (te.empty()) throw std::runtime_error("eval_timestamp is required when iid_mode=false");
+                i64 tmax = ts[0];
+                for (int i = 1; i < n; ++i) tmax = std::max(tmax, ts[i]);
+                i64 tmin_e = te[0];
+                for (int i = 1; i < n_eval; ++i) tmin_e = std::min(tmin_e, te[i]);
+                if (tmin_e < tmax - embargo)
+                    throw std::runtime_error(
+                        "eval timestamps overlap the training future. "
+                        "Validation must be strictly later than train (minus embargo/horizon).");
+            }
+            bins_of(Xe.data(), n_eval, xbe);
+        }
+        const bool eval_ic = n_eval > 0 && (c.eval_metric == 2 || (c.eval_metric == 0 && (c.objective == 5 || !c.iid)));
+        double best_valid = eval_ic ? -1e300 : 1e300;
+        int stall = 0;
+        best_iteration = 0;
+        evals_valid.clear();
         trees.clear();
         evals_train.clear();
         std::mt19937 rng((uint32_t)c.random_state);
@@ -760,8 +836,46 @@ struct Model {
             const double lr = c.learning_rate;
             for (int i = 0; i < n; ++i) pred[i] += lr * T.prefix[(size_t)bucket[i] * nn + nid[i]];
             evals_train.push_back(loss_of(c.objective, y.data(), pred.data(), ws.data(), grp, n));
+            if (n_eval > 0) {  // booster.cpp:707-732
+                // predict_multi_binned (booster.cpp:775-799) scores the first best_iteration trees
+                // once an iteration has improved (all grown trees before that)
+                const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : c.n_estimators);
+                std::vector<double> ev(n_eval);
+                for (int i = 0; i < n_eval; ++i) {
+                    double s = init_score;
+                    for (int k = 0; k < use; ++k)
+                        s += c.learning_rate * trees[k].values[leaf_of(trees[k], &xbe[(size_t)i * F], nb)];
+                    ev[i] = s;
+                }
+                double vl;
+                if (eval_ic) {
+                    vl = grouped_ic(ye.data(), ev.data(), te.empty() ? nullptr : te.data(), n_eval);
+                } else {  // Objective::loss with unit weights and no groups: mean squared error
+                    double s = 0, sw = 0;
+                    for (int i = 0; i < n_eval; ++i) {
+                        const double d = ev[i] - ye[i];
+                        s += 1.0 * d * d;
+                        sw += 1.0;
+                    }
+                    vl = sw > 0 ? s / sw : 0;
+                }
+                evals_valid.push_back(vl);
+                const bool improved = eval_ic ? (vl > best_valid + 1e-12) : (vl + 1e-12 < best_valid);
+                if (improved) {
+                    best_valid = vl;
+                    stall = 0;
+                    best_iteration = m + 1;
+                } else {
+                    stall += 1;
+                    if (c.early_stopping_rounds > 0 && stall >= c.early_stopping_rounds) {
+                        const int keep = best_iteration;
+                        if (keep > 0 && keep < (int)trees.size()) trees.resize(keep);
+                        break;
+                    }
+                }
+            }
         }
-        best_iteration = c.n_estimators;
+        if (best_iteration <= 0) best_iteration = c.n_estimators;  // booster.cpp:739
         fitted = true;
     }
 
@@ -869,6 +983,14 @@ int orc_set(void* hd, const char* key, double v) {
     else if (k == "bin_sample") c.bin_sample = iv;
     else if (k == "finer_torb") c.finer_torb = iv != 0;
     else if (k == "torb_min_cover") c.torb_min_cover = v;
+    else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
+    else if (k == "eval_metric") {
+        if (iv < 0 || iv > 2) {
+            g_orc_err = "eval_metric code must be 0 (auto), 1 (loss) or 2 (ic)";
+            return -1;
+        }
+        c.eval_metric = iv;
+    }
     else if (k == "n_jobs") { /* scalar oracle */ }
     else {
         g_orc_err = "unknown key " + k;
@@ -930,6 +1052,24 @@ int orc_edges(void* hd, int f, double* out, int cap) {
     return (int)e.size();
 }
 int orc_n_trees(void* hd) { return (int)static_cast<Model*>(hd)->trees.size(); }
+// Eval set of later fits: X (n x n_features, row-major), y, ts (may be NULL in iid mode); n = 0 clears.
+int orc_set_eval(void* hd, const double* X, const double* y, const long long* ts, int n, int n_features) {
+    Model* m = static_cast<Model*>(hd);
+    return orc_guard([&] {
+        if (n < 0 || (n > 0 && (!X || !y || n_features <= 0))) throw std::runtime_error("bad eval set");
+        m->n_eval = n;
+        m->Xe.assign(X, X + (size_t)n * std::max(n_features, 0));
+        m->ye.assign(y, y + n);
+        if (ts) m->te.assign(ts, ts + n);
+        else m->te.clear();
+    });
+}
+int orc_best_iteration(void* hd) { return static_cast<Model*>(hd)->best_iteration; }
+int orc_evals_valid(void* hd, double* out, int cap) {
+    const auto& e = static_cast<Model*>(hd)->evals_valid;
+    if (out) std::copy(e.begin(), e.begin() + std::min<size_t>(e.size(), (size_t)std::max(cap, 0)), out);
+    return (int)e.size();
+}
 int orc_evals_train(void* hd, double* out, int cap) {
     const auto& e = static_cast<Model*>(hd)->evals_train;
     for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];

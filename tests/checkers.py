@@ -115,6 +115,28 @@ class CpuModel:
         self.lib.fn("evals_train")(self.h, _ptr(out), n)
         return out
 
+    def evals_valid(self):
+        """Validation metric after each tree of the last fit (reference evals.valid)."""
+        n = self.lib.fn("evals_valid")(self.h, None, 0)
+        out = np.empty(n)
+        self.lib.fn("evals_valid")(self.h, _ptr(out), n)
+        return out
+
+    def best_iteration(self):
+        return self.lib.fn("best_iteration")(self.h)
+
+    def set_eval(self, X, y, ts=None):
+        """Eval set of later fits (Booster::fit's X_eval / y_eval / ts_eval); X=None clears."""
+        if X is None:
+            self.lib.fn("set_eval")(self.h, None, None, None, 0, 0)
+            return self
+        self._eval = (np.ascontiguousarray(X, np.float64), np.ascontiguousarray(y, np.float64),
+                      None if ts is None else np.ascontiguousarray(ts, np.int64))
+        Xe, ye, te = self._eval
+        if self.lib.fn("set_eval")(self.h, _ptr(Xe), _ptr(ye), _ptr(te), Xe.shape[0], Xe.shape[1]) != 0:
+            raise ValueError(self.lib.error())
+        return self
+
 
 class CpuLib:
     def __init__(self, path: Path, prefix: str):
@@ -129,7 +151,8 @@ class CpuLib:
             "tree_info": (_I, [_P, _I, _P, _P]), "tree_get": (_I, [_P, _I, _P, _P, _P, _P, _P]),
             "edges": (_I, [_P, _I, _P, _I]), "n_bins": (_I, [_P]), "init_prefix": (_I, [_P, _P, _I]),
             "init_score": (_D, [_P]), "last_error": (ctypes.c_char_p, []),
-            "evals_train": (_I, [_P, _P, _I]),
+            "evals_train": (_I, [_P, _P, _I]), "evals_valid": (_I, [_P, _P, _I]),
+            "best_iteration": (_I, [_P]), "set_eval": (_I, [_P, _P, _P, _P, _I, _I]),
         }
         for k, (res, args) in sig.items():
             f = getattr(self.lib, f"{prefix}_{k}")

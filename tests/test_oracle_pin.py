@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import checkers
+from eval_cases import EVAL_CASES, eval_target, split_by_date
 from tidegbm_amd import synth
 
 REF = checkers.reference()
@@ -120,6 +121,48 @@ def test_training_loss_per_tree(objective):
     assert np.array_equal(out[0], out[1]), (out[0], out[1])
     if objective == 0:
         assert np.all(np.diff(out[0]) < 0)      # the training MSE falls tree by tree
+
+
+@pytest.mark.parametrize("case", sorted(EVAL_CASES))
+def test_eval_set_and_early_stopping(case):
+    iid, objective, metric, esr, nan_y = EVAL_CASES[case]
+    p = synth.panel_small(21, n_dates=120, n_entities=40)
+    tr, va = split_by_date(p)
+    ye = eval_target(p, va, nan_y)
+    out = []
+    for lib in (checkers.oracle(), REF):
+        m = lib.model(n_estimators=15, max_depth=5, learning_rate=0.1, goss=0, objective=objective,
+                      eval_metric=metric, early_stopping_rounds=esr)
+        m.set_eval(p.X[va], ye, None if iid else p.ts[va])
+        m.fit(p.X[tr], p.y[tr], ts=None if iid else p.ts[tr], entity=None if iid else p.entity[tr],
+              rank_groups=p.ts[tr] if objective == 5 else None, iid=iid, horizon=1)
+        out.append(m)
+    o, r = out
+    vo, vr = o.evals_valid(), r.evals_valid()
+    assert len(vo) >= 1
+    assert np.array_equal(vo, vr, equal_nan=True), (vo, vr)
+    assert o.best_iteration() == r.best_iteration()
+    assert np.array_equal(o.evals_train(), r.evals_train())
+    _assert_identical(o, r, p.X)
+    if nan_y:
+        # a NaN metric never improves: every tree is scored and kept until the stall limit
+        assert np.isnan(vo).all() and len(o.trees()) == esr and o.best_iteration() == 15
+    elif esr:
+        # the first iteration improves; the validation score then uses that one tree, so the
+        # metric stays put and the fit stops after esr more trees, keeping one
+        assert len(vo) == esr + 1 and len(o.trees()) == 1 and o.best_iteration() == 1
+        assert np.all(vo == vo[0])
+
+
+def test_eval_in_the_past_raises():
+    p = synth.panel_small(22, n_dates=80, n_entities=20)
+    tr = p.ts < 50
+    va = p.ts < 20
+    for lib in (checkers.oracle(), REF):
+        m = lib.model(n_estimators=5, early_stopping_rounds=2, goss=0)
+        m.set_eval(p.X[va], p.y[va], p.ts[va])
+        with pytest.raises(ValueError, match="overlap"):
+            m.fit(p.X[tr], p.y[tr], ts=p.ts[tr], entity=p.entity[tr], horizon=1)
 
 
 def test_histogram_and_scan_functions():

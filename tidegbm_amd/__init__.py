@@ -67,6 +67,8 @@ _sig("tide_fit_times", _I, _P, _P, _P)
 _sig("tide_goss_stats", _I, _P, _P, _P, _P)
 _sig("tide_hist_variant", _I, _P)
 _sig("tide_evals_train", _I, _P, _P, _I)
+_sig("tide_evals_valid", _I, _P, _P, _I)
+_sig("tide_set_eval", _I, _P, _P, _P, _P, _I, _I)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
 _sig("tide_fit_batch", _I, _P, _P, _I, _I, _P, _P, _I, _P, _P, _P, _I, _P)
@@ -88,7 +90,7 @@ _INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_bucke
              "n_jobs", "iid_mode", "halo", "verbose", "horizon", "embargo", "goss", "cat_min_count",
              "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
              "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune",
-             "force_host_driver"}
+             "force_host_driver", "early_stopping_rounds"}
 _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight",
                 "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",
                 "max_delta_step", "torb_min_cover", "time_decay", "tree_decay"}
@@ -134,7 +136,7 @@ class Booster:
             if v is None:
                 continue
             kb = k.encode()
-            if k in ("device",):
+            if k in ("device", "eval_metric"):
                 rc = _lib.tide_set_string(self._h, kb, str(v).encode())
             elif k == "objective":
                 if isinstance(v, (int, np.integer)):
@@ -152,10 +154,23 @@ class Booster:
         return self
 
     def fit(self, X, y, *, timestamps=None, weight=None, entity=None, rank_groups=None,
-            iid: bool = False, horizon: int = -1):
+            iid: bool = False, horizon: int = -1, eval_set=None, eval_timestamp=None):
+        """Booster::fit.  eval_set=(X_eval, y_eval) (with eval_timestamp outside iid mode) scores
+        every tree on those rows and enables early_stopping_rounds, as the reference does."""
         X = np.ascontiguousarray(X, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)
         n, F = X.shape
+        if eval_set is not None:
+            Xe = np.ascontiguousarray(eval_set[0], dtype=np.float64)
+            ye = np.ascontiguousarray(eval_set[1], dtype=np.float64)
+            te = None if eval_timestamp is None else np.ascontiguousarray(eval_timestamp, dtype=np.int64)
+            if Xe.ndim != 2 or Xe.shape[0] != ye.shape[0] or (te is not None and te.shape[0] != ye.shape[0]):
+                raise TideError("eval_set rows, targets and timestamps must have the same length")
+            rc = _lib.tide_set_eval(self._h, _ptr(Xe), _ptr(ye), _ptr(te), Xe.shape[0], Xe.shape[1])
+        else:
+            rc = _lib.tide_set_eval(self._h, None, None, None, 0, 0)
+        if rc != 0:
+            raise TideError(_err())
         ts = None if timestamps is None else np.ascontiguousarray(timestamps, dtype=np.int64)
         w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
         ent = None if entity is None else np.ascontiguousarray(entity, dtype=np.int64)
@@ -250,12 +265,19 @@ class Booster:
         return a.value, b.value
 
     def evals_result(self) -> dict:
-        """The reference's evals_result(): training loss per tree (no eval sets on this path)."""
-        n = int(_lib.tide_evals_train(self._h, None, 0))
-        out = np.empty(n)
-        if n:
-            _lib.tide_evals_train(self._h, _ptr(out), n)
-        return {"train": out, "valid": np.empty(0)}
+        """The reference's evals_result(): training loss and validation metric per tree."""
+        res = {}
+        for key, fn in (("train", _lib.tide_evals_train), ("valid", _lib.tide_evals_valid)):
+            n = int(fn(self._h, None, 0))
+            out = np.empty(n)
+            if n:
+                fn(self._h, _ptr(out), n)
+            res[key] = out
+        return res
+
+    @property
+    def best_iteration(self) -> int:
+        return self.model_info()["best_iteration"]
 
     def hist_variant(self) -> int:
         """Histogram kernel variant the last fit ended with (tuned or forced; same trees)."""
@@ -438,7 +460,8 @@ class TideRegressor:
         self.iid_mode = iid_mode
         self.booster_: Booster | None = None
 
-    def fit(self, X, y, *, timestamp=None, entity=None, sample_weight=None):
+    def fit(self, X, y, *, timestamp=None, entity=None, sample_weight=None, eval_set=None,
+            eval_timestamp=None):
         X = np.asarray(X, dtype=np.float64)
         y = np.asarray(y, dtype=np.float64).reshape(-1)
         if not self.iid_mode and timestamp is None:
@@ -463,9 +486,19 @@ class TideRegressor:
                 X, y, ts, w, ent = X[order], y[order], ts[order], w[order], ent[order]
             if OBJECTIVES.get(str(p["objective"])) == 5:
                 rg = ts
+        if p.get("early_stopping_rounds") is None:  # api.py:23-24
+            p["early_stopping_rounds"] = 0
+        te = None
+        if eval_set is not None and not self.iid_mode:   # native.py:140-152
+            if eval_timestamp is None:
+                raise ValueError("eval_timestamp is required when iid_mode=False")
+            te = np.asarray(eval_timestamp).astype(np.int64)
         b = Booster(**p)
         b.fit(X, y, timestamps=ts, weight=None if np.all(w == 1.0) else w, entity=ent,
-              rank_groups=rg, iid=self.iid_mode, horizon=horizon)
+              rank_groups=rg, iid=self.iid_mode, horizon=horizon,
+              eval_set=None if eval_set is None else (np.asarray(eval_set[0], dtype=np.float64),
+                                                      np.asarray(eval_set[1], dtype=np.float64).reshape(-1)),
+              eval_timestamp=te)
         self.booster_ = b
         return self
 
@@ -473,6 +506,14 @@ class TideRegressor:
         if self.booster_ is None:
             raise ValueError("not fitted")
         return self.booster_.predict(X)
+
+    @property
+    def evals_result_(self) -> dict:
+        return self.booster_.evals_result()
+
+    @property
+    def best_iteration_(self) -> int:
+        return self.booster_.best_iteration
 
 
 class TideRanker(TideRegressor):

@@ -44,6 +44,28 @@ void tgbm_forest::reserve(int n) {
     cap = c;
 }
 
+// Leaf reached by one binned row from node `k` (tree.cpp apply_one): bin 0 is missing and goes
+// the node's missing direction; a pair node compares the hashed bin pair with its threshold.
+__device__ __forceinline__ int walk_tree(const FNode* __restrict__ nodes, const unsigned char* __restrict__ flags,
+                                         const unsigned char* __restrict__ row, int k, int nb) {
+    unsigned char fl = flags[k];
+    while (!(fl & 1)) {
+        const FNode nd = nodes[k];
+        const int b1 = row[nd.fx & 0xffff];
+        bool left;
+        if (((fl >> 1) & 3) == 1) {
+            const int b2 = row[nd.fx >> 16];
+            if (b1 == 0 || b2 == 0) left = (fl >> 3) & 1;
+            else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)nb) <= nd.thr;
+        } else {
+            left = b1 == 0 ? ((fl >> 3) & 1) : (b1 <= nd.thr);
+        }
+        k = left ? nd.left : nd.right;
+        fl = flags[k];
+    }
+    return k;
+}
+
 // One thread per row; the row's bins are re-read through L1/L2 at every visited node.
 __global__ __launch_bounds__(256) void k_forest(const FNode* __restrict__ nodes,
                                                 const unsigned char* __restrict__ flags,
@@ -56,25 +78,7 @@ __global__ __launch_bounds__(256) void k_forest(const FNode* __restrict__ nodes,
     if (i >= n) return;
     const unsigned char* row = bins + (size_t)i * fp;
     double s = init;
-    for (int t = 0; t < n_trees; ++t) {
-        int k = roots[t];
-        unsigned char fl = flags[k];
-        while (!(fl & 1)) {
-            const FNode nd = nodes[k];
-            const int b1 = row[nd.fx & 0xffff];
-            bool left;
-            if (((fl >> 1) & 3) == 1) {
-                const int b2 = row[nd.fx >> 16];
-                if (b1 == 0 || b2 == 0) left = (fl >> 3) & 1;
-                else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)nb) <= nd.thr;
-            } else {
-                left = b1 == 0 ? ((fl >> 3) & 1) : (b1 <= nd.thr);
-            }
-            k = left ? nd.left : nd.right;
-            fl = flags[k];
-        }
-        s += lr[t] * values[k];
-    }
+    for (int t = 0; t < n_trees; ++t) s += lr[t] * values[walk_tree(nodes, flags, row, roots[t], nb)];
     out[i] = s;
 }
 
@@ -281,6 +285,223 @@ int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int
             throw;
         }
         (void)hipFree(d);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Validation set scored tree by tree during a fit (reference Booster::fit with an eval set,
+// booster.cpp:459-478 and :707-732; scores as predict_multi_binned, booster.cpp:775-799).
+
+struct tgbm_evalset {
+    int F = 0, fp = 0, nb = 255, n = 0, n_groups = 0;
+    unsigned char* bins = nullptr;    // n x fp row-major uint8 bins
+    double* y = nullptr;
+    double* pred = nullptr;           // init + sum of lr * leaf value over the trees added so far
+    int* order = nullptr;             // rows of the listed dates, date by date
+    int* gstart = nullptr;            // [n_groups + 1] offsets into order
+    double* gvals = nullptr;          // per-date correlation
+    double* out = nullptr;            // metric
+    FNode* nodes = nullptr;           // the tree being added (at most 255 nodes)
+    unsigned char* flags = nullptr;
+    double* values = nullptr;
+    hipStream_t stream = nullptr;
+    ~tgbm_evalset() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : {(void*)bins, (void*)y, (void*)pred, (void*)order, (void*)gstart, (void*)gvals, (void*)out,
+                        (void*)nodes, (void*)flags, (void*)values})
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+__global__ __launch_bounds__(256) void k_eval_fill(double* __restrict__ pred, int n, double v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) pred[i] = v;
+}
+
+// pred += lr * value of the row's leaf: the running sum of predict_multi_binned, one tree at a time.
+__global__ __launch_bounds__(256) void k_eval_add(const FNode* __restrict__ nodes, const unsigned char* __restrict__ flags,
+                                                  const double* __restrict__ values, double lr,
+                                                  const unsigned char* __restrict__ bins, int fp, int n, int nb,
+                                                  double* __restrict__ pred) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pred[i] = pred[i] + lr * values[walk_tree(nodes, flags, bins + (size_t)i * fp, 0, nb)];
+}
+
+// Mean squared error with unit weights (Objective::loss, objective.cpp:323-328).  The reference
+// adds the rows' terms in row order; one wave keeps that order: each step forms 64 terms at once
+// and adds them lane by lane.
+__global__ __launch_bounds__(64) void k_eval_mse(const double* __restrict__ y, const double* __restrict__ pred, int n,
+                                                 double* __restrict__ out) {
+    const int lane = threadIdx.x;
+    double s = 0.0;
+    for (int base = 0; base < n; base += 64) {
+        const int i = base + lane;
+        double t = 0.0;
+        if (i < n) {
+            const double d = pred[i] - y[i];
+            t = 1.0 * d * d;
+        }
+        const int m = min(64, n - base);
+        for (int k = 0; k < m; ++k) s += __shfl(t, k, 64);
+    }
+    if (lane == 0) out[0] = n > 0 ? s / (double)n : 0.0;
+}
+
+// One thread per date: pearson_ix (booster.cpp:132-153) over the date's rows in the listed order.
+__global__ __launch_bounds__(64) void k_eval_ic(const double* __restrict__ y, const double* __restrict__ p,
+                                                const int* __restrict__ order, const int* __restrict__ gstart, int ng,
+                                                double* __restrict__ vals) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    const int* ix = order + gstart[g];
+    const int m = gstart[g + 1] - gstart[g];
+    double v = 0.0;
+    if (m >= 3) {
+        double my = 0.0, mp = 0.0;
+        for (int k = 0; k < m; ++k) {
+            my += y[ix[k]];
+            mp += p[ix[k]];
+        }
+        my /= m;
+        mp /= m;
+        double vy = 0.0, vp = 0.0, c = 0.0;
+        for (int k = 0; k < m; ++k) {
+            const double dy = y[ix[k]] - my;
+            const double dp = p[ix[k]] - mp;
+            vy += dy * dy;
+            vp += dp * dp;
+            c += dy * dp;
+        }
+        v = (vy < 1e-18 || vp < 1e-18) ? 0.0 : c / sqrt(vy * vp);
+    }
+    vals[g] = v;
+}
+
+// grouped_ic's mean over dates (booster.cpp:163-174), summed in date order.
+__global__ void k_eval_ic_mean(const double* __restrict__ vals, int ng, double* __restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0.0;
+    for (int g = 0; g < ng; ++g) s += vals[g];
+    out[0] = ng > 0 ? s / ng : 0.0;
+}
+
+extern "C" {
+
+int tgbm_eval_create(tgbm_evalset** out, int n_features, int n_bins, const double* edges, const int* n_edges,
+                     const double* X, const double* y, int n, const int* order, const int* group_start, int n_groups) {
+    *out = nullptr;
+    return fguard([&] {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw std::runtime_error("GPU device not available: no HIP device found");
+        if (n_features <= 0 || n <= 0 || n_groups < 0) throw std::runtime_error("bad eval set shape");
+        if (n_groups > 0) {
+            if (!order || !group_start) throw std::runtime_error("eval set dates need order and group_start");
+            if (group_start[0] < 0) throw std::runtime_error("eval date offsets out of range");
+            for (int g = 0; g < n_groups; ++g)
+                if (group_start[g + 1] < group_start[g]) throw std::runtime_error("eval date offsets must not decrease");
+            if (group_start[n_groups] > n) throw std::runtime_error("eval date offsets out of range");
+            for (int k = 0; k < group_start[n_groups]; ++k)
+                if (order[k] < 0 || order[k] >= n) throw std::runtime_error("eval row id out of range");
+        }
+        auto* e = new tgbm_evalset();
+        std::unique_ptr<tgbm_evalset> guard(e);
+        e->F = n_features;
+        e->fp = (n_features + 15) / 16 * 16;
+        e->nb = n_bins;
+        e->n = n;
+        e->n_groups = n_groups;
+        FHIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+        FHIP(hipMalloc(&e->bins, (size_t)n * e->fp + 16));
+        FHIP(hipMalloc(&e->y, sizeof(double) * n));
+        FHIP(hipMalloc(&e->pred, sizeof(double) * n));
+        FHIP(hipMalloc(&e->gvals, sizeof(double) * std::max(n_groups, 1)));
+        FHIP(hipMalloc(&e->out, sizeof(double)));
+        FHIP(hipMalloc(&e->nodes, sizeof(FNode) * 256));
+        FHIP(hipMalloc(&e->flags, 256));
+        FHIP(hipMalloc(&e->values, sizeof(double) * 256));
+        if (n_groups > 0) {
+            const int m = group_start[n_groups];
+            FHIP(hipMalloc(&e->order, sizeof(int) * std::max(m, 1)));
+            FHIP(hipMalloc(&e->gstart, sizeof(int) * (n_groups + 1)));
+            if (m) FHIP(hipMemcpy(e->order, order, sizeof(int) * m, hipMemcpyHostToDevice));
+            FHIP(hipMemcpy(e->gstart, group_start, sizeof(int) * (n_groups + 1), hipMemcpyHostToDevice));
+        }
+        FHIP(hipMemcpy(e->y, y, sizeof(double) * n, hipMemcpyHostToDevice));
+        // bin the rows once with the fit's edges (same kernel as the training matrix)
+        double *Xd = nullptr, *Ed = nullptr;
+        int* NEd = nullptr;
+        try {
+            FHIP(hipMalloc(&Xd, sizeof(double) * (size_t)n * n_features));
+            FHIP(hipMalloc(&Ed, sizeof(double) * (size_t)n_features * NBIN));
+            FHIP(hipMalloc(&NEd, sizeof(int) * n_features));
+            FHIP(hipMemcpy(Xd, X, sizeof(double) * (size_t)n * n_features, hipMemcpyHostToDevice));
+            FHIP(hipMemcpy(Ed, edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyHostToDevice));
+            FHIP(hipMemcpy(NEd, n_edges, sizeof(int) * n_features, hipMemcpyHostToDevice));
+            FHIP(hipMemsetAsync(e->bins, 0, (size_t)n * e->fp, e->stream));
+            launch_bin(e->stream, Xd, n, n_features, Ed, NEd, e->bins, e->fp, nullptr, n + 64);
+            FHIP(hipGetLastError());
+            FHIP(hipStreamSynchronize(e->stream));
+        } catch (...) {
+            for (void* p : {(void*)Xd, (void*)Ed, (void*)NEd})
+                if (p) (void)hipFree(p);
+            throw;
+        }
+        for (void* p : {(void*)Xd, (void*)Ed, (void*)NEd}) (void)hipFree(p);
+        *out = guard.release();
+    });
+}
+
+void tgbm_eval_destroy(tgbm_evalset* e) { delete e; }
+
+int tgbm_eval_reset(tgbm_evalset* e, double init_score) {
+    return fguard([&] {
+        hipLaunchKernelGGL(k_eval_fill, dim3((e->n + 255) / 256), dim3(256), 0, e->stream, e->pred, e->n, init_score);
+        FHIP(hipGetLastError());
+        FHIP(hipStreamSynchronize(e->stream));
+    });
+}
+
+int tgbm_eval_add_tree(tgbm_evalset* e, int n_nodes, const int* node_ints, const double* values, double lr) {
+    return fguard([&] {
+        const HostNodes h = build_nodes(1, &n_nodes, node_ints, e->F, 0);
+        FHIP(hipMemcpyAsync(e->nodes, h.nodes.data(), sizeof(FNode) * n_nodes, hipMemcpyHostToDevice, e->stream));
+        FHIP(hipMemcpyAsync(e->flags, h.flags.data(), n_nodes, hipMemcpyHostToDevice, e->stream));
+        FHIP(hipMemcpyAsync(e->values, values, sizeof(double) * n_nodes, hipMemcpyHostToDevice, e->stream));
+        hipLaunchKernelGGL(k_eval_add, dim3((e->n + 255) / 256), dim3(256), 0, e->stream, e->nodes, e->flags, e->values,
+                           lr, e->bins, e->fp, e->n, e->nb, e->pred);
+        FHIP(hipGetLastError());
+        FHIP(hipStreamSynchronize(e->stream));   // the host node tables above end with this call
+    });
+}
+
+int tgbm_eval_metric(tgbm_evalset* e, int metric, double* out) {
+    return fguard([&] {
+        if (metric == 0) {
+            hipLaunchKernelGGL(k_eval_mse, dim3(1), dim3(64), 0, e->stream, e->y, e->pred, e->n, e->out);
+        } else if (metric == 1) {
+            if (e->n_groups > 0)
+                hipLaunchKernelGGL(k_eval_ic, dim3((e->n_groups + 63) / 64), dim3(64), 0, e->stream, e->y, e->pred,
+                                   e->order, e->gstart, e->n_groups, e->gvals);
+            hipLaunchKernelGGL(k_eval_ic_mean, dim3(1), dim3(64), 0, e->stream, e->gvals, e->n_groups, e->out);
+        } else {
+            throw std::runtime_error("eval metric must be 0 (mean squared error) or 1 (per-date correlation)");
+        }
+        FHIP(hipGetLastError());
+        FHIP(hipMemcpyAsync(out, e->out, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        FHIP(hipStreamSynchronize(e->stream));
+    });
+}
+
+int tgbm_eval_predictions(tgbm_evalset* e, double* out, int cap) {
+    return fguard([&] {
+        const int m = std::min(std::max(cap, 0), e->n);
+        if (m) FHIP(hipMemcpyAsync(out, e->pred, sizeof(double) * m, hipMemcpyDeviceToHost, e->stream));
+        FHIP(hipStreamSynchronize(e->stream));
     });
 }
 

@@ -499,12 +499,101 @@ void Config::validate() const {  // reference config.hpp:78-97
     if (min_samples_leaf < 0) throw Error("min_samples_leaf must be >= 0");
 }
 
+struct Booster::EvalRun {
+    tgbm_evalset* set = nullptr;
+    bool ic = false;        // per-date correlation (higher is better) instead of squared error
+    bool scored = false;
+    int use = 0;            // trees summed into the device predictions
+    double last = 0.0;      // metric of those predictions
+    double best = 0.0;
+    int stall = 0;
+    ~EvalRun() {
+        if (set) tgbm_eval_destroy(set);
+    }
+};
+
 Booster::~Booster() { drop_forest(); }
+
+void Booster::set_eval(const double* X, const double* y, const long long* ts, int n, int nf) {
+    if (n < 0) throw Error("n_eval must be >= 0");
+    if (n > 0 && (!X || !y || nf <= 0)) throw Error("an eval set needs X, y and n_features > 0");
+    n_eval = n;
+    eval_features = n > 0 ? nf : 0;
+    eval_X.assign(X, X + (size_t)n * eval_features);
+    eval_y.assign(y, y + n);
+    if (n > 0 && ts) eval_ts.assign(ts, ts + n);
+    else eval_ts.clear();
+}
+
+// The validation step after tree m (booster.cpp:707-732).  The reference scores the eval rows
+// with predict_multi_binned, which sums the first min(trees, best_iteration) trees once an
+// iteration has improved (all trees before that): after the first finite score the scored
+// prefix stops growing and so does the metric.  The device predictions hold the running sum of
+// the scored prefix; a step whose prefix is unchanged reuses the metric it already computed.
+bool Booster::eval_step(int m) {
+    EvalRun& ev = *eval_run_;
+    const int use = std::min((int)trees.size(), best_iteration > 0 ? best_iteration : cfg.n_estimators);
+    if (!ev.scored || use != ev.use) {
+        int from = ev.use;
+        if (!ev.scored || use < ev.use) {
+            dev_check(tgbm_eval_reset(ev.set, init_score));
+            from = 0;
+        }
+        std::vector<int> ints;
+        std::vector<double> vals;
+        for (int t = from; t < use; ++t) {
+            ints.clear();
+            vals.clear();
+            tree_node_ints(trees[t], ints, vals);
+            dev_check(tgbm_eval_add_tree(ev.set, (int)trees[t].nodes.size(), ints.data(), vals.data(),
+                                         cfg.learning_rate));
+        }
+        dev_check(tgbm_eval_metric(ev.set, ev.ic ? 1 : 0, &ev.last));
+        ev.use = use;
+        ev.scored = true;
+    }
+    const double vl = ev.last;
+    evals_valid.push_back(vl);
+    const bool improved = ev.ic ? (vl > ev.best + 1e-12) : (vl + 1e-12 < ev.best);
+    if (improved) {
+        ev.best = vl;
+        ev.stall = 0;
+        best_iteration = m + 1;
+        return false;
+    }
+    ev.stall += 1;
+    if (cfg.early_stopping_rounds > 0 && ev.stall >= cfg.early_stopping_rounds) {
+        const int keep = best_iteration;
+        if (keep > 0 && keep < (int)trees.size()) trees.resize(keep);
+        return true;
+    }
+    return false;
+}
 
 void Booster::drop_forest() {
     if (forest_) tgbm_forest_destroy(forest_);
     forest_ = nullptr;
     forest_trees_ = -1;
+}
+
+void tree_node_ints(const Tree& tr, std::vector<int>& ints, std::vector<double>& vals) {
+    for (size_t k = 0; k < tr.nodes.size(); ++k) {
+        const Node& d = tr.nodes[k];
+        const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.threshold, d.left, d.right,
+                           d.missing_go_left, d.n_samples, d.depth};
+        ints.insert(ints.end(), v, v + 10);
+        vals.push_back(tr.values.empty() ? d.value : tr.values[k]);
+    }
+}
+
+void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne) {
+    const int F = (int)edges.size();
+    E.assign((size_t)F * 256, 0.0);
+    ne.assign(F, 0);
+    for (int f = 0; f < F; ++f) {
+        ne[f] = (int)edges[f].size();
+        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+    }
 }
 
 tgbm_forest* make_forest(const Booster& b) {
@@ -514,22 +603,12 @@ tgbm_forest* make_forest(const Booster& b) {
     std::vector<int> ints;
     std::vector<double> vals, lrs(use, b.cfg.learning_rate);
     for (int t = 0; t < use; ++t) {
-        const Tree& tr = b.trees[t];
-        tn[t] = (int)tr.nodes.size();
-        for (size_t k = 0; k < tr.nodes.size(); ++k) {
-            const Node& d = tr.nodes[k];
-            const int v[10] = {d.is_leaf, d.split_type, d.feature, d.feature2, d.threshold, d.left, d.right,
-                               d.missing_go_left, d.n_samples, d.depth};
-            ints.insert(ints.end(), v, v + 10);
-            vals.push_back(tr.values.empty() ? d.value : tr.values[k]);
-        }
+        tn[t] = (int)b.trees[t].nodes.size();
+        tree_node_ints(b.trees[t], ints, vals);
     }
-    std::vector<double> E((size_t)F * 256, 0.0);
-    std::vector<int> ne(F, 0);
-    for (int f = 0; f < F; ++f) {
-        ne[f] = (int)b.edges[f].size();
-        std::copy(b.edges[f].begin(), b.edges[f].end(), E.begin() + (size_t)f * 256);
-    }
+    std::vector<double> E;
+    std::vector<int> ne;
+    flat_edges(b.edges, E, ne);
     tgbm_forest* out = nullptr;
     dev_check(tgbm_forest_create(&out, F, b.n_bins, use, tn.data(), ints.data(), vals.data(), lrs.data(), b.init_score,
                                  E.data(), ne.data()));
@@ -641,6 +720,55 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     fit_edges(X, n, F, cfg.max_bin, cfg.bin_sample, (uint32_t)cfg.random_state, edges);
     n_bins = cfg.max_bin;
     for (auto& e : edges) n_bins = std::max(n_bins, (int)e.size() + 2);
+    // ---- eval set (booster.cpp:459-478): later than the training rows, binned with their edges
+    std::unique_ptr<EvalRun> ev;
+    evals_valid.clear();
+    if (n_eval > 0) {
+        if (eval_features != F) throw Error("eval set: n_features mismatch");
+        if (!cfg.iid_mode) {
+            if (eval_ts.empty()) throw Error("eval_timestamp is required when iid_mode=false");
+            long long tmax = ts[0];
+            for (int i = 1; i < n; ++i) tmax = std::max(tmax, ts[i]);
+            long long tmin_e = eval_ts[0];
+            for (int i = 1; i < n_eval; ++i) tmin_e = std::min(tmin_e, eval_ts[i]);
+            const long long embargo = cfg.embargo + std::max<long long>(cfg.horizon, 0);
+            if (tmin_e < tmax - embargo)
+                throw Error(
+                    "eval timestamps overlap the training future. "
+                    "Validation must be strictly later than train (minus embargo/horizon).");
+        }
+        ev.reset(new EvalRun());
+        ev->ic = cfg.eval_metric == "ic" || (cfg.eval_metric == "auto" && (cfg.objective == 5 || !cfg.iid_mode));
+        ev->best = ev->ic ? -1e300 : 1e300;
+        // dates of the correlation metric: grouped_ic orders the rows with std::sort by timestamp
+        // (not stable; the order inside a date feeds the sums) and keeps dates of >= 3 rows
+        std::vector<int> order, gstart{0};
+        if (ev->ic && n_eval >= 3) {
+            std::vector<int> o(n_eval);
+            std::iota(o.begin(), o.end(), 0);
+            if (!eval_ts.empty()) {
+                const long long* te = eval_ts.data();
+                std::sort(o.begin(), o.end(), [&](int a, int b) { return te[a] < te[b]; });
+                for (int i = 0; i < n_eval;) {
+                    int j = i + 1;
+                    while (j < n_eval && te[o[j]] == te[o[i]]) ++j;
+                    if (j - i >= 3) {
+                        order.insert(order.end(), o.begin() + i, o.begin() + j);
+                        gstart.push_back((int)order.size());
+                    }
+                    i = j;
+                }
+            } else {
+                order = o;
+                gstart.push_back(n_eval);
+            }
+        }
+        std::vector<double> E;
+        std::vector<int> ne;
+        flat_edges(edges, E, ne);
+        dev_check(tgbm_eval_create(&ev->set, F, n_bins, E.data(), ne.data(), eval_X.data(), eval_y.data(), n_eval,
+                                   order.empty() ? nullptr : order.data(), gstart.data(), (int)gstart.size() - 1));
+    }
     // ---- device session
     const bool use_goss = (cfg.goss > 0) || (cfg.goss < 0 && n >= 250000);
     const bool goss_active = use_goss && cfg.goss_top < 1.0 && n > 64;
@@ -656,7 +784,15 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     t_setup = now_s() - t0;
     const double t1 = now_s();
     std::mt19937 rng((uint32_t)cfg.random_state);
-    grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, tl.cuts, rng, goss_active, nullptr, false);
+    best_iteration = 0;   // booster.cpp:491
+    eval_run_ = ev.get();
+    try {
+        grow_trees(sg.s, n, cfg.n_estimators, !cfg.iid_mode, tl.cuts, rng, goss_active, nullptr, false);
+    } catch (...) {
+        eval_run_ = nullptr;
+        throw;
+    }
+    eval_run_ = nullptr;
     if (cfg.profile) {
         for (int k = 0; k < 8; ++k) {
             dev_check(tgbm_timer_read(sg.s, k, &prof_ms[k], &prof_n[k], &prof_bytes[k]));
@@ -667,7 +803,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
             if (m > 0) tgbm_timer_log(sg.s, k, prof_log_ms[k].data(), prof_log_rows[k].data(), m);
         }
     }
-    best_iteration = cfg.n_estimators;
+    if (best_iteration <= 0) best_iteration = cfg.n_estimators;   // booster.cpp:739
     fitted = true;
     t_boost = now_s() - t1;
 }
@@ -749,6 +885,7 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         }
         trees.push_back(std::move(t));
         if (!for_update) tree_end_times.push_back(now_s() - t_loop);  // tree_end synchronized
+        if (!for_update && eval_run_ && eval_step(m)) break;            // early stopping
     }
     hist_variant = gp.hist_mode;
 }

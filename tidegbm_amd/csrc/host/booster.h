@@ -64,6 +64,8 @@ struct Config {
     int hist_mode = 0;       // histogram kernel variant; 0: defaults, tuned per fit when autotune
     int autotune = 1;        // choose the variant online over the first trees (tuner.h)
     int force_host_driver = 0;
+    int early_stopping_rounds = 0;   // config.hpp early_stopping_rounds (0: never stop early)
+    std::string eval_metric = "auto";  // config.hpp:76: auto | loss | ic
 
     void validate() const;  // config.hpp:78-97 plus the device path's limits
 };
@@ -101,6 +103,10 @@ long long launch_rows_of(long long n);
 class Booster;
 // Device-resident forest of a fitted model's first best_iteration trees (lr per tree = cfg lr).
 tgbm_forest* make_forest(const Booster& b);
+// One tree in the export node layout (TGBM_NODE_INTS ints per node) with its leaf values.
+void tree_node_ints(const Tree& t, std::vector<int>& ints, std::vector<double>& vals);
+// Edges as the device takes them: edges[f*256 + k], k < n_edges[f].
+void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne);
 void dev_check(int rc);
 double now_s();
 
@@ -120,6 +126,11 @@ public:
     double t_setup = 0.0, t_boost = 0.0;
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree
     std::vector<double> evals_train;     // training loss after each tree (reference evals.train)
+    std::vector<double> evals_valid;     // validation metric after each tree (reference evals.valid)
+    // Eval set of later fits (Booster::fit's X_eval, y_eval, ts_eval, n_eval; booster.cpp:231-234)
+    std::vector<double> eval_X, eval_y;
+    std::vector<long long> eval_ts;
+    int n_eval = 0, eval_features = 0;
     int goss_host_trees = 0;             // GOSS trees of the last fit sampled on the host (equal |g|)
     int hist_variant = 0;                // histogram kernel variant the last fit ended with
     double goss_wait_s = 0.0, goss_sample_s = 0.0;  // GOSS: waiting for draws, the sample step
@@ -137,6 +148,8 @@ public:
     void update(const double* X, const double* y, int n, int nf, const double* w, int extra_trees);
     void predict_raw(const double* X, int n, int nf, double* out);
     void predict_binned_u16(const unsigned short* xb, int n, double* out);
+    // Keep an eval set for later fits (X: n x nf row-major; ts may be null); n = 0 clears it.
+    void set_eval(const double* X, const double* y, const long long* ts, int n, int nf);
 
     // Copy of the fitted model (configuration, edges, trees, initial scores); no device state.
     std::shared_ptr<Booster> clone_model() const;
@@ -145,7 +158,11 @@ public:
     std::string save_json() const;
     static Booster* load_binary(const std::string& blob);
 
+    struct EvalRun;   // device eval set and early-stopping state of one fit
+
 private:
+    EvalRun* eval_run_ = nullptr;   // set while a fit with an eval set grows its trees
+    bool eval_step(int m);          // after tree m: score, record, decide; true = stop early
     tgbm_forest* forest_ = nullptr;
     int forest_trees_ = -1;
     void ensure_forest();

@@ -67,6 +67,7 @@ int tide_set_int(TideHandle h, const char* key, int v) {
     else if (k == "hist_mode") c.hist_mode = v;
     else if (k == "autotune") c.autotune = v;
     else if (k == "force_host_driver") c.force_host_driver = v;
+    else if (k == "early_stopping_rounds") c.early_stopping_rounds = v;
     else {
         g_err = "unknown int key";
         return -1;
@@ -105,6 +106,10 @@ int tide_set_string(TideHandle h, const char* key, const char* value) {
     const std::string k(key), v(value ? value : "");
     if (k == "device") {
         c.device = v;
+        return 0;
+    }
+    if (k == "eval_metric") {  // config.hpp:76 (any other value scores the loss, booster.cpp:492-495)
+        c.eval_metric = v;
         return 0;
     }
     if (k == "objective") {  // names of bindings.cpp:108-119
@@ -292,6 +297,18 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
 
 int tide_hist_variant(TideHandle h) { return static_cast<Booster*>(h)->hist_variant; }
 
+int tide_set_eval(TideHandle h, const double* X_eval, const double* y_eval, const long long* ts_eval, int n_eval,
+                  int n_features) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] { b->set_eval(X_eval, y_eval, ts_eval, n_eval, n_features); });
+}
+
+int tide_evals_valid(TideHandle h, double* out, int cap) {
+    const auto& e = static_cast<Booster*>(h)->evals_valid;
+    for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
+    return (int)e.size();
+}
+
 int tide_evals_train(TideHandle h, double* out, int cap) {
     const auto& e = static_cast<Booster*>(h)->evals_train;
     for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];
@@ -312,6 +329,7 @@ int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features, con
     auto* p = static_cast<Booster*>(proto);
     return guard([&] {
         if (!p || !X || !task_rows || !y || !out) throw tide::Error("null argument");
+        if (p->n_eval > 0) throw tide::Error("eval sets are not supported by batched fits");
         std::vector<std::unique_ptr<Booster>> res;
         tide::fit_batch(p->cfg, X, n, n_features, timestamps, entity, n_tasks, task_rows, y, horizons, schedule, res);
         for (int t = 0; t < n_tasks; ++t) out[t] = res[t].release();
