@@ -29,6 +29,7 @@ struct EvalSet {
     std::vector<double> X, y;
     std::vector<long long> ts;
     int n = 0;
+    std::vector<int> is_cat;   // Booster::fit's is_cat of later fits (empty: none)
 };
 static std::map<void*, EvalSet> g_eval;
 static std::mutex g_eval_mu;
@@ -57,11 +58,18 @@ void tref_free(void* h) {
     delete static_cast<Booster*>(h);
 }
 
+// Categorical flags of later fits (Booster::fit's is_cat); nf = 0 clears.
+int tref_set_cat(void* h, const int* is_cat, int nf) {
+    std::lock_guard<std::mutex> lk(g_eval_mu);
+    g_eval[h].is_cat.assign(is_cat, is_cat + (nf > 0 ? nf : 0));
+    return 0;
+}
+
 // Eval set of later fits: X (n x nf, row-major), y, ts (may be NULL); n = 0 clears.
 int tref_set_eval(void* h, const double* X, const double* y, const long long* ts, int n, int nf) {
     std::lock_guard<std::mutex> lk(g_eval_mu);
     EvalSet& e = g_eval[h];
-    e.n = n > 0 ? n : 0;
+    e.n = n > 0 ? n : 0;  // keeps e.is_cat
     e.X.assign(X, X + (size_t)e.n * (nf > 0 ? nf : 0));
     e.y.assign(y, y + e.n);
     if (ts) e.ts.assign(ts, ts + e.n);
@@ -104,6 +112,8 @@ int tref_set(void* h, const char* key, double v) {
     else if (k == "col_major_hist") c.col_major_hist = iv != 0;
     else if (k == "ooc_bytes") c.ooc_bytes = static_cast<size_t>(v);
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
+    else if (k == "cat_min_count") c.cat_min_count = iv;
+    else if (k == "cat_smooth") c.cat_smooth = v;
     else if (k == "eval_metric") c.eval_metric = iv == 1 ? "loss" : (iv == 2 ? "ic" : "auto");
     else {
         g_ref_err = "unknown key " + k;
@@ -125,7 +135,8 @@ int tref_fit(void* h, const double* X, const double* y, int n, int nf, const lon
     return guarded([&] {
         b->cfg.iid_mode = iid != 0;
         if (!b->cfg.iid_mode) b->cfg.horizon = horizon;
-        b->fit(X, y, n, nf, reinterpret_cast<const i64*>(ts), w, nullptr, nullptr, nullptr,
+        b->fit(X, y, n, nf, reinterpret_cast<const i64*>(ts), w,
+               (int)ev.is_cat.size() == nf ? ev.is_cat.data() : nullptr, nullptr, nullptr,
                reinterpret_cast<const i64*>(rank_groups), ev.n > 0 ? ev.X.data() : nullptr,
                ev.n > 0 ? ev.y.data() : nullptr,
                ev.n > 0 && !ev.ts.empty() ? reinterpret_cast<const i64*>(ev.ts.data()) : nullptr, ev.n,
@@ -157,6 +168,21 @@ int tref_n_bins(void* h) { return static_cast<Booster*>(h)->n_bins; }
 int tref_n_time_buckets(void* h) { return static_cast<Booster*>(h)->n_time_buckets; }
 double tref_init_score(void* h) { return static_cast<Booster*>(h)->init_score; }
 int tref_best_iteration(void* h) { return static_cast<Booster*>(h)->best_iteration; }
+// Categorical split masks of tree t: 4 words per node.
+int tref_tree_cat(void* h, int t, unsigned long long* out) {
+    const Tree& tr = static_cast<Booster*>(h)->trees.at(t);
+    for (size_t k = 0; k < tr.nodes.size(); ++k)
+        for (int w = 0; w < 4; ++w) out[4 * k + w] = tr.nodes[k].cat_left[w];
+    return (int)tr.nodes.size();
+}
+
+// Category vocabulary of feature f of the last fit.
+int tref_cat_values(void* h, int f, double* out, int cap) {
+    const auto& v = static_cast<Booster*>(h)->binner.cat_values.at(f);
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
+
 int tref_evals_valid(void* h, double* out, int cap) {
     const auto& e = static_cast<Booster*>(h)->evals.valid;
     for (int i = 0; i < (int)e.size() && i < cap; ++i) out[i] = e[i];

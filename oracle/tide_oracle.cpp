@@ -54,6 +54,8 @@ struct Cfg {  // defaults: reference cpp/include/tidegbm/config.hpp:13-76, cpp/s
     int bin_sample = 262144;
     bool finer_torb = true;
     double torb_min_cover = 0.0;
+    int cat_min_count = 5;          // config.hpp / bindings.cpp cat_min_count
+    double cat_smooth = 1.0;        // config.hpp:70
     int early_stopping_rounds = 0;  // config.hpp early_stopping_rounds (api.py:23-24: None -> 0)
     int eval_metric = 0;            // config.hpp:76 "auto" | "loss" | "ic" as 0 | 1 | 2
 };
@@ -62,6 +64,7 @@ struct Node {  // tree.hpp:11-25
     int is_leaf = 1, split_type = 0, feature = -1, feature2 = -1, thr = -1, left = -1, right = -1,
         mgl = 1, n_samples = 0, depth = 0;
     double value = 0.0, gain = 0.0;
+    uint64_t cat_left[4] = {0, 0, 0, 0};   // categorical split: bins that go left
 };
 
 struct Tree {  // tree.hpp:32-41
@@ -145,16 +148,63 @@ static void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample
     }
 }
 
+// Category vocabulary of one categorical feature (binning.cpp:48-81): the distinct non-NaN
+// values seen at least cat_min_count times (the most frequent one if none is), at most
+// max_bin - 1 of them by frequency (std::partial_sort), in ascending order.
+static std::vector<double> categories_of(const double* X, int n, int F, int f, int max_bin, int cat_min_count) {
+    std::vector<double> raw;
+    raw.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        const double v = X[(size_t)i * F + f];
+        if (v == v) raw.push_back(v);
+    }
+    std::sort(raw.begin(), raw.end());
+    std::vector<std::pair<int, double>> freq;
+    for (size_t i = 0; i < raw.size();) {
+        size_t j = i + 1;
+        while (j < raw.size() && raw[j] == raw[i]) ++j;
+        freq.emplace_back((int)(j - i), raw[i]);
+        i = j;
+    }
+    const int min_c = std::max(1, cat_min_count);
+    std::vector<std::pair<int, double>> kept;
+    for (const auto& q : freq)
+        if (q.first >= min_c) kept.push_back(q);
+    if (kept.empty() && !freq.empty())
+        kept.push_back(*std::max_element(freq.begin(), freq.end(),
+                                         [](const auto& a, const auto& b) { return a.first < b.first; }));
+    const int cap = max_bin - 1;
+    if ((int)kept.size() > cap) {
+        std::partial_sort(kept.begin(), kept.begin() + cap, kept.end(),
+                          [](const auto& a, const auto& b) { return a.first > b.first; });
+        kept.resize(cap);
+    }
+    std::vector<double> u;
+    for (const auto& q : kept) u.push_back(q.second);
+    std::sort(u.begin(), u.end());
+    return u;
+}
+
 // Bin of one value: 0 for NaN, else 1 + number of edges <= v (binning.cpp:110-123).
 static int bin_of(const std::vector<double>& e, double x) {
     if (!(x == x)) return 0;
     return 1 + (int)(std::upper_bound(e.begin(), e.end(), x) - e.begin());
 }
 
-// n_bins = max(max_bin, max over features of n_edges+2) (binning.cpp:99,130-134).
-static int total_bins(const std::vector<std::vector<double>>& edges, int max_bin) {
+// Bin of a categorical value: 1 + its index in the vocabulary, 0 for NaN or an unseen value
+// (binning.cpp:107-112).
+static int cat_bin_of(const std::vector<double>& voc, double x) {
+    if (!(x == x)) return 0;
+    const auto it = std::lower_bound(voc.begin(), voc.end(), x);
+    return (it != voc.end() && *it == x) ? (int)(it - voc.begin()) + 1 : 0;
+}
+
+// n_bins = max(max_bin, max over features of n_edges+2, or categories+1) (binning.cpp:82,99,130-134).
+static int total_bins(const std::vector<std::vector<double>>& edges, const std::vector<int>& is_cat,
+                      const std::vector<std::vector<double>>& cats, int max_bin) {
     int m = 2;
-    for (auto& e : edges) m = std::max(m, (int)e.size() + 2);
+    for (size_t f = 0; f < edges.size(); ++f)
+        m = std::max(m, (!is_cat.empty() && is_cat[f]) ? (int)cats[f].size() + 1 : (int)edges[f].size() + 2);
     return std::max(m, max_bin);
 }
 
@@ -202,6 +252,7 @@ static void subtract_into(Hist& parent, const Hist& child) {  // histogram.cpp:1
 struct Cand {  // histogram.hpp:65-74
     double gain = -1e300;
     int feature = -1, feature2 = -1, thr = -1, type = 0, mgl = 1;
+    uint64_t cat_left[4] = {0, 0, 0, 0};
 };
 
 // Best numeric split of feature f with bin 0 = missing, thresholds b in [1, nb-2], empty bins
@@ -252,6 +303,56 @@ static Cand scan_feature(const Hist& H, int f, double lam, double gamma, int min
     return best;
 }
 
+// Best categorical split of feature f (histogram.cpp:195-244): the bins with rows, missing
+// (bin 0) included, ordered by G / (H + cat_smooth) with std::sort (not stable: equal ratios
+// take the library's order), then a left-to-right scan of that order with sequential FP64
+// sums; the left set is the bin mask up to the best position, strict '>' keeps the first.
+static Cand scan_categorical(const Hist& H, int f, double lam, double gamma, int min_leaf, double min_hess,
+                             double cat_smooth, double l1) {
+    Cand best;
+    const int nb = H.nb;
+    const double* G = &H.g[(size_t)f * nb];
+    const double* Hh = &H.h[(size_t)f * nb];
+    const i64* C = &H.c[(size_t)f * nb];
+    std::vector<int> order;
+    double tG = 0, tH = 0;
+    i64 tC = 0;
+    for (int b = 0; b < nb; ++b) {
+        tG += G[b];
+        tH += Hh[b];
+        tC += C[b];
+        if (C[b] > 0) order.push_back(b);
+    }
+    if (order.size() < 2) return best;
+    const double sm = std::max(cat_smooth, 0.0);
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return G[a] / (Hh[a] + sm) < G[b] / (Hh[b] + sm); });
+    const double parent = score_of(tG, tH, lam, l1);
+    double GL = 0, HL = 0;
+    i64 CL = 0;
+    uint64_t mask[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i + 1 < order.size(); ++i) {
+        const int b = order[i];
+        GL += G[b];
+        HL += Hh[b];
+        CL += C[b];
+        mask[b >> 6] |= uint64_t(1) << (b & 63);
+        const double GR = tG - GL, HR = tH - HL;
+        const i64 CR = tC - CL;
+        if (CL < min_leaf || CR < min_leaf) continue;
+        if (HL < min_hess || HR < min_hess) continue;
+        const double gain = 0.5 * (score_of(GL, HL, lam, l1) + score_of(GR, HR, lam, l1) - parent) - gamma;
+        if (gain > best.gain) {
+            best.gain = gain;
+            best.feature = f;
+            best.type = 2;
+            best.thr = -1;
+            best.mgl = (mask[0] & 1ull) ? 1 : 0;
+            for (int k = 0; k < 4; ++k) best.cat_left[k] = mask[k];
+        }
+    }
+    return best;
+}
+
 // pair hash (histogram.hpp:81-85)
 static inline int pair_hash(int b1, int b2, int nb) {
     return (int)(((uint32_t)b1 * 2654435761u + (uint32_t)b2 * 40503u) % (uint32_t)nb);
@@ -260,6 +361,11 @@ static inline int pair_hash(int b1, int b2, int nb) {
 // Node routing of one bin row (tree.cpp:15-41 / 68-91): Axis: bin 0 -> missing direction,
 // else bin <= threshold; Pair: either bin 0 -> missing direction, else hash <= threshold.
 static bool goes_left(const Node& nd, const uint16_t* row, int nb) {
+    if (nd.split_type == 2 || nd.split_type == 3) {  // cat_goes_left (tree.hpp:27-30)
+        const int b = row[nd.feature];
+        if (b >= 256) return nd.mgl != 0;
+        return (nd.cat_left[b >> 6] >> (b & 63)) & 1ull;
+    }
     if (nd.split_type == 1) {
         const int b1 = row[nd.feature], b2 = row[nd.feature2];
         if (b1 == 0 || b2 == 0) return nd.mgl != 0;
@@ -282,6 +388,7 @@ struct GrowCtx {
     int F, nb;
     const double *g, *h;
     const Cfg* cfg;
+    const int* is_cat;   // per feature, or null
 };
 
 static Cand leaf_candidate(const GrowCtx& cx, const std::vector<int>& rows, const Hist& hist,
@@ -289,9 +396,12 @@ static Cand leaf_candidate(const GrowCtx& cx, const std::vector<int>& rows, cons
     const Cfg& c = *cx.cfg;
     const int F = cx.F;
     std::vector<Cand> per_f(F);
-    for (int f = 0; f < F; ++f)
-        per_f[f] = scan_feature(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf,
-                                c.min_child_weight, c.l1);
+    for (int f = 0; f < F; ++f)  // tree.cpp:126-133
+        per_f[f] = (cx.is_cat && cx.is_cat[f])
+                       ? scan_categorical(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
+                                          c.cat_smooth, c.l1)
+                       : scan_feature(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
+                                      c.l1);
     Cand best;
     for (int f = 0; f < F; ++f) {  // ascending feature order, strict '>' (tree.cpp:160-166)
         if (per_f[f].feature < 0 || per_f[f].gain < c.min_split_gain) continue;
@@ -385,6 +495,7 @@ static Tree grow(const GrowCtx& cx, std::vector<int> sample) {
         p.split_type = best.type;
         p.mgl = best.mgl;
         p.gain = best.gain;
+        for (int k = 0; k < 4; ++k) p.cat_left[k] = best.cat_left[k];
         if (best.type == 1) {  // a PAIR split consumes budget and invalidates every cache
             ++pair_used;
             std::fill(cached.begin(), cached.end(), 0);
@@ -608,11 +719,19 @@ struct Model {
     // Caller-supplied bin edges replace Binner::fit (checker mode for fits that share a binned
     // matrix, e.g. walk-forward folds binned once); empty = fit edges as the reference does.
     std::vector<std::vector<double>> fixed_edges;
+    // Categorical features (Booster::fit's is_cat, booster.cpp:246-247) and their vocabularies.
+    std::vector<int> is_cat;
+    std::vector<std::vector<double>> cat_values;
+
+    bool cat(int f) const { return !is_cat.empty() && is_cat[f] != 0; }
 
     void bins_of(const double* X, int n, std::vector<uint16_t>& xb) const {
         xb.resize((size_t)n * F);
         for (int i = 0; i < n; ++i)
-            for (int f = 0; f < F; ++f) xb[(size_t)i * F + f] = (uint16_t)bin_of(edges[f], X[(size_t)i * F + f]);
+            for (int f = 0; f < F; ++f) {
+                const double x = X[(size_t)i * F + f];
+                xb[(size_t)i * F + f] = (uint16_t)(cat(f) ? cat_bin_of(cat_values[f], x) : bin_of(edges[f], x));
+            }
     }
 
     double init_of(const double* y, const double* w, int n) const {  // objective.cpp:80-100
@@ -711,7 +830,14 @@ struct Model {
         } else {
             fit_edges(X.data(), n, F, c.max_bin, c.bin_sample, (uint32_t)c.random_state, edges);
         }
-        nb = total_bins(edges, c.max_bin);
+        if (!is_cat.empty() && (int)is_cat.size() != F) throw std::runtime_error("is_cat: feature count mismatch");
+        cat_values.assign(F, {});
+        for (int f = 0; f < F; ++f)
+            if (cat(f)) {  // categorical features have no edges, only a vocabulary (binning.cpp:48-82)
+                edges[f].clear();
+                cat_values[f] = categories_of(X.data(), n, F, f, c.max_bin, c.cat_min_count);
+            }
+        nb = total_bins(edges, is_cat, cat_values, c.max_bin);
         std::vector<uint16_t> xb;
         bins_of(X.data(), n, xb);
         // initial scores (booster.cpp:399-457)
@@ -793,7 +919,7 @@ struct Model {
                 gu = gg.data();
                 hu = hh.data();
             }
-            GrowCtx cx{xb.data(), F, nb, gu, hu, &c};
+            GrowCtx cx{xb.data(), F, nb, gu, hu, &c, is_cat.empty() ? nullptr : is_cat.data()};
             (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample uses it
             Tree t = grow(cx, sample);
             const int nn = (int)t.nodes.size();
@@ -905,7 +1031,7 @@ struct Model {
         std::iota(all.begin(), all.end(), 0);
         for (int m = 0; m < extra; ++m) {
             grad(y, pred.data(), w.data(), nullptr, n, g.data(), h.data());
-            GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg};
+            GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg, is_cat.empty() ? nullptr : is_cat.data()};
             (void)rng();
             Tree t = grow(cx, all);
             const int nn = (int)t.nodes.size();
@@ -984,6 +1110,8 @@ int orc_set(void* hd, const char* key, double v) {
     else if (k == "finer_torb") c.finer_torb = iv != 0;
     else if (k == "torb_min_cover") c.torb_min_cover = v;
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
+    else if (k == "cat_min_count") c.cat_min_count = iv;
+    else if (k == "cat_smooth") c.cat_smooth = v;
     else if (k == "eval_metric") {
         if (iv < 0 || iv > 2) {
             g_orc_err = "eval_metric code must be 0 (auto), 1 (loss) or 2 (ic)";
@@ -1097,6 +1225,28 @@ int orc_tree_get(void* hd, int t, int* ints, double* dbl, double* values, int* c
     return 0;
 }
 
+// Categorical split masks of tree t: 4 words per node (bins that go left).
+int orc_tree_cat(void* hd, int t, unsigned long long* out) {
+    const orc::Tree& tr = static_cast<Model*>(hd)->trees.at(t);
+    for (size_t k = 0; k < tr.nodes.size(); ++k)
+        for (int w = 0; w < 4; ++w) out[4 * k + w] = tr.nodes[k].cat_left[w];
+    return (int)tr.nodes.size();
+}
+
+// Categorical flags of later fits (is_cat[f] != 0); n_features = 0 clears.
+int orc_set_cat(void* hd, const int* is_cat, int n_features) {
+    Model* m = static_cast<Model*>(hd);
+    m->is_cat.assign(is_cat, is_cat + std::max(n_features, 0));
+    return 0;
+}
+
+// Category vocabulary of feature f of the last fit (ascending); returns its size.
+int orc_cat_values(void* hd, int f, double* out, int cap) {
+    const auto& v = static_cast<Model*>(hd)->cat_values.at(f);
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
+
 // ---- per-function checkers over caller buffers ----
 
 // Edges per feature into edges[f*256 + k], counts into n_edges[f]; returns n_bins.
@@ -1108,7 +1258,7 @@ int orc_fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, un
         n_edges[f] = (int)e[f].size();
         std::copy(e[f].begin(), e[f].end(), edges + (size_t)f * 256);
     }
-    return orc::total_bins(e, max_bin);
+    return orc::total_bins(e, {}, {}, max_bin);
 }
 
 // Row-major u8 bins from edges laid out as in orc_fit_edges.

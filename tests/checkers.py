@@ -89,7 +89,9 @@ class CpuModel:
             cov = np.empty(nn.value, np.int32)
             pref = np.empty(nn.value * nb.value)
             self.lib.fn("tree_get")(self.h, t, _ptr(ints), _ptr(dbl), _ptr(vals), _ptr(cov), _ptr(pref))
-            out.append(dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value))
+            cat = np.zeros((nn.value, 4), np.uint64)
+            self.lib.fn("tree_cat")(self.h, t, _ptr(cat))
+            out.append(dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value, cat=cat))
         return out
 
     def edges(self, f):
@@ -125,6 +127,20 @@ class CpuModel:
     def best_iteration(self):
         return self.lib.fn("best_iteration")(self.h)
 
+    def set_cat(self, is_cat):
+        """Categorical flags of later fits (Booster::fit's is_cat); None clears."""
+        if is_cat is None:
+            self.lib.fn("set_cat")(self.h, None, 0)
+            return self
+        self._cat = np.ascontiguousarray(is_cat, np.int32)
+        self.lib.fn("set_cat")(self.h, _ptr(self._cat), self._cat.size)
+        return self
+
+    def cat_values(self, f):
+        buf = np.empty(256)
+        m = self.lib.fn("cat_values")(self.h, f, _ptr(buf), 256)
+        return buf[:m].copy()
+
     def set_eval(self, X, y, ts=None):
         """Eval set of later fits (Booster::fit's X_eval / y_eval / ts_eval); X=None clears."""
         if X is None:
@@ -153,6 +169,7 @@ class CpuLib:
             "init_score": (_D, [_P]), "last_error": (ctypes.c_char_p, []),
             "evals_train": (_I, [_P, _P, _I]), "evals_valid": (_I, [_P, _P, _I]),
             "best_iteration": (_I, [_P]), "set_eval": (_I, [_P, _P, _P, _P, _I, _I]),
+            "tree_cat": (_I, [_P, _I, _P]), "set_cat": (_I, [_P, _P, _I]), "cat_values": (_I, [_P, _I, _P, _I]),
         }
         for k, (res, args) in sig.items():
             f = getattr(self.lib, f"{prefix}_{k}")
@@ -211,9 +228,25 @@ def bins_of(X, edges):
     return xb
 
 
-def rows_reaching(ints, xb, node, n_bins=256):
-    """Boolean mask of the rows whose path through the tree passes `node`.  Pair splits use the
-    reference pair_bin hash (histogram.hpp:81-85) over n_bins."""
+def node_left(ints, k, xb, n_bins=256, cat=None):
+    """Rows that node k sends left (reference apply_one, tree.cpp:15-41): Axis: bin 0 -> missing
+    direction, else bin <= threshold; Pair: either bin 0 -> missing direction, else the
+    pair_bin hash (histogram.hpp:81-85) <= threshold; Cat / Regime: the bin's bit in the mask."""
+    f, f2, thr, ml = ints[k, 2], ints[k, 3], ints[k, 4], ints[k, 7]
+    b = xb[:, f]
+    if ints[k, 1] == 1:
+        b2 = xb[:, f2]
+        z = ((b.astype(np.uint64) * 2654435761 + b2.astype(np.uint64) * 40503) & 0xFFFFFFFF) % n_bins
+        return np.where((b == 0) | (b2 == 0), ml == 1, z.astype(np.int64) <= thr)
+    if ints[k, 1] in (2, 3):
+        words = np.asarray(cat[k], np.uint64)[np.minimum(b, 255) >> 6]
+        bit = (words >> (b & 63).astype(np.uint64)) & np.uint64(1)
+        return np.where(b >= 256, ml == 1, bit == 1)
+    return np.where(b == 0, ml == 1, b <= thr)
+
+
+def rows_reaching(ints, xb, node, n_bins=256, cat=None):
+    """Boolean mask of the rows whose path through the tree passes `node`."""
     reach = np.zeros(xb.shape[0], bool)
     stack = [(0, np.ones(xb.shape[0], bool))]
     while stack:
@@ -222,20 +255,13 @@ def rows_reaching(ints, xb, node, n_bins=256):
             return m
         if ints[k, 0]:
             continue
-        f, f2, thr, ml = ints[k, 2], ints[k, 3], ints[k, 4], ints[k, 7]
-        b = xb[:, f]
-        if ints[k, 1] == 1:
-            b2 = xb[:, f2]
-            z = ((b.astype(np.uint64) * 2654435761 + b2.astype(np.uint64) * 40503) & 0xFFFFFFFF) % n_bins
-            left = np.where((b == 0) | (b2 == 0), ml == 1, z.astype(np.int64) <= thr)
-        else:
-            left = np.where(b == 0, ml == 1, b <= thr)
+        left = node_left(ints, k, xb, n_bins, cat)
         stack.append((ints[k, 5], m & left))
         stack.append((ints[k, 6], m & ~left))
     return reach
 
 
-def leaves_of(ints, xb, n_bins=256):
+def leaves_of(ints, xb, n_bins=256, cat=None):
     """Leaf node of every row (reference apply_one, tree.cpp:15-55)."""
     leaf = np.zeros(xb.shape[0], np.int64)
     stack = [(0, np.ones(xb.shape[0], bool))]
@@ -246,35 +272,25 @@ def leaves_of(ints, xb, n_bins=256):
         if ints[k, 0]:
             leaf[m] = k
             continue
-        f, f2, thr, ml = ints[k, 2], ints[k, 3], ints[k, 4], ints[k, 7]
-        b = xb[:, f]
-        if ints[k, 1] == 1:
-            b2 = xb[:, f2]
-            z = ((b.astype(np.uint64) * 2654435761 + b2.astype(np.uint64) * 40503) & 0xFFFFFFFF) % n_bins
-            left = np.where((b == 0) | (b2 == 0), ml == 1, z.astype(np.int64) <= thr)
-        else:
-            left = np.where(b == 0, ml == 1, b <= thr)
+        left = node_left(ints, k, xb, n_bins, cat)
         stack.append((ints[k, 5], m & left))
         stack.append((ints[k, 6], m & ~left))
     return leaf
 
 
 def tie_split(x, y, xb, k, n_bins=256):
-    """Node k is an axis split in both trees and the two splits cut the rows reaching it into
-    the same two sets (equal, or with the sides swapped): their gains are equal in exact
-    arithmetic, and only rounding in the reference's FP64 sums can order them."""
+    """Node k is an axis or categorical split in both trees and the two splits cut the rows
+    reaching it into the same two sets (equal, or with the sides swapped): their gains are equal
+    in exact arithmetic, and only rounding in the reference's FP64 sums can order them."""
     xi, yi = x["ints"], y["ints"]
-    if xi[k, 0] or yi[k, 0] or xi[k, 1] != 0 or yi[k, 1] != 0:
+    xc, yc = x.get("cat"), y.get("cat")
+    if xi[k, 0] or yi[k, 0] or xi[k, 1] == 1 or yi[k, 1] == 1:
         return False
-    m = rows_reaching(xi, xb, k, n_bins)
-    if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins)):
+    m = rows_reaching(xi, xb, k, n_bins, xc)
+    if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins, yc)):
         return False
-
-    def goes_left(ints):
-        b = xb[:, ints[k, 2]]
-        return m & np.where(b == 0, ints[k, 7] == 1, b <= ints[k, 4])
-
-    lx, ly = goes_left(xi), goes_left(yi)
+    lx = m & node_left(xi, k, xb, n_bins, xc)
+    ly = m & node_left(yi, k, xb, n_bins, yc)
     return bool(np.array_equal(lx, ly) or np.array_equal(lx, m & ~ly))
 
 
@@ -310,8 +326,8 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
             for k in {int(k) for k, c in bad if c == 2}:
                 if xi[k, 0] or yi[k, 0] or xi[k, 1] != 0 or yi[k, 1] != 0:
                     continue
-                m = rows_reaching(xi, xb, k, n_bins)
-                if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins)):
+                m = rows_reaching(xi, xb, k, n_bins, x.get("cat"))
+                if not np.array_equal(m, rows_reaching(yi, xb, k, n_bins, y.get("cat"))):
                     continue
                 def goes_left(ints):
                     b = xb[:, ints[k, 2]]
@@ -323,22 +339,28 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
                 if k in tied and c in (2, 4, 7):
                     continue
                 if c == 7 and xi[k, 1] == 0 and xi[k, 0] == 0:
-                    m = rows_reaching(xi, xb, k, n_bins)
+                    m = rows_reaching(xi, xb, k, n_bins, x.get("cat"))
                     if not np.any(xb[m, xi[k, 2]] == 0):
                         continue
                 keep.append((k, c))
             bad = np.array(keep).reshape(-1, 2)
+        xc, yc = x.get("cat"), y.get("cat")
+        if xc is not None and yc is not None:
+            # categorical masks of the nodes that match so far
+            for k in np.flatnonzero(np.isin(x["ints"][:, 1], (2, 3)) & (x["ints"][:, 0] == 0)):
+                if k not in set(bad[:, 0].tolist()) and not np.array_equal(xc[k], yc[k]):
+                    bad = np.vstack([bad.reshape(-1, 2), [[k, 1]]])
         if bad.size and xb is not None:
             # a split decided by an exact-arithmetic tie: the same function on the training rows
             k = int(bad[:, 0].min())
             if tie_split(x, y, xb, k, n_bins):
-                lx, ly = leaves_of(x["ints"], xb, n_bins), leaves_of(y["ints"], xb, n_bins)
+                lx, ly = leaves_of(x["ints"], xb, n_bins, xc), leaves_of(y["ints"], xb, n_bins, yc)
                 if (np.allclose(x["values"][lx], y["values"][ly], rtol=rtol, atol=atol)
                         and np.array_equal(x["cover"][lx], y["cover"][ly])):
                     continue
         if bad.size:
-            k, c = bad[0]
-            return f"tree {t} node {k} {STRUCT_COLS[c]}: {x['ints'][k, c]} != {y['ints'][k, c]}"
+            k, c = bad[np.argmin(bad[:, 0])]
+            return f"tree {t} node {k} {STRUCT_COLS[c]}: {x['ints'][k, c]} != {y['ints'][k, c]} (or its category mask)"
         if not np.array_equal(x["cover"], y["cover"]):
             return f"tree {t}: cover differs"
         if check_values:

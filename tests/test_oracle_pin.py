@@ -165,6 +165,59 @@ def test_eval_in_the_past_raises():
             m.fit(p.X[tr], p.y[tr], ts=p.ts[tr], entity=p.entity[tr], horizon=1)
 
 
+def cat_panel(seed, n_dates=150, n_entities=40):
+    """Panel with three categorical columns: 11 codes (some rare, NaN sprinkled), 40 codes with
+    per-code effects on y, and 300 codes (more than max_bin - 1, so the vocabulary keeps the most
+    frequent)."""
+    p = synth.panel_small(seed, n_dates=n_dates, n_entities=n_entities)
+    rng = np.random.default_rng(seed + 100)
+    X = p.X.copy()
+    n = p.n
+    X[:, 3] = np.clip(np.round(X[:, 3] * 2.0), -5, 5)
+    X[rng.random(n) < 0.03, 3] = np.nan
+    codes = rng.integers(0, 40, size=n)
+    effect = rng.normal(size=40)
+    X[:, 7] = codes.astype(np.float64)
+    X[:, 11] = rng.integers(0, 300, size=n).astype(np.float64)
+    y = p.y + 0.8 * effect[codes] + 0.3 * np.nan_to_num(X[:, 3])
+    is_cat = np.zeros(p.n_features, np.int32)
+    is_cat[[3, 7, 11]] = 1
+    return synth.Panel(X, y, p.ts, p.entity), is_cat
+
+
+CAT_CASES = {
+    "time_pair_default": dict(iid=False, objective=0, params=dict(n_estimators=12, max_depth=6)),
+    "iid_device_like": dict(iid=True, objective=0, params=dict(n_estimators=10, max_depth=5, pair_splits=0)),
+    "pairwise": dict(iid=False, objective=5, params=dict(n_estimators=8, max_depth=5, pair_splits=0)),
+    "min_count_smooth": dict(iid=False, objective=0,
+                             params=dict(n_estimators=8, max_depth=5, cat_min_count=40, cat_smooth=5.0)),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CAT_CASES))
+def test_categorical_features(case):
+    cc = CAT_CASES[case]
+    p, is_cat = cat_panel(31)
+    out = []
+    for lib in (checkers.oracle(), REF):
+        m = lib.model(learning_rate=0.1, goss=0, objective=cc["objective"], **cc["params"])
+        m.set_cat(is_cat)
+        m.fit(p.X, p.y, ts=None if cc["iid"] else p.ts, entity=None if cc["iid"] else p.entity,
+              rank_groups=p.ts if cc["objective"] == 5 else None, iid=cc["iid"], horizon=1)
+        out.append(m)
+    o, r = out
+    for f in (3, 7, 11):
+        assert np.array_equal(o.cat_values(f), r.cat_values(f))
+    # 300 codes of about 20 rows each: the 254 most frequent are kept; with cat_min_count 40
+    # none qualifies and the single most frequent code is kept (binning.cpp:66-74)
+    assert o.cat_values(11).size == (1 if cc["params"].get("cat_min_count") else 254)
+    _assert_identical(o, r, p.X)
+    for x, y in zip(o.trees(), r.trees()):
+        assert np.array_equal(x["cat"], y["cat"])
+    n_cat = sum(int(np.isin(t["ints"][:, 1], (2, 3)).sum()) for t in o.trees())
+    assert n_cat >= 5, "case should exercise categorical splits"
+
+
 def test_histogram_and_scan_functions():
     # per-function pins: the restated histogram / split scan / gradients vs the reference's own
     rng = np.random.default_rng(3)
