@@ -35,6 +35,7 @@ typedef struct tgbm_grow_params {
     double min_child_weight;
     double l2, l1, max_delta_step, min_split_gain;
     double pair_l2;         /* Config::pair_l2, used by PAIR scans */
+    double cat_smooth;      /* Config::cat_smooth (config.hpp:70), categorical ratio ranking */
     int hist_mode;          /* histogram kernel variant, 0 = defaults (all variants give the same
                                sums): bits 0-7 child tile (>= 16: 16 << (v-16)/8 features and
                                256 << (v-16)%8 minimum rows per block; 2: the row kernel also for
@@ -46,7 +47,8 @@ typedef struct tgbm_grow_params {
 /* One split decision made by the host (PAIR-capable driver). */
 typedef struct tgbm_decision {
     int leaf;               /* node id to split, -1 = stop growing */
-    int type;               /* 0 axis, 1 pair */
+    int type;               /* 0 axis, 1 pair, 2 categorical (bins of the leaf's best split of
+                               `feature`, as its scan found them, go left) */
     int feature, feature2, threshold, missing_go_left;
     double gain;
 } tgbm_decision;
@@ -69,8 +71,20 @@ void tgbm_session_destroy(tgbm_session* s);
 
 /* Upload raw features (host row-major f64 n_rows x n_features) and bin them on the device
  * with per-feature right edges edges[f*256 + k], k < n_edges[f] (reference
- * Binner::transform, binning.cpp:104-128): NaN -> 0, else 1 + #edges <= x. */
+ * Binner::transform, binning.cpp:104-128): NaN -> 0, else 1 + #edges <= x.  A categorical
+ * feature passes its sorted category values as edges with TGBM_CAT_EDGES set in n_edges[f]:
+ * a value bins to 1 + its index, an unlisted value or NaN to 0.  Every call that takes edges
+ * (forests, eval sets) reads them this way. */
+#define TGBM_CAT_EDGES 0x10000
 int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges);
+/* Mark categorical features (is_cat[f] != 0; reference Booster::fit's is_cat, booster.cpp:246):
+ * their split search becomes best_categorical (histogram.cpp:195-244) and tgbm_decision type 2
+ * routes by the bin mask the leaf's scan found.  Once per session, before the first tree;
+ * growth must then be host-driven. */
+int tgbm_session_set_cat(tgbm_session* s, const int* is_cat);
+/* Bin masks (4 words per node, bins that go left) of the task's last exported tree, for the
+ * nodes whose split_type is 2. */
+int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out);
 /* Restrict every later gradient, tree_begin/grow/end launch to tasks [first, first + count)
  * (all tasks after create).  Batched fits keep every task active; the one-fit-at-a-time
  * schedule activates one task at a time over the same resident bins.  The results of a task
@@ -154,18 +168,21 @@ int tgbm_timer_log(tgbm_session* s, int which, double* ms, double* rows, int cap
 /* ---------------- predict / serving: resident ensembles ---------------- */
 typedef struct tgbm_forest tgbm_forest;
 
-/* Flatten trees (node arrays as in the export layout) into a device-resident forest. */
+/* Flatten trees (node arrays as in the export layout) into a device-resident forest.
+ * cat_masks: 4 words per node (read for categorical nodes, split_type 2 or 3) or NULL.  The
+ * batched predict kernel (tgbm_predict_batch, tgbm_task_predict_forest) refuses forests with
+ * categorical nodes; tgbm_forest_predict routes them. */
 int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_trees,
                        const int* tree_nodes, const int* node_ints, const double* values,
                        const double* lr, double init_score, const double* edges,
-                       const int* n_edges);
+                       const int* n_edges, const unsigned long long* cat_masks);
 void tgbm_forest_destroy(tgbm_forest* f);
 /* New forest = the first keep_trees trees of `base` (copied on the device) followed by n_trees
  * new trees (same node layout as tgbm_forest_create); features, bins, edges and init score are
  * base's.  Lets a refit publish a version without re-uploading its parent's trees. */
 int tgbm_forest_append(tgbm_forest** out, const tgbm_forest* base, int keep_trees, int n_trees,
                        const int* tree_nodes, const int* node_ints, const double* values,
-                       const double* lr);
+                       const double* lr, const unsigned long long* cat_masks);
 /* Predict raw scores for host rows X (row-major f64) or for device-resident uint8 bins. */
 int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out);
 int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out);
@@ -211,7 +228,7 @@ void tgbm_eval_destroy(tgbm_evalset* e);
 int tgbm_eval_reset(tgbm_evalset* e, double init_score);
 /* Add one tree (export node layout, TGBM_NODE_INTS ints per node; values per node) times lr. */
 int tgbm_eval_add_tree(tgbm_evalset* e, int n_nodes, const int* node_ints, const double* values,
-                       double lr);
+                       double lr, const unsigned long long* cat_masks);
 /* metric 0: mean squared error, unit weights, rows summed in order (objective.cpp:323-328);
  * 1: mean over the listed dates of the Pearson correlation of y and prediction (0 without dates). */
 int tgbm_eval_metric(tgbm_evalset* e, int metric, double* out);

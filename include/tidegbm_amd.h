@@ -87,6 +87,12 @@ int tide_hist_variant(TideHandle h);
 /* Training loss after each tree of the last fit, the reference's evals_result()["train"]
  * (booster.cpp:704-706, bindings.cpp:318-321).  Returns the number of trees (at most cap copied). */
 int tide_evals_train(TideHandle h, double* out, int cap);
+/* Bin masks of tree `tree`'s categorical nodes (split_type 2): 4 words per node, bit b of word
+ * b >> 6 set when bin b goes left (reference Node::cat_left, tree.hpp:24-30); zeros elsewhere. */
+int tide_tree_cat(TideHandle h, int tree, unsigned long long* out);
+/* Sorted category values of a categorical feature (reference Binner::cat_values, binning.cpp:48-82);
+ * returns their number (0 for a numeric feature), at most cap copied. */
+int tide_get_cat_values(TideHandle h, int feature, double* out, int cap);
 /* Eval set of the following fits: the X_eval, y_eval, ts_eval, n_eval arguments of the reference's
  * Booster::fit (booster.hpp:45-48, booster.cpp:231-234), which its C API cannot pass.  X_eval is
  * n_eval x n_features row-major; ts_eval may be NULL in iid mode; n_eval = 0 clears the set.

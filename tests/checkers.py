@@ -218,10 +218,18 @@ STRUCT_COLS = ["is_leaf", "split_type", "feature", "feature2", "threshold", "lef
                "missing_go_left", "n_samples", "depth"]
 
 
-def bins_of(X, edges):
-    """uint8-range bins of raw rows: NaN -> 0, else 1 + #edges <= x (reference Binner::transform)."""
+def bins_of(X, edges, cat_values=None):
+    """uint8-range bins of raw rows (reference Binner::transform, binning.cpp:104-128): NaN -> 0,
+    else 1 + #edges <= x; a feature with category values (cat_values[f] not None) bins to
+    1 + the value's index, 0 when the value is not listed."""
     xb = np.zeros(X.shape, np.int64)
     for f, e in enumerate(edges):
+        if cat_values is not None and cat_values[f] is not None:
+            v = np.asarray(cat_values[f])
+            i = np.searchsorted(v, X[:, f], side="left")
+            hit = (i < v.size) & (v[np.minimum(i, max(v.size - 1, 0))] == X[:, f]) if v.size else np.zeros(X.shape[0], bool)
+            xb[:, f] = np.where(hit, i + 1, 0)
+            continue
         b = 1 + np.searchsorted(e, X[:, f], side="right")
         b[np.isnan(X[:, f])] = 0
         xb[:, f] = b

@@ -38,7 +38,7 @@ def _lib():
     L.tgbm_eval_destroy.argtypes = [_P]
     L.tgbm_eval_destroy.restype = None
     L.tgbm_eval_reset.argtypes = [_P, _D]
-    L.tgbm_eval_add_tree.argtypes = [_P, _I, _P, _P, _D]
+    L.tgbm_eval_add_tree.argtypes = [_P, _I, _P, _P, _D, _P]
     L.tgbm_eval_metric.argtypes = [_P, _I, _P]
     L.tgbm_eval_predictions.argtypes = [_P, _P, _I]
     L.tgbm_last_error.restype = ctypes.c_char_p
@@ -127,7 +127,7 @@ def test_eval_kernels_match_reference_arithmetic():
         assert L.tgbm_eval_reset(h, init) == 0
         expect = [init] * n
         for ints, vals in trees:
-            assert L.tgbm_eval_add_tree(h, ints.shape[0], ints.ctypes.data, vals.ctypes.data, lr) == 0
+            assert L.tgbm_eval_add_tree(h, ints.shape[0], ints.ctypes.data, vals.ctypes.data, lr, None) == 0
             for i in range(n):
                 expect[i] = expect[i] + lr * float(vals[_walk(ints, xb[i], nb)])
         got = np.empty(n)

@@ -68,6 +68,8 @@ _sig("tide_goss_stats", _I, _P, _P, _P, _P)
 _sig("tide_hist_variant", _I, _P)
 _sig("tide_evals_train", _I, _P, _P, _I)
 _sig("tide_evals_valid", _I, _P, _P, _I)
+_sig("tide_tree_cat", _I, _P, _I, _P)
+_sig("tide_get_cat_values", _I, _P, _I, _P, _I)
 _sig("tide_set_eval", _I, _P, _P, _P, _P, _I, _I)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
 _sig("tide_tree_times", _I, _P, _P, _I)
@@ -91,7 +93,7 @@ _INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_bucke
              "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
              "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune",
              "force_host_driver", "early_stopping_rounds"}
-_DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight",
+_DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight", "cat_smooth",
                 "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",
                 "max_delta_step", "torb_min_cover", "time_decay", "tree_decay"}
 
@@ -154,9 +156,10 @@ class Booster:
         return self
 
     def fit(self, X, y, *, timestamps=None, weight=None, entity=None, rank_groups=None,
-            iid: bool = False, horizon: int = -1, eval_set=None, eval_timestamp=None):
+            iid: bool = False, horizon: int = -1, eval_set=None, eval_timestamp=None, is_cat=None):
         """Booster::fit.  eval_set=(X_eval, y_eval) (with eval_timestamp outside iid mode) scores
-        every tree on those rows and enables early_stopping_rounds, as the reference does."""
+        every tree on those rows and enables early_stopping_rounds, as the reference does;
+        is_cat flags categorical features (one entry per feature)."""
         X = np.ascontiguousarray(X, dtype=np.float64)
         y = np.ascontiguousarray(y, dtype=np.float64)
         n, F = X.shape
@@ -175,7 +178,10 @@ class Booster:
         w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
         ent = None if entity is None else np.ascontiguousarray(entity, dtype=np.int64)
         rg = None if rank_groups is None else np.ascontiguousarray(rank_groups, dtype=np.int64)
-        rc = _lib.tide_fit_ex(self._h, _ptr(X), _ptr(y), n, F, _ptr(ts), _ptr(w), None, _ptr(ent),
+        cat = None if is_cat is None else np.ascontiguousarray(np.asarray(is_cat) != 0, dtype=np.int32)
+        if cat is not None and cat.shape != (F,):
+            raise TideError("is_cat needs one entry per feature")
+        rc = _lib.tide_fit_ex(self._h, _ptr(X), _ptr(y), n, F, _ptr(ts), _ptr(w), _ptr(cat), _ptr(ent),
                               _ptr(rg), 1 if iid else 0, int(horizon))
         if rc != 0:
             raise TideError(_err())
@@ -225,8 +231,14 @@ class Booster:
                     best_iteration=bi.value, init_score=init.value)
 
     def trees(self):
-        return [tree_arrays(self._h, _lib.tide_tree_info, _lib.tide_tree_get, t)
+        return [tree_arrays(self._h, _lib.tide_tree_info, _lib.tide_tree_get, t, _lib.tide_tree_cat)
                 for t in range(_lib.tide_num_trees(self._h))]
+
+    def cat_values(self, f):
+        """Sorted category values of categorical feature f (empty for a numeric one)."""
+        buf = np.empty(256)
+        m = _lib.tide_get_cat_values(self._h, f, _ptr(buf), 256)
+        return buf[:m].copy()
 
     def edges(self, f):
         buf = np.empty(256)
@@ -407,8 +419,9 @@ class Server:
                     refit_publish_s=float(b[4]))
 
 
-def tree_arrays(h, info_fn, get_fn, t):
-    """Trees as dicts of numpy arrays: ints (nn,10), dbl (nn,2), values, cover, prefix (nb*nn)."""
+def tree_arrays(h, info_fn, get_fn, t, cat_fn=None):
+    """Trees as dicts of numpy arrays: ints (nn,10), dbl (nn,2), values, cover, prefix (nb*nn),
+    and with cat_fn the categorical bin masks cat (nn,4) uint64."""
     nn, nb = _I(), _I()
     info_fn(h, t, ctypes.byref(nn), ctypes.byref(nb))
     ints = np.empty((nn.value, 10), np.int32)
@@ -417,7 +430,12 @@ def tree_arrays(h, info_fn, get_fn, t):
     cov = np.empty(nn.value, np.int32)
     pref = np.empty(nn.value * nb.value)
     get_fn(h, t, _ptr(ints), _ptr(dbl), _ptr(vals), _ptr(cov), _ptr(pref))
-    return dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value)
+    out = dict(ints=ints, dbl=dbl, values=vals, cover=cov, prefix=pref, n_buckets=nb.value)
+    if cat_fn is not None:
+        cat = np.zeros((nn.value, 4), np.uint64)
+        cat_fn(h, t, _ptr(cat))
+        out["cat"] = cat
+    return out
 
 
 # ---------------------------------------------------------------- estimator mirror
@@ -461,7 +479,8 @@ class TideRegressor:
         self.booster_: Booster | None = None
 
     def fit(self, X, y, *, timestamp=None, entity=None, sample_weight=None, eval_set=None,
-            eval_timestamp=None):
+            eval_timestamp=None, categorical_feature=None):
+        """categorical_feature: indices of categorical columns (reference data.py:66-78)."""
         X = np.asarray(X, dtype=np.float64)
         y = np.asarray(y, dtype=np.float64).reshape(-1)
         if not self.iid_mode and timestamp is None:
@@ -493,8 +512,15 @@ class TideRegressor:
             if eval_timestamp is None:
                 raise ValueError("eval_timestamp is required when iid_mode=False")
             te = np.asarray(eval_timestamp).astype(np.int64)
+        is_cat = None
+        if categorical_feature:
+            is_cat = np.zeros(X.shape[1], np.int32)
+            for c in categorical_feature:
+                if not isinstance(c, (int, np.integer)) or not 0 <= int(c) < X.shape[1]:
+                    raise ValueError(f"categorical feature {c!r} not in X")
+                is_cat[int(c)] = 1
         b = Booster(**p)
-        b.fit(X, y, timestamps=ts, weight=None if np.all(w == 1.0) else w, entity=ent,
+        b.fit(X, y, timestamps=ts, weight=None if np.all(w == 1.0) else w, entity=ent, is_cat=is_cat,
               rank_groups=rg, iid=self.iid_mode, horizon=horizon,
               eval_set=None if eval_set is None else (np.asarray(eval_set[0], dtype=np.float64),
                                                       np.asarray(eval_set[1], dtype=np.float64).reshape(-1)),

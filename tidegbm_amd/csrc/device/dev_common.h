@@ -8,6 +8,9 @@ namespace tgbm {
 // ---------------------------------------------------------------- sizes
 constexpr int NBIN = 256;          // histogram stride per feature (uint8 bins: 0 = missing)
 constexpr int MAX_NODES = 256;     // per tree (max_leaves <= 128)
+// n_edges[f] with this bit set: feature f is categorical and its "edges" are its sorted category
+// values; a value bins to 1 + its index, an unlisted value or NaN to 0 (binning.cpp:107-112).
+constexpr int CAT_EDGE_FLAG = 1 << 16;
 constexpr int MAX_FEATURES = 1024;
 constexpr int MAX_BUCKETS = 256;   // TORB buckets (+1 sweep segment for rows past the last cut)
 constexpr int MAX_RED_BLOCKS = 2048;
@@ -168,6 +171,9 @@ struct Task {
     double* values;        // [MAX_NODES]
     double* prefix;        // [n_buckets * MAX_NODES]
     int* cover;            // [MAX_NODES]
+    // ---- categorical splits (null when the session has no categorical feature)
+    unsigned long long* fmask;  // [MAX_NODES][n_features][4] bins a feature's best split sends left
+    unsigned long long* nmask;  // [MAX_NODES][4] the split node's mask (exported with the tree)
 };
 
 struct Shared {            // the session-wide matrix
@@ -175,6 +181,7 @@ struct Shared {            // the session-wide matrix
     int part_cap;          // per-task capacity of per-block partial histograms
     const unsigned char* bins_row;   // [n_rows][fp]
     const unsigned char* bins_col;   // [n_features][n_pad]
+    const unsigned char* is_cat;     // [n_features] categorical flags, or null
 };
 
 // A run of rows inside one TORB sweep segment (rows are time sorted, segment s = rows
@@ -186,7 +193,7 @@ struct TorbChunk {
 // Grow parameters copied to kernels by value.
 struct GrowP {
     int max_depth, max_leaves, min_leaf;
-    double min_hess, l2, l1, mds, gamma, pair_l2;
+    double min_hess, l2, l1, mds, gamma, pair_l2, cat_smooth;
 };
 
 // ---------------------------------------------------------------- leaf formulas

@@ -10,8 +10,9 @@
 
 namespace tgbm {
 
-// x = feature | feature2 << 16, y = threshold, z = left, w = right (absolute node ids); flags in
-// a separate byte array: bit0 leaf, bits1-2 split type, bit3 missing_go_left.
+// x = feature | feature2 << 16, y = threshold (categorical node: index of its 4-word bin mask in
+// the forest's mask table), z = left, w = right (absolute node ids); flags in a separate byte
+// array: bit0 leaf, bits1-2 split type (0 axis, 1 pair, 2/3 categorical), bit3 missing_go_left.
 struct FNode {
     int fx, thr, left, right;
 };
@@ -41,6 +42,9 @@ struct tgbm_forest {
     double* lr = nullptr;
     double* edges = nullptr;
     int* n_edges = nullptr;
+    unsigned long long* cmask = nullptr;  // [n_cat][4] bin masks of categorical nodes
+    int n_cat = 0;
+    std::vector<int> cat_off_h;           // [n_trees + 1] first mask of each tree
     hipStream_t stream = nullptr;
     // staging for the single-call predict entry points
     double* Xd = nullptr;
@@ -52,7 +56,7 @@ struct tgbm_forest {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {(void*)nodes, (void*)flags, (void*)snodes, (void*)tree_off, (void*)values, (void*)roots,
                         (void*)lr, (void*)edges, (void*)n_edges, (void*)Xd, (void*)bins, (void*)bins_col_dummy,
-                        (void*)outd})
+                        (void*)outd, (void*)cmask})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }

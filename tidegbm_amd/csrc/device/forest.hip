@@ -47,13 +47,16 @@ void tgbm_forest::reserve(int n) {
 // Leaf reached by one binned row from node `k` (tree.cpp apply_one): bin 0 is missing and goes
 // the node's missing direction; a pair node compares the hashed bin pair with its threshold.
 __device__ __forceinline__ int walk_tree(const FNode* __restrict__ nodes, const unsigned char* __restrict__ flags,
+                                         const unsigned long long* __restrict__ cmask,
                                          const unsigned char* __restrict__ row, int k, int nb) {
     unsigned char fl = flags[k];
     while (!(fl & 1)) {
         const FNode nd = nodes[k];
         const int b1 = row[nd.fx & 0xffff];
         bool left;
-        if (((fl >> 1) & 3) == 1) {
+        if (((fl >> 1) & 3) >= 2) {   // categorical: the bin's bit in the node's mask
+            left = (cmask[(size_t)nd.thr * 4 + (b1 >> 6)] >> (b1 & 63)) & 1ull;
+        } else if (((fl >> 1) & 3) == 1) {
             const int b2 = row[nd.fx >> 16];
             if (b1 == 0 || b2 == 0) left = (fl >> 3) & 1;
             else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)nb) <= nd.thr;
@@ -73,12 +76,13 @@ __global__ __launch_bounds__(256) void k_forest(const FNode* __restrict__ nodes,
                                                 const int* __restrict__ roots,
                                                 const double* __restrict__ lr, int n_trees,
                                                 double init, const unsigned char* __restrict__ bins,
-                                                int fp, int n, int nb, double* __restrict__ out) {
+                                                int fp, int n, int nb, const unsigned long long* __restrict__ cmask,
+                                                double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const unsigned char* row = bins + (size_t)i * fp;
     double s = init;
-    for (int t = 0; t < n_trees; ++t) s += lr[t] * values[walk_tree(nodes, flags, row, roots[t], nb)];
+    for (int t = 0; t < n_trees; ++t) s += lr[t] * values[walk_tree(nodes, flags, cmask, row, roots[t], nb)];
     out[i] = s;
 }
 
@@ -107,11 +111,17 @@ struct HostNodes {
     std::vector<SNode> snodes;
     std::vector<unsigned char> flags;
     std::vector<int> tree_nodes;   // per tree
+    std::vector<unsigned long long> cmask;   // 4 words per categorical node, in node order
+    std::vector<int> tree_cats;    // categorical nodes per tree
     int total = 0;
 };
 
-static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node_ints, int n_features, int base) {
+// cat_masks: 4 words per node (all nodes of all trees), or null when no node is categorical;
+// a categorical node's mask gets index cat_base + (its rank among the categorical nodes).
+static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node_ints, int n_features, int base,
+                             const unsigned long long* cat_masks = nullptr, int cat_base = 0) {
     HostNodes h;
+    h.tree_cats.assign(n_trees, 0);
     for (int t = 0; t < n_trees; ++t) h.total += tree_nodes[t];
     h.nodes.resize(std::max(h.total, 1));
     h.snodes.resize(std::max(h.total, 1));
@@ -127,18 +137,25 @@ static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node
             if (!leaf) {
                 if (v[2] < 0 || v[2] >= n_features || v[5] < 0 || v[6] < 0 || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
                     throw std::runtime_error("malformed tree node");
-                if (v[1] != 0 && v[1] != 1) throw std::runtime_error("only axis and pair splits run on the device");
+                if (v[1] < 0 || v[1] > 3) throw std::runtime_error("unknown split type");
                 const int f2 = v[1] == 1 ? v[3] : 0;
                 if (f2 < 0 || f2 >= n_features) throw std::runtime_error("malformed pair node");
                 nd.fx = v[2] | (f2 << 16);
                 nd.thr = v[4];
+                if (v[1] >= 2) {
+                    if (!cat_masks) throw std::runtime_error("categorical node without its bin mask");
+                    nd.thr = cat_base + (int)(h.cmask.size() / 4);
+                    const unsigned long long* m = cat_masks + (size_t)(off + k) * 4;
+                    h.cmask.insert(h.cmask.end(), m, m + 4);
+                    h.tree_cats[t] += 1;
+                }
                 nd.left = base + off + v[5];
                 nd.right = base + off + v[6];
             }
             h.nodes[off + k] = nd;
             h.flags[off + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
             SNode sn{0, 0, 0, 0, 0, h.flags[off + k]};
-            if (!leaf) {
+            if (!leaf && v[1] < 2) {
                 if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
                 sn.f = (unsigned short)v[2];
                 sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
@@ -171,6 +188,19 @@ static tgbm_forest* assemble(const tgbm_forest* base, int keep_trees, int keep_n
                                                                             : f->toff_h.begin());
     if (f->toff_h.empty()) f->toff_h.push_back(0);
     for (int t = 0; t < n_new; ++t) f->toff_h.push_back(f->toff_h.back() + h.tree_nodes[t]);
+    if (base) f->cat_off_h.assign(base->cat_off_h.begin(), base->cat_off_h.begin() + keep_trees + 1);
+    if (f->cat_off_h.empty()) f->cat_off_h.push_back(0);
+    const int keep_cat = f->cat_off_h.back();
+    for (int t = 0; t < n_new; ++t) f->cat_off_h.push_back(f->cat_off_h.back() + h.tree_cats[t]);
+    f->n_cat = f->cat_off_h.back();
+    if (f->n_cat) {
+        FHIP(hipMalloc(&f->cmask, sizeof(unsigned long long) * 4 * f->n_cat));
+        if (base && keep_cat)
+            FHIP(hipMemcpy(f->cmask, base->cmask, sizeof(unsigned long long) * 4 * keep_cat, hipMemcpyDeviceToDevice));
+        if (!h.cmask.empty())
+            FHIP(hipMemcpy(f->cmask + (size_t)4 * keep_cat, h.cmask.data(), sizeof(unsigned long long) * h.cmask.size(),
+                           hipMemcpyHostToDevice));
+    }
     f->max_chunk_nodes = 1;
     for (int t0 = 0; t0 < f->n_trees; t0 += SERVE_TREES_PER_CHUNK) {
         const int t1 = std::min(f->n_trees, t0 + SERVE_TREES_PER_CHUNK);
@@ -216,26 +246,28 @@ extern "C" {
 
 int tgbm_forest_create(tgbm_forest** out, int n_features, int n_bins, int n_trees, const int* tree_nodes,
                        const int* node_ints, const double* values, const double* lr, double init_score,
-                       const double* edges, const int* n_edges) {
+                       const double* edges, const int* n_edges, const unsigned long long* cat_masks) {
     *out = nullptr;
     return fguard([&] {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
             throw std::runtime_error("GPU device not available: no HIP device found");
         if (n_features <= 0 || n_trees < 0) throw std::runtime_error("bad forest shape");
-        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, n_features, 0);
+        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, n_features, 0, cat_masks, 0);
         *out = assemble(nullptr, 0, 0, h, values, lr, n_features, n_bins, init_score, edges, n_edges);
     });
 }
 
 int tgbm_forest_append(tgbm_forest** out, const tgbm_forest* base, int keep_trees, int n_trees, const int* tree_nodes,
-                       const int* node_ints, const double* values, const double* lr) {
+                       const int* node_ints, const double* values, const double* lr,
+                       const unsigned long long* cat_masks) {
     *out = nullptr;
     return fguard([&] {
         if (!base) throw std::runtime_error("forest append needs a base forest");
         if (keep_trees < 0 || keep_trees > base->n_trees || n_trees < 0) throw std::runtime_error("bad forest append shape");
         const int keep_nodes = base->toff_h[keep_trees];
-        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, base->F, keep_nodes);
+        const HostNodes h = build_nodes(n_trees, tree_nodes, node_ints, base->F, keep_nodes, cat_masks,
+                                        base->cat_off_h[keep_trees]);
         *out = assemble(base, keep_trees, keep_nodes, h, values, lr, base->F, base->nb, base->init, nullptr, nullptr);
     });
 }
@@ -245,7 +277,7 @@ void tgbm_forest_destroy(tgbm_forest* f) { delete f; }
 static void forest_run(tgbm_forest* f, int n, double* out) {
     if (n > 0) {
         hipLaunchKernelGGL(k_forest, dim3((n + 255) / 256), dim3(256), 0, f->stream, f->nodes, f->flags, f->values,
-                           f->roots, f->lr, f->n_trees, f->init, f->bins, f->fp, n, f->nb, f->outd);
+                           f->roots, f->lr, f->n_trees, f->init, f->bins, f->fp, n, f->nb, f->cmask, f->outd);
         FHIP(hipGetLastError());
         FHIP(hipMemcpyAsync(out, f->outd, sizeof(double) * n, hipMemcpyDeviceToHost, f->stream));
     }
@@ -306,11 +338,12 @@ struct tgbm_evalset {
     FNode* nodes = nullptr;           // the tree being added (at most 255 nodes)
     unsigned char* flags = nullptr;
     double* values = nullptr;
+    unsigned long long* cmask = nullptr;   // its categorical nodes' bin masks
     hipStream_t stream = nullptr;
     ~tgbm_evalset() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : {(void*)bins, (void*)y, (void*)pred, (void*)order, (void*)gstart, (void*)gvals, (void*)out,
-                        (void*)nodes, (void*)flags, (void*)values})
+                        (void*)nodes, (void*)flags, (void*)values, (void*)cmask})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -323,12 +356,13 @@ __global__ __launch_bounds__(256) void k_eval_fill(double* __restrict__ pred, in
 
 // pred += lr * value of the row's leaf: the running sum of predict_multi_binned, one tree at a time.
 __global__ __launch_bounds__(256) void k_eval_add(const FNode* __restrict__ nodes, const unsigned char* __restrict__ flags,
+                                                  const unsigned long long* __restrict__ cmask,
                                                   const double* __restrict__ values, double lr,
                                                   const unsigned char* __restrict__ bins, int fp, int n, int nb,
                                                   double* __restrict__ pred) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    pred[i] = pred[i] + lr * values[walk_tree(nodes, flags, bins + (size_t)i * fp, 0, nb)];
+    pred[i] = pred[i] + lr * values[walk_tree(nodes, flags, cmask, bins + (size_t)i * fp, 0, nb)];
 }
 
 // Mean squared error with unit weights (Objective::loss, objective.cpp:323-328).  The reference
@@ -424,6 +458,7 @@ int tgbm_eval_create(tgbm_evalset** out, int n_features, int n_bins, const doubl
         FHIP(hipMalloc(&e->nodes, sizeof(FNode) * 256));
         FHIP(hipMalloc(&e->flags, 256));
         FHIP(hipMalloc(&e->values, sizeof(double) * 256));
+        FHIP(hipMalloc(&e->cmask, sizeof(unsigned long long) * 4 * 256));
         if (n_groups > 0) {
             const int m = group_start[n_groups];
             FHIP(hipMalloc(&e->order, sizeof(int) * std::max(m, 1)));
@@ -466,14 +501,18 @@ int tgbm_eval_reset(tgbm_evalset* e, double init_score) {
     });
 }
 
-int tgbm_eval_add_tree(tgbm_evalset* e, int n_nodes, const int* node_ints, const double* values, double lr) {
+int tgbm_eval_add_tree(tgbm_evalset* e, int n_nodes, const int* node_ints, const double* values, double lr,
+                       const unsigned long long* cat_masks) {
     return fguard([&] {
-        const HostNodes h = build_nodes(1, &n_nodes, node_ints, e->F, 0);
+        const HostNodes h = build_nodes(1, &n_nodes, node_ints, e->F, 0, cat_masks, 0);
+        if (!h.cmask.empty())
+            FHIP(hipMemcpyAsync(e->cmask, h.cmask.data(), sizeof(unsigned long long) * h.cmask.size(),
+                                hipMemcpyHostToDevice, e->stream));
         FHIP(hipMemcpyAsync(e->nodes, h.nodes.data(), sizeof(FNode) * n_nodes, hipMemcpyHostToDevice, e->stream));
         FHIP(hipMemcpyAsync(e->flags, h.flags.data(), n_nodes, hipMemcpyHostToDevice, e->stream));
         FHIP(hipMemcpyAsync(e->values, values, sizeof(double) * n_nodes, hipMemcpyHostToDevice, e->stream));
-        hipLaunchKernelGGL(k_eval_add, dim3((e->n + 255) / 256), dim3(256), 0, e->stream, e->nodes, e->flags, e->values,
-                           lr, e->bins, e->fp, e->n, e->nb, e->pred);
+        hipLaunchKernelGGL(k_eval_add, dim3((e->n + 255) / 256), dim3(256), 0, e->stream, e->nodes, e->flags, e->cmask,
+                           e->values, lr, e->bins, e->fp, e->n, e->nb, e->pred);
         FHIP(hipGetLastError());
         FHIP(hipStreamSynchronize(e->stream));   // the host node tables above end with this call
     });

@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ X, int n
     const int f0 = blockIdx.y * FB;
     for (int j = threadIdx.x; j < FB * NBIN; j += blockDim.x) {
         const int fl = j / NBIN, k = j % NBIN, f = f0 + fl;
-        e[fl][k] = (f < F && k < n_edges[f]) ? edges[(size_t)f * NBIN + k] : 0.0;
+        e[fl][k] = (f < F && k < (n_edges[f] & (CAT_EDGE_FLAG - 1))) ? edges[(size_t)f * NBIN + k] : 0.0;
     }
     if (threadIdx.x < FB) ne[threadIdx.x] = (f0 + (int)threadIdx.x < F) ? n_edges[f0 + threadIdx.x] : 0;
     __syncthreads();
@@ -36,18 +36,28 @@ __global__ __launch_bounds__(256) void k_bin(const double* __restrict__ X, int n
     if (f >= F) return;
     const int r0 = blockIdx.x * rows_per_block;
     const int r1 = min(n, r0 + rows_per_block);
-    const int cnt = ne[fl];
+    const int cnt = ne[fl] & (CAT_EDGE_FLAG - 1);
+    const bool cat = (ne[fl] & CAT_EDGE_FLAG) != 0;
     for (int i = r0 + (int)threadIdx.x / FB; i < r1; i += blockDim.x / FB) {
         const double x = X[(size_t)i * F + f];
         int b = 0;
         if (x == x) {
             int lo = 0, hi = cnt;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (e[fl][mid] <= x) lo = mid + 1;
-                else hi = mid;
+            if (cat) {  // lower_bound, then an exact match (binning.cpp:107-112)
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (e[fl][mid] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                b = (lo < cnt && e[fl][lo] == x) ? 1 + lo : 0;
+            } else {   // upper_bound (binning.cpp:113-116)
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (e[fl][mid] <= x) lo = mid + 1;
+                    else hi = mid;
+                }
+                b = 1 + lo;
             }
-            b = 1 + lo;
         }
         bins_row[(size_t)i * fp + f] = (unsigned char)b;
         if (bins_col) bins_col[(size_t)f * n_pad + i] = (unsigned char)b;
@@ -1179,6 +1189,125 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
     }
 }
 
+// Best categorical split of one feature (reference best_categorical, histogram.cpp:195-244) for
+// the node of every wave with `mine` set (one node per wave).  The bins holding rows, missing
+// (bin 0) included, are ranked by G / (H + cat_smooth); equal ratios rank by bin index (the
+// reference's std::sort is not stable, so it may order equal ratios differently).  The scan over
+// that order uses exact int64 prefix sums; every position but the last is a candidate and the
+// first maximum wins.  The left set is the bins ranked up to the best position.  Every wave of
+// the block calls this (it synchronizes the block); kq/kh/kc are the block's [4][NBIN] LDS rows,
+// whose contents every wave has consumed on entry (row `wave` is this wave's scratch).
+__device__ void scan_cat(bool mine, const long long* qg, const int* c, const long long* qh, bool hconst,
+                         double inv_g, double inv_h, const GrowP& P, long long (*kq)[NBIN], long long (*kh)[NBIN],
+                         int (*kc)[NBIN], double& out_gain, int& out_pos, int& out_ml, int& out_cl,
+                         unsigned long long (&mask)[4], long long& totQ, long long& totQh, long long& totC) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const double sm = P.cat_smooth > 0.0 ? P.cat_smooth : 0.0;
+    double key[4];
+    int rank[4] = {0, 0, 0, 0};
+    __syncthreads();
+    if (mine) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double g = (double)qg[t] * inv_g;
+            const double h = hconst ? (double)c[t] : (double)qh[t] * inv_h;
+            key[t] = g / (h + sm);
+            kq[wave][lane * 4 + t] = __double_as_longlong(key[t]);
+            kc[wave][lane * 4 + t] = c[t];
+        }
+    }
+    __syncthreads();
+    if (mine) {
+        for (int j = 0; j < NBIN; ++j) {
+            const double kj = __longlong_as_double(kq[wave][j]);
+            const bool vj = kc[wave][j] > 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                rank[t] += (vj && (kj < key[t] || (kj == key[t] && j < lane * 4 + t))) ? 1 : 0;
+        }
+    }
+    __syncthreads();
+    int m = 0;
+    if (mine) {
+        m = wave_sum_i((c[0] > 0) + (c[1] > 0) + (c[2] > 0) + (c[3] > 0));
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (c[t] > 0) {
+                kq[wave][rank[t]] = qg[t];
+                kh[wave][rank[t]] = hconst ? 0 : qh[t];
+                kc[wave][rank[t]] = c[t];
+            }
+    }
+    __syncthreads();
+    out_gain = -INFINITY;
+    out_pos = -1;
+    out_ml = 1;
+    out_cl = 0;
+    mask[0] = mask[1] = mask[2] = mask[3] = 0;
+    if (!mine) return;
+    // exact prefix sums over ranked positions lane*4 .. lane*4+3
+    long long pq[4], ph[4], pc[4];
+    long long rq = 0, rh = 0, rc = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int p = lane * 4 + t;
+        if (p < m) {
+            rq += kq[wave][p];
+            rh += kh[wave][p];
+            rc += kc[wave][p];
+        }
+        pq[t] = rq;
+        ph[t] = rh;
+        pc[t] = rc;
+    }
+    const long long iq = wave_incl_ll(rq), ih = hconst ? 0 : wave_incl_ll(rh), ic = wave_incl_ll(rc);
+    const long long eq = iq - rq, eh = ih - (hconst ? 0 : rh), ec = ic - rc;
+    totQ = (long long)read_lane_u64((unsigned long long)iq, 63);
+    totQh = hconst ? 0 : (long long)read_lane_u64((unsigned long long)ih, 63);
+    totC = (long long)read_lane_u64((unsigned long long)ic, 63);
+    const double tG = (double)totQ * inv_g;
+    const double tH = hconst ? (double)totC : (double)totQh * inv_h;
+    const double parent = leaf_score(tG, tH, P.l2, P.l1);
+    double best = -INFINITY;
+    int bkey = 0x7fffffff, bcl = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int p = lane * 4 + t;
+        const long long ql = eq + pq[t], qhl = eh + ph[t], cl = ec + pc[t];
+        const long long cr = totC - cl;
+        const double gl = (double)ql * inv_g, gr = (double)(totQ - ql) * inv_g;
+        const double hl = hconst ? (double)cl : (double)qhl * inv_h;
+        const double hr = hconst ? (double)cr : (double)(totQh - qhl) * inv_h;
+        const bool ok = (p + 1 < m) & (cl >= P.min_leaf) & (cr >= P.min_leaf) & (hl >= P.min_hess) & (hr >= P.min_hess);
+        const double gain = 0.5 * (leaf_score(gl, hl, P.l2, P.l1) + leaf_score(gr, hr, P.l2, P.l1) - parent) - P.gamma;
+        const bool take = ok & (gain > best);
+        best = take ? gain : best;
+        bkey = take ? p : bkey;
+        bcl = take ? (int)cl : bcl;
+    }
+    const int wl = wave_best_lane(best, bkey);
+    best = __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(best), wl));
+    bkey = __builtin_amdgcn_readlane(bkey, wl);
+    bcl = __builtin_amdgcn_readlane(bcl, wl);
+    if (bkey == 0x7fffffff) return;
+    // left set: bin lane*4+t is bit (lane & 15)*4 + t of word lane >> 4
+    unsigned long long bal[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) bal[t] = __ballot(c[t] > 0 && rank[t] <= bkey);
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        unsigned long long word = 0;
+        for (int j = 0; j < 16; ++j)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) word |= ((bal[t] >> (16 * w + j)) & 1ull) << (4 * j + t);
+        mask[w] = word;
+    }
+    out_gain = best;
+    out_pos = bkey;
+    out_cl = bcl;
+    out_ml = (int)(mask[0] & 1ull);
+}
+
 // mode 0: the root (node 0) from the root launch's partials.  mode 1: both children of the
 // current split: the smaller from the child launch's partials, the larger as parent - smaller
 // (exact in fixed point; reference subtract_hist_inplace, histogram.cpp:108-117).  A block owns
@@ -1241,6 +1370,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     double gain = -INFINITY;
     int thr = -1, ml = 1, cl = 0;
     long long tq = 0, tqh = 0, tc = 0;
+    const bool catf = S.is_cat != nullptr && S.is_cat[f] != 0;
     const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
     if (active) {
         if (side == 0) {
@@ -1271,9 +1401,14 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
                 c[t] = 0;
                 qh[t] = 0;
             }
-        scan_feature_half(qg, c, qh, T.hconst, half, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf,
-                          P.min_hess, P.l1, gain, thr, ml, cl, tq, tqh, tc);
+        if (!catf)
+            scan_feature_half(qg, c, qh, T.hconst, half, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf,
+                              P.min_hess, P.l1, gain, thr, ml, cl, tq, tqh, tc);
     }
+    unsigned long long cmask[4] = {0, 0, 0, 0};
+    if (catf)   // block-uniform: every wave takes the categorical scan's barriers
+        scan_cat(active && half == 0, qg, c, qh, T.hconst, inv_g, inv_h, P, rq, rh, rc, gain, thr, ml, cl, cmask,
+                 tq, tqh, tc);
     __shared__ double xg[4];
     __shared__ int xt[4], xm[4], xc[4];
     if (lane == 0) {
@@ -1301,6 +1436,11 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     fc.cl = cl;
     fc.pad = 0;
     stg(T.fcand + (size_t)node * S.n_features + f, fc);
+    if (catf) {
+        const auto fm = gbl(T.fmask + ((size_t)node * S.n_features + f) * 4);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) fm[w] = cmask[w];
+    }
     if (f == 0) {
         // node value from the node's own sums (tree.cpp:323-328, 407-416)
         const double G = (double)tq * inv_g;
@@ -1327,6 +1467,10 @@ constexpr int PART_RPT = 4;
 
 __device__ inline bool row_goes_left(const Shared& S, const Task& T, const Decision& d, int i) {
     const int b1 = S.bins_col[(size_t)d.feat * S.n_pad + T.row0 + i];
+    if (d.type == 2) {   // cat_goes_left (tree.hpp:27-30): the bin's bit in the leaf's mask
+        const auto fm = gbl((const unsigned long long*)T.fmask + ((size_t)d.leaf * S.n_features + d.feat) * 4);
+        return (fm[b1 >> 6] >> (b1 & 63)) & 1ull;
+    }
     if (d.type == 1) {
         const int b2 = S.bins_col[(size_t)d.feat2 * S.n_pad + T.row0 + i];
         if (b1 == 0 || b2 == 0) return d.mgl != 0;
@@ -1359,7 +1503,8 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
             const int leaf = T.cur_leaf, F = S.n_features;
             const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
                              T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
-                             (T.cur_type != 0 && T.cur_type != 1) || T.n_nodes + 2 > MAX_NODES ||
+                             (T.cur_type != 0 && T.cur_type != 1 && !(T.cur_type == 2 && T.fmask)) ||
+                             (T.cur_type == 2 && !(S.is_cat && S.is_cat[T.cur_feat])) || T.n_nodes + 2 > MAX_NODES ||
                              T.n_nodes + 2 > 2 * P.max_leaves - 1;
             sd.leaf = bad ? -1 : leaf;   // a decision that does not name a splittable leaf stops the tree
             sd.type = T.cur_type;
@@ -1389,7 +1534,13 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
         Tw.cur_type = d.type;
         Tw.cur_feat = d.feat;
         Tw.cur_feat2 = d.feat2;
-        Tw.cur_thr = d.thr;
+        Tw.cur_thr = d.type == 2 ? -1 : d.thr;   // a categorical node has no threshold bin
+        if (d.type == 2) {   // the split node keeps its mask for the exported tree
+            const auto fm = gbl((const unsigned long long*)T.fmask + ((size_t)d.leaf * S.n_features + d.feat) * 4);
+            const auto nm = gbl(T.nmask + (size_t)d.leaf * 4);
+#pragma unroll
+            for (int w = 0; w < 4; ++w) nm[w] = fm[w];
+        }
         Tw.cur_mgl = d.mgl;
         Tw.cur_gain = d.gain;
         Tw.cur_start = st;

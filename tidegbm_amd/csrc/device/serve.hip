@@ -215,6 +215,7 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
             const tgbm_predict_item& it = items[i];
             const tgbm_forest* f = it.forest;
             if (!f) throw std::runtime_error("predict item without an ensemble");
+            if (f->n_cat) throw std::runtime_error("the batched predict kernel does not route categorical splits");
             if (it.n < 0) throw std::runtime_error("predict item with n < 0");
             const int from_x = it.rows ? 0 : 1;
             if (!from_x) {
@@ -293,6 +294,7 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
 int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f) {
     return sguard([&] {
         if (!f) throw std::runtime_error("no ensemble");
+        if (f->n_cat) throw std::runtime_error("the batched predict kernel does not route categorical splits");
         const unsigned char* pbins = nullptr;
         int pfp = 0, pn = 0, pF = 0;
         session_panel_info(s, &pbins, &pfp, &pn, &pF);

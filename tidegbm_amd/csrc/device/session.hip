@@ -176,6 +176,7 @@ struct tgbm_session {
         g.mds = p->max_delta_step;
         g.gamma = p->min_split_gain;
         g.pair_l2 = p->pair_l2;
+        g.cat_smooth = p->cat_smooth;
         return g;
     }
     Task* td() const { return tasks_d + a0; }   // first active task (device)
@@ -410,11 +411,50 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
 
 void tgbm_session_destroy(tgbm_session* s) { delete s; }
 
+static_assert(CAT_EDGE_FLAG == TGBM_CAT_EDGES, "categorical edge flag");
+
+int tgbm_session_set_cat(tgbm_session* s, const int* is_cat) {
+    return dev_guard([&] {
+        if (s->S.is_cat) throw std::runtime_error("categorical flags are set once per session");
+        std::vector<unsigned char> flags(s->F, 0);
+        bool any = false;
+        for (int f = 0; f < s->F; ++f) {
+            flags[f] = is_cat && is_cat[f] ? 1 : 0;
+            any |= flags[f] != 0;
+        }
+        if (!any) return;
+        unsigned char* d = s->alloc<unsigned char>(s->F);
+        HIP_OK(hipMemcpy(d, flags.data(), s->F, hipMemcpyHostToDevice));
+        for (int t = 0; t < s->T; ++t) {
+            Task& T = s->tasks_h[t];
+            T.fmask = s->alloc<unsigned long long>((size_t)MAX_NODES * s->F * 4);
+            T.nmask = s->alloc<unsigned long long>((size_t)MAX_NODES * 4);
+            HIP_OK(hipMemsetAsync(T.nmask, 0, sizeof(unsigned long long) * MAX_NODES * 4, s->stream));
+            s->set_field_ptr(t, offsetof(Task, fmask), T.fmask);
+            s->set_field_ptr(t, offsetof(Task, nmask), T.nmask);
+        }
+        s->S.is_cat = d;
+    });
+}
+
+int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out) {
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        const Task& T = s->tasks_h[task];
+        if (!T.nmask) throw std::runtime_error("the session has no categorical feature");
+        if (n_nodes < 0 || n_nodes > MAX_NODES) throw std::runtime_error("node count out of range");
+        HIP_OK(hipMemcpyAsync(out, T.nmask, sizeof(unsigned long long) * 4 * n_nodes, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
 int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges) {
     return dev_guard([&] {
-        for (int f = 0; f < s->F; ++f)
-            if (n_edges[f] < 0 || n_edges[f] > NBIN - 2)
+        for (int f = 0; f < s->F; ++f) {
+            const int ne = n_edges[f] & ~CAT_EDGE_FLAG;
+            if (ne < 0 || ne > NBIN - 2 || (n_edges[f] & ~(CAT_EDGE_FLAG | (CAT_EDGE_FLAG - 1))))
                 throw std::runtime_error("a feature has more than 254 bin edges; the device path stores uint8 bins (max_bin <= 255)");
+        }
         // stream X in row blocks to bound device memory
         const int rows_blk = std::max(1, (int)std::min<size_t>((size_t)s->n, (512ull << 20) / (sizeof(double) * s->F)));
         double* Xd = dalloc<double>((size_t)rows_blk * s->F);

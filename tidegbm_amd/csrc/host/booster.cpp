@@ -12,6 +12,7 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <iterator>
 #include <numeric>
 #include <thread>
 
@@ -120,6 +121,7 @@ tgbm_grow_params grow_params(const Config& c) {
     p.max_delta_step = c.max_delta_step;
     p.min_split_gain = c.min_split_gain;
     p.pair_l2 = c.pair_l2;
+    p.cat_smooth = c.cat_smooth;
     p.hist_mode = c.hist_mode;
     p.host_driven = (c.pair_splits || c.force_host_driver) ? 1 : 0;
     return p;
@@ -145,11 +147,12 @@ void fetch_leaf(tgbm_session* s, int node, int F, LeafInfo& li) {
     dev_check(tgbm_leaf_scan_results(s, 0, node, li.gain.data(), li.thr.data(), li.ml.data(), &li.energy, &li.rows));
 }
 
-// Candidate of one leaf (reference best_for_leaf, tree.cpp:106-281, with no categorical,
-// interaction-group, regime or HALO branches): axis argmax over ascending features, then the
-// PAIR search on weak leaves over the top-k features ordered by std::sort.
+// Candidate of one leaf (reference best_for_leaf, tree.cpp:106-281, with no interaction-group,
+// regime or HALO branches): argmax over ascending features of their axis or categorical
+// (is_cat) best splits, then the PAIR search on weak leaves over the top-k features ordered by
+// std::sort.
 tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& c, int F,
-                         int pair_budget, const tgbm_grow_params& gp) {
+                         int pair_budget, const tgbm_grow_params& gp, const std::vector<int>& is_cat) {
     tgbm_cand best{};
     best.gain = -1e300;
     best.feature = -1;
@@ -162,8 +165,9 @@ tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& 
         if (li.gain[f] > best.gain) {
             best.gain = li.gain[f];
             best.feature = f;
-            best.threshold = li.thr[f];
+            best.threshold = li.thr[f];   // categorical: the best position of the ranked bins
             best.missing_go_left = li.ml[f];
+            best.type = (!is_cat.empty() && is_cat[f]) ? 2 : 0;
         }
     }
     const bool weak = (best.gain < c.pair_trigger * std::max(li.energy * 0.01, 1e-12)) || best.gain <= 0;
@@ -193,7 +197,8 @@ tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& 
 }
 
 // Host-driven best-first loop (reference grow_tree, tree.cpp:343-463).
-void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_params& gp) {
+void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_params& gp,
+                      const std::vector<int>& is_cat) {
     std::vector<LeafInfo> nodes(1);
     fetch_leaf(s, 0, F, nodes[0]);
     nodes[0].depth = 0;
@@ -207,7 +212,7 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
             if (!li.leaf || li.depth >= c.max_depth) continue;
             if (li.rows < 2 * c.min_samples_leaf) continue;
             if (!li.cached) {
-                li.cand = leaf_candidate(s, lid, li, c, F, c.max_pair_splits_per_tree - pair_used, gp);
+                li.cand = leaf_candidate(s, lid, li, c, F, c.max_pair_splits_per_tree - pair_used, gp, is_cat);
                 li.cached = true;
             }
             const tgbm_cand& r = li.cand;
@@ -437,15 +442,62 @@ TimeLayout time_layout(const Config& cfg, const long long* ts, const double* y, 
     return tl;
 }
 
-void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges) {
-    const int F = (int)edges.size();
-    std::vector<double> E((size_t)F * 256, 0.0);
-    std::vector<int> ne(F);
-    for (int f = 0; f < F; ++f) {
-        ne[f] = (int)edges[f].size();
-        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
-    }
+void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges,
+                   const std::vector<int>& is_cat, const std::vector<std::vector<double>>& cats) {
+    std::vector<double> E;
+    std::vector<int> ne;
+    flat_edges(edges, E, ne, is_cat, cats);
     dev_check(tgbm_session_bin(s, X, E.data(), ne.data()));
+}
+
+// Binner::fit's categorical branch (binning.cpp:48-82): count each distinct non-NaN value, keep
+// those seen at least cat_min_count times (else the single most frequent), keep the
+// max_bin - 1 most frequent with std::partial_sort when more remain, and sort the survivors.
+std::vector<double> category_values(const double* X, int n, int F, int f, int max_bin, int cat_min_count) {
+    std::vector<double> vals;
+    vals.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        const double x = X[(size_t)i * F + f];
+        if (!std::isnan(x)) vals.push_back(x);
+    }
+    std::sort(vals.begin(), vals.end());
+    std::vector<std::pair<int, double>> counts;   // (occurrences, value)
+    for (size_t i = 0; i < vals.size();) {
+        size_t j = i;
+        while (j < vals.size() && vals[j] == vals[i]) ++j;
+        counts.emplace_back((int)(j - i), vals[i]);
+        i = j;
+    }
+    const int need = std::max(1, cat_min_count);
+    std::vector<std::pair<int, double>> keep;
+    std::copy_if(counts.begin(), counts.end(), std::back_inserter(keep),
+                 [&](const std::pair<int, double>& q) { return q.first >= need; });
+    if (keep.empty() && !counts.empty())
+        keep.push_back(*std::max_element(counts.begin(), counts.end(),
+                                         [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+                                             return a.first < b.first;
+                                         }));
+    const size_t cap = (size_t)std::max(0, max_bin - 1);
+    if (keep.size() > cap) {
+        std::partial_sort(keep.begin(), keep.begin() + cap, keep.end(),
+                          [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+                              return a.first > b.first;
+                          });
+        keep.resize(cap);
+    }
+    std::vector<double> out(keep.size());
+    std::transform(keep.begin(), keep.end(), out.begin(), [](const std::pair<int, double>& q) { return q.second; });
+    std::sort(out.begin(), out.end());
+    return out;
+}
+
+std::vector<unsigned long long> tree_cat_masks(const Tree& t) {
+    bool any = false;
+    for (const Node& d : t.nodes) any |= !d.is_leaf && d.split_type >= 2;
+    if (!any) return {};
+    std::vector<unsigned long long> m(t.nodes.size() * 4, 0ull);
+    for (size_t k = 0; k < t.nodes.size(); ++k) std::copy(t.nodes[k].cat_left, t.nodes[k].cat_left + 4, &m[4 * k]);
+    return m;
 }
 
 Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov, const double* pref,
@@ -545,8 +597,9 @@ bool Booster::eval_step(int m) {
             ints.clear();
             vals.clear();
             tree_node_ints(trees[t], ints, vals);
+            const std::vector<unsigned long long> cm = tree_cat_masks(trees[t]);
             dev_check(tgbm_eval_add_tree(ev.set, (int)trees[t].nodes.size(), ints.data(), vals.data(),
-                                         cfg.learning_rate));
+                                         cfg.learning_rate, cm.empty() ? nullptr : cm.data()));
         }
         dev_check(tgbm_eval_metric(ev.set, ev.ic ? 1 : 0, &ev.last));
         ev.use = use;
@@ -586,13 +639,17 @@ void tree_node_ints(const Tree& tr, std::vector<int>& ints, std::vector<double>&
     }
 }
 
-void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne) {
+void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne,
+                const std::vector<int>& is_cat, const std::vector<std::vector<double>>& cats) {
     const int F = (int)edges.size();
     E.assign((size_t)F * 256, 0.0);
     ne.assign(F, 0);
     for (int f = 0; f < F; ++f) {
-        ne[f] = (int)edges[f].size();
-        std::copy(edges[f].begin(), edges[f].end(), E.begin() + (size_t)f * 256);
+        const bool cat = (int)is_cat.size() == F && is_cat[f];
+        const std::vector<double>& e = cat ? cats.at(f) : edges[f];
+        if (e.size() > 254) throw Error("more than 254 bin edges or categories in a feature");
+        ne[f] = (int)e.size() | (cat ? TGBM_CAT_EDGES : 0);
+        std::copy(e.begin(), e.end(), E.begin() + (size_t)f * 256);
     }
 }
 
@@ -602,16 +659,20 @@ tgbm_forest* make_forest(const Booster& b) {
     std::vector<int> tn(use);
     std::vector<int> ints;
     std::vector<double> vals, lrs(use, b.cfg.learning_rate);
+    std::vector<unsigned long long> cm;
+    const bool cat = b.has_cat();
     for (int t = 0; t < use; ++t) {
         tn[t] = (int)b.trees[t].nodes.size();
         tree_node_ints(b.trees[t], ints, vals);
+        if (cat)
+            for (const Node& d : b.trees[t].nodes) cm.insert(cm.end(), d.cat_left, d.cat_left + 4);
     }
     std::vector<double> E;
     std::vector<int> ne;
-    flat_edges(b.edges, E, ne);
+    flat_edges(b.edges, E, ne, b.is_cat, b.cat_values);
     tgbm_forest* out = nullptr;
     dev_check(tgbm_forest_create(&out, F, b.n_bins, use, tn.data(), ints.data(), vals.data(), lrs.data(), b.init_score,
-                                 E.data(), ne.data()));
+                                 E.data(), ne.data(), cat ? cm.data() : nullptr));
     return out;
 }
 
@@ -625,6 +686,8 @@ std::shared_ptr<Booster> Booster::clone_model() const {
     m->init_score = init_score;
     m->init_prefix = init_prefix;
     m->edges = edges;
+    m->is_cat = is_cat;
+    m->cat_values = cat_values;
     m->trees = trees;
     m->fitted = fitted;
     return m;
@@ -656,13 +719,14 @@ void Booster::predict_binned_u16(const unsigned short* xb, int n, double* out) {
 }
 
 void Booster::fit(const double* X0, const double* y0, int n, int F, const long long* ts0, const double* w0,
-                  const int* is_cat, const long long* ent0, const long long* rg0) {
+                  const int* is_cat_in, const long long* ent0, const long long* rg0) {
     const double t0 = now_s();
     cfg.validate();
     if (n <= 0 || F <= 0) throw Error("empty input");
-    if (is_cat)
-        for (int f = 0; f < F; ++f)
-            if (is_cat[f]) throw Error("categorical features are not supported on the GPU path");
+    is_cat.assign(F, 0);
+    if (is_cat_in)
+        for (int f = 0; f < F; ++f) is_cat[f] = is_cat_in[f] ? 1 : 0;
+    if (!has_cat()) is_cat.clear();
     if (tgbm_device_count() < 1) throw Error("GPU device not available: no HIP device was found for device=gpu");
     if (n >= 1000000) cfg.pair_splits = false;  // booster.cpp:237-240
     drop_forest();
@@ -718,8 +782,16 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     init_score = tl.init_score;
     // ---- bin edges (booster.cpp:378-380) and bins on the device
     fit_edges(X, n, F, cfg.max_bin, cfg.bin_sample, (uint32_t)cfg.random_state, edges);
-    n_bins = cfg.max_bin;
-    for (auto& e : edges) n_bins = std::max(n_bins, (int)e.size() + 2);
+    cat_values.assign(F, {});
+    for (int f = 0; f < F; ++f)
+        if (!is_cat.empty() && is_cat[f]) {   // a categorical feature has a vocabulary, no edges
+            edges[f].clear();
+            cat_values[f] = category_values(X, n, F, f, cfg.max_bin, cfg.cat_min_count);
+        }
+    n_bins = cfg.max_bin;   // Binner::max_bins (binning.cpp:130-134)
+    for (int f = 0; f < F; ++f)
+        n_bins = std::max(n_bins, (!is_cat.empty() && is_cat[f]) ? (int)cat_values[f].size() + 1
+                                                                  : (int)edges[f].size() + 2);
     // ---- eval set (booster.cpp:459-478): later than the training rows, binned with their edges
     std::unique_ptr<EvalRun> ev;
     evals_valid.clear();
@@ -765,7 +837,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
         }
         std::vector<double> E;
         std::vector<int> ne;
-        flat_edges(edges, E, ne);
+        flat_edges(edges, E, ne, is_cat, cat_values);
         dev_check(tgbm_eval_create(&ev->set, F, n_bins, E.data(), ne.data(), eval_X.data(), eval_y.data(), n_eval,
                                    order.empty() ? nullptr : order.data(), gstart.data(), (int)gstart.size() - 1));
     }
@@ -775,7 +847,8 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     const bool hconst = cfg.objective == 0 && !any_weight && !goss_active;
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, F, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
-    bin_on_device(sg.s, X, edges);
+    if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
+    bin_on_device(sg.s, X, edges, is_cat, cat_values);
     const long long* groups = (cfg.objective == 5) ? rg : nullptr;
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, tl.pred.data(), tl.bucket.data(),
                                  nullptr, tl.n_buckets, cfg.iid_mode ? nullptr : tl.cuts.data(), groups));
@@ -814,7 +887,9 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
     (void)cuts;
     tgbm_grow_params gp = grow_params(cfg);
     const int F = n_features;
-    const bool host_driven = cfg.pair_splits || cfg.force_host_driver;
+    // categorical splits are decided on the host (the device-driven selection handles axis splits)
+    const bool host_driven = cfg.pair_splits || cfg.force_host_driver || has_cat();
+    gp.host_driven = host_driven ? 1 : 0;
     // the histogram kernel variant: forced, or chosen over the first trees (all give the same trees)
     const std::string shape = "fit:" + std::to_string(launch_rows_of(n)) + ":" + std::to_string(F) + ":" +
                               std::to_string(cfg.objective) + ":" + std::to_string(goss) + ":" +
@@ -865,7 +940,7 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         if (cfg.hist_mode == 0) gp.hist_mode = tuner.mode(m);
         const double t_tree = now_s();
         dev_check(tgbm_tree_begin(s, &gp));
-        if (host_driven) grow_host_driven(s, cfg, F, gp);
+        if (host_driven) grow_host_driven(s, cfg, F, gp, is_cat);
         else dev_check(tgbm_tree_grow_device(s, &gp));
         int nn = 0;
         dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
@@ -877,6 +952,12 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
             evals_train.push_back(loss);   // booster.cpp:704-706
         }
         Tree t = unpack_tree(nn, ints.data(), dbl.data(), vals.data(), cov.data(), pref.data(), n_time_buckets);
+        if (has_cat()) {
+            std::vector<unsigned long long> cm((size_t)nn * 4);
+            dev_check(tgbm_task_cat_masks(s, 0, nn, cm.data()));
+            for (int k = 0; k < nn; ++k)
+                if (!t.nodes[k].is_leaf && t.nodes[k].split_type >= 2) std::copy(&cm[4 * k], &cm[4 * k] + 4, t.nodes[k].cat_left);
+        }
         if (for_update) {
             // Booster::update keeps grow_tree's cover (node sample counts) and one bucket
             for (int k = 0; k < nn; ++k) t.cover[k] = t.nodes[k].n_samples;
@@ -902,7 +983,8 @@ void Booster::update(const double* X, const double* y, int n, int nf, const doub
     const bool hconst = cfg.objective == 0 && !any_weight;
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, nf, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
-    bin_on_device(sg.s, X, edges);
+    if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
+    bin_on_device(sg.s, X, edges, is_cat, cat_values);
     std::vector<int> zeros(n, 0);
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? w : nullptr, pred.data(), zeros.data(), nullptr, 1, nullptr,
                                  nullptr));

@@ -58,6 +58,7 @@ struct Config {
     bool finer_torb = true;
     double torb_min_cover = 0.0;
     int cat_min_count = 5;
+    double cat_smooth = 1.0;   // config.hpp:70: Laplace term of the categorical G/H ranking
     bool col_major_hist = true;
     std::string device = "gpu";
     int profile = 0;
@@ -74,6 +75,7 @@ struct Node {  // reference Node (tree.hpp:11-25)
     int is_leaf = 1, split_type = 0, feature = -1, feature2 = -1, threshold = -1, left = -1,
         right = -1, missing_go_left = 1, n_samples = 0, depth = 0;
     double value = 0.0, gain = 0.0;
+    unsigned long long cat_left[4] = {0, 0, 0, 0};   // categorical split: bins that go left
 };
 
 struct Tree {  // reference Tree (tree.hpp:32-41)
@@ -95,7 +97,10 @@ struct TimeLayout {
 TimeLayout time_layout(const Config& cfg, const long long* ts, const double* y, const double* w, int n);
 void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
                std::vector<std::vector<double>>& edges);
-void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges);
+// Sorted category values of categorical feature f (reference Binner::fit, binning.cpp:48-82).
+std::vector<double> category_values(const double* X, int n, int F, int f, int max_bin, int cat_min_count);
+void bin_on_device(tgbm_session* s, const double* X, const std::vector<std::vector<double>>& edges,
+                   const std::vector<int>& is_cat = {}, const std::vector<std::vector<double>>& cats = {});
 Tree unpack_tree(int nn, const int* ints, const double* dbl, const double* vals, const int* cov,
                  const double* pref, int n_buckets);
 tgbm_grow_params grow_params(const Config& c);
@@ -105,8 +110,13 @@ class Booster;
 tgbm_forest* make_forest(const Booster& b);
 // One tree in the export node layout (TGBM_NODE_INTS ints per node) with its leaf values.
 void tree_node_ints(const Tree& t, std::vector<int>& ints, std::vector<double>& vals);
-// Edges as the device takes them: edges[f*256 + k], k < n_edges[f].
-void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne);
+// Edges as the device takes them: edges[f*256 + k], k < n_edges[f]; a categorical feature
+// passes its category values with TGBM_CAT_EDGES set in n_edges[f].
+void flat_edges(const std::vector<std::vector<double>>& edges, std::vector<double>& E, std::vector<int>& ne,
+                const std::vector<int>& is_cat = {}, const std::vector<std::vector<double>>& cats = {});
+// Bin masks of a tree's categorical nodes, 4 words per node (zeros elsewhere), or empty when the
+// tree has none.
+std::vector<unsigned long long> tree_cat_masks(const Tree& t);
 void dev_check(int rc);
 double now_s();
 
@@ -120,8 +130,15 @@ public:
     double init_score = 0.0;
     std::vector<double> init_prefix;
     std::vector<std::vector<double>> edges;
+    std::vector<int> is_cat;                       // per feature (empty: none categorical)
+    std::vector<std::vector<double>> cat_values;   // categorical features' sorted category values
     std::vector<Tree> trees;
     bool fitted = false;
+    bool has_cat() const {
+        for (int v : is_cat)
+            if (v) return true;
+        return false;
+    }
     // last fit diagnostics
     double t_setup = 0.0, t_boost = 0.0;
     std::vector<double> tree_end_times;  // seconds since the boosting loop started, per tree

@@ -93,6 +93,7 @@ int tide_set_double(TideHandle h, const char* key, double v) {
     else if (k == "max_delta_step") c.max_delta_step = v;
     else if (k == "torb_min_cover") c.torb_min_cover = v;
     else if (k == "time_decay") c.time_decay = v;
+    else if (k == "cat_smooth") c.cat_smooth = v;
     else if (k == "tree_decay") c.tree_decay = v;
     else {
         g_err = "unknown double key";
@@ -296,6 +297,22 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
 }
 
 int tide_hist_variant(TideHandle h) { return static_cast<Booster*>(h)->hist_variant; }
+
+int tide_tree_cat(TideHandle h, int tree, unsigned long long* out) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        const tide::Tree& t = b->trees.at(tree);
+        for (size_t k = 0; k < t.nodes.size(); ++k) std::copy(t.nodes[k].cat_left, t.nodes[k].cat_left + 4, out + 4 * k);
+    });
+}
+
+int tide_get_cat_values(TideHandle h, int feature, double* out, int cap) {
+    auto* b = static_cast<Booster*>(h);
+    if (feature < 0 || feature >= (int)b->cat_values.size()) return 0;
+    const auto& v = b->cat_values[feature];
+    for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
+    return (int)v.size();
+}
 
 int tide_set_eval(TideHandle h, const double* X_eval, const double* y_eval, const long long* ts_eval, int n_eval,
                   int n_features) {

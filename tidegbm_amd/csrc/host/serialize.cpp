@@ -106,8 +106,9 @@ std::string Booster::save_binary() const {
     for (double v : init_prefix) put_f64(s, v);
     put_i32(s, n_features);  // feature names f0..f{F-1} (booster.cpp:244-245)
     for (int i = 0; i < n_features; ++i) put_str(s, "f" + std::to_string(i));
+    auto cat = [&](int f) { return (int)is_cat.size() == n_features && is_cat[f] ? 1 : 0; };
     put_i32(s, n_features);  // is_cat
-    for (int i = 0; i < n_features; ++i) put_i32(s, 0);
+    for (int i = 0; i < n_features; ++i) put_i32(s, cat(i));
     put_i32(s, n_features);  // group_of
     for (int i = 0; i < n_features; ++i) put_i32(s, -1);
     put_i32(s, n_features);  // monotone
@@ -115,11 +116,14 @@ std::string Booster::save_binary() const {
     put_i32(s, cfg.max_bin);  // binner
     put_i32(s, n_features);
     for (int f = 0; f < n_features; ++f) {
-        put_i32(s, 0);
+        put_i32(s, cat(f));
         put_i32(s, (int)edges[f].size());
         for (double e : edges[f]) put_f64(s, e);
-        put_i32(s, 0);
-        put_i32(s, (int)edges[f].size() + 2);
+        const std::vector<double> none;
+        const std::vector<double>& cv = cat(f) ? cat_values[f] : none;
+        put_i32(s, (int)cv.size());
+        for (double v : cv) put_f64(s, v);
+        put_i32(s, cat(f) ? (int)cv.size() + 1 : (int)edges[f].size() + 2);   // n_bins_col
     }
     put_i32(s, (int)trees.size());
     for (const Tree& t : trees) {
@@ -138,7 +142,7 @@ std::string Booster::save_binary() const {
             put_f64(s, n.gain);
             put_i32(s, n.n_samples);
             put_i32(s, n.depth);
-            for (int k = 0; k < 4; ++k) put_i64(s, 0);
+            for (int k = 0; k < 4; ++k) put_i64(s, (long long)n.cat_left[k]);
         }
         put_i32(s, (int)t.values.size());
         for (double v : t.values) put_f64(s, v);
@@ -194,8 +198,9 @@ Booster* Booster::load_binary(const std::string& blob) {
         const int nn = r.count();
         for (int i = 0; i < nn; ++i) (void)r.str();
         const int nc = r.count();
-        for (int i = 0; i < nc; ++i)
-            if (r.i32() != 0) throw Error("categorical features are not supported on the GPU path");
+        b->is_cat.assign(nc, 0);
+        for (int i = 0; i < nc; ++i) b->is_cat[i] = r.i32() != 0 ? 1 : 0;
+        if (!b->has_cat()) b->is_cat.clear();
         const int ng = r.count();
         for (int i = 0; i < ng; ++i) (void)r.i32();
         const int nm = r.count();
@@ -205,12 +210,16 @@ Booster* Booster::load_binary(const std::string& blob) {
         const int bf = r.count(1 << 20);
         if (bf != b->n_features) throw Error("binner feature count mismatch");
         b->edges.assign(bf, {});
+        b->cat_values.assign(bf, {});
         for (int f = 0; f < bf; ++f) {
-            if (r.i32() != 0) throw Error("categorical features are not supported on the GPU path");
+            const bool cat = r.i32() != 0;
+            if (cat != (b->has_cat() && b->is_cat[f] != 0)) throw Error("binner categorical flags mismatch");
             b->edges[f].resize(r.count(1 << 16));
             for (double& e : b->edges[f]) e = r.f64();
-            const int nv = r.count(1 << 16);
-            for (int i = 0; i < nv; ++i) (void)r.f64();
+            b->cat_values[f].resize(r.count(1 << 16));
+            for (double& v : b->cat_values[f]) v = r.f64();
+            if (b->cat_values[f].size() > 254 || b->edges[f].size() > 254)
+                throw Error("more than 254 bin edges or categories in a feature");
             (void)r.i32();
         }
         b->trees.resize(r.count());
@@ -230,9 +239,10 @@ Booster* Booster::load_binary(const std::string& blob) {
                 n.gain = r.f64();
                 n.n_samples = r.i32();
                 n.depth = r.i32();
-                for (int k = 0; k < 4; ++k) (void)r.i64();
-                if (!n.is_leaf && n.split_type != 0 && n.split_type != 1)
-                    throw Error("only axis and pair splits are supported on the GPU path");
+                for (int k = 0; k < 4; ++k) n.cat_left[k] = (unsigned long long)r.i64();
+                if (!n.is_leaf && (n.split_type < 0 || n.split_type > 3)) throw Error("unknown split type");
+                if (!n.is_leaf && n.split_type >= 2 && !b->has_cat())
+                    throw Error("categorical split in a model without categorical features");
             }
             t.values.resize(r.count());
             for (double& v : t.values) v = r.f64();
@@ -254,6 +264,7 @@ Booster* Booster::load_binary(const std::string& blob) {
 
 std::string Booster::save_json() const {  // reference serialize.cpp:266-324
     std::ostringstream o;
+    auto cat = [&](int f) { return (int)is_cat.size() == n_features && is_cat[f] ? 1 : 0; };
     o << "{\n  \"format\": \"tidegbm\",\n  \"version\": " << 3 << ",\n";
     o << "  \"n_features\": " << n_features << ",\n";
     o << "  \"n_bins\": " << n_bins << ",\n";
@@ -271,9 +282,14 @@ std::string Booster::save_json() const {  // reference serialize.cpp:266-324
         o << "]";
     }
     o << "],\n    \"is_cat\": [";
-    for (int f = 0; f < n_features; ++f) o << (f ? ", " : "") << 0;
+    for (int f = 0; f < n_features; ++f) o << (f ? ", " : "") << cat(f);
     o << "],\n    \"cat_values\": [";
-    for (int f = 0; f < n_features; ++f) o << (f ? ", " : "") << "[]";
+    for (int f = 0; f < n_features; ++f) {
+        o << (f ? ", " : "") << "[";
+        if (cat(f))
+            for (size_t i = 0; i < cat_values[f].size(); ++i) o << (i ? ", " : "") << cat_values[f][i];
+        o << "]";
+    }
     o << "]\n  },\n  \"trees\": [\n";
     for (size_t ti = 0; ti < trees.size(); ++ti) {
         const Tree& t = trees[ti];

@@ -85,6 +85,7 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     const Booster& m0 = *models[0];
     for (const Booster* m : models) {
         if (!m || !m->fitted) throw Error("every ensemble must be fitted");
+        if (m->has_cat()) throw Error("the serving kernel does not route categorical splits");
         if (m->n_features != F) throw Error("ensemble and panel feature counts differ");
         if (m->edges != m0.edges || m->n_bins != m0.n_bins)
             throw Error("all ensembles of a server must share bin edges (one binned panel serves them all)");
@@ -357,7 +358,8 @@ void Server::publish_refit(Slot& sl) {
         }
     }
     tgbm_forest* f = nullptr;
-    dev_check(tgbm_forest_append(&f, base.forest.get(), keep, add, tn.data(), ints.data(), vals.data(), lrs.data()));
+    dev_check(tgbm_forest_append(&f, base.forest.get(), keep, add, tn.data(), ints.data(), vals.data(), lrs.data(),
+                                 nullptr));
     auto v = std::make_shared<Version>();
     v->snap = snap;
     v->forest = own(f);
