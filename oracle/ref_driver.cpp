@@ -30,6 +30,7 @@ struct EvalSet {
     std::vector<long long> ts;
     int n = 0;
     std::vector<int> is_cat;   // Booster::fit's is_cat of later fits (empty: none)
+    std::vector<int> group_of; // Booster::fit's group_of of later fits (empty: none)
 };
 static std::map<void*, EvalSet> g_eval;
 static std::mutex g_eval_mu;
@@ -56,6 +57,19 @@ void tref_free(void* h) {
         g_eval.erase(h);
     }
     delete static_cast<Booster*>(h);
+}
+
+// Feature groups of later fits (Booster::fit's group_of); nf = 0 clears.
+int tref_set_groups(void* h, const int* group_of, int nf) {
+    std::lock_guard<std::mutex> lk(g_eval_mu);
+    g_eval[h].group_of.assign(group_of, group_of + (nf > 0 ? nf : 0));
+    return 0;
+}
+
+// Config::interaction_group (config.hpp:59); n = 0 clears.
+int tref_set_interaction(void* h, const int* groups, int n) {
+    static_cast<Booster*>(h)->cfg.interaction_group.assign(groups, groups + (n > 0 ? n : 0));
+    return 0;
 }
 
 // Categorical flags of later fits (Booster::fit's is_cat); nf = 0 clears.
@@ -113,6 +127,8 @@ int tref_set(void* h, const char* key, double v) {
     else if (k == "ooc_bytes") c.ooc_bytes = static_cast<size_t>(v);
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
     else if (k == "cat_min_count") c.cat_min_count = iv;
+    else if (k == "regime_feature") c.regime_feature = iv;
+    else if (k == "regime_priority") c.regime_priority = iv != 0;
     else if (k == "cat_smooth") c.cat_smooth = v;
     else if (k == "eval_metric") c.eval_metric = iv == 1 ? "loss" : (iv == 2 ? "ic" : "auto");
     else {
@@ -136,7 +152,8 @@ int tref_fit(void* h, const double* X, const double* y, int n, int nf, const lon
         b->cfg.iid_mode = iid != 0;
         if (!b->cfg.iid_mode) b->cfg.horizon = horizon;
         b->fit(X, y, n, nf, reinterpret_cast<const i64*>(ts), w,
-               (int)ev.is_cat.size() == nf ? ev.is_cat.data() : nullptr, nullptr, nullptr,
+               (int)ev.is_cat.size() == nf ? ev.is_cat.data() : nullptr,
+               (int)ev.group_of.size() == nf ? reinterpret_cast<const i32*>(ev.group_of.data()) : nullptr, nullptr,
                reinterpret_cast<const i64*>(rank_groups), ev.n > 0 ? ev.X.data() : nullptr,
                ev.n > 0 ? ev.y.data() : nullptr,
                ev.n > 0 && !ev.ts.empty() ? reinterpret_cast<const i64*>(ev.ts.data()) : nullptr, ev.n,

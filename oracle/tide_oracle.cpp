@@ -17,6 +17,7 @@
 #include <cstring>
 #include <numeric>
 #include <random>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -55,6 +56,9 @@ struct Cfg {  // defaults: reference cpp/include/tidegbm/config.hpp:13-76, cpp/s
     bool finer_torb = true;
     double torb_min_cover = 0.0;
     int cat_min_count = 5;          // config.hpp / bindings.cpp cat_min_count
+    int regime_feature = -1;        // config.hpp:55-56
+    bool regime_priority = true;
+    std::vector<int> interaction_group;  // config.hpp:59, per feature, -1 = unconstrained
     double cat_smooth = 1.0;        // config.hpp:70
     int early_stopping_rounds = 0;  // config.hpp early_stopping_rounds (api.py:23-24: None -> 0)
     int eval_metric = 0;            // config.hpp:76 "auto" | "loss" | "ic" as 0 | 1 | 2
@@ -389,33 +393,85 @@ struct GrowCtx {
     const double *g, *h;
     const Cfg* cfg;
     const int* is_cat;   // per feature, or null
+    const int* group_of; // per feature feature group (Booster::fit's group_of), or null
 };
 
+// Features a node may split on (tree.cpp:304-317): every feature, or under an interaction group
+// ixg >= 0 the unconstrained features and those of group ixg, with the regime feature appended
+// when it is missing; an empty list falls back to every feature.
+static std::vector<int> active_features(const Cfg& c, int F, int ixg) {
+    std::vector<int> all(F);
+    std::iota(all.begin(), all.end(), 0);
+    if (c.regime_feature >= 0 && c.regime_feature < F &&
+        std::find(all.begin(), all.end(), c.regime_feature) == all.end())
+        all.push_back(c.regime_feature);
+    if (ixg < 0 || c.interaction_group.empty()) return all;
+    std::vector<int> out;
+    for (int f : all) {
+        const int g = f < (int)c.interaction_group.size() ? c.interaction_group[f] : -1;
+        if (g < 0 || g == ixg) out.push_back(f);
+    }
+    if (c.regime_feature >= 0 && std::find(out.begin(), out.end(), c.regime_feature) == out.end())
+        out.push_back(c.regime_feature);
+    return out.empty() ? all : out;
+}
+
 static Cand leaf_candidate(const GrowCtx& cx, const std::vector<int>& rows, const Hist& hist,
-                           int pair_budget, double energy) {
+                           int pair_budget, double energy, const std::vector<int>& active) {
     const Cfg& c = *cx.cfg;
     const int F = cx.F;
     std::vector<Cand> per_f(F);
-    for (int f = 0; f < F; ++f)  // tree.cpp:126-133
+    for (int f : active)  // tree.cpp:126-133
         per_f[f] = (cx.is_cat && cx.is_cat[f])
                        ? scan_categorical(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
                                           c.cat_smooth, c.l1)
                        : scan_feature(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
                                       c.l1);
+    // feature groups (tree.cpp:135-158): a group is represented by its best-gain feature (the
+    // first one in `active` order wins ties); groups numbered below 256 appear once
+    std::map<int, int> group_best;
+    for (int f : active) {
+        const int grp = cx.group_of ? cx.group_of[f] : -1;
+        if (grp < 0) continue;
+        auto it = group_best.find(grp);
+        if (it == group_best.end()) group_best[grp] = f;
+        else if (per_f[f].gain > per_f[it->second].gain) it->second = f;
+    }
+    std::vector<int> allowed;
+    std::vector<char> seen(256, 0);
+    for (int f : active) {
+        const int grp = cx.group_of ? cx.group_of[f] : -1;
+        if (grp >= 0) {
+            if (grp < 256 && seen[grp]) continue;
+            if (grp < 256) seen[grp] = 1;
+            f = group_best[grp];
+        }
+        allowed.push_back(f);
+    }
     Cand best;
-    for (int f = 0; f < F; ++f) {  // ascending feature order, strict '>' (tree.cpp:160-166)
+    for (int f : allowed) {  // strict '>' keeps the first maximum (tree.cpp:160-166)
         if (per_f[f].feature < 0 || per_f[f].gain < c.min_split_gain) continue;
         if (per_f[f].gain > best.gain) best = per_f[f];
+    }
+    // regime priority (tree.cpp:168-176): the regime feature's split wins when its gain is at
+    // least half the best; on a categorical regime feature it becomes a Regime node (type 3)
+    if (c.regime_priority && c.regime_feature >= 0 && c.regime_feature < F) {
+        const int rf = c.regime_feature;
+        if (per_f[rf].feature >= 0 && per_f[rf].gain > 0) {
+            if (best.feature < 0 || per_f[rf].gain >= 0.5 * std::max(best.gain, 0.0)) {
+                best = per_f[rf];
+                best.type = (cx.is_cat && cx.is_cat[rf]) ? 3 : 0;
+            }
+        }
     }
     // PAIR search on weak leaves (tree.cpp:215-279)
     const bool weak =
         (best.gain < c.pair_trigger * std::max(energy * 0.01, 1e-12)) || best.gain <= 0;
-    if (c.pair_splits && pair_budget > 0 && F >= 2 && weak) {
-        std::vector<int> top(F);
-        std::iota(top.begin(), top.end(), 0);
+    if (c.pair_splits && pair_budget > 0 && allowed.size() >= 2 && weak) {
+        std::vector<int> top = allowed;
         std::sort(top.begin(), top.end(),
                   [&](int a, int b) { return per_f[a].gain > per_f[b].gain; });
-        const int k = std::min(c.pair_top_k, F);
+        const int k = std::min(c.pair_top_k, (int)top.size());
         std::vector<std::pair<int, int>> pairs;
         for (int a = 0; a < k; ++a)
             for (int b = a + 1; b < k; ++b) pairs.emplace_back(top[a], top[b]);
@@ -464,6 +520,7 @@ static Tree grow(const GrowCtx& cx, std::vector<int> sample) {
     build(cx.xb, cx.F, sample.data(), (int)sample.size(), cx.g, cx.h, hists[0], cx.nb);
     std::vector<Cand> cache(1);
     std::vector<char> cached(1, 0);
+    std::vector<int> ixg(1, -1);   // interaction group a node's path is bound to (tree.cpp:440-445)
     int pair_used = 0;
     for (int it = 0; it < c.max_leaves - 1; ++it) {
         int best_leaf = -1;
@@ -475,8 +532,8 @@ static Tree grow(const GrowCtx& cx, std::vector<int> sample) {
             if (!cached[lid]) {
                 double energy = 0;
                 for (int i : rows[lid]) energy += cx.g[i] * cx.g[i];
-                cache[lid] = leaf_candidate(cx, rows[lid], hists[lid],
-                                            c.max_pair_splits - pair_used, energy);
+                cache[lid] = leaf_candidate(cx, rows[lid], hists[lid], c.max_pair_splits - pair_used, energy,
+                                            active_features(c, cx.F, ixg[lid]));
                 cached[lid] = 1;
             }
             const Cand& r = cache[lid];
@@ -537,6 +594,12 @@ static Tree grow(const GrowCtx& cx, std::vector<int> sample) {
         cache.emplace_back();
         cached.push_back(0);
         cached.push_back(0);
+        int child_ixg = ixg[best_leaf];
+        if (child_ixg < 0 && best.feature >= 0 && best.feature < (int)c.interaction_group.size() &&
+            c.interaction_group[best.feature] >= 0)
+            child_ixg = c.interaction_group[best.feature];
+        ixg.push_back(child_ixg);
+        ixg.push_back(child_ixg);
     }
     return t;
 }
@@ -722,6 +785,7 @@ struct Model {
     // Categorical features (Booster::fit's is_cat, booster.cpp:246-247) and their vocabularies.
     std::vector<int> is_cat;
     std::vector<std::vector<double>> cat_values;
+    std::vector<int> group_of;   // Booster::fit's group_of (feature groups), empty = none
 
     bool cat(int f) const { return !is_cat.empty() && is_cat[f] != 0; }
 
@@ -831,6 +895,7 @@ struct Model {
             fit_edges(X.data(), n, F, c.max_bin, c.bin_sample, (uint32_t)c.random_state, edges);
         }
         if (!is_cat.empty() && (int)is_cat.size() != F) throw std::runtime_error("is_cat: feature count mismatch");
+        if (!group_of.empty() && (int)group_of.size() != F) throw std::runtime_error("group_of: feature count mismatch");
         cat_values.assign(F, {});
         for (int f = 0; f < F; ++f)
             if (cat(f)) {  // categorical features have no edges, only a vocabulary (binning.cpp:48-82)
@@ -919,7 +984,8 @@ struct Model {
                 gu = gg.data();
                 hu = hh.data();
             }
-            GrowCtx cx{xb.data(), F, nb, gu, hu, &c, is_cat.empty() ? nullptr : is_cat.data()};
+            GrowCtx cx{xb.data(), F, nb, gu, hu, &c, is_cat.empty() ? nullptr : is_cat.data(),
+                       group_of.empty() ? nullptr : group_of.data()};
             (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample uses it
             Tree t = grow(cx, sample);
             const int nn = (int)t.nodes.size();
@@ -1031,7 +1097,8 @@ struct Model {
         std::iota(all.begin(), all.end(), 0);
         for (int m = 0; m < extra; ++m) {
             grad(y, pred.data(), w.data(), nullptr, n, g.data(), h.data());
-            GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg, is_cat.empty() ? nullptr : is_cat.data()};
+            GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg, is_cat.empty() ? nullptr : is_cat.data(),
+                       group_of.empty() ? nullptr : group_of.data()};
             (void)rng();
             Tree t = grow(cx, all);
             const int nn = (int)t.nodes.size();
@@ -1111,6 +1178,8 @@ int orc_set(void* hd, const char* key, double v) {
     else if (k == "torb_min_cover") c.torb_min_cover = v;
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = iv;
     else if (k == "cat_min_count") c.cat_min_count = iv;
+    else if (k == "regime_feature") c.regime_feature = iv;
+    else if (k == "regime_priority") c.regime_priority = iv != 0;
     else if (k == "cat_smooth") c.cat_smooth = v;
     else if (k == "eval_metric") {
         if (iv < 0 || iv > 2) {
@@ -1231,6 +1300,18 @@ int orc_tree_cat(void* hd, int t, unsigned long long* out) {
     for (size_t k = 0; k < tr.nodes.size(); ++k)
         for (int w = 0; w < 4; ++w) out[4 * k + w] = tr.nodes[k].cat_left[w];
     return (int)tr.nodes.size();
+}
+
+// Feature groups of later fits (Booster::fit's group_of, -1 = none); n_features = 0 clears.
+int orc_set_groups(void* hd, const int* group_of, int n_features) {
+    static_cast<Model*>(hd)->group_of.assign(group_of, group_of + std::max(n_features, 0));
+    return 0;
+}
+
+// Interaction groups (Config::interaction_group, -1 = unconstrained); n = 0 clears.
+int orc_set_interaction(void* hd, const int* groups, int n) {
+    static_cast<Model*>(hd)->cfg.interaction_group.assign(groups, groups + std::max(n, 0));
+    return 0;
 }
 
 // Categorical flags of later fits (is_cat[f] != 0); n_features = 0 clears.

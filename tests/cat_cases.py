@@ -31,3 +31,43 @@ CAT_CASES = {
     "min_count_smooth": dict(iid=False, objective=0,
                              params=dict(n_estimators=8, max_depth=5, cat_min_count=40, cat_smooth=5.0)),
 }
+
+
+# feature groups (Booster::fit's group_of), interaction groups (Config::interaction_group) and the
+# regime feature (tree.cpp:135-176, 300-317, 436-445), on the categorical panel
+GROUP_CASES = {
+    # xor target: weak leaves take PAIR splits, whose top-k list the groups reshape
+    "feature_groups_pair": dict(xor=True, group_of={0: 0, 2: 0, 1: 1, 4: 1, 9: 300, 10: 300},
+                                params=dict(n_estimators=8)),
+    # the strongest feature binds its subtree to group 0, keeping the next two out of it
+    "interaction_groups": dict(interaction={19: 0, 15: 0, 18: 1, 13: 1}, params=dict(n_estimators=8, pair_splits=0)),
+    "regime_numeric": dict(params=dict(n_estimators=8, regime_feature=18, pair_splits=0)),
+    "regime_categorical": dict(params=dict(n_estimators=8, regime_feature=7)),
+    "all_together": dict(group_of={0: 0, 1: 0, 5: 1, 6: 1}, interaction={2: 0, 3: 0, 4: 1, 8: 1},
+                         params=dict(n_estimators=8, regime_feature=7, max_depth=5)),
+}
+
+
+def group_panel(case, seed=33):
+    """The categorical panel, or for the cases that need PAIR splits a numeric panel with an XOR
+    target of features 0 and 1."""
+    if GROUP_CASES[case].get("xor"):   # numeric features only
+        p = synth.panel_small(seed)
+        rng = np.random.default_rng(seed + 7)
+        y = np.sign(p.X[:, 0] * p.X[:, 1]) + 0.1 * rng.normal(size=p.n)
+        return synth.Panel(p.X, y, p.ts, p.entity), np.zeros(p.n_features, np.int32)
+    return cat_panel(seed)
+
+
+def group_arrays(case, n_features):
+    gc = GROUP_CASES[case]
+    grp = ix = None
+    if "group_of" in gc:
+        grp = np.full(n_features, -1, np.int32)
+        for f, g in gc["group_of"].items():
+            grp[f] = g
+    if "interaction" in gc:
+        ix = np.full(n_features, -1, np.int32)
+        for f, g in gc["interaction"].items():
+            ix[f] = g
+    return grp, ix

@@ -136,6 +136,15 @@ class CpuModel:
         self.lib.fn("set_cat")(self.h, _ptr(self._cat), self._cat.size)
         return self
 
+    def set_groups(self, group_of=None, interaction=None):
+        """Feature groups (Booster::fit's group_of) and interaction groups
+        (Config::interaction_group) of later fits; None clears."""
+        self._grp = None if group_of is None else np.ascontiguousarray(group_of, np.int32)
+        self._ix = None if interaction is None else np.ascontiguousarray(interaction, np.int32)
+        self.lib.fn("set_groups")(self.h, _ptr(self._grp), 0 if self._grp is None else self._grp.size)
+        self.lib.fn("set_interaction")(self.h, _ptr(self._ix), 0 if self._ix is None else self._ix.size)
+        return self
+
     def cat_values(self, f):
         buf = np.empty(256)
         m = self.lib.fn("cat_values")(self.h, f, _ptr(buf), 256)
@@ -170,6 +179,7 @@ class CpuLib:
             "evals_train": (_I, [_P, _P, _I]), "evals_valid": (_I, [_P, _P, _I]),
             "best_iteration": (_I, [_P]), "set_eval": (_I, [_P, _P, _P, _P, _I, _I]),
             "tree_cat": (_I, [_P, _I, _P]), "set_cat": (_I, [_P, _P, _I]), "cat_values": (_I, [_P, _I, _P, _I]),
+            "set_groups": (_I, [_P, _P, _I]), "set_interaction": (_I, [_P, _P, _I]),
         }
         for k, (res, args) in sig.items():
             f = getattr(self.lib, f"{prefix}_{k}")

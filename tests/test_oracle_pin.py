@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import checkers
-from cat_cases import CAT_CASES, cat_panel
+from cat_cases import CAT_CASES, GROUP_CASES, cat_panel, group_arrays, group_panel
 from eval_cases import EVAL_CASES, eval_target, split_by_date
 from tidegbm_amd import synth
 
@@ -188,6 +188,26 @@ def test_categorical_features(case):
         assert np.array_equal(x["cat"], y["cat"])
     n_cat = sum(int(np.isin(t["ints"][:, 1], (2, 3)).sum()) for t in o.trees())
     assert n_cat >= 5, "case should exercise categorical splits"
+
+
+@pytest.mark.parametrize("case", sorted(GROUP_CASES))
+def test_feature_groups_interactions_regime(case):
+    gc = GROUP_CASES[case]
+    p, is_cat = group_panel(case)
+    grp, ix = group_arrays(case, p.n_features)
+    out = []
+    for lib in (checkers.oracle(), REF):
+        m = lib.model(learning_rate=0.1, goss=0, **gc["params"])
+        m.set_cat(is_cat)
+        m.set_groups(grp, ix)
+        m.fit(p.X, p.y, ts=p.ts, entity=p.entity, horizon=1)
+        out.append(m)
+    o, r = out
+    _assert_identical(o, r, p.X)
+    for x, y in zip(o.trees(), r.trees()):
+        assert np.array_equal(x["cat"], y["cat"])
+    if gc["params"].get("regime_feature") == 7:
+        assert sum(int((t["ints"][:, 1] == 3).sum()) for t in o.trees()) >= 3, "expected Regime nodes"
 
 
 def test_histogram_and_scan_functions():
