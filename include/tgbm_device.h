@@ -47,8 +47,9 @@ typedef struct tgbm_grow_params {
 /* One split decision made by the host (PAIR-capable driver). */
 typedef struct tgbm_decision {
     int leaf;               /* node id to split, -1 = stop growing */
-    int type;               /* 0 axis, 1 pair, 2 categorical (bins of the leaf's best split of
-                               `feature`, as its scan found them, go left) */
+    int type;               /* 0 axis, 1 pair, 2 categorical, 3 regime (2 and 3: the bins of the
+                               leaf's best split of categorical `feature`, as its scan found
+                               them, go left) */
     int feature, feature2, threshold, missing_go_left;
     double gain;
 } tgbm_decision;

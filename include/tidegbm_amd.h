@@ -87,6 +87,15 @@ int tide_hist_variant(TideHandle h);
 /* Training loss after each tree of the last fit, the reference's evals_result()["train"]
  * (booster.cpp:704-706, bindings.cpp:318-321).  Returns the number of trees (at most cap copied). */
 int tide_evals_train(TideHandle h, double* out, int cap);
+/* Feature groups of the following fits (Booster::fit's group_of, booster.hpp:45-48; -1 = none):
+ * a node considers only the best-gain feature of each group (tree.cpp:135-158).  n_features = 0
+ * clears them. */
+int tide_set_feature_groups(TideHandle h, const int* group_of, int n_features);
+/* Interaction groups (Config::interaction_group, config.hpp:59; -1 = unconstrained): below a
+ * split on a feature of group g, nodes split only on unconstrained features and those of g
+ * (tree.cpp:304-317, 436-445).  n_features = 0 clears them.  With the int keys
+ * "regime_feature" / "regime_priority" (tree.cpp:168-176) these run in the host-driven grower. */
+int tide_set_interaction_groups(TideHandle h, const int* groups, int n_features);
 /* Bin masks of tree `tree`'s categorical nodes (split_type 2): 4 words per node, bit b of word
  * b >> 6 set when bin b goes left (reference Node::cat_left, tree.hpp:24-30); zeros elsewhere. */
 int tide_tree_cat(TideHandle h, int tree, unsigned long long* out);

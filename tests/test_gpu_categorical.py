@@ -1,5 +1,7 @@
 """Categorical features on the GPU path (reference Binner::fit's categorical branch,
-binning.cpp:48-82; best_categorical, histogram.cpp:195-244; cat_goes_left, tree.hpp:27-30).
+binning.cpp:48-82; best_categorical, histogram.cpp:195-244; cat_goes_left, tree.hpp:27-30), and
+the leaf-candidate rules that use them: feature groups, interaction groups and the regime feature
+(tree.cpp:135-176, 293-317, 436-445).
 
 Bar: category vocabularies equal; tree structure and the categorical bin masks bit-exact against
 the CPU oracle and the reference core (a split decided by equal G/(H+cat_smooth) ratios may differ
@@ -13,7 +15,7 @@ import numpy as np
 import pytest
 
 import checkers
-from cat_cases import CAT_CASES, cat_panel
+from cat_cases import CAT_CASES, GROUP_CASES, cat_panel, group_arrays, group_panel
 from tidegbm_amd import Booster, TideError, TideRegressor
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +95,36 @@ def test_estimator_categorical_feature_indices():
     with pytest.raises(TideError, match="categorical"):
         from tidegbm_amd import Server
         Server([m.booster_], p.X[:1000], refit_max_rows=500)
+
+
+@pytest.mark.parametrize("case", sorted(GROUP_CASES))
+def test_groups_interactions_regime_match_oracle(case):
+    gc = GROUP_CASES[case]
+    _say(f"[case] groups {case} {gc}")
+    p, is_cat = group_panel(case)
+    grp, ix = group_arrays(case, p.n_features)
+    params = dict(learning_rate=0.1, goss=0, **gc["params"])
+    extra = {}
+    if grp is not None:
+        extra["feature_groups"] = grp
+    if ix is not None:
+        extra["interaction_group"] = ix
+    gpu = Booster(**params, **extra)
+    gpu.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, horizon=1, is_cat=is_cat if is_cat.any() else None)
+    F = p.n_features
+    cats = [gpu.cat_values(f) if is_cat[f] else None for f in range(F)]
+    xb = checkers.bins_of(p.X, [gpu.edges(f) for f in range(F)], cats)
+    nb = gpu.model_info()["n_bins"]
+    for lib in (checkers.oracle(), checkers.reference()):
+        if lib is None:
+            continue
+        m = lib.model(**params)
+        m.set_cat(is_cat)
+        m.set_groups(grp, ix)
+        m.fit(p.X, p.y, ts=p.ts, entity=p.entity, horizon=1)
+        diff = checkers.compare_trees(gpu.trees(), m.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb, n_bins=nb)
+        assert diff is None, diff
+        a, b = gpu.predict(p.X), m.predict(p.X)
+        assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < PRED_RTOL
+    types = np.concatenate([t["ints"][t["ints"][:, 0] == 0, 1] for t in gpu.trees()])
+    _say(f"  split types: {np.bincount(types, minlength=4).tolist()}")

@@ -69,6 +69,8 @@ _sig("tide_hist_variant", _I, _P)
 _sig("tide_evals_train", _I, _P, _P, _I)
 _sig("tide_evals_valid", _I, _P, _P, _I)
 _sig("tide_tree_cat", _I, _P, _I, _P)
+_sig("tide_set_feature_groups", _I, _P, _P, _I)
+_sig("tide_set_interaction_groups", _I, _P, _P, _I)
 _sig("tide_get_cat_values", _I, _P, _I, _P, _I)
 _sig("tide_set_eval", _I, _P, _P, _P, _P, _I, _I)
 _sig("tide_profile_log", _I, _P, _I, _P, _P, _I)
@@ -92,7 +94,7 @@ _INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_bucke
              "n_jobs", "iid_mode", "halo", "verbose", "horizon", "embargo", "goss", "cat_min_count",
              "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
              "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune",
-             "force_host_driver", "early_stopping_rounds"}
+             "force_host_driver", "early_stopping_rounds", "regime_feature", "regime_priority"}
 _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight", "cat_smooth",
                 "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",
                 "max_delta_step", "torb_min_cover", "time_decay", "tree_decay"}
@@ -140,6 +142,10 @@ class Booster:
             kb = k.encode()
             if k in ("device", "eval_metric"):
                 rc = _lib.tide_set_string(self._h, kb, str(v).encode())
+            elif k in ("interaction_group", "feature_groups"):
+                a = np.ascontiguousarray(v, dtype=np.int32)
+                fn = _lib.tide_set_interaction_groups if k == "interaction_group" else _lib.tide_set_feature_groups
+                rc = fn(self._h, _ptr(a), a.size)
             elif k == "objective":
                 if isinstance(v, (int, np.integer)):
                     rc = _lib.tide_set_int(self._h, kb, int(v))

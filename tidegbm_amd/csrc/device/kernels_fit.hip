@@ -1467,7 +1467,7 @@ constexpr int PART_RPT = 4;
 
 __device__ inline bool row_goes_left(const Shared& S, const Task& T, const Decision& d, int i) {
     const int b1 = S.bins_col[(size_t)d.feat * S.n_pad + T.row0 + i];
-    if (d.type == 2) {   // cat_goes_left (tree.hpp:27-30): the bin's bit in the leaf's mask
+    if (d.type >= 2) {   // Cat / Regime, cat_goes_left (tree.hpp:27-30): the bin's bit in the leaf's mask
         const auto fm = gbl((const unsigned long long*)T.fmask + ((size_t)d.leaf * S.n_features + d.feat) * 4);
         return (fm[b1 >> 6] >> (b1 & 63)) & 1ull;
     }
@@ -1503,8 +1503,8 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
             const int leaf = T.cur_leaf, F = S.n_features;
             const bool bad = leaf < 0 || leaf >= T.n_nodes || !T.nodes[leaf].v[N_LEAF] || T.cur_feat < 0 ||
                              T.cur_feat >= F || (T.cur_type == 1 && (T.cur_feat2 < 0 || T.cur_feat2 >= F)) ||
-                             (T.cur_type != 0 && T.cur_type != 1 && !(T.cur_type == 2 && T.fmask)) ||
-                             (T.cur_type == 2 && !(S.is_cat && S.is_cat[T.cur_feat])) || T.n_nodes + 2 > MAX_NODES ||
+                             T.cur_type < 0 || T.cur_type > 3 || (T.cur_type >= 2 && !T.fmask) ||
+                             (T.cur_type >= 2 && !(S.is_cat && S.is_cat[T.cur_feat])) || T.n_nodes + 2 > MAX_NODES ||
                              T.n_nodes + 2 > 2 * P.max_leaves - 1;
             sd.leaf = bad ? -1 : leaf;   // a decision that does not name a splittable leaf stops the tree
             sd.type = T.cur_type;
@@ -1534,8 +1534,8 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
         Tw.cur_type = d.type;
         Tw.cur_feat = d.feat;
         Tw.cur_feat2 = d.feat2;
-        Tw.cur_thr = d.type == 2 ? -1 : d.thr;   // a categorical node has no threshold bin
-        if (d.type == 2) {   // the split node keeps its mask for the exported tree
+        Tw.cur_thr = d.type >= 2 ? -1 : d.thr;   // a categorical node has no threshold bin
+        if (d.type >= 2) {   // the split node keeps its mask for the exported tree
             const auto fm = gbl((const unsigned long long*)T.fmask + ((size_t)d.leaf * S.n_features + d.feat) * 4);
             const auto nm = gbl(T.nmask + (size_t)d.leaf * 4);
 #pragma unroll

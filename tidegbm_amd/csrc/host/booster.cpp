@@ -11,6 +11,7 @@
 #include <cstring>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <iterator>
 #include <numeric>
@@ -151,8 +152,28 @@ void fetch_leaf(tgbm_session* s, int node, int F, LeafInfo& li) {
 // regime or HALO branches): argmax over ascending features of their axis or categorical
 // (is_cat) best splits, then the PAIR search on weak leaves over the top-k features ordered by
 // std::sort.
+// Features a node may split on (tree.cpp:293-317): all, or under interaction group ixg >= 0 the
+// unconstrained ones and those of the group, with the regime feature appended when missing.
+std::vector<int> node_features(const Config& c, int F, int ixg) {
+    std::vector<int> all(F);
+    std::iota(all.begin(), all.end(), 0);
+    if (c.regime_feature >= 0 && c.regime_feature < F &&
+        std::find(all.begin(), all.end(), c.regime_feature) == all.end())
+        all.push_back(c.regime_feature);
+    if (ixg < 0 || c.interaction_group.empty()) return all;
+    std::vector<int> out;
+    for (int f : all) {
+        const int g = f < (int)c.interaction_group.size() ? c.interaction_group[f] : -1;
+        if (g < 0 || g == ixg) out.push_back(f);
+    }
+    if (c.regime_feature >= 0 && std::find(out.begin(), out.end(), c.regime_feature) == out.end())
+        out.push_back(c.regime_feature);
+    return out.empty() ? all : out;
+}
+
 tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& c, int F,
-                         int pair_budget, const tgbm_grow_params& gp, const std::vector<int>& is_cat) {
+                         int pair_budget, const tgbm_grow_params& gp, const std::vector<int>& is_cat,
+                         const std::vector<int>& group_of, const std::vector<int>& active) {
     tgbm_cand best{};
     best.gain = -1e300;
     best.feature = -1;
@@ -160,22 +181,55 @@ tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& 
     best.threshold = -1;
     best.type = 0;
     best.missing_go_left = 1;
-    for (int f = 0; f < F; ++f) {
+    auto gain_of = [&](int f) { return li.thr[f] >= 0 ? li.gain[f] : -1e300; };
+    auto cand_of = [&](int f) {
+        tgbm_cand r{};
+        r.gain = li.gain[f];
+        r.feature = f;
+        r.feature2 = -1;
+        r.threshold = li.thr[f];   // categorical: the best position of the ranked bins
+        r.missing_go_left = li.ml[f];
+        r.type = (!is_cat.empty() && is_cat[f]) ? 2 : 0;
+        return r;
+    };
+    // one feature per feature group: its best-gain member, first in `active` order on ties;
+    // groups numbered below 256 are listed once (tree.cpp:135-158)
+    std::map<int, int> group_best;
+    for (int f : active) {
+        const int grp = group_of.empty() ? -1 : group_of[f];
+        if (grp < 0) continue;
+        auto it = group_best.find(grp);
+        if (it == group_best.end()) group_best[grp] = f;
+        else if (gain_of(f) > gain_of(it->second)) it->second = f;
+    }
+    std::vector<int> allowed;
+    std::vector<char> seen(256, 0);
+    for (int f : active) {
+        const int grp = group_of.empty() ? -1 : group_of[f];
+        if (grp >= 0) {
+            if (grp < 256 && seen[grp]) continue;
+            if (grp < 256) seen[grp] = 1;
+            f = group_best[grp];
+        }
+        allowed.push_back(f);
+    }
+    for (int f : allowed) {
         if (li.thr[f] < 0 || li.gain[f] < c.min_split_gain) continue;
-        if (li.gain[f] > best.gain) {
-            best.gain = li.gain[f];
-            best.feature = f;
-            best.threshold = li.thr[f];   // categorical: the best position of the ranked bins
-            best.missing_go_left = li.ml[f];
-            best.type = (!is_cat.empty() && is_cat[f]) ? 2 : 0;
+        if (li.gain[f] > best.gain) best = cand_of(f);
+    }
+    // regime priority (tree.cpp:168-176)
+    if (c.regime_priority && c.regime_feature >= 0 && c.regime_feature < F) {
+        const int rf = c.regime_feature;
+        if (li.thr[rf] >= 0 && li.gain[rf] > 0 && (best.feature < 0 || li.gain[rf] >= 0.5 * std::max(best.gain, 0.0))) {
+            best = cand_of(rf);
+            best.type = (!is_cat.empty() && is_cat[rf]) ? 3 : 0;
         }
     }
     const bool weak = (best.gain < c.pair_trigger * std::max(li.energy * 0.01, 1e-12)) || best.gain <= 0;
-    if (c.pair_splits && pair_budget > 0 && F >= 2 && weak) {
-        std::vector<int> top(F);
-        std::iota(top.begin(), top.end(), 0);
-        std::sort(top.begin(), top.end(), [&](int a, int b) { return li.gain[a] > li.gain[b]; });
-        const int k = std::min(c.pair_top_k, F);
+    if (c.pair_splits && pair_budget > 0 && allowed.size() >= 2 && weak) {
+        std::vector<int> top = allowed;
+        std::sort(top.begin(), top.end(), [&](int a, int b) { return gain_of(a) > gain_of(b); });
+        const int k = std::min(c.pair_top_k, (int)top.size());
         std::vector<int> f1, f2;
         for (int a = 0; a < k; ++a)
             for (int b = a + 1; b < k; ++b) {
@@ -198,10 +252,11 @@ tgbm_cand leaf_candidate(tgbm_session* s, int node, LeafInfo& li, const Config& 
 
 // Host-driven best-first loop (reference grow_tree, tree.cpp:343-463).
 void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_params& gp,
-                      const std::vector<int>& is_cat) {
+                      const std::vector<int>& is_cat, const std::vector<int>& group_of) {
     std::vector<LeafInfo> nodes(1);
     fetch_leaf(s, 0, F, nodes[0]);
     nodes[0].depth = 0;
+    std::vector<int> ixg(1, -1);   // interaction group each node's path is bound to (tree.cpp:436-445)
     int pair_used = 0;
     for (int it = 0; it < c.max_leaves - 1; ++it) {
         int best_leaf = -1;
@@ -212,7 +267,8 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
             if (!li.leaf || li.depth >= c.max_depth) continue;
             if (li.rows < 2 * c.min_samples_leaf) continue;
             if (!li.cached) {
-                li.cand = leaf_candidate(s, lid, li, c, F, c.max_pair_splits_per_tree - pair_used, gp, is_cat);
+                li.cand = leaf_candidate(s, lid, li, c, F, c.max_pair_splits_per_tree - pair_used, gp, is_cat,
+                                         group_of, node_features(c, F, ixg[lid]));
                 li.cached = true;
             }
             const tgbm_cand& r = li.cand;
@@ -235,6 +291,12 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
         dev_check(tgbm_apply_split(s, 0, &d, &gp, &left));
         if (left < 0) break;
         nodes.resize(left + 2);
+        int child_ixg = ixg[best_leaf];
+        if (child_ixg < 0 && best.feature >= 0 && best.feature < (int)c.interaction_group.size() &&
+            c.interaction_group[best.feature] >= 0)
+            child_ixg = c.interaction_group[best.feature];
+        ixg.resize(left + 2, -1);
+        ixg[left] = ixg[left + 1] = child_ixg;
         for (int side = 0; side < 2; ++side) {
             LeafInfo& ch = nodes[left + side];
             fetch_leaf(s, left + side, F, ch);
@@ -723,6 +785,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     const double t0 = now_s();
     cfg.validate();
     if (n <= 0 || F <= 0) throw Error("empty input");
+    if (!group_of.empty() && (int)group_of.size() != F) throw Error("feature groups need one entry per feature");
     is_cat.assign(F, 0);
     if (is_cat_in)
         for (int f = 0; f < F; ++f) is_cat[f] = is_cat_in[f] ? 1 : 0;
@@ -888,7 +951,9 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
     tgbm_grow_params gp = grow_params(cfg);
     const int F = n_features;
     // categorical splits are decided on the host (the device-driven selection handles axis splits)
-    const bool host_driven = cfg.pair_splits || cfg.force_host_driver || has_cat();
+    // (and so are feature groups, interaction groups and the regime feature)
+    const bool host_driven = cfg.pair_splits || cfg.force_host_driver || has_cat() || !group_of.empty() ||
+                             !cfg.interaction_group.empty() || cfg.regime_feature >= 0;
     gp.host_driven = host_driven ? 1 : 0;
     // the histogram kernel variant: forced, or chosen over the first trees (all give the same trees)
     const std::string shape = "fit:" + std::to_string(launch_rows_of(n)) + ":" + std::to_string(F) + ":" +
@@ -940,7 +1005,7 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         if (cfg.hist_mode == 0) gp.hist_mode = tuner.mode(m);
         const double t_tree = now_s();
         dev_check(tgbm_tree_begin(s, &gp));
-        if (host_driven) grow_host_driven(s, cfg, F, gp, is_cat);
+        if (host_driven) grow_host_driven(s, cfg, F, gp, is_cat, group_of);
         else dev_check(tgbm_tree_grow_device(s, &gp));
         int nn = 0;
         dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,

@@ -59,6 +59,9 @@ struct Config {
     double torb_min_cover = 0.0;
     int cat_min_count = 5;
     double cat_smooth = 1.0;   // config.hpp:70: Laplace term of the categorical G/H ranking
+    int regime_feature = -1;   // config.hpp:55-56: split on this feature when it is half as good
+    int regime_priority = 1;
+    std::vector<int> interaction_group;   // config.hpp:59: per feature, -1 = unconstrained
     bool col_major_hist = true;
     std::string device = "gpu";
     int profile = 0;
@@ -131,6 +134,7 @@ public:
     std::vector<double> init_prefix;
     std::vector<std::vector<double>> edges;
     std::vector<int> is_cat;                       // per feature (empty: none categorical)
+    std::vector<int> group_of;                     // feature groups of later fits (Booster::fit's group_of)
     std::vector<std::vector<double>> cat_values;   // categorical features' sorted category values
     std::vector<Tree> trees;
     bool fitted = false;

@@ -68,6 +68,8 @@ int tide_set_int(TideHandle h, const char* key, int v) {
     else if (k == "autotune") c.autotune = v;
     else if (k == "force_host_driver") c.force_host_driver = v;
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = v;
+    else if (k == "regime_feature") c.regime_feature = v;
+    else if (k == "regime_priority") c.regime_priority = v != 0;
     else {
         g_err = "unknown int key";
         return -1;
@@ -297,6 +299,22 @@ int tide_fit_times(TideHandle h, double* setup_s, double* boost_s) {
 }
 
 int tide_hist_variant(TideHandle h) { return static_cast<Booster*>(h)->hist_variant; }
+
+int tide_set_feature_groups(TideHandle h, const int* group_of, int n_features) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        if (n_features < 0 || (n_features > 0 && !group_of)) throw tide::Error("bad feature groups");
+        b->group_of.assign(group_of, group_of + n_features);
+    });
+}
+
+int tide_set_interaction_groups(TideHandle h, const int* groups, int n_features) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        if (n_features < 0 || (n_features > 0 && !groups)) throw tide::Error("bad interaction groups");
+        b->cfg.interaction_group.assign(groups, groups + n_features);
+    });
+}
 
 int tide_tree_cat(TideHandle h, int tree, unsigned long long* out) {
     auto* b = static_cast<Booster*>(h);
