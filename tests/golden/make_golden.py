@@ -16,20 +16,30 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "tests"))
 
 import checkers  # noqa: E402
+from cat_cases import cat_panel  # noqa: E402
 from tidegbm_amd import synth  # noqa: E402
 
 OUT = Path(__file__).resolve().parent
 
 
 def cases():
-    """(name, panel, rank_groups, iid, horizon, params) of every fixture."""
+    """(name, panel, rank_groups, iid, horizon, params, opts) of every fixture; opts may hold
+    is_cat, feature groups / interaction groups and an eval set (train mask, eval mask)."""
     small = synth.panel_small(0, n_dates=120)
     rng = np.random.default_rng(0)
     xor = synth.Panel(small.X, np.sign(small.X[:, 0] * small.X[:, 1]) + 0.1 * rng.normal(size=small.n),
                       small.ts, small.entity)
     rank = synth.panel_small(3, n_dates=60, n_entities=70)
     nan = synth.panel_large(5, n_dates=60, n_entities=50, n_features=10, nan_frac=0.05)
-    return [
+    cat, is_cat = cat_panel(35, n_dates=80, n_entities=30)
+    grp = np.full(cat.n_features, -1, np.int32)
+    grp[[0, 1, 2]] = 0
+    ixg = np.full(cat.n_features, -1, np.int32)
+    ixg[[19, 15]] = 0
+    ixg[[18, 13]] = 1
+    dates = np.unique(small.ts)
+    cut = dates[int(len(dates) * 0.7)]
+    cases = [
         ("mse_time", small, None, False, 1, dict(n_estimators=8, max_depth=6, learning_rate=0.08, goss=0)),
         ("mse_pair", xor, None, False, 1, dict(n_estimators=8, max_depth=6, learning_rate=0.08, goss=0)),
         ("mse_iid", small, None, True, 1, dict(n_estimators=6, max_depth=5, learning_rate=0.1, goss=0)),
@@ -38,13 +48,31 @@ def cases():
         ("goss", small, None, False, 1, dict(n_estimators=5, max_depth=5, learning_rate=0.1, goss=1)),
         ("nan_h10", nan, None, False, 10, dict(n_estimators=5, max_depth=6, learning_rate=0.1, goss=0,
                                                bin_sample=500, pair_splits=0)),
+        ("categorical", cat, None, False, 1, dict(n_estimators=5, max_depth=5, learning_rate=0.1, goss=0),
+         dict(is_cat=is_cat)),
+        ("groups_regime", cat, None, False, 1, dict(n_estimators=5, max_depth=5, learning_rate=0.1, goss=0,
+                                                    regime_feature=7),
+         dict(is_cat=is_cat, group_of=grp, interaction=ixg)),
+        ("eval_ic_es", small, None, False, 1, dict(n_estimators=8, max_depth=5, learning_rate=0.1, goss=0,
+                                                   early_stopping_rounds=3),
+         dict(train=small.ts < cut, eval=small.ts >= cut + 1)),
     ]
+    return [c if len(c) == 7 else c + ({},) for c in cases]
 
 
-def fit(lib, name, p, rg, iid, horizon, params):
+def fit(lib, name, p, rg, iid, horizon, params, opts=None):
+    opts = opts or {}
     m = lib.model(**params)
-    m.fit(p.X, p.y, ts=None if iid else p.ts, entity=None if iid else p.entity, rank_groups=rg, iid=iid,
-          horizon=horizon)
+    if "is_cat" in opts:
+        m.set_cat(opts["is_cat"])
+    if "group_of" in opts or "interaction" in opts:
+        m.set_groups(opts.get("group_of"), opts.get("interaction"))
+    rows = opts.get("train", np.ones(p.n, bool))
+    if "eval" in opts:
+        va = opts["eval"]
+        m.set_eval(p.X[va], p.y[va], None if iid else p.ts[va])
+    m.fit(p.X[rows], p.y[rows], ts=None if iid else p.ts[rows], entity=None if iid else p.entity[rows],
+          rank_groups=None if rg is None else rg[rows], iid=iid, horizon=horizon)
     return m
 
 
@@ -65,6 +93,11 @@ def pack(m, p):
         edges=np.concatenate(edges),
         init_prefix=m.init_prefix(),
         pred=m.predict(p.X[:400]),
+        cat=np.concatenate([t["cat"] for t in trees]),
+        cat_values=np.concatenate([m.cat_values(f) for f in range(p.n_features)]),
+        evals_train=m.evals_train(),
+        evals_valid=m.evals_valid(),
+        best_iteration=np.array([m.best_iteration()], np.int32),
     )
 
 
@@ -72,8 +105,8 @@ def main():
     ref = checkers.reference()
     if ref is None:
         raise SystemExit("oracle/_ref/libtide_ref.so is not built")
-    for name, p, rg, iid, h, params in cases():
-        np.savez_compressed(OUT / f"{name}.npz", **pack(fit(ref, name, p, rg, iid, h, params), p))
+    for name, p, rg, iid, h, params, opts in cases():
+        np.savez_compressed(OUT / f"{name}.npz", **pack(fit(ref, name, p, rg, iid, h, params, opts), p))
         print("wrote", name)
 
 

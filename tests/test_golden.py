@@ -20,10 +20,10 @@ GOLDEN = Path(__file__).resolve().parent / "golden"
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_oracle_matches_golden(name):
-    _, p, rg, iid, h, params = CASES[name]
+    _, p, rg, iid, h, params, opts = CASES[name]
     want = np.load(GOLDEN / f"{name}.npz", allow_pickle=False)
     assert np.allclose(want["input_checksum"], [np.nansum(p.X), np.sum(p.y)], rtol=0, atol=0), \
         "panel generator drifted; regenerate fixtures"
-    got = make_golden.pack(make_golden.fit(checkers.oracle(), name, p, rg, iid, h, params), p)
+    got = make_golden.pack(make_golden.fit(checkers.oracle(), name, p, rg, iid, h, params, opts), p)
     for k in want.files:
-        assert np.array_equal(want[k], got[k]), f"{name}: {k}"
+        assert np.array_equal(want[k], got[k], equal_nan=want[k].dtype.kind == "f"), f"{name}: {k}"
