@@ -28,7 +28,8 @@ constexpr int QH_BITS = 39;
 constexpr int QR_BITS = 39;        // refit / TORB sums over all rows (plain int64)
 constexpr int QE_BITS = 39;        // leaf energy sum of g^2
 constexpr int MAX_FLUSH_ROWS = 32768;
-constexpr int LOSS_BLOCKS = 256;   // blocks of the training-loss reduction
+constexpr int PRED_CHUNK = 1024;   // rows per block of the prediction update and its loss partial
+__host__ __device__ inline int loss_chunks(int rows) { return (rows + PRED_CHUNK - 1) / PRED_CHUNK; }
 constexpr unsigned long long CNT_ONE = 1ull << CNT_SHIFT;
 
 __host__ __device__ inline long long unpack_q(unsigned long long v) {
@@ -124,7 +125,8 @@ struct Task {
     int hist_parts;            // partial histograms written by the last histogram launch
     int pad_h;
     double train_loss;         // training loss after the last finished tree (exported)
-    double* loss_part;         // [2 * LOSS_BLOCKS + n_groups] partial sums of the loss
+    double* loss_part;         // [2 * loss_chunks(cap_rows)] per-chunk (sum w d^2, sum w), then
+                               // [n_groups] per-group correlations (pairwise)
     // ---- reductions
     unsigned long long max_g_bits, max_h_bits;
     unsigned long long* red;   // per-block maxima of |g| and |h| [2][MAX_RED_BLOCKS]

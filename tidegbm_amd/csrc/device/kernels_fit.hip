@@ -1925,12 +1925,45 @@ __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double 
     }
 }
 
-// pred += lr * prefix[bucket, leaf] (booster.cpp:655-664).
-__global__ __launch_bounds__(256) void k_pred_update(Task* tasks, double lr) {
+// pred += lr * prefix[bucket, leaf] (booster.cpp:655-664), one chunk of PRED_CHUNK rows per
+// block.  The same pass forms the chunk's share of the training loss recorded after the tree
+// (objective.cpp:323-328: sum of w (pred - y)^2 and of w; booster.cpp:704-706), except for the
+// pairwise objective with groups, whose loss k_loss_groups computes.  Thread sums, then waves,
+// then the block's waves in order: a fixed order that depends only on the task's own rows.
+__global__ __launch_bounds__(256) void k_pred_update(Task* tasks, double lr, int pairwise) {
     Task& T = tasks[blockIdx.y];
+    const int r0 = blockIdx.x * PRED_CHUNK;
+    if (r0 >= T.n_rows) return;
+    const int r1 = min(T.n_rows, r0 + PRED_CHUNK);
     const int nn = T.n_nodes;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n_rows; i += gridDim.x * blockDim.x)
-        T.pred[i] += lr * T.prefix[(size_t)T.bucket[i] * nn + T.nid[i]];
+    const bool mse = !(pairwise && T.has_groups);
+    double s = 0.0, sw = 0.0;
+    for (int i = r0 + (int)threadIdx.x; i < r1; i += blockDim.x) {
+        const double p = T.pred[i] + lr * T.prefix[(size_t)T.bucket[i] * nn + T.nid[i]];
+        T.pred[i] = p;
+        if (mse) {
+            const double wi = T.w ? T.w[i] : 1.0;
+            const double d = p - T.y[i];
+            s += wi * d * d;
+            sw += wi;
+        }
+    }
+    if (!mse) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_down(s, o, 64);
+        sw += __shfl_down(sw, o, 64);
+    }
+    __shared__ double ss[4], sws[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        ss[wave] = s;
+        sws[wave] = sw;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T.loss_part[2 * blockIdx.x] = ((ss[0] + ss[1]) + ss[2]) + ss[3];
+        T.loss_part[2 * blockIdx.x + 1] = ((sws[0] + sws[1]) + sws[2]) + sws[3];
+    }
 }
 
 __global__ void k_zero_torb(Task* tasks, int words) {
@@ -1944,7 +1977,7 @@ __global__ void k_zero_torb(Task* tasks, int words) {
 
 void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
-                     int max_chunks, const GrowP& P, double min_cover, int iid, double lr) {
+                     int max_chunks, const GrowP& P, double min_cover, int iid, double lr, int pairwise) {
     const int rb = std::max(1, std::min(2048, (max_rows + 1023) / 1024));
     if (walk) {
         hipLaunchKernelGGL(k_nid_walk, dim3(rb, n_tasks), dim3(256), 0, st, S, tasks);
@@ -1957,53 +1990,14 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
     hipLaunchKernelGGL(k_torb_accum, dim3(std::max(1, max_chunks), n_tasks), dim3(256), 0, st, tasks,
                        chunks, n_chunks);
     hipLaunchKernelGGL(k_torb_final, dim3(MAX_NODES, n_tasks), dim3(64), 0, st, tasks, P, min_cover, iid);
-    hipLaunchKernelGGL(k_pred_update, dim3(rb, n_tasks), dim3(256), 0, st, tasks, lr);
+    hipLaunchKernelGGL(k_pred_update, dim3(std::max(1, loss_chunks(max_rows)), n_tasks), dim3(256), 0, st, tasks, lr,
+                       pairwise);
 }
 
 // =================================================================== training loss
-// The loss recorded after every tree (objective.cpp:262-329 via booster.cpp:704-706), from the
-// updated predictions of all rows.  Weighted mean squared error: fixed-order two-level sums
-// (deterministic; equal to the reference's sequential sum up to rounding).  Pairwise objective:
-// one thread per group computes the group's Pearson correlation with the reference's
-// sequential sums, and one thread averages the groups in order (equal to the reference bit for
-// bit when the group values are).
-__global__ __launch_bounds__(256) void k_loss_mse(Task* tasks) {
-    const Task& T = tasks[blockIdx.y];
-    const int n = T.n_rows;
-    const auto y = gbl(T.y);
-    const auto pr = gbl((const double*)T.pred);
-    const double* w = T.w;
-    double s = 0.0, sw = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double wi = w ? gbl(w)[i] : 1.0;
-        const double d = pr[i] - y[i];
-        s += wi * d * d;
-        sw += wi;
-    }
-    __shared__ double ss[8], sws[8];
-    // wave sums (fixed lane order), then the waves in order
-    double ws = s, wsw = sw;
-    for (int o = 32; o > 0; o >>= 1) {
-        ws += __shfl_down(ws, o, 64);
-        wsw += __shfl_down(wsw, o, 64);
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        ss[wave] = ws;
-        sws[wave] = wsw;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-            a += ss[k];
-            b += sws[k];
-        }
-        T.loss_part[2 * blockIdx.x] = a;
-        T.loss_part[2 * blockIdx.x + 1] = b;
-    }
-}
-
+// Pairwise objective with groups (objective.cpp:291-321): one thread per group computes the
+// group's Pearson correlation with the reference's sequential sums; k_export averages the groups
+// in order.  The mean squared error of the other objectives comes from k_pred_update's chunks.
 __global__ __launch_bounds__(256) void k_loss_groups(Task* tasks) {
     const Task& T = tasks[blockIdx.y];
     const int grp = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2030,53 +2024,66 @@ __global__ __launch_bounds__(256) void k_loss_groups(Task* tasks) {
         }
         if (vy > 1e-18 && vp > 1e-18) val = c / sqrt(vy * vp);
     }
-    T.loss_part[2 * LOSS_BLOCKS + grp] = val;
+    T.loss_part[2 * loss_chunks(T.cap_rows) + grp] = val;
 }
 
-__global__ __launch_bounds__(64) void k_loss_final(Task* tasks, int pairwise) {
-    Task& T = tasks[blockIdx.x];
-    if (threadIdx.x != 0) return;
-    if (pairwise && T.has_groups) {
-        double acc = 0.0;
-        int ng = 0;
-        for (int g = 0; g < T.n_groups; ++g) {
-            const double v = T.loss_part[2 * LOSS_BLOCKS + g];
-            if (v == v) {
-                acc += v;
-                ng += 1;
-            }
-        }
-        T.train_loss = ng > 0 ? -acc / ng : 0.0;
-        return;
-    }
-    double s = 0.0, sw = 0.0;
-    for (int b = 0; b < LOSS_BLOCKS; ++b) {
-        s += T.loss_part[2 * b];
-        sw += T.loss_part[2 * b + 1];
-    }
-    T.train_loss = sw > 0 ? s / sw : 0.0;
-}
-
-void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int pairwise, int max_groups) {
-    if (pairwise)
-        hipLaunchKernelGGL(k_loss_groups, dim3(std::max(1, (max_groups + 255) / 256), n_tasks), dim3(256), 0, st,
-                           tasks);
-    else
-        hipLaunchKernelGGL(k_loss_mse, dim3(LOSS_BLOCKS, n_tasks), dim3(256), 0, st, tasks);
-    hipLaunchKernelGGL(k_loss_final, dim3(n_tasks), dim3(64), 0, st, tasks, pairwise);
+void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int max_groups) {
+    hipLaunchKernelGGL(k_loss_groups, dim3(std::max(1, (max_groups + 255) / 256), n_tasks), dim3(256), 0, st, tasks);
 }
 
 // Finished trees of all active tasks packed into one staging buffer (one copy to the host):
 // task i at out + i*L.stride holds n_nodes, nodes[0..nn), values[0..nn), cover[0..nn) and
 // prefix[0..nn*n_buckets) at the offsets of L.
-__global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, char* out) {
-    const Task& T = tasks[blockIdx.x];
+// The header also carries the tree's training loss: the in-order mean of the group correlations
+// (pairwise with groups; minus their mean, as objective.cpp:320-321 reports it), else the chunk
+// sums of k_pred_update added in a fixed order.
+__global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, char* out, int pairwise) {
+    Task& T = tasks[blockIdx.x];
     char* base = out + (size_t)blockIdx.x * L.stride;
     const int nn = T.n_nodes;
+    double loss = 0.0;
+    if (pairwise && T.has_groups) {
+        if (threadIdx.x == 0) {
+            const double* gv = T.loss_part + 2 * loss_chunks(T.cap_rows);
+            double acc = 0.0;
+            int ng = 0;
+            for (int g = 0; g < T.n_groups; ++g) {
+                const double v = gv[g];
+                if (v == v) {
+                    acc += v;
+                    ng += 1;
+                }
+            }
+            loss = ng > 0 ? -acc / ng : 0.0;
+        }
+    } else {
+        double a = 0.0, b = 0.0;
+        const int nb = loss_chunks(T.n_rows);
+        for (int j = threadIdx.x; j < nb; j += blockDim.x) {
+            a += T.loss_part[2 * j];
+            b += T.loss_part[2 * j + 1];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, 64);
+            b += __shfl_down(b, o, 64);
+        }
+        __shared__ double sa[4], sb[4];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) {
+            sa[wave] = a;
+            sb[wave] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double s = ((sa[0] + sa[1]) + sa[2]) + sa[3], sw = ((sb[0] + sb[1]) + sb[2]) + sb[3];
+            loss = sw > 0 ? s / sw : 0.0;
+        }
+    }
     if (threadIdx.x == 0) {
+        T.train_loss = loss;
         *(int*)base = nn;
         *((int*)base + 1) = T.err;
-        *((double*)base + 1) = T.train_loss;   // header bytes 8..15
+        *((double*)base + 1) = loss;   // header bytes 8..15
     }
     const int* src = (const int*)T.nodes;
     int* dst = (int*)(base + L.off_nodes);
@@ -2093,8 +2100,8 @@ __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, cha
     for (int j = threadIdx.x; j < np; j += blockDim.x) pre[j] = T.prefix[j];
 }
 
-void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out) {
-    hipLaunchKernelGGL(k_export, dim3(n_tasks), dim3(256), 0, st, tasks, L, out);
+void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out, int pairwise) {
+    hipLaunchKernelGGL(k_export, dim3(n_tasks), dim3(256), 0, st, tasks, L, out, pairwise);
 }
 
 // Dynamic LDS above 64 KiB needs a per-kernel opt-in; done once, outside any stream capture.

@@ -39,10 +39,11 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
                  bool hconst, const GrowP& P, FCand* out);
 void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
-                     int max_chunks, const GrowP& P, double min_cover, int iid, double lr);
-void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out);
-// Training loss of every active task from its updated predictions into Task::train_loss.
-void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int pairwise, int max_groups);
+                     int max_chunks, const GrowP& P, double min_cover, int iid, double lr, int pairwise);
+void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout& L, char* out, int pairwise);
+// Group correlations of the pairwise training loss from the updated predictions (k_export
+// finishes the loss; the mean squared error comes from the prediction update).
+void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int max_groups);
 // GOSS selection (see goss.hip): temp storage bytes for n rows, and the launch sequence.
 size_t goss_temp_bytes(int n);
 hipError_t launch_goss(hipStream_t st, const double* g, int n, int n_top, int n_other, const int* perm,

@@ -945,17 +945,12 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
     return dev_guard([&] {
         const GrowP P = s->growp(p);
         bool walk = !s->all_contiguous();
-        EventPair ep;
-        if (s->timing) s->tic(3, 0.0, ep);
-        launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
-                        (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
-                        min_cover, iid, learning_rate);
-        // training loss from the updated predictions (exported with the tree)
+        // training-loss partials (the prediction update and the pairwise group pass write them)
         const bool pairwise = s->objective == 5;
-        int max_groups = 1;
+        int max_groups = 0;
         for (int t = s->a0; t < s->a0 + s->an; ++t) {
             TaskHost& H = s->th[t];
-            const int need = 2 * LOSS_BLOCKS + H.n_groups;
+            const int need = 2 * loss_chunks(H.cap_rows) + H.n_groups;
             if (need > H.loss_cap) {
                 H.loss_d = s->alloc<double>((size_t)need);
                 H.loss_cap = need;
@@ -964,7 +959,12 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
             }
             max_groups = std::max(max_groups, H.n_groups);
         }
-        launch_loss(s->stream, s->td(), s->an, pairwise ? 1 : 0, max_groups);
+        EventPair ep;
+        if (s->timing) s->tic(3, 0.0, ep);
+        launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
+                        (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
+                        min_cover, iid, learning_rate, pairwise ? 1 : 0);
+        if (pairwise && max_groups > 0) launch_loss(s->stream, s->td(), s->an, max_groups);
         if (s->timing) s->toc(ep);
         HIP_OK(hipGetLastError());
         int bmax = 1;
@@ -978,7 +978,7 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
             HIP_OK(hipHostMalloc((void**)&s->export_h, need, hipHostMallocDefault));
             s->export_bytes = need;
         }
-        launch_export(s->stream, s->td(), s->an, L, s->export_d);
+        launch_export(s->stream, s->td(), s->an, L, s->export_d, pairwise ? 1 : 0);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(s->export_h, s->export_d, L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));

@@ -67,20 +67,21 @@ void HistTuner::record(int tree, double seconds) {
     const int round = (tree - base) / (int)c.size(), i = (tree - base) % (int)c.size();
     std::vector<double>& t = tree < stage2_start_ ? t1_ : t2_;
     if (round > 0 && seconds < t[i]) t[i] = seconds;
+    // the fastest candidate, if it beats the stage's first candidate by the margin
+    auto pick = [](const std::vector<double>& t) {
+        size_t b = 0;
+        for (size_t k = 1; k < t.size(); ++k)
+            if (t[k] < t[b]) b = k;
+        return t[b] < (1.0 - kMargin) * t[0] ? b : 0;
+    };
     if (tree == stage2_start_ - 1) {
         // stage 1 done: the best child tiles, then the root variants with them
-        size_t b = 0;
-        for (size_t k = 1; k < t1_.size(); ++k)
-            if (t1_[k] < t1_[b]) b = k;
-        const int child = c1_[b];
+        const int child = c1_[pick(t1_)];
         c2_ = {child, child | kRootFpb16, child | kRootFpb32, child | kRootFpb64};
         t2_.assign(c2_.size(), std::numeric_limits<double>::infinity());
         best_ = child;
     } else if (tree == end_ - 1) {
-        size_t b = 0;
-        for (size_t k = 1; k < t2_.size(); ++k)
-            if (t2_[k] < t2_[b]) b = k;
-        best_ = c2_[b];
+        best_ = c2_[pick(t2_)];
         std::lock_guard<std::mutex> lk(g_mu);
         cache()[key_] = best_;
     }

@@ -10,9 +10,10 @@ namespace tide {
 // trees of a fit are grown with different candidates: each candidate grows one tree that captures
 // its launch graph, then two timed trees, in rounds over the candidates (so their timed trees sit
 // at similar places of the boosting sequence); the candidate with the least timed tree is kept
-// for the remaining trees.  Two stages: child-build tiles with the default root build, then
-// root-build variants with the best child tiles.  A choice is remembered per workload shape for
-// later fits of the process.
+// for the remaining trees when it beats the stage's first candidate (the defaults) by kMargin,
+// since tree times vary by a few percent from tree to tree.  Two stages: child-build tiles with
+// the default root build, then root-build variants with the best child tiles.  A choice is
+// remembered per workload shape for later fits of the process.
 class HistTuner {
   public:
     // key: the workload shape; n_trees: trees of the fit; enabled: tuning allowed (the caller's
@@ -25,7 +26,8 @@ class HistTuner {
     static constexpr int kMinTrees = 54;
 
   private:
-    static constexpr int kRounds = 3;          // one capture round, two timed rounds
+    static constexpr int kRounds = 4;          // one capture round, three timed rounds
+    static constexpr double kMargin = 0.03;    // a candidate must be this much faster than the first
     std::string key_;
     bool active_ = false;
     int best_ = 0;
