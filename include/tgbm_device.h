@@ -147,6 +147,15 @@ int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d,
 int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int iid,
                   const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
                   double* values, int* cover, double* prefix);
+/* tgbm_tree_end in two halves, so the host can enqueue the next tree before it reads this one:
+ * tgbm_tree_end_async enqueues the tree end and the copy of the finished trees (at most two may
+ * wait); tgbm_tree_collect waits for the oldest and unpacks it like tgbm_tree_end.  Between the
+ * two only gradient, tree-begin, device-growth and further async tree-end calls may be made
+ * (host-driven growth and the task readers need the pending trees collected first). */
+int tgbm_tree_end_async(tgbm_session* s, double learning_rate, double min_cover, int iid,
+                        const tgbm_grow_params* p);
+int tgbm_tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_nodes, int* node_ints,
+                      double* node_dbl, double* values, int* cover, double* prefix);
 
 int tgbm_task_get_pred(tgbm_session* s, int task, double* pred);
 /* Training loss of `task` after its last finished tree (reference Objective::loss,

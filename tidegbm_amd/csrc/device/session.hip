@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -117,8 +118,15 @@ struct tgbm_session {
     size_t small_scratch_bytes = 0;
     // finished-tree staging (device + pinned host), sized for every task at the largest bucket count
     char* export_d = nullptr;
-    char* export_h = nullptr;
+    char* export_h[2] = {nullptr, nullptr};   // two host slots: a tree is read while the next runs
+    hipEvent_t export_ev[2] = {nullptr, nullptr};
     size_t export_bytes = 0;
+    struct PendingExport {
+        int slot, a0, an;
+        ExportLayout L;
+    };
+    std::deque<PendingExport> pending;       // enqueued tree ends not yet collected (at most 2)
+    int next_slot = 0;
     // timers
     bool timing = false;
     std::vector<EventPair> events;
@@ -144,7 +152,10 @@ struct tgbm_session {
             (void)hipEventDestroy(e.b);
         }
         for (void* p : allocs) (void)hipFree(p);
-        if (export_h) (void)hipHostFree(export_h);
+        for (int k = 0; k < 2; ++k) {
+            if (export_h[k]) (void)hipHostFree(export_h[k]);
+            if (export_ev[k]) (void)hipEventDestroy(export_ev[k]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
     void set_field_int(int t, size_t off, int v) {
@@ -442,6 +453,7 @@ int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long lo
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         const Task& T = s->tasks_h[task];
         if (!T.nmask) throw std::runtime_error("the session has no categorical feature");
+        if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
         if (n_nodes < 0 || n_nodes > MAX_NODES) throw std::runtime_error("node count out of range");
         HIP_OK(hipMemcpyAsync(out, T.nmask, sizeof(unsigned long long) * 4 * n_nodes, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -827,6 +839,7 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
         const std::vector<long long> key{s->launch_rows(), p->max_leaves, p->hist_mode, s->a0, s->an};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
+            if (!s->pending.empty()) HIP_OK(hipStreamSynchronize(s->stream));   // capture on an idle stream
             if (s->graphs.size() >= 256) {  // bound the cache; captures are cheap to redo
                 for (auto& kv : s->graphs) {
                     HIP_OK(hipGraphExecDestroy(kv.second.second));
@@ -939,86 +952,123 @@ int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tg
     });
 }
 
+// Enqueue the end of every active task's tree: leaf table, TORB sums, prediction update, loss,
+// export into the next host slot (the copy completes in stream order; an event marks it).
+static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_cover, int iid,
+                             const tgbm_grow_params* p) {
+    if (s->pending.size() >= 2) throw std::runtime_error("at most two tree ends may wait for collection");
+    const GrowP P = s->growp(p);
+    bool walk = !s->all_contiguous();
+    // training-loss partials (the prediction update and the pairwise group pass write them)
+    const bool pairwise = s->objective == 5;
+    int max_groups = 0;
+    for (int t = s->a0; t < s->a0 + s->an; ++t) {
+        TaskHost& H = s->th[t];
+        const int need = 2 * loss_chunks(H.cap_rows) + H.n_groups;
+        if (need > H.loss_cap) {
+            H.loss_d = s->alloc<double>((size_t)need);
+            H.loss_cap = need;
+            s->tasks_h[t].loss_part = H.loss_d;
+            s->set_field_ptr(t, offsetof(Task, loss_part), H.loss_d);
+        }
+        max_groups = std::max(max_groups, H.n_groups);
+    }
+    EventPair ep;
+    if (s->timing) s->tic(3, 0.0, ep);
+    launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
+                    (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
+                    min_cover, iid, learning_rate, pairwise ? 1 : 0);
+    if (pairwise && max_groups > 0) launch_loss(s->stream, s->td(), s->an, max_groups);
+    if (s->timing) s->toc(ep);
+    HIP_OK(hipGetLastError());
+    int bmax = 1;
+    for (int t = 0; t < s->T; ++t) bmax = std::max(bmax, s->th[t].n_buckets);
+    const ExportLayout L = export_layout(s->max_nodes, bmax);
+    const size_t need = L.stride * s->T;
+    if (need > s->export_bytes) {
+        if (!s->pending.empty()) HIP_OK(hipStreamSynchronize(s->stream));   // the old slots stay readable below
+        for (int k = 0; k < 2; ++k) {
+            if (s->export_h[k] && s->pending.empty()) HIP_OK(hipHostFree(s->export_h[k]));
+            else if (s->export_h[k]) throw std::runtime_error("internal: export slots resized with a tree pending");
+            s->export_h[k] = nullptr;
+            HIP_OK(hipHostMalloc((void**)&s->export_h[k], need, hipHostMallocDefault));
+            if (!s->export_ev[k]) HIP_OK(hipEventCreateWithFlags(&s->export_ev[k], hipEventDisableTiming));
+        }
+        s->export_d = s->alloc<char>(need);
+        s->export_bytes = need;
+    }
+    launch_export(s->stream, s->td(), s->an, L, s->export_d, pairwise ? 1 : 0);
+    HIP_OK(hipGetLastError());
+    const int slot = s->next_slot;
+    HIP_OK(hipMemcpyAsync(s->export_h[slot], s->export_d, L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipEventRecord(s->export_ev[slot], s->stream));
+    s->pending.push_back(tgbm_session::PendingExport{slot, s->a0, s->an, L});
+    s->next_slot ^= 1;
+}
+
+// Wait for the oldest enqueued tree end and unpack its trees (one per task of its active range).
+static void tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
+                         double* values, int* cover, double* prefix) {
+    if (s->pending.empty()) throw std::runtime_error("no tree end to collect");
+    const tgbm_session::PendingExport pe = s->pending.front();
+    s->pending.pop_front();
+    HIP_OK(hipEventSynchronize(s->export_ev[pe.slot]));
+    const ExportLayout& L = pe.L;
+    const int MN = s->max_nodes;
+    for (int i = 0; i < pe.an; ++i) {
+        const int t = pe.a0 + i;
+        const char* base = s->export_h[pe.slot] + L.stride * i;
+        const int nn = *(const int*)base;
+        if (nn < 1 || nn > MN) throw std::runtime_error("internal: exported tree size out of range");
+        if (*((const int*)base + 1) != 0)
+            throw std::runtime_error("internal: a partition placed a different number of rows left than its split's histogram counts");
+        n_nodes[i] = nn;
+        s->th[t].last_loss = *((const double*)base + 1);
+        const int B = s->th[t].n_buckets;
+        const DNode* nodes = (const DNode*)(base + L.off_nodes);
+        for (int k = 0; k < nn; ++k) {
+            std::memcpy(node_ints + ((size_t)i * MN + k) * TGBM_NODE_INTS, nodes[k].v, sizeof(int) * 10);
+            node_dbl[((size_t)i * MN + k) * 2] = nodes[k].value;
+            node_dbl[((size_t)i * MN + k) * 2 + 1] = nodes[k].gain;
+        }
+        std::memcpy(values + (size_t)i * MN, base + L.off_values, sizeof(double) * nn);
+        std::memcpy(cover + (size_t)i * MN, base + L.off_cover, sizeof(int) * nn);
+        std::memcpy(prefix + (size_t)i * MN * MAX_BUCKETS, base + L.off_prefix, sizeof(double) * nn * B);
+        if (s->timing) {  // rows of the root and of every child histogram build, in launch order
+            const int steps = p->max_leaves - 1;
+            s->log_rows[0].push_back(nodes[0].seg_count);
+            for (int k = 0; k < steps; ++k) {
+                const bool grew = 2 * k + 2 < nn;
+                s->log_rows[5].push_back(grew ? std::min(nodes[2 * k + 1].seg_count, nodes[2 * k + 2].seg_count) : 0);
+                s->log_rows[1].push_back(grew ? nodes[2 * k + 1].seg_count + nodes[2 * k + 2].seg_count : 0);
+            }
+            for (int k = 0; k < nn; ++k) {
+                if (nodes[k].v[N_LEAF]) continue;
+                const int Lc = nodes[k].v[N_LEFT], Rc = nodes[k].v[N_RIGHT];
+                const double m = std::min(nodes[Lc].seg_count, nodes[Rc].seg_count);
+                s->t_bytes[5] += m * s->F + m * (s->hconst ? 8.0 : 16.0) + 4.0 * m + (double)s->F * s->nb * 24.0;
+            }
+        }
+    }
+}
+
 int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int iid,
                   const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
                   double* values, int* cover, double* prefix) {
     return dev_guard([&] {
-        const GrowP P = s->growp(p);
-        bool walk = !s->all_contiguous();
-        // training-loss partials (the prediction update and the pairwise group pass write them)
-        const bool pairwise = s->objective == 5;
-        int max_groups = 0;
-        for (int t = s->a0; t < s->a0 + s->an; ++t) {
-            TaskHost& H = s->th[t];
-            const int need = 2 * loss_chunks(H.cap_rows) + H.n_groups;
-            if (need > H.loss_cap) {
-                H.loss_d = s->alloc<double>((size_t)need);
-                H.loss_cap = need;
-                s->tasks_h[t].loss_part = H.loss_d;
-                s->set_field_ptr(t, offsetof(Task, loss_part), H.loss_d);
-            }
-            max_groups = std::max(max_groups, H.n_groups);
-        }
-        EventPair ep;
-        if (s->timing) s->tic(3, 0.0, ep);
-        launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
-                        (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
-                        min_cover, iid, learning_rate, pairwise ? 1 : 0);
-        if (pairwise && max_groups > 0) launch_loss(s->stream, s->td(), s->an, max_groups);
-        if (s->timing) s->toc(ep);
-        HIP_OK(hipGetLastError());
-        int bmax = 1;
-        for (int t = 0; t < s->T; ++t) bmax = std::max(bmax, s->th[t].n_buckets);
-        const ExportLayout L = export_layout(s->max_nodes, bmax);
-        const size_t need = L.stride * s->T;
-        if (need > s->export_bytes) {
-            if (s->export_h) HIP_OK(hipHostFree(s->export_h));
-            s->export_h = nullptr;
-            s->export_d = s->alloc<char>(need);
-            HIP_OK(hipHostMalloc((void**)&s->export_h, need, hipHostMallocDefault));
-            s->export_bytes = need;
-        }
-        launch_export(s->stream, s->td(), s->an, L, s->export_d, pairwise ? 1 : 0);
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(s->export_h, s->export_d, L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
-        HIP_OK(hipStreamSynchronize(s->stream));
-        const int MN = s->max_nodes;
-        for (int i = 0; i < s->an; ++i) {
-            const int t = s->a0 + i;
-            const char* base = s->export_h + L.stride * i;
-            const int nn = *(const int*)base;
-            if (nn < 1 || nn > MN) throw std::runtime_error("internal: exported tree size out of range");
-            if (*((const int*)base + 1) != 0)
-                throw std::runtime_error("internal: a partition placed a different number of rows left than its split's histogram counts");
-            n_nodes[i] = nn;
-            s->th[t].last_loss = *((const double*)base + 1);
-            const int B = s->th[t].n_buckets;
-            const DNode* nodes = (const DNode*)(base + L.off_nodes);
-            for (int k = 0; k < nn; ++k) {
-                std::memcpy(node_ints + ((size_t)i * MN + k) * TGBM_NODE_INTS, nodes[k].v, sizeof(int) * 10);
-                node_dbl[((size_t)i * MN + k) * 2] = nodes[k].value;
-                node_dbl[((size_t)i * MN + k) * 2 + 1] = nodes[k].gain;
-            }
-            std::memcpy(values + (size_t)i * MN, base + L.off_values, sizeof(double) * nn);
-            std::memcpy(cover + (size_t)i * MN, base + L.off_cover, sizeof(int) * nn);
-            std::memcpy(prefix + (size_t)i * MN * MAX_BUCKETS, base + L.off_prefix, sizeof(double) * nn * B);
-            if (s->timing) {  // rows of the root and of every child histogram build, in launch order
-                const int steps = p->max_leaves - 1;
-                s->log_rows[0].push_back(nodes[0].seg_count);
-                for (int k = 0; k < steps; ++k) {
-                    const bool grew = 2 * k + 2 < nn;
-                    s->log_rows[5].push_back(grew ? std::min(nodes[2 * k + 1].seg_count, nodes[2 * k + 2].seg_count) : 0);
-                    s->log_rows[1].push_back(grew ? nodes[2 * k + 1].seg_count + nodes[2 * k + 2].seg_count : 0);
-                }
-                for (int k = 0; k < nn; ++k) {
-                    if (nodes[k].v[N_LEAF]) continue;
-                    const int Lc = nodes[k].v[N_LEFT], Rc = nodes[k].v[N_RIGHT];
-                    const double m = std::min(nodes[Lc].seg_count, nodes[Rc].seg_count);
-                    s->t_bytes[5] += m * s->F + m * (s->hconst ? 8.0 : 16.0) + 4.0 * m + (double)s->F * s->nb * 24.0;
-                }
-            }
-        }
+        if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
+        tree_end_enqueue(s, learning_rate, min_cover, iid, p);
+        tree_collect(s, p, n_nodes, node_ints, node_dbl, values, cover, prefix);
     });
+}
+
+int tgbm_tree_end_async(tgbm_session* s, double learning_rate, double min_cover, int iid, const tgbm_grow_params* p) {
+    return dev_guard([&] { tree_end_enqueue(s, learning_rate, min_cover, iid, p); });
+}
+
+int tgbm_tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
+                      double* values, int* cover, double* prefix) {
+    return dev_guard([&] { tree_collect(s, p, n_nodes, node_ints, node_dbl, values, cover, prefix); });
 }
 
 int tgbm_task_train_loss(tgbm_session* s, int task, double* loss) {

@@ -981,7 +981,14 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
         tree_end_times.clear();
         evals_train.clear();   // booster.cpp:481: a fit starts a new eval record
     }
-    for (int m = 0; m < n_trees; ++m) {
+    // Device-driven fits without GOSS or an eval set enqueue tree m + 1 before reading tree m back
+    // (its gradients follow tree m's prediction update in stream order), so the device does not
+    // wait for the host between trees.  Trees whose variant the tuner has yet to decide run one
+    // at a time.
+    const bool pipeline = !host_driven && !goss && !eval_run_ && !for_update;
+    const int iid = (time_mode && !for_update) ? 0 : 1;
+    std::vector<double> t_enq(n_trees, 0.0);
+    auto enqueue = [&](int m) {
         dev_check(tgbm_task_grad(s, -1, cfg.objective));
         if (goss) {
             const double t0 = now_s();
@@ -1003,14 +1010,19 @@ void Booster::grow_trees(tgbm_session* s, int n, int n_trees, bool time_mode, co
             (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample would use it
         }
         if (cfg.hist_mode == 0) gp.hist_mode = tuner.mode(m);
-        const double t_tree = now_s();
+        t_enq[m] = now_s();
         dev_check(tgbm_tree_begin(s, &gp));
         if (host_driven) grow_host_driven(s, cfg, F, gp, is_cat, group_of);
         else dev_check(tgbm_tree_grow_device(s, &gp));
+        dev_check(tgbm_tree_end_async(s, cfg.learning_rate, min_cover, iid, &gp));
+    };
+    int enqueued = 0;
+    for (int m = 0; m < n_trees; ++m) {
+        if (enqueued == m) enqueue(enqueued++);
+        if (pipeline && enqueued == m + 1 && m + 1 < n_trees && tuner.settled(m + 1)) enqueue(enqueued++);
         int nn = 0;
-        dev_check(tgbm_tree_end(s, cfg.learning_rate, min_cover, (time_mode && !for_update) ? 0 : 1, &gp, &nn,
-                                ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
-        tuner.record(m, now_s() - t_tree);   // tree_end synchronized
+        dev_check(tgbm_tree_collect(s, &gp, &nn, ints.data(), dbl.data(), vals.data(), cov.data(), pref.data()));
+        tuner.record(m, now_s() - t_enq[m]);   // the collect waited for tree m
         if (!for_update) {
             double loss = 0.0;
             dev_check(tgbm_task_train_loss(s, 0, &loss));

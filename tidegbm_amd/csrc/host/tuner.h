@@ -22,6 +22,8 @@ class HistTuner {
     int mode(int tree) const;                  // variant for tree `tree`
     void record(int tree, double seconds);     // wall time of tree `tree`
     int chosen() const { return best_; }
+    // the variant of tree `tree` no longer depends on timings still to be recorded
+    bool settled(int tree) const { return !active_ || tree >= end_; }
     // trees a tuned fit needs (shorter fits keep the defaults)
     static constexpr int kMinTrees = 54;
 
