@@ -101,10 +101,24 @@ def pack(m, p):
     )
 
 
+def toolchain():
+    """The library versions whose implementation-defined behaviour the fixtures capture
+    (std::sort / std::shuffle tie orders in libstdc++, exp in glibc)."""
+    import subprocess
+    out = []
+    for cmd in (["g++", "--version"], ["ldd", "--version"]):
+        try:
+            out.append(subprocess.run(cmd, capture_output=True, text=True).stdout.splitlines()[0])
+        except (OSError, IndexError):
+            out.append(f"{cmd[0]}: unavailable")
+    return "\n".join(out) + "\n"
+
+
 def main():
     ref = checkers.reference()
     if ref is None:
         raise SystemExit("oracle/_ref/libtide_ref.so is not built")
+    (OUT / "TOOLCHAIN.txt").write_text(toolchain())
     for name, p, rg, iid, h, params, opts in cases():
         np.savez_compressed(OUT / f"{name}.npz", **pack(fit(ref, name, p, rg, iid, h, params, opts), p))
         print("wrote", name)
