@@ -246,6 +246,10 @@ __device__ inline unsigned long long read_lane_u64(unsigned long long v, int lan
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
     return (unsigned long long)hi << 32 | lo;
 }
+// Lane `lane`'s double, in every lane (lane must be wave-uniform).
+__device__ inline double read_lane_f64(double v, int lane) {
+    return __longlong_as_double((long long)read_lane_u64((unsigned long long)__double_as_longlong(v), lane));
+}
 // Inclusive scan over the lanes with an associative operation (Hillis-Steele in each row, then
 // the row totals): lane 63 ends with the whole wave's result.
 template <class Op>

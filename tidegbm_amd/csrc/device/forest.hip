@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64) void k_eval_mse(const double* __restrict__ y, c
             t = 1.0 * d * d;
         }
         const int m = min(64, n - base);
-        for (int k = 0; k < m; ++k) s += __shfl(t, k, 64);
+        for (int k = 0; k < m; ++k) s += read_lane_f64(t, k);
     }
     if (lane == 0) out[0] = n > 0 ? s / (double)n : 0.0;
 }

@@ -1995,40 +1995,59 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
 }
 
 // =================================================================== training loss
-// Pairwise objective with groups (objective.cpp:291-321): one thread per group computes the
-// group's Pearson correlation with the reference's sequential sums; k_export averages the groups
-// in order.  The mean squared error of the other objectives comes from k_pred_update's chunks.
-__global__ __launch_bounds__(256) void k_loss_groups(Task* tasks) {
+// Pairwise objective with groups (objective.cpp:291-321): one wave per group computes the
+// group's Pearson correlation with the reference's sequential sums.  The wave stages the group's
+// rows in LDS (their loads all in flight), then adds them in row order (every lane holds the
+// same running sums), so the sums match the reference's loop.  k_export averages the groups in
+// order.  The mean squared error of the other objectives comes from k_pred_update.
+__global__ __launch_bounds__(64) void k_loss_groups(Task* tasks) {
     const Task& T = tasks[blockIdx.y];
-    const int grp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int grp = blockIdx.x;
     if (grp >= T.n_groups) return;
+    constexpr int STAGE = 1024;   // rows staged in LDS at a time (all their loads in flight)
+    __shared__ double ys[STAGE], ps[STAGE];
     const int g0 = T.group_start[grp], m = T.group_start[grp + 1] - g0;
     const int* gr = T.group_rows ? T.group_rows + g0 : nullptr;
     double val = __longlong_as_double(0x7ff8000000000000ll);   // NaN: the group does not count
     if (m >= 3) {
+        auto stage = [&](int base, int cnt) {
+            __syncthreads();
+            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+                const int r = gr ? gr[base + k] : g0 + base + k;
+                ys[k] = T.y[r];
+                ps[k] = T.pred[r];
+            }
+            __syncthreads();
+        };
         double my = 0.0, mp = 0.0;
-        for (int k = 0; k < m; ++k) {
-            const int r = gr ? gr[k] : g0 + k;
-            my += T.y[r];
-            mp += T.pred[r];
+        for (int base = 0; base < m; base += STAGE) {
+            const int cnt = min(STAGE, m - base);
+            stage(base, cnt);
+            for (int k = 0; k < cnt; ++k) {
+                my += ys[k];
+                mp += ps[k];
+            }
         }
         my /= m;
         mp /= m;
         double vy = 0.0, vp = 0.0, c = 0.0;
-        for (int k = 0; k < m; ++k) {
-            const int r = gr ? gr[k] : g0 + k;
-            const double dy = T.y[r] - my, dp = T.pred[r] - mp;
-            vy += dy * dy;
-            vp += dp * dp;
-            c += dy * dp;
+        for (int base = 0; base < m; base += STAGE) {
+            const int cnt = min(STAGE, m - base);
+            if (base > 0 || m > STAGE) stage(base, cnt);   // a group of <= STAGE rows is still staged
+            for (int k = 0; k < cnt; ++k) {
+                const double dy = ys[k] - my, dp = ps[k] - mp;
+                vy += dy * dy;
+                vp += dp * dp;
+                c += dy * dp;
+            }
         }
         if (vy > 1e-18 && vp > 1e-18) val = c / sqrt(vy * vp);
     }
-    T.loss_part[2 * loss_chunks(T.cap_rows) + grp] = val;
+    if (threadIdx.x == 0) T.loss_part[2 * loss_chunks(T.cap_rows) + grp] = val;
 }
 
 void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int max_groups) {
-    hipLaunchKernelGGL(k_loss_groups, dim3(std::max(1, (max_groups + 255) / 256), n_tasks), dim3(256), 0, st, tasks);
+    hipLaunchKernelGGL(k_loss_groups, dim3(std::max(1, max_groups), n_tasks), dim3(64), 0, st, tasks);
 }
 
 // Finished trees of all active tasks packed into one staging buffer (one copy to the host):
@@ -2043,19 +2062,29 @@ __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, cha
     const int nn = T.n_nodes;
     double loss = 0.0;
     if (pairwise && T.has_groups) {
-        if (threadIdx.x == 0) {
-            const double* gv = T.loss_part + 2 * loss_chunks(T.cap_rows);
-            double acc = 0.0;
-            int ng = 0;
-            for (int g = 0; g < T.n_groups; ++g) {
-                const double v = gv[g];
-                if (v == v) {
-                    acc += v;
-                    ng += 1;
+        // the block stages 2048 group values at a time in LDS (all loads in flight at once);
+        // thread 0 adds them in group order
+        constexpr int STAGE = 2048;
+        __shared__ double gvs[STAGE];
+        const double* gv = T.loss_part + 2 * loss_chunks(T.cap_rows);
+        const int ngr = T.n_groups;
+        double acc = 0.0;
+        int ng = 0;
+        for (int base = 0; base < ngr; base += STAGE) {
+            const int cnt = min(STAGE, ngr - base);
+            for (int j = threadIdx.x; j < cnt; j += blockDim.x) gvs[j] = gv[base + j];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int j = 0; j < cnt; ++j) {
+                    const double v = gvs[j];
+                    if (v == v) {
+                        acc += v;
+                        ng += 1;
+                    }
                 }
-            }
-            loss = ng > 0 ? -acc / ng : 0.0;
+            __syncthreads();
         }
+        loss = ng > 0 ? -acc / ng : 0.0;
     } else {
         double a = 0.0, b = 0.0;
         const int nb = loss_chunks(T.n_rows);
