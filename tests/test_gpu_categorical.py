@@ -128,3 +128,25 @@ def test_groups_interactions_regime_match_oracle(case):
         assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < PRED_RTOL
     types = np.concatenate([t["ints"][t["ints"][:, 0] == 0, 1] for t in gpu.trees()])
     _say(f"  split types: {np.bincount(types, minlength=4).tolist()}")
+
+
+def test_categorical_with_eval_set():
+    # the eval rows are binned with the vocabularies and scored through the categorical masks
+    p, is_cat = cat_panel(37)
+    dates = np.unique(p.ts)
+    cut = dates[int(len(dates) * 0.7)]
+    tr, va = p.ts < cut, p.ts >= cut + 1
+    params = dict(n_estimators=10, max_depth=5, learning_rate=0.1, goss=0, pair_splits=0, early_stopping_rounds=0)
+    gpu = Booster(eval_metric="loss", **params)
+    gpu.fit(p.X[tr], p.y[tr], timestamps=p.ts[tr], entity=p.entity[tr], horizon=1, is_cat=is_cat,
+            eval_set=(p.X[va], p.y[va]), eval_timestamp=p.ts[va])
+    m = checkers.oracle().model(eval_metric=1, **params)
+    m.set_cat(is_cat)
+    m.set_eval(p.X[va], p.y[va], p.ts[va])
+    m.fit(p.X[tr], p.y[tr], ts=p.ts[tr], entity=p.entity[tr], horizon=1)
+    ev, eo = gpu.evals_result()["valid"], m.evals_valid()
+    assert len(ev) == len(eo) == 10
+    assert np.allclose(ev, eo, rtol=1e-6, atol=0), (ev, eo)
+    assert gpu.best_iteration == m.best_iteration() == 1
+    a, b = gpu.predict(p.X[va]), m.predict(p.X[va])
+    assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-3)) < PRED_RTOL
