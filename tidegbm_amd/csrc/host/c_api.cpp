@@ -365,6 +365,9 @@ int tide_fit_batch(TideHandle proto, const double* X, int n, int n_features, con
     return guard([&] {
         if (!p || !X || !task_rows || !y || !out) throw tide::Error("null argument");
         if (p->n_eval > 0) throw tide::Error("eval sets are not supported by batched fits");
+        if (!p->group_of.empty() || !p->cfg.interaction_group.empty() || p->cfg.regime_feature >= 0)
+            throw tide::Error("feature groups, interaction groups and a regime feature need the host-driven grower; "
+                              "batched fits grow on the device");
         std::vector<std::unique_ptr<Booster>> res;
         tide::fit_batch(p->cfg, X, n, n_features, timestamps, entity, n_tasks, task_rows, y, horizons, schedule, res);
         for (int t = 0; t < n_tasks; ++t) out[t] = res[t].release();

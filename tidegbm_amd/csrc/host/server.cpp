@@ -86,6 +86,8 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     for (const Booster* m : models) {
         if (!m || !m->fitted) throw Error("every ensemble must be fitted");
         if (m->has_cat()) throw Error("the serving kernel does not route categorical splits");
+        if (!m->cfg.interaction_group.empty() || m->cfg.regime_feature >= 0)
+            throw Error("serving refits grow on the device: ensembles need no interaction groups or regime feature");
         if (m->n_features != F) throw Error("ensemble and panel feature counts differ");
         if (m->edges != m0.edges || m->n_bins != m0.n_bins)
             throw Error("all ensembles of a server must share bin edges (one binned panel serves them all)");
