@@ -100,6 +100,11 @@ _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_chil
                 "max_delta_step", "torb_min_cover", "time_decay", "tree_decay"}
 
 
+#: Every configuration key Booster.set_params accepts.
+KNOWN_KEYS = frozenset(_INT_KEYS | _DOUBLE_KEYS | {"device", "eval_metric", "objective", "interaction_group",
+                                                   "feature_groups"})
+
+
 class TideError(ValueError):
     pass
 
@@ -553,4 +558,4 @@ class TideRanker(TideRegressor):
 
 
 __all__ = ["Booster", "TideRegressor", "TideRanker", "TideError", "gpu_compiled", "cuda_compiled",
-           "device_count", "parse_horizon", "tree_arrays", "OBJECTIVES"]
+           "device_count", "parse_horizon", "tree_arrays", "OBJECTIVES", "KNOWN_KEYS", "Server", "fit_batch"]
