@@ -93,3 +93,43 @@ def test_parse_horizon_units():
     assert t.parse_horizon(3, ts) == 15
     assert t.parse_horizon(None, ts) == 0
     assert t.parse_horizon(2.0, ts) == 10
+
+
+def test_extension_setters_validate_arguments():
+    h = ctypes.c_void_p()
+    L = t._lib
+    assert L.tide_create(ctypes.byref(h)) == 0
+    try:
+        # eval set: X and y are required for n_eval > 0; n_eval = 0 clears
+        assert L.tide_set_eval(h, None, None, None, 5, 3) == -1
+        assert b"eval set" in L.tide_last_error()
+        X = np.zeros((4, 3))
+        y = np.zeros(4)
+        assert L.tide_set_eval(h, X.ctypes.data, y.ctypes.data, None, 4, 3) == 0
+        assert L.tide_set_eval(h, None, None, None, 0, 0) == 0
+        # groups: a pointer is required for n > 0
+        assert L.tide_set_feature_groups(h, None, 3) == -1
+        assert L.tide_set_interaction_groups(h, None, 3) == -1
+        g = np.array([0, 0, -1], np.int32)
+        assert L.tide_set_feature_groups(h, g.ctypes.data, 3) == 0
+        assert L.tide_set_interaction_groups(h, g.ctypes.data, 3) == 0
+        assert L.tide_set_int(h, b"regime_feature", 1) == 0
+        assert L.tide_set_string(h, b"eval_metric", b"ic") == 0
+        assert L.tide_set_double(h, b"cat_smooth", 2.0) == 0
+        # an unfitted model has no trees and no categorical values
+        assert L.tide_num_trees(h) == 0
+        buf = np.zeros(4)
+        assert L.tide_get_cat_values(h, 0, buf.ctypes.data, 4) == 0
+        out = np.zeros(8, np.uint64)
+        assert L.tide_tree_cat(h, 0, out.ctypes.data) == -1
+    finally:
+        L.tide_free(h)
+
+
+def test_python_booster_rejects_bad_inputs():
+    b = t.Booster()
+    with pytest.raises(t.TideError, match="one entry per feature"):
+        b.fit(np.zeros((10, 3)), np.zeros(10), iid=True, is_cat=[1, 0])
+    with pytest.raises(t.TideError, match="same length"):
+        b.fit(np.zeros((10, 3)), np.zeros(10), iid=True, eval_set=(np.zeros((4, 3)), np.zeros(5)))
+    assert {"early_stopping_rounds", "eval_metric", "cat_smooth", "feature_groups"} <= t.KNOWN_KEYS
