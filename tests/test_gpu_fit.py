@@ -138,6 +138,31 @@ def test_larger_panel_depth8_missing_values():
     _check("large-subset-mse", p, n_estimators=8, max_depth=8, learning_rate=0.05, goss=0, pair_splits=0)
 
 
+def test_full_size_panel_matches_reference_core():
+    # the headline configuration at full size (1,000,000 x 128, depth 8, 31 leaves, lr 0.05, GOSS
+    # off, PAIR auto-off at >= 1M rows): the device trees against the reference core itself
+    ref = checkers.reference()
+    if ref is None:
+        pytest.skip("reference core (oracle/_ref) not built")
+    p = synth.panel_large(1)
+    params = dict(n_estimators=5, max_depth=8, learning_rate=0.05, goss=0)
+    _say(f"[case] full-size n={p.n} F={p.n_features} params={params}")
+    gpu = _fit_gpu(p, **params)
+    r = _fit_cpu(ref, p, n_jobs=16, **params)
+    diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9)
+    if diff is not None:   # let the checker verify a characterized difference on the rows
+        xb = checkers.bins_of(p.X, [gpu.edges(f) for f in range(p.n_features)])
+        diff = checkers.compare_trees(gpu.trees(), r.trees(), rtol=VALUE_RTOL, atol=1e-9, xb=xb,
+                                      n_bins=gpu.model_info()["n_bins"])
+    assert diff is None, diff
+    for f in range(0, p.n_features, 17):
+        assert np.array_equal(gpu.edges(f), r.edges(f))
+    Xp = p.X[:20000]
+    rel = _pred_close(gpu.predict(Xp), r.predict(Xp))
+    assert rel < PRED_RTOL, rel
+    assert np.allclose(gpu.evals_result()["train"], r.evals_train(), rtol=LOSS_RTOL, atol=0)
+
+
 def test_wide_panel_300_features():
     # more features than a 256-thread selection block and than 16-feature tiles divide evenly
     p = synth.panel_small(13, n_dates=300, n_entities=40, n_features=300)
