@@ -986,10 +986,10 @@ static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_c
     const ExportLayout L = export_layout(s->max_nodes, bmax);
     const size_t need = L.stride * s->T;
     if (need > s->export_bytes) {
-        if (!s->pending.empty()) HIP_OK(hipStreamSynchronize(s->stream));   // the old slots stay readable below
+        // the layout grows only when a task's bucket count does (a new fit): no tree is pending
+        if (!s->pending.empty()) throw std::runtime_error("internal: export slots resized with a tree pending");
         for (int k = 0; k < 2; ++k) {
-            if (s->export_h[k] && s->pending.empty()) HIP_OK(hipHostFree(s->export_h[k]));
-            else if (s->export_h[k]) throw std::runtime_error("internal: export slots resized with a tree pending");
+            if (s->export_h[k]) HIP_OK(hipHostFree(s->export_h[k]));
             s->export_h[k] = nullptr;
             HIP_OK(hipHostMalloc((void**)&s->export_h[k], need, hipHostMallocDefault));
             if (!s->export_ev[k]) HIP_OK(hipEventCreateWithFlags(&s->export_ev[k], hipEventDisableTiming));
