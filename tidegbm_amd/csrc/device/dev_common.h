@@ -132,9 +132,7 @@ struct Task {
     unsigned long long* red;   // per-block maxima of |g| and |h| [2][MAX_RED_BLOCKS]
     int n_red;                 // blocks that wrote red[] in the last gradient launch
     int scan_ticket;           // completion counter of the scan launch (last block selects)
-    int n_leaves_tab;          // entries of leaf_tab
     int pad_t;
-    int* leaf_tab;             // [MAX_NODES][4] {seg_start, seg_count, node, buf} sorted by start
     // ---- per-row arrays (length n_rows)
     const double* y;
     const double* w;       // may be null (all 1)

@@ -1753,10 +1753,15 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
 
 // =================================================================== tree end
 // Leaves sorted by the start of their row range (one block per task).
-__global__ __launch_bounds__(256) void k_leaf_table(Task* tasks) {
-    Task& T = tasks[blockIdx.x];
+// Leaf id of every sampled row from the final partition (all-rows samples) — reference
+// apply_all (tree.cpp:43-55) restricted to the rows the tree was grown on.  Every block first
+// orders the non-empty leaves by their segment start in LDS (at most MAX_NODES nodes), then
+// finds each of its rows' segment by binary search.
+__global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
+    Task& T = tasks[blockIdx.y];
+    __shared__ int st[MAX_NODES];        // segment start of non-empty leaves, -1 otherwise
+    __shared__ int tab[MAX_NODES * 4];   // {seg_start, seg_count, node, buf} by start
     __shared__ int nl;
-    __shared__ int st[MAX_NODES];  // segment start of non-empty leaves, -1 otherwise
     if (threadIdx.x == 0) nl = 0;
     const int nn = T.n_nodes;
     for (int k = threadIdx.x; k < MAX_NODES; k += blockDim.x) {
@@ -1773,24 +1778,14 @@ __global__ __launch_bounds__(256) void k_leaf_table(Task* tasks) {
         int r = 0;  // starts of non-empty leaves are distinct
         for (int j = 0; j < nn; ++j) r += (st[j] >= 0 && st[j] < s0);
         const DNode& n = T.nodes[k];
-        T.leaf_tab[4 * r] = s0;
-        T.leaf_tab[4 * r + 1] = n.seg_count;
-        T.leaf_tab[4 * r + 2] = k;
-        T.leaf_tab[4 * r + 3] = n.buf;
+        tab[4 * r] = s0;
+        tab[4 * r + 1] = n.seg_count;
+        tab[4 * r + 2] = k;
+        tab[4 * r + 3] = n.buf;
         atomicAdd(&nl, 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) T.n_leaves_tab = nl;
-}
-
-// Leaf id of every sampled row from the final partition (all-rows samples) — reference
-// apply_all (tree.cpp:43-55) restricted to the rows the tree was grown on.
-__global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
-    __shared__ int tab[MAX_NODES * 4];
-    const int L = T.n_leaves_tab;
-    for (int k = threadIdx.x; k < 4 * L; k += blockDim.x) tab[k] = T.leaf_tab[k];
-    __syncthreads();
+    const int L = nl;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < T.n_sample; k += gridDim.x * blockDim.x) {
         int lo = 0, hi = L - 1;
         while (lo < hi) {
@@ -1898,10 +1893,9 @@ __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double 
         }
         if (iid) {
             for (int b = lane; b < B; b += 64) T.prefix[(size_t)b * nn + k] = full;
-            continue;
         }
         long long cg = 0, ch = 0, cc = 0;  // carry of the prefix across 64-segment chunks
-        for (int s0 = 0; s0 < B; s0 += 64) {
+        for (int s0 = 0; s0 < (iid ? 0 : B); s0 += 64) {
             const int b = s0 + lane;
             long long xg = 0, xh = 0, xc = 0;
             if (b < B) {
@@ -1921,6 +1915,14 @@ __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double 
             cg = __shfl(pg, 63, 64);
             ch = __shfl(ph, 63, 64);
             cc = __shfl(pc, 63, 64);
+        }
+        // the node's sums are consumed: clear them for the next tree's k_torb_accum (the
+        // session starts them at zero)
+        for (int s = lane; s <= B; s += 64) {
+            const size_t o = (size_t)s * MAX_NODES + k;
+            T.tg[o] = 0;
+            if (!T.hconst) T.th[o] = 0;
+            T.tc[o] = 0;
         }
     }
 }
@@ -1966,15 +1968,6 @@ __global__ __launch_bounds__(256) void k_pred_update(Task* tasks, double lr, int
     }
 }
 
-__global__ void k_zero_torb(Task* tasks, int words) {
-    Task& T = tasks[blockIdx.y];
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < words; j += gridDim.x * blockDim.x) {
-        T.tg[j] = 0;
-        if (!T.hconst) T.th[j] = 0;
-        T.tc[j] = 0;
-    }
-}
-
 void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool walk,
                      int max_rows, const TorbChunk* const* chunks, const int* n_chunks,
                      int max_chunks, const GrowP& P, double min_cover, int iid, double lr, int pairwise) {
@@ -1982,11 +1975,8 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
     if (walk) {
         hipLaunchKernelGGL(k_nid_walk, dim3(rb, n_tasks), dim3(256), 0, st, S, tasks);
     } else {
-        hipLaunchKernelGGL(k_leaf_table, dim3(n_tasks), dim3(256), 0, st, tasks);
         hipLaunchKernelGGL(k_nid_segments, dim3(rb, n_tasks), dim3(256), 0, st, tasks);
     }
-    const int words = (MAX_BUCKETS + 1) * MAX_NODES;
-    hipLaunchKernelGGL(k_zero_torb, dim3((words + 1023) / 1024, n_tasks), dim3(256), 0, st, tasks, words);
     hipLaunchKernelGGL(k_torb_accum, dim3(std::max(1, max_chunks), n_tasks), dim3(256), 0, st, tasks,
                        chunks, n_chunks);
     hipLaunchKernelGGL(k_torb_final, dim3(MAX_NODES, n_tasks), dim3(64), 0, st, tasks, P, min_cover, iid);

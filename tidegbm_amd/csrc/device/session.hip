@@ -170,6 +170,14 @@ struct tgbm_session {
         HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(tasks_d + t) + off, &v, sizeof v, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
     }
+    // TORB (segment, node) sums of task t to zero (on the session stream).
+    void clear_torb(int t) {
+        const Task& T = tasks_h[t];
+        const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
+        HIP_OK(hipMemsetAsync(T.tg, 0, sizeof(long long) * tw, stream));
+        if (T.th) HIP_OK(hipMemsetAsync(T.th, 0, sizeof(long long) * tw, stream));
+        HIP_OK(hipMemsetAsync(T.tc, 0, sizeof(int) * tw, stream));
+    }
     Task read_task(int t) {
         Task out;
         HIP_OK(hipMemcpyAsync(&out, tasks_d + t, sizeof(Task), hipMemcpyDeviceToHost, stream));
@@ -383,7 +391,6 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.idx_tmp = s->alloc<int>(m);
             T.chunk_cnt = s->alloc<int>(m / 2048 + 2);
             T.red = s->alloc<unsigned long long>(2 * MAX_RED_BLOCKS);
-            T.leaf_tab = s->alloc<int>(4 * MAX_NODES);
             T.nodes = s->alloc<DNode>(MAX_NODES);
             T.cands = s->alloc<DCand>(MAX_NODES);
             T.fcand = s->alloc<FCand>((size_t)MAX_NODES * n_features);
@@ -398,6 +405,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.tg = s->alloc<long long>(tw);
             T.th = s->hconst ? nullptr : s->alloc<long long>(tw);
             T.tc = s->alloc<int>(tw);
+            s->clear_torb(t);   // k_torb_final clears what each tree adds; they start at zero
             T.values = s->alloc<double>(MAX_NODES);
             T.prefix = s->alloc<double>((size_t)MAX_BUCKETS * MAX_NODES);
             T.cover = s->alloc<int>(MAX_NODES);
@@ -532,6 +540,7 @@ int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows) {
         s->set_field_int(task, offsetof(Task, goss), 0);
         s->set_field_int(task, offsetof(Task, n_buckets), 1);
         HIP_OK(hipMemsetAsync((void*)T.bucket, 0, sizeof(int) * n_rows, s->stream));
+        s->clear_torb(task);
         s->upload_chunks(task);
     });
 }
@@ -568,6 +577,7 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
         if (n_buckets < 1 || n_buckets > MAX_BUCKETS) throw std::runtime_error("n_buckets out of range for the device path");
         Task& T = s->tasks_h[task];
         TaskHost& H = s->th[task];
+        s->clear_torb(task);   // a fit starts from zero sums whatever an earlier one left
         const int m = H.n_rows;
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy((void*)T.y, y, sizeof(double) * m, hipMemcpyHostToDevice));
