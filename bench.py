@@ -361,8 +361,9 @@ def main():
         copy_probe(args.copy_probe)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the fit shards by row "
-              "ranges across GPUs only in design (see DESIGN.md), not in this build.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this library runs a fit on one "
+              "GPU; sharding rows across GPUs (one histogram all-reduce per built node) is not built.",
+              file=sys.stderr)
         raise SystemExit(2)
     if args.warmup < 1:
         args.warmup = 1
