@@ -193,6 +193,10 @@ SWEEP = [
     ("iid-weighted", dict(), dict(iid=True, weighted=True)),
     ("time-weighted", dict(min_child_weight=2.0), dict(weighted=True)),
     ("pairwise-l1", dict(objective=5, l1=0.1, max_depth=5), dict(groups=True)),
+    # host-driven PAIR search with its knobs moved: every leaf counts as weak
+    ("pair-knobs", dict(pair_splits=1, pair_trigger=1e6, pair_top_k=4, max_pair_splits_per_tree=5, pair_l2=0.5), {}),
+    ("goss-knobs", dict(goss=1, goss_top=0.3, goss_other=0.2), {}),
+    ("bin-sample-seed", dict(bin_sample=300, random_state=7), {}),
 ]
 
 
