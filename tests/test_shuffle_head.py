@@ -1,0 +1,20 @@
+"""The head-of-shuffle used for GOSS draws and bin sub-samples (csrc/host/shuffle_head.h) against
+std::mt19937 and std::shuffle, the calls the reference makes (booster.cpp:572, binning.cpp:95).
+A C++ check program is compiled with the host compiler and run; it prints each failure."""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs the host C++ compiler")
+def test_shuffle_head_matches_std_shuffle(tmp_path):
+    exe = tmp_path / "shuffle_head_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "tidegbm_amd" / "csrc" / "host"),
+                    "-o", str(exe), str(ROOT / "tests" / "cpp" / "shuffle_head_check.cpp")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-4000:]
+    assert r.stdout.startswith("ok")

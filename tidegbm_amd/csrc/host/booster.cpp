@@ -3,6 +3,7 @@
 // that the rows, bins and samples handed to the device are identical; every per-row and
 // per-histogram computation runs on the GPU.
 #include "booster.h"
+#include "shuffle_head.h"
 #include "tuner.h"
 
 #include <algorithm>
@@ -79,8 +80,10 @@ void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint3
         }
         if (bin_sample > 0 && (int)v.size() > bin_sample) {
             std::mt19937 rng(seed + (uint32_t)f * 2654435761u);
-            std::shuffle(v.begin(), v.end(), rng);
-            v.resize((size_t)bin_sample);
+            Mt19937Stream g(rng);
+            std::vector<double> head;
+            shuffle_head<double>(g, v.size(), (size_t)bin_sample, [&](size_t i) { return v[i]; }, head);
+            v.swap(head);
         }
         edges[f] = quantile_edges(v, max_bin - 1);
     }
@@ -320,11 +323,12 @@ GossCounts goss_counts(int S, const Config& c) {
 }
 
 // The generator draws of a GOSS fit, produced ahead of the trees.  Per tree the reference makes
-// one std::shuffle of the rest (booster.cpp:571) and one tree seed draw (:587).  The shuffle's
+// one std::shuffle of the rest (booster.cpp:572) and one tree seed draw (:588).  The shuffle's
 // swaps depend on the generator and the rest's size only, never on the values shuffled, so the
 // rest after the shuffle is rest[P[i]] with P the same call applied to 0..n_rest-1.  A thread
-// computes P for each tree in order and queues its first n_other entries (the drawn positions);
-// the generator is that thread's alone until the object is destroyed.
+// computes the first n_other entries of P for each tree in order (shuffle_head: the same draws,
+// without shuffling n_rest entries) and queues them (the drawn positions).  The thread draws from
+// a stream taken from the generator, which is not to be used again by the caller.
 class GossDraws {
   public:
     GossDraws(std::mt19937& rng, int n_rest, int n_other, int n_trees)
@@ -352,12 +356,11 @@ class GossDraws {
   private:
     static constexpr size_t kAhead = 4;
     void run() {
-        std::vector<int> p(n_rest_);
+        Mt19937Stream g(rng_);
         for (int m = 0; m < n_trees_; ++m) {
-            std::iota(p.begin(), p.end(), 0);
-            std::shuffle(p.begin(), p.end(), rng_);
-            (void)rng_();  // the tree's own seed; only colsample would use it
-            std::vector<int> head(p.begin(), p.begin() + n_other_);
+            std::vector<int> head;
+            shuffle_head<int>(g, (size_t)n_rest_, (size_t)n_other_, [](size_t i) { return (int)i; }, head);
+            (void)g();  // the tree's own seed; only colsample would use it
             std::unique_lock<std::mutex> lk(mu_);
             cv_.wait(lk, [&] { return stop_ || q_.size() < kAhead; });
             if (stop_) return;
