@@ -5,6 +5,7 @@
 #include "booster.h"
 #include "shuffle_head.h"
 #include "tuner.h"
+#include "value_sort.h"
 
 #include <algorithm>
 #include <chrono>
@@ -43,7 +44,7 @@ struct SessionGuard {
 
 // Quantile right-edges of one feature (reference quantile_edges, binning.cpp:14-32).
 std::vector<double> quantile_edges(std::vector<double>& v, int nb) {
-    std::sort(v.begin(), v.end());
+    sort_values(v);   // std::sort's sequence
     v.erase(std::unique(v.begin(), v.end()), v.end());
     if (v.empty()) return {0.0};
     if ((int)v.size() <= nb) return v;
@@ -66,26 +67,42 @@ std::vector<double> quantile_edges(std::vector<double>& v, int nb) {
 }  // namespace
 
 // Per-feature edges over non-NaN values with the reference's sub-sampling
-// (Binner::fit numeric branch, binning.cpp:84-99); features run in parallel.
+// (Binner::fit numeric branch, binning.cpp:84-99).  Groups of up to 8 adjacent features run in
+// parallel: one pass over the row-major rows gathers a group's columns (8 doubles share a cache
+// line), with at most 32M gathered values held per thread.
 void fit_edges(const double* X, int n, int F, int max_bin, int bin_sample, uint32_t seed,
                std::vector<std::vector<double>>& edges) {
     edges.assign(F, {});
-#pragma omp parallel for schedule(dynamic) if (F > 1)
-    for (int f = 0; f < F; ++f) {
-        std::vector<double> v;
-        v.reserve(n);
+    if (F <= 0) return;
+    const int threads = std::max(1, omp_get_max_threads());
+    long long gs = std::min(8, std::max(1, F / threads));
+    gs = std::max(1ll, std::min(gs, (32ll << 20) / std::max(n, 1)));
+    const int n_groups = (int)((F + gs - 1) / gs);
+#pragma omp parallel for schedule(dynamic) if (n_groups > 1)
+    for (int gi = 0; gi < n_groups; ++gi) {
+        const int f0 = (int)(gi * gs), nf = std::min(F - f0, (int)gs);
+        std::vector<std::vector<double>> cols(nf);
+        for (auto& v : cols) v.reserve(n);
         for (int i = 0; i < n; ++i) {
-            const double x = X[(size_t)i * F + f];
-            if (x == x) v.push_back(x);
+            const double* r = X + (size_t)i * F + f0;
+            for (int q = 0; q < nf; ++q) {
+                const double x = r[q];
+                if (x == x) cols[q].push_back(x);
+            }
         }
-        if (bin_sample > 0 && (int)v.size() > bin_sample) {
-            std::mt19937 rng(seed + (uint32_t)f * 2654435761u);
-            Mt19937Stream g(rng);
-            std::vector<double> head;
-            shuffle_head<double>(g, v.size(), (size_t)bin_sample, [&](size_t i) { return v[i]; }, head);
-            v.swap(head);
+        for (int q = 0; q < nf; ++q) {
+            const int f = f0 + q;
+            std::vector<double>& v = cols[q];
+            if (bin_sample > 0 && (int)v.size() > bin_sample) {
+                std::mt19937 rng(seed + (uint32_t)f * 2654435761u);
+                Mt19937Stream g(rng);
+                std::vector<double> head;
+                shuffle_head<double>(g, v.size(), (size_t)bin_sample, [&](size_t i) { return v[i]; }, head);
+                v.swap(head);
+            }
+            edges[f] = quantile_edges(v, max_bin - 1);
+            std::vector<double>().swap(v);
         }
-        edges[f] = quantile_edges(v, max_bin - 1);
     }
 }
 
