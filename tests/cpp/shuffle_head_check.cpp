@@ -22,17 +22,33 @@ static void expect(bool ok, const char* what, long a, long b) {
 }
 
 int main() {
-    // the stream, from generators at several positions within and across 624-word blocks
-    for (int skip : {0, 1, 311, 623, 624, 625, 5000}) {
-        std::mt19937 a(1234u + skip), b(1234u + skip);
-        for (int i = 0; i < skip; ++i) {
-            a();
-            b();
+    // the stream, from generators at several positions within and across 624-word blocks, with
+    // every block-loop variant this CPU can run
+    for (int isa = tide::mt_detail::kBase; isa <= tide::mt_detail::kAvx512; ++isa) {
+        if (!tide::mt_detail::isa_supported(isa)) {
+            std::printf("note: vector variant %d not supported here, not checked\n", isa);
+            continue;
         }
-        Mt19937Stream s(b);
-        bool same = true;
-        for (int i = 0; i < 200000 && same; ++i) same = a() == s();
-        expect(same, "stream", skip, 0);
+        for (int skip : {0, 1, 311, 623, 624, 625, 5000}) {
+            std::mt19937 a(1234u + skip), b(1234u + skip);
+            for (int i = 0; i < skip; ++i) {
+                a();
+                b();
+            }
+            Mt19937Stream s(b, isa);
+            bool same = true;
+            for (int i = 0; i < 200000 && same; ++i) same = a() == s();
+            expect(same, "stream", skip, isa);
+        }
+        // the bounded draws of a large shuffle with this variant
+        std::mt19937 a(99), b(99);
+        std::vector<int> p(300000);
+        std::iota(p.begin(), p.end(), 0);
+        std::shuffle(p.begin(), p.end(), a);
+        Mt19937Stream s(b, isa);
+        std::vector<int> head;
+        shuffle_head<int>(s, p.size(), 30000, [](size_t i) { return (int)i; }, head);
+        expect(std::equal(head.begin(), head.end(), p.begin()) && a() == s(), "variant prefix", isa, 0);
     }
     // the standard's own check value: the 10000th output of a default-seeded mt19937
     {

@@ -18,7 +18,7 @@ def _run_check(tmp_path, name):
                     "-o", str(exe), str(ROOT / "tests" / "cpp" / f"{name}.cpp")], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-4000:]
-    assert r.stdout.startswith("ok")
+    assert r.stdout.strip().splitlines()[-1].startswith("ok"), r.stdout[-4000:]
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs the host C++ compiler")

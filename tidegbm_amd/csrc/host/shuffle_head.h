@@ -21,18 +21,100 @@
 
 namespace tide {
 
+namespace mt_detail {
+
+constexpr int N = 624, M = 397;
+constexpr uint32_t kMatrix = 0x9908b0dfu, kB = 0x9d2c5680u, kC = 0xefc60000u;
+
+inline uint32_t temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & kB;
+    y ^= (y << 15) & kC;
+    y ^= y >> 18;
+    return y;
+}
+// x[k + N] = x[k + M] ^ twist(upper bit of x[k], lower 31 bits of x[k + 1])
+inline uint32_t next_term(uint32_t xk, uint32_t xk1, uint32_t xkm) {
+    const uint32_t y = (xk & 0x80000000u) | (xk1 & 0x7fffffffu);
+    return xkm ^ (y >> 1) ^ ((0u - (y & 1u)) & kMatrix);
+}
+// The next N terms of the recurrence in place (x[0] the oldest) and their tempered outputs.
+__attribute__((always_inline)) inline void refill_body(uint32_t* __restrict x, uint32_t* __restrict out) {
+    int i = 0;
+    for (; i < N - M; ++i) x[i] = next_term(x[i], x[i + 1], x[i + M]);
+    for (; i < N - 1; ++i) x[i] = next_term(x[i], x[i + 1], x[i + M - N]);
+    x[N - 1] = next_term(x[N - 1], x[0], x[M - 1]);
+    for (int k = 0; k < N; ++k) out[k] = temper(x[k]);
+}
+// Lemire products of one run: js[c] = high word of x[c] * (r0 + c); returns nonzero when some
+// low word is below its range (a possible rejection).
+__attribute__((always_inline)) inline uint32_t run_body(const uint32_t* __restrict x, uint32_t r0, uint32_t run,
+                                                        unsigned long* __restrict js) {
+    uint32_t low_below = 0;
+    for (uint32_t c = 0; c < run; ++c) {
+        const uint32_t r = r0 + c;
+        const uint64_t prod = (uint64_t)x[c] * r;
+        js[c] = (unsigned long)(prod >> 32);
+        low_below |= (uint32_t)((uint32_t)prod < r);
+    }
+    return low_below;
+}
+
+// The same loops compiled for wider vector units; the widest the CPU supports is picked once.
+inline void refill_base(uint32_t* x, uint32_t* out) { refill_body(x, out); }
+__attribute__((target("avx2"))) inline void refill_avx2(uint32_t* x, uint32_t* out) { refill_body(x, out); }
+__attribute__((target("avx512f,avx512bw"))) inline void refill_avx512(uint32_t* x, uint32_t* out) {
+    refill_body(x, out);
+}
+inline uint32_t run_base(const uint32_t* x, uint32_t r0, uint32_t run, unsigned long* js) {
+    return run_body(x, r0, run, js);
+}
+__attribute__((target("avx2"))) inline uint32_t run_avx2(const uint32_t* x, uint32_t r0, uint32_t run,
+                                                          unsigned long* js) {
+    return run_body(x, r0, run, js);
+}
+__attribute__((target("avx512f,avx512bw"))) inline uint32_t run_avx512(const uint32_t* x, uint32_t r0, uint32_t run,
+                                                                      unsigned long* js) {
+    return run_body(x, r0, run, js);
+}
+
+struct Kernels {
+    void (*refill)(uint32_t*, uint32_t*);
+    uint32_t (*run)(const uint32_t*, uint32_t, uint32_t, unsigned long*);
+};
+enum Isa { kBase = 0, kAvx2 = 1, kAvx512 = 2 };
+inline bool isa_supported(int isa) {
+    __builtin_cpu_init();
+    if (isa == kAvx512) return __builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512bw");
+    if (isa == kAvx2) return __builtin_cpu_supports("avx2");
+    return true;
+}
+inline Kernels kernels(int isa) {
+    if (isa == kAvx512) return {refill_avx512, run_avx512};
+    if (isa == kAvx2) return {refill_avx2, run_avx2};
+    return {refill_base, run_base};
+}
+inline int best_isa() { return isa_supported(kAvx512) ? kAvx512 : (isa_supported(kAvx2) ? kAvx2 : kBase); }
+
+}  // namespace mt_detail
+
 // MT19937 (Matsumoto and Nishimura, ACM TOMACS 8(1), 1998) producing the outputs of a
 // std::mt19937 from its current position on.  The state is recovered from 624 outputs drawn from
 // the source (tempering is invertible, and 624 consecutive untempered outputs are the recurrence's
 // last 624 terms); those outputs are served first.  The source is left 624 draws further on.
-// Outputs are generated 624 at a time in plain loops the compiler vectorizes.
+// Outputs are generated 624 at a time in plain loops the compiler vectorizes (built for the
+// widest vector unit the CPU has).
 class Mt19937Stream {
   public:
     using result_type = std::mt19937::result_type;
     static constexpr result_type min() { return std::mt19937::min(); }
     static constexpr result_type max() { return std::mt19937::max(); }
 
-    explicit Mt19937Stream(std::mt19937& src) {
+    // isa: mt_detail::kBase / kAvx2 / kAvx512 variant of the block loops (-1: the widest the CPU
+    // supports; a variant it does not support falls back to kBase).
+    explicit Mt19937Stream(std::mt19937& src, int isa = -1) {
+        if (isa < 0 || !mt_detail::isa_supported(isa)) isa = isa < 0 ? mt_detail::best_isa() : mt_detail::kBase;
+        k_ = mt_detail::kernels(isa);
         for (int i = 0; i < N; ++i) out_[i] = (uint32_t)src();
         for (int i = 0; i < N; ++i) x_[i] = untemper(out_[i]);
         p_ = 0;
@@ -53,16 +135,7 @@ class Mt19937Stream {
         while (t < m) {
             if (p_ == N) refill();
             const size_t run = std::min<size_t>({kRun, m - t, (size_t)(N - p_)});
-            const uint32_t* x = out_ + p_;
-            const uint64_t r0 = (uint64_t)(i0 + t) + 1;
-            uint32_t low_below = 0;
-            for (size_t c = 0; c < run; ++c) {
-                const uint64_t r = r0 + c;
-                const uint64_t prod = (uint64_t)x[c] * r;
-                js[t + c] = (unsigned long)(prod >> 32);
-                low_below |= (uint32_t)((uint64_t)(uint32_t)prod < r);
-            }
-            if (!low_below) {
+            if (!k_.run(out_ + p_, (uint32_t)(i0 + t + 1), (uint32_t)run, js + t)) {
                 p_ += (int)run;
                 t += run;
                 continue;
@@ -72,7 +145,7 @@ class Mt19937Stream {
     }
 
   private:
-    static constexpr int N = 624, M = 397;
+    static constexpr int N = mt_detail::N;
     static constexpr size_t kRun = 64;
 
     // One draw in [0, r) by Lemire's method (see positions()).
@@ -88,15 +161,6 @@ class Mt19937Stream {
         }
         return (unsigned long)(prod >> 32);
     }
-    static constexpr uint32_t kMatrix = 0x9908b0dfu, kB = 0x9d2c5680u, kC = 0xefc60000u;
-
-    static uint32_t temper(uint32_t y) {
-        y ^= y >> 11;
-        y ^= (y << 7) & kB;
-        y ^= (y << 15) & kC;
-        y ^= y >> 18;
-        return y;
-    }
     // Inverse of y ^= y >> s: each pass fixes s more high bits.
     static uint32_t unshift_right(uint32_t y, int s) {
         uint32_t r = y;
@@ -111,24 +175,16 @@ class Mt19937Stream {
     }
     static uint32_t untemper(uint32_t y) {
         y = unshift_right(y, 18);
-        y = unshift_left(y, 15, kC);
-        y = unshift_left(y, 7, kB);
+        y = unshift_left(y, 15, mt_detail::kC);
+        y = unshift_left(y, 7, mt_detail::kB);
         return unshift_right(y, 11);
     }
-    // x[k + N] = x[k + M] ^ twist(upper bit of x[k], lower 31 bits of x[k + 1])
-    static uint32_t next_term(uint32_t xk, uint32_t xk1, uint32_t xkm) {
-        const uint32_t y = (xk & 0x80000000u) | (xk1 & 0x7fffffffu);
-        return xkm ^ (y >> 1) ^ ((0u - (y & 1u)) & kMatrix);
-    }
     void refill() {
-        int i = 0;
-        for (; i < N - M; ++i) x_[i] = next_term(x_[i], x_[i + 1], x_[i + M]);
-        for (; i < N - 1; ++i) x_[i] = next_term(x_[i], x_[i + 1], x_[i + M - N]);
-        x_[N - 1] = next_term(x_[N - 1], x_[0], x_[M - 1]);
-        for (int k = 0; k < N; ++k) out_[k] = temper(x_[k]);
+        k_.refill(x_, out_);
         p_ = 0;
     }
 
+    mt_detail::Kernels k_;
     uint32_t x_[N];
     uint32_t out_[N];
     int p_;
