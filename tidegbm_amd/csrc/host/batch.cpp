@@ -18,7 +18,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <memory>
+
+#include <omp.h>
 
 namespace tide {
 
@@ -73,15 +76,37 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
     SessionOwner so;
     dev_check(tgbm_session_create(&so.s, n_fit, F, n_bins, n_tasks, task_rows, proto.max_leaves, hconst ? 1 : 0));
     bin_on_device(so.s, X, edges);
-    // ---- per-fold preparation, exactly as a plain fit of that fold
+    // ---- per-fold preparation, exactly as a plain fit of that fold; the folds' time layouts are
+    // computed in parallel waves of one fold per host thread (the first failing fold's error is
+    // reported, as in fold order)
     out.clear();
     std::vector<double> ones(n_fit, 1.0);
+    const int wave = std::max(1, omp_get_max_threads());
+    std::vector<TimeLayout> layouts;
+    std::vector<std::exception_ptr> errs;
     for (int t = 0; t < n_tasks; ++t) {
-        const int m = task_rows[t];
+        const int w0 = t - t % wave;
+        if (t == w0) {
+            const int nw = std::min(wave, n_tasks - w0);
+            layouts.assign(nw, TimeLayout());
+            errs.assign(nw, nullptr);
+#pragma omp parallel for schedule(dynamic) if (nw > 1)
+            for (int q = 0; q < nw; ++q) {
+                try {
+                    Config c = proto;
+                    if (horizons) c.horizon = horizons[w0 + q];
+                    layouts[q] = time_layout(c, ts, ys[w0 + q], ones.data(), task_rows[w0 + q]);
+                } catch (...) {
+                    errs[q] = std::current_exception();
+                }
+            }
+            for (auto& e : errs)
+                if (e) std::rethrow_exception(e);
+        }
         auto b = std::make_unique<Booster>();
         b->cfg = proto;
         if (horizons) b->cfg.horizon = horizons[t];
-        TimeLayout tl = time_layout(b->cfg, ts, ys[t], ones.data(), m);
+        TimeLayout tl = std::move(layouts[t - w0]);
         const long long* groups = proto.objective == 5 ? ts : nullptr;  // rank groups = dates (native.py:134-137)
         if (proto.objective == 5 && !groups) throw Error("the pairwise objective needs timestamps as groups");
         dev_check(tgbm_task_set_rows(so.s, t, ys[t], nullptr, tl.pred.data(), tl.bucket.data(), nullptr, tl.n_buckets,
