@@ -2000,12 +2000,30 @@ __global__ __launch_bounds__(64) void k_loss_groups(Task* tasks) {
     const int* gr = T.group_rows ? T.group_rows + g0 : nullptr;
     double val = __longlong_as_double(0x7ff8000000000000ll);   // NaN: the group does not count
     if (m >= 3) {
+        // a lane's STAGE / 64 rows are loaded together: every row index first, then every value,
+        // so each stage costs two memory round trips instead of two per 64 rows
         auto stage = [&](int base, int cnt) {
+            constexpr int PER_LANE = STAGE / 64;
             __syncthreads();
-            for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-                const int r = gr ? gr[base + k] : g0 + base + k;
-                ys[k] = T.y[r];
-                ps[k] = T.pred[r];
+            int r[PER_LANE];
+#pragma unroll
+            for (int u = 0; u < PER_LANE; ++u) {
+                const int k = threadIdx.x + u * 64;
+                r[u] = k < cnt ? (gr ? gr[base + k] : g0 + base + k) : -1;
+            }
+            double yv[PER_LANE], pv[PER_LANE];
+#pragma unroll
+            for (int u = 0; u < PER_LANE; ++u) {
+                yv[u] = r[u] >= 0 ? T.y[r[u]] : 0.0;
+                pv[u] = r[u] >= 0 ? T.pred[r[u]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PER_LANE; ++u) {
+                const int k = threadIdx.x + u * 64;
+                if (k < cnt) {
+                    ys[k] = yv[u];
+                    ps[k] = pv[u];
+                }
             }
             __syncthreads();
         };
