@@ -6,6 +6,8 @@
 // sort over the order-preserving 64-bit keys (sign bit flipped for positives, all bits flipped for
 // negatives) gives the same values in the same order.  Digit passes in which every key holds the
 // same byte are skipped: doubles converted from float keep their low 29 mantissa bits zero.
+// The radix sort holds two 8-byte keys per value; above 4M values (bin sub-sampling off on a
+// large column) std::sort sorts in place instead.
 #pragma once
 
 #include <algorithm>
@@ -17,7 +19,7 @@ namespace tide {
 
 inline void sort_values(std::vector<double>& v) {
     const size_t n = v.size();
-    if (n < 2048) {
+    if (n < 2048 || n > ((size_t)1 << 22)) {
         std::sort(v.begin(), v.end());
         return;
     }
