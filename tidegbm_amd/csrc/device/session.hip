@@ -779,8 +779,8 @@ int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other
     });
 }
 
-// Root of every task's tree: quantize, zero the node histograms, build and scan the root.  In
-// device-driven mode the scan's last block also selects the first split.
+// Root of every task's tree: quantize, zero the node histograms, build and scan the root.  The
+// first split is selected by the first split step (its partition launch in device-driven mode).
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     const GrowP P = s->growp(p);
     launch_tree_start(s->stream, s->S, s->td(), s->an, s->max_rows(), s->max_nodes);
@@ -801,15 +801,16 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
                 p->hist_mode, s->timing ? ep.a : nullptr, s->timing ? ep.b : nullptr);
     if (s->timing) s->events.push_back(ep);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 0);
-    (void)device_mode;   // the first split is selected by the first partition launch
+    (void)device_mode;   // both modes begin a tree the same way
     if (energy)
         for (int t = s->a0; t < s->a0 + s->an; ++t) launch_energy(s->stream, s->tasks_d, t, 0, s->th[t].has_sample ? s->th[t].n_sample : s->th[t].n_rows);
     HIP_OK(hipGetLastError());
 }
 
-// One split step of every task: partition the decided leaf, build the smaller child, scan both
-// children (device mode: the scan's last block selects the next split; host mode: the host
-// applied its own decision first).
+// One split step of every task: decide the split, partition the split leaf, build the smaller
+// child, scan both children.  Device mode: every partition block runs the best-first selection
+// over the last scan's results (with several active tasks one selection launch decides once per
+// task and the partition reads the decision); host mode: the host wrote its own decision first.
 static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_grow_params* p) {
     const int ms = s->launch_rows();
     EventPair e1, e2;
