@@ -2078,9 +2078,20 @@ __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, cha
         const int ngr = T.n_groups;
         double acc = 0.0;
         int ng = 0;
+        constexpr int PER_THREAD = STAGE / 256;   // launched with 256 threads
         for (int base = 0; base < ngr; base += STAGE) {
             const int cnt = min(STAGE, ngr - base);
-            for (int j = threadIdx.x; j < cnt; j += blockDim.x) gvs[j] = gv[base + j];
+            double v[PER_THREAD];   // a thread's loads issued together, then stored
+#pragma unroll
+            for (int u = 0; u < PER_THREAD; ++u) {
+                const int j = threadIdx.x + u * 256;
+                v[u] = j < cnt ? gv[base + j] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PER_THREAD; ++u) {
+                const int j = threadIdx.x + u * 256;
+                if (j < cnt) gvs[j] = v[u];
+            }
             __syncthreads();
             if (threadIdx.x == 0)
                 for (int j = 0; j < cnt; ++j) {
