@@ -165,13 +165,23 @@ def run_plain(args, p):
 
 def copy_probe(mib):
     """A plain device copy of known size: the reference point that scales rocprofv3's FETCH_SIZE
-    and WRITE_SIZE counters to bytes (each dispatch reads and writes exactly mib MiB)."""
-    import torch
-    a = torch.ones(mib << 20, dtype=torch.uint8, device="cuda")
-    b = torch.empty_like(a)
+    and WRITE_SIZE counters to bytes (each dispatch reads and writes exactly mib MiB).  The HIP
+    runtime's device-to-device hipMemcpy runs as its own copy kernel (__amd_rocclr_copyBuffer)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), nbytes) != 0:
+            raise SystemExit("copy probe: hipMalloc failed")
+    if hip.hipMemset(a, 1, nbytes) != 0:
+        raise SystemExit("copy probe: hipMemset failed")
     for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
+        if hip.hipMemcpy(b, a, nbytes, 3) != 0:   # hipMemcpyDeviceToDevice
+            raise SystemExit("copy probe: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
     print(json.dumps({"copy_probe_mib": mib, "dispatches": 5}), flush=True)
 
 

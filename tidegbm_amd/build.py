@@ -3,7 +3,6 @@
 * tidegbm_amd/lib/libtgbm_hip.so   — gfx950 HIP kernels + thin device C ABI (hipcc)
 * tidegbm_amd/lib/libtidegbm_amd.so — C++ host booster + drop-in tide_* C API (g++), linked to
   the HIP library through the thin ABI
-* tidegbm_amd/lib/probe_*           — measurement probes (csrc/probes), e.g. the LDS atomic rate
 * oracle/liboracle.so              — the CPU restatement used by the tests (checker only)
 * oracle/_ref/libtide_ref.so       — the reference core built from its own sources, when the
   reference tree is present (checker only)
@@ -63,14 +62,6 @@ def build_host():
     return out
 
 
-def build_probes():
-    """Measurement probes (stand-alone programs; not part of the libraries)."""
-    for src in sorted((PKG / "csrc" / "probes").glob("*.hip")):
-        out = LIB / ("probe_" + src.stem)
-        _run([HIPCC, *HIP_FLAGS, "-o", str(out) + ".tmp", str(src)])
-        os.replace(str(out) + ".tmp", out)
-
-
 def build_oracle():
     _run(["make", "-s", "-f", str(ROOT / "oracle" / "Makefile")])
     if Path("/root/reference/cpp/src/booster.cpp").exists():
@@ -80,7 +71,6 @@ def build_oracle():
 def build(oracle: bool = True):
     build_device()
     build_host()
-    build_probes()
     if oracle and shutil.which("make"):
         build_oracle()
 
