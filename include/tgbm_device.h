@@ -83,6 +83,11 @@ int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, cons
  * routes by the bin mask the leaf's scan found.  Once per session, before the first tree;
  * growth must then be host-driven. */
 int tgbm_session_set_cat(tgbm_session* s, const int* is_cat);
+/* Monotone constraints of the numeric split scan (reference Booster::fit's monotone,
+ * booster.cpp:250-251; checked in best_numeric_missing, histogram.cpp:177-180): for feature f with
+ * monotone[f] > 0 a candidate whose left leaf weight is below the right one is skipped, for
+ * monotone[f] < 0 one whose left weight is above.  Once per session, before the first tree. */
+int tgbm_session_set_monotone(tgbm_session* s, const int* monotone);
 /* Bin masks (4 words per node, bins that go left) of the task's last exported tree, for the
  * nodes whose split_type is 2. */
 int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out);

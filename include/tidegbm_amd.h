@@ -91,6 +91,11 @@ int tide_evals_train(TideHandle h, double* out, int cap);
  * a node considers only the best-gain feature of each group (tree.cpp:135-158).  n_features = 0
  * clears them. */
 int tide_set_feature_groups(TideHandle h, const int* group_of, int n_features);
+/* Monotone constraints of the following fits (Booster::fit's monotone, booster.cpp:250-251; the
+ * Python layer's monotone_constraints): per feature +1 (leaf weights must not decrease across a
+ * split: left >= right), -1 (left <= right) or 0, checked by the split scan as in
+ * best_numeric_missing (histogram.cpp:177-180).  n_features = 0 clears them. */
+int tide_set_monotone(TideHandle h, const int* monotone, int n_features);
 /* Interaction groups (Config::interaction_group, config.hpp:59; -1 = unconstrained): below a
  * split on a feature of group g, nodes split only on unconstrained features and those of g
  * (tree.cpp:304-317, 436-445).  n_features = 0 clears them.  With the int keys

@@ -31,6 +31,7 @@ struct EvalSet {
     int n = 0;
     std::vector<int> is_cat;   // Booster::fit's is_cat of later fits (empty: none)
     std::vector<int> group_of; // Booster::fit's group_of of later fits (empty: none)
+    std::vector<int8_t> monotone; // Booster::fit's monotone of later fits (empty: none)
 };
 static std::map<void*, EvalSet> g_eval;
 static std::mutex g_eval_mu;
@@ -63,6 +64,13 @@ void tref_free(void* h) {
 int tref_set_groups(void* h, const int* group_of, int nf) {
     std::lock_guard<std::mutex> lk(g_eval_mu);
     g_eval[h].group_of.assign(group_of, group_of + (nf > 0 ? nf : 0));
+    return 0;
+}
+
+// Monotone constraints of later fits (Booster::fit's monotone, +1 / -1 / 0); nf = 0 clears.
+int tref_set_monotone(void* h, const int* monotone, int nf) {
+    std::lock_guard<std::mutex> lk(g_eval_mu);
+    g_eval[h].monotone.assign(monotone, monotone + (nf > 0 ? nf : 0));
     return 0;
 }
 
@@ -153,7 +161,8 @@ int tref_fit(void* h, const double* X, const double* y, int n, int nf, const lon
         if (!b->cfg.iid_mode) b->cfg.horizon = horizon;
         b->fit(X, y, n, nf, reinterpret_cast<const i64*>(ts), w,
                (int)ev.is_cat.size() == nf ? ev.is_cat.data() : nullptr,
-               (int)ev.group_of.size() == nf ? reinterpret_cast<const i32*>(ev.group_of.data()) : nullptr, nullptr,
+               (int)ev.group_of.size() == nf ? reinterpret_cast<const i32*>(ev.group_of.data()) : nullptr,
+               (int)ev.monotone.size() == nf ? ev.monotone.data() : nullptr,
                reinterpret_cast<const i64*>(rank_groups), ev.n > 0 ? ev.X.data() : nullptr,
                ev.n > 0 ? ev.y.data() : nullptr,
                ev.n > 0 && !ev.ts.empty() ? reinterpret_cast<const i64*>(ev.ts.data()) : nullptr, ev.n,

@@ -263,7 +263,7 @@ struct Cand {  // histogram.hpp:65-74
 // skipped, missing-right tried before missing-left, strict '>' keeps the first maximum
 // (histogram.cpp:144-193).
 static Cand scan_feature(const Hist& H, int f, double lam, double gamma, int min_leaf,
-                         double min_hess, double l1) {
+                         double min_hess, double l1, int mono = 0) {
     Cand best;
     const int nb = H.nb;
     const double* G = &H.g[(size_t)f * nb];
@@ -292,7 +292,13 @@ static Cand scan_feature(const Hist& H, int f, double lam, double gamma, int min
             const i64 cr = tC - cl;
             if (cl < min_leaf || cr < min_leaf) continue;
             if (hl < min_hess || hr < min_hess) continue;
-            (void)weight_of;  // monotone constraints are not on this path
+            // monotone constraint of the feature (histogram.cpp:177-180): the leaf weights must
+            // keep its order
+            if (mono != 0) {
+                const double wl = weight_of(gl, hl, lam, l1), wr = weight_of(gr, hr, lam, l1);
+                if (mono > 0 && wl < wr) continue;
+                if (mono < 0 && wl > wr) continue;
+            }
             const double gain =
                 0.5 * (score_of(gl, hl, lam, l1) + score_of(gr, hr, lam, l1) - parent) - gamma;
             if (gain > best.gain) {
@@ -394,6 +400,7 @@ struct GrowCtx {
     const Cfg* cfg;
     const int* is_cat;   // per feature, or null
     const int* group_of; // per feature feature group (Booster::fit's group_of), or null
+    const int* monotone; // per feature monotone constraint (Booster::fit's monotone), or null
 };
 
 // Features a node may split on (tree.cpp:304-317): every feature, or under an interaction group
@@ -426,7 +433,7 @@ static Cand leaf_candidate(const GrowCtx& cx, const std::vector<int>& rows, cons
                        ? scan_categorical(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
                                           c.cat_smooth, c.l1)
                        : scan_feature(hist, f, c.l2, c.min_split_gain, c.min_samples_leaf, c.min_child_weight,
-                                      c.l1);
+                                      c.l1, cx.monotone ? cx.monotone[f] : 0);
     // feature groups (tree.cpp:135-158): a group is represented by its best-gain feature (the
     // first one in `active` order wins ties); groups numbered below 256 appear once
     std::map<int, int> group_best;
@@ -786,6 +793,7 @@ struct Model {
     std::vector<int> is_cat;
     std::vector<std::vector<double>> cat_values;
     std::vector<int> group_of;   // Booster::fit's group_of (feature groups), empty = none
+    std::vector<int> monotone;   // Booster::fit's monotone (booster.cpp:250-251), empty = none
 
     bool cat(int f) const { return !is_cat.empty() && is_cat[f] != 0; }
 
@@ -896,6 +904,7 @@ struct Model {
         }
         if (!is_cat.empty() && (int)is_cat.size() != F) throw std::runtime_error("is_cat: feature count mismatch");
         if (!group_of.empty() && (int)group_of.size() != F) throw std::runtime_error("group_of: feature count mismatch");
+        if (!monotone.empty() && (int)monotone.size() != F) throw std::runtime_error("monotone: feature count mismatch");
         cat_values.assign(F, {});
         for (int f = 0; f < F; ++f)
             if (cat(f)) {  // categorical features have no edges, only a vocabulary (binning.cpp:48-82)
@@ -985,7 +994,8 @@ struct Model {
                 hu = hh.data();
             }
             GrowCtx cx{xb.data(), F, nb, gu, hu, &c, is_cat.empty() ? nullptr : is_cat.data(),
-                       group_of.empty() ? nullptr : group_of.data()};
+                       group_of.empty() ? nullptr : group_of.data(),
+                       monotone.empty() ? nullptr : monotone.data()};
             (void)rng();  // the tree's own seed (booster.cpp:586-588); only colsample uses it
             Tree t = grow(cx, sample);
             const int nn = (int)t.nodes.size();
@@ -1098,7 +1108,8 @@ struct Model {
         for (int m = 0; m < extra; ++m) {
             grad(y, pred.data(), w.data(), nullptr, n, g.data(), h.data());
             GrowCtx cx{xb.data(), F, nb, g.data(), h.data(), &cfg, is_cat.empty() ? nullptr : is_cat.data(),
-                       group_of.empty() ? nullptr : group_of.data()};
+                       group_of.empty() ? nullptr : group_of.data(),
+                       monotone.empty() ? nullptr : monotone.data()};
             (void)rng();
             Tree t = grow(cx, all);
             const int nn = (int)t.nodes.size();
@@ -1308,6 +1319,13 @@ int orc_set_groups(void* hd, const int* group_of, int n_features) {
     return 0;
 }
 
+// Monotone constraints of later fits (Booster::fit's monotone: +1 / -1 / 0 per feature);
+// n_features = 0 clears.
+int orc_set_monotone(void* hd, const int* monotone, int n_features) {
+    static_cast<Model*>(hd)->monotone.assign(monotone, monotone + std::max(n_features, 0));
+    return 0;
+}
+
 // Interaction groups (Config::interaction_group, -1 = unconstrained); n = 0 clears.
 int orc_set_interaction(void* hd, const int* groups, int n) {
     static_cast<Model*>(hd)->cfg.interaction_group.assign(groups, groups + std::max(n, 0));
@@ -1387,6 +1405,23 @@ int orc_scan(const double* G, const double* H, const long long* C, int F, int nb
         if (c.feature >= 0 && c.gain >= gamma && c.gain > best.gain) best = c;
     }
     *best_feature = best.feature;
+    return 0;
+}
+
+// Best split of one feature with its monotone constraint (histogram.cpp:144-193), the same
+// entry point as the reference driver's tref_best_numeric.
+int orc_scan_feature(const double* G, const double* H, const long long* C, int F, int nb, int f, double lam,
+                     double gamma, int min_leaf, double min_hess, int monotone, double l1, double* gain, int* bin,
+                     int* miss_left) {
+    orc::Hist hist;
+    hist.init(F, nb);
+    std::copy(G, G + (size_t)F * nb, hist.g.begin());
+    std::copy(H, H + (size_t)F * nb, hist.h.begin());
+    std::copy(C, C + (size_t)F * nb, hist.c.begin());
+    const orc::Cand c = orc::scan_feature(hist, f, lam, gamma, min_leaf, min_hess, l1, monotone);
+    *gain = c.gain;
+    *bin = c.thr;
+    *miss_left = c.mgl;
     return 0;
 }
 

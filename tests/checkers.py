@@ -136,6 +136,29 @@ class CpuModel:
         self.lib.fn("set_cat")(self.h, _ptr(self._cat), self._cat.size)
         return self
 
+    def set_monotone(self, monotone):
+        """Booster::fit's monotone constraints of later fits (+1 / -1 / 0 per feature)."""
+        m = np.ascontiguousarray(monotone, np.int32)
+        if self.lib.fn("set_monotone")(self.h, _ptr(m), m.size) != 0:
+            raise ValueError(self.lib.error())
+        return self
+
+    def save(self, path, json=False):
+        """Reference core only: its TGBM v3 writer (serialize.cpp:65-148) or JSON dump."""
+        if self.lib.fn("save")(self.h, str(path).encode(), 1 if json else 0) != 0:
+            raise ValueError(self.lib.error())
+
+    @classmethod
+    def load(cls, lib, path):
+        """Reference core only: a model read by its TGBM v3 reader (serialize.cpp:150-255)."""
+        h = lib.lib.tref_load(str(path).encode())
+        if not h:
+            raise ValueError(lib.error())
+        m = cls.__new__(cls)
+        m.lib = lib
+        m.h = _P(h)
+        return m
+
     def set_groups(self, group_of=None, interaction=None):
         """Feature groups (Booster::fit's group_of) and interaction groups
         (Config::interaction_group) of later fits; None clears."""
@@ -180,6 +203,7 @@ class CpuLib:
             "best_iteration": (_I, [_P]), "set_eval": (_I, [_P, _P, _P, _P, _I, _I]),
             "tree_cat": (_I, [_P, _I, _P]), "set_cat": (_I, [_P, _P, _I]), "cat_values": (_I, [_P, _I, _P, _I]),
             "set_groups": (_I, [_P, _P, _I]), "set_interaction": (_I, [_P, _P, _I]),
+            "set_monotone": (_I, [_P, _P, _I]),
         }
         for k, (res, args) in sig.items():
             f = getattr(self.lib, f"{prefix}_{k}")
@@ -206,6 +230,7 @@ def oracle() -> CpuLib:
     L.orc_scan.argtypes = [_P, _P, _P, _I, _I, _D, _D, _I, _D, _D, _P, _P, _P, _P]
     L.orc_grad.argtypes = [_I, _P, _P, _P, _P, _I, _P, _P]
     L.orc_set_edges.argtypes = [_P, _P, _P, _I]
+    L.orc_scan_feature.argtypes = [_P, _P, _P, _I, _I, _I, _D, _D, _I, _D, _I, _D, _P, _P, _P]
     return lib
 
 

@@ -70,6 +70,7 @@ _sig("tide_evals_train", _I, _P, _P, _I)
 _sig("tide_evals_valid", _I, _P, _P, _I)
 _sig("tide_tree_cat", _I, _P, _I, _P)
 _sig("tide_set_feature_groups", _I, _P, _P, _I)
+_sig("tide_set_monotone", _I, _P, _P, _I)
 _sig("tide_set_interaction_groups", _I, _P, _P, _I)
 _sig("tide_get_cat_values", _I, _P, _I, _P, _I)
 _sig("tide_set_eval", _I, _P, _P, _P, _P, _I, _I)
@@ -102,7 +103,7 @@ _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_chil
 
 #: Every configuration key Booster.set_params accepts.
 KNOWN_KEYS = frozenset(_INT_KEYS | _DOUBLE_KEYS | {"device", "eval_metric", "objective", "interaction_group",
-                                                   "feature_groups"})
+                                                   "feature_groups", "monotone"})
 
 
 class TideError(ValueError):
@@ -147,9 +148,10 @@ class Booster:
             kb = k.encode()
             if k in ("device", "eval_metric"):
                 rc = _lib.tide_set_string(self._h, kb, str(v).encode())
-            elif k in ("interaction_group", "feature_groups"):
+            elif k in ("interaction_group", "feature_groups", "monotone"):
                 a = np.ascontiguousarray(v, dtype=np.int32)
-                fn = _lib.tide_set_interaction_groups if k == "interaction_group" else _lib.tide_set_feature_groups
+                fn = {"interaction_group": _lib.tide_set_interaction_groups,
+                      "feature_groups": _lib.tide_set_feature_groups, "monotone": _lib.tide_set_monotone}[k]
                 rc = fn(self._h, _ptr(a), a.size)
             elif k == "objective":
                 if isinstance(v, (int, np.integer)):

@@ -182,6 +182,7 @@ struct Shared {            // the session-wide matrix
     const unsigned char* bins_row;   // [n_rows][fp]
     const unsigned char* bins_col;   // [n_features][n_pad]
     const unsigned char* is_cat;     // [n_features] categorical flags, or null
+    const signed char* monotone;     // [n_features] monotone constraints (+1 / -1 / 0), or null
 };
 
 // A run of rows inside one TORB sweep segment (rows are time sorted, segment s = rows
@@ -206,6 +207,15 @@ __device__ inline double soft_thr(double G, double l1) {
 __device__ inline double leaf_score(double G, double H, double lam, double l1) {
     const double t = soft_thr(G, l1);
     return (t * t) / (H + lam);
+}
+// Leaf weight used by the monotone check of the split scan (histogram.cpp:136-142; no
+// max_delta_step clip there).
+__device__ inline double weight_of(double G, double H, double lam, double l1) {
+    const double den = H + lam;
+    if (l1 <= 0) return -G / den;
+    if (G > l1) return -(G - l1) / den;
+    if (G < -l1) return -(G + l1) / den;
+    return 0.0;
 }
 __device__ inline double leaf_value(double G, double H, double l2, double l1, double mds) {
     double v;

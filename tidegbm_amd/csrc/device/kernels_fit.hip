@@ -817,7 +817,7 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
 template <bool hconst, int T0, int T1>
 __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, const long long* qh,
                                            double inv_g, double inv_h, int nb, double lam, double gamma,
-                                           int min_leaf, double min_hess, double l1, double& out_gain,
+                                           int min_leaf, double min_hess, double l1, int mono, double& out_gain,
                                            int& out_thr, int& out_ml, int& out_cl, long long& totQ,
                                            long long& totQh, long long& totC) {
     const int lane = threadIdx.x & 63;
@@ -872,7 +872,12 @@ __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, co
             const double gl = (double)ql * inv_g, gr = (double)(totQ - ql) * inv_g;
             const double hl = hconst ? (double)cl : (double)qhl * inv_h;
             const double hr = hconst ? (double)cr : (double)(totQh - qhl) * inv_h;
-            cok[t * 2 + ml] = bin_ok & (cl >= min_leaf) & (cr >= min_leaf) & (hl >= min_hess) & (hr >= min_hess);
+            bool ok = bin_ok & (cl >= min_leaf) & (cr >= min_leaf) & (hl >= min_hess) & (hr >= min_hess);
+            if (mono != 0) {   // wave-uniform: the feature's monotone constraint (histogram.cpp:177-180)
+                const double wl = weight_of(gl, hl, lam, l1), wr = weight_of(gr, hr, lam, l1);
+                ok &= mono > 0 ? !(wl < wr) : !(wl > wr);
+            }
+            cok[t * 2 + ml] = ok;
             cg[t * 2 + ml] = 0.5 * (leaf_score(gl, hl, lam, l1) + leaf_score(gr, hr, lam, l1) - parent) - gamma;
             cclv[t * 2 + ml] = (int)cl;
         }
@@ -901,34 +906,36 @@ __device__ inline void scan_feature_wave(const long long* qg, const int* c, cons
                                          int min_leaf, double min_hess, double l1, double& out_gain,
                                          int& out_thr, int& out_ml, int& out_cl, long long& totQ,
                                          long long& totQh, long long& totC) {
+    // (PAIR splits: the reference scans the pair histogram without a monotone constraint,
+    // tree.cpp:268-269)
     if (hconst)
-        scan_feature_wave_t<true, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
-                                        out_thr, out_ml, out_cl, totQ, totQh, totC);
+        scan_feature_wave_t<true, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, 0,
+                                        out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
     else
-        scan_feature_wave_t<false, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, out_gain,
-                                         out_thr, out_ml, out_cl, totQ, totQh, totC);
+        scan_feature_wave_t<false, 0, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1, 0,
+                                         out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
 }
 // Half of the candidates (bins lane*4 + {0,1} for half 0, lane*4 + {2,3} for half 1): two waves
 // holding the same sums split one feature's scan.
 __device__ inline void scan_feature_half(const long long* qg, const int* c, const long long* qh, bool hconst,
                                          int half, double inv_g, double inv_h, int nb, double lam, double gamma,
-                                         int min_leaf, double min_hess, double l1, double& out_gain, int& out_thr,
+                                         int min_leaf, double min_hess, double l1, int mono, double& out_gain, int& out_thr,
                                          int& out_ml, int& out_cl, long long& totQ, long long& totQh,
                                          long long& totC) {
     if (hconst) {
         if (half == 0)
             scan_feature_wave_t<true, 0, 2>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
-                                            out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+                                            mono, out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
         else
             scan_feature_wave_t<true, 2, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
-                                            out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+                                            mono, out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
     } else {
         if (half == 0)
             scan_feature_wave_t<false, 0, 2>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
-                                             out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+                                             mono, out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
         else
             scan_feature_wave_t<false, 2, 4>(qg, c, qh, inv_g, inv_h, nb, lam, gamma, min_leaf, min_hess, l1,
-                                             out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
+                                             mono, out_gain, out_thr, out_ml, out_cl, totQ, totQh, totC);
     }
 }
 
@@ -1403,7 +1410,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
             }
         if (!catf)
             scan_feature_half(qg, c, qh, T.hconst, half, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf,
-                              P.min_hess, P.l1, gain, thr, ml, cl, tq, tqh, tc);
+                              P.min_hess, P.l1, S.monotone ? (int)S.monotone[f] : 0, gain, thr, ml, cl, tq, tqh, tc);
     }
     unsigned long long cmask[4] = {0, 0, 0, 0};
     if (catf)   // block-uniform: every wave takes the categorical scan's barriers

@@ -456,6 +456,24 @@ int tgbm_session_set_cat(tgbm_session* s, const int* is_cat) {
     });
 }
 
+int tgbm_session_set_monotone(tgbm_session* s, const int* monotone) {
+    return dev_guard([&] {
+        if (s->S.monotone) throw std::runtime_error("monotone constraints are set once per session");
+        if (!s->graphs.empty()) throw std::runtime_error("set monotone constraints before the first tree");
+        std::vector<signed char> m(s->F, 0);
+        bool any = false;
+        for (int f = 0; f < s->F; ++f) {
+            const int v = monotone ? monotone[f] : 0;
+            m[f] = (signed char)(v > 0 ? 1 : (v < 0 ? -1 : 0));
+            any |= m[f] != 0;
+        }
+        if (!any) return;
+        signed char* d = s->alloc<signed char>(s->F);
+        HIP_OK(hipMemcpy(d, m.data(), s->F, hipMemcpyHostToDevice));
+        s->S.monotone = d;
+    });
+}
+
 int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out) {
     return dev_guard([&] {
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");

@@ -75,6 +75,10 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
     const bool hconst = proto.objective == 0;
     SessionOwner so;
     dev_check(tgbm_session_create(&so.s, n_fit, F, n_bins, n_tasks, task_rows, proto.max_leaves, hconst ? 1 : 0));
+    if (!proto.monotone.empty()) {
+        if ((int)proto.monotone.size() != F) throw Error("monotone constraints need one entry per feature");
+        dev_check(tgbm_session_set_monotone(so.s, proto.monotone.data()));
+    }
     bin_on_device(so.s, X, edges);
     // ---- per-fold preparation, exactly as a plain fit of that fold; the folds' time layouts are
     // computed in parallel waves of one fold per host thread (the first failing fold's error is

@@ -769,6 +769,8 @@ std::shared_ptr<Booster> Booster::clone_model() const {
     m->init_prefix = init_prefix;
     m->edges = edges;
     m->is_cat = is_cat;
+    m->group_of = group_of;
+    m->feature_names = feature_names;
     m->cat_values = cat_values;
     m->trees = trees;
     m->fitted = fitted;
@@ -806,6 +808,9 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     cfg.validate();
     if (n <= 0 || F <= 0) throw Error("empty input");
     if (!group_of.empty() && (int)group_of.size() != F) throw Error("feature groups need one entry per feature");
+    if (!cfg.monotone.empty() && (int)cfg.monotone.size() != F)
+        throw Error("monotone constraints need one entry per feature");
+    feature_names.clear();   // the model file then names the features f0 .. f{F-1} (booster.cpp:244-245)
     is_cat.assign(F, 0);
     if (is_cat_in)
         for (int f = 0; f < F; ++f) is_cat[f] = is_cat_in[f] ? 1 : 0;
@@ -931,6 +936,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, F, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
     if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
+    if (!cfg.monotone.empty()) dev_check(tgbm_session_set_monotone(sg.s, cfg.monotone.data()));
     bin_on_device(sg.s, X, edges, is_cat, cat_values);
     const long long* groups = (cfg.objective == 5) ? rg : nullptr;
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, tl.pred.data(), tl.bucket.data(),
@@ -1081,6 +1087,7 @@ void Booster::update(const double* X, const double* y, int n, int nf, const doub
     SessionGuard sg;
     dev_check(tgbm_session_create(&sg.s, n, nf, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
     if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
+    if (!cfg.monotone.empty()) dev_check(tgbm_session_set_monotone(sg.s, cfg.monotone.data()));
     bin_on_device(sg.s, X, edges, is_cat, cat_values);
     std::vector<int> zeros(n, 0);
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? w : nullptr, pred.data(), zeros.data(), nullptr, 1, nullptr,

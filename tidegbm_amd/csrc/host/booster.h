@@ -62,6 +62,12 @@ struct Config {
     int regime_feature = -1;   // config.hpp:55-56: split on this feature when it is half as good
     int regime_priority = 1;
     std::vector<int> interaction_group;   // config.hpp:59: per feature, -1 = unconstrained
+    // Booster::fit's monotone argument (booster.cpp:250-251): per feature +1 / -1 / 0; a split
+    // whose leaf weights break the order is skipped (histogram.cpp:179-180).  Empty: none.
+    std::vector<int> monotone;
+    // objective parameters of the model file that this path does not use (config.hpp); kept so a
+    // loaded file saves back unchanged
+    double quantile = 0.5, huber_delta = 1.0, rank_temperature = 1.0;
     bool col_major_hist = true;
     std::string device = "gpu";
     int profile = 0;
@@ -135,6 +141,7 @@ public:
     std::vector<std::vector<double>> edges;
     std::vector<int> is_cat;                       // per feature (empty: none categorical)
     std::vector<int> group_of;                     // feature groups of later fits (Booster::fit's group_of)
+    std::vector<std::string> feature_names;        // names in the model file (f0 .. f{F-1} at fit)
     std::vector<std::vector<double>> cat_values;   // categorical features' sorted category values
     std::vector<Tree> trees;
     bool fitted = false;

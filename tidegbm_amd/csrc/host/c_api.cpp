@@ -308,6 +308,15 @@ int tide_set_feature_groups(TideHandle h, const int* group_of, int n_features) {
     });
 }
 
+int tide_set_monotone(TideHandle h, const int* monotone, int n_features) {
+    auto* b = static_cast<Booster*>(h);
+    return guard([&] {
+        if (n_features < 0 || (n_features > 0 && !monotone)) throw tide::Error("bad monotone constraints");
+        b->cfg.monotone.clear();
+        for (int f = 0; f < n_features; ++f) b->cfg.monotone.push_back(monotone[f] > 0 ? 1 : (monotone[f] < 0 ? -1 : 0));
+    });
+}
+
 int tide_set_interaction_groups(TideHandle h, const int* groups, int n_features) {
     auto* b = static_cast<Booster*>(h);
     return guard([&] {

@@ -98,7 +98,7 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
         if (a.max_depth != b.max_depth || a.max_leaves != b.max_leaves || a.min_samples_leaf != b.min_samples_leaf ||
             a.min_child_weight != b.min_child_weight || a.l2 != b.l2 || a.l1 != b.l1 ||
             a.max_delta_step != b.max_delta_step || a.min_split_gain != b.min_split_gain ||
-            a.learning_rate != b.learning_rate)
+            a.learning_rate != b.learning_rate || a.monotone != b.monotone)
             throw Error("ensembles of a server must share their tree-growth settings (refits are grown together)");
     }
     if (tgbm_device_count() < 1) throw Error("GPU device not available: no HIP device was found for device=gpu");
@@ -110,6 +110,7 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     // panel bins + refit slots in one session
     std::vector<int> caps(K_, refit_max_rows);
     dev_check(tgbm_session_create(&sess_, n, F, m0.n_bins, K_, caps.data(), cfg_.max_leaves, 1));
+    if (!cfg_.monotone.empty()) dev_check(tgbm_session_set_monotone(sess_, cfg_.monotone.data()));
     bin_on_device(sess_, X, m0.edges);
     dev_check(tgbm_predictor_create(&pred_, predict_budget));
     predict_cap_ = std::max<size_t>(predict_budget, 1 << 20);
