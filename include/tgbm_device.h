@@ -103,6 +103,13 @@ int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows);
 /* Exchange the state of tasks a and b (row windows, per-row arrays, trees in progress), so
  * the tasks a caller wants to grow can be kept in one contiguous active range. */
 int tgbm_task_swap(tgbm_session* s, int a, int b);
+/* Histogram of node `node` of the task's current (or last finished) tree, for checking: per
+ * (feature, bin), F x 256 entries, the exact fixed-point sums the split scan used.  The sum of
+ * g over the node's rows in the bin is qg * 2^-s_g; qh * 2^-s_h the sum of h (when h == 1 for
+ * every row, qh holds the counts and s_h = 0); count the rows.  qh may be NULL.  Valid for the
+ * nodes the tree has (built or derived as parent - sibling) until the next tgbm_tree_begin. */
+int tgbm_task_node_hist(tgbm_session* s, int task, int node, long long* qg, int* count, long long* qh,
+                        int* s_g, int* s_h);
 /* Row-major uint8 bins back to the host (n_rows x n_features), for checking. */
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out);
 

@@ -18,19 +18,32 @@ constexpr int MAX_RED_BLOCKS = 2048;
 // ---------------------------------------------------------------- fixed point
 // Histogram sums are exact integer sums of per-row quantized values, so they are independent
 // of summation order (deterministic under any atomic schedule) and sibling subtraction is
-// exact.  g is quantized as q = rint(g * 2^s) with |q| < 2^QG_BITS; in LDS one 64-bit word per
-// (feature, bin) packs count << 48 | q-sum (48-bit two's complement), valid while a block folds
-// at most MAX_FLUSH_ROWS rows (|sum| < 2^47, count < 2^16).  h (when not identically 1) gets
-// its own unsigned 64-bit word with |q_h| < 2^QH_BITS.
-constexpr int CNT_SHIFT = 48;
-constexpr int QG_BITS = 31;
+// exact.  g is quantized as q = rint(g * 2^s_g) with |q| <= 2^qg, qg = min(QG_BITS, 62 -
+// ceil(log2 rows)), so a node's int64 total cannot wrap; h (when not identically 1) likewise
+// with QH_BITS.  In LDS one 64-bit word per (feature, bin) packs count << CNT_SHIFT | q-sum
+// (53-bit two's complement), valid while the word takes at most WORD_ROWS rows: |sum| <=
+// 2^10 * 2^41 < 2^52 and count <= 1024 < 2^11.  Blocks that fold more rows keep an int64 g word
+// and a 32-bit count word per bin instead (the "wide" form).
+constexpr int CNT_SHIFT = 53;
+constexpr int QG_BITS = 41;
+constexpr int WORD_ROWS = 1024;
 constexpr int QH_BITS = 39;
 constexpr int QR_BITS = 39;        // refit / TORB sums over all rows (plain int64)
 constexpr int QE_BITS = 39;        // leaf energy sum of g^2
-constexpr int MAX_FLUSH_ROWS = 32768;
+constexpr int MAX_BLOCK_ROWS = 32768;   // rows one histogram block may fold (h sums: 2^15 * 2^39)
 constexpr int PRED_CHUNK = 1024;   // rows per block of the prediction update and its loss partial
 __host__ __device__ inline int loss_chunks(int rows) { return (rows + PRED_CHUNK - 1) / PRED_CHUNK; }
 constexpr unsigned long long CNT_ONE = 1ull << CNT_SHIFT;
+static_assert((1LL << (QG_BITS + 10)) < (1LL << (CNT_SHIFT - 1)) && WORD_ROWS < (1 << (64 - CNT_SHIFT)),
+              "packed word: q-sum and count of WORD_ROWS rows must fit their fields");
+
+// Bits per row of a fixed-point quantity summed over `rows` rows in int64: at most `cap`, and
+// rows * 2^bits <= 2^62.
+__host__ __device__ inline int sum_bits(int cap, long long rows) {
+    int l = 0;
+    while ((1LL << l) < rows) ++l;
+    return cap < 62 - l ? cap : 62 - l;
+}
 
 __host__ __device__ inline long long unpack_q(unsigned long long v) {
     return (long long)(v << (64 - CNT_SHIFT)) >> (64 - CNT_SHIFT);
@@ -123,7 +136,7 @@ struct Task {
     int cur_nl;                 // rows the current split sends left (-1: not known in advance)
     int err;                    // nonzero: an internal consistency check failed in this tree
     int hist_parts;            // partial histograms written by the last histogram launch
-    int pad_h;
+    int hist_wide;             // those partials are wide: int64 g sums in part, counts in part_c
     double train_loss;         // training loss after the last finished tree (exported)
     double* loss_part;         // [2 * loss_chunks(cap_rows)] per-chunk (sum w d^2, sum w), then
                                // [n_groups] per-group correlations (pairwise)
@@ -161,7 +174,9 @@ struct Task {
     int* hc;               // [MAX_NODES][F][NBIN]
     long long* hh;         // [MAX_NODES][F][NBIN] (only when !hconst)
     long long* energy;     // [MAX_NODES] fixed-point sum of g^2 (host-driven mode)
-    unsigned long long* part;    // [part_cap][F][NBIN] per-block partial histograms (packed)
+    unsigned long long* part;    // [part_cap][F][NBIN] per-block partial histograms (packed, or
+                                 // int64 g sums when the launch was wide)
+    unsigned int* part_c;        // [part_cap][F][NBIN] partial counts of wide launches
     unsigned long long* part_h;  // [part_cap][F][NBIN] partial h sums (only when !hconst)
     // ---- TORB
     const int* cuts;       // [n_buckets]

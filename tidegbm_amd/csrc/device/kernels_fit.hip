@@ -240,11 +240,14 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
         eg += es;
         eh += es;
     }
-    T.s_g = QG_BITS - eg;
-    T.s_h = QH_BITS - eh;
-    T.s_r = QR_BITS - mag_exponent(a);
-    T.s_rh = QR_BITS - mag_exponent(b);
-    T.s_e = QE_BITS - 2 * eg;
+    // bits per row shrink with the row count, so no int64 sum over the sample (histograms,
+    // energy) or over all rows (refit, TORB) can wrap
+    const int ns = max(T.n_sample, 1), nr = max(T.n_rows, 1);
+    T.s_g = sum_bits(QG_BITS, ns) - eg;
+    T.s_h = sum_bits(QH_BITS, ns) - eh;
+    T.s_r = sum_bits(QR_BITS, nr) - mag_exponent(a);
+    T.s_rh = sum_bits(QR_BITS, nr) - mag_exponent(b);
+    T.s_e = sum_bits(QE_BITS, ns) - 2 * eg;
     T.n_nodes = 1;
     T.done = 0;
     T.err = 0;
@@ -336,9 +339,10 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
 
 // Store a block's LDS histogram as partial number `slot` of the launch (plain stores; the scan
 // sums the partials exactly, so no global atomics are needed and nothing has to be zeroed).
+// lA holds packed words (lB the h sums when h varies).
 template <bool HV>
-__device__ inline void hist_store_partial(const Task& T, const unsigned long long* lA,
-                                          const unsigned long long* lB, int FB, int f0, int F, int slot) {
+__device__ inline void hist_store_partial(const Task& T, const unsigned long long* lA, const unsigned long long* lB,
+                                          int FB, int f0, int F, int slot) {
     const size_t base = (size_t)slot * F * NBIN + (size_t)f0 * NBIN;
     const int words = min(FB, F - f0) * NBIN;
     auto part = gbl(T.part);
@@ -349,46 +353,74 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
     }
 }
 
-// Row kernel (partitioned rows): each lane loads 16 consecutive feature bins of one row
-// (128-bit load); LPR lanes cover the block's FB = 16*LPR features of a row.  A lane issues the
-// index loads of RPL rows, then their bin and gradient loads, then 16*RPL LDS atomics, so one
-// block pass costs two memory round trips.  The block's row range is sized from the node's
-// actual row count, so small nodes still spread over many blocks.
+// Row kernel (partitioned rows, or a sampled root): each lane loads 16 consecutive feature bins
+// of one row (128-bit load); LPR lanes cover the block's FB = 16*LPR features of a row.  The
+// block walks its rows in steps of 8 rows per lane: the index loads of the 8 rows, then their bin
+// and gradient loads, then the LDS atomics of the packed operands, so one step costs two memory
+// round trips.  The block's row range is sized from the node's actual row count, so small nodes
+// still spread over many blocks.
+//
+// A packed LDS word may take at most WORD_ROWS rows.  A block of at most that many rows stores its
+// packed words as its partial.  A larger block (the "wide" form, the same choice in every block of
+// a launch) adds a step's rows in rounds of WORD_ROWS rows; after each round every thread folds the
+// words of its bins into int64 g sums and counts held in registers and clears them, and the
+// partial is stored from those registers (g sums in part, counts in part_c).  The h sums (h
+// varying) never come near a word's limit and stay in LDS.
+// LDS: FB*NBIN words for lA, then (HV) FB*NBIN words for lB.
 template <int LPR, bool HV>
 __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows) {
     constexpr int FB = 16 * LPR;
-    constexpr int RPL = 8;
+    constexpr int W = FB * NBIN;
+    constexpr int NT = 512;                  // threads per block
+    constexpr int RPL = 8;                   // rows per lane per step
+    constexpr int STRIDE = NT / LPR;         // rows per lane-row of a step
+    constexpr int UPR = WORD_ROWS / STRIDE;  // lane-rows per round of WORD_ROWS rows
+    constexpr int OWN = W / NT;              // bins each thread folds in the wide form
+    static_assert(RPL % UPR == 0 && W % NT == 0, "rounds and bin ownership");
     extern __shared__ unsigned long long lds[];
     unsigned long long* lA = lds;
-    unsigned long long* lB = lds + FB * NBIN;
+    unsigned long long* lB = lds + W;
     const Task T = tasks[blockIdx.z];
     int node, start, count, buf;
-    if (!hist_target(T, tasks + blockIdx.z, root != 0, blockIdx.x == 0 && blockIdx.y == 0, node, start, count, buf))
-        return;
+    const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
+    if (!hist_target(T, tasks + blockIdx.z, root != 0, leader, node, start, count, buf)) return;
+    (void)node;
     int per = (count + gridDim.x - 1) / gridDim.x;
     per = max(per, min_rows);
-    per = min((per + 63) & ~63, MAX_FLUSH_ROWS);
+    per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
+    const bool wide = per > WORD_ROWS;
+    if (leader && threadIdx.x == 0) {
+        tasks[blockIdx.z].hist_parts = (count + per - 1) / per;
+        tasks[blockIdx.z].hist_wide = wide ? 1 : 0;
+    }
     const int r0 = blockIdx.x * per;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) tasks[blockIdx.z].hist_parts = (count + per - 1) / per;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + per);
     const int f0 = blockIdx.y * FB;
-    for (int j = threadIdx.x; j < FB * NBIN * (HV ? 2 : 1); j += blockDim.x) lds[j] = 0;
+    for (int j = threadIdx.x; j < W * (HV ? 2 : 1); j += NT) lds[j] = 0;
     const int sub = threadIdx.x % LPR;
     const int fcol = f0 + sub * 16;
+    const bool active = fcol < S.fp;
     const auto rows = gbl((const int*)(buf ? T.idx_tmp : T.idx) + start);
     const auto pk = gbl((const unsigned long long*)T.pk);
     const auto qhp = gbl((const unsigned long long*)T.qh);
     const bool contig = T.contiguous && root;
-    const int stride = blockDim.x / LPR;
-    const bool active = fcol < S.fp;
+    long long aq[OWN];
+    unsigned int ac[OWN];
+#pragma unroll
+    for (int o = 0; o < OWN; ++o) {
+        aq[o] = 0;
+        ac[o] = 0;
+    }
     __syncthreads();
-    for (int k = r0 + (int)threadIdx.x / LPR; k < r1; k += stride * RPL) {
+    // every thread takes the same number of steps (the rounds synchronize the block)
+    for (int s0 = r0; s0 < r1; s0 += STRIDE * RPL) {
+        const int k = s0 + (int)threadIdx.x / LPR;
         int i[RPL];
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            const int kk = k + u * stride;
-            i[u] = kk < r1 ? (contig ? kk : rows[kk]) : -1;
+            const int kk = k + u * STRIDE;
+            i[u] = (kk < r1 && active) ? (contig ? kk : rows[kk]) : -1;
         }
         uint4 b[RPL];
         unsigned long long p[RPL], q[RPL];
@@ -399,133 +431,50 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
             p[u] = pk[ii];
             q[u] = HV ? qhp[ii] : 0ull;
         }
-        if (!active) continue;
 #pragma unroll
         for (int u = 0; u < RPL; ++u) {
-            if (i[u] < 0) continue;
-            const unsigned int w4[4] = {b[u].x, b[u].y, b[u].z, b[u].w};
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
-                const int slot = (sub * 16 + t) * NBIN + bin;
-                atomicAdd(&lA[slot], p[u]);
-                if (HV) atomicAdd(&lB[slot], q[u]);
-            }
-        }
-    }
-    __syncthreads();
-    (void)node;
-    hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
-}
-
-// Column kernel for the contiguous root: each lane loads 16 consecutive rows of each of the
-// block's FB feature columns (128-bit loads) plus their 16 packed gradients, then issues the
-// FB*16 LDS atomics back to back.  Blocks cover long row runs so per-block partial histograms
-// stay small next to the bytes read.
-//
-// With xcd_major set, the launch's blocks are renumbered so that the feature blocks of one row
-// chunk run on one XCD (workgroups are dealt to the 8 XCDs round-robin by linear id): the chunk's
-// packed gradients are then fetched into that XCD's L2 once instead of once per feature block.
-// K copies per bin (K > 1): lane l adds into copy l % K and the K copies of a bin sit in
-// consecutive words, so for K = 32 the 64 lanes of a wave spread over all LDS banks whatever their
-// bins; the copies are summed (exactly: integer adds) before the partial is stored.
-template <int FB, int K, bool HV>
-__global__ __launch_bounds__(K > 1 ? 1024 : 512) void k_hist_col(Shared S, Task* tasks, int rows_per_block,
-                                                                 int xcd_major) {
-    __shared__ unsigned long long lA[FB * NBIN * K];
-    __shared__ unsigned long long lB[HV ? FB * NBIN * K : 1];
-    const Task T = tasks[blockIdx.z];
-    const int count = T.n_sample;
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (xcd_major) {
-        const int nb = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * gridDim.x;
-        const int x = id % 8, l = id / 8;
-        const int base = nb / 8, rem = nb % 8;   // XCD x' holds base + (x' < rem) of the ids
-        const int j = x * base + min(x, rem) + l;
-        bx = j / gridDim.y;
-        by = j % gridDim.y;
-    }
-    const int r0 = bx * rows_per_block;
-    if (bx == 0 && by == 0 && threadIdx.x == 0)
-        tasks[blockIdx.z].hist_parts = (count + rows_per_block - 1) / rows_per_block;
-    if (r0 >= count) return;
-    const int r1 = min(count, r0 + rows_per_block);
-    const int f0 = by * FB;
-    const int nf = min(FB, S.n_features - f0);
-    for (int j = threadIdx.x; j < FB * NBIN * K; j += blockDim.x) {
-        lA[j] = 0;
-        if (HV) lB[j] = 0;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int rep = lane % K;
-    const unsigned char* colb = S.bins_col + T.row0;  // row0 is a multiple of 16 here (launch_hist)
-    for (int base = r0 + wave * 1024; base < r1; base += nwave * 1024) {
-        const int row = base + lane * 16;
-        if (base + 1024 <= r1 && nf == FB) {
-            // full tile: 16 rows per lane, no per-row checks
-            const uint4* pp = (const uint4*)(T.pk + row);
-            uint4 qv[8];
-#pragma unroll
-            for (int t = 0; t < 8; ++t) qv[t] = pp[t];
-            uint4 hv[HV ? 8 : 1];
-            if (HV) {
-                const uint4* qq = (const uint4*)(T.qh + row);
-#pragma unroll
-                for (int t = 0; t < 8; ++t) hv[t] = qq[t];
-            }
-            uint4 b[FB];
-#pragma unroll
-            for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(colb + (size_t)(f0 + fl) * S.n_pad + row);
-#pragma unroll
-            for (int fl = 0; fl < FB; ++fl) {
-                const unsigned w4[4] = {b[fl].x, b[fl].y, b[fl].z, b[fl].w};
+            if (i[u] >= 0) {
+                const unsigned int w4[4] = {b[u].x, b[u].y, b[u].z, b[u].w};
 #pragma unroll
                 for (int t = 0; t < 16; ++t) {
-                    const uint4 v = qv[t >> 1];
-                    const unsigned long long p = (t & 1) ? ((unsigned long long)v.w << 32 | v.z)
-                                                         : ((unsigned long long)v.y << 32 | v.x);
                     const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
-                    atomicAdd(&lA[(fl * NBIN + bin) * K + rep], p);
-                    if (HV) {
-                        const uint4 hw = hv[t >> 1];
-                        const unsigned long long q = (t & 1) ? ((unsigned long long)hw.w << 32 | hw.z)
-                                                             : ((unsigned long long)hw.y << 32 | hw.x);
-                        atomicAdd(&lB[(fl * NBIN + bin) * K + rep], q);
-                    }
+                    const int slot = (sub * 16 + t) * NBIN + bin;
+                    atomicAdd(&lA[slot], p[u]);
+                    if (HV) atomicAdd(&lB[slot], q[u]);
                 }
             }
-        } else if (row < r1) {
-            // tail tile: per-row checks
-            for (int fl = 0; fl < nf; ++fl) {
-                const unsigned char* col = colb + (size_t)(f0 + fl) * S.n_pad;
-                for (int t = 0; t < 16 && row + t < r1; ++t) {
-                    const int bin = col[row + t];
-                    atomicAdd(&lA[(fl * NBIN + bin) * K + rep], T.pk[row + t]);
-                    if (HV) atomicAdd(&lB[(fl * NBIN + bin) * K + rep], T.qh[row + t]);
+            if (wide && (u % UPR) == UPR - 1) {
+                // a round of WORD_ROWS rows is in: fold the packed words into registers
+                __syncthreads();
+#pragma unroll
+                for (int o = 0; o < OWN; ++o) {
+                    const unsigned long long v = lA[threadIdx.x + o * NT];
+                    lA[threadIdx.x + o * NT] = 0;
+                    aq[o] += unpack_q(v);
+                    ac[o] += unpack_c(v);
                 }
+                __syncthreads();
             }
         }
     }
     __syncthreads();
-    if (K == 1) {
-        hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, bx);
+    if (!wide) {
+        hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
         return;
     }
-    // Sum the copies of each bin.  Lane j starts at copy j / (32 / K): the 32 lanes j..j+31 of a
-    // step then read 32 different bank pairs.
-    const size_t pbase = (size_t)bx * S.n_features * NBIN + (size_t)f0 * NBIN;
-    for (int j = threadIdx.x; j < nf * NBIN; j += blockDim.x) {
-        const int k0 = (j / (32 / K > 0 ? 32 / K : 1)) % K;
-        unsigned long long a = 0, c = 0;
+    const size_t base = (size_t)blockIdx.x * S.n_features * NBIN + (size_t)f0 * NBIN;
+    const int words = min(FB, S.n_features - f0) * NBIN;
+    auto part = gbl(T.part);
+    auto part_c = gbl(T.part_c);
+    auto part_h = gbl(T.part_h);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int r = (k0 + k) % K;
-            a += lA[j * K + r];
-            if (HV) c += lB[j * K + r];
+    for (int o = 0; o < OWN; ++o) {
+        const int j = threadIdx.x + o * NT;
+        if (j < words) {
+            part[base + j] = (unsigned long long)aq[o];
+            part_c[base + j] = ac[o];
+            if (HV) part_h[base + j] = lB[j];
         }
-        T.part[pbase + j] = a;
-        if (HV) T.part_h[pbase + j] = c;
     }
 }
 
@@ -534,11 +483,12 @@ __global__ __launch_bounds__(K > 1 ? 1024 : 512) void k_hist_col(Shared S, Task*
 // passes of FB = 64 / K features (32 / K when h varies): per pass it adds into a K-copy LDS
 // histogram (lane l updates copy l % K, and the K copies of a bin sit in consecutive words, so a
 // wave's adds spread over the LDS banks whatever their bins) while the next pass's bin columns
-// load.  After a pass the threads sum the copies of each bin.  The copies are never cleared: they
-// keep accumulating over the passes, and a thread stores the difference from its previous sum
-// (exact modulo 2^64, the packed words' own arithmetic; one pass adds at most 16384 rows, so the
-// packed count cannot carry).  Every lane takes the same path through the passes, so the next
-// pass's loads stay in flight during this pass's adds.
+// load.  A copy takes 1024 rows per pass (64 lanes x 16 rows), within a packed word's WORD_ROWS.
+// After a pass each thread unpacks the copies of one bin into an int64 g sum and a count and
+// clears them for the next pass (a copy's 1024 rows of one pass fit a packed word; the 16384 rows
+// of all copies would not, so each copy is unpacked on its own).  The block's partial is wide
+// (g sums and counts in two planes).  Every lane takes the same path through the passes, so the
+// next pass's loads stay in flight during this pass's adds.
 constexpr int ROOT_RPL = 16;
 constexpr int ROOT_THREADS = 1024;
 constexpr int ROOT_ROWS = ROOT_THREADS * ROOT_RPL;
@@ -578,8 +528,10 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
         bx = j / gridDim.y;
         by = j % gridDim.y;
     }
-    if (bx == 0 && by == 0 && threadIdx.x == 0)
+    if (bx == 0 && by == 0 && threadIdx.x == 0) {
         tasks[blockIdx.z].hist_parts = (count + ROOT_ROWS - 1) / ROOT_ROWS;
+        tasks[blockIdx.z].hist_wide = 1;
+    }
     const int r0 = bx * ROOT_ROWS;
     const int fr0 = by * feats_per_block;
     const int fr1 = min(S.n_features, fr0 + feats_per_block);
@@ -620,27 +572,34 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
     }
     uint4 bc[FB], bn[FB];
     root_load_bins<FB>(bc, colb, S.n_pad, fr0, fr1, ld_row);
-    // Sums: thread t owns bins w = t + i * ROOT_THREADS (i < SPT) of the pass, w < WORDS in lA and
-    // w >= WORDS in lB; it starts at copy (w / (32 / K)) % K, so 32 lanes read 32 bank pairs.
-    // Stores go through a global-segment pointer, so they count only against the vector memory
-    // counter.  A pass's sums are stored at the start of the next pass, ahead of that pass's
-    // loads: the wait for those loads then never waits on a store issued after them.
+    // Sums: thread t owns word w = t of the pass: bin w of the g copies when w < WORDS, else bin
+    // w - WORDS of the h copies.  Stores go through a global-segment
+    // pointer, so they count only against the vector memory counter.  A pass's sums are stored at
+    // the start of the next pass, ahead of that pass's loads: the wait for those loads then never
+    // waits on a store issued after them.
+    static_assert(SPT == 1, "one word per thread and pass");
     const auto outA = gbl(T.part);
+    const auto outC = gbl(T.part_c);
     const auto outB = gbl(T.part_h);
     const size_t pbase = (size_t)bx * S.n_features * NBIN;
-    unsigned long long prev[SPT], pend[SPT];
-#pragma unroll
-    for (int i = 0; i < SPT; ++i) prev[i] = pend[i] = 0;
+    const int w = (int)threadIdx.x;
+    const bool gword = w < WORDS;
+    const int ws = gword ? w : w - WORDS;   // bin of the pass
+    const int pr0 = (w / 2) % (K / 2);
+    long long pend_q = 0;
+    unsigned int pend_c = 0;
     int pend_f = -1;
     __syncthreads();
     auto flush = [&]() {
         if (pend_f < 0) return;
         const int nw = min(FB, fr1 - pend_f) * NBIN;
-#pragma unroll
-        for (int i = 0; i < SPT; ++i) {
-            const int w = (int)threadIdx.x + i * ROOT_THREADS;
-            const int ws = w < WORDS ? w : w - WORDS;   // bin of the pass
-            if (ws < nw) (w < WORDS ? outA : outB)[pbase + (size_t)pend_f * NBIN + ws] = pend[i];
+        if (ws >= nw) return;
+        const size_t o = pbase + (size_t)pend_f * NBIN + ws;
+        if (gword) {
+            outA[o] = (unsigned long long)pend_q;
+            outC[o] = pend_c;
+        } else {
+            outB[o] = (unsigned long long)pend_q;
         }
     };
     auto pass = [&](const uint4(&b)[FB], int f) {
@@ -657,16 +616,35 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
             }
         }
         __syncthreads();
+        // The bin's copies in pairs (16-byte LDS accesses), read, cleared and unpacked in batches
+        // (all eight pairs at once when h == 1; one at a time when h varies, which holds more
+        // registers).  Thread w starts at pair (w / 2) % 8, so the 16 lanes an LDS cycle serves
+        // touch 16 different 16-byte bank groups.
+        constexpr int BATCH = HV ? 2 : 8;
+        long long q = 0;
+        unsigned int c = 0;
+        uint4* cp = (uint4*)(lds + (size_t)w * K);
 #pragma unroll
-        for (int i = 0; i < SPT; ++i) {
-            const int w = (int)threadIdx.x + i * ROOT_THREADS;
-            const int k0 = (w / (32 / K)) % K;
-            unsigned long long sum = 0;
+        for (int j0 = 0; j0 < K / 2; j0 += BATCH) {
+            uint4 v[BATCH];
 #pragma unroll
-            for (int k = 0; k < K; ++k) sum += lds[w * K + (k0 + k) % K];
-            pend[i] = sum - prev[i];
-            prev[i] = sum;
+            for (int j = 0; j < BATCH; ++j) v[j] = cp[(pr0 + j0 + j) % (K / 2)];
+#pragma unroll
+            for (int j = 0; j < BATCH; ++j) cp[(pr0 + j0 + j) % (K / 2)] = make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int j = 0; j < BATCH; ++j) {
+                const unsigned long long a = (unsigned long long)v[j].y << 32 | v[j].x;
+                const unsigned long long b = (unsigned long long)v[j].w << 32 | v[j].z;
+                if (gword) {
+                    q += unpack_q(a) + unpack_q(b);
+                    c += unpack_c(a) + unpack_c(b);
+                } else {
+                    q += (long long)(a + b);   // h copies hold plain sums
+                }
+            }
         }
+        pend_q = q;
+        pend_c = c;
         pend_f = f;
         __syncthreads();
     };
@@ -699,75 +677,23 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                  hipEvent_t e1) {
     if (max_count <= 0) return;
     if (root && contiguous_root && (variant & 0xff) != 2) {
-        // Root variants (variant >> 20; the child variants use bits 0-15).  0: k_hist_root with
-        // XCD-major placement and the features per block sized so the grid about fills the GPU
-        // once.  Bits 4-6 == 6:
-        // k_hist_root with fpb = 16 << bits 7-8 and bit 3 XCD-major placement.  Otherwise the
-        // single-pass k_hist_col: bit 0 selects 16 features per block (else 8), bits 1-2 a 512-
-        // or 1024-thread block (else 256; 1024 only with copies), bit 3 XCD-major placement, bits
-        // 4-6 log2 of the copies per bin K (0, or 2..5: K = 4..32 with 64 / K features per block,
-        // 32 / K when h varies; 128 KB of LDS).
+        // Root variants (variant >> 20; the child variants use bits 0-15), all k_hist_root with
+        // 16384 rows per block.  0: features per block sized so the grid about fills the GPU once,
+        // XCD-major placement.  Otherwise 16 << bits 7-8 features per block and bit 3 XCD-major
+        // placement.
         const int rv = (variant >> 20) & 0x3ff;
-        const int lk = (rv >> 4) & 7;
-        if (rv == 0 || lk == 6) {
-            // k_hist_root: 16384 rows x fpb features per block
-            int fpb = 16 << ((rv >> 7) & 3);
-            int xcd = (rv >> 3) & 1;
-            if (rv == 0) {
-                // smallest power of two in [4, 64] with at most ~one block per CU (256 CUs)
-                const long long want = (long long)((max_count + ROOT_ROWS - 1) / ROOT_ROWS) * n_tasks * S.n_features;
-                fpb = 4;
-                while (fpb < 64 && want > 256LL * fpb) fpb *= 2;
-                xcd = 1;
-            }
-            dim3 grid((max_count + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
-            if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
-            else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
-            return;
+        int fpb = 16 << ((rv >> 7) & 3);
+        int xcd = (rv >> 3) & 1;
+        if (rv == 0) {
+            // smallest power of two in [4, 64] with at most ~one block per CU (256 CUs)
+            const long long want = (long long)((max_count + ROOT_ROWS - 1) / ROOT_ROWS) * n_tasks * S.n_features;
+            fpb = 4;
+            while (fpb < 64 && want > 256LL * fpb) fpb *= 2;
+            xcd = 1;
         }
-        if (lk >= 2 && lk <= 5) {
-            const int K = 1 << lk;
-            const int FB = (hconst ? 64 : 32) / K;
-            const int threads = 256 << std::min(2, (rv >> 1) & 3);
-            const int xcd = (rv >> 3) & 1;
-            const int fbs = (S.n_features + FB - 1) / FB;
-            int rpb = (int)(((long long)max_count * fbs / 256 + 4095) / 4096 * 4096);
-            rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, rpb));
-            const int need = (int)(((long long)max_count + S.part_cap - 1) / S.part_cap);
-            if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
-            dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
-#define ROOT_REP(KK)                                                                                          \
-    do {                                                                                                      \
-        if (hconst) TIMED_LAUNCH((k_hist_col<64 / KK, KK, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd); \
-        else TIMED_LAUNCH((k_hist_col<32 / KK, KK, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);      \
-    } while (0)
-            switch (K) {
-                case 4: ROOT_REP(4); break;
-                case 8: ROOT_REP(8); break;
-                case 16: ROOT_REP(16); break;
-                default: ROOT_REP(32); break;
-            }
-#undef ROOT_REP
-            return;
-        }
-        const int FB = (rv & 1) ? 16 : 8;
-        const int threads = 256 << std::min(1, (rv >> 1) & 3);
-        const int xcd = (rv >> 3) & 1;
-        // ~2 blocks per CU over the whole matrix; rows per block a multiple of the 4-wave tile
-        const int fbs = (S.n_features + FB - 1) / FB;
-        int rpb = (int)(((long long)max_count * fbs / 512 + 4095) / 4096 * 4096);
-        rpb = std::min(MAX_FLUSH_ROWS, std::max(4096, rpb));
-        // at most part_cap partials (part_cap >= rows / MAX_FLUSH_ROWS by construction)
-        const int need = (int)(((long long)max_count + S.part_cap - 1) / S.part_cap);
-        if (need > rpb) rpb = std::min(MAX_FLUSH_ROWS, (need + 1023) / 1024 * 1024);
-        dim3 grid((max_count + rpb - 1) / rpb, fbs, n_tasks);
-        if (FB == 8) {
-            if (hconst) TIMED_LAUNCH((k_hist_col<8, 1, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-            else TIMED_LAUNCH((k_hist_col<8, 1, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-        } else {
-            if (hconst) TIMED_LAUNCH((k_hist_col<16, 1, false>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-            else TIMED_LAUNCH((k_hist_col<16, 1, true>), grid, dim3(threads), 0, st, e0, e1, S, tasks, rpb, xcd);
-        }
+        dim3 grid((max_count + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
+        if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
+        else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
         return;
     }
     // Row kernel: LPR lanes per row (16 features each), 512 threads.  Default LPR = 1 (16
@@ -780,11 +706,14 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         lpr = 1 << std::min(2, (cv - 16) / 8);
         min_rows = 256 << std::min(7, (cv - 16) % 8);
     }
-    if (!hconst && lpr > 2) lpr = 2;   // two LDS words per bin: 64 features would not fit
+    // at most 32 features per block (the wide form holds 8 * LPR bins per thread in registers),
+    // 16 when h varies
+    if (!hconst) lpr = 1;
+    if (lpr > 2) lpr = 2;
     const int FB = 16 * lpr;
     const int fbs = (S.fp + FB - 1) / FB;
-    // Blocks per feature block: no block folds more than MAX_FLUSH_ROWS rows, and there are at most
-    // part_cap partials (part_cap >= 2 * rows / MAX_FLUSH_ROWS + 1 > max_count / MAX_FLUSH_ROWS).
+    // Blocks per feature block: no block folds more than MAX_BLOCK_ROWS rows, and there are at most
+    // part_cap partials (part_cap >= 2 * rows / MAX_BLOCK_ROWS + 1 > max_count / MAX_BLOCK_ROWS).
     // At most 32 partials by default: more blocks speed the build up less than the extra
     // partials slow the scan's sums down (whole-tree sweep at 1M x 128: 32 -> 836 trees/s, 64 ->
     // 787, 16 -> 745).  Bits 12-13 of the variant pick 32, 64, 16 or 8.
@@ -793,19 +722,14 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     constexpr int pcaps[4] = {32, 64, 16, 8};
     const int pcap = pcaps[(variant >> 12) & 3];
     int gx = std::max(1, std::min(pcap, (max_count + min_rows - 1) / min_rows));
-    gx = std::max(gx, (max_count + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS);
+    gx = std::max(gx, (max_count + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS);
     gx = std::min(gx, S.part_cap);
     dim3 grid(gx, fbs, n_tasks);
     const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
     const int rt = root ? 1 : 0;
-    if (hconst) {
-        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-        else if (lpr == 2) TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-        else TIMED_LAUNCH((k_hist_row<4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    } else {
-        if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-        else TIMED_LAUNCH((k_hist_row<2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    }
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
 }
 
 // =================================================================== split scan
@@ -1147,7 +1071,8 @@ __device__ void apply_nodes(const Task& T, Task* Tw, int nl) {
 }
 
 // Exact sums, over partials k = k0, k0 + dk, ... < np of the last histogram launch, of bins
-// lane*4 .. lane*4+3 of feature f.  Four partials are loaded per round so their latencies overlap.
+// lane*4 .. lane*4+3 of feature f: packed words, or (wide launches) int64 g sums with their
+// counts in part_c.  Four partials are loaded per round so their latencies overlap.
 __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk, int np, long long sq[4],
                                     long long sc[4], long long sh[4]) {
     const int lane = threadIdx.x & 63;
@@ -1160,31 +1085,66 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
         sh[t] = 0;
     }
     int k = k0;
-    for (; k + 3 * dk < np; k += 4 * dk) {
-        uint4 v[8];
+    if (T.hist_wide) {
+        for (; k + 3 * dk < np; k += 4 * dk) {
+            uint4 v[8], c[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
-            v[2 * u] = ldg(a);
-            v[2 * u + 1] = ldg(a + 1);
+            for (int u = 0; u < 4; ++u) {
+                const size_t o = (size_t)(k + u * dk) * stride + off;
+                const uint4* a = (const uint4*)(T.part + o);
+                v[2 * u] = ldg(a);
+                v[2 * u + 1] = ldg(a + 1);
+                c[u] = ldg((const uint4*)(T.part_c + o));
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                sq[0] += (long long)((unsigned long long)v[2 * u].y << 32 | v[2 * u].x);
+                sq[1] += (long long)((unsigned long long)v[2 * u].w << 32 | v[2 * u].z);
+                sq[2] += (long long)((unsigned long long)v[2 * u + 1].y << 32 | v[2 * u + 1].x);
+                sq[3] += (long long)((unsigned long long)v[2 * u + 1].w << 32 | v[2 * u + 1].z);
+                sc[0] += c[u].x;
+                sc[1] += c[u].y;
+                sc[2] += c[u].z;
+                sc[3] += c[u].w;
+            }
         }
+        for (; k < np; k += dk) {
+            const size_t o = (size_t)k * stride + off;
+            const auto w = gbl((const long long*)T.part + o);
+            const auto c = gbl((const unsigned int*)T.part_c + o);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const unsigned long long w0 = (unsigned long long)v[u].y << 32 | v[u].x;
-            const unsigned long long w1 = (unsigned long long)v[u].w << 32 | v[u].z;
-            const int t = (u & 1) * 2;
-            sq[t] += unpack_q(w0);
-            sc[t] += unpack_c(w0);
-            sq[t + 1] += unpack_q(w1);
-            sc[t + 1] += unpack_c(w1);
+            for (int t = 0; t < 4; ++t) {
+                sq[t] += w[t];
+                sc[t] += c[t];
+            }
         }
-    }
-    for (; k < np; k += dk) {
-        const auto w = gbl((const unsigned long long*)T.part + (size_t)k * stride + off);
+    } else {
+        for (; k + 3 * dk < np; k += 4 * dk) {
+            uint4 v[8];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            sq[t] += unpack_q(w[t]);
-            sc[t] += unpack_c(w[t]);
+            for (int u = 0; u < 4; ++u) {
+                const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
+                v[2 * u] = ldg(a);
+                v[2 * u + 1] = ldg(a + 1);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const unsigned long long w0 = (unsigned long long)v[u].y << 32 | v[u].x;
+                const unsigned long long w1 = (unsigned long long)v[u].w << 32 | v[u].z;
+                const int t = (u & 1) * 2;
+                sq[t] += unpack_q(w0);
+                sc[t] += unpack_c(w0);
+                sq[t + 1] += unpack_q(w1);
+                sc[t + 1] += unpack_c(w1);
+            }
+        }
+        for (; k < np; k += dk) {
+            const auto w = gbl((const unsigned long long*)T.part + (size_t)k * stride + off);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                sq[t] += unpack_q(w[t]);
+                sc[t] += unpack_c(w[t]);
+            }
         }
     }
     if (!T.hconst) {
@@ -1749,7 +1709,7 @@ __global__ __launch_bounds__(64) void k_pair_scan(Shared S, Task* tasks, int tas
 void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int node, int count,
                  const int* pf, int n_pairs, long long* ph_g, int* ph_c, long long* ph_h,
                  bool hconst, const GrowP& P, FCand* out) {
-    const int rpb = 4096;
+    const int rpb = WORD_ROWS;   // a block's packed words then take at most WORD_ROWS rows
     const int blocks = std::max(1, (count + rpb - 1) / rpb);
     const size_t lds = (size_t)n_pairs * NBIN * 8 * (hconst ? 1 : 2);
     hipLaunchKernelGGL(k_pair_hist, dim3(blocks), dim3(256), lds, st, S, tasks, task, node, pf,
@@ -2166,8 +2126,6 @@ void init_kernel_attributes() {
     (void)hipFuncSetAttribute((const void*)k_hist_row<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_hist_row<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_hist_row<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     done = true;
 }

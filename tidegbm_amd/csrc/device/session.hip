@@ -358,8 +358,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         int max_task_rows = 1;
         for (int t = 0; t < n_tasks; ++t) max_task_rows = std::max(max_task_rows, task_rows ? task_rows[t] : n_rows);
         // Partials per launch: launches of the split loop are sized for the largest sample rounded
-        // up to a power of two (< 2x the rows), and no block may fold more than MAX_FLUSH_ROWS rows.
-        s->S.part_cap = std::max(64, (int)((2LL * max_task_rows + MAX_FLUSH_ROWS - 1) / MAX_FLUSH_ROWS) + 1);
+        // up to a power of two (< 2x the rows), and no block may fold more than MAX_BLOCK_ROWS rows.
+        s->S.part_cap = std::max(64, (int)((2LL * max_task_rows + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
         s->tasks_h.assign(n_tasks, Task{});
         s->th.assign(n_tasks, TaskHost{});
         const size_t per_node = (size_t)n_features * NBIN;
@@ -399,6 +399,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * s->max_nodes);
             T.energy = s->alloc<long long>(MAX_NODES);
             T.part = s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
+            T.part_c = s->alloc<unsigned int>((size_t)s->S.part_cap * per_node);
             T.part_h = s->hconst ? nullptr : s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
             T.cuts = s->alloc<int>(MAX_BUCKETS);
             const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
@@ -575,6 +576,28 @@ int tgbm_task_swap(tgbm_session* s, int a, int b) {
         std::swap(s->th[a], s->th[b]);
         s->upload_chunks(a);
         s->upload_chunks(b);
+    });
+}
+
+int tgbm_task_node_hist(tgbm_session* s, int task, int node, long long* qg, int* count, long long* qh, int* s_g,
+                        int* s_h) {
+    return dev_guard([&] {
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        if (node < 0 || node >= s->max_nodes) throw std::runtime_error("node out of range");
+        if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
+        const Task& T = s->tasks_h[task];
+        const size_t words = (size_t)s->F * NBIN, off = (size_t)node * words;
+        Task dev;
+        HIP_OK(hipMemcpyAsync(&dev, s->tasks_d + task, sizeof(Task), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(qg, T.hg + off, words * sizeof(long long), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(count, T.hc + off, words * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        if (qh && T.hh)
+            HIP_OK(hipMemcpyAsync(qh, T.hh + off, words * sizeof(long long), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        if (qh && !T.hh)
+            for (size_t k = 0; k < words; ++k) qh[k] = count[k];
+        *s_g = dev.s_g;
+        *s_h = T.hh ? dev.s_h : 0;
     });
 }
 

@@ -15,9 +15,9 @@ std::map<std::string, int>& cache() {
 }
 
 // Child-build tiles (bits 0-15 of the variant): defaults (16 features and >= 1024 rows per
-// block, at most 32 partials), 512- or 2048-row minimum tiles, 32 or 64 features per block
-// (64 only when h = 1; with h varying it means 32).
-const std::vector<int> kChild = {0, 17, 19, 26, 34};
+// block, at most 32 partials), 512- or 2048-row minimum tiles, 32 features per block (only when
+// h = 1; with h varying it means 16).
+const std::vector<int> kChild = {0, 17, 19, 26};
 // Root-build variants (bits 20-29): defaults (features per block sized to the grid), and the
 // multi-pass root kernel with 16, 32 or 64 features per block, XCD-major placement.
 const int kRootFpb16 = ((6 << 4) | (0 << 7) | 8) << 20;
