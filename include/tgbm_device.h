@@ -208,6 +208,16 @@ int tgbm_forest_append(tgbm_forest** out, const tgbm_forest* base, int keep_tree
 /* Predict raw scores for host rows X (row-major f64) or for device-resident uint8 bins. */
 int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out);
 int tgbm_forest_predict_binned_u16(tgbm_forest* f, const unsigned short* xb, int n, double* out);
+/* tgbm_forest_predict that allocates nothing (no counterpart in the reference): for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same kernels and bit-identical scores; the staging of rows, bins and
+ * scores lives in a fixed TGBM_NOALLOC_SCRATCH_BYTES array on each GPU, held until the process
+ * ends, so n may be at most (TGBM_NOALLOC_SCRATCH_BYTES - 1040 - 64 fp) / (8 F + fp + 8) rows,
+ * fp = F rounded up to 16 (72,307 rows at F = 128); larger n is rejected with an error.  Calls
+ * on one GPU share the array and would race if they overlapped: the library runs them one at a
+ * time.  The forest must be resident already (tgbm_forest_create). */
+#define TGBM_NOALLOC_SCRATCH_BYTES (80u << 20)
+int tgbm_forest_predict_noalloc(tgbm_forest* f, const double* X, int n, double* out);
 
 /* ---- batched predict over many resident ensembles (serving) ----
  * A predictor owns a stream and pinned/device staging memory.  tgbm_predict_batch scores all

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -294,6 +295,80 @@ int tgbm_forest_predict(tgbm_forest* f, const double* X, int n, double* out) {
         launch_bin(f->stream, f->Xd, n, f->F, f->edges, f->n_edges, f->bins, f->fp, nullptr, n + 64);
         FHIP(hipGetLastError());
         forest_run(f, n, out);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Predict without allocating: the staging of tgbm_forest_predict (raw rows, their bins, the
+// scores) lives in `scratch`, one file-scope array per GPU that the process holds until it ends.
+// In host code `scratch` names a host-side stand-in; the array's device address comes from
+// hipGetSymbolAddress, once per device, and only that address is handed to copies and kernels.
+namespace {
+
+__device__ __align__(256) unsigned char scratch[TGBM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu[kMaxDevices];   // one call at a time per device (the calls share the array)
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's scratch array, looked up on the device's first call.
+unsigned char* scratch_on(int dev) {
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char*)ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+size_t up256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Rows the scratch holds for a forest: raw rows (8 F bytes each), bins (fp bytes each, plus 64
+// rows and 16 bytes of slack) and scores (8 bytes each), each region rounded up to 256 bytes.
+int noalloc_max_rows(const tgbm_forest* f) {
+    const size_t per = (size_t)8 * f->F + f->fp + 8;
+    const size_t n = (TGBM_NOALLOC_SCRATCH_BYTES - 4 * 256 - 16 - (size_t)64 * f->fp) / per;
+    return (int)std::min<size_t>(n, 1 << 30);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tgbm_forest_predict_noalloc(tgbm_forest* f, const double* X, int n, double* out) {
+    return fguard([&] {
+        if (!f) throw std::runtime_error("no forest");
+        if (n < 0) throw std::runtime_error("n < 0");
+        if (n == 0) return;
+        const int max_rows = noalloc_max_rows(f);
+        if (n > max_rows)
+            throw std::runtime_error("tgbm_forest_predict_noalloc: " + std::to_string(n) + " rows exceed the " +
+                                     std::to_string(max_rows) + " rows of " + std::to_string(f->F) +
+                                     " features its fixed scratch holds (" +
+                                     std::to_string(TGBM_NOALLOC_SCRATCH_BYTES >> 20) + " MiB)");
+        int dev = 0;
+        FHIP(hipGetDevice(&dev));
+        if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device id out of range");
+        std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+        unsigned char* base = scratch_on(dev);
+        double* Xd = (double*)base;
+        unsigned char* bins = base + up256(sizeof(double) * (size_t)n * f->F);
+        double* outd = (double*)(bins + up256((size_t)(n + 64) * f->fp + 16));
+        // the forest's own stream when a plain predict created it, else the null stream
+        hipStream_t st = f->stream;
+        FHIP(hipMemcpyAsync(Xd, X, sizeof(double) * (size_t)n * f->F, hipMemcpyHostToDevice, st));
+        FHIP(hipMemsetAsync(bins, 0, (size_t)n * f->fp, st));
+        launch_bin(st, Xd, n, f->F, f->edges, f->n_edges, bins, f->fp, nullptr, n + 64);
+        FHIP(hipGetLastError());
+        hipLaunchKernelGGL(k_forest, dim3((n + 255) / 256), dim3(256), 0, st, f->nodes, f->flags, f->values, f->roots,
+                           f->lr, f->n_trees, f->init, bins, f->fp, n, f->nb, f->cmask, outd);
+        FHIP(hipGetLastError());
+        FHIP(hipMemcpyAsync(out, outd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        FHIP(hipStreamSynchronize(st));
     });
 }
 
