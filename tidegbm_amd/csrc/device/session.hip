@@ -111,7 +111,13 @@ struct tgbm_session {
     FCand* pair_out_d = nullptr;
     // captured device-driven split loops (one graph launch per tree), keyed by what the capture
     // baked in: largest sample, leaf budget, histogram variant and the active task range
-    std::map<std::vector<long long>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    struct CachedGraph {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        long long last_use = 0;
+    };
+    std::map<std::vector<long long>, CachedGraph> graphs;
+    long long graph_clock = 0;
     bool use_graphs = true;
     // small device scratch for descriptor tables of launches made on behalf of other modules
     char* small_scratch = nullptr;
@@ -144,8 +150,8 @@ struct tgbm_session {
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
         for (auto& kv : graphs) {
-            (void)hipGraphExecDestroy(kv.second.second);
-            (void)hipGraphDestroy(kv.second.first);
+            (void)hipGraphExecDestroy(kv.second.ge);
+            (void)hipGraphDestroy(kv.second.g);
         }
         for (auto& e : events) {
             (void)hipEventDestroy(e.a);
@@ -891,27 +897,31 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
         const std::vector<long long> key{s->launch_rows(), p->max_leaves, p->hist_mode, s->a0, s->an};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
-            if (!s->pending.empty()) HIP_OK(hipStreamSynchronize(s->stream));   // capture on an idle stream
-            if (s->graphs.size() >= 256) {  // bound the cache; captures are cheap to redo
-                for (auto& kv : s->graphs) {
-                    HIP_OK(hipGraphExecDestroy(kv.second.second));
-                    HIP_OK(hipGraphDestroy(kv.second.first));
-                }
-                s->graphs.clear();
+            // Capture on a drained stream, and destroy an executable graph only once no launch of
+            // it can still be running: the cache is bounded by evicting the least recently used
+            // graph after a stream synchronize.
+            HIP_OK(hipStreamSynchronize(s->stream));
+            if (s->graphs.size() >= 256) {
+                auto lru = s->graphs.begin();
+                for (auto j = s->graphs.begin(); j != s->graphs.end(); ++j)
+                    if (j->second.last_use < lru->second.last_use) lru = j;
+                HIP_OK(hipGraphExecDestroy(lru->second.ge));
+                HIP_OK(hipGraphDestroy(lru->second.g));
+                s->graphs.erase(lru);
             }
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
+            tgbm_session::CachedGraph c;
             HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < p->max_leaves - 1; ++k) split_step(s, P, 0, p);
-            HIP_OK(hipStreamEndCapture(s->stream, &g));
-            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            HIP_OK(hipStreamEndCapture(s->stream, &c.g));
+            const hipError_t e = hipGraphInstantiate(&c.ge, c.g, nullptr, nullptr, 0);
             if (e != hipSuccess) {
-                (void)hipGraphDestroy(g);
+                (void)hipGraphDestroy(c.g);
                 HIP_OK(e);
             }
-            it = s->graphs.emplace(key, std::make_pair(g, ge)).first;
+            it = s->graphs.emplace(key, c).first;
         }
-        HIP_OK(hipGraphLaunch(it->second.second, s->stream));
+        it->second.last_use = ++s->graph_clock;
+        HIP_OK(hipGraphLaunch(it->second.ge, s->stream));
     });
 }
 
