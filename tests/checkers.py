@@ -337,8 +337,11 @@ def tie_split(x, y, xb, k, n_bins=256):
     return bool(np.array_equal(lx, ly) or np.array_equal(lx, m & ~ly))
 
 
-def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins=256):
+def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins=256, stats=None):
     """First structural difference between two tree lists, or None.
+
+    `stats`, when given (a dict), counts the accepted exceptions below by kind and lists them
+    as (tree, node, kind) under "where".
 
     Integer node fields must match exactly; value/gain/values/prefix within tolerance.
 
@@ -359,6 +362,12 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
     """
     if len(a) != len(b):
         return f"tree count {len(a)} != {len(b)}"
+
+    def note(t, k, kind):
+        if stats is not None:
+            stats[kind] = stats.get(kind, 0) + 1
+            stats.setdefault("where", []).append((t, int(k), kind))
+
     for t, (x, y) in enumerate(zip(a, b)):
         if x["ints"].shape != y["ints"].shape:
             return f"tree {t}: node count {x['ints'].shape[0]} != {y['ints'].shape[0]}"
@@ -378,12 +387,15 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
                 if np.array_equal(m & goes_left(xi), m & goes_left(yi)):
                     tied.add(k)
             keep = []
+            for k in sorted(tied):
+                note(t, k, "feature_tie")
             for k, c in bad:
                 if k in tied and c in (2, 4, 7):
                     continue
                 if c == 7 and xi[k, 1] == 0 and xi[k, 0] == 0:
                     m = rows_reaching(xi, xb, k, n_bins, x.get("cat"))
                     if not np.any(xb[m, xi[k, 2]] == 0):
+                        note(t, k, "missing_go_left")
                         continue
                 keep.append((k, c))
             bad = np.array(keep).reshape(-1, 2)
@@ -400,6 +412,7 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
                 lx, ly = leaves_of(x["ints"], xb, n_bins, xc), leaves_of(y["ints"], xb, n_bins, yc)
                 if (np.allclose(x["values"][lx], y["values"][ly], rtol=rtol, atol=atol)
                         and np.array_equal(x["cover"][lx], y["cover"][ly])):
+                    note(t, k, "mirrored_or_categorical_tie")
                     continue
         if bad.size:
             k, c = bad[np.argmin(bad[:, 0])]
@@ -415,3 +428,26 @@ def compare_trees(a, b, rtol=1e-6, atol=1e-9, check_values=True, xb=None, n_bins
                 d = np.max(np.abs(x["dbl"] - y["dbl"]))
                 return f"tree {t}: node value/gain differ (max abs {d:g})"
     return None
+
+
+def leaf_values(tree, xb, n_bins=256):
+    """Per row of bins xb, the value of the leaf the tree sends it to (values[leaf])."""
+    return tree["values"][leaves_of(tree["ints"], xb, n_bins, tree.get("cat"))]
+
+
+def held_out_differences(a, b, xb, rtol=1e-7, atol=1e-9, stats=None, n_bins=256):
+    """Compare two tree lists on rows they were not fitted on (bins xb), tree by tree.
+
+    Returns (trees whose leaf values differ on some row although the checker accepted no exception
+    in them, boolean mask of rows whose leaf value differs in some tree).  A row may differ only in
+    a tree where compare_trees accepted an exception (stats["where"])."""
+    excepted = {t for t, _, _ in (stats or {}).get("where", [])}
+    unexplained, rows = [], np.zeros(xb.shape[0], bool)
+    for t, (x, y) in enumerate(zip(a, b)):
+        va, vb = leaf_values(x, xb, n_bins), leaf_values(y, xb, n_bins)
+        d = ~np.isclose(va, vb, rtol=rtol, atol=atol)
+        if d.any():
+            rows |= d
+            if t not in excepted:
+                unexplained.append(t)
+    return unexplained, rows
