@@ -92,9 +92,41 @@ def test_estimator_categorical_feature_indices():
     assert m.booster_.cat_values(7).size == 40
     with pytest.raises(ValueError, match="not in X"):
         TideRegressor(n_estimators=2, iid_mode=True).fit(p.X, p.y, categorical_feature=[p.n_features])
-    with pytest.raises(TideError, match="categorical"):
-        from tidegbm_amd import Server
-        Server([m.booster_], p.X[:1000], refit_max_rows=500)
+
+
+def test_categorical_ensembles_are_served(tmp_path):
+    # The batched serving kernel routes categorical splits by their bin masks: served scores of
+    # panel rows and of raw rows (unseen categories and NaN included) equal the one-call predict
+    # bit for bit, and the reference core reading the same model files; refits of categorical
+    # ensembles are refused (their splits are chosen by the host-driven grower).
+    from tidegbm_amd import Server
+    p, is_cat = cat_panel(12)
+    models = []
+    for k, y in enumerate((p.y, p.y + 0.5 * np.nan_to_num(p.X[:, 0]))):
+        b = Booster(n_estimators=12, max_depth=6, learning_rate=0.1, goss=0, pair_splits=0)
+        b.fit(p.X, y, timestamps=p.ts, entity=p.entity, horizon=1, is_cat=is_cat)
+        models.append(b)
+    assert any((t["ints"][:, 1] == 2).any() for t in models[0].trees()), "the case should have categorical splits"
+    X = p.X[:2500].copy()
+    X[:40, 7] = 1234.0
+    X[40:60, 11] = np.nan
+    srv = Server(models, p.X, predict_budget=64 << 20, refit_slots=2, refit_max_rows=2000)
+    try:
+        rows = np.arange(300, 2800, dtype=np.int32)
+        for m, b in enumerate(models):
+            a, _, _ = srv.wait(srv.predict(m, rows))
+            c, _, _ = srv.wait(srv.predict_x(m, X))
+            assert np.array_equal(a, b.predict(p.X[rows]))
+            assert np.array_equal(c, b.predict(X))
+            ref = checkers.reference()
+            if ref is not None:
+                b.save(tmp_path / f"m{m}.tgbm")
+                r = checkers.CpuModel.load(ref, tmp_path / f"m{m}.tgbm")
+                assert np.max(np.abs(c - r.predict(X)) / np.maximum(np.abs(r.predict(X)), 1e-12)) <= 1e-12
+        with pytest.raises(TideError, match="categorical"):
+            srv.refit(0, 0, p.y[:1000], extra_trees=2)
+    finally:
+        srv.close()
 
 
 @pytest.mark.parametrize("case", sorted(GROUP_CASES))

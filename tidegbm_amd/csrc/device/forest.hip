@@ -156,13 +156,19 @@ static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node
             h.nodes[off + k] = nd;
             h.flags[off + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
             SNode sn{0, 0, 0, 0, 0, h.flags[off + k]};
-            if (!leaf && v[1] < 2) {
-                if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
+            if (!leaf) {
                 sn.f = (unsigned short)v[2];
-                sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
-                sn.thr = (unsigned char)v[4];
                 sn.left = (unsigned char)v[5];
                 sn.right = (unsigned char)v[6];
+                if (v[1] < 2) {
+                    if (v[4] < 0 || v[4] > 255) throw std::runtime_error("split threshold outside the uint8 bin range");
+                    sn.f2 = (unsigned short)(v[1] == 1 ? v[3] : 0);
+                    sn.thr = (unsigned char)v[4];
+                } else {
+                    // categorical: f2 holds the index of the node's bin mask in the forest's table
+                    if (nd.thr > 0xffff) throw std::runtime_error("more than 65536 categorical nodes in a forest");
+                    sn.f2 = (unsigned short)nd.thr;
+                }
             }
             h.snodes[off + k] = sn;
         }

@@ -47,6 +47,7 @@ constexpr size_t SERVE_MAX_LDS = 152 * 1024;   // dynamic LDS opt-in (below 160 
 
 struct SReq {
     const SNode* nodes;
+    const unsigned long long* cmask;   // bin masks of categorical nodes (4 words each), or null
     const int* tree_off;
     const double* values;
     const double* lr;
@@ -114,7 +115,9 @@ __global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, in
                 while (!(nd.flags & 1)) {
                     const int b1 = row[nd.f];
                     bool left;
-                    if (((nd.flags >> 1) & 3) == 1) {
+                    if (((nd.flags >> 1) & 3) >= 2) {   // categorical: the bin's bit in the node's mask
+                        left = (R.cmask[(size_t)nd.f2 * 4 + (b1 >> 6)] >> (b1 & 63)) & 1ull;
+                    } else if (((nd.flags >> 1) & 3) == 1) {
                         const int b2 = row[nd.f2];
                         if (b1 == 0 || b2 == 0) left = (nd.flags >> 3) & 1;
                         else left = (int)(((unsigned)b1 * 2654435761u + (unsigned)b2 * 40503u) % (unsigned)R.nb) <= (int)nd.thr;
@@ -215,7 +218,6 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
             const tgbm_predict_item& it = items[i];
             const tgbm_forest* f = it.forest;
             if (!f) throw std::runtime_error("predict item without an ensemble");
-            if (f->n_cat) throw std::runtime_error("the batched predict kernel does not route categorical splits");
             if (it.n < 0) throw std::runtime_error("predict item with n < 0");
             const int from_x = it.rows ? 0 : 1;
             if (!from_x) {
@@ -243,6 +245,7 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
             const tgbm_forest* f = it.forest;
             SReq& R = reqs[i];
             R.nodes = f->snodes;
+            R.cmask = f->cmask;
             R.tree_off = f->tree_off;
             R.values = f->values;
             R.lr = f->lr;
@@ -294,7 +297,6 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
 int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f) {
     return sguard([&] {
         if (!f) throw std::runtime_error("no ensemble");
-        if (f->n_cat) throw std::runtime_error("the batched predict kernel does not route categorical splits");
         const unsigned char* pbins = nullptr;
         int pfp = 0, pn = 0, pF = 0;
         session_panel_info(s, &pbins, &pfp, &pn, &pF);
@@ -313,6 +315,7 @@ int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f) {
         }
         SReq R;
         R.nodes = f->snodes;
+        R.cmask = f->cmask;
         R.tree_off = f->tree_off;
         R.values = f->values;
         R.lr = f->lr;

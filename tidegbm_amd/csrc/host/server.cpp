@@ -85,12 +85,13 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     const Booster& m0 = *models[0];
     for (const Booster* m : models) {
         if (!m || !m->fitted) throw Error("every ensemble must be fitted");
-        if (m->has_cat()) throw Error("the serving kernel does not route categorical splits");
         if (!m->cfg.interaction_group.empty() || m->cfg.regime_feature >= 0)
             throw Error("serving refits grow on the device: ensembles need no interaction groups or regime feature");
         if (m->n_features != F) throw Error("ensemble and panel feature counts differ");
-        if (m->edges != m0.edges || m->n_bins != m0.n_bins)
-            throw Error("all ensembles of a server must share bin edges (one binned panel serves them all)");
+        if (m->edges != m0.edges || m->n_bins != m0.n_bins || m->is_cat != m0.is_cat ||
+            (m->has_cat() && m->cat_values != m0.cat_values))
+            throw Error("all ensembles of a server must share bin edges and categories (one binned panel serves "
+                        "them all)");
         const Config& a = m->cfg;
         const Config& b = m0.cfg;
         if (a.objective != 0) throw Error("serving refits support the mse objective only");
@@ -111,7 +112,8 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     std::vector<int> caps(K_, refit_max_rows);
     dev_check(tgbm_session_create(&sess_, n, F, m0.n_bins, K_, caps.data(), cfg_.max_leaves, 1));
     if (!cfg_.monotone.empty()) dev_check(tgbm_session_set_monotone(sess_, cfg_.monotone.data()));
-    bin_on_device(sess_, X, m0.edges);
+    bin_on_device(sess_, X, m0.edges, m0.is_cat, m0.cat_values);
+    has_cat_ = m0.has_cat();
     dev_check(tgbm_predictor_create(&pred_, predict_budget));
     predict_cap_ = std::max<size_t>(predict_budget, 1 << 20);
     for (const Booster* m : models) {
@@ -208,6 +210,8 @@ long long Server::submit_refit(int model, int row0, int n, const double* y, int 
     if (n < 1 || n > cap_rows_) throw Error("refit rows exceed the server's refit slot capacity");
     if (row0 < 0 || (long long)row0 + n > n_rows_) throw Error("refit window outside the panel");
     if (extra_trees < 1) throw Error("extra_trees must be >= 1");
+    // categorical splits are decided by the host-driven grower, which the refit slots do not run
+    if (has_cat_) throw Error("refits of ensembles with categorical features are not served (fit them with update)");
     auto r = std::make_shared<Req>();
     r->kind = 2;
     r->model = model;

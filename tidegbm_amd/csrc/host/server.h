@@ -83,6 +83,7 @@ private:
     void unbatched_loop();
 
     bool batched_;
+    bool has_cat_ = false;   // the ensembles have categorical features (predicts only)
     std::chrono::steady_clock::time_point t0_;
     Config cfg_;
     int F_ = 0, n_rows_ = 0, cap_rows_ = 0, K_ = 1;
