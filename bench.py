@@ -15,8 +15,14 @@ HBin); HIP events around every launch on the session stream give the average dur
 root build (k_hist_root, m = n, no row index) is reported beside it.  `traffic` comes from the
 FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
 
+The root build is timed twice: as the fit runs (its 128 MB of bins may then sit in the 256 MB
+Infinity Cache in front of HBM) and with a 512 MiB buffer read and rewritten before every root
+build (profile mode 2), so that its inputs come from HBM; roofline_root reports the latter.
+
 CPU baseline: the reference core (oracle/_ref, built from the reference's own C++ sources) on
-the host's cores, same panel, timed as (fit of 1+C trees) - (fit of 1 tree) over C trees.
+the host's cores, same panel, timed as (fit of 1+C trees) - (fit of 1 tree) over C trees, after
+a warm-up fit, median of 3 runs with all the threads OMP_NUM_THREADS allows, and one run on one
+thread.  The ranking and the small panel carry their own (single-run) reference figures.
 
 Usage: python bench.py [--gpus 1] [--steps 190] [--warmup 10] [--cpu-trees 6]
 """
@@ -41,29 +47,54 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _ref_fit_seconds(ref, p, params, n_trees, threads, fit_kw):
+    m = ref.model(**{**params, "n_estimators": n_trees, "n_jobs": threads})
+    t0 = time.perf_counter()
+    m.fit(p.X, p.y, **fit_kw)
+    return time.perf_counter() - t0
+
+
+def ref_trees_per_s(ref, p, params, cpu_trees, threads, reps, fit_kw, tag):
+    """Reference core trees/s: (fit(1 + C) - fit(1)) / C, the median over `reps` runs."""
+    per = []
+    for _ in range(reps):
+        t1 = _ref_fit_seconds(ref, p, params, 1, threads, fit_kw)
+        tn = _ref_fit_seconds(ref, p, params, 1 + cpu_trees, threads, fit_kw)
+        per.append((tn - t1) / cpu_trees)
+        log(f"[cpu] {tag}: fit(1) {t1:.2f}s fit({1 + cpu_trees}) {tn:.2f}s on {threads} threads")
+    per_tree = float(sorted(per)[len(per) // 2])
+    return 1.0 / per_tree, per
+
+
 def cpu_baseline(p, params, cpu_trees):
-    """Reference core on the host cores; returns a cpu_baseline dict or None."""
+    """Reference core on the host's cores (and on one thread); returns a cpu_baseline dict."""
     try:
         import checkers
     except Exception:
         return None
     ref = checkers.reference()
-    kind = "reference"
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if ref is None:
         return None
-    times = {}
-    for nt in (1, 1 + cpu_trees):
-        m = ref.model(**{**params, "n_estimators": nt, "n_jobs": threads})
-        t0 = time.perf_counter()
-        m.fit(p.X, p.y, ts=p.ts, entity=p.entity, horizon=1)
-        times[nt] = time.perf_counter() - t0
-        log(f"[cpu] reference fit {nt} trees: {times[nt]:.2f}s")
-    per_tree = (times[1 + cpu_trees] - times[1]) / cpu_trees
-    return {"value": round(1.0 / per_tree, 4), "unit": "trees/s", "cores": threads, "kind": kind,
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    fit_kw = dict(ts=p.ts, entity=p.entity, horizon=1)
+    _ref_fit_seconds(ref, p, params, 1, threads, fit_kw)   # warm-up (page cache, thread pool)
+    v, runs = ref_trees_per_s(ref, p, params, cpu_trees, threads, 3, fit_kw, "reference 1M x 128")
+    v1, _ = ref_trees_per_s(ref, p, params, 1, 1, 1, fit_kw, "reference 1M x 128, one thread")
+    return {"value": round(v, 4), "unit": "trees/s", "cores": threads, "kind": "reference",
             "sample": f"reference core, panel 1,000,000 x 128, {cpu_trees} trees after the first "
-                      f"(fit({1 + cpu_trees}) - fit(1)), {threads} OpenMP threads",
-            "setup_s": round(times[1] - per_tree, 3)}
+                      f"(fit({1 + cpu_trees}) - fit(1)), {threads} OpenMP threads, median of 3 after a warm-up fit",
+            "runs_s_per_tree": [round(x, 4) for x in runs], "one_thread": round(v1, 4),
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
 
 
 def _traffic():
@@ -103,9 +134,15 @@ def run_plain(args, p):
     ms0, n0, by0 = bp.profile(0)   # root histogram builds
     ms5, n5, by5 = bp.profile(5)   # child histogram builds
     ms1, n1, _ = bp.profile(1)     # partitions (with the split selection)
-    root_ms = ms0 / max(n0, 1)
-    root_bytes = by0 / max(n0, 1)
+    # the root builds again with the Infinity Cache flushed before each (profile mode 2)
+    bf = t.Booster(n_estimators=n_prof, profile=2, **params)
+    log(f"[bench] profiled fit {n_prof} trees, caches flushed before each root build")
+    bf.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, horizon=1)
+    msf, nf, byf = bf.profile(0)
+    root_ms = msf / max(nf, 1)
+    root_bytes = byf / max(nf, 1)
     root_gbs = root_bytes / (root_ms * 1e-3) / 1e9
+    warm_root_ms = ms0 / max(n0, 1)
     child_ms = ms5 / max(n5, 1)
     child_bytes = by5 / max(n5, 1)
     achieved = child_bytes / (child_ms * 1e-3) / 1e9
@@ -144,11 +181,13 @@ def run_plain(args, p):
         "roofline_root": {
             "bound": "hbm", "achieved": round(root_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(root_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("k_hist_root"),
-            "kernel": "root histogram build (k_hist_root), 1,000,000 x 128",
-            "bytes_per_launch": root_bytes, "avg_launch_ms": round(root_ms, 5), "launches": n0,
+            "kernel": "root histogram build (k_hist_root), 1,000,000 x 128, Infinity Cache flushed before each",
+            "bytes_per_launch": root_bytes, "avg_launch_ms": round(root_ms, 5), "launches": nf,
+            "avg_launch_ms_unflushed": round(warm_root_ms, 5),
+            "frac_unflushed": round(root_bytes / (warm_root_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         },
         "hist_tree_gbs": round(tree_gbs, 1),
-        "device_ms_per_tree": {"root_hist": round(ms0 / max(n0, 1), 4),
+        "device_ms_per_tree": {"root_hist": round(warm_root_ms, 4),
                                "child_hist": round(ms5 / max(n0, 1), 4),
                                "partition_select": round(ms1 / max(n0, 1), 4)},
         "fit_setup_s": round(setup_s, 3),
@@ -338,6 +377,21 @@ def run_ranking(args):
     setup_s, boost_s = b.fit_times()
     tt = b.tree_times()
     host_trees, wait_s, sample_s = b.goss_stats()
+    cpu = None
+    if not args.no_cpu:
+        try:
+            import checkers
+            ref = checkers.reference()
+        except Exception:
+            ref = None
+        if ref is not None:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            rp = dict(max_depth=6, learning_rate=0.08, objective=5)
+            v, _ = ref_trees_per_s(ref, p, rp, 2, threads, 1,
+                                   dict(ts=p.ts, entity=p.entity, rank_groups=p.ts, horizon=1), "reference ranking")
+            cpu = {"value": round(v, 4), "unit": "trees/s", "cores": threads, "kind": "reference",
+                   "sample": "reference core, ranking panel, fit(3) - fit(1), one run (memory-mapped row-major "
+                             "histogram path at this size)"}
     return {
         "workload": f"ranking panel: 5,000 dates x U(200,1000) entities = {p.n:,} rows x 64 features, "
                     f"pairwise within dates, {args.rank_trees} trees, depth 6, lr 0.08, GOSS auto (on), "
@@ -347,7 +401,43 @@ def run_ranking(args):
         "ms_per_tree_steady": round(1e3 * float(tt[-1] - tt[min(4, len(tt) - 1)]) / max(1, len(tt) - 5), 3),
         "goss_host_trees": host_trees, "goss_wait_ms_per_tree": round(1e3 * wait_s / args.rank_trees, 3),
         "goss_sample_ms_per_tree": round(1e3 * sample_s / args.rank_trees, 3),
+        "cpu_baseline": cpu,
     }
+
+
+def run_small(args):
+    """The reference's CPU-runnable case: 500 dates x 30 entities = 15,000 rows x 20 AR(1) features
+    (phi 0.9), mse, 80 trees, depth 6, lr 0.08, max_bin 255, GOSS off, horizon 1, PAIR splits on
+    (the reference default below 1M rows: growth is driven from the host, with device histograms
+    and scans)."""
+    import tidegbm_amd as t
+    from tidegbm_amd import synth
+
+    p = synth.panel_small(0)
+    params = dict(max_depth=6, learning_rate=0.08, max_bin=255, goss=0)
+    b = t.Booster(n_estimators=80, **params)
+    t0 = time.perf_counter()
+    b.fit(p.X, p.y, timestamps=p.ts, entity=p.entity, horizon=1)
+    wall = time.perf_counter() - t0
+    setup_s, boost_s = b.fit_times()
+    n_pair = sum(int((tr["ints"][:, 1] == 1).sum()) for tr in b.trees())
+    cpu = None
+    if not args.no_cpu:
+        try:
+            import checkers
+            ref = checkers.reference()
+        except Exception:
+            ref = None
+        if ref is not None:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            v, _ = ref_trees_per_s(ref, p, params, 79, threads, 3, dict(ts=p.ts, entity=p.entity, horizon=1),
+                                   "reference small panel")
+            cpu = {"value": round(v, 2), "unit": "trees/s", "cores": threads, "kind": "reference",
+                   "sample": "reference core, 15,000 x 20 panel, fit(80) - fit(1), median of 3"}
+    return {"workload": "small panel: 500 dates x 30 entities = 15,000 rows x 20 features, mse, 80 trees, "
+                        "depth 6, lr 0.08, GOSS off, PAIR splits on (host-driven growth)",
+            "trees_per_s": round(80 / boost_s, 1), "fit_wall_s": round(wall, 4), "setup_s": round(setup_s, 4),
+            "pair_splits": n_pair, "cpu_baseline": cpu}
 
 
 def main():
@@ -358,7 +448,8 @@ def main():
     ap.add_argument("--cpu-trees", type=int, default=6)
     ap.add_argument("--profile-trees", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["all", "plain", "walkforward", "serve", "ranking"], default="all")
+    ap.add_argument("--workload", choices=["all", "plain", "walkforward", "serve", "ranking", "small"],
+                    default="all")
     ap.add_argument("--rank-trees", type=int, default=100, help="trees of the ranking workload")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving request stream")
     ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
@@ -384,6 +475,9 @@ def main():
     if args.workload == "ranking":
         print(json.dumps({"ranking": run_ranking(args)}), flush=True)
         return
+    if args.workload == "small":
+        print(json.dumps({"small": run_small(args)}), flush=True)
+        return
     log("[bench] generating the 1M x 128 panel")
     p = synth.panel_large(1)
     if args.workload == "walkforward":
@@ -396,6 +490,9 @@ def main():
     if args.workload == "all":
         out["walk_forward"] = run_walkforward(args, p)
         out["serve"] = run_serve(args, p)
+        del p
+        out["ranking"] = run_ranking(args)
+        out["small"] = run_small(args)
     print(json.dumps(out), flush=True)
 
 

@@ -178,7 +178,9 @@ int tgbm_task_train_loss(tgbm_session* s, int task, double* loss);
 
 /* Timers (HIP events on the session stream) summed per kernel family since the last reset:
  * which = 0 root histogram, 1 partition (with the split selection), 2 split scan, 3 tree end,
- * 4 gradients, 5 child histograms. */
+ * 4 gradients, 5 child histograms.  on = 2 also reads and rewrites a 512 MiB buffer before every
+ * root histogram build (outside its timed region), so the root build's inputs come from HBM and
+ * not from the 256 MB Infinity Cache. */
 int tgbm_timer_enable(tgbm_session* s, int on);
 int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches,
                     double* bytes);

@@ -67,7 +67,8 @@ int tide_get_edges(TideHandle h, int feature, double* out, int cap);
 int tide_get_init_prefix(TideHandle h, double* out, int cap);
 /* Per-kernel-family device time of the last fit (ms, launches, algorithmic bytes); family 0 =
  * root histograms, 5 = child histograms, 1 = partitions, 3 = tree end, 4 = gradients.
- * Collected only when the int key "profile" was set to 1 before the fit. */
+ * Collected only when the int key "profile" was set to 1 before the fit (2: as 1, and the
+ * Infinity Cache is flushed before every root histogram build, outside its timed region). */
 int tide_profile(TideHandle h, int family, double* ms, long long* launches, double* bytes);
 /* Per-launch log of a profiled fit's kernel family: milliseconds and rows per launch, in launch
  * order.  Returns the number of launches (at most cap copied). */

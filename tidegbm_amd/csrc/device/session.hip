@@ -135,6 +135,11 @@ struct tgbm_session {
     int next_slot = 0;
     // timers
     bool timing = false;
+    // timing mode 2: a 512 MiB buffer read and rewritten before every root histogram build, so
+    // the build's inputs come from HBM and not from the 256 MB Infinity Cache
+    bool flush = false;
+    unsigned long long* flush_buf = nullptr;
+    static constexpr size_t kFlushBytes = size_t(512) << 20;
     std::vector<EventPair> events;
     double t_ms[8] = {0};
     long long t_n[8] = {0};
@@ -149,6 +154,7 @@ struct tgbm_session {
     }
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (flush_buf) (void)hipFree(flush_buf);
         for (auto& kv : graphs) {
             (void)hipGraphExecDestroy(kv.second.ge);
             (void)hipGraphDestroy(kv.second.g);
@@ -828,6 +834,12 @@ int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other
 
 // Root of every task's tree: quantize, zero the node histograms, build and scan the root.  The
 // first split is selected by the first split step (its partition launch in device-driven mode).
+// Reads and rewrites every word of a buffer (evicts earlier data from the caches in front of HBM).
+__global__ __launch_bounds__(256) void k_flush_cache(unsigned long long* buf, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        buf[i] = buf[i] + 1ull;
+}
+
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     const GrowP P = s->growp(p);
     launch_tree_start(s->stream, s->S, s->td(), s->an, s->max_rows(), s->max_nodes);
@@ -842,6 +854,9 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
             const double m = t.has_sample ? t.n_sample : t.n_rows;
             bytes += m * s->F + m * (s->hconst ? 8.0 : 16.0) + (t.has_sample ? 4.0 * m : 0.0) + (double)s->F * s->nb * 24.0;
         }
+        if (s->flush)
+            hipLaunchKernelGGL(k_flush_cache, dim3(4096), dim3(256), 0, s->stream, s->flush_buf,
+                               tgbm_session::kFlushBytes / 8);
         s->kernel_events(0, bytes, ep);
     }
     launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous() && s->all_aligned(), s->hconst, ms,
@@ -1144,6 +1159,11 @@ int tgbm_timer_enable(tgbm_session* s, int on) {
     return dev_guard([&] {
         s->drain_events();
         s->timing = on != 0;
+        s->flush = on == 2;
+        if (s->flush && !s->flush_buf) {
+            HIP_OK(hipMalloc(&s->flush_buf, tgbm_session::kFlushBytes));
+            HIP_OK(hipMemsetAsync(s->flush_buf, 0, tgbm_session::kFlushBytes, s->stream));
+        }
         for (int k = 0; k < 8; ++k) {
             s->t_ms[k] = 0;
             s->t_n[k] = 0;

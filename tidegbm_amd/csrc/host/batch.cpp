@@ -123,7 +123,7 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
         b->init_score = tl.init_score;
         out.push_back(std::move(b));
     }
-    if (proto.profile) dev_check(tgbm_timer_enable(so.s, 1));
+    if (proto.profile) dev_check(tgbm_timer_enable(so.s, proto.profile));
     st.setup_s = now_s() - t0;
     // ---- boosting
     const double t1 = now_s();

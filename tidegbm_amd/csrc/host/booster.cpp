@@ -942,7 +942,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, tl.pred.data(), tl.bucket.data(),
                                  nullptr, tl.n_buckets, cfg.iid_mode ? nullptr : tl.cuts.data(), groups));
     trees.clear();
-    if (cfg.profile) dev_check(tgbm_timer_enable(sg.s, 1));
+    if (cfg.profile) dev_check(tgbm_timer_enable(sg.s, cfg.profile));
     t_setup = now_s() - t0;
     const double t1 = now_s();
     std::mt19937 rng((uint32_t)cfg.random_state);
