@@ -339,8 +339,9 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
 
 // Store a block's LDS histogram as partial number `slot` of the launch (plain stores; the scan
 // sums the partials exactly, so no global atomics are needed and nothing has to be zeroed).
-// lA holds packed words (lB the h sums when h varies).
-template <bool HV>
+// lA holds KC packed copies per (feature, bin) in consecutive words, summed here (a block of at
+// most WORD_ROWS rows: their sum is a valid packed word); lB the h sums when h varies.
+template <bool HV, int KC>
 __device__ inline void hist_store_partial(const Task& T, const unsigned long long* lA, const unsigned long long* lB,
                                           int FB, int f0, int F, int slot) {
     const size_t base = (size_t)slot * F * NBIN + (size_t)f0 * NBIN;
@@ -348,7 +349,10 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
     auto part = gbl(T.part);
     auto part_h = gbl(T.part_h);
     for (int j = threadIdx.x; j < words; j += blockDim.x) {
-        part[base + j] = lA[j];
+        unsigned long long v = 0;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) v += lA[j * KC + c];
+        part[base + j] = v;
         if (HV) part_h[base + j] = lB[j];
     }
 }
@@ -360,26 +364,28 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
 // round trips.  The block's row range is sized from the node's actual row count, so small nodes
 // still spread over many blocks.
 //
-// A packed LDS word may take at most WORD_ROWS rows.  A block of at most that many rows stores its
-// packed words as its partial.  A larger block (the "wide" form, the same choice in every block of
-// a launch) adds a step's rows in rounds of WORD_ROWS rows; after each round every thread folds the
-// words of its bins into int64 g sums and counts held in registers and clears them, and the
+// The histogram has KC copies per (feature, bin) in consecutive words; thread t adds into copy
+// t % KC, which spreads a wave's atomics over the LDS banks.  A packed word may take at most
+// WORD_ROWS rows.  A block of at most that many rows stores the sum of its copies as its partial.
+// A larger block (the "wide" form, the same choice in every block of a launch) adds a step's rows
+// in rounds of KC * WORD_ROWS rows (WORD_ROWS per copy); after each round every thread folds the
+// copies of its bins into int64 g sums and counts held in registers and clears them, and the
 // partial is stored from those registers (g sums in part, counts in part_c).  The h sums (h
 // varying) never come near a word's limit and stay in LDS.
-// LDS: FB*NBIN words for lA, then (HV) FB*NBIN words for lB.
-template <int LPR, bool HV>
+// LDS: KC*FB*NBIN words for lA, then (HV) FB*NBIN words for lB.
+template <int LPR, bool HV, int KC>
 __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows) {
     constexpr int FB = 16 * LPR;
     constexpr int W = FB * NBIN;
-    constexpr int NT = 512;                  // threads per block
-    constexpr int RPL = 8;                   // rows per lane per step
-    constexpr int STRIDE = NT / LPR;         // rows per lane-row of a step
-    constexpr int UPR = WORD_ROWS / STRIDE;  // lane-rows per round of WORD_ROWS rows
-    constexpr int OWN = W / NT;              // bins each thread folds in the wide form
-    static_assert(RPL % UPR == 0 && W % NT == 0, "rounds and bin ownership");
+    constexpr int NT = 512;                       // threads per block
+    constexpr int RPL = 8;                        // rows per lane per step
+    constexpr int STRIDE = NT / LPR;              // rows per lane-row of a step
+    constexpr int UPR = KC * WORD_ROWS / STRIDE;  // lane-rows per round (WORD_ROWS rows per copy)
+    constexpr int OWN = W / NT;                   // bins each thread folds in the wide form
+    static_assert(RPL % UPR == 0 && W % NT == 0 && NT % KC == 0, "rounds, copies and bin ownership");
     extern __shared__ unsigned long long lds[];
     unsigned long long* lA = lds;
-    unsigned long long* lB = lds + W;
+    unsigned long long* lB = lds + (size_t)W * KC;
     const Task T = tasks[blockIdx.z];
     int node, start, count, buf;
     const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
@@ -397,8 +403,9 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     if (r0 >= count) return;
     const int r1 = min(count, r0 + per);
     const int f0 = blockIdx.y * FB;
-    for (int j = threadIdx.x; j < W * (HV ? 2 : 1); j += NT) lds[j] = 0;
+    for (int j = threadIdx.x; j < W * (KC + (HV ? 1 : 0)); j += NT) lds[j] = 0;
     const int sub = threadIdx.x % LPR;
+    const int cp = threadIdx.x % KC;
     const int fcol = f0 + sub * 16;
     const bool active = fcol < S.fp;
     const auto rows = gbl((const int*)(buf ? T.idx_tmp : T.idx) + start);
@@ -439,19 +446,26 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
                 for (int t = 0; t < 16; ++t) {
                     const int bin = (w4[t >> 2] >> ((t & 3) * 8)) & 255;
                     const int slot = (sub * 16 + t) * NBIN + bin;
-                    atomicAdd(&lA[slot], p[u]);
+                    atomicAdd(&lA[slot * KC + cp], p[u]);
                     if (HV) atomicAdd(&lB[slot], q[u]);
                 }
             }
             if (wide && (u % UPR) == UPR - 1) {
-                // a round of WORD_ROWS rows is in: fold the packed words into registers
+                // a round is in (WORD_ROWS rows per copy): fold the copies into registers; thread
+                // w reads its bins' copies starting at copy (w / 8) % KC, so the lanes of an LDS
+                // cycle touch different banks
                 __syncthreads();
 #pragma unroll
                 for (int o = 0; o < OWN; ++o) {
-                    const unsigned long long v = lA[threadIdx.x + o * NT];
-                    lA[threadIdx.x + o * NT] = 0;
-                    aq[o] += unpack_q(v);
-                    ac[o] += unpack_c(v);
+                    const int j = (threadIdx.x + o * NT) * KC;
+#pragma unroll
+                    for (int c = 0; c < KC; ++c) {
+                        const int cc = (c + (int)threadIdx.x / 8) % KC;
+                        const unsigned long long v = lA[j + cc];
+                        lA[j + cc] = 0;
+                        aq[o] += unpack_q(v);
+                        ac[o] += unpack_c(v);
+                    }
                 }
                 __syncthreads();
             }
@@ -459,7 +473,7 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     }
     __syncthreads();
     if (!wide) {
-        hist_store_partial<HV>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
+        hist_store_partial<HV, KC>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
         return;
     }
     const size_t base = (size_t)blockIdx.x * S.n_features * NBIN + (size_t)f0 * NBIN;
@@ -725,11 +739,16 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     gx = std::max(gx, (max_count + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS);
     gx = std::min(gx, S.part_cap);
     dim3 grid(gx, fbs, n_tasks);
-    const size_t lds = (size_t)FB * NBIN * 8 * (hconst ? 1 : 2);
+    // LDS copies per bin: 4 for one task with h == 1 and 16 features per block (the launch has at
+    // most 32 x 8 blocks, one per CU: 128 KB costs no occupancy), 2 when several tasks share the
+    // launch (two 512-thread blocks per CU are resident by registers anyway), 1 otherwise
+    const int kc = (!hconst || lpr > 1) ? 1 : (n_tasks == 1 ? 4 : 2);
+    const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
     const int rt = root ? 1 : 0;
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    else if (lpr == 1) TIMED_LAUNCH((k_hist_row<1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    else TIMED_LAUNCH((k_hist_row<2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
 }
 
 // =================================================================== split scan
@@ -2123,9 +2142,10 @@ void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout&
 void init_kernel_attributes() {
     static bool done = false;
     if (done) return;
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     done = true;
 }
