@@ -136,7 +136,11 @@ struct Task {
     int cur_nl;                 // rows the current split sends left (-1: not known in advance)
     int err;                    // nonzero: an internal consistency check failed in this tree
     int hist_parts;            // partial histograms written by the last histogram launch
-    int hist_wide;             // those partials are wide: int64 g sums in part, counts in part_c
+    int hist_wide;             // how those partials hold the sums: 0 packed words; 1 int64 g sums in
+                               // part with counts in part_c; 2 the packed words' sums modulo 2^64
+                               // in part, counts from root_cnt; 3 as 1, and the build was the
+                               // task's whole-window root, so the scan records its counts
+    int cnt_row0, cnt_n;       // the row window root_cnt holds counts of (cnt_n = -1: none)
     double train_loss;         // training loss after the last finished tree (exported)
     double* loss_part;         // [2 * loss_chunks(cap_rows)] per-chunk (sum w d^2, sum w), then
                                // [n_groups] per-group correlations (pairwise)
@@ -178,6 +182,7 @@ struct Task {
                                  // int64 g sums when the launch was wide)
     unsigned int* part_c;        // [part_cap][F][NBIN] partial counts of wide launches
     unsigned long long* part_h;  // [part_cap][F][NBIN] partial h sums (only when !hconst)
+    int* root_cnt;               // [F][NBIN] per-bin row counts of the task's whole-window root
     // ---- TORB
     const int* cuts;       // [n_buckets]
     long long* tg;         // [(MAX_BUCKETS+1) * MAX_NODES]

@@ -501,8 +501,9 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
 // After a pass each thread unpacks the copies of one bin into an int64 g sum and a count and
 // clears them for the next pass (a copy's 1024 rows of one pass fit a packed word; the 16384 rows
 // of all copies would not, so each copy is unpacked on its own).  The block's partial is wide
-// (g sums and counts in two planes).  Every lane takes the same path through the passes, so the
-// next pass's loads stay in flight during this pass's adds.
+// (g sums and counts in two planes), or, once the task's root counts are cached, raw (below).
+// Every lane takes the same path through the passes, so the next pass's loads stay in flight
+// during this pass's adds.
 constexpr int ROOT_RPL = 16;
 constexpr int ROOT_THREADS = 1024;
 constexpr int ROOT_ROWS = ROOT_THREADS * ROOT_RPL;
@@ -518,38 +519,16 @@ __device__ inline void root_load_bins(uint4 (&b)[FB], const unsigned char* colb,
     for (int fl = 0; fl < FB; ++fl) b[fl] = *(const uint4*)(colb + (size_t)min(f + fl, fl1 - 1) * n_pad + row);
 }
 
-template <int K, bool HV>
-__global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared S, int feats_per_block,
-                                                             int xcd_major) {
-    static_assert(K == 16, "copies per bin: the sums' bank spread and the register budget assume 16");
-    constexpr int FB = (HV ? 32 : 64) / K;        // 128 KB of LDS
-    constexpr int WORDS = FB * NBIN;              // bins of one pass (per sum)
-    constexpr int SPT = WORDS * (HV ? 2 : 1) / ROOT_THREADS;   // bins summed per thread
-    __shared__ unsigned long long lds[WORDS * K * (HV ? 2 : 1)];
+// The passes of one root block (k_hist_root); RAW: packed words added modulo 2^64 (h == 1, the
+// task's root counts cached).
+template <int K, bool HV, bool RAW>
+__device__ __forceinline__ void hist_root_block(const Task& T, const Shared& S, unsigned long long* lds, int count,
+                                                int bx, int r0, int fr0, int fr1) {
+    constexpr int FB = (HV ? 32 : 64) / K;
+    constexpr int WORDS = FB * NBIN;
+    constexpr int SPT = WORDS * (HV ? 2 : 1) / ROOT_THREADS;
     unsigned long long* lA = lds;
-    unsigned long long* lB = lds + WORDS * K;     // h sums (when h varies)
-    const Task T = tasks[blockIdx.z];
-    const int count = T.n_sample;
-    int bx = blockIdx.x, by = blockIdx.y;
-    if (xcd_major) {
-        // Blocks go out round-robin over the 8 XCDs (id % 8).  Each XCD gets a contiguous run of
-        // (row range, feature range) pairs, feature range fastest, so the blocks that read one
-        // row range's gradients share that XCD's L2.
-        const int nb = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * gridDim.x;
-        const int x = id % 8, l = id / 8;
-        const int base = nb / 8, rem = nb % 8;
-        const int j = x * base + min(x, rem) + l;
-        bx = j / gridDim.y;
-        by = j % gridDim.y;
-    }
-    if (bx == 0 && by == 0 && threadIdx.x == 0) {
-        tasks[blockIdx.z].hist_parts = (count + ROOT_ROWS - 1) / ROOT_ROWS;
-        tasks[blockIdx.z].hist_wide = 1;
-    }
-    const int r0 = bx * ROOT_ROWS;
-    const int fr0 = by * feats_per_block;
-    const int fr1 = min(S.n_features, fr0 + feats_per_block);
-    if (r0 >= count || fr0 >= fr1) return;
+    unsigned long long* lB = lds + WORDS * K;
     for (int j = threadIdx.x; j < WORDS * K * (HV ? 2 : 1); j += ROOT_THREADS) lds[j] = 0;
     const int lane = threadIdx.x & 63;
     const int rep = lane % K;
@@ -603,6 +582,7 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
     long long pend_q = 0;
     unsigned int pend_c = 0;
     int pend_f = -1;
+    unsigned long long raw_prev = 0;   // raw builds: the previous pass's copy sum of this bin
     __syncthreads();
     auto flush = [&]() {
         if (pend_f < 0) return;
@@ -611,7 +591,7 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
         const size_t o = pbase + (size_t)pend_f * NBIN + ws;
         if (gword) {
             outA[o] = (unsigned long long)pend_q;
-            outC[o] = pend_c;
+            if (!RAW) outC[o] = pend_c;
         } else {
             outB[o] = (unsigned long long)pend_q;
         }
@@ -638,6 +618,23 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
         long long q = 0;
         unsigned int c = 0;
         uint4* cp = (uint4*)(lds + (size_t)w * K);
+        if (RAW) {
+            unsigned long long s = 0;
+#pragma unroll
+            for (int j0 = 0; j0 < K / 2; j0 += 4) {
+                uint4 v[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) v[j] = cp[(pr0 + j0 + j) % (K / 2)];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    s += ((unsigned long long)v[j].y << 32 | v[j].x) + ((unsigned long long)v[j].w << 32 | v[j].z);
+            }
+            pend_q = (long long)(s - raw_prev);
+            raw_prev = s;
+            pend_f = f;
+            __syncthreads();
+            return;
+        }
 #pragma unroll
         for (int j0 = 0; j0 < K / 2; j0 += BATCH) {
             uint4 v[BATCH];
@@ -676,6 +673,46 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
     flush();
     pass(bc, f);
     flush();
+}
+
+template <int K, bool HV>
+__global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared S, int feats_per_block,
+                                                             int xcd_major) {
+    static_assert(K == 16, "copies per bin: the sums' bank spread and the register budget assume 16");
+    constexpr int FB = (HV ? 32 : 64) / K;        // 128 KB of LDS
+    constexpr int WORDS = FB * NBIN;              // bins of one pass (per sum)
+    static_assert(WORDS * (HV ? 2 : 1) == ROOT_THREADS, "one word per thread and pass");
+    __shared__ unsigned long long lds[WORDS * K * (HV ? 2 : 1)];
+    const Task T = tasks[blockIdx.z];
+    const int count = T.n_sample;
+    int bx = blockIdx.x, by = blockIdx.y;
+    if (xcd_major) {
+        // Blocks go out round-robin over the 8 XCDs (id % 8).  Each XCD gets a contiguous run of
+        // (row range, feature range) pairs, feature range fastest, so the blocks that read one
+        // row range's gradients share that XCD's L2.
+        const int nb = gridDim.x * gridDim.y, id = blockIdx.x + blockIdx.y * gridDim.x;
+        const int x = id % 8, l = id / 8;
+        const int base = nb / 8, rem = nb % 8;
+        const int j = x * base + min(x, rem) + l;
+        bx = j / gridDim.y;
+        by = j % gridDim.y;
+    }
+    // With h == 1 the per-bin counts of the task's window are the same in every tree: once the
+    // scan has recorded them (root_cnt), the build adds the packed words modulo 2^64 instead
+    // ("raw": sum = count * 2^CNT_SHIFT + q-sum, the q-sum recovered by the scan from the count).
+    // A raw pass needs no clearing: each pass stores the difference of its bin's copy sums from
+    // the previous pass's.
+    const bool raw = !HV && T.cnt_n == count && T.cnt_row0 == T.row0;
+    if (bx == 0 && by == 0 && threadIdx.x == 0) {
+        tasks[blockIdx.z].hist_parts = (count + ROOT_ROWS - 1) / ROOT_ROWS;
+        tasks[blockIdx.z].hist_wide = HV ? 1 : raw ? 2 : 3;
+    }
+    const int r0 = bx * ROOT_ROWS;
+    const int fr0 = by * feats_per_block;
+    const int fr1 = min(S.n_features, fr0 + feats_per_block);
+    if (r0 >= count || fr0 >= fr1) return;
+    if (raw) hist_root_block<K, false, true>(T, S, lds, count, bx, r0, fr0, fr1);
+    else hist_root_block<K, HV, false>(T, S, lds, count, bx, r0, fr0, fr1);
 }
 
 // Launch with optional start/stop events recorded at the kernel's own boundaries (AMD extended
@@ -1104,7 +1141,32 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
         sh[t] = 0;
     }
     int k = k0;
-    if (T.hist_wide) {
+    if (T.hist_wide == 2) {
+        // raw packed-word sums modulo 2^64 (the counts come from the task's root counts)
+        unsigned long long uq[4] = {0, 0, 0, 0};
+        for (; k + 3 * dk < np; k += 4 * dk) {
+            uint4 v[8];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
+                v[2 * u] = ldg(a);
+                v[2 * u + 1] = ldg(a + 1);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int t = (u & 1) * 2;
+                uq[t] += (unsigned long long)v[u].y << 32 | v[u].x;
+                uq[t + 1] += (unsigned long long)v[u].w << 32 | v[u].z;
+            }
+        }
+        for (; k < np; k += dk) {
+            const auto w = gbl((const unsigned long long*)T.part + (size_t)k * stride + off);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) uq[t] += w[t];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) sq[t] = (long long)uq[t];
+    } else if (T.hist_wide) {
         for (; k + 3 * dk < np; k += 4 * dk) {
             uint4 v[8], c[4];
 #pragma unroll
@@ -1348,9 +1410,29 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int b = lane * 4 + t;
-        qg[t] = rq[0][b] + rq[1][b] + rq[2][b] + rq[3][b];
+        // modulo 2^64 (raw sums wrap; the other forms never do)
+        qg[t] = (long long)((unsigned long long)rq[0][b] + (unsigned long long)rq[1][b] +
+                            (unsigned long long)rq[2][b] + (unsigned long long)rq[3][b]);
         c[t] = rc[0][b] + rc[1][b] + rc[2][b] + rc[3][b];
         qh[t] = rh[0][b] + rh[1][b] + rh[2][b] + rh[3][b];
+    }
+    if (T.hist_wide == 2) {
+        // raw root sums: count * 2^CNT_SHIFT + q-sum modulo 2^64, |q-sum| < 2^62
+        const auto rc_in = gbl((const int*)T.root_cnt);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            c[t] = rc_in[(size_t)f * NBIN + lane * 4 + t];
+            qg[t] = (long long)((unsigned long long)qg[t] - ((unsigned long long)(unsigned)c[t] << CNT_SHIFT));
+        }
+    } else if (T.hist_wide == 3 && wave == 0) {
+        // the task's whole-window root: keep its counts for the raw builds of later trees
+        auto rc_out = gbl(T.root_cnt);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) rc_out[(size_t)f * NBIN + lane * 4 + t] = c[t];
+        if (f == 0 && lane == 0) {
+            Tw.cnt_row0 = T.row0;
+            Tw.cnt_n = T.n_sample;
+        }
     }
     int node = -1;
     double gain = -INFINITY;

@@ -413,6 +413,9 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.part = s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
             T.part_c = s->alloc<unsigned int>((size_t)s->S.part_cap * per_node);
             T.part_h = s->hconst ? nullptr : s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
+            T.root_cnt = s->alloc<int>(per_node);
+            T.cnt_row0 = -1;
+            T.cnt_n = -1;
             T.cuts = s->alloc<int>(MAX_BUCKETS);
             const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
             T.tg = s->alloc<long long>(tw);
@@ -522,6 +525,8 @@ int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, cons
                 HIP_OK(hipGetLastError());
                 HIP_OK(hipStreamSynchronize(s->stream));
             }
+            // new bins: the cached root counts no longer hold
+            for (int k = 0; k < s->T; ++k) s->set_field_int(k, offsetof(Task, cnt_n), -1);
         } catch (...) {
             (void)hipFree(Xd);
             (void)hipFree(Ed);
