@@ -13,7 +13,8 @@
 // The sort is a least-significant-digit radix sort of (key, row) pairs, 8 bits per pass, each
 // pass stable: per 4096-key tile a digit count, one exclusive scan over (digit, tile) in digit
 // order, then a scatter in which every wave ranks its keys in order (lanes with the same digit
-// found by ballots over the digit's bits).  Digits are taken from the complemented key, so the
+// found by ballots over the digit's bits) and the tile, ordered by digit in LDS, is written out
+// in runs of consecutive destinations.  Digits are taken from the complemented key, so the
 // ascending sort puts |g| in descending order and equal keys stay in row order.  A pass whose
 // digit is the same for every key is skipped; which passes run, and so which buffer holds the
 // result, follows from the OR and AND of all keys, computed on the device first.
@@ -120,7 +121,8 @@ __global__ __launch_bounds__(1024) void k_goss_mask(const unsigned long long* pa
     }
 }
 
-// Digit counts of one tile: counts[digit * n_tiles + tile].
+// Digit counts of one tile: counts[digit * n_tiles + tile].  Each lane loads its 16 keys first
+// (one memory round trip), then the waves count them.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_count(const unsigned long long* __restrict__ ka,
                                                          const unsigned long long* __restrict__ kb, int n, int pass,
                                                          const unsigned long long* mask, unsigned int* counts,
@@ -130,14 +132,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_count(const unsigned long lon
     const unsigned long long* src = (before & 1) ? kb : ka;
     __shared__ unsigned int h[256];
     h[threadIdx.x] = 0;
-    __syncthreads();
     const int t0 = blockIdx.x * RS_TILE;
+    constexpr int KPT = RS_TILE / RS_THREADS;
+    unsigned long long key[KPT];
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
+        const int i = t0 + j * RS_THREADS + (int)threadIdx.x;
+        key[j] = i < n ? src[i] : 0ull;
+    }
+    __syncthreads();
     const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
-    for (int j = threadIdx.x; j < RS_TILE; j += RS_THREADS) {
+#pragma unroll
+    for (int j = 0; j < KPT; ++j) {
         // one atomic per group of lanes with the same digit (digits cluster in the high bits)
-        const int i = t0 + j;
-        const bool ok = i < n;
-        const int dg = ok ? rs_digit(src[i], 8 * pass) : 0;
+        const bool ok = t0 + j * RS_THREADS + (int)threadIdx.x < n;
+        const int dg = ok ? rs_digit(key[j], 8 * pass) : 0;
         const unsigned long long same = rs_match(ok, dg);
         if (ok && (same & below) == 0) atomicAdd(&h[dg], (unsigned int)__popcll(same));
     }
@@ -206,7 +215,11 @@ __global__ __launch_bounds__(1024) void k_rs_scan(unsigned int* c, int m, int pa
     }
 }
 
-// Stable scatter of one tile: wave w ranks keys w*1024 .. w*1024+1023 of the tile in order.
+// Stable scatter of one tile.  Wave w holds keys w*1024 .. w*1024+1023 of the tile (16 per lane,
+// loaded first) and ranks them in order: a key's place among the wave's keys of its digit is the
+// count of that digit in the wave's earlier steps plus its rank among the lanes of its step.  The
+// keys are placed in tile order by digit in LDS, then written out by consecutive threads, so the
+// keys of one digit (a run of consecutive destinations) are stored together.
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(unsigned long long* ka, int* va, unsigned long long* kb,
                                                            int* vb, int n, int pass, const unsigned long long* mask,
                                                            const unsigned int* __restrict__ offs,
@@ -218,53 +231,83 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(unsigned long long* k
     unsigned long long* kd = (before & 1) ? ka : kb;
     int* vd = (before & 1) ? va : vb;
     const int shift = 8 * pass;
-    __shared__ unsigned int wh[4][256];   // per-wave digit counts, then each wave's running bases
+    constexpr int STEPS = RS_TILE / RS_THREADS;   // 16 keys per lane
+    __shared__ unsigned int wh[4][256];   // per-wave digit counts, then each wave's first tile position
+    __shared__ unsigned int dstart[256];  // the digit's first position in the tile
+    __shared__ unsigned int gbase[256];   // the digit's first destination of this tile's keys
+    __shared__ unsigned long long sk[RS_TILE];
+    __shared__ int sv[RS_TILE];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int t0 = blockIdx.x * RS_TILE;
+    const int w0 = t0 + wave * 1024;
+    unsigned long long key[STEPS];
+    int val[STEPS];
+#pragma unroll
+    for (int j = 0; j < STEPS; ++j) {
+        const int i = w0 + j * 64 + lane;
+        key[j] = i < n ? ks[i] : 0ull;
+        val[j] = i < n ? vs[i] : 0;
+    }
     for (int d = threadIdx.x; d < 4 * 256; d += RS_THREADS) (&wh[0][0])[d] = 0;
     __syncthreads();
-    const int w0 = blockIdx.x * RS_TILE + wave * 1024;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int j = 0; j < 16; ++j) {
-        const int i = w0 + j * 64 + lane;
-        const bool ok = i < n;
-        const int dg = ok ? rs_digit(ks[i], shift) : 0;
+    int pos[STEPS];
+#pragma unroll
+    for (int j = 0; j < STEPS; ++j) {
+        const bool ok = w0 + j * 64 + lane < n;
+        const int dg = ok ? rs_digit(key[j], shift) : 0;
         const unsigned long long same = rs_match(ok, dg);
-        if (ok && (same & below) == 0) atomicAdd(&wh[wave][dg], (unsigned int)__popcll(same));
+        // every lane reads the count before the group's first lane adds to it (in-order LDS
+        // accesses of one wave)
+        pos[j] = ok ? (int)wh[wave][dg] + __popcll(same & below) : 0;
+        if (ok && (same & below) == 0) wh[wave][dg] += (unsigned int)__popcll(same);
     }
     __syncthreads();
     {
-        // 256 threads: one digit each; the digit's first position is the sum of the smaller
-        // digits' totals (an exclusive scan over the 4 waves) plus this tile's offset in the digit
+        // thread d: digit d's count in the tile, its first tile position (an exclusive scan over
+        // the digits), its first destination (the smaller digits' totals plus this tile's offset
+        // among the tiles), and each wave's first tile position within the digit
         const int d = threadIdx.x;
+        const unsigned int c0 = wh[0][d], c1 = wh[1][d], c2 = wh[2][d], c3 = wh[3][d];
+        const unsigned int cnt = c0 + c1 + c2 + c3;
         const unsigned int tot = totals[d];
-        const int incl = wave_incl_i((int)tot);
-        __shared__ unsigned int dw[4];
-        if ((threadIdx.x & 63) == 63) dw[wave] = (unsigned int)incl;
+        __shared__ unsigned int dw[4], tw[4];
+        const int incl_c = wave_incl_i((int)cnt);
+        const int incl_t = wave_incl_i((int)tot);
+        if ((threadIdx.x & 63) == 63) {
+            dw[wave] = (unsigned int)incl_c;
+            tw[wave] = (unsigned int)incl_t;
+        }
         __syncthreads();
-        unsigned int base = (unsigned int)incl - tot;
-        for (int w = 0; w < wave; ++w) base += dw[w];
-        unsigned int run = base + offs[(size_t)d * n_tiles + blockIdx.x];
-        for (int w = 0; w < 4; ++w) {
-            const unsigned int c = wh[w][d];
-            wh[w][d] = run;
-            run += c;
+        unsigned int ls = (unsigned int)incl_c - cnt, gs = (unsigned int)incl_t - tot;
+        for (int w = 0; w < wave; ++w) {
+            ls += dw[w];
+            gs += tw[w];
+        }
+        dstart[d] = ls;
+        gbase[d] = gs + offs[(size_t)d * n_tiles + blockIdx.x];
+        wh[0][d] = ls;
+        wh[1][d] = ls + c0;
+        wh[2][d] = ls + c0 + c1;
+        wh[3][d] = ls + c0 + c1 + c2;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < STEPS; ++j) {
+        if (w0 + j * 64 + lane < n) {
+            const int p = (int)wh[wave][rs_digit(key[j], shift)] + pos[j];
+            sk[p] = key[j];
+            sv[p] = val[j];
         }
     }
     __syncthreads();
-    for (int j = 0; j < 16; ++j) {
-        const int i = w0 + j * 64 + lane;
-        const bool ok = i < n;
-        const unsigned long long key = ok ? ks[i] : 0ull;
-        const int val = ok ? vs[i] : 0;
-        const int dg = ok ? rs_digit(key, shift) : 0;
-        const unsigned long long same = rs_match(ok, dg);
-        const int rank = __popcll(same & below);
-        const unsigned int base = ok ? wh[wave][dg] : 0u;
-        if (ok) {
-            kd[base + rank] = key;
-            vd[base + rank] = val;
-            if (rank == 0) wh[wave][dg] = base + (unsigned int)__popcll(same);   // the group's first lane
-        }
+    const int m = min(RS_TILE, n - t0);
+    for (int p = threadIdx.x; p < m; p += RS_THREADS) {
+        const unsigned long long k = sk[p];
+        const int d = rs_digit(k, shift);
+        const unsigned int g = gbase[d] + (unsigned int)p - dstart[d];
+        kd[g] = k;
+        vd[g] = sv[p];
     }
 }
 
