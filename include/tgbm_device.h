@@ -88,6 +88,12 @@ int tgbm_session_set_cat(tgbm_session* s, const int* is_cat);
  * monotone[f] > 0 a candidate whose left leaf weight is below the right one is skipped, for
  * monotone[f] < 0 one whose left weight is above.  Once per session, before the first tree. */
 int tgbm_session_set_monotone(tgbm_session* s, const int* monotone);
+/* Leaves split per step of tgbm_tree_grow_device (0 or 1: one split per step, the reference's
+ * loop, tree.cpp:343-463).  With width w > 1 a step splits the leaf best-first takes next and up
+ * to w - 1 next-best leaves of the frontier; the tree is the best-first replay over the splits
+ * made, so nodes, ids and values equal one-split-per-step growth (a split the replay does not
+ * take is dropped).  Takes effect in one-task sessions with max_leaves <= 64; w <= 8. */
+int tgbm_session_set_spec(tgbm_session* s, int width);
 /* Bin masks (4 words per node, bins that go left) of the task's last exported tree, for the
  * nodes whose split_type is 2. */
 int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out);

@@ -94,7 +94,7 @@ OBJECTIVES = {"mse": 0, "auto": 0, "l2": 0, "regression": 0, "pairwise": 5, "lam
 _INT_KEYS = {"n_estimators", "max_depth", "max_leaves", "max_bin", "n_time_buckets", "random_state",
              "n_jobs", "iid_mode", "halo", "verbose", "horizon", "embargo", "goss", "cat_min_count",
              "col_major_hist", "min_samples_leaf", "pair_splits", "pair_top_k",
-             "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune",
+             "max_pair_splits_per_tree", "bin_sample", "finer_torb", "profile", "hist_mode", "autotune", "spec_width",
              "force_host_driver", "early_stopping_rounds", "regime_feature", "regime_priority"}
 _DOUBLE_KEYS = {"learning_rate", "l2", "l1", "subsample", "colsample", "min_child_weight", "cat_smooth",
                 "min_split_gain", "pair_trigger", "pair_l2", "goss_top", "goss_other",

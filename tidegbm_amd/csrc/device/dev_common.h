@@ -74,6 +74,23 @@ struct DNode {
 };
 enum { N_LEAF = 0, N_TYPE, N_FEAT, N_FEAT2, N_THR, N_LEFT, N_RIGHT, N_MGL, N_NS, N_DEPTH };
 
+// One leaf split of a multi-leaf step (speculative best-first growth, one task per session):
+// written by the step's selection, then read by its partition, histogram and scan launches.
+constexpr int SPEC_W = 8;   // leaves split per step at most
+struct Slot {
+    int leaf, left;             // the split leaf; its children get ids left and left + 1
+    int start, count, buf;      // the leaf's rows: range in index buffer `buf`
+    int type, feat, feat2, thr, mgl;
+    int nl;                     // rows the split sends left (from the leaf's histogram counts)
+    int chunk0;                 // the slot's first partition block in the step's launch
+    int blk0, gx, per;          // first histogram row block, row blocks, rows per block
+    int small_node, big_node, small_start, small_count, small_buf;
+    int hist_parts, hist_wide;  // partials of the slot's histogram blocks and their form
+    int pad;
+    double gain;
+    unsigned long long part_lr; // partition reservations: right << 32 | left
+};
+
 struct FCand {      // best split of one feature of one node
     double gain;
     int thr, ml;
@@ -156,7 +173,7 @@ struct Task {
     double* pred;
     double* g;
     double* h;
-    unsigned long long* pk;   // packed (1<<48) + q_g
+    unsigned long long* pk;   // packed (1 << CNT_SHIFT) + q_g
     unsigned long long* qh;   // q_h (only when !hconst)
     const int* bucket;
     const unsigned char* seg; // TORB sweep segment per row
@@ -194,6 +211,18 @@ struct Task {
     // ---- categorical splits (null when the session has no categorical feature)
     unsigned long long* fmask;  // [MAX_NODES][n_features][4] bins a feature's best split sends left
     unsigned long long* nmask;  // [MAX_NODES][4] the split node's mask (exported with the tree)
+    // ---- multi-leaf growth (null slots: one split per step)
+    Slot* slots;           // [SPEC_W] the current step's splits
+    DNode* tnodes;         // [MAX_NODES] the grown nodes in creation order (kept by the finalize)
+    int* tmap;             // [MAX_NODES] the final leaf holding each grown leaf's rows
+    int n_slots;           // splits of the current step
+    int spec_steps;        // steps taken in this tree
+    int pend_first, n_pend; // nodes [pend_first, pend_first + n_pend) scanned, not yet reduced
+    int n_tnodes;          // grown nodes (0: the tree grew one split per step)
+    int spec_cap;          // nodes the task's histogram buffers hold
+    int* host_flag;        // mapped host memory: {done, steps} written when a tree's growth ends
+    int skip_end;          // the tree's growth was not finished: the tree-end kernels do nothing
+    int pad_e;
 };
 
 struct Shared {            // the session-wide matrix

@@ -254,6 +254,10 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     T.cur_nl = -1;
     T.pending0 = 0;
     T.pending1 = -1;
+    T.n_slots = 0;
+    T.spec_steps = 0;
+    T.n_tnodes = 0;
+    T.skip_end = 0;
     T.cur_leaf = -1;
     T.scan_ticket = 0;
     T.part_lr = 0;
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     r.gain = 0.0;
 }
 
-// Packed per-row histogram operands: pk = 2^48 + rint(g_use 2^s_g), qh = rint(h_use 2^s_h).
+// Packed per-row histogram operands: pk = 2^CNT_SHIFT + rint(g_use 2^s_g), qh = rint(h_use 2^s_h).
 __global__ __launch_bounds__(256) void k_pack(Task* tasks) {
     Task& T = tasks[blockIdx.y];
     const int sg = T.s_g, sh = T.s_h;
@@ -301,6 +305,8 @@ void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks
 
 // =================================================================== histogram build
 __device__ void apply_nodes(const Task& T, Task* Tw, int nl);
+__device__ void apply_nodes_d(const Task& T, Task* Tw, int leaf, int type, int feat, int feat2, int thr, int mgl,
+                              double gain, int L, int start, int count, int buf, int nl, bool set_count);
 
 // Which node a histogram launch fills: the root (node 0) or the smaller child of the current
 // split (reference tree.cpp:418-428: left when n_left <= n_right), with its row range.  The
@@ -333,6 +339,44 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
         Tw->small_start = start;
         Tw->small_count = count;
         Tw->small_buf = buf;
+    }
+    return true;
+}
+
+// Multi-leaf step: block x of the launch builds row block rb of slot j's smaller child, where the
+// blocks of slot j are blk0[j] .. blk0[j] + gx[j] - 1.  The slot's first block applies the node
+// bookkeeping of its split (the selection assigned the children's ids and the node count) and
+// records the smaller child and the partials' form for the scan.
+__device__ inline bool spec_target(const Task& T, Task* Tw, int bx, bool by0, int& sj, int& rb, int& node,
+                                   int& start, int& count, int& buf, int& per) {
+    if (T.done) return false;
+    sj = -1;
+    for (int q = 0; q < T.n_slots; ++q)
+        if (bx >= T.slots[q].blk0) sj = q;
+    if (sj < 0) return false;
+    Slot& sl = T.slots[sj];
+    rb = bx - sl.blk0;
+    if (rb >= sl.gx) return false;
+    const int nl = (int)(sl.part_lr & 0xffffffffull), cnt = sl.count, st = sl.start;
+    const int L = sl.left, R = sl.left + 1;
+    const bool left_small = nl <= cnt - nl;
+    node = left_small ? L : R;
+    start = left_small ? st : st + nl;
+    count = left_small ? nl : cnt - nl;
+    buf = 1 - sl.buf;
+    per = sl.per;
+    if (rb == 0 && by0 && threadIdx.x == 0) {
+        // the partition must send exactly the rows the chosen split's histogram counts left
+        if (sl.nl != nl) Tw->err = 1;
+        apply_nodes_d(T, Tw, sl.leaf, sl.type, sl.feat, sl.feat2, sl.thr, sl.mgl, sl.gain, L, st, cnt, sl.buf, nl,
+                      false);
+        sl.small_node = node;
+        sl.big_node = left_small ? R : L;
+        sl.small_start = start;
+        sl.small_count = count;
+        sl.small_buf = buf;
+        sl.hist_parts = (count + per - 1) / per;
+        sl.hist_wide = per > WORD_ROWS ? 1 : 0;
     }
     return true;
 }
@@ -373,8 +417,13 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
 // partial is stored from those registers (g sums in part, counts in part_c).  The h sums (h
 // varying) never come near a word's limit and stay in LDS.
 // LDS: KC*FB*NBIN words for lA, then (HV) FB*NBIN words for lB.
+//
+// One block's rows [rb * per, (rb + 1) * per) of a node's range (start, count in index buffer
+// buf, or rows start + k directly when contig) for features f0 .. f0 + 16 * LPR - 1, stored as
+// partial pidx.  Every thread of the block calls it (it synchronizes the block).
 template <int LPR, bool HV, int KC>
-__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows) {
+__device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, unsigned long long* lds, int start,
+                                               int count, int buf, int per, int rb, int pidx, int f0, bool contig) {
     constexpr int FB = 16 * LPR;
     constexpr int W = FB * NBIN;
     constexpr int NT = 512;                       // threads per block
@@ -383,26 +432,12 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     constexpr int UPR = KC * WORD_ROWS / STRIDE;  // lane-rows per round (WORD_ROWS rows per copy)
     constexpr int OWN = W / NT;                   // bins each thread folds in the wide form
     static_assert(RPL % UPR == 0 && W % NT == 0 && NT % KC == 0, "rounds, copies and bin ownership");
-    extern __shared__ unsigned long long lds[];
     unsigned long long* lA = lds;
     unsigned long long* lB = lds + (size_t)W * KC;
-    const Task T = tasks[blockIdx.z];
-    int node, start, count, buf;
-    const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
-    if (!hist_target(T, tasks + blockIdx.z, root != 0, leader, node, start, count, buf)) return;
-    (void)node;
-    int per = (count + gridDim.x - 1) / gridDim.x;
-    per = max(per, min_rows);
-    per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
     const bool wide = per > WORD_ROWS;
-    if (leader && threadIdx.x == 0) {
-        tasks[blockIdx.z].hist_parts = (count + per - 1) / per;
-        tasks[blockIdx.z].hist_wide = wide ? 1 : 0;
-    }
-    const int r0 = blockIdx.x * per;
+    const int r0 = rb * per;
     if (r0 >= count) return;
     const int r1 = min(count, r0 + per);
-    const int f0 = blockIdx.y * FB;
     for (int j = threadIdx.x; j < W * (KC + (HV ? 1 : 0)); j += NT) lds[j] = 0;
     const int sub = threadIdx.x % LPR;
     const int cp = threadIdx.x % KC;
@@ -411,7 +446,6 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     const auto rows = gbl((const int*)(buf ? T.idx_tmp : T.idx) + start);
     const auto pk = gbl((const unsigned long long*)T.pk);
     const auto qhp = gbl((const unsigned long long*)T.qh);
-    const bool contig = T.contiguous && root;
     long long aq[OWN];
     unsigned int ac[OWN];
 #pragma unroll
@@ -473,10 +507,10 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     }
     __syncthreads();
     if (!wide) {
-        hist_store_partial<HV, KC>(T, lA, lB, FB, f0, S.n_features, blockIdx.x);
+        hist_store_partial<HV, KC>(T, lA, lB, FB, f0, S.n_features, pidx);
         return;
     }
-    const size_t base = (size_t)blockIdx.x * S.n_features * NBIN + (size_t)f0 * NBIN;
+    const size_t base = (size_t)pidx * S.n_features * NBIN + (size_t)f0 * NBIN;
     const int words = min(FB, S.n_features - f0) * NBIN;
     auto part = gbl(T.part);
     auto part_c = gbl(T.part_c);
@@ -489,6 +523,52 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
             part_c[base + j] = ac[o];
             if (HV) part_h[base + j] = lB[j];
         }
+    }
+}
+
+// root: 0 the smaller child of T.cur_*, 1 a sampled or whole-window root (grid: row blocks x
+// feature blocks x tasks); 2 the slots of a multi-leaf step, whose (row block, feature block)
+// items a fixed grid of blocks takes in turn (item = global row block * feature blocks +
+// feature block, so consecutive items read the same rows).
+template <int LPR, bool HV, int KC, bool SPEC>
+__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows, int slot_parts) {
+    constexpr int FB = 16 * LPR;
+    extern __shared__ unsigned long long lds[];
+    const Task T = tasks[blockIdx.z];
+    if (SPEC) {   // root == 2
+        if (T.done || T.n_slots <= 0) return;
+        // Blocks go out round-robin over the 8 XCDs (block id % 8, and gridDim.x is a multiple of
+        // 8), so item i runs on XCD i % 8.  Row block gb's feature blocks all run on XCD gb % 8:
+        // the 128-byte rows they share come into that XCD's L2 once.
+        const int fbs = (S.fp + FB - 1) / FB;
+        const Slot& last = T.slots[T.n_slots - 1];
+        const int n_gb = last.blk0 + last.gx;
+        const int items = (n_gb + 7) / 8 * 8 * fbs;
+        for (int item = blockIdx.x; item < items; item += gridDim.x) {
+            const int x = item % 8, j = item / 8;
+            const int fb = j % fbs, gb = (j / fbs) * 8 + x;
+            if (gb >= n_gb) continue;
+            int sj, rb, node, start, count, buf, per;
+            if (spec_target(T, tasks + blockIdx.z, gb, fb == 0, sj, rb, node, start, count, buf, per))
+                hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, rb, sj * slot_parts + rb, fb * FB,
+                                            false);
+            __syncthreads();   // the next item clears the LDS this one read
+        }
+    } else {
+    int node, start, count, buf;
+    const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
+    if (!hist_target(T, tasks + blockIdx.z, root != 0, leader, node, start, count, buf)) return;
+    (void)node;
+    int per = (count + gridDim.x - 1) / gridDim.x;
+    per = max(per, min_rows);
+    per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
+    if (leader && threadIdx.x == 0) {
+        tasks[blockIdx.z].hist_parts = (count + per - 1) / per;
+        tasks[blockIdx.z].hist_wide = per > WORD_ROWS ? 1 : 0;
+    }
+    // the whole-window root reads rows directly
+    hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, blockIdx.x, blockIdx.x, blockIdx.y * FB,
+                                T.contiguous && root == 1);
     }
 }
 
@@ -723,6 +803,31 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
         else hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__);                         \
     } while (0)
 
+// Child-build tiles of a histogram variant.  Row kernel: LPR lanes per row (16 features each),
+// 512 threads.  Default LPR = 1 (16 features per block: more blocks per node at the same partial
+// count) and 1024-row minimum tiles, the fastest setting over the node sizes of a 1M x 128,
+// 31-leaf fit.  Variants >= 16 (low byte) pick LPR = 1 << ((v - 16) / 8) and min_rows =
+// 256 << ((v - 16) % 8).  At most 32 features per block (the wide form holds 8 * LPR bins per
+// thread in registers), 16 when h varies.  Blocks per feature block: no block folds more than
+// MAX_BLOCK_ROWS rows, and there are at most pcap partials per build: 32 by default, since more
+// blocks speed the build up less than the extra partials slow the scan's sums down (whole-tree
+// sweep at 1M x 128: 32 -> 836 trees/s, 64 -> 787, 16 -> 745); bits 12-13 of the variant pick
+// 32, 64, 16 or 8.  (Fewer blocks per task when many tasks share the launch was slower too:
+// batched walk-forward 2.27 s at 4096 blocks per launch vs 1.88 s at 32 partials per task.)
+void child_tiles(int variant, bool hconst, int& lpr, int& min_rows, int& pcap) {
+    lpr = 1;
+    min_rows = 1024;
+    const int cv = variant & 0xff;
+    if (cv >= 16) {
+        lpr = 1 << std::min(2, (cv - 16) / 8);
+        min_rows = 256 << std::min(7, (cv - 16) % 8);
+    }
+    if (!hconst) lpr = 1;
+    if (lpr > 2) lpr = 2;
+    constexpr int pcaps[4] = {32, 64, 16, 8};
+    pcap = pcaps[(variant >> 12) & 3];
+}
+
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant, hipEvent_t e0,
                  hipEvent_t e1) {
@@ -747,31 +852,10 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
         return;
     }
-    // Row kernel: LPR lanes per row (16 features each), 512 threads.  Default LPR = 1 (16
-    // features per block: more blocks per node at the same partial count) and 1024-row minimum
-    // tiles, the fastest setting over the node sizes of a 1M x 128, 31-leaf fit.  Variants >= 16
-    // (low byte) pick LPR = 1 << ((v - 16) / 8) and min_rows = 256 << ((v - 16) % 8).
-    int lpr = 1, min_rows = 1024;
-    const int cv = variant & 0xff;
-    if (cv >= 16) {
-        lpr = 1 << std::min(2, (cv - 16) / 8);
-        min_rows = 256 << std::min(7, (cv - 16) % 8);
-    }
-    // at most 32 features per block (the wide form holds 8 * LPR bins per thread in registers),
-    // 16 when h varies
-    if (!hconst) lpr = 1;
-    if (lpr > 2) lpr = 2;
+    int lpr, min_rows, pcap;
+    child_tiles(variant, hconst, lpr, min_rows, pcap);
     const int FB = 16 * lpr;
     const int fbs = (S.fp + FB - 1) / FB;
-    // Blocks per feature block: no block folds more than MAX_BLOCK_ROWS rows, and there are at most
-    // part_cap partials (part_cap >= 2 * rows / MAX_BLOCK_ROWS + 1 > max_count / MAX_BLOCK_ROWS).
-    // At most 32 partials by default: more blocks speed the build up less than the extra
-    // partials slow the scan's sums down (whole-tree sweep at 1M x 128: 32 -> 836 trees/s, 64 ->
-    // 787, 16 -> 745).  Bits 12-13 of the variant pick 32, 64, 16 or 8.
-    // (Fewer blocks per task when many tasks share the launch was slower too: batched
-    // walk-forward 2.27 s at 4096 blocks per launch vs 1.88 s at 32 partials per task.)
-    constexpr int pcaps[4] = {32, 64, 16, 8};
-    const int pcap = pcaps[(variant >> 12) & 3];
     int gx = std::max(1, std::min(pcap, (max_count + min_rows - 1) / min_rows));
     gx = std::max(gx, (max_count + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS);
     gx = std::min(gx, S.part_cap);
@@ -782,10 +866,23 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     const int kc = (!hconst || lpr > 1) ? 1 : (n_tasks == 1 ? 4 : 2);
     const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
     const int rt = root ? 1 : 0;
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
-    else TIMED_LAUNCH((k_hist_row<1, false, 2>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
+    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
+    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
+}
+
+// Child builds of a multi-leaf step: every slot's smaller child in one launch of `blocks` blocks
+// (two per CU: 16 features and 64 KB of LDS each), which take the step's (row block, feature
+// block) items in turn; partials of slot j start at partial j * slot_parts.
+void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
+                      int slot_parts, hipEvent_t e0, hipEvent_t e1) {
+    constexpr int FB = 16;
+    dim3 grid(std::max(1, blocks), 1, n_tasks);
+    const int kc = hconst ? 2 : 1;
+    const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts);
 }
 
 // =================================================================== split scan
@@ -1091,7 +1188,7 @@ __device__ void select_decide(const Task& T, int F, const GrowP& P, bool writer,
 // Node bookkeeping of the split in T.cur_* (tree.cpp:385-392, 445-456): the split node becomes
 // internal, its children get ids L = cur_left and L + 1 with their row ranges; one thread.
 __device__ void apply_nodes_d(const Task& T, Task* Tw, int leaf, int type, int feat, int feat2, int thr, int mgl,
-                              double gain, int L, int start, int count, int buf, int nl) {
+                              double gain, int L, int start, int count, int buf, int nl, bool set_count) {
     const int R = L + 1;
     DNode& p = T.nodes[leaf];
     p.v[N_LEAF] = 0;
@@ -1118,18 +1215,27 @@ __device__ void apply_nodes_d(const Task& T, Task* Tw, int leaf, int type, int f
         c.gain = 0.0;
         T.cands[side == 0 ? L : R].valid = 0;
     }
-    Tw->n_nodes = L + 2;
+    if (set_count) Tw->n_nodes = L + 2;
 }
 
 __device__ void apply_nodes(const Task& T, Task* Tw, int nl) {
     apply_nodes_d(T, Tw, T.cur_leaf, T.cur_type, T.cur_feat, T.cur_feat2, T.cur_thr, T.cur_mgl, T.cur_gain,
-                  T.cur_left, T.cur_start, T.cur_count, T.cur_buf, nl);
+                  T.cur_left, T.cur_start, T.cur_count, T.cur_buf, nl, true);
 }
 
-// Exact sums, over partials k = k0, k0 + dk, ... < np of the last histogram launch, of bins
-// lane*4 .. lane*4+3 of feature f: packed words, or (wide launches) int64 g sums with their
-// counts in part_c.  Four partials are loaded per round so their latencies overlap.
-__device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk, int np, long long sq[4],
+// The partial histograms of one histogram build: packed words or (wide) int64 g sums with
+// counts in part_c, or (2) raw packed-word sums; h sums in part_h when h varies.
+struct Parts {
+    const unsigned long long* part;
+    const unsigned int* part_c;
+    const unsigned long long* part_h;
+    int wide, hconst;
+};
+
+// Exact sums, over partials k = k0, k0 + dk, ... < np of one histogram build, of bins
+// lane*4 .. lane*4+3 of feature f.  Four partials are loaded per round so their latencies
+// overlap.
+__device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk, int np, long long sq[4],
                                     long long sc[4], long long sh[4]) {
     const int lane = threadIdx.x & 63;
     const size_t stride = (size_t)F * NBIN;
@@ -1141,7 +1247,7 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
         sh[t] = 0;
     }
     int k = k0;
-    if (T.hist_wide == 2) {
+    if (T.wide == 2) {
         // raw packed-word sums modulo 2^64 (the counts come from the task's root counts)
         unsigned long long uq[4] = {0, 0, 0, 0};
         for (; k + 3 * dk < np; k += 4 * dk) {
@@ -1166,7 +1272,7 @@ __device__ inline void sum_partials(const Task& T, int F, int f, int k0, int dk,
         }
 #pragma unroll
         for (int t = 0; t < 4; ++t) sq[t] = (long long)uq[t];
-    } else if (T.hist_wide) {
+    } else if (T.wide) {
         for (; k + 3 * dk < np; k += 4 * dk) {
             uint4 v[8], c[4];
 #pragma unroll
@@ -1362,18 +1468,47 @@ __device__ void scan_cat(bool mine, const long long* qg, const int* c, const lon
 // one feature: its four waves sum interleaved quarters of the partials, then waves 0 and 2 scan
 // the smaller child and waves 1 and 3 the larger (half of the candidates each); wave 0 / 1 stores
 // the child's histogram, kept for later use as a parent, and its best split of the feature.
-__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode) {
-    const Task T = tasks[blockIdx.z];
-    Task& Tw = tasks[blockIdx.z];
+__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int slot_parts) {
+    // mode 2 (multi-leaf steps): grid z = task * SPEC_W + slot
+    const int task = mode == 2 ? (int)blockIdx.z / SPEC_W : (int)blockIdx.z;
+    const int sj = mode == 2 ? (int)blockIdx.z % SPEC_W : 0;
+    const Task T = tasks[task];
+    Task& Tw = tasks[task];
     if (T.done) return;
+    if (mode == 2 && sj >= T.n_slots) return;
     const int F = S.n_features;
     const size_t per = (size_t)F * NBIN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int f = blockIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // the build this scan reads: its parent, children and partials
+    int parent = T.cur_leaf, small_node = T.small_node, big_node = T.big_node;
+    Parts pp;
+    pp.part = T.part;
+    pp.part_c = T.part_c;
+    pp.part_h = T.part_h;
+    pp.wide = T.hist_wide;
+    pp.hconst = T.hconst;
+    int n_parts = T.hist_parts;
+    if (mode == 2) {
+        const Slot& sl = T.slots[sj];
+        parent = sl.leaf;
+        small_node = sl.small_node;
+        big_node = sl.big_node;
+        const size_t o = (size_t)sj * slot_parts * per;
+        pp.part += o;
+        pp.part_c += o;
+        if (pp.part_h) pp.part_h += o;
+        pp.wide = sl.hist_wide;
+        n_parts = sl.hist_parts;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && mode != 2) {
         Tw.pending0 = mode == 0 ? 0 : T.cur_left;
         Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
         Tw.part_lr = 0;   // the histogram launch has consumed it; the next partition counts anew
+        if (mode == 0) {
+            Tw.pend_first = 0;   // multi-leaf growth: the root's candidate is reduced next
+            Tw.n_pend = 1;
+        }
     }
     __shared__ long long rq[4][NBIN], rh[4][NBIN];
     __shared__ int rc[4][NBIN];
@@ -1381,8 +1516,8 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     // now, so their latency overlaps the partial loads
     long long par_q[4] = {0, 0, 0, 0}, par_h[4] = {0, 0, 0, 0};
     int par_c[4] = {0, 0, 0, 0};
-    if (mode == 1 && (wave & 1)) {
-        const size_t po = (size_t)T.cur_leaf * per + (size_t)f * NBIN + lane * 4;
+    if (mode != 0 && (wave & 1)) {
+        const size_t po = (size_t)parent * per + (size_t)f * NBIN + lane * 4;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             par_q[t] = gbl(T.hg)[po + t];
@@ -1392,7 +1527,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     }
     {
         long long sq[4], sc[4], sh[4];
-        sum_partials(T, F, f, wave, 4, T.hist_parts, sq, sc, sh);
+        sum_partials(pp, F, f, wave, 4, n_parts, sq, sc, sh);
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             rq[wave][lane * 4 + t] = sq[t];
@@ -1404,7 +1539,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     // waves 0 and 2 scan the smaller child (the root in mode 0), waves 1 and 3 the larger; the
     // two waves of a node hold the same sums and each evaluates half of its candidates
     const int side = wave & 1, half = wave >> 1;
-    const bool active = mode == 1 || side == 0;
+    const bool active = mode != 0 || side == 0;
     long long qg[4], qh[4];
     int c[4];
 #pragma unroll
@@ -1416,7 +1551,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
         c[t] = rc[0][b] + rc[1][b] + rc[2][b] + rc[3][b];
         qh[t] = rh[0][b] + rh[1][b] + rh[2][b] + rh[3][b];
     }
-    if (T.hist_wide == 2) {
+    if (mode == 0 && T.hist_wide == 2) {
         // raw root sums: count * 2^CNT_SHIFT + q-sum modulo 2^64, |q-sum| < 2^62
         const auto rc_in = gbl((const int*)T.root_cnt);
 #pragma unroll
@@ -1424,7 +1559,7 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
             c[t] = rc_in[(size_t)f * NBIN + lane * 4 + t];
             qg[t] = (long long)((unsigned long long)qg[t] - ((unsigned long long)(unsigned)c[t] << CNT_SHIFT));
         }
-    } else if (T.hist_wide == 3 && wave == 0) {
+    } else if (mode == 0 && T.hist_wide == 3 && wave == 0) {
         // the task's whole-window root: keep its counts for the raw builds of later trees
         auto rc_out = gbl(T.root_cnt);
 #pragma unroll
@@ -1442,9 +1577,9 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
     if (active) {
         if (side == 0) {
-            node = mode == 0 ? 0 : T.small_node;
+            node = mode == 0 ? 0 : small_node;
         } else {
-            node = T.big_node;
+            node = big_node;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 qg[t] = par_q[t] - qg[t];
@@ -1517,9 +1652,10 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     }
 }
 
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
-    dim3 grid(S.n_features, 1, n_tasks);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
+                 int slot_parts) {
+    dim3 grid(S.n_features, 1, n_tasks * (mode == 2 ? SPEC_W : 1));
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, slot_parts);
 }
 
 // =================================================================== partition
@@ -1564,7 +1700,34 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
     Task& Tw = tasks[blockIdx.y];
     if (T.done) return;
     __shared__ Decision sd;
-    if (mode == 0) {
+    __shared__ int s_slot, s_chunk;
+    if (mode == 2) {
+        // multi-leaf step: the blocks of slot j are chunk0[j] .. chunk0[j + 1) - 1
+        if (threadIdx.x == 0) {
+            int j = -1;
+            for (int q = 0; q < T.n_slots; ++q)
+                if ((int)blockIdx.x >= T.slots[q].chunk0) j = q;
+            s_slot = j;
+            sd.leaf = -1;
+            if (j >= 0) {
+                const Slot& sl = T.slots[j];
+                s_chunk = (int)blockIdx.x - sl.chunk0;
+                sd.leaf = s_chunk * PART_CHUNK < sl.count ? sl.leaf : -1;
+                sd.type = sl.type;
+                sd.feat = sl.feat;
+                sd.feat2 = sl.feat2;
+                sd.thr = sl.thr;
+                sd.mgl = sl.mgl;
+                sd.nl = sl.nl;
+                sd.gain = sl.gain;
+                sd.start = sl.start;
+                sd.count = sl.count;
+                sd.buf = sl.buf;
+            }
+        }
+        __syncthreads();
+        if (sd.leaf < 0) return;   // past the step's last chunk (the selection decides when the tree ends)
+    } else if (mode == 0) {
         select_decide(T, S.n_features, P, blockIdx.x == 0, sd);
     } else {
         if (threadIdx.x == 0) {
@@ -1597,7 +1760,9 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
         return;
     }
     const int cnt = d.count, st = d.start, lbuf = d.buf;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int chunk = mode == 2 ? s_chunk : (int)blockIdx.x;
+    unsigned long long* lr = mode == 2 ? &T.slots[s_slot].part_lr : &Tw.part_lr;
+    if (mode != 2 && blockIdx.x == 0 && threadIdx.x == 0) {
         Tw.cur_leaf = d.leaf;
         Tw.cur_type = d.type;
         Tw.cur_feat = d.feat;
@@ -1617,7 +1782,7 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
         Tw.cur_left = T.n_nodes;
         Tw.cur_nl = d.nl;
     }
-    const int c0 = blockIdx.x * PART_CHUNK;
+    const int c0 = chunk * PART_CHUNK;
     if (c0 >= cnt) return;
     __shared__ int wsum[4], wsumr[4];
     __shared__ int baseL, baseR;
@@ -1653,7 +1818,7 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
             tr += wsumr[w];
         }
         // one reservation per block for both sides: same-address device atomics serialize
-        const unsigned long long old = atomicAdd(&Tw.part_lr, (unsigned long long)tr << 32 | (unsigned)tl);
+        const unsigned long long old = atomicAdd(lr, (unsigned long long)tr << 32 | (unsigned)tl);
         baseL = (int)(old & 0xffffffffull);
         baseR = (int)(old >> 32);
     }
@@ -1696,13 +1861,356 @@ __global__ __launch_bounds__(256) void k_select(Task* tasks, Shared S, GrowP P) 
     Tw.cur_nl = sd.nl;
 }
 
+// =================================================================== multi-leaf growth
+// Best-first growth (tree.cpp:343-392, 445-456) splits one leaf at a time, always the leaf with
+// the best candidate.  A leaf's candidate depends only on its rows (PAIR splits off), so leaves
+// may be split ahead of their turn: each step below splits the leaf best-first takes next plus
+// the next-best leaves of the frontier (up to SPEC_W), and the tree is the best-first replay over
+// the splits made.  A split the replay does not reach is dropped at the end (its leaf stays a
+// leaf; only the work is lost), so the tree, its node ids and every value are the one-split-
+// per-step tree's.
+
+// A grown node's candidate as a replay key: its gain if best-first may split it, else -inf.
+__device__ inline double spec_key(const DCand& c, const GrowP& P) {
+    return (c.valid && c.feature >= 0 && c.gain > P.gamma) ? c.gain : -INFINITY;
+}
+
+// The best-first replay over the grown nodes, by one whole wave: lane i holds frontier entry i
+// (node, replay id, key).  It takes the best entry (largest key, then lowest replay id, as
+// select_decide) while that node has been split (left[] >= 0) and the leaf budget allows,
+// replacing it by its children.  Returns 1 when the replay ends (no candidate or the budget is
+// spent: the tree is complete), 0 when the best entry is a leaf not split yet.  fid[] (if given)
+// receives the replay ids, fsplit[] marks the replayed splits.
+__device__ inline int spec_replay(const double* key, const int* left, const GrowP& P, int& fnode, int& fsim,
+                                  double& fg, int& n_front, int& n_final, int* fid, unsigned char* fsplit) {
+    const int lane = threadIdx.x & 63;
+    fnode = lane == 0 ? 0 : -1;
+    fsim = 0;
+    fg = lane == 0 ? key[0] : -INFINITY;
+    n_front = 1;
+    n_final = 1;
+    if (fid && lane == 0) fid[0] = 0;
+    const int budget = min(2 * P.max_leaves - 1, MAX_NODES);
+    while (true) {
+        const double g = fnode >= 0 ? fg : -INFINITY;
+        const int wl = wave_best_lane(g, fnode >= 0 ? fsim : 0x7fffffff);
+        const double bg = read_lane_f64(g, wl);
+        const int bx = __builtin_amdgcn_readlane(fnode, wl);
+        if (!(bg > -INFINITY) || n_final + 2 > budget) return 1;
+        const int L = left[bx];
+        if (L < 0) return 0;
+        if (lane == wl) {
+            fnode = L;
+            fsim = n_final;
+            fg = key[L];
+        }
+        if (lane == n_front) {
+            fnode = L + 1;
+            fsim = n_final + 1;
+            fg = key[L + 1];
+        }
+        if (lane == 0) {
+            if (fid) {
+                fid[L] = n_final;
+                fid[L + 1] = n_final + 1;
+            }
+            if (fsplit) fsplit[bx] = 1;
+        }
+        ++n_front;
+        n_final += 2;
+    }
+}
+
+// Candidates (tree.cpp:160-166: ascending features, first maximum kept) of the nodes the last
+// scan produced, [pend_first, pend_first + n_pend), into T.cands, key[] and pc[]; a 256-thread
+// block, 16 threads per pending node (n_pend <= 2 * SPEC_W = 16), each over features sub,
+// sub + 16, ... with its loads issued together, the 16 partial bests combined in feature order.
+// Recomputing them gives the same values, so the step's selection and the tree's final numbering
+// may both call it.
+__device__ void spec_pending(const Task& T, int F, const GrowP& P, double* key, DCand* pc) {
+    const int p0 = T.pend_first, np = T.n_pend;
+    __shared__ double rg[256];
+    __shared__ int rk[256], rt[256], rm[256], rcl[256];
+    {
+        const int pi = threadIdx.x >> 4, sub = threadIdx.x & 15;
+        double best = -INFINITY;
+        int bk = 0x7fffffff, bt = -1, bm = 1, bcl = 0;
+        if (pi < np) {
+            const int nd = p0 + pi;
+            for (int f0 = sub; f0 < F; f0 += 16 * 8) {
+                FCand c[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int f = f0 + 16 * u;
+                    if (f < F) c[u] = ldg(T.fcand + (size_t)nd * F + f);
+                    else c[u].thr = -1;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int f = f0 + 16 * u;
+                    if (c[u].thr < 0 || c[u].gain < P.gamma) continue;
+                    if (cand_better(c[u].gain, f, best, bk)) {
+                        best = c[u].gain;
+                        bk = f;
+                        bt = c[u].thr;
+                        bm = c[u].ml;
+                        bcl = c[u].cl;
+                    }
+                }
+            }
+        }
+        rg[threadIdx.x] = best;
+        rk[threadIdx.x] = bk;
+        rt[threadIdx.x] = bt;
+        rm[threadIdx.x] = bm;
+        rcl[threadIdx.x] = bcl;
+        __syncthreads();
+        if (sub == 0 && pi < np) {
+            for (int j = 1; j < 16; ++j) {
+                const int o = threadIdx.x + j;
+                if (cand_better(rg[o], rk[o], best, bk)) {
+                    best = rg[o];
+                    bk = rk[o];
+                    bt = rt[o];
+                    bm = rm[o];
+                    bcl = rcl[o];
+                }
+            }
+            const int nd = p0 + pi;
+            const DNode d = ldg(T.nodes + nd);
+            DCand c;
+            c.valid = bk != 0x7fffffff && d.v[N_DEPTH] < P.max_depth && d.v[N_NS] >= 2 * P.min_leaf;
+            c.gain = best;
+            c.feature = bk == 0x7fffffff ? -1 : bk;
+            c.feature2 = -1;
+            c.type = 0;
+            c.thr = bk == 0x7fffffff ? -1 : bt;
+            c.mgl = bk == 0x7fffffff ? 1 : bm;
+            c.cl = bk == 0x7fffffff ? 0 : bcl;
+            c.pad = 0;
+            pc[pi] = c;
+            T.cands[nd] = c;
+            key[nd] = spec_key(c, P);
+        }
+    }
+}
+
+// Selection of a multi-leaf step (one block per task): reduce the candidates of the nodes the last
+// scan produced (one wave per node), replay best-first over all grown nodes, and split the leaf the
+// replay stops at together with the next-best unsplit leaves of its frontier: at most SPEC_W, at
+// most the splits the leaf budget still allows, and never so many that the remaining splits could
+// not get node ids.  Writes the slots (with each slot's partition blocks and histogram tiles), the
+// children's ids and the node count; sets done when the replay completes the tree.
+__global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, GrowP P, int width, int min_per,
+                                                     int rb_target, int slot_parts) {
+    Task& Tw = tasks[blockIdx.x];
+    const Task T = tasks[blockIdx.x];
+    if (T.done) return;
+    const int F = S.n_features;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __shared__ double key[MAX_NODES];
+    __shared__ int left[MAX_NODES];
+    __shared__ DCand pc[2 * SPEC_W];
+    const int nn = T.n_nodes;
+    const int p0 = T.pend_first, np = T.n_pend;
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        const DNode nd = ldg(T.nodes + k);
+        left[k] = nd.v[N_LEAF] ? -1 : nd.v[N_LEFT];
+        if (k < p0 || k >= p0 + np) key[k] = spec_key(ldg(T.cands + k), P);
+    }
+    spec_pending(T, F, P, key, pc);
+    __syncthreads();
+    if (wave != 0) return;
+    int fnode, fsim, n_front, n_final;
+    double fg;
+    const int complete = spec_replay(key, left, P, fnode, fsim, fg, n_front, n_final, nullptr, nullptr);
+    if (complete) {
+        if (lane == 0) {
+            Tw.done = 1;
+            Tw.n_slots = 0;
+        }
+        return;
+    }
+    // the replay stopped at its best entry, an unsplit leaf: it and the next-best unsplit leaves
+    const int remaining = (min(2 * P.max_leaves - 1, MAX_NODES) - n_final) / 2;
+    int kmax = min(min(width, SPEC_W), remaining);
+    kmax = min(kmax, (T.spec_cap - nn) / 2 - (remaining - 1));
+    kmax = max(kmax, 1);
+    double g = (fnode >= 0 && left[fnode] < 0) ? fg : -INFINITY;
+    const int kk = fnode >= 0 ? fsim : 0x7fffffff;
+    int chosen = -1, n_chosen = 0;   // lane j < n_chosen holds the j-th chosen leaf
+    for (int j = 0; j < kmax; ++j) {
+        const int wl = wave_best_lane(g, kk);
+        if (!(read_lane_f64(g, wl) > -INFINITY)) break;
+        const int leaf = __builtin_amdgcn_readlane(fnode, wl);
+        if (lane == j) chosen = leaf;
+        if (lane == wl) g = -INFINITY;
+        ++n_chosen;
+    }
+    // lane j < n_chosen writes slot j; chunk and tile offsets are exclusive prefix sums
+    int chunks = 0, gx = 0, small = 0;
+    Slot sl;
+    if (lane < n_chosen) {
+        const DNode d = ldg(T.nodes + chosen);
+        const DCand c = chosen >= p0 && chosen < p0 + np ? pc[chosen - p0] : ldg(T.cands + chosen);
+        sl.leaf = chosen;
+        sl.left = nn + 2 * lane;
+        sl.start = d.seg_start;
+        sl.count = d.seg_count;
+        sl.buf = d.buf;
+        sl.type = c.type;
+        sl.feat = c.feature;
+        sl.feat2 = c.feature2;
+        sl.thr = c.thr;
+        sl.mgl = c.mgl;
+        sl.nl = c.cl;
+        sl.gain = c.gain;
+        sl.part_lr = 0;
+        sl.small_node = sl.big_node = sl.small_start = sl.small_count = sl.small_buf = -1;
+        sl.hist_parts = 0;
+        sl.hist_wide = 0;
+        sl.pad = 0;
+        chunks = (sl.count + PART_CHUNK - 1) / PART_CHUNK;
+        small = min(c.cl, sl.count - c.cl);
+    }
+    // rows per histogram tile, the same for every slot: the step's smaller children in about
+    // rb_target row blocks (so the step's items about fill the resident blocks), at least min_per
+    // rows, at most MAX_BLOCK_ROWS, and no slot with more than slot_parts blocks
+    const int total_small = wave_sum_i(small);
+    const int max_small = -wave_min_i(-small);
+    int per = max(min_per, (total_small + rb_target - 1) / rb_target);
+    per = max(per, (max_small + slot_parts - 1) / slot_parts);
+    per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
+    if (lane < n_chosen) gx = max(1, (small + per - 1) / per);
+    const int c_incl = wave_incl_i(chunks), g_incl = wave_incl_i(gx);
+    if (lane < n_chosen) {
+        sl.chunk0 = c_incl - chunks;
+        sl.blk0 = g_incl - gx;
+        sl.gx = gx;
+        sl.per = per;
+        T.slots[lane] = sl;
+    }
+    if (nn + 2 * n_chosen > T.spec_cap) {   // cannot happen while spec_cap >= 2 * max_leaves - 1
+        if (lane == 0) {
+            Tw.err = 3;
+            Tw.done = 1;
+            Tw.n_slots = 0;
+        }
+        return;
+    }
+    if (lane == 0) {
+        Tw.n_slots = n_chosen;
+        Tw.n_nodes = nn + 2 * n_chosen;
+        Tw.pend_first = nn;
+        Tw.n_pend = 2 * n_chosen;
+        Tw.spec_steps = T.spec_steps + 1;
+    }
+}
+
+// The grown tree in best-first order (one block per task, after the last step): replay
+// best-first over all grown nodes to the end, number the replayed nodes by the replay (the ids
+// one-split-per-step growth assigns), and rewrite the node table in that numbering.  A grown
+// split the replay did not take leaves its node a leaf.  The grown nodes are kept (tnodes) with,
+// for each grown leaf, the final leaf that holds its rows (tmap): the leaf-id pass walks their
+// row ranges.  An unfinished replay (more steps are needed) sets skip_end, so the tree-end
+// kernels enqueued behind it change nothing.
+__global__ __launch_bounds__(256) void k_spec_finalize(Task* tasks, Shared S, GrowP P) {
+    Task& Tw = tasks[blockIdx.x];
+    const Task T = tasks[blockIdx.x];
+    __shared__ DNode sn[MAX_NODES];
+    __shared__ double key[MAX_NODES];
+    __shared__ int left[MAX_NODES], fid[MAX_NODES], par[MAX_NODES];
+    __shared__ unsigned char fsplit[MAX_NODES];
+    __shared__ DCand pc[2 * SPEC_W];
+    __shared__ int s_final, s_ok;
+    const int nn = T.n_nodes;
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        sn[k] = T.nodes[k];
+        key[k] = spec_key(T.cands[k], P);
+        fid[k] = -1;
+        par[k] = -1;
+        fsplit[k] = 0;
+    }
+    __syncthreads();
+    // the last step's children may not have their candidates yet (their selection has not run)
+    if (!T.done) spec_pending(T, S.n_features, P, key, pc);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        left[k] = sn[k].v[N_LEAF] ? -1 : sn[k].v[N_LEFT];
+        if (!sn[k].v[N_LEAF]) {
+            par[sn[k].v[N_LEFT]] = k;
+            par[sn[k].v[N_RIGHT]] = k;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        int fnode, fsim, n_front, n_final;
+        double fg;
+        const int complete = spec_replay(key, left, P, fnode, fsim, fg, n_front, n_final, fid, fsplit);
+        if (threadIdx.x == 0) {
+            s_final = n_final;
+            s_ok = complete;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Tw.skip_end = s_ok ? 0 : 1;
+        if (T.host_flag) {   // the host waits for this: {1 finished | 2 more steps needed, steps}
+            T.host_flag[1] = T.spec_steps;
+            __threadfence_system();
+            T.host_flag[0] = s_ok ? 1 : 2;
+        }
+    }
+    if (!s_ok) return;   // more steps are needed: the host runs them and enqueues the tree end again
+    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
+        DNode d = sn[k];
+        T.tnodes[k] = d;
+        // the final leaf holding a grown leaf's rows: its deepest replayed ancestor-or-self
+        int a = k;
+        while (fid[a] < 0) a = par[a];
+        T.tmap[k] = d.v[N_LEAF] ? fid[a] : -1;
+        if (fid[k] < 0) continue;
+        if (fsplit[k]) {
+            d.v[N_LEFT] = fid[sn[k].v[N_LEFT]];
+            d.v[N_RIGHT] = fid[sn[k].v[N_RIGHT]];
+        } else if (!d.v[N_LEAF]) {
+            // a grown split best-first does not take: the node is a leaf of the tree
+            d.v[N_LEAF] = 1;
+            d.v[N_TYPE] = 0;
+            d.v[N_FEAT] = -1;
+            d.v[N_FEAT2] = -1;
+            d.v[N_THR] = -1;
+            d.v[N_LEFT] = -1;
+            d.v[N_RIGHT] = -1;
+            d.v[N_MGL] = 1;
+            d.gain = 0.0;
+        }
+        T.nodes[fid[k]] = d;
+    }
+    if (threadIdx.x == 0) {
+        Tw.n_tnodes = nn;
+        Tw.n_nodes = s_final;
+    }
+}
+
+void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int width,
+                        int min_per, int rb_target, int slot_parts) {
+    hipLaunchKernelGGL(k_spec_select, dim3(n_tasks), dim3(256), 0, st, tasks, S, P, width, min_per, rb_target,
+                       slot_parts);
+}
+
+void launch_spec_finalize(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P) {
+    hipLaunchKernelGGL(k_spec_finalize, dim3(n_tasks), dim3(256), 0, st, tasks, S, P);
+}
+
 void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P) {
     hipLaunchKernelGGL(k_select, dim3(n_tasks), dim3(256), 0, st, tasks, S, P);
 }
 
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
                       int mode) {
-    const int nch = std::max(1, (max_count + PART_CHUNK - 1) / PART_CHUNK);
+    // mode 2: the slots' leaves are disjoint, so their chunks number at most rows / chunk + slots
+    const int nch = std::max(1, (max_count + PART_CHUNK - 1) / PART_CHUNK) + (mode == 2 ? SPEC_W : 0);
     dim3 grid(nch, n_tasks);
     hipLaunchKernelGGL(k_partition, grid, dim3(256), 0, st, tasks, S, P, mode);
 }
@@ -1827,15 +2335,19 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
 // finds each of its rows' segment by binary search.
 __global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
     Task& T = tasks[blockIdx.y];
+    if (T.skip_end) return;
     __shared__ int st[MAX_NODES];        // segment start of non-empty leaves, -1 otherwise
     __shared__ int tab[MAX_NODES * 4];   // {seg_start, seg_count, node, buf} by start
     __shared__ int nl;
     if (threadIdx.x == 0) nl = 0;
-    const int nn = T.n_nodes;
+    // multi-leaf growth: the grown leaves' row ranges, each mapped to the final leaf holding it
+    const bool grown = T.n_tnodes > 0;
+    const DNode* nodes = grown ? T.tnodes : T.nodes;
+    const int nn = grown ? T.n_tnodes : T.n_nodes;
     for (int k = threadIdx.x; k < MAX_NODES; k += blockDim.x) {
         st[k] = -1;
         if (k < nn) {
-            const DNode& n = T.nodes[k];
+            const DNode& n = nodes[k];
             if (n.v[N_LEAF] && n.seg_count > 0) st[k] = n.seg_start;
         }
     }
@@ -1845,10 +2357,10 @@ __global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
         if (s0 < 0) continue;
         int r = 0;  // starts of non-empty leaves are distinct
         for (int j = 0; j < nn; ++j) r += (st[j] >= 0 && st[j] < s0);
-        const DNode& n = T.nodes[k];
+        const DNode& n = nodes[k];
         tab[4 * r] = s0;
         tab[4 * r + 1] = n.seg_count;
-        tab[4 * r + 2] = k;
+        tab[4 * r + 2] = grown ? T.tmap[k] : k;
         tab[4 * r + 3] = n.buf;
         atomicAdd(&nl, 1);
     }
@@ -1869,6 +2381,7 @@ __global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
 // Tree walk over the bins for every row (GOSS samples).
 __global__ __launch_bounds__(256) void k_nid_walk(Shared S, Task* tasks) {
     Task& T = tasks[blockIdx.y];
+    if (T.skip_end) return;
     __shared__ int nf[MAX_NODES], nf2[MAX_NODES], nt[MAX_NODES], nlft[MAX_NODES], nrgt[MAX_NODES], nflag[MAX_NODES];
     for (int k = threadIdx.x; k < T.n_nodes; k += blockDim.x) {
         const DNode& n = T.nodes[k];
@@ -1906,7 +2419,7 @@ __global__ __launch_bounds__(256) void k_nid_walk(Shared S, Task* tasks) {
 __global__ __launch_bounds__(256) void k_torb_accum(Task* tasks, const TorbChunk* const* chunks,
                                                     const int* n_chunks) {
     Task& T = tasks[blockIdx.y];
-    if ((int)blockIdx.x >= n_chunks[blockIdx.y]) return;
+    if (T.skip_end || (int)blockIdx.x >= n_chunks[blockIdx.y]) return;
     const TorbChunk c = chunks[blockIdx.y][blockIdx.x];
     __shared__ long long ag[MAX_NODES], ah[MAX_NODES];
     __shared__ int ac[MAX_NODES];
@@ -1937,6 +2450,7 @@ __global__ __launch_bounds__(256) void k_torb_accum(Task* tasks, const TorbChunk
 // lanes hold sweep segments, an exact int64 wave scan gives the prefix sums at every bucket cut.
 __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double min_cover, int iid) {
     const Task T = tasks[blockIdx.y];
+    if (T.skip_end) return;
     const int nn = T.n_nodes;
     const int B = T.n_buckets;
     const int lane = threadIdx.x & 63;
@@ -2003,7 +2517,7 @@ __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double 
 __global__ __launch_bounds__(256) void k_pred_update(Task* tasks, double lr, int pairwise) {
     Task& T = tasks[blockIdx.y];
     const int r0 = blockIdx.x * PRED_CHUNK;
-    if (r0 >= T.n_rows) return;
+    if (T.skip_end || r0 >= T.n_rows) return;
     const int r1 = min(T.n_rows, r0 + PRED_CHUNK);
     const int nn = T.n_nodes;
     const bool mse = !(pairwise && T.has_groups);
@@ -2061,7 +2575,7 @@ void launch_tree_end(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, 
 __global__ __launch_bounds__(64) void k_loss_groups(Task* tasks) {
     const Task& T = tasks[blockIdx.y];
     const int grp = blockIdx.x;
-    if (grp >= T.n_groups) return;
+    if (T.skip_end || grp >= T.n_groups) return;
     constexpr int STAGE = 1024;   // rows staged in LDS at a time (all their loads in flight)
     __shared__ double ys[STAGE], ps[STAGE];
     const int g0 = T.group_start[grp], m = T.group_start[grp + 1] - g0;
@@ -2134,6 +2648,7 @@ void launch_loss(hipStream_t st, Task* tasks, int n_tasks, int max_groups) {
 // sums of k_pred_update added in a fixed order.
 __global__ __launch_bounds__(256) void k_export(Task* tasks, ExportLayout L, char* out, int pairwise) {
     Task& T = tasks[blockIdx.x];
+    if (T.skip_end) return;
     char* base = out + (size_t)blockIdx.x * L.stride;
     const int nn = T.n_nodes;
     double loss = 0.0;
@@ -2224,10 +2739,12 @@ void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout&
 void init_kernel_attributes() {
     static bool done = false;
     if (done) return;
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false, 1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     done = true;
 }

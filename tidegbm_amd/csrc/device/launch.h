@@ -27,10 +27,24 @@ void launch_tree_start(hipStream_t st, const Shared& S, Task* tasks, int n_tasks
 void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool root,
                  bool contiguous_root, bool hconst, int max_count, int variant,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
+// mode 0: the root; 1: the smaller child of T.cur_* and its sibling; 2: every slot of a
+// multi-leaf step (partials of slot j from partial j * slot_parts).
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
+                 int slot_parts = 0);
+// Child-build tiles of a histogram variant: lanes per row, minimum rows per block, partials cap.
+void child_tiles(int variant, bool hconst, int& lpr, int& min_rows, int& pcap);
+// Multi-leaf growth (one task per session): the step's selection (histogram tiles of at least
+// min_per rows, about rb_target row blocks per step), its child builds (a fixed grid of `blocks`
+// blocks), and the tree's final numbering after the last step.
+void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int width,
+                        int min_per, int rb_target, int slot_parts);
+void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
+                      int slot_parts, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_spec_finalize(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
 // One block per task selects the next split into T.cur_* (for launches over many tasks).
 void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
-// mode 0: the partition launch selects the split itself; mode 1: the decision is in T.cur_*.
+// mode 0: the partition launch selects the split itself; mode 1: the decision is in T.cur_*;
+// mode 2: the slots of a multi-leaf step (max_count: the task's sample rows).
 void launch_partition(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, int max_count, const GrowP& P,
                       int mode);
 void launch_energy(hipStream_t st, Task* tasks, int task, int node, int count);

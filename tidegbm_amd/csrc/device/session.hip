@@ -119,6 +119,30 @@ struct tgbm_session {
     std::map<std::vector<long long>, CachedGraph> graphs;
     long long graph_clock = 0;
     bool use_graphs = true;
+    // multi-leaf growth (one-task sessions with max_leaves <= 64): leaves per step (0: off), the
+    // node capacity and partial slots it was allocated with, the steps of the captured loop (the
+    // most a recent tree needed, plus one), whether the current tree grows this way, and a pinned
+    // copy of the tree's {done, steps}
+    int n_cu = 256;         // compute units of the device
+    int spec_width = 0;
+    int spec_cap = 0;
+    int slot_parts = 0;
+    int spec_k = 8;
+    bool spec_tree = false;
+    int* flag_h = nullptr;      // pinned, mapped: the selection writes {done, steps} when a tree ends
+    int* flag_d = nullptr;      // its device address
+    bool spec_launched = false;     // the current tree's captured loop was launched without a wait
+    bool spec_wait = false;         // a multi-leaf tree's queued end has not been waited for
+    int spec_k_used = 0;            // the steps of that tree's captured loop
+    struct EndArgs {                // what a tree end was queued with
+        GrowP P;
+        tgbm_grow_params p;
+        double lr, min_cover;
+        int iid, max_groups, slot;
+        bool walk, pairwise, spec;
+        ExportLayout L;
+    };
+    EndArgs spec_end{};             // the unresolved multi-leaf tree's end
     // small device scratch for descriptor tables of launches made on behalf of other modules
     char* small_scratch = nullptr;
     size_t small_scratch_bytes = 0;
@@ -130,6 +154,7 @@ struct tgbm_session {
     struct PendingExport {
         int slot, a0, an;
         ExportLayout L;
+        bool spec;   // grown by multi-leaf steps (their rows were logged per step)
     };
     std::deque<PendingExport> pending;       // enqueued tree ends not yet collected (at most 2)
     int next_slot = 0;
@@ -155,6 +180,7 @@ struct tgbm_session {
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (flush_buf) (void)hipFree(flush_buf);
+        if (flag_h) (void)hipHostFree(flag_h);
         for (auto& kv : graphs) {
             (void)hipGraphExecDestroy(kv.second.ge);
             (void)hipGraphDestroy(kv.second.g);
@@ -314,6 +340,8 @@ void session_task_window(tgbm_session* s, int task, double** pred, int* row0, in
 }
 }  // namespace tgbm
 
+static void spec_resolve(tgbm_session* s);
+
 extern "C" {
 
 const char* tgbm_last_error(void) { return g_dev_err.c_str(); }
@@ -355,6 +383,11 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->hconst = h_const != 0;
         if (const char* e = std::getenv("TGBM_NO_GRAPH")) s->use_graphs = !(e[0] == '1');
         init_kernel_attributes();
+        {
+            int dev = 0, cu = 0;
+            HIP_OK(hipGetDevice(&dev));
+            if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cu > 0) s->n_cu = cu;
+        }
         HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         s->bins_row = s->alloc<unsigned char>((size_t)n_rows * s->fp);
         s->bins_col = s->alloc<unsigned char>((size_t)n_features * s->n_pad);
@@ -372,6 +405,16 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         // Partials per launch: launches of the split loop are sized for the largest sample rounded
         // up to a power of two (< 2x the rows), and no block may fold more than MAX_BLOCK_ROWS rows.
         s->S.part_cap = std::max(64, (int)((2LL * max_task_rows + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
+        if (n_tasks == 1 && max_leaves <= 64) {
+            // multi-leaf growth: room for the grown nodes (at most twice the final tree's) and for
+            // SPEC_W slots of partials, each as many as one smaller child's build may write
+            s->spec_cap = std::min(MAX_NODES, 2 * s->max_nodes);
+            s->slot_parts = std::max(64, (int)(((long long)max_task_rows / 2 + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
+            s->S.part_cap = std::max(s->S.part_cap, SPEC_W * s->slot_parts);
+            HIP_OK(hipHostMalloc((void**)&s->flag_h, 64, hipHostMallocMapped | hipHostMallocCoherent));
+            HIP_OK(hipHostGetDevicePointer((void**)&s->flag_d, s->flag_h, 0));
+        }
+        const int node_cap = std::max(s->max_nodes, s->spec_cap);
         s->tasks_h.assign(n_tasks, Task{});
         s->th.assign(n_tasks, TaskHost{});
         const size_t per_node = (size_t)n_features * NBIN;
@@ -406,9 +449,14 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.nodes = s->alloc<DNode>(MAX_NODES);
             T.cands = s->alloc<DCand>(MAX_NODES);
             T.fcand = s->alloc<FCand>((size_t)MAX_NODES * n_features);
-            T.hg = s->alloc<long long>(per_node * s->max_nodes);
-            T.hc = s->alloc<int>(per_node * s->max_nodes);
-            T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * s->max_nodes);
+            T.hg = s->alloc<long long>(per_node * node_cap);
+            T.hc = s->alloc<int>(per_node * node_cap);
+            T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * node_cap);
+            T.slots = s->alloc<Slot>(SPEC_W);
+            T.tnodes = s->alloc<DNode>(MAX_NODES);
+            T.tmap = s->alloc<int>(MAX_NODES);
+            T.spec_cap = node_cap;
+            T.host_flag = s->flag_d;
             T.energy = s->alloc<long long>(MAX_NODES);
             T.part = s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
             T.part_c = s->alloc<unsigned int>((size_t)s->S.part_cap * per_node);
@@ -450,6 +498,7 @@ static_assert(CAT_EDGE_FLAG == TGBM_CAT_EDGES, "categorical edge flag");
 
 int tgbm_session_set_cat(tgbm_session* s, const int* is_cat) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (s->S.is_cat) throw std::runtime_error("categorical flags are set once per session");
         std::vector<unsigned char> flags(s->F, 0);
         bool any = false;
@@ -472,8 +521,17 @@ int tgbm_session_set_cat(tgbm_session* s, const int* is_cat) {
     });
 }
 
+int tgbm_session_set_spec(tgbm_session* s, int width) {
+    return dev_guard([&] {
+        spec_resolve(s);
+        if (width < 0 || width > SPEC_W) throw std::runtime_error("multi-leaf width must be in [0, 8]");
+        s->spec_width = s->spec_cap > 0 ? width : 0;
+    });
+}
+
 int tgbm_session_set_monotone(tgbm_session* s, const int* monotone) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (s->S.monotone) throw std::runtime_error("monotone constraints are set once per session");
         if (!s->graphs.empty()) throw std::runtime_error("set monotone constraints before the first tree");
         std::vector<signed char> m(s->F, 0);
@@ -492,6 +550,7 @@ int tgbm_session_set_monotone(tgbm_session* s, const int* monotone) {
 
 int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long long* out) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         const Task& T = s->tasks_h[task];
         if (!T.nmask) throw std::runtime_error("the session has no categorical feature");
@@ -504,6 +563,7 @@ int tgbm_task_cat_masks(tgbm_session* s, int task, int n_nodes, unsigned long lo
 
 int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, const int* n_edges) {
     return dev_guard([&] {
+        spec_resolve(s);
         for (int f = 0; f < s->F; ++f) {
             const int ne = n_edges[f] & ~CAT_EDGE_FLAG;
             if (ne < 0 || ne > NBIN - 2 || (n_edges[f] & ~(CAT_EDGE_FLAG | (CAT_EDGE_FLAG - 1))))
@@ -541,6 +601,7 @@ int tgbm_session_bin(tgbm_session* s, const double* X, const double* edges, cons
 
 int tgbm_session_set_active(tgbm_session* s, int first, int count) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (first < 0 || count < 1 || first + count > s->T) throw std::runtime_error("active task range out of range");
         HIP_OK(hipStreamSynchronize(s->stream));
         s->a0 = first;
@@ -550,6 +611,7 @@ int tgbm_session_set_active(tgbm_session* s, int first, int count) {
 
 int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         TaskHost& H = s->th[task];
         if (n_rows < 1 || n_rows > H.cap_rows) throw std::runtime_error("task rows exceed the task's capacity");
@@ -583,6 +645,7 @@ int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows) {
 
 int tgbm_task_swap(tgbm_session* s, int a, int b) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (a < 0 || a >= s->T || b < 0 || b >= s->T) throw std::runtime_error("task out of range");
         if (a == b) return;
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -599,6 +662,7 @@ int tgbm_task_swap(tgbm_session* s, int a, int b) {
 int tgbm_task_node_hist(tgbm_session* s, int task, int node, long long* qg, int* count, long long* qh, int* s_g,
                         int* s_h) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         if (node < 0 || node >= s->max_nodes) throw std::runtime_error("node out of range");
         if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
@@ -620,6 +684,7 @@ int tgbm_task_node_hist(tgbm_session* s, int task, int node, long long* qg, int*
 
 int tgbm_session_get_bins(tgbm_session* s, unsigned char* out) {
     return dev_guard([&] {
+        spec_resolve(s);
         std::vector<unsigned char> tmp((size_t)s->n * s->fp);
         HIP_OK(hipMemcpy(tmp.data(), s->bins_row, tmp.size(), hipMemcpyDeviceToHost));
         for (int i = 0; i < s->n; ++i) std::memcpy(out + (size_t)i * s->F, &tmp[(size_t)i * s->fp], s->F);
@@ -631,6 +696,7 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
                        const int* cuts, const long long* groups) {
     (void)sweep_seg;
     return dev_guard([&] {
+        spec_resolve(s);
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         if (n_buckets < 1 || n_buckets > MAX_BUCKETS) throw std::runtime_error("n_buckets out of range for the device path");
         Task& T = s->tasks_h[task];
@@ -710,6 +776,7 @@ int tgbm_task_set_rows(tgbm_session* s, int task, const double* y, const double*
 
 int tgbm_task_grad(tgbm_session* s, int task, int objective) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (objective != 0 && objective != 5) throw std::runtime_error("objective not supported on the device path");
         if (task >= s->T) throw std::runtime_error("task out of range");
         const int t0 = task < 0 ? s->a0 : task, nt = task < 0 ? s->an : 1;
@@ -726,6 +793,7 @@ int tgbm_task_grad(tgbm_session* s, int task, int objective) {
 
 int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h) {
     return dev_guard([&] {
+        spec_resolve(s);
         const Task& T = s->tasks_h[task];
         const int m = s->th[task].n_rows;
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -736,6 +804,7 @@ int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h) {
 
 int tgbm_task_get_pred(tgbm_session* s, int task, double* pred) {
     return dev_guard([&] {
+        spec_resolve(s);
         HIP_OK(hipStreamSynchronize(s->stream));
         HIP_OK(hipMemcpy(pred, s->tasks_h[task].pred, sizeof(double) * s->th[task].n_rows, hipMemcpyDeviceToHost));
     });
@@ -744,6 +813,7 @@ int tgbm_task_get_pred(tgbm_session* s, int task, double* pred) {
 int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sample,
                          const unsigned char* scaled, double scale) {
     return dev_guard([&] {
+        spec_resolve(s);
         Task& T = s->tasks_h[task];
         TaskHost& H = s->th[task];
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -787,6 +857,7 @@ int tgbm_task_set_sample(tgbm_session* s, int task, const int* sample, int n_sam
 int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other, const int* perm, int n_perm,
                    int* tie) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
         if (s->hconst) throw std::runtime_error("GOSS scales h: the session must not declare h == 1");
         Task& T = s->tasks_h[task];
@@ -846,7 +917,10 @@ __global__ __launch_bounds__(256) void k_flush_cache(unsigned long long* buf, si
 }
 
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
+    spec_resolve(s);
     const GrowP P = s->growp(p);
+    s->spec_tree = false;
+    s->spec_launched = false;
     launch_tree_start(s->stream, s->S, s->td(), s->an, s->max_rows(), s->max_nodes);
     if (energy)
         for (int t = s->a0; t < s->a0 + s->an; ++t) HIP_OK(hipMemsetAsync(s->tasks_h[t].energy, 0, sizeof(long long) * MAX_NODES, s->stream));
@@ -901,13 +975,124 @@ static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_gro
     HIP_OK(hipGetLastError());
 }
 
+// The current tree's {done, steps} (one task), after the stream drains.
+static bool spec_done(tgbm_session* s, int* steps) {
+    const char* base = reinterpret_cast<const char*>(s->tasks_d + s->a0);
+    HIP_OK(hipMemcpyAsync(s->flag_h, base + offsetof(Task, done), sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(s->flag_h + 1, base + offsetof(Task, spec_steps), sizeof(int), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    if (steps) *steps = s->flag_h[1];
+    return s->flag_h[0] != 0;
+}
+
+// One multi-leaf step: the selection, the slots' partitions, their smaller children's builds and
+// the scans of both children of every slot.
+static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
+    const int ms = s->launch_rows();
+    // child builds: two resident 512-thread blocks per CU take the step's (row block, feature
+    // block) items; tiles of at least 2,048 rows, sized so the items about fill those blocks
+    const int blocks = (2 * s->n_cu + 7) / 8 * 8;   // a multiple of the 8 XCDs
+    const int fbs = (s->fp + 15) / 16;
+    EventPair e0, e1, e2, e3;
+    if (s->timing) s->tic(6, 0.0, e0);   // the step's selection (timer 6)
+    launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 2048, std::max(1, blocks / fbs),
+                       s->slot_parts);
+    if (s->timing) s->toc(e0);
+    if (s->timing) s->tic(1, 0.0, e1);
+    launch_partition(s->stream, s->S, s->td(), s->an, ms, P, 2);
+    if (s->timing) s->toc(e1);
+    if (s->timing) s->kernel_events(5, 0.0, e2);
+    launch_hist_spec(s->stream, s->S, s->td(), s->an, s->hconst, blocks, s->slot_parts,
+                     s->timing ? e2.a : nullptr, s->timing ? e2.b : nullptr);
+    if (s->timing) s->events.push_back(e2);
+    if (s->timing) s->tic(2, 0.0, e3);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 2, s->slot_parts);
+    if (s->timing) s->toc(e3);
+    HIP_OK(hipGetLastError());
+}
+
+// Rows of the step just run (timing mode): the smaller children built and the leaves partitioned.
+static void spec_log_rows(tgbm_session* s) {
+    Task T;
+    HIP_OK(hipMemcpy(&T, s->tasks_d + s->a0, sizeof(Task), hipMemcpyDeviceToHost));
+    Slot sl[SPEC_W];
+    HIP_OK(hipMemcpy(sl, T.slots, sizeof(sl), hipMemcpyDeviceToHost));
+    double small = 0, rows = 0;
+    for (int j = 0; j < (T.done ? 0 : T.n_slots); ++j) {
+        small += sl[j].small_count;
+        rows += sl[j].count;
+    }
+    s->log_rows[5].push_back(small);
+    s->log_rows[1].push_back(rows);
+    s->log_rows[6].push_back(T.done ? 0 : T.n_slots);   // splits the selection made
+}
+
+// Multi-leaf growth of the current tree.  The captured loop holds spec_k steps (the most the last
+// tree needed, plus one) and is only launched here: the tree end is queued behind it and then
+// waits for it (tree_end_enqueue), so the device never waits for the host in between.  A tree
+// that needs more steps continues with single steps (every step makes at least one split the tree
+// keeps, so at most max_leaves - 1 steps end any tree).  Timed and graph-less growth runs step by
+// step, reading the tree's state after each.
+static void grow_spec(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
+    const int limit = std::max(1, p->max_leaves - 1);
+    int steps = 0;
+    bool done = false;
+    if (!s->use_graphs || s->timing) {
+        for (int it = 0; it < limit && !done; ++it) {
+            spec_step(s, P, p);
+            done = spec_done(s, &steps);
+            if (s->timing) spec_log_rows(s);
+        }
+    } else {
+        const int K = std::min(limit, s->spec_k);
+        const std::vector<long long> key{s->launch_rows(), p->max_leaves, p->hist_mode, s->a0, s->an, -K};
+        auto it = s->graphs.find(key);
+        if (it == s->graphs.end()) {
+            HIP_OK(hipStreamSynchronize(s->stream));
+            if (s->graphs.size() >= 256) {
+                auto lru = s->graphs.begin();
+                for (auto j = s->graphs.begin(); j != s->graphs.end(); ++j)
+                    if (j->second.last_use < lru->second.last_use) lru = j;
+                HIP_OK(hipGraphExecDestroy(lru->second.ge));
+                HIP_OK(hipGraphDestroy(lru->second.g));
+                s->graphs.erase(lru);
+            }
+            tgbm_session::CachedGraph c;
+            HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < K; ++k) spec_step(s, P, p);
+            HIP_OK(hipStreamEndCapture(s->stream, &c.g));
+            const hipError_t e = hipGraphInstantiate(&c.ge, c.g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(c.g);
+                HIP_OK(e);
+            }
+            it = s->graphs.emplace(key, c).first;
+        }
+        it->second.last_use = ++s->graph_clock;
+        s->flag_h[0] = 0;
+        HIP_OK(hipGraphLaunch(it->second.ge, s->stream));
+        // the tree end is queued behind the loop and waited for later (spec_resolve)
+        s->spec_launched = true;
+        s->spec_k_used = K;
+        return;
+    }
+    if (!done) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
+}
+
 int tgbm_tree_begin(tgbm_session* s, const tgbm_grow_params* p) {
     return dev_guard([&] { tree_begin_impl(s, p, p->host_driven != 0, p->host_driven == 0); });
 }
 
 int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
     return dev_guard([&] {
+        spec_resolve(s);
         const GrowP P = s->growp(p);
+        s->spec_tree = s->spec_width > 0 && s->an == 1 && p->max_leaves <= 64;
+        if (s->spec_tree) {
+            grow_spec(s, P, p);
+            return;
+        }
         if (!s->use_graphs || s->timing) {
             for (int it = 0; it < p->max_leaves - 1; ++it) split_step(s, P, 0, p);
             return;
@@ -948,6 +1133,7 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
 int tgbm_leaf_scan_results(tgbm_session* s, int task, int node, double* gain_f, int* thr_f, int* ml_f,
                            double* energy, int* n_rows) {
     return dev_guard([&] {
+        spec_resolve(s);
         const Task& T = s->tasks_h[task];
         std::vector<FCand> fc(s->F);
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -970,6 +1156,7 @@ int tgbm_leaf_scan_results(tgbm_session* s, int task, int node, double* gain_f, 
 int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const int* f2, int n_pairs,
                      const tgbm_grow_params* p, tgbm_cand* out) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (n_pairs <= 0) return;
         if (n_pairs > 32) throw std::runtime_error("too many PAIR candidates");
         std::vector<int> pf(2 * n_pairs);
@@ -1007,6 +1194,7 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
 int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tgbm_grow_params* p,
                      int* left_id) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (s->an != 1 || task != s->a0)
             throw std::runtime_error("host-driven growth needs exactly one active task, the one named");
         struct {
@@ -1036,14 +1224,72 @@ int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tg
 
 // Enqueue the end of every active task's tree: leaf table, TORB sums, prediction update, loss,
 // export into the next host slot (the copy completes in stream order; an event marks it).
+// The tree end: final numbering (multi-leaf trees), leaf ids, TORB, predictions, loss, export
+// into the tree's host slot.
+static void enqueue_end(tgbm_session* s, const tgbm_session::EndArgs& E) {
+    EventPair ep;
+    if (s->timing) s->tic(3, 0.0, ep);
+    if (E.spec) launch_spec_finalize(s->stream, s->S, s->td(), s->an, E.P);
+    launch_tree_end(s->stream, s->S, s->td(), s->an, E.walk, s->max_rows(),
+                    (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, E.P,
+                    E.min_cover, E.iid, E.lr, E.pairwise ? 1 : 0);
+    if (E.pairwise && E.max_groups > 0) launch_loss(s->stream, s->td(), s->an, E.max_groups);
+    if (s->timing) s->toc(ep);
+    launch_export(s->stream, s->td(), s->an, E.L, s->export_d, E.pairwise ? 1 : 0);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(s->export_h[E.slot], s->export_d, E.L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipEventRecord(s->export_ev[E.slot], s->stream));
+}
+
+// A multi-leaf tree whose end was queued behind its captured loop: the end's first kernel (the
+// final numbering) reports whether the loop finished the tree, and if not the other kernels do
+// nothing.  Wait for that report (the host meanwhile did its own work), and for an unfinished
+// tree grow on step by step, queue the tree end again and let it drain (so no late report meets
+// the next tree's).  Every entry point that queues work or reads task state calls this first.
+static void spec_resolve(tgbm_session* s) {
+    if (!s->spec_wait) return;
+    s->spec_wait = false;
+    const tgbm_session::EndArgs E = s->spec_end;
+    volatile int* flag = s->flag_h;
+    for (long spin = 0; !flag[0]; ++spin) {
+        if ((spin & 255) == 255) {
+            const hipError_t q = hipEventQuery(s->export_ev[E.slot]);
+            if (q == hipSuccess && !flag[0]) throw std::runtime_error("internal: the tree end reported nothing");
+            if (q != hipSuccess && q != hipErrorNotReady) HIP_OK(q);
+        }
+    }
+    bool done = flag[0] == 1;
+    int steps = flag[1];
+    const int limit = std::max(1, E.p.max_leaves - 1);
+    if (!done) {
+        for (int k = s->spec_k_used; k < limit && !done; ++k) {
+            spec_step(s, E.P, &E.p);
+            done = spec_done(s, &steps);
+        }
+        if (!done) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
+        enqueue_end(s, E);
+        HIP_OK(hipStreamSynchronize(s->stream));
+        if (flag[0] != 1) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
+        steps = flag[1];
+    }
+    s->spec_k = std::min(limit, steps + 1);
+}
+
 static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_cover, int iid,
                              const tgbm_grow_params* p) {
+    spec_resolve(s);
     if (s->pending.size() >= 2) throw std::runtime_error("at most two tree ends may wait for collection");
-    const GrowP P = s->growp(p);
-    bool walk = !s->all_contiguous();
+    tgbm_session::EndArgs E;
+    E.P = s->growp(p);
+    E.p = *p;
+    E.lr = learning_rate;
+    E.min_cover = min_cover;
+    E.iid = iid;
+    E.walk = !s->all_contiguous();
+    E.spec = s->spec_tree;
     // training-loss partials (the prediction update and the pairwise group pass write them)
-    const bool pairwise = s->objective == 5;
-    int max_groups = 0;
+    E.pairwise = s->objective == 5;
+    E.max_groups = 0;
     for (int t = s->a0; t < s->a0 + s->an; ++t) {
         TaskHost& H = s->th[t];
         const int need = 2 * loss_chunks(H.cap_rows) + H.n_groups;
@@ -1053,23 +1299,16 @@ static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_c
             s->tasks_h[t].loss_part = H.loss_d;
             s->set_field_ptr(t, offsetof(Task, loss_part), H.loss_d);
         }
-        max_groups = std::max(max_groups, H.n_groups);
+        E.max_groups = std::max(E.max_groups, H.n_groups);
     }
-    EventPair ep;
-    if (s->timing) s->tic(3, 0.0, ep);
-    launch_tree_end(s->stream, s->S, s->td(), s->an, walk, s->max_rows(),
-                    (const TorbChunk* const*)(s->chunk_ptrs_d + s->a0), s->n_chunks_d + s->a0, s->max_chunks, P,
-                    min_cover, iid, learning_rate, pairwise ? 1 : 0);
-    if (pairwise && max_groups > 0) launch_loss(s->stream, s->td(), s->an, max_groups);
-    if (s->timing) s->toc(ep);
-    HIP_OK(hipGetLastError());
     int bmax = 1;
     for (int t = 0; t < s->T; ++t) bmax = std::max(bmax, s->th[t].n_buckets);
-    const ExportLayout L = export_layout(s->max_nodes, bmax);
-    const size_t need = L.stride * s->T;
+    E.L = export_layout(s->max_nodes, bmax);
+    const size_t need = E.L.stride * s->T;
     if (need > s->export_bytes) {
         // the layout grows only when a task's bucket count does (a new fit): no tree is pending
         if (!s->pending.empty()) throw std::runtime_error("internal: export slots resized with a tree pending");
+        HIP_OK(hipStreamSynchronize(s->stream));
         for (int k = 0; k < 2; ++k) {
             if (s->export_h[k]) HIP_OK(hipHostFree(s->export_h[k]));
             s->export_h[k] = nullptr;
@@ -1079,12 +1318,16 @@ static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_c
         s->export_d = s->alloc<char>(need);
         s->export_bytes = need;
     }
-    launch_export(s->stream, s->td(), s->an, L, s->export_d, pairwise ? 1 : 0);
-    HIP_OK(hipGetLastError());
-    const int slot = s->next_slot;
-    HIP_OK(hipMemcpyAsync(s->export_h[slot], s->export_d, L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
-    HIP_OK(hipEventRecord(s->export_ev[slot], s->stream));
-    s->pending.push_back(tgbm_session::PendingExport{slot, s->a0, s->an, L});
+    E.slot = s->next_slot;
+    enqueue_end(s, E);
+    if (s->spec_launched) {
+        // the captured loop ran without a wait: resolve the tree when the host next needs it
+        s->spec_launched = false;
+        s->spec_end = E;
+        s->spec_wait = true;
+    }
+    s->pending.push_back(tgbm_session::PendingExport{E.slot, s->a0, s->an, E.L, E.spec});
+    s->spec_tree = false;
     s->next_slot ^= 1;
 }
 
@@ -1093,6 +1336,8 @@ static void tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_node
                          double* values, int* cover, double* prefix) {
     if (s->pending.empty()) throw std::runtime_error("no tree end to collect");
     const tgbm_session::PendingExport pe = s->pending.front();
+    // the unresolved multi-leaf tree's export is final only once the tree is resolved
+    if (s->spec_wait && s->spec_end.slot == pe.slot) spec_resolve(s);
     s->pending.pop_front();
     HIP_OK(hipEventSynchronize(s->export_ev[pe.slot]));
     const ExportLayout& L = pe.L;
@@ -1102,8 +1347,10 @@ static void tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_node
         const char* base = s->export_h[pe.slot] + L.stride * i;
         const int nn = *(const int*)base;
         if (nn < 1 || nn > MN) throw std::runtime_error("internal: exported tree size out of range");
-        if (*((const int*)base + 1) != 0)
+        const int err = *((const int*)base + 1);
+        if (err == 1)
             throw std::runtime_error("internal: a partition placed a different number of rows left than its split's histogram counts");
+        if (err != 0) throw std::runtime_error("internal: multi-leaf growth left the tree unfinished or out of node space");
         n_nodes[i] = nn;
         s->th[t].last_loss = *((const double*)base + 1);
         const int B = s->th[t].n_buckets;
@@ -1117,7 +1364,7 @@ static void tree_collect(tgbm_session* s, const tgbm_grow_params* p, int* n_node
         std::memcpy(cover + (size_t)i * MN, base + L.off_cover, sizeof(int) * nn);
         std::memcpy(prefix + (size_t)i * MN * MAX_BUCKETS, base + L.off_prefix, sizeof(double) * nn * B);
         if (s->timing) {  // rows of the root and of every child histogram build, in launch order
-            const int steps = p->max_leaves - 1;
+            const int steps = pe.spec ? 0 : p->max_leaves - 1;
             s->log_rows[0].push_back(nodes[0].seg_count);
             for (int k = 0; k < steps; ++k) {
                 const bool grew = 2 * k + 2 < nn;
@@ -1138,6 +1385,7 @@ int tgbm_tree_end(tgbm_session* s, double learning_rate, double min_cover, int i
                   const tgbm_grow_params* p, int* n_nodes, int* node_ints, double* node_dbl,
                   double* values, int* cover, double* prefix) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
         tree_end_enqueue(s, learning_rate, min_cover, iid, p);
         tree_collect(s, p, n_nodes, node_ints, node_dbl, values, cover, prefix);
@@ -1162,6 +1410,7 @@ int tgbm_task_train_loss(tgbm_session* s, int task, double* loss) {
 
 int tgbm_timer_enable(tgbm_session* s, int on) {
     return dev_guard([&] {
+        spec_resolve(s);
         s->drain_events();
         s->timing = on != 0;
         s->flush = on == 2;
@@ -1182,6 +1431,7 @@ int tgbm_timer_enable(tgbm_session* s, int on) {
 int tgbm_timer_log(tgbm_session* s, int which, double* ms, double* rows, int cap) {
     int n = -1;
     if (dev_guard([&] {
+            spec_resolve(s);
             if (which < 0 || which >= 8) throw std::runtime_error("timer index");
             s->drain_events();
             n = (int)s->log_ms[which].size();
@@ -1196,6 +1446,7 @@ int tgbm_timer_log(tgbm_session* s, int which, double* ms, double* rows, int cap
 
 int tgbm_timer_read(tgbm_session* s, int which, double* ms, long long* launches, double* bytes) {
     return dev_guard([&] {
+        spec_resolve(s);
         if (which < 0 || which >= 8) throw std::runtime_error("timer index");
         s->drain_events();
         *ms = s->t_ms[which];

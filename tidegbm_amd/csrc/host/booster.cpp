@@ -937,6 +937,7 @@ void Booster::fit(const double* X0, const double* y0, int n, int F, const long l
     dev_check(tgbm_session_create(&sg.s, n, F, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
     if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
     if (!cfg.monotone.empty()) dev_check(tgbm_session_set_monotone(sg.s, cfg.monotone.data()));
+    dev_check(tgbm_session_set_spec(sg.s, cfg.spec_width > 1 ? cfg.spec_width : 0));
     bin_on_device(sg.s, X, edges, is_cat, cat_values);
     const long long* groups = (cfg.objective == 5) ? rg : nullptr;
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? ws.data() : nullptr, tl.pred.data(), tl.bucket.data(),
@@ -1088,6 +1089,7 @@ void Booster::update(const double* X, const double* y, int n, int nf, const doub
     dev_check(tgbm_session_create(&sg.s, n, nf, n_bins, 1, &n, cfg.max_leaves, hconst ? 1 : 0));
     if (has_cat()) dev_check(tgbm_session_set_cat(sg.s, is_cat.data()));
     if (!cfg.monotone.empty()) dev_check(tgbm_session_set_monotone(sg.s, cfg.monotone.data()));
+    dev_check(tgbm_session_set_spec(sg.s, cfg.spec_width > 1 ? cfg.spec_width : 0));
     bin_on_device(sg.s, X, edges, is_cat, cat_values);
     std::vector<int> zeros(n, 0);
     dev_check(tgbm_task_set_rows(sg.s, 0, y, any_weight ? w : nullptr, pred.data(), zeros.data(), nullptr, 1, nullptr,

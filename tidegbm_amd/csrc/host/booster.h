@@ -73,6 +73,7 @@ struct Config {
     int profile = 0;
     int hist_mode = 0;       // histogram kernel variant; 0: defaults, tuned per fit when autotune
     int autotune = 1;        // choose the variant online over the first trees (tuner.h)
+    int spec_width = 8;      // leaves split per step of device-driven growth (1: one at a time)
     int force_host_driver = 0;
     int early_stopping_rounds = 0;   // config.hpp early_stopping_rounds (0: never stop early)
     std::string eval_metric = "auto";  // config.hpp:76: auto | loss | ic

@@ -66,6 +66,13 @@ int tide_set_int(TideHandle h, const char* key, int v) {
     else if (k == "profile") c.profile = v;
     else if (k == "hist_mode") c.hist_mode = v;
     else if (k == "autotune") c.autotune = v;
+    else if (k == "spec_width") {
+        if (v < 1 || v > 8) {
+            g_err = "spec_width must be in [1, 8]";
+            return -1;
+        }
+        c.spec_width = v;
+    }
     else if (k == "force_host_driver") c.force_host_driver = v;
     else if (k == "early_stopping_rounds") c.early_stopping_rounds = v;
     else if (k == "regime_feature") c.regime_feature = v;
