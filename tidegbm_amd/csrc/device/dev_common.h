@@ -222,7 +222,10 @@ struct Task {
     int spec_cap;          // nodes the task's histogram buffers hold
     int* host_flag;        // mapped host memory: {done, steps} written when a tree's growth ends
     int skip_end;          // the tree's growth was not finished: the tree-end kernels do nothing
-    int pad_e;
+    int rs_valid;          // the replay's frontier below is this tree's (0: replay from the root)
+    int rs_front, rs_final; // frontier entries and replay ids used at the last stop
+    int* rs_node;          // [64] frontier nodes at the last stop (-1: empty entry)
+    int* rs_sim;           // [64] their replay ids
 };
 
 struct Shared {            // the session-wide matrix

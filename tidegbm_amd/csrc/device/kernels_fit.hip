@@ -258,6 +258,7 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     T.spec_steps = 0;
     T.n_tnodes = 0;
     T.skip_end = 0;
+    T.rs_valid = 0;
     T.cur_leaf = -1;
     T.scan_ticket = 0;
     T.part_lr = 0;
@@ -1881,15 +1882,23 @@ __device__ inline double spec_key(const DCand& c, const GrowP& P) {
 // replacing it by its children.  Returns 1 when the replay ends (no candidate or the budget is
 // spent: the tree is complete), 0 when the best entry is a leaf not split yet.  fid[] (if given)
 // receives the replay ids, fsplit[] marks the replayed splits.
+// With resume set, the replay continues from the frontier in fnode / fsim / n_front / n_final
+// (a stop of an earlier replay over the same tree: splits are only ever added, so continuing
+// from there equals replaying from the root).
 __device__ inline int spec_replay(const double* key, const int* left, const GrowP& P, int& fnode, int& fsim,
-                                  double& fg, int& n_front, int& n_final, int* fid, unsigned char* fsplit) {
+                                  double& fg, int& n_front, int& n_final, int* fid, unsigned char* fsplit,
+                                  bool resume = false) {
     const int lane = threadIdx.x & 63;
-    fnode = lane == 0 ? 0 : -1;
-    fsim = 0;
-    fg = lane == 0 ? key[0] : -INFINITY;
-    n_front = 1;
-    n_final = 1;
-    if (fid && lane == 0) fid[0] = 0;
+    if (resume) {
+        fg = fnode >= 0 ? key[fnode] : -INFINITY;
+    } else {
+        fnode = lane == 0 ? 0 : -1;
+        fsim = 0;
+        fg = lane == 0 ? key[0] : -INFINITY;
+        n_front = 1;
+        n_final = 1;
+        if (fid && lane == 0) fid[0] = 0;
+    }
     const int budget = min(2 * P.max_leaves - 1, MAX_NODES);
     while (true) {
         const double g = fnode >= 0 ? fg : -INFINITY;
@@ -2021,9 +2030,24 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
     spec_pending(T, F, P, key, pc);
     __syncthreads();
     if (wave != 0) return;
-    int fnode, fsim, n_front, n_final;
+    int fnode = -1, fsim = 0, n_front = 1, n_final = 1;
     double fg;
-    const int complete = spec_replay(key, left, P, fnode, fsim, fg, n_front, n_final, nullptr, nullptr);
+    const bool resume = T.rs_valid != 0;
+    if (resume) {
+        fnode = T.rs_node[lane];
+        fsim = T.rs_sim[lane];
+        n_front = T.rs_front;
+        n_final = T.rs_final;
+    }
+    const int complete = spec_replay(key, left, P, fnode, fsim, fg, n_front, n_final, nullptr, nullptr, resume);
+    // keep the stop for the next step's selection
+    T.rs_node[lane] = fnode;
+    T.rs_sim[lane] = fsim;
+    if (lane == 0) {
+        Tw.rs_front = n_front;
+        Tw.rs_final = n_final;
+        Tw.rs_valid = 1;
+    }
     if (complete) {
         if (lane == 0) {
             Tw.done = 1;

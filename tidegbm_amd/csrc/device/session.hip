@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <deque>
 #include <map>
 #include <stdexcept>
@@ -134,6 +135,8 @@ struct tgbm_session {
     bool spec_launched = false;     // the current tree's captured loop was launched without a wait
     bool spec_wait = false;         // a multi-leaf tree's queued end has not been waited for
     int spec_k_used = 0;            // the steps of that tree's captured loop
+    int spec_hist[32] = {0};        // steps the last trees needed (ring)
+    int spec_hist_n = 0;
     struct EndArgs {                // what a tree end was queued with
         GrowP P;
         tgbm_grow_params p;
@@ -455,6 +458,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.slots = s->alloc<Slot>(SPEC_W);
             T.tnodes = s->alloc<DNode>(MAX_NODES);
             T.tmap = s->alloc<int>(MAX_NODES);
+            T.rs_node = s->alloc<int>(64);
+            T.rs_sim = s->alloc<int>(64);
             T.spec_cap = node_cap;
             T.host_flag = s->flag_d;
             T.energy = s->alloc<long long>(MAX_NODES);
@@ -1251,12 +1256,16 @@ static void spec_resolve(tgbm_session* s) {
     s->spec_wait = false;
     const tgbm_session::EndArgs E = s->spec_end;
     volatile int* flag = s->flag_h;
-    for (long spin = 0; !flag[0]; ++spin) {
-        if ((spin & 255) == 255) {
-            const hipError_t q = hipEventQuery(s->export_ev[E.slot]);
-            if (q == hipSuccess && !flag[0]) throw std::runtime_error("internal: the tree end reported nothing");
-            if (q != hipSuccess && q != hipErrorNotReady) HIP_OK(q);
-        }
+    // Poll the flag alone; ask the runtime (which takes its locks) only every 200 us, so the
+    // wait does not slow the runtime's own work on the stream.
+    auto last = std::chrono::steady_clock::now();
+    while (!flag[0]) {
+        const auto now = std::chrono::steady_clock::now();
+        if (now - last < std::chrono::microseconds(200)) continue;
+        last = now;
+        const hipError_t q = hipEventQuery(s->export_ev[E.slot]);
+        if (q == hipSuccess && !flag[0]) throw std::runtime_error("internal: the tree end reported nothing");
+        if (q != hipSuccess && q != hipErrorNotReady) HIP_OK(q);
     }
     bool done = flag[0] == 1;
     int steps = flag[1];
@@ -1272,7 +1281,12 @@ static void spec_resolve(tgbm_session* s) {
         if (flag[0] != 1) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
         steps = flag[1];
     }
-    s->spec_k = std::min(limit, steps + 1);
+    // the next loop's steps: the most any of the last 32 trees needed (its splitting steps plus
+    // the selection that finds it finished), so a tree rarely needs the step-by-step path
+    s->spec_hist[s->spec_hist_n++ % 32] = steps + 1;
+    int kmax = 1;
+    for (int k = 0; k < std::min(s->spec_hist_n, 32); ++k) kmax = std::max(kmax, s->spec_hist[k]);
+    s->spec_k = std::min(limit, kmax);
 }
 
 static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_cover, int iid,
