@@ -90,6 +90,8 @@ struct Slot {
     double gain;
     unsigned long long part_lr; // partition reservations: right << 32 | left
 };
+// kernels copy the slot table into LDS as 4-byte words, one per thread of a 256-thread block
+static_assert(sizeof(Slot) % 8 == 0 && sizeof(Slot) / 4 * SPEC_W <= 256, "slot table copy");
 
 struct FCand {      // best split of one feature of one node
     double gain;

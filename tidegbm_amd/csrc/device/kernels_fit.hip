@@ -6,6 +6,9 @@
 // formula must round like the reference's x86-64 build (no fused multiply-add).
 #include <hip/hip_ext.h>
 
+#include <stdexcept>
+#include <string>
+
 #include "dev_common.h"
 #include "launch.h"
 
@@ -348,14 +351,15 @@ __device__ inline bool hist_target(const Task& T, Task* Tw, bool root, bool lead
 // blocks of slot j are blk0[j] .. blk0[j] + gx[j] - 1.  The slot's first block applies the node
 // bookkeeping of its split (the selection assigned the children's ids and the node count) and
 // records the smaller child and the partials' form for the scan.
-__device__ inline bool spec_target(const Task& T, Task* Tw, int bx, bool by0, int& sj, int& rb, int& node,
-                                   int& start, int& count, int& buf, int& per) {
+__device__ inline bool spec_target(const Task& T, Task* Tw, const Slot* slots, int bx, bool by0, int& sj, int& rb,
+                                   int& node, int& start, int& count, int& buf, int& per) {
     if (T.done) return false;
     sj = -1;
     for (int q = 0; q < T.n_slots; ++q)
-        if (bx >= T.slots[q].blk0) sj = q;
+        if (bx >= slots[q].blk0) sj = q;
     if (sj < 0) return false;
-    Slot& sl = T.slots[sj];
+    const Slot& sl = slots[sj];
+    Slot& slw = T.slots[sj];   // the slot in global memory, for what the scan reads
     rb = bx - sl.blk0;
     if (rb >= sl.gx) return false;
     const int nl = (int)(sl.part_lr & 0xffffffffull), cnt = sl.count, st = sl.start;
@@ -371,13 +375,13 @@ __device__ inline bool spec_target(const Task& T, Task* Tw, int bx, bool by0, in
         if (sl.nl != nl) Tw->err = 1;
         apply_nodes_d(T, Tw, sl.leaf, sl.type, sl.feat, sl.feat2, sl.thr, sl.mgl, sl.gain, L, st, cnt, sl.buf, nl,
                       false);
-        sl.small_node = node;
-        sl.big_node = left_small ? R : L;
-        sl.small_start = start;
-        sl.small_count = count;
-        sl.small_buf = buf;
-        sl.hist_parts = (count + per - 1) / per;
-        sl.hist_wide = per > WORD_ROWS ? 1 : 0;
+        slw.small_node = node;
+        slw.big_node = left_small ? R : L;
+        slw.small_start = start;
+        slw.small_count = count;
+        slw.small_buf = buf;
+        slw.hist_parts = (count + per - 1) / per;
+        slw.hist_wide = per > WORD_ROWS ? 1 : 0;
     }
     return true;
 }
@@ -538,11 +542,20 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
     const Task T = tasks[blockIdx.z];
     if (SPEC) {   // root == 2
         if (T.done || T.n_slots <= 0) return;
+        // the step's slot table, read once into LDS (one memory round trip for all items)
+        __shared__ Slot slots[SPEC_W];
+        {
+            constexpr int WORDS = (int)(sizeof(Slot) / 4) * SPEC_W;
+            const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
+            if ((int)threadIdx.x < nw && (int)threadIdx.x < WORDS)
+                ((int*)slots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
+        }
+        __syncthreads();
         // Blocks go out round-robin over the 8 XCDs (block id % 8, and gridDim.x is a multiple of
         // 8), so item i runs on XCD i % 8.  Row block gb's feature blocks all run on XCD gb % 8:
         // the 128-byte rows they share come into that XCD's L2 once.
         const int fbs = (S.fp + FB - 1) / FB;
-        const Slot& last = T.slots[T.n_slots - 1];
+        const Slot& last = slots[T.n_slots - 1];
         const int n_gb = last.blk0 + last.gx;
         const int items = (n_gb + 7) / 8 * 8 * fbs;
         for (int item = blockIdx.x; item < items; item += gridDim.x) {
@@ -550,7 +563,7 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
             const int fb = j % fbs, gb = (j / fbs) * 8 + x;
             if (gb >= n_gb) continue;
             int sj, rb, node, start, count, buf, per;
-            if (spec_target(T, tasks + blockIdx.z, gb, fb == 0, sj, rb, node, start, count, buf, per))
+            if (spec_target(T, tasks + blockIdx.z, slots, gb, fb == 0, sj, rb, node, start, count, buf, per))
                 hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, rb, sj * slot_parts + rb, fb * FB,
                                             false);
             __syncthreads();   // the next item clears the LDS this one read
@@ -1702,16 +1715,21 @@ __global__ __launch_bounds__(256) void k_partition(Task* tasks, Shared S, GrowP 
     if (T.done) return;
     __shared__ Decision sd;
     __shared__ int s_slot, s_chunk;
+    __shared__ Slot pslots[SPEC_W];
     if (mode == 2) {
-        // multi-leaf step: the blocks of slot j are chunk0[j] .. chunk0[j + 1) - 1
+        // multi-leaf step: the blocks of slot j are chunk0[j] .. chunk0[j + 1) - 1; the slot
+        // table comes into LDS in one round trip
+        const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
+        if ((int)threadIdx.x < nw) ((int*)pslots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
+        __syncthreads();
         if (threadIdx.x == 0) {
             int j = -1;
             for (int q = 0; q < T.n_slots; ++q)
-                if ((int)blockIdx.x >= T.slots[q].chunk0) j = q;
+                if ((int)blockIdx.x >= pslots[q].chunk0) j = q;
             s_slot = j;
             sd.leaf = -1;
             if (j >= 0) {
-                const Slot& sl = T.slots[j];
+                const Slot& sl = pslots[j];
                 s_chunk = (int)blockIdx.x - sl.chunk0;
                 sd.leaf = s_chunk * PART_CHUNK < sl.count ? sl.leaf : -1;
                 sd.type = sl.type;
@@ -2005,11 +2023,14 @@ __device__ void spec_pending(const Task& T, int F, const GrowP& P, double* key, 
 }
 
 // Selection of a multi-leaf step (one block per task): reduce the candidates of the nodes the last
-// scan produced (one wave per node), replay best-first over all grown nodes, and split the leaf the
-// replay stops at together with the next-best unsplit leaves of its frontier: at most SPEC_W, at
-// most the splits the leaf budget still allows, and never so many that the remaining splits could
-// not get node ids.  Writes the slots (with each slot's partition blocks and histogram tiles), the
-// children's ids and the node count; sets done when the replay completes the tree.
+// scan produced, replay best-first over all grown nodes (resuming from the last step's stop), and
+// split the leaf the replay stops at together with the next-best unsplit leaves of its frontier:
+// at most SPEC_W, at most the splits the leaf budget still allows, and never so many that the
+// remaining splits could not get node ids.  Writes the slots (with each slot's partition blocks
+// and histogram tiles), the children's ids and the node count; sets done when the replay
+// completes the tree.  Every global load is issued before the first use (one node and its
+// cached candidate per thread, the pending nodes' per-feature results, the stored stop), so the
+// kernel waits for memory about once.
 __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, GrowP P, int width, int min_per,
                                                      int rb_target, int slot_parts) {
     Task& Tw = tasks[blockIdx.x];
@@ -2017,28 +2038,111 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
     if (T.done) return;
     const int F = S.n_features;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    static_assert(MAX_NODES <= 256, "one node per thread");
     __shared__ double key[MAX_NODES];
-    __shared__ int left[MAX_NODES];
-    __shared__ DCand pc[2 * SPEC_W];
+    __shared__ int left[MAX_NODES], nst[MAX_NODES], ncnt[MAX_NODES], nbuf[MAX_NODES], ndep[MAX_NODES], nns[MAX_NODES];
+    __shared__ DCand scand[MAX_NODES];
+    __shared__ double rg[256];
+    __shared__ int rk[256], rt[256], rm[256], rcl[256];
     const int nn = T.n_nodes;
     const int p0 = T.pend_first, np = T.n_pend;
-    for (int k = threadIdx.x; k < nn; k += blockDim.x) {
-        const DNode nd = ldg(T.nodes + k);
-        left[k] = nd.v[N_LEAF] ? -1 : nd.v[N_LEFT];
-        if (k < p0 || k >= p0 + np) key[k] = spec_key(ldg(T.cands + k), P);
+    // ---- loads
+    const int k = threadIdx.x;
+    DNode nd;
+    DCand cd;
+    if (k < nn) {
+        nd = ldg(T.nodes + k);
+        cd = ldg(T.cands + k);
     }
-    spec_pending(T, F, P, key, pc);
-    __syncthreads();
-    if (wave != 0) return;
-    int fnode = -1, fsim = 0, n_front = 1, n_final = 1;
-    double fg;
+    const int pi = threadIdx.x >> 4, sub = threadIdx.x & 15;   // 16 threads per pending node
+    FCand fc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int f = sub + 16 * u;
+        if (pi < np && f < F) fc[u] = ldg(T.fcand + (size_t)(p0 + pi) * F + f);
+        else fc[u].thr = -1;
+    }
+    int fnode = -1, fsim = 0;
     const bool resume = T.rs_valid != 0;
-    if (resume) {
+    if (wave == 0 && resume) {
         fnode = T.rs_node[lane];
         fsim = T.rs_sim[lane];
-        n_front = T.rs_front;
-        n_final = T.rs_final;
     }
+    // ---- the grown nodes into LDS
+    if (k < nn) {
+        left[k] = nd.v[N_LEAF] ? -1 : nd.v[N_LEFT];
+        nst[k] = nd.seg_start;
+        ncnt[k] = nd.seg_count;
+        nbuf[k] = nd.buf;
+        ndep[k] = nd.v[N_DEPTH];
+        nns[k] = nd.v[N_NS];
+        scand[k] = cd;
+        key[k] = spec_key(cd, P);
+    }
+    // ---- the pending nodes' candidates (tree.cpp:160-166): ascending features, first maximum
+    // kept; each thread's features sub, sub + 16, ..., combined in feature order
+    {
+        double best = -INFINITY;
+        int bk = 0x7fffffff, bt = -1, bm = 1, bcl = 0;
+        for (int f0 = 0;; f0 += 128) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int f = f0 + sub + 16 * u;
+                if (fc[u].thr < 0 || fc[u].gain < P.gamma) continue;
+                if (cand_better(fc[u].gain, f, best, bk)) {
+                    best = fc[u].gain;
+                    bk = f;
+                    bt = fc[u].thr;
+                    bm = fc[u].ml;
+                    bcl = fc[u].cl;
+                }
+            }
+            if (f0 + 128 >= F) break;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {   // more than 128 features: the next 128
+                const int f = f0 + 128 + sub + 16 * u;
+                if (pi < np && f < F) fc[u] = ldg(T.fcand + (size_t)(p0 + pi) * F + f);
+                else fc[u].thr = -1;
+            }
+        }
+        rg[threadIdx.x] = best;
+        rk[threadIdx.x] = bk;
+        rt[threadIdx.x] = bt;
+        rm[threadIdx.x] = bm;
+        rcl[threadIdx.x] = bcl;
+        __syncthreads();
+        if (sub == 0 && pi < np) {
+            for (int j = 1; j < 16; ++j) {
+                const int o = threadIdx.x + j;
+                if (cand_better(rg[o], rk[o], best, bk)) {
+                    best = rg[o];
+                    bk = rk[o];
+                    bt = rt[o];
+                    bm = rm[o];
+                    bcl = rcl[o];
+                }
+            }
+            const int node = p0 + pi;
+            DCand c;
+            c.valid = bk != 0x7fffffff && ndep[node] < P.max_depth && nns[node] >= 2 * P.min_leaf;
+            c.gain = best;
+            c.feature = bk == 0x7fffffff ? -1 : bk;
+            c.feature2 = -1;
+            c.type = 0;
+            c.thr = bk == 0x7fffffff ? -1 : bt;
+            c.mgl = bk == 0x7fffffff ? 1 : bm;
+            c.cl = bk == 0x7fffffff ? 0 : bcl;
+            c.pad = 0;
+            scand[node] = c;
+            T.cands[node] = c;
+            key[node] = spec_key(c, P);
+        }
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    // ---- the replay (wave 0)
+    int n_front = resume ? T.rs_front : 1, n_final = resume ? T.rs_final : 1;
+    double fg;
     const int complete = spec_replay(key, left, P, fnode, fsim, fg, n_front, n_final, nullptr, nullptr, resume);
     // keep the stop for the next step's selection
     T.rs_node[lane] = fnode;
@@ -2075,13 +2179,12 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
     int chunks = 0, gx = 0, small = 0;
     Slot sl;
     if (lane < n_chosen) {
-        const DNode d = ldg(T.nodes + chosen);
-        const DCand c = chosen >= p0 && chosen < p0 + np ? pc[chosen - p0] : ldg(T.cands + chosen);
+        const DCand c = scand[chosen];
         sl.leaf = chosen;
         sl.left = nn + 2 * lane;
-        sl.start = d.seg_start;
-        sl.count = d.seg_count;
-        sl.buf = d.buf;
+        sl.start = nst[chosen];
+        sl.count = ncnt[chosen];
+        sl.buf = nbuf[chosen];
         sl.type = c.type;
         sl.feat = c.feature;
         sl.feat2 = c.feature2;
@@ -2107,13 +2210,6 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
     per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
     if (lane < n_chosen) gx = max(1, (small + per - 1) / per);
     const int c_incl = wave_incl_i(chunks), g_incl = wave_incl_i(gx);
-    if (lane < n_chosen) {
-        sl.chunk0 = c_incl - chunks;
-        sl.blk0 = g_incl - gx;
-        sl.gx = gx;
-        sl.per = per;
-        T.slots[lane] = sl;
-    }
     if (nn + 2 * n_chosen > T.spec_cap) {   // cannot happen while spec_cap >= 2 * max_leaves - 1
         if (lane == 0) {
             Tw.err = 3;
@@ -2121,6 +2217,13 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
             Tw.n_slots = 0;
         }
         return;
+    }
+    if (lane < n_chosen) {
+        sl.chunk0 = c_incl - chunks;
+        sl.blk0 = g_incl - gx;
+        sl.gx = gx;
+        sl.per = per;
+        T.slots[lane] = sl;
     }
     if (lane == 0) {
         Tw.n_slots = n_chosen;
@@ -2763,13 +2866,19 @@ void launch_export(hipStream_t st, Task* tasks, int n_tasks, const ExportLayout&
 void init_kernel_attributes() {
     static bool done = false;
     if (done) return;
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 4, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<2, false, 1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, false, 2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_hist_row<1, true, 1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    (void)hipFuncSetAttribute((const void*)k_pair_hist, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // dynamic LDS opt-ins: what each kernel is launched with at most (the multi-leaf child
+    // kernels also hold their slot table in static LDS, and static plus dynamic must fit 160 KB)
+    auto opt_in = [](const void* f, int bytes) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) throw std::runtime_error(std::string("kernel LDS attribute: ") + hipGetErrorString(e));
+    };
+    opt_in((const void*)k_hist_row<1, false, 4, false>, 128 * 1024);
+    opt_in((const void*)k_hist_row<1, false, 2, false>, 128 * 1024);
+    opt_in((const void*)k_hist_row<1, true, 1, false>, 128 * 1024);
+    opt_in((const void*)k_hist_row<2, false, 1, false>, 128 * 1024);
+    opt_in((const void*)k_hist_row<1, false, 2, true>, 64 * 1024);
+    opt_in((const void*)k_hist_row<1, true, 1, true>, 64 * 1024);
+    opt_in((const void*)k_pair_hist, 160 * 1024);
     done = true;
 }
 

@@ -995,8 +995,9 @@ static bool spec_done(tgbm_session* s, int* steps) {
 // the scans of both children of every slot.
 static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
     const int ms = s->launch_rows();
-    // child builds: two resident 512-thread blocks per CU take the step's (row block, feature
-    // block) items; tiles of at least 2,048 rows, sized so the items about fill those blocks
+    // child builds: two resident 512-thread blocks per CU (2 LDS copies per bin, 64 KB each; one
+    // block per CU with 4 copies measured slower) take the step's (row block, feature block)
+    // items; tiles of at least 2,048 rows, sized so the items about fill those blocks
     const int blocks = (2 * s->n_cu + 7) / 8 * 8;   // a multiple of the 8 XCDs
     const int fbs = (s->fp + 15) / 16;
     EventPair e0, e1, e2, e3;
