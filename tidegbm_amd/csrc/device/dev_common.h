@@ -226,6 +226,8 @@ struct Task {
     int skip_end;          // the tree's growth was not finished: the tree-end kernels do nothing
     int rs_valid;          // the replay's frontier below is this tree's (0: replay from the root)
     int rs_front, rs_final; // frontier entries and replay ids used at the last stop
+    int spec_gb;           // histogram row blocks of the current step (all slots)
+    int end_done;          // this tree's end has run (a later end of the same tree does nothing)
     int* rs_node;          // [64] frontier nodes at the last stop (-1: empty entry)
     int* rs_sim;           // [64] their replay ids
 };
@@ -237,6 +239,7 @@ struct Shared {            // the session-wide matrix
     const unsigned char* bins_col;   // [n_features][n_pad]
     const unsigned char* is_cat;     // [n_features] categorical flags, or null
     const signed char* monotone;     // [n_features] monotone constraints (+1 / -1 / 0), or null
+    int* spec_ctr;                   // [8] item counters of multi-leaf child builds, one per XCD
 };
 
 // A run of rows inside one TORB sweep segment (rows are time sorted, segment s = rows

@@ -262,6 +262,8 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
     T.n_tnodes = 0;
     T.skip_end = 0;
     T.rs_valid = 0;
+    T.spec_gb = 0;
+    T.end_done = 0;
     T.cur_leaf = -1;
     T.scan_ticket = 0;
     T.part_lr = 0;
@@ -531,44 +533,99 @@ __device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, u
     }
 }
 
+// Exclusive prefix over v[0 .. n) into pre[0 .. n] (pre[n] = total) by a 512-thread block, two
+// entries per thread (n <= 1024); v is read by each thread for its own entries only.
+template <class GetV>
+__device__ inline void block_prefix_1024(int n, GetV v, int* pre) {
+    __shared__ int wtot[8];
+    const int t = threadIdx.x, wave = t >> 6;
+    const int a = 2 * t < n ? v(2 * t) : 0, b = 2 * t + 1 < n ? v(2 * t + 1) : 0;
+    const int incl = wave_incl_i(a + b);
+    if ((t & 63) == 63) wtot[wave] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += wtot[w];
+    const int ex = base + incl - (a + b);
+    if (2 * t < n) pre[2 * t] = ex;
+    if (2 * t + 1 < n) pre[2 * t + 1] = ex + a;
+    if (t == 0) {
+        int tot = 0;
+        for (int w = 0; w < 8; ++w) tot += wtot[w];
+        pre[n] = tot;
+    }
+    __syncthreads();
+}
+
 // root: 0 the smaller child of T.cur_*, 1 a sampled or whole-window root (grid: row blocks x
-// feature blocks x tasks); 2 the slots of a multi-leaf step, whose (row block, feature block)
-// items a fixed grid of blocks takes in turn (item = global row block * feature blocks +
-// feature block, so consecutive items read the same rows).
+// feature blocks x tasks); 2 the slots of a multi-leaf step of n_tasks tasks: a fixed grid of
+// blocks takes the (task, row block, feature block) items of all tasks.  Each task's items are
+// numbered row block * feature blocks + feature block within groups of 8 row blocks, and each
+// task's first item is a multiple of 8, so row block gb's feature blocks all have item % 8 ==
+// gb % 8; blocks go out round-robin over the 8 XCDs, and the blocks of XCD x take exactly the
+// items with item % 8 == x: the 128-byte rows a row block's feature blocks share come into that
+// XCD's L2 once.
 template <int LPR, bool HV, int KC, bool SPEC>
-__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows, int slot_parts) {
+__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows, int slot_parts,
+                                                  int n_tasks) {
     constexpr int FB = 16 * LPR;
     extern __shared__ unsigned long long lds[];
-    const Task T = tasks[blockIdx.z];
     if (SPEC) {   // root == 2
-        if (T.done || T.n_slots <= 0) return;
-        // the step's slot table, read once into LDS (one memory round trip for all items)
-        __shared__ Slot slots[SPEC_W];
-        {
-            constexpr int WORDS = (int)(sizeof(Slot) / 4) * SPEC_W;
-            const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
-            if ((int)threadIdx.x < nw && (int)threadIdx.x < WORDS)
-                ((int*)slots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
-        }
-        __syncthreads();
-        // Blocks go out round-robin over the 8 XCDs (block id % 8, and gridDim.x is a multiple of
-        // 8), so item i runs on XCD i % 8.  Row block gb's feature blocks all run on XCD gb % 8:
-        // the 128-byte rows they share come into that XCD's L2 once.
         const int fbs = (S.fp + FB - 1) / FB;
-        const Slot& last = slots[T.n_slots - 1];
-        const int n_gb = last.blk0 + last.gx;
-        const int items = (n_gb + 7) / 8 * 8 * fbs;
-        for (int item = blockIdx.x; item < items; item += gridDim.x) {
-            const int x = item % 8, j = item / 8;
+        __shared__ int ipre[1025];
+        if (n_tasks == 1) {   // one task: no prefix to form
+            if (threadIdx.x == 0) {
+                const Task& T0 = tasks[0];
+                ipre[0] = 0;
+                ipre[1] = T0.done ? 0 : (T0.spec_gb + 7) / 8 * 8 * fbs;
+            }
+            __syncthreads();
+        } else {
+            block_prefix_1024(n_tasks, [&](int i) {
+                const Task& Ti = tasks[i];
+                return Ti.done ? 0 : (Ti.spec_gb + 7) / 8 * 8 * fbs;
+            }, ipre);
+        }
+        const int items = ipre[n_tasks];
+        __shared__ Slot slots[SPEC_W];
+        __shared__ int s_item;
+        int cur = -1;
+        // Items i with i % 8 == x go to the blocks of XCD x (block id % 8): block b takes item b
+        // first; items past the grid go out in turn from XCD x's counter (when there are any), so
+        // large and small items balance within each XCD, and every block stops once its counter
+        // passes the last item.
+        const int xq = blockIdx.x % 8;
+        for (int item = blockIdx.x; item < items;) {
+            int lo = 0, hi = n_tasks - 1;   // the task: the last with ipre[t] <= item
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (ipre[mid] <= item) lo = mid;
+                else hi = mid - 1;
+            }
+            const int tk = lo;
+            const Task T = tasks[tk];
+            if (tk != cur) {   // the task's slot table, into LDS in one round trip
+                const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
+                if ((int)threadIdx.x < nw) ((int*)slots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
+                __syncthreads();
+                cur = tk;
+            }
+            const int li = item - ipre[tk];
+            const int x = li % 8, j = li / 8;
             const int fb = j % fbs, gb = (j / fbs) * 8 + x;
-            if (gb >= n_gb) continue;
             int sj, rb, node, start, count, buf, per;
-            if (spec_target(T, tasks + blockIdx.z, slots, gb, fb == 0, sj, rb, node, start, count, buf, per))
+            if (gb < T.spec_gb &&
+                spec_target(T, tasks + tk, slots, gb, fb == 0, sj, rb, node, start, count, buf, per))
                 hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, rb, sj * slot_parts + rb, fb * FB,
                                             false);
-            __syncthreads();   // the next item clears the LDS this one read
+            __syncthreads();   // the next item clears the LDS this one read (and may reload the slots)
+            if (items <= (int)gridDim.x) break;
+            if (threadIdx.x == 0) s_item = 8 * ((int)gridDim.x / 8 + atomicAdd(&S.spec_ctr[xq], 1)) + xq;
+            __syncthreads();
+            item = s_item;
+            __syncthreads();
         }
     } else {
+    const Task T = tasks[blockIdx.z];
     int node, start, count, buf;
     const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
     if (!hist_target(T, tasks + blockIdx.z, root != 0, leader, node, start, count, buf)) return;
@@ -880,10 +937,10 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     const int kc = (!hconst || lpr > 1) ? 1 : (n_tasks == 1 ? 4 : 2);
     const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
     const int rt = root ? 1 : 0;
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
-    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
-    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
-    else TIMED_LAUNCH((k_hist_row<1, false, 2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0);
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
+    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
+    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
 }
 
 // Child builds of a multi-leaf step: every slot's smaller child in one launch of `blocks` blocks
@@ -892,11 +949,11 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
 void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
                       int slot_parts, hipEvent_t e0, hipEvent_t e1) {
     constexpr int FB = 16;
-    dim3 grid(std::max(1, blocks), 1, n_tasks);
+    dim3 grid(std::max(1, blocks), 1, 1);   // the tasks' items share the grid
     const int kc = hconst ? 2 : 1;
     const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts);
-    else TIMED_LAUNCH((k_hist_row<1, false, 2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts);
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts, n_tasks);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts, n_tasks);
 }
 
 // =================================================================== split scan
@@ -1482,18 +1539,17 @@ __device__ void scan_cat(bool mine, const long long* qg, const int* c, const lon
 // one feature: its four waves sum interleaved quarters of the partials, then waves 0 and 2 scan
 // the smaller child and waves 1 and 3 the larger (half of the candidates each); wave 0 / 1 stores
 // the child's histogram, kept for later use as a parent, and its best split of the feature.
-__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int slot_parts) {
-    // mode 2 (multi-leaf steps): grid z = task * SPEC_W + slot
-    const int task = mode == 2 ? (int)blockIdx.z / SPEC_W : (int)blockIdx.z;
-    const int sj = mode == 2 ? (int)blockIdx.z % SPEC_W : 0;
+// One scan of feature f: task `task`'s root (mode 0), the children of its current split (mode
+// 1), or the children of slot sj of its multi-leaf step (mode 2).  Every thread of the block
+// calls it; it synchronizes the block.
+__device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const GrowP& P, int mode, int slot_parts,
+                                         int task, int sj, int f) {
     const Task T = tasks[task];
     Task& Tw = tasks[task];
     if (T.done) return;
-    if (mode == 2 && sj >= T.n_slots) return;
     const int F = S.n_features;
     const size_t per = (size_t)F * NBIN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int f = blockIdx.x;
     // the build this scan reads: its parent, children and partials
     int parent = T.cur_leaf, small_node = T.small_node, big_node = T.big_node;
     Parts pp;
@@ -1666,10 +1722,67 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
     }
 }
 
+// The split scan.  Modes 0 and 1: grid (features, 1, tasks).  Mode 2 (multi-leaf steps): grid
+// (features, 1, z) whose z-blocks take the active (task, slot) pairs of all tasks in turn.
+__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int slot_parts, int n_tasks) {
+    if (mode != 2) {
+        scan_one(tasks, S, P, mode, slot_parts, blockIdx.z, 0, blockIdx.x);
+        return;
+    }
+    if (n_tasks == 1) {   // one task: its slots in turn, no prefix to form
+        const int ns = tasks[0].done ? 0 : tasks[0].n_slots;
+        for (int q = blockIdx.z; q < ns; q += gridDim.z) {
+            scan_one(tasks, S, P, 2, slot_parts, 0, q, blockIdx.x);
+            __syncthreads();
+        }
+        return;
+    }
+    __shared__ int pre[1025];
+    {   // exclusive prefix of the tasks' slot counts (256 threads, four entries each; <= 1024 tasks)
+        __shared__ int wtot[4];
+        const int tt = threadIdx.x, wave = tt >> 6;
+        int v[4], s = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * tt + u;
+            v[u] = i < n_tasks && !tasks[i].done ? tasks[i].n_slots : 0;
+            s += v[u];
+        }
+        const int incl = wave_incl_i(s);
+        if ((tt & 63) == 63) wtot[wave] = incl;
+        __syncthreads();
+        int base = 0;
+        for (int w = 0; w < wave; ++w) base += wtot[w];
+        int ex = base + incl - s;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * tt + u;
+            if (i < n_tasks) pre[i] = ex;
+            ex += v[u];
+        }
+        if (tt == 0) pre[n_tasks] = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+        __syncthreads();
+    }
+    const int pairs = pre[n_tasks];
+    for (int q = blockIdx.z; q < pairs; q += gridDim.z) {
+        int lo = 0, hi = n_tasks - 1;   // the task: the last with pre[t] <= q
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        scan_one(tasks, S, P, 2, slot_parts, lo, q - pre[lo], blockIdx.x);
+        __syncthreads();   // the next pair reuses the block's LDS
+    }
+}
+
 void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
                  int slot_parts) {
-    dim3 grid(S.n_features, 1, n_tasks * (mode == 2 ? SPEC_W : 1));
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, slot_parts);
+    // mode 2: as many z-blocks as one task's slots, at least the tasks (one pair each on average
+    // at four slots per task), at most 64
+    const int z = mode == 2 ? std::min(64, std::max(SPEC_W, n_tasks)) : n_tasks;
+    dim3 grid(S.n_features, 1, z);
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, slot_parts, n_tasks);
 }
 
 // =================================================================== partition
@@ -2035,6 +2148,9 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
                                                      int rb_target, int slot_parts) {
     Task& Tw = tasks[blockIdx.x];
     const Task T = tasks[blockIdx.x];
+    // the step's child builds hand out their items from these counters (the first task's block
+    // resets them, done or not)
+    if (blockIdx.x == 0 && threadIdx.x < 8 && S.spec_ctr) S.spec_ctr[threadIdx.x] = 0;
     if (T.done) return;
     const int F = S.n_features;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2156,6 +2272,7 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
         if (lane == 0) {
             Tw.done = 1;
             Tw.n_slots = 0;
+            Tw.spec_gb = 0;
         }
         return;
     }
@@ -2215,6 +2332,7 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
             Tw.err = 3;
             Tw.done = 1;
             Tw.n_slots = 0;
+            Tw.spec_gb = 0;
         }
         return;
     }
@@ -2225,12 +2343,14 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
         sl.per = per;
         T.slots[lane] = sl;
     }
+    const int n_gb = __builtin_amdgcn_readlane(g_incl, 63);
     if (lane == 0) {
         Tw.n_slots = n_chosen;
         Tw.n_nodes = nn + 2 * n_chosen;
         Tw.pend_first = nn;
         Tw.n_pend = 2 * n_chosen;
         Tw.spec_steps = T.spec_steps + 1;
+        Tw.spec_gb = n_gb;
     }
 }
 
@@ -2244,6 +2364,10 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
 __global__ __launch_bounds__(256) void k_spec_finalize(Task* tasks, Shared S, GrowP P) {
     Task& Tw = tasks[blockIdx.x];
     const Task T = tasks[blockIdx.x];
+    if (T.end_done) {   // a later end of a tree that has ended (another task needed more steps)
+        if (threadIdx.x == 0) Tw.skip_end = 1;
+        return;
+    }
     __shared__ DNode sn[MAX_NODES];
     __shared__ double key[MAX_NODES];
     __shared__ int left[MAX_NODES], fid[MAX_NODES], par[MAX_NODES];
@@ -2282,6 +2406,7 @@ __global__ __launch_bounds__(256) void k_spec_finalize(Task* tasks, Shared S, Gr
     __syncthreads();
     if (threadIdx.x == 0) {
         Tw.skip_end = s_ok ? 0 : 1;
+        if (s_ok) Tw.end_done = 1;
         if (T.host_flag) {   // the host waits for this: {1 finished | 2 more steps needed, steps}
             T.host_flag[1] = T.spec_steps;
             __threadfence_system();
@@ -2318,6 +2443,30 @@ __global__ __launch_bounds__(256) void k_spec_finalize(Task* tasks, Shared S, Gr
         Tw.n_tnodes = nn;
         Tw.n_nodes = s_final;
     }
+}
+
+// Growth state of the active tasks for the host: out[0] = 1 when every task's growth is done,
+// out[1] = the most steps a task took (one block).
+__global__ __launch_bounds__(256) void k_spec_status(const Task* tasks, int n_tasks, int* out) {
+    __shared__ int s_all, s_max;
+    if (threadIdx.x == 0) {
+        s_all = 1;
+        s_max = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_tasks; i += blockDim.x) {
+        if (!tasks[i].done) atomicAnd(&s_all, 0);
+        atomicMax(&s_max, tasks[i].spec_steps);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[1] = s_max;
+        out[0] = s_all;
+    }
+}
+
+void launch_spec_status(hipStream_t st, const Task* tasks, int n_tasks, int* out) {
+    hipLaunchKernelGGL(k_spec_status, dim3(1), dim3(256), 0, st, tasks, n_tasks, out);
 }
 
 void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int width,

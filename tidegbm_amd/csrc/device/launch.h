@@ -41,6 +41,8 @@ void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_task
 void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
                       int slot_parts, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_spec_finalize(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
+// out (device-visible) = {every active task's growth done, the most steps a task took}.
+void launch_spec_status(hipStream_t st, const Task* tasks, int n_tasks, int* out);
 // One block per task selects the next split into T.cur_* (for launches over many tasks).
 void launch_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
 // mode 0: the partition launch selects the split itself; mode 1: the decision is in T.cur_*;

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <type_traits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -128,9 +129,11 @@ struct tgbm_session {
     int spec_width = 0;
     int spec_cap = 0;
     int slot_parts = 0;
+    int max_task_rows = 1;
     int spec_k = 8;
     bool spec_tree = false;
-    int* flag_h = nullptr;      // pinned, mapped: the selection writes {done, steps} when a tree ends
+    int* flag_h = nullptr;      // pinned, mapped: per task {status, steps} from the final numbering,
+                                // then {every task done, most steps} from the status kernel
     int* flag_d = nullptr;      // its device address
     bool spec_launched = false;     // the current tree's captured loop was launched without a wait
     bool spec_wait = false;         // a multi-leaf tree's queued end has not been waited for
@@ -179,6 +182,14 @@ struct tgbm_session {
         T* p = dalloc<T>(n);
         allocs.push_back(p);
         return p;
+    }
+    void release(void* p) {
+        for (auto it = allocs.begin(); it != allocs.end(); ++it)
+            if (*it == p) {
+                allocs.erase(it);
+                HIP_OK(hipFree(p));
+                return;
+            }
     }
     ~tgbm_session() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -408,16 +419,8 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         // Partials per launch: launches of the split loop are sized for the largest sample rounded
         // up to a power of two (< 2x the rows), and no block may fold more than MAX_BLOCK_ROWS rows.
         s->S.part_cap = std::max(64, (int)((2LL * max_task_rows + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
-        if (n_tasks == 1 && max_leaves <= 64) {
-            // multi-leaf growth: room for the grown nodes (at most twice the final tree's) and for
-            // SPEC_W slots of partials, each as many as one smaller child's build may write
-            s->spec_cap = std::min(MAX_NODES, 2 * s->max_nodes);
-            s->slot_parts = std::max(64, (int)(((long long)max_task_rows / 2 + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
-            s->S.part_cap = std::max(s->S.part_cap, SPEC_W * s->slot_parts);
-            HIP_OK(hipHostMalloc((void**)&s->flag_h, 64, hipHostMallocMapped | hipHostMallocCoherent));
-            HIP_OK(hipHostGetDevicePointer((void**)&s->flag_d, s->flag_h, 0));
-        }
-        const int node_cap = std::max(s->max_nodes, s->spec_cap);
+        s->max_task_rows = max_task_rows;
+        const int node_cap = s->max_nodes;
         s->tasks_h.assign(n_tasks, Task{});
         s->th.assign(n_tasks, TaskHost{});
         const size_t per_node = (size_t)n_features * NBIN;
@@ -461,7 +464,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.rs_node = s->alloc<int>(64);
             T.rs_sim = s->alloc<int>(64);
             T.spec_cap = node_cap;
-            T.host_flag = s->flag_d;
+            T.host_flag = nullptr;
             T.energy = s->alloc<long long>(MAX_NODES);
             T.part = s->alloc<unsigned long long>((size_t)s->S.part_cap * per_node);
             T.part_c = s->alloc<unsigned int>((size_t)s->S.part_cap * per_node);
@@ -530,6 +533,47 @@ int tgbm_session_set_spec(tgbm_session* s, int width) {
     return dev_guard([&] {
         spec_resolve(s);
         if (width < 0 || width > SPEC_W) throw std::runtime_error("multi-leaf width must be in [0, 8]");
+        if (width > 1 && s->spec_cap == 0 && s->max_leaves <= 64 && s->T <= 1024) {
+            // Room for the grown nodes (at most twice the final tree's) and for SPEC_W slots of
+            // partials per task, each as many as one smaller child's build may write (one task:
+            // 64 tiles of a step; several: fewer, larger tiles each).
+            HIP_OK(hipStreamSynchronize(s->stream));
+            const int cap = std::min(MAX_NODES, 2 * s->max_nodes);
+            const int sp = std::max(s->T == 1 ? 64 : 16,
+                                    (int)(((long long)s->max_task_rows / 2 + MAX_BLOCK_ROWS - 1) / MAX_BLOCK_ROWS) + 1);
+            const int pc = std::max(s->S.part_cap, SPEC_W * sp);
+            const size_t per_node = (size_t)s->F * NBIN;
+            int* fh = nullptr;
+            HIP_OK(hipHostMalloc((void**)&fh, sizeof(int) * (2 * s->T + 2), hipHostMallocMapped | hipHostMallocCoherent));
+            s->flag_h = fh;
+            HIP_OK(hipHostGetDevicePointer((void**)&s->flag_d, s->flag_h, 0));
+            for (int t = 0; t < s->T; ++t) {
+                Task& T = s->tasks_h[t];
+                auto swap_buf = [&](auto*& field, size_t n, size_t off) {
+                    using E = std::remove_pointer_t<std::remove_reference_t<decltype(field)>>;
+                    if (!field) return;
+                    s->release((void*)field);
+                    field = s->alloc<std::remove_const_t<E>>(n);
+                    s->set_field_ptr(t, off, field);
+                };
+                swap_buf(T.hg, per_node * cap, offsetof(Task, hg));
+                swap_buf(T.hc, per_node * cap, offsetof(Task, hc));
+                swap_buf(T.hh, per_node * cap, offsetof(Task, hh));
+                swap_buf(T.part, per_node * pc, offsetof(Task, part));
+                swap_buf(T.part_c, per_node * pc, offsetof(Task, part_c));
+                swap_buf(T.part_h, per_node * pc, offsetof(Task, part_h));
+                T.spec_cap = cap;
+                s->set_field_int(t, offsetof(Task, spec_cap), cap);
+                T.host_flag = s->flag_d + 2 * t;
+                s->set_field_ptr(t, offsetof(Task, host_flag), T.host_flag);
+            }
+            s->spec_cap = cap;
+            s->slot_parts = sp;
+            s->S.part_cap = pc;
+            s->S.spec_ctr = s->alloc<int>(8);
+            HIP_OK(hipMemsetAsync(s->S.spec_ctr, 0, 8 * sizeof(int), s->stream));
+            HIP_OK(hipStreamSynchronize(s->stream));
+        }
         s->spec_width = s->spec_cap > 0 ? width : 0;
     });
 }
@@ -980,15 +1024,16 @@ static void split_step(tgbm_session* s, const GrowP& P, int mode, const tgbm_gro
     HIP_OK(hipGetLastError());
 }
 
-// The current tree's {done, steps} (one task), after the stream drains.
+// Whether every active task's growth is done, and the most steps a task took, after the stream
+// drains (the status kernel writes them into the mapped words after the tasks' flags).
 static bool spec_done(tgbm_session* s, int* steps) {
-    const char* base = reinterpret_cast<const char*>(s->tasks_d + s->a0);
-    HIP_OK(hipMemcpyAsync(s->flag_h, base + offsetof(Task, done), sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIP_OK(hipMemcpyAsync(s->flag_h + 1, base + offsetof(Task, spec_steps), sizeof(int), hipMemcpyDeviceToHost,
-                          s->stream));
+    int* out = s->flag_d + 2 * s->T;
+    launch_spec_status(s->stream, s->td(), s->an, out);
+    HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s->stream));
-    if (steps) *steps = s->flag_h[1];
-    return s->flag_h[0] != 0;
+    volatile int* h = s->flag_h + 2 * s->T;
+    if (steps) *steps = h[1];
+    return h[0] != 0;
 }
 
 // One multi-leaf step: the selection, the slots' partitions, their smaller children's builds and
@@ -1002,7 +1047,7 @@ static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p
     const int fbs = (s->fp + 15) / 16;
     EventPair e0, e1, e2, e3;
     if (s->timing) s->tic(6, 0.0, e0);   // the step's selection (timer 6)
-    launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 2048, std::max(1, blocks / fbs),
+    launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 2048, std::max(1, blocks / (fbs * s->an)),
                        s->slot_parts);
     if (s->timing) s->toc(e0);
     if (s->timing) s->tic(1, 0.0, e1);
@@ -1020,18 +1065,22 @@ static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p
 
 // Rows of the step just run (timing mode): the smaller children built and the leaves partitioned.
 static void spec_log_rows(tgbm_session* s) {
-    Task T;
-    HIP_OK(hipMemcpy(&T, s->tasks_d + s->a0, sizeof(Task), hipMemcpyDeviceToHost));
-    Slot sl[SPEC_W];
-    HIP_OK(hipMemcpy(sl, T.slots, sizeof(sl), hipMemcpyDeviceToHost));
-    double small = 0, rows = 0;
-    for (int j = 0; j < (T.done ? 0 : T.n_slots); ++j) {
-        small += sl[j].small_count;
-        rows += sl[j].count;
+    double small = 0, rows = 0, splits = 0;
+    for (int k = s->a0; k < s->a0 + s->an; ++k) {
+        Task T;
+        HIP_OK(hipMemcpy(&T, s->tasks_d + k, sizeof(Task), hipMemcpyDeviceToHost));
+        if (T.done) continue;
+        Slot sl[SPEC_W];
+        HIP_OK(hipMemcpy(sl, T.slots, sizeof(sl), hipMemcpyDeviceToHost));
+        for (int j = 0; j < T.n_slots; ++j) {
+            small += sl[j].small_count;
+            rows += sl[j].count;
+        }
+        splits += T.n_slots;
     }
     s->log_rows[5].push_back(small);
     s->log_rows[1].push_back(rows);
-    s->log_rows[6].push_back(T.done ? 0 : T.n_slots);   // splits the selection made
+    s->log_rows[6].push_back(splits);   // splits the selections made
 }
 
 // Multi-leaf growth of the current tree.  The captured loop holds spec_k steps (the most the last
@@ -1076,7 +1125,7 @@ static void grow_spec(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p
             it = s->graphs.emplace(key, c).first;
         }
         it->second.last_use = ++s->graph_clock;
-        s->flag_h[0] = 0;
+        for (int k = s->a0; k < s->a0 + s->an; ++k) s->flag_h[2 * k] = 0;
         HIP_OK(hipGraphLaunch(it->second.ge, s->stream));
         // the tree end is queued behind the loop and waited for later (spec_resolve)
         s->spec_launched = true;
@@ -1094,7 +1143,7 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
     return dev_guard([&] {
         spec_resolve(s);
         const GrowP P = s->growp(p);
-        s->spec_tree = s->spec_width > 0 && s->an == 1 && p->max_leaves <= 64;
+        s->spec_tree = s->spec_width > 0 && p->max_leaves <= 64;
         if (s->spec_tree) {
             grow_spec(s, P, p);
             return;
@@ -1257,30 +1306,45 @@ static void spec_resolve(tgbm_session* s) {
     s->spec_wait = false;
     const tgbm_session::EndArgs E = s->spec_end;
     volatile int* flag = s->flag_h;
-    // Poll the flag alone; ask the runtime (which takes its locks) only every 200 us, so the
+    const int t0 = s->a0, t1 = s->a0 + s->an;
+    auto reported = [&] {
+        for (int k = t0; k < t1; ++k)
+            if (!flag[2 * k]) return false;
+        return true;
+    };
+    auto all_finished = [&](int& steps) {
+        bool ok = true;
+        steps = 0;
+        for (int k = t0; k < t1; ++k) {
+            ok = ok && flag[2 * k] == 1;
+            steps = std::max(steps, (int)flag[2 * k + 1]);
+        }
+        return ok;
+    };
+    // Poll the flags alone; ask the runtime (which takes its locks) only every 200 us, so the
     // wait does not slow the runtime's own work on the stream.
     auto last = std::chrono::steady_clock::now();
-    while (!flag[0]) {
+    while (!reported()) {
         const auto now = std::chrono::steady_clock::now();
         if (now - last < std::chrono::microseconds(200)) continue;
         last = now;
         const hipError_t q = hipEventQuery(s->export_ev[E.slot]);
-        if (q == hipSuccess && !flag[0]) throw std::runtime_error("internal: the tree end reported nothing");
+        if (q == hipSuccess && !reported()) throw std::runtime_error("internal: the tree end reported nothing");
         if (q != hipSuccess && q != hipErrorNotReady) HIP_OK(q);
     }
-    bool done = flag[0] == 1;
-    int steps = flag[1];
+    int steps = 0;
+    bool done = all_finished(steps);
     const int limit = std::max(1, E.p.max_leaves - 1);
     if (!done) {
+        // tasks whose trees have ended skip the steps and the second end (done, end_done)
         for (int k = s->spec_k_used; k < limit && !done; ++k) {
             spec_step(s, E.P, &E.p);
-            done = spec_done(s, &steps);
+            done = spec_done(s, nullptr);
         }
         if (!done) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
         enqueue_end(s, E);
         HIP_OK(hipStreamSynchronize(s->stream));
-        if (flag[0] != 1) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
-        steps = flag[1];
+        if (!all_finished(steps)) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
     }
     // the next loop's steps: the most any of the last 32 trees needed (its splitting steps plus
     // the selection that finds it finished), so a tree rarely needs the step-by-step path

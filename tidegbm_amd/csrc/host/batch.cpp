@@ -79,6 +79,8 @@ BatchStats fit_batch(const Config& proto, const double* X, int n, int F, const l
         if ((int)proto.monotone.size() != F) throw Error("monotone constraints need one entry per feature");
         dev_check(tgbm_session_set_monotone(so.s, proto.monotone.data()));
     }
+    // multi-leaf steps for every fold, batched or one after another
+    dev_check(tgbm_session_set_spec(so.s, proto.spec_width > 1 ? proto.spec_width : 0));
     bin_on_device(so.s, X, edges);
     // ---- per-fold preparation, exactly as a plain fit of that fold; the folds' time layouts are
     // computed in parallel waves of one fold per host thread (the first failing fold's error is
