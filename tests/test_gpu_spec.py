@@ -5,8 +5,9 @@ leaf best-first takes next plus the next-best unsplit leaves of the frontier, an
 best-first replay over the splits made (reference grow_tree, tree.cpp:343-463, one split at a
 time).  Bar: every tree identical field for field (node ints, gains, values, covers, TORB
 prefixes) to spec_width = 1, and predictions bit-identical.  Cases: the plain panel, weighted rows
-(h varies), GOSS samples, a depth limit that binds and a split-gain floor.  Width 2 with 31 leaves
-needs more steps than the first captured loop holds, so the step-by-step continuation runs too.
+(h varies), GOSS samples, a depth limit that binds, a split-gain floor, and features with most rows
+in one bin.  Width 2 with 31 leaves needs more steps than the first captured loop holds, so the
+step-by-step continuation runs too.
 The trees of spec_width = 1 are themselves checked against the oracle and the reference core by
 the other GPU tests.
 """
@@ -37,6 +38,7 @@ def _fit(p, width, w=None, **kw):
 
 CASES = {
     "plain": dict(goss=0),
+    "skewed": dict(goss=0, skewed=True),
     "weighted": dict(goss=0, weighted=True),
     "goss": dict(goss=1),
     "depth4": dict(goss=0, max_depth=4),
@@ -48,7 +50,14 @@ CASES = {
 def test_multi_leaf_steps_equal_one_split_per_step(case):
     kw = dict(CASES[case])
     weighted = kw.pop("weighted", False)
+    skewed = kw.pop("skewed", False)
     p = synth.panel_large(3, n_dates=750)   # 300,000 rows x 128 features
+    if skewed:
+        # 8 features with 85% of rows on one value: every child build's rounds put more rows
+        # into one bin than a packed LDS word holds unless the rows alternate between copies
+        rng = np.random.default_rng(11)
+        for f in range(0, 128, 16):
+            p.X[rng.random(p.n) < 0.85, f] = 0.0
     w = np.random.default_rng(5).uniform(0.5, 2.0, p.n) if weighted else None
     _say(f"[case] multi-leaf {case}: {p.n} x {p.n_features}, {kw}")
     base = _fit(p, 1, w, **kw)

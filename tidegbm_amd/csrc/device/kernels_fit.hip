@@ -410,31 +410,37 @@ __device__ inline void hist_store_partial(const Task& T, const unsigned long lon
 
 // Row kernel (partitioned rows, or a sampled root): each lane loads 16 consecutive feature bins
 // of one row (128-bit load); LPR lanes cover the block's FB = 16*LPR features of a row.  The
-// block walks its rows in steps of 8 rows per lane: the index loads of the 8 rows, then their bin
-// and gradient loads, then the LDS atomics of the packed operands, so one step costs two memory
-// round trips.  The block's row range is sized from the node's actual row count, so small nodes
-// still spread over many blocks.
+// block of NT threads walks its rows in steps of RPL rows per lane (8 at 512 threads, 4 at 1024):
+// the index loads of those rows, then their bin and gradient loads, then the LDS atomics of the
+// packed operands, so one step costs two memory round trips.  The block's row range is sized from
+// the node's actual row count, so small nodes still spread over many blocks.
+//
+// The gathers bound this kernel: a wave's 128-bit loads touch 64 / LPR rows' 128-byte lines
+// per instruction, and every feature block of a row block gathers the same rows again, so the
+// cost per row falls as LPR grows (1M gathered rows of 128 features, loads alone: 170 us at
+// LPR = 1, 86 us at 2, 44 us at 4, 29 us at 8).  LPR = 4 holds 64 features x 256 bins of one
+// copy in 128 KB of LDS (one 1024-thread block per CU).
 //
 // The histogram has KC copies per (feature, bin) in consecutive words; thread t adds into copy
-// t % KC, which spreads a wave's atomics over the LDS banks.  A packed word may take at most
-// WORD_ROWS rows.  A block of at most that many rows stores the sum of its copies as its partial.
-// A larger block (the "wide" form, the same choice in every block of a launch) adds a step's rows
-// in rounds of KC * WORD_ROWS rows (WORD_ROWS per copy); after each round every thread folds the
-// copies of its bins into int64 g sums and counts held in registers and clears them, and the
-// partial is stored from those registers (g sums in part, counts in part_c).  The h sums (h
-// varying) never come near a word's limit and stay in LDS.
+// (t / LPR) % KC, so the rows of a round alternate between the copies of every feature and a
+// wave's atomics spread over the LDS banks.  A packed word may take at most WORD_ROWS rows.  A
+// block of at most that many rows stores the sum of its copies as its partial.  A larger block
+// (the "wide" form, the same choice in every block of a launch) adds a step's rows in rounds of
+// KC * WORD_ROWS rows (WORD_ROWS per copy); after each round every thread folds the copies of its
+// bins into int64 g sums and counts held in registers and clears them (one LDS exchange per word
+// reads and clears it), and the partial is stored from those registers (g sums in part, counts in
+// part_c).  The h sums (h varying) never come near a word's limit and stay in LDS.
 // LDS: KC*FB*NBIN words for lA, then (HV) FB*NBIN words for lB.
 //
 // One block's rows [rb * per, (rb + 1) * per) of a node's range (start, count in index buffer
 // buf, or rows start + k directly when contig) for features f0 .. f0 + 16 * LPR - 1, stored as
 // partial pidx.  Every thread of the block calls it (it synchronizes the block).
-template <int LPR, bool HV, int KC>
+template <int LPR, bool HV, int KC, int NT>
 __device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, unsigned long long* lds, int start,
                                                int count, int buf, int per, int rb, int pidx, int f0, bool contig) {
     constexpr int FB = 16 * LPR;
     constexpr int W = FB * NBIN;
-    constexpr int NT = 512;                       // threads per block
-    constexpr int RPL = 8;                        // rows per lane per step
+    constexpr int RPL = NT >= 1024 ? 4 : 8;       // rows per lane per step
     constexpr int STRIDE = NT / LPR;              // rows per lane-row of a step
     constexpr int UPR = KC * WORD_ROWS / STRIDE;  // lane-rows per round (WORD_ROWS rows per copy)
     constexpr int OWN = W / NT;                   // bins each thread folds in the wide form
@@ -447,19 +453,23 @@ __device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, u
     const int r1 = min(count, r0 + per);
     for (int j = threadIdx.x; j < W * (KC + (HV ? 1 : 0)); j += NT) lds[j] = 0;
     const int sub = threadIdx.x % LPR;
-    const int cp = threadIdx.x % KC;
+    // copies alternate by row (lane-row), so each copy takes half of a round's rows of every feature
+    const int cp = ((int)threadIdx.x / LPR) % KC;
     const int fcol = f0 + sub * 16;
     const bool active = fcol < S.fp;
     const auto rows = gbl((const int*)(buf ? T.idx_tmp : T.idx) + start);
     const auto pk = gbl((const unsigned long long*)T.pk);
     const auto qhp = gbl((const unsigned long long*)T.qh);
+    // folded sums of the bins this thread owns: int64 g sums, and counts two to a register (a
+    // block folds at most MAX_BLOCK_ROWS < 2^16 rows, so a count never carries into its neighbour)
+    static_assert(MAX_BLOCK_ROWS < 65536, "16-bit folded counts");
+    constexpr int ACW = (OWN + 1) / 2;
     long long aq[OWN];
-    unsigned int ac[OWN];
+    unsigned int ac[ACW];
 #pragma unroll
-    for (int o = 0; o < OWN; ++o) {
-        aq[o] = 0;
-        ac[o] = 0;
-    }
+    for (int o = 0; o < OWN; ++o) aq[o] = 0;
+#pragma unroll
+    for (int o = 0; o < ACW; ++o) ac[o] = 0;
     __syncthreads();
     // every thread takes the same number of steps (the rounds synchronize the block)
     for (int s0 = r0; s0 < r1; s0 += STRIDE * RPL) {
@@ -502,10 +512,9 @@ __device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, u
 #pragma unroll
                     for (int c = 0; c < KC; ++c) {
                         const int cc = (c + (int)threadIdx.x / 8) % KC;
-                        const unsigned long long v = lA[j + cc];
-                        lA[j + cc] = 0;
+                        const unsigned long long v = atomicExch(&lA[j + cc], 0ull);
                         aq[o] += unpack_q(v);
-                        ac[o] += unpack_c(v);
+                        ac[o >> 1] += unpack_c(v) << (16 * (o & 1));
                     }
                 }
                 __syncthreads();
@@ -527,104 +536,52 @@ __device__ __forceinline__ void hist_row_block(const Task& T, const Shared& S, u
         const int j = threadIdx.x + o * NT;
         if (j < words) {
             part[base + j] = (unsigned long long)aq[o];
-            part_c[base + j] = ac[o];
+            part_c[base + j] = (ac[o >> 1] >> (16 * (o & 1))) & 0xffffu;
             if (HV) part_h[base + j] = lB[j];
         }
     }
 }
 
-// Exclusive prefix over v[0 .. n) into pre[0 .. n] (pre[n] = total) by a 512-thread block, two
-// entries per thread (n <= 1024); v is read by each thread for its own entries only.
-template <class GetV>
+// Exclusive prefix over v[0 .. n) into pre[0 .. n] (pre[n] = total) by a block of NT threads
+// (512 or 1024), 1024 / NT consecutive entries per thread (n <= 1024); v is read by each thread
+// for its own entries only.
+template <int NT, class GetV>
 __device__ inline void block_prefix_1024(int n, GetV v, int* pre) {
-    __shared__ int wtot[8];
+    constexpr int PER = 1024 / NT, NW = NT / 64;
+    static_assert(PER * NT == 1024, "entries per thread");
+    __shared__ int wtot[NW];
     const int t = threadIdx.x, wave = t >> 6;
-    const int a = 2 * t < n ? v(2 * t) : 0, b = 2 * t + 1 < n ? v(2 * t + 1) : 0;
-    const int incl = wave_incl_i(a + b);
+    int e[PER], sum = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        e[u] = PER * t + u < n ? v(PER * t + u) : 0;
+        sum += e[u];
+    }
+    const int incl = wave_incl_i(sum);
     if ((t & 63) == 63) wtot[wave] = incl;
     __syncthreads();
     int base = 0;
     for (int w = 0; w < wave; ++w) base += wtot[w];
-    const int ex = base + incl - (a + b);
-    if (2 * t < n) pre[2 * t] = ex;
-    if (2 * t + 1 < n) pre[2 * t + 1] = ex + a;
+    int ex = base + incl - sum;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        if (PER * t + u < n) pre[PER * t + u] = ex;
+        ex += e[u];
+    }
     if (t == 0) {
         int tot = 0;
-        for (int w = 0; w < 8; ++w) tot += wtot[w];
+        for (int w = 0; w < NW; ++w) tot += wtot[w];
         pre[n] = tot;
     }
     __syncthreads();
 }
 
 // root: 0 the smaller child of T.cur_*, 1 a sampled or whole-window root (grid: row blocks x
-// feature blocks x tasks); 2 the slots of a multi-leaf step of n_tasks tasks: a fixed grid of
-// blocks takes the (task, row block, feature block) items of all tasks.  Each task's items are
-// numbered row block * feature blocks + feature block within groups of 8 row blocks, and each
-// task's first item is a multiple of 8, so row block gb's feature blocks all have item % 8 ==
-// gb % 8; blocks go out round-robin over the 8 XCDs, and the blocks of XCD x take exactly the
-// items with item % 8 == x: the 128-byte rows a row block's feature blocks share come into that
-// XCD's L2 once.
-template <int LPR, bool HV, int KC, bool SPEC>
-__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows, int slot_parts,
-                                                  int n_tasks) {
+// feature blocks x tasks).
+template <int LPR, bool HV, int KC>
+__global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int root, int min_rows) {
     constexpr int FB = 16 * LPR;
     extern __shared__ unsigned long long lds[];
-    if (SPEC) {   // root == 2
-        const int fbs = (S.fp + FB - 1) / FB;
-        __shared__ int ipre[1025];
-        if (n_tasks == 1) {   // one task: no prefix to form
-            if (threadIdx.x == 0) {
-                const Task& T0 = tasks[0];
-                ipre[0] = 0;
-                ipre[1] = T0.done ? 0 : (T0.spec_gb + 7) / 8 * 8 * fbs;
-            }
-            __syncthreads();
-        } else {
-            block_prefix_1024(n_tasks, [&](int i) {
-                const Task& Ti = tasks[i];
-                return Ti.done ? 0 : (Ti.spec_gb + 7) / 8 * 8 * fbs;
-            }, ipre);
-        }
-        const int items = ipre[n_tasks];
-        __shared__ Slot slots[SPEC_W];
-        __shared__ int s_item;
-        int cur = -1;
-        // Items i with i % 8 == x go to the blocks of XCD x (block id % 8): block b takes item b
-        // first; items past the grid go out in turn from XCD x's counter (when there are any), so
-        // large and small items balance within each XCD, and every block stops once its counter
-        // passes the last item.
-        const int xq = blockIdx.x % 8;
-        for (int item = blockIdx.x; item < items;) {
-            int lo = 0, hi = n_tasks - 1;   // the task: the last with ipre[t] <= item
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (ipre[mid] <= item) lo = mid;
-                else hi = mid - 1;
-            }
-            const int tk = lo;
-            const Task T = tasks[tk];
-            if (tk != cur) {   // the task's slot table, into LDS in one round trip
-                const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
-                if ((int)threadIdx.x < nw) ((int*)slots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
-                __syncthreads();
-                cur = tk;
-            }
-            const int li = item - ipre[tk];
-            const int x = li % 8, j = li / 8;
-            const int fb = j % fbs, gb = (j / fbs) * 8 + x;
-            int sj, rb, node, start, count, buf, per;
-            if (gb < T.spec_gb &&
-                spec_target(T, tasks + tk, slots, gb, fb == 0, sj, rb, node, start, count, buf, per))
-                hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, rb, sj * slot_parts + rb, fb * FB,
-                                            false);
-            __syncthreads();   // the next item clears the LDS this one read (and may reload the slots)
-            if (items <= (int)gridDim.x) break;
-            if (threadIdx.x == 0) s_item = 8 * ((int)gridDim.x / 8 + atomicAdd(&S.spec_ctr[xq], 1)) + xq;
-            __syncthreads();
-            item = s_item;
-            __syncthreads();
-        }
-    } else {
     const Task T = tasks[blockIdx.z];
     int node, start, count, buf;
     const bool leader = blockIdx.x == 0 && blockIdx.y == 0;
@@ -638,8 +595,80 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
         tasks[blockIdx.z].hist_wide = per > WORD_ROWS ? 1 : 0;
     }
     // the whole-window root reads rows directly
-    hist_row_block<LPR, HV, KC>(T, S, lds, start, count, buf, per, blockIdx.x, blockIdx.x, blockIdx.y * FB,
-                                T.contiguous && root == 1);
+    hist_row_block<LPR, HV, KC, 512>(T, S, lds, start, count, buf, per, blockIdx.x, blockIdx.x, blockIdx.y * FB,
+                                     T.contiguous && root == 1);
+}
+
+// Child builds of a multi-leaf step of n_tasks tasks: a fixed grid of blocks takes the (task, row
+// block, feature block) items of all tasks.  h == 1: 1024 threads (one block per CU), 32 features
+// per item (2 lanes per row), two copies per bin (128 KB of LDS).  Wider rows per item gather
+// fewer cache lines per row: 1M x 128 fit, child builds 0.242 ms per tree against 0.271 for 16
+// features per item in two 512-thread blocks per CU, and 0.263 with 64-feature items (4 lanes
+// per row, one copy per bin: its LDS atomics collide more on the panel's bin distributions).
+// h varies: 512 threads, 16 features per item with g and h planes (64 KB).  Each task's items are
+// numbered row block * feature blocks + feature block within groups of 8 row blocks, and each
+// task's first item is a multiple of 8, so row block gb's feature blocks all have item % 8 ==
+// gb % 8; blocks go out round-robin over the 8 XCDs, and the blocks of XCD x take exactly the
+// items with item % 8 == x: the 128-byte rows a row block's feature blocks share come into that
+// XCD's L2 once.
+template <bool HV>
+__global__ __launch_bounds__(HV ? 512 : 1024) void k_hist_spec(Task* tasks, Shared S, int n_tasks) {
+    constexpr int NT = HV ? 512 : 1024, FB = HV ? 16 : 32;
+    extern __shared__ unsigned long long lds[];
+    const int fbs = (S.fp + FB - 1) / FB;
+    auto task_items = [&](const Task& Ti) { return Ti.done ? 0 : (Ti.spec_gb + 7) / 8 * 8 * fbs; };
+    __shared__ int ipre[1025];
+    if (n_tasks == 1) {   // one task: no prefix to form
+        if (threadIdx.x == 0) {
+            ipre[0] = 0;
+            ipre[1] = task_items(tasks[0]);
+        }
+        __syncthreads();
+    } else {
+        block_prefix_1024<NT>(n_tasks, [&](int i) { return task_items(tasks[i]); }, ipre);
+    }
+    const int items = ipre[n_tasks];
+    __shared__ Slot slots[SPEC_W];
+    __shared__ int s_item;
+    int cur = -1;
+    // Items i with i % 8 == x go to the blocks of XCD x (block id % 8): block b takes item b
+    // first; items past the grid go out in turn from XCD x's counter (when there are any), so
+    // large and small items balance within each XCD, and every block stops once its counter
+    // passes the last item.
+    const int xq = blockIdx.x % 8;
+    for (int item = blockIdx.x; item < items;) {
+        int lo = 0, hi = n_tasks - 1;   // the task: the last with ipre[t] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ipre[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        const int tk = lo;
+        const Task T = tasks[tk];
+        if (tk != cur) {   // the task's slot table, into LDS in one round trip
+            const int nw = (int)(sizeof(Slot) / 4) * T.n_slots;
+            if ((int)threadIdx.x < nw) ((int*)slots)[threadIdx.x] = ((const int*)T.slots)[threadIdx.x];
+            __syncthreads();
+            cur = tk;
+        }
+        const int li = item - ipre[tk];
+        const int x = li % 8, j = li / 8;
+        const int fb = j % fbs, gb = (j / fbs) * 8 + x;
+        int sj, rb, node, start, count, buf, per;
+        if (gb < T.spec_gb && spec_target(T, tasks + tk, slots, gb, fb == 0, sj, rb, node, start, count, buf, per)) {
+            if (HV)
+                hist_row_block<1, true, 1, NT>(T, S, lds, start, count, buf, per, rb, slots[sj].blk0 + rb, fb * FB,
+                                               false);
+            else
+                hist_row_block<2, false, 2, NT>(T, S, lds, start, count, buf, per, rb, slots[sj].blk0 + rb, fb * FB,
+                                                false);
+        }
+        __syncthreads();   // the next item clears the LDS this one read (and may reload the slots)
+        if (items <= (int)gridDim.x) break;
+        if (threadIdx.x == 0) s_item = 8 * ((int)gridDim.x / 8 + atomicAdd(&S.spec_ctr[xq], 1)) + xq;
+        __syncthreads();
+        item = s_item;
+        __syncthreads();
     }
 }
 
@@ -937,23 +966,27 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
     const int kc = (!hconst || lpr > 1) ? 1 : (n_tasks == 1 ? 4 : 2);
     const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
     const int rt = root ? 1 : 0;
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
-    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
-    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
-    else TIMED_LAUNCH((k_hist_row<1, false, 2, false>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows, 0, 0);
+    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else if (lpr > 1) TIMED_LAUNCH((k_hist_row<2, false, 1>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else if (kc == 4) TIMED_LAUNCH((k_hist_row<1, false, 4>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
+    else TIMED_LAUNCH((k_hist_row<1, false, 2>), grid, dim3(512), lds, st, e0, e1, tasks, S, rt, min_rows);
 }
 
-// Child builds of a multi-leaf step: every slot's smaller child in one launch of `blocks` blocks
-// (two per CU: 16 features and 64 KB of LDS each), which take the step's (row block, feature
-// block) items in turn; partials of slot j start at partial j * slot_parts.
+// Child builds of a multi-leaf step: every slot's smaller child in one launch of a fixed grid whose
+// blocks take the step's (row block, feature block) items in turn; row block rb of slot j stores
+// partial blk0_j + rb (the slots' row blocks numbered in turn).  h == 1: one 1024-thread block per
+// CU, items of 32 features; h varies: two 512-thread blocks per CU, items of 16 features.
+void spec_child_shape(bool hconst, int n_cu, int fp, int& blocks, int& fbs) {
+    blocks = ((hconst ? 1 : 2) * n_cu + 7) / 8 * 8;   // a multiple of the 8 XCDs
+    fbs = hconst ? (fp + 31) / 32 : (fp + 15) / 16;
+}
+
 void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
-                      int slot_parts, hipEvent_t e0, hipEvent_t e1) {
-    constexpr int FB = 16;
+                      hipEvent_t e0, hipEvent_t e1) {
     dim3 grid(std::max(1, blocks), 1, 1);   // the tasks' items share the grid
-    const int kc = hconst ? 2 : 1;
-    const size_t lds = (size_t)FB * NBIN * 8 * (kc + (hconst ? 0 : 1));
-    if (!hconst) TIMED_LAUNCH((k_hist_row<1, true, 1, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts, n_tasks);
-    else TIMED_LAUNCH((k_hist_row<1, false, 2, true>), grid, dim3(512), lds, st, e0, e1, tasks, S, 2, 0, slot_parts, n_tasks);
+    const size_t lds = (size_t)32 * NBIN * 8 * 2;   // 32 features x 2 copies, or 16 features x (g, h)
+    if (!hconst) TIMED_LAUNCH((k_hist_spec<true>), grid, dim3(512), lds / 2, st, e0, e1, tasks, S, n_tasks);
+    else TIMED_LAUNCH((k_hist_spec<false>), grid, dim3(1024), lds, st, e0, e1, tasks, S, n_tasks);
 }
 
 // =================================================================== split scan
@@ -1542,7 +1575,7 @@ __device__ void scan_cat(bool mine, const long long* qg, const int* c, const lon
 // One scan of feature f: task `task`'s root (mode 0), the children of its current split (mode
 // 1), or the children of slot sj of its multi-leaf step (mode 2).  Every thread of the block
 // calls it; it synchronizes the block.
-__device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const GrowP& P, int mode, int slot_parts,
+__device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const GrowP& P, int mode,
                                          int task, int sj, int f) {
     const Task T = tasks[task];
     Task& Tw = tasks[task];
@@ -1564,7 +1597,7 @@ __device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const Gro
         parent = sl.leaf;
         small_node = sl.small_node;
         big_node = sl.big_node;
-        const size_t o = (size_t)sj * slot_parts * per;
+        const size_t o = (size_t)sl.blk0 * per;
         pp.part += o;
         pp.part_c += o;
         if (pp.part_h) pp.part_h += o;
@@ -1724,15 +1757,15 @@ __device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const Gro
 
 // The split scan.  Modes 0 and 1: grid (features, 1, tasks).  Mode 2 (multi-leaf steps): grid
 // (features, 1, z) whose z-blocks take the active (task, slot) pairs of all tasks in turn.
-__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int slot_parts, int n_tasks) {
+__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int n_tasks) {
     if (mode != 2) {
-        scan_one(tasks, S, P, mode, slot_parts, blockIdx.z, 0, blockIdx.x);
+        scan_one(tasks, S, P, mode, blockIdx.z, 0, blockIdx.x);
         return;
     }
     if (n_tasks == 1) {   // one task: its slots in turn, no prefix to form
         const int ns = tasks[0].done ? 0 : tasks[0].n_slots;
         for (int q = blockIdx.z; q < ns; q += gridDim.z) {
-            scan_one(tasks, S, P, 2, slot_parts, 0, q, blockIdx.x);
+            scan_one(tasks, S, P, 2, 0, q, blockIdx.x);
             __syncthreads();
         }
         return;
@@ -1771,18 +1804,17 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
             if (pre[mid] <= q) lo = mid;
             else hi = mid - 1;
         }
-        scan_one(tasks, S, P, 2, slot_parts, lo, q - pre[lo], blockIdx.x);
+        scan_one(tasks, S, P, 2, lo, q - pre[lo], blockIdx.x);
         __syncthreads();   // the next pair reuses the block's LDS
     }
 }
 
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
-                 int slot_parts) {
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
     // mode 2: as many z-blocks as one task's slots, at least the tasks (one pair each on average
     // at four slots per task), at most 64
     const int z = mode == 2 ? std::min(64, std::max(SPEC_W, n_tasks)) : n_tasks;
     dim3 grid(S.n_features, 1, z);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, slot_parts, n_tasks);
+    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, n_tasks);
 }
 
 // =================================================================== partition
@@ -2145,7 +2177,7 @@ __device__ void spec_pending(const Task& T, int F, const GrowP& P, double* key, 
 // cached candidate per thread, the pending nodes' per-feature results, the stored stop), so the
 // kernel waits for memory about once.
 __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, GrowP P, int width, int min_per,
-                                                     int rb_target, int slot_parts) {
+                                                     int rb_target, int step_parts) {
     Task& Tw = tasks[blockIdx.x];
     const Task T = tasks[blockIdx.x];
     // the step's child builds hand out their items from these counters (the first task's block
@@ -2317,13 +2349,14 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
         chunks = (sl.count + PART_CHUNK - 1) / PART_CHUNK;
         small = min(c.cl, sl.count - c.cl);
     }
-    // rows per histogram tile, the same for every slot: the step's smaller children in about
-    // rb_target row blocks (so the step's items about fill the resident blocks), at least min_per
-    // rows, at most MAX_BLOCK_ROWS, and no slot with more than slot_parts blocks
+    // rows per histogram tile, the same for every slot: the step's smaller children in at most
+    // rb_target row blocks (so the step's items fit the resident blocks: each slot rounds its
+    // row blocks up, so the tile is sized for rb_target - n_chosen of them), at least min_per
+    // rows, at most MAX_BLOCK_ROWS, and at most step_parts blocks over all slots
     const int total_small = wave_sum_i(small);
-    const int max_small = -wave_min_i(-small);
-    int per = max(min_per, (total_small + rb_target - 1) / rb_target);
-    per = max(per, (max_small + slot_parts - 1) / slot_parts);
+    const int rb = max(1, rb_target - n_chosen);
+    int per = max(min_per, (total_small + rb - 1) / rb);
+    per = max(per, (total_small + step_parts - n_chosen - 1) / (step_parts - n_chosen));
     per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
     if (lane < n_chosen) gx = max(1, (small + per - 1) / per);
     const int c_incl = wave_incl_i(chunks), g_incl = wave_incl_i(gx);
@@ -2470,9 +2503,9 @@ void launch_spec_status(hipStream_t st, const Task* tasks, int n_tasks, int* out
 }
 
 void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int width,
-                        int min_per, int rb_target, int slot_parts) {
+                        int min_per, int rb_target, int step_parts) {
     hipLaunchKernelGGL(k_spec_select, dim3(n_tasks), dim3(256), 0, st, tasks, S, P, width, min_per, rb_target,
-                       slot_parts);
+                       step_parts);
 }
 
 void launch_spec_finalize(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P) {
@@ -3021,12 +3054,12 @@ void init_kernel_attributes() {
         const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) throw std::runtime_error(std::string("kernel LDS attribute: ") + hipGetErrorString(e));
     };
-    opt_in((const void*)k_hist_row<1, false, 4, false>, 128 * 1024);
-    opt_in((const void*)k_hist_row<1, false, 2, false>, 128 * 1024);
-    opt_in((const void*)k_hist_row<1, true, 1, false>, 128 * 1024);
-    opt_in((const void*)k_hist_row<2, false, 1, false>, 128 * 1024);
-    opt_in((const void*)k_hist_row<1, false, 2, true>, 64 * 1024);
-    opt_in((const void*)k_hist_row<1, true, 1, true>, 64 * 1024);
+    opt_in((const void*)k_hist_row<1, false, 4>, 128 * 1024);
+    opt_in((const void*)k_hist_row<1, false, 2>, 128 * 1024);
+    opt_in((const void*)k_hist_row<1, true, 1>, 128 * 1024);
+    opt_in((const void*)k_hist_row<2, false, 1>, 128 * 1024);
+    opt_in((const void*)k_hist_spec<false>, 128 * 1024);
+    opt_in((const void*)k_hist_spec<true>, 64 * 1024);
     opt_in((const void*)k_pair_hist, 160 * 1024);
     done = true;
 }

@@ -28,18 +28,19 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                  bool contiguous_root, bool hconst, int max_count, int variant,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // mode 0: the root; 1: the smaller child of T.cur_* and its sibling; 2: every slot of a
-// multi-leaf step (partials of slot j from partial j * slot_parts).
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
-                 int slot_parts = 0);
+// multi-leaf step (partials of slot j from partial blk0_j).
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
 // Child-build tiles of a histogram variant: lanes per row, minimum rows per block, partials cap.
 void child_tiles(int variant, bool hconst, int& lpr, int& min_rows, int& pcap);
-// Multi-leaf growth (one task per session): the step's selection (histogram tiles of at least
-// min_per rows, about rb_target row blocks per step), its child builds (a fixed grid of `blocks`
-// blocks), and the tree's final numbering after the last step.
+// Multi-leaf growth: the step's selection (histogram tiles of at least min_per rows, at most
+// rb_target row blocks and step_parts partials per task step), its child builds (a fixed grid of
+// `blocks` blocks, fbs feature blocks per row block), and the tree's final numbering after the
+// last step.
 void launch_spec_select(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int width,
-                        int min_per, int rb_target, int slot_parts);
+                        int min_per, int rb_target, int step_parts);
+void spec_child_shape(bool hconst, int n_cu, int fp, int& blocks, int& fbs);
 void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool hconst, int blocks,
-                      int slot_parts, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_spec_finalize(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P);
 // out (device-visible) = {every active task's growth done, the most steps a task took}.
 void launch_spec_status(hipStream_t st, const Task* tasks, int n_tasks, int* out);

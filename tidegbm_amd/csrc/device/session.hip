@@ -128,7 +128,7 @@ struct tgbm_session {
     int n_cu = 256;         // compute units of the device
     int spec_width = 0;
     int spec_cap = 0;
-    int slot_parts = 0;
+    int step_parts = 0;     // partials one task's multi-leaf step may write (all slots)
     int max_task_rows = 1;
     int spec_k = 8;
     bool spec_tree = false;
@@ -534,9 +534,11 @@ int tgbm_session_set_spec(tgbm_session* s, int width) {
         spec_resolve(s);
         if (width < 0 || width > SPEC_W) throw std::runtime_error("multi-leaf width must be in [0, 8]");
         if (width > 1 && s->spec_cap == 0 && s->max_leaves <= 64 && s->T <= 1024) {
-            // Room for the grown nodes (at most twice the final tree's) and for SPEC_W slots of
-            // partials per task, each as many as one smaller child's build may write (one task:
-            // 64 tiles of a step; several: fewer, larger tiles each).
+            // Room for the grown nodes (at most twice the final tree's) and for the partials of
+            // one step of a task, SPEC_W * sp of them over all its slots (one task: 512; several:
+            // 128, enough for one active task's step to fill the GPU).  The smaller children of a
+            // step hold at most half the task's rows, so sp above their MAX_BLOCK_ROWS tiles plus
+            // one per slot bounds every step.
             HIP_OK(hipStreamSynchronize(s->stream));
             const int cap = std::min(MAX_NODES, 2 * s->max_nodes);
             const int sp = std::max(s->T == 1 ? 64 : 16,
@@ -568,7 +570,7 @@ int tgbm_session_set_spec(tgbm_session* s, int width) {
                 s->set_field_ptr(t, offsetof(Task, host_flag), T.host_flag);
             }
             s->spec_cap = cap;
-            s->slot_parts = sp;
+            s->step_parts = SPEC_W * sp;
             s->S.part_cap = pc;
             s->S.spec_ctr = s->alloc<int>(8);
             HIP_OK(hipMemsetAsync(s->S.spec_ctr, 0, 8 * sizeof(int), s->stream));
@@ -1040,25 +1042,24 @@ static bool spec_done(tgbm_session* s, int* steps) {
 // the scans of both children of every slot.
 static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
     const int ms = s->launch_rows();
-    // child builds: two resident 512-thread blocks per CU (2 LDS copies per bin, 64 KB each; one
-    // block per CU with 4 copies measured slower) take the step's (row block, feature block)
+    // child builds: a fixed grid of resident blocks takes the step's (row block, feature block)
     // items; tiles of at least 2,048 rows, sized so the items about fill those blocks
-    const int blocks = (2 * s->n_cu + 7) / 8 * 8;   // a multiple of the 8 XCDs
-    const int fbs = (s->fp + 15) / 16;
+    int blocks, fbs;
+    spec_child_shape(s->hconst, s->n_cu, s->fp, blocks, fbs);
     EventPair e0, e1, e2, e3;
     if (s->timing) s->tic(6, 0.0, e0);   // the step's selection (timer 6)
     launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 2048, std::max(1, blocks / (fbs * s->an)),
-                       s->slot_parts);
+                       s->step_parts);
     if (s->timing) s->toc(e0);
     if (s->timing) s->tic(1, 0.0, e1);
     launch_partition(s->stream, s->S, s->td(), s->an, ms, P, 2);
     if (s->timing) s->toc(e1);
     if (s->timing) s->kernel_events(5, 0.0, e2);
-    launch_hist_spec(s->stream, s->S, s->td(), s->an, s->hconst, blocks, s->slot_parts,
+    launch_hist_spec(s->stream, s->S, s->td(), s->an, s->hconst, blocks,
                      s->timing ? e2.a : nullptr, s->timing ? e2.b : nullptr);
     if (s->timing) s->events.push_back(e2);
     if (s->timing) s->tic(2, 0.0, e3);
-    launch_scan(s->stream, s->S, s->td(), s->an, P, 2, s->slot_parts);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 2);
     if (s->timing) s->toc(e3);
     HIP_OK(hipGetLastError());
 }
