@@ -995,7 +995,7 @@ void launch_hist_spec(hipStream_t st, const Shared& S, Task* tasks, int n_tasks,
 // empty bins skipped, missing-right (ml=0) before missing-left, first maximum kept.  One wave per
 // feature, 4 bins per lane; prefix sums are exact int64, so the scan order cannot change them.
 // Lane j holds bins 4j..4j+3 (bins >= nb must be zero).
-template <bool hconst, int T0, int T1>
+template <bool hconst, int T0, int T1, int TP = T1 - T0>
 __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, const long long* qh,
                                            double inv_g, double inv_h, int nb, double lam, double gamma,
                                            int min_leaf, double min_hess, double l1, int mono, double& out_gain,
@@ -1033,43 +1033,48 @@ __device__ inline void scan_feature_wave_t(const long long* qg, const int* c, co
     double best = -INFINITY;
     int bkey = 0x7fffffff;
     int bcl = 0;
-    // The lane's 8 candidates are evaluated as straight-line code (an invalid one is computed
-    // and then ignored), so their divisions are independent and overlap; they are then taken in
-    // key order with the same strict comparison, so the result equals a sequential scan's.
-    // only bins lane*4+t with T0 <= t < T1 are candidates of this call
-    double cg[8];
-    int cclv[8];
-    bool cok[8];
+    // The lane's candidates are evaluated as straight-line code, TP bins (2 * TP candidates) per
+    // pass (an invalid one is computed and then ignored), so their divisions are independent and
+    // overlap; they are then taken in key order with the same strict comparison, so the result
+    // equals a sequential scan's.  Only bins lane*4+t with T0 <= t < T1 are candidates of this call.
+    static_assert((T1 - T0) % TP == 0, "passes of TP bins");
 #pragma unroll
-    for (int t = T0; t < T1; ++t) {
-        const int b = lane * 4 + t;
-        const bool bin_ok = (b >= 1) & (b <= nb - 2) & (c[t] != 0);
-        const long long QL = eg + pg[t], QhL = eh + ph[t], CL = ec + pc[t];
+    for (int p0 = T0; p0 < T1; p0 += TP) {
+        double cg[2 * TP];
+        int cclv[2 * TP];
+        bool cok[2 * TP];
 #pragma unroll
-        for (int ml = 0; ml <= 1; ++ml) {
-            const long long ql = QL + (ml ? missQ : 0), qhl = QhL + (ml ? missQh : 0);
-            const long long cl = CL + (ml ? missC : 0);
-            const long long cr = totC - cl;
-            const double gl = (double)ql * inv_g, gr = (double)(totQ - ql) * inv_g;
-            const double hl = hconst ? (double)cl : (double)qhl * inv_h;
-            const double hr = hconst ? (double)cr : (double)(totQh - qhl) * inv_h;
-            bool ok = bin_ok & (cl >= min_leaf) & (cr >= min_leaf) & (hl >= min_hess) & (hr >= min_hess);
-            if (mono != 0) {   // wave-uniform: the feature's monotone constraint (histogram.cpp:177-180)
-                const double wl = weight_of(gl, hl, lam, l1), wr = weight_of(gr, hr, lam, l1);
-                ok &= mono > 0 ? !(wl < wr) : !(wl > wr);
+        for (int u = 0; u < TP; ++u) {
+            const int t = p0 + u;
+            const int b = lane * 4 + t;
+            const bool bin_ok = (b >= 1) & (b <= nb - 2) & (c[t] != 0);
+            const long long QL = eg + pg[t], QhL = eh + ph[t], CL = ec + pc[t];
+#pragma unroll
+            for (int ml = 0; ml <= 1; ++ml) {
+                const long long ql = QL + (ml ? missQ : 0), qhl = QhL + (ml ? missQh : 0);
+                const long long cl = CL + (ml ? missC : 0);
+                const long long cr = totC - cl;
+                const double gl = (double)ql * inv_g, gr = (double)(totQ - ql) * inv_g;
+                const double hl = hconst ? (double)cl : (double)qhl * inv_h;
+                const double hr = hconst ? (double)cr : (double)(totQh - qhl) * inv_h;
+                bool ok = bin_ok & (cl >= min_leaf) & (cr >= min_leaf) & (hl >= min_hess) & (hr >= min_hess);
+                if (mono != 0) {   // wave-uniform: the feature's monotone constraint (histogram.cpp:177-180)
+                    const double wl = weight_of(gl, hl, lam, l1), wr = weight_of(gr, hr, lam, l1);
+                    ok &= mono > 0 ? !(wl < wr) : !(wl > wr);
+                }
+                cok[u * 2 + ml] = ok;
+                cg[u * 2 + ml] = 0.5 * (leaf_score(gl, hl, lam, l1) + leaf_score(gr, hr, lam, l1) - parent) - gamma;
+                cclv[u * 2 + ml] = (int)cl;
             }
-            cok[t * 2 + ml] = ok;
-            cg[t * 2 + ml] = 0.5 * (leaf_score(gl, hl, lam, l1) + leaf_score(gr, hr, lam, l1) - parent) - gamma;
-            cclv[t * 2 + ml] = (int)cl;
         }
-    }
 #pragma unroll
-    for (int j = T0 * 2; j < T1 * 2; ++j) {
-        const int key = (lane * 4 + (j >> 1)) * 2 + (j & 1);
-        const bool take = cok[j] & ((cg[j] > best) | ((cg[j] == best) & (key < bkey)));
-        best = take ? cg[j] : best;
-        bkey = take ? key : bkey;
-        bcl = take ? cclv[j] : bcl;
+        for (int j = 0; j < 2 * TP; ++j) {
+            const int key = (lane * 4 + p0 + (j >> 1)) * 2 + (j & 1);
+            const bool take = cok[j] & ((cg[j] > best) | ((cg[j] == best) & (key < bkey)));
+            best = take ? cg[j] : best;
+            bkey = take ? key : bkey;
+            bcl = take ? cclv[j] : bcl;
+        }
     }
     // lanes hold increasing keys, so the wave's first maximum is the best lane's best
     const int wl = wave_best_lane(best, bkey);
@@ -1337,8 +1342,9 @@ struct Parts {
 };
 
 // Exact sums, over partials k = k0, k0 + dk, ... < np of one histogram build, of bins
-// lane*4 .. lane*4+3 of feature f.  Four partials are loaded per round so their latencies
+// lane*4 .. lane*4+3 of feature f.  PR partials (4 or 2) are loaded per round so their latencies
 // overlap.
+template <int PR = 4>
 __device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk, int np, long long sq[4],
                                     long long sc[4], long long sh[4]) {
     const int lane = threadIdx.x & 63;
@@ -1354,16 +1360,16 @@ __device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk
     if (T.wide == 2) {
         // raw packed-word sums modulo 2^64 (the counts come from the task's root counts)
         unsigned long long uq[4] = {0, 0, 0, 0};
-        for (; k + 3 * dk < np; k += 4 * dk) {
-            uint4 v[8];
+        for (; k + (PR - 1) * dk < np; k += PR * dk) {
+            uint4 v[2 * PR];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < PR; ++u) {
                 const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
                 v[2 * u] = ldg(a);
                 v[2 * u + 1] = ldg(a + 1);
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 2 * PR; ++u) {
                 const int t = (u & 1) * 2;
                 uq[t] += (unsigned long long)v[u].y << 32 | v[u].x;
                 uq[t + 1] += (unsigned long long)v[u].w << 32 | v[u].z;
@@ -1377,10 +1383,10 @@ __device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk
 #pragma unroll
         for (int t = 0; t < 4; ++t) sq[t] = (long long)uq[t];
     } else if (T.wide) {
-        for (; k + 3 * dk < np; k += 4 * dk) {
-            uint4 v[8], c[4];
+        for (; k + (PR - 1) * dk < np; k += PR * dk) {
+            uint4 v[2 * PR], c[PR];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < PR; ++u) {
                 const size_t o = (size_t)(k + u * dk) * stride + off;
                 const uint4* a = (const uint4*)(T.part + o);
                 v[2 * u] = ldg(a);
@@ -1388,7 +1394,7 @@ __device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk
                 c[u] = ldg((const uint4*)(T.part_c + o));
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < PR; ++u) {
                 sq[0] += (long long)((unsigned long long)v[2 * u].y << 32 | v[2 * u].x);
                 sq[1] += (long long)((unsigned long long)v[2 * u].w << 32 | v[2 * u].z);
                 sq[2] += (long long)((unsigned long long)v[2 * u + 1].y << 32 | v[2 * u + 1].x);
@@ -1410,16 +1416,16 @@ __device__ inline void sum_partials(const Parts& T, int F, int f, int k0, int dk
             }
         }
     } else {
-        for (; k + 3 * dk < np; k += 4 * dk) {
-            uint4 v[8];
+        for (; k + (PR - 1) * dk < np; k += PR * dk) {
+            uint4 v[2 * PR];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < PR; ++u) {
                 const uint4* a = (const uint4*)(T.part + (size_t)(k + u * dk) * stride + off);
                 v[2 * u] = ldg(a);
                 v[2 * u + 1] = ldg(a + 1);
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < 2 * PR; ++u) {
                 const unsigned long long w0 = (unsigned long long)v[u].y << 32 | v[u].x;
                 const unsigned long long w1 = (unsigned long long)v[u].w << 32 | v[u].z;
                 const int t = (u & 1) * 2;
@@ -1575,8 +1581,8 @@ __device__ void scan_cat(bool mine, const long long* qg, const int* c, const lon
 // One scan of feature f: task `task`'s root (mode 0), the children of its current split (mode
 // 1), or the children of slot sj of its multi-leaf step (mode 2).  Every thread of the block
 // calls it; it synchronizes the block.
-__device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const GrowP& P, int mode,
-                                         int task, int sj, int f) {
+__device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const GrowP& P, int mode, int task,
+                                         int f) {
     const Task T = tasks[task];
     Task& Tw = tasks[task];
     if (T.done) return;
@@ -1584,27 +1590,15 @@ __device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const Gro
     const size_t per = (size_t)F * NBIN;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     // the build this scan reads: its parent, children and partials
-    int parent = T.cur_leaf, small_node = T.small_node, big_node = T.big_node;
+    const int parent = T.cur_leaf, small_node = T.small_node, big_node = T.big_node;
     Parts pp;
     pp.part = T.part;
     pp.part_c = T.part_c;
     pp.part_h = T.part_h;
     pp.wide = T.hist_wide;
     pp.hconst = T.hconst;
-    int n_parts = T.hist_parts;
-    if (mode == 2) {
-        const Slot& sl = T.slots[sj];
-        parent = sl.leaf;
-        small_node = sl.small_node;
-        big_node = sl.big_node;
-        const size_t o = (size_t)sl.blk0 * per;
-        pp.part += o;
-        pp.part_c += o;
-        if (pp.part_h) pp.part_h += o;
-        pp.wide = sl.hist_wide;
-        n_parts = sl.hist_parts;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && mode != 2) {
+    const int n_parts = T.hist_parts;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         Tw.pending0 = mode == 0 ? 0 : T.cur_left;
         Tw.pending1 = mode == 0 ? -1 : T.cur_left + 1;
         Tw.part_lr = 0;   // the histogram launch has consumed it; the next partition counts anew
@@ -1755,38 +1749,38 @@ __device__ __forceinline__ void scan_one(Task* tasks, const Shared& S, const Gro
     }
 }
 
-// The split scan.  Modes 0 and 1: grid (features, 1, tasks).  Mode 2 (multi-leaf steps): grid
-// (features, 1, z) whose z-blocks take the active (task, slot) pairs of all tasks in turn.
-__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode, int n_tasks) {
-    if (mode != 2) {
-        scan_one(tasks, S, P, mode, blockIdx.z, 0, blockIdx.x);
-        return;
-    }
-    if (n_tasks == 1) {   // one task: its slots in turn, no prefix to form
-        const int ns = tasks[0].done ? 0 : tasks[0].n_slots;
-        for (int q = blockIdx.z; q < ns; q += gridDim.z) {
-            scan_one(tasks, S, P, 2, 0, q, blockIdx.x);
-            __syncthreads();
-        }
-        return;
-    }
+// The split scan of the root (mode 0) or of a split's two children (mode 1): grid (features, 1,
+// tasks).
+__global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, int mode) {
+    scan_one(tasks, S, P, mode, blockIdx.z, blockIdx.x);
+}
+
+// The split scan of a multi-leaf step (numeric features: categorical features grow one split at a
+// time on the host).  One wave per (task slot, feature, child), with no block barrier between
+// slots, so the waves of a CU overlap one another's loads: wave w of block x scans feature
+// 2x + (w >> 1) of child w & 1 (0 the built smaller child, 1 the larger = parent - smaller; both
+// waves sum the slot's partials, the second from the cache) for the active (task, slot) pairs
+// q = z, z + gridDim.z, ... of all tasks.  Each wave stores its child's histogram row (the
+// parent of a later split), the feature's best split, and for feature 0 the child's value.
+template <bool HC>   // h == 1 for every task of the session
+__global__ __launch_bounds__(256, 4) void k_scan_spec(Task* tasks, Shared S, GrowP P, int n_tasks) {
     __shared__ int pre[1025];
     {   // exclusive prefix of the tasks' slot counts (256 threads, four entries each; <= 1024 tasks)
         __shared__ int wtot[4];
         const int tt = threadIdx.x, wave = tt >> 6;
-        int v[4], s = 0;
+        int v[4], sum = 0;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = 4 * tt + u;
             v[u] = i < n_tasks && !tasks[i].done ? tasks[i].n_slots : 0;
-            s += v[u];
+            sum += v[u];
         }
-        const int incl = wave_incl_i(s);
+        const int incl = wave_incl_i(sum);
         if ((tt & 63) == 63) wtot[wave] = incl;
         __syncthreads();
         int base = 0;
         for (int w = 0; w < wave; ++w) base += wtot[w];
-        int ex = base + incl - s;
+        int ex = base + incl - sum;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int i = 4 * tt + u;
@@ -1797,6 +1791,11 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
         __syncthreads();
     }
     const int pairs = pre[n_tasks];
+    const int F = S.n_features;
+    const size_t per_node = (size_t)F * NBIN;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int f = 2 * blockIdx.x + (wave >> 1), side = wave & 1;
+    if (f >= F) return;   // no barrier follows
     for (int q = blockIdx.z; q < pairs; q += gridDim.z) {
         int lo = 0, hi = n_tasks - 1;   // the task: the last with pre[t] <= q
         while (lo < hi) {
@@ -1804,17 +1803,87 @@ __global__ __launch_bounds__(256) void k_scan(Task* tasks, Shared S, GrowP P, in
             if (pre[mid] <= q) lo = mid;
             else hi = mid - 1;
         }
-        scan_one(tasks, S, P, 2, lo, q - pre[lo], blockIdx.x);
-        __syncthreads();   // the next pair reuses the block's LDS
+        const Task T = tasks[lo];
+        const Slot sl = ldg(T.slots + (q - pre[lo]));
+        Parts pp;
+        pp.part = T.part + (size_t)sl.blk0 * per_node;
+        pp.part_c = T.part_c + (size_t)sl.blk0 * per_node;
+        pp.part_h = T.part_h ? T.part_h + (size_t)sl.blk0 * per_node : nullptr;
+        pp.wide = sl.hist_wide;
+        pp.hconst = HC;
+        const size_t fo = (size_t)f * NBIN + lane * 4;
+        long long par_q[4] = {0, 0, 0, 0}, par_h[4] = {0, 0, 0, 0};
+        int par_c[4] = {0, 0, 0, 0};
+        if (side) {   // the parent's sums, loaded before the partials so the latencies overlap
+            const size_t po = (size_t)sl.leaf * per_node + fo;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                par_q[t] = gbl(T.hg)[po + t];
+                par_c[t] = gbl(T.hc)[po + t];
+                if (!HC) par_h[t] = gbl(T.hh)[po + t];
+            }
+        }
+        long long sq[4], sc[4], sh[4];
+        sum_partials<2>(pp, F, f, 0, 1, sl.hist_parts, sq, sc, sh);   // two per round: fewer live registers
+        long long qg[4], qh[4];
+        int c[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            qg[t] = side ? par_q[t] - sq[t] : sq[t];
+            c[t] = side ? par_c[t] - (int)sc[t] : (int)sc[t];
+            qh[t] = HC ? 0 : (side ? par_h[t] - sh[t] : sh[t]);
+        }
+        const int node = side ? sl.big_node : sl.small_node;
+        const size_t o = (size_t)node * per_node + fo;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            gbl(T.hg)[o + t] = qg[t];
+            gbl(T.hc)[o + t] = c[t];
+            if (!HC) gbl(T.hh)[o + t] = qh[t];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (lane * 4 + t >= S.n_bins) {
+                qg[t] = 0;
+                c[t] = 0;
+                qh[t] = 0;
+            }
+        const double inv_g = ldexp(1.0, -T.s_g), inv_h = ldexp(1.0, -T.s_h);
+        const int mono = S.monotone ? (int)S.monotone[f] : 0;
+        double gain;
+        int thr, ml, cl;
+        long long tq, tqh, tc;
+        // the lane's 8 candidates in two passes of 4 (fewer live registers: more resident waves)
+        scan_feature_wave_t<HC, 0, 4, 2>(qg, c, qh, inv_g, inv_h, S.n_bins, P.l2, P.gamma, P.min_leaf, P.min_hess,
+                                         P.l1, mono, gain, thr, ml, cl, tq, tqh, tc);
+        if (lane == 0) {
+            FCand fc;
+            fc.gain = gain;
+            fc.thr = thr;
+            fc.ml = ml;
+            fc.cl = cl;
+            fc.pad = 0;
+            stg(T.fcand + (size_t)node * F + f, fc);
+            if (f == 0) {
+                // node value from the node's own sums (tree.cpp:323-328, 407-416)
+                const double G = (double)tq * inv_g;
+                const double H = HC ? (double)tc : (double)tqh * inv_h;
+                T.nodes[node].value = leaf_value(G, H, P.l2, P.l1, P.mds);
+            }
+        }
     }
 }
 
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode) {
-    // mode 2: as many z-blocks as one task's slots, at least the tasks (one pair each on average
-    // at four slots per task), at most 64
-    const int z = mode == 2 ? std::min(64, std::max(SPEC_W, n_tasks)) : n_tasks;
-    dim3 grid(S.n_features, 1, z);
-    hipLaunchKernelGGL(k_scan, grid, dim3(256), 0, st, tasks, S, P, mode, n_tasks);
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode, bool hconst) {
+    if (mode == 2) {
+        // as many z-blocks as one task's slots, or up to 64 pairs in flight at a time over the tasks
+        const int z = std::min(64, std::max(SPEC_W, n_tasks));
+        const dim3 grid((S.n_features + 1) / 2, 1, z);
+        if (hconst) hipLaunchKernelGGL(k_scan_spec<true>, grid, dim3(256), 0, st, tasks, S, P, n_tasks);
+        else hipLaunchKernelGGL(k_scan_spec<false>, grid, dim3(256), 0, st, tasks, S, P, n_tasks);
+        return;
+    }
+    hipLaunchKernelGGL(k_scan, dim3(S.n_features, 1, n_tasks), dim3(256), 0, st, tasks, S, P, mode);
 }
 
 // =================================================================== partition
@@ -1824,7 +1893,8 @@ void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, cons
 // leaf's range, rights at the back); the reference keeps them in ascending order, this kernel
 // does not need to (see k_partition).
 // Rows per block (4 per thread): smaller chunks start the gathers of a leaf's rows on more
-// CUs at once (1M x 128 fit: 0.979 ms/tree at 1024 rows, 1.00 at 2048, 1.03 at 512).
+// CUs at once (1M x 128 fit: 0.979 ms/tree at 1024 rows, 1.00 at 2048, 1.03 at 512; the 64-fit
+// walk-forward batch: 122 ms of partitions at 1024 rows, 159 ms at 4096).
 constexpr int PART_CHUNK = 1024;
 constexpr int PART_RPT = 4;
 

@@ -28,8 +28,9 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
                  bool contiguous_root, bool hconst, int max_count, int variant,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // mode 0: the root; 1: the smaller child of T.cur_* and its sibling; 2: every slot of a
-// multi-leaf step (partials of slot j from partial blk0_j).
-void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode);
+// multi-leaf step (partials of slot j from partial blk0_j; hconst: h == 1 for every task).
+void launch_scan(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, const GrowP& P, int mode,
+                 bool hconst = true);
 // Child-build tiles of a histogram variant: lanes per row, minimum rows per block, partials cap.
 void child_tiles(int variant, bool hconst, int& lpr, int& min_rows, int& pcap);
 // Multi-leaf growth: the step's selection (histogram tiles of at least min_per rows, at most

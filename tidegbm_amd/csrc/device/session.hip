@@ -1059,7 +1059,7 @@ static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p
                      s->timing ? e2.a : nullptr, s->timing ? e2.b : nullptr);
     if (s->timing) s->events.push_back(e2);
     if (s->timing) s->tic(2, 0.0, e3);
-    launch_scan(s->stream, s->S, s->td(), s->an, P, 2);
+    launch_scan(s->stream, s->S, s->td(), s->an, P, 2, s->hconst);
     if (s->timing) s->toc(e3);
     HIP_OK(hipGetLastError());
 }
@@ -1146,6 +1146,8 @@ int tgbm_tree_grow_device(tgbm_session* s, const tgbm_grow_params* p) {
         const GrowP P = s->growp(p);
         s->spec_tree = s->spec_width > 0 && p->max_leaves <= 64;
         if (s->spec_tree) {
+            // the multi-leaf scan is numeric only; categorical features grow on the host
+            if (s->S.is_cat) throw std::runtime_error("multi-leaf growth needs numeric features only");
             grow_spec(s, P, p);
             return;
         }
