@@ -214,7 +214,7 @@ struct Task {
     unsigned long long* fmask;  // [MAX_NODES][n_features][4] bins a feature's best split sends left
     unsigned long long* nmask;  // [MAX_NODES][4] the split node's mask (exported with the tree)
     // ---- multi-leaf growth (null slots: one split per step)
-    Slot* slots;           // [SPEC_W] the current step's splits
+    Slot* slots;           // [SPEC_W] the current step's splits (this task's slot_tab)
     DNode* tnodes;         // [MAX_NODES] the grown nodes in creation order (kept by the finalize)
     int* tmap;             // [MAX_NODES] the final leaf holding each grown leaf's rows
     int n_slots;           // splits of the current step
@@ -230,6 +230,9 @@ struct Task {
     int end_done;          // this tree's end has run (a later end of the same tree does nothing)
     int* rs_node;          // [64] frontier nodes at the last stop (-1: empty entry)
     int* rs_sim;           // [64] their replay ids
+    // The step's slot table inside the descriptor: a kernel that knows the task's index reaches a
+    // slot at a fixed offset from it, without waiting for the slots pointer first.
+    Slot slot_tab[SPEC_W];
 };
 
 struct Shared {            // the session-wide matrix

@@ -79,7 +79,7 @@ void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edg
 
 // =================================================================== reductions
 // Block maximum of two values, written to the task's per-block slots (no global atomics).
-__device__ inline void block_max2_store(unsigned long long a, unsigned long long b, Task& T) {
+__device__ inline void block_max2_store(unsigned long long a, unsigned long long b, const Task& T) {
     __shared__ unsigned long long sa[16], sb[16];
     a = wave_max_u64(a);
     b = wave_max_u64(b);
@@ -102,20 +102,20 @@ __device__ inline void block_max2_store(unsigned long long a, unsigned long long
 // =================================================================== gradients
 // MSE (objective.cpp:105-119, cost terms off): g = w (pred - y), h = w.
 __global__ __launch_bounds__(256) void k_grad_mse(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
+    const Task T = tasks[blockIdx.y];   // a copy: the stores below cannot alias its fields
     unsigned long long mg = 0, mh = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n_rows; i += gridDim.x * blockDim.x) {
         const double wi = T.w ? T.w[i] : 1.0;
         const double g = wi * (T.pred[i] - T.y[i]);
         T.g[i] = g;
-        T.h[i] = wi;
+        if (!T.hconst) T.h[i] = wi;   // h == 1 for every row otherwise (no weights), never stored
         const unsigned long long bg = (unsigned long long)__double_as_longlong(fabs(g));
         const unsigned long long bh = (unsigned long long)__double_as_longlong(fabs(wi));
         mg = bg > mg ? bg : mg;
         mh = bh > mh ? bh : mh;
     }
     block_max2_store(mg, mh, T);
-    if (blockIdx.x == 0 && threadIdx.x == 0) T.n_red = gridDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) tasks[blockIdx.y].n_red = gridDim.x;
 }
 
 // Within-group pairwise gradient (objective.cpp:179-219).  One block per group; each row
@@ -283,12 +283,12 @@ __global__ __launch_bounds__(256) void k_tree_init(Task* tasks) {
 
 // Packed per-row histogram operands: pk = 2^CNT_SHIFT + rint(g_use 2^s_g), qh = rint(h_use 2^s_h).
 __global__ __launch_bounds__(256) void k_pack(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
+    const Task T = tasks[blockIdx.y];   // a copy: the stores below cannot alias its fields
     const int sg = T.s_g, sh = T.s_h;
     const bool gs = T.goss && T.scaled;
     const double sc = T.goss_scale;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < T.n_rows; i += gridDim.x * blockDim.x) {
-        double g = T.g[i], h = T.h[i];
+        double g = T.g[i], h = T.hconst ? 1.0 : T.h[i];
         if (gs && T.scaled[i]) {
             g *= sc;
             h *= sc;
@@ -1804,7 +1804,7 @@ __global__ __launch_bounds__(256, 4) void k_scan_spec(Task* tasks, Shared S, Gro
             else hi = mid - 1;
         }
         const Task T = tasks[lo];
-        const Slot sl = ldg(T.slots + (q - pre[lo]));
+        const Slot sl = ldg(tasks[lo].slot_tab + (q - pre[lo]));   // issued beside the task's loads
         Parts pp;
         pp.part = T.part + (size_t)sl.blk0 * per_node;
         pp.part_c = T.part_c + (size_t)sl.blk0 * per_node;
@@ -2713,7 +2713,7 @@ void launch_pair(hipStream_t st, const Shared& S, Task* tasks, int task, int nod
 // orders the non-empty leaves by their segment start in LDS (at most MAX_NODES nodes), then
 // finds each of its rows' segment by binary search.
 __global__ __launch_bounds__(256) void k_nid_segments(Task* tasks) {
-    Task& T = tasks[blockIdx.y];
+    const Task T = tasks[blockIdx.y];   // a copy: the stores below cannot alias its fields
     if (T.skip_end) return;
     __shared__ int st[MAX_NODES];        // segment start of non-empty leaves, -1 otherwise
     __shared__ int tab[MAX_NODES * 4];   // {seg_start, seg_count, node, buf} by start
@@ -2894,7 +2894,7 @@ __global__ __launch_bounds__(64) void k_torb_final(Task* tasks, GrowP P, double 
 // pairwise objective with groups, whose loss k_loss_groups computes.  Thread sums, then waves,
 // then the block's waves in order: a fixed order that depends only on the task's own rows.
 __global__ __launch_bounds__(256) void k_pred_update(Task* tasks, double lr, int pairwise) {
-    Task& T = tasks[blockIdx.y];
+    const Task T = tasks[blockIdx.y];   // a copy: the stores below cannot alias its fields
     const int r0 = blockIdx.x * PRED_CHUNK;
     if (T.skip_end || r0 >= T.n_rows) return;
     const int r1 = min(T.n_rows, r0 + PRED_CHUNK);

@@ -458,7 +458,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             T.hg = s->alloc<long long>(per_node * node_cap);
             T.hc = s->alloc<int>(per_node * node_cap);
             T.hh = s->hconst ? nullptr : s->alloc<long long>(per_node * node_cap);
-            T.slots = s->alloc<Slot>(SPEC_W);
+            T.slots = nullptr;   // set to the task's inline table once the descriptors are allocated
             T.tnodes = s->alloc<DNode>(MAX_NODES);
             T.tmap = s->alloc<int>(MAX_NODES);
             T.rs_node = s->alloc<int>(64);
@@ -486,6 +486,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
             s->th[t].chunks.push_back(TorbChunk{0, m, 1, 0});  // iid default: one segment past cut 0
         }
         s->tasks_d = s->alloc<Task>(n_tasks);
+        for (int t = 0; t < n_tasks; ++t) s->tasks_h[t].slots = s->tasks_d[t].slot_tab;
         HIP_OK(hipMemcpyAsync(s->tasks_d, s->tasks_h.data(), sizeof(Task) * n_tasks, hipMemcpyHostToDevice, s->stream));
         s->chunk_ptrs_d = s->alloc<TorbChunk*>(n_tasks);
         s->n_chunks_d = s->alloc<int>(n_tasks);
@@ -701,9 +702,16 @@ int tgbm_task_swap(tgbm_session* s, int a, int b) {
         if (a == b) return;
         HIP_OK(hipStreamSynchronize(s->stream));
         Task ta = s->read_task(a), tb = s->read_task(b);
+        // the slot table pointer and the mapped completion flag belong to the index, not the task
+        auto keep_index = [](Task& x, Task& y) {
+            std::swap(x.slots, y.slots);
+            std::swap(x.host_flag, y.host_flag);
+        };
+        keep_index(ta, tb);
         HIP_OK(hipMemcpy(s->tasks_d + a, &tb, sizeof(Task), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->tasks_d + b, &ta, sizeof(Task), hipMemcpyHostToDevice));
         std::swap(s->tasks_h[a], s->tasks_h[b]);
+        keep_index(s->tasks_h[a], s->tasks_h[b]);
         std::swap(s->th[a], s->th[b]);
         s->upload_chunks(a);
         s->upload_chunks(b);
@@ -849,7 +857,9 @@ int tgbm_task_get_grad(tgbm_session* s, int task, double* g, double* h) {
         const int m = s->th[task].n_rows;
         HIP_OK(hipStreamSynchronize(s->stream));
         if (g) HIP_OK(hipMemcpy(g, T.g, sizeof(double) * m, hipMemcpyDeviceToHost));
-        if (h) HIP_OK(hipMemcpy(h, T.h, sizeof(double) * m, hipMemcpyDeviceToHost));
+        // h == 1 sessions (MSE without weights or GOSS) never store h on the device
+        if (h && s->hconst) std::fill(h, h + m, 1.0);
+        else if (h) HIP_OK(hipMemcpy(h, T.h, sizeof(double) * m, hipMemcpyDeviceToHost));
     });
 }
 
