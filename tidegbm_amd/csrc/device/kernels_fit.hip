@@ -611,6 +611,8 @@ __global__ __launch_bounds__(512) void k_hist_row(Task* tasks, Shared S, int roo
 // gb % 8; blocks go out round-robin over the 8 XCDs, and the blocks of XCD x take exactly the
 // items with item % 8 == x: the 128-byte rows a row block's feature blocks share come into that
 // XCD's L2 once.
+constexpr int SPEC_ITEM_ROWS = 8192;   // rows of one multi-leaf child-build item at most
+
 template <bool HV>
 __global__ __launch_bounds__(HV ? 512 : 1024) void k_hist_spec(Task* tasks, Shared S, int n_tasks) {
     constexpr int NT = HV ? 512 : 1024, FB = HV ? 16 : 32;
@@ -2422,10 +2424,13 @@ __global__ __launch_bounds__(256) void k_spec_select(Task* tasks, Shared S, Grow
     // rows per histogram tile, the same for every slot: the step's smaller children in at most
     // rb_target row blocks (so the step's items fit the resident blocks: each slot rounds its
     // row blocks up, so the tile is sized for rb_target - n_chosen of them), at least min_per
-    // rows, at most MAX_BLOCK_ROWS, and at most step_parts blocks over all slots
+    // rows and at most SPEC_ITEM_ROWS (a launch's last items then end close together: 64-fit
+    // walk-forward batch 1.25 s at 8,192 rows, 1.32 s at 4,096 and at 16,384, 1.50 s at 32,768),
+    // yet enough for at most step_parts blocks over all slots, and at most MAX_BLOCK_ROWS
     const int total_small = wave_sum_i(small);
     const int rb = max(1, rb_target - n_chosen);
     int per = max(min_per, (total_small + rb - 1) / rb);
+    per = min(per, SPEC_ITEM_ROWS);
     per = max(per, (total_small + step_parts - n_chosen - 1) / (step_parts - n_chosen));
     per = min((per + 63) & ~63, MAX_BLOCK_ROWS);
     if (lane < n_chosen) gx = max(1, (small + per - 1) / per);
