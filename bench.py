@@ -8,10 +8,12 @@ A step is one boosting iteration (one tree).  `--warmup W` trees are grown untim
 `--steps K` trees are timed from the per-tree wall clock of the native loop (every tree ends with
 a device synchronization).  Inputs are resident in HBM before the timed trees start.
 
-Roofline: the dominant kernel by device time is the child histogram build (k_hist_row, one per
-split).  A build over m rows reads m*F bin bytes + m*8 packed-gradient bytes + m*4 row-index
-bytes and writes F*n_bins*24 histogram bytes (the algorithmic bytes, with the reference's 24-byte
-HBin); HIP events around every launch on the session stream give the average duration.  The
+Roofline: the dominant kernel by device time is the child histogram build (k_hist_spec, one launch
+per multi-leaf step, building the smaller child of each of the step's splits).  A child of m rows
+costs m*F bin bytes + m*8 packed-gradient bytes + m*4 row-index bytes read and F*n_bins*24
+histogram bytes written (the algorithmic bytes, with the reference's 24-byte HBin), summed over
+the splits the finished trees keep; HIP events around every launch on the session stream give
+the average duration.  The
 root build (k_hist_root, m = n, no row index) is reported beside it.  `traffic` comes from the
 FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
 
@@ -176,8 +178,8 @@ def run_plain(args, p):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic.get("k_hist_row"),
-            "kernel": "child histogram build (k_hist_row), smaller child of every split, 128 features",
+            "traffic": traffic.get("k_hist_spec"),
+            "kernel": "child histogram build (k_hist_spec), the smaller children of a multi-leaf step, 128 features",
             "bytes_per_launch": round(child_bytes),
             "avg_launch_ms": round(child_ms, 5),
             "launches": n5,
@@ -248,6 +250,8 @@ def run_walkforward(args, p):
     log(f"[walk-forward] {len(tasks)} fits, rows {min(rows)}..{max(rows)}, {args.wf_trees} trees each: batched")
     mb = t.fit_batch(p.X, p.ts, rows, ys, schedule="batched", **kw)
     sb, bb = mb[0].fit_times()
+    if args.wf_batched_only:   # one schedule per process (kernel-trace profiles of the batched launches)
+        return {"batched_s": round(bb, 3), "setup_s": round(sb, 3), "fits": len(tasks)}
     log(f"[walk-forward] batched boost {bb:.3f}s (setup {sb:.3f}s); sequential")
     ms = t.fit_batch(p.X, p.ts, rows, ys, schedule="sequential", **kw)
     ss, bs = ms[0].fit_times()
@@ -462,6 +466,8 @@ def main():
     ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
     ap.add_argument("--serve-slots", type=int, default=32, help="concurrent refits when batched")
     ap.add_argument("--wf-trees", type=int, default=100, help="trees per walk-forward fit")
+    ap.add_argument("--wf-batched-only", action="store_true",
+                    help="walk-forward workload: run the batched schedule only (no sequential fits, no speedup)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (HBM counter calibration)")
     args = ap.parse_args()
