@@ -18,8 +18,11 @@ root build (k_hist_root, m = n, no row index) is reported beside it.  `traffic` 
 FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
 
 The root build is timed twice: as the fit runs (its 128 MB of bins may then sit in the 256 MB
-Infinity Cache in front of HBM) and with a 512 MiB buffer read and rewritten before every root
-build (profile mode 2), so that its inputs come from HBM; roofline_root reports the latter.
+Infinity Cache in front of HBM) and with a 512 MiB buffer read before every root build (profile
+mode 2), so that its inputs come from HBM; roofline_root reports the latter.  The sweep only
+reads: a sweep that rewrote the buffer would leave dirty lines whose write-back the root build
+would pay for (a plain 128 MiB read: 21.2 us warm, 24.9 us after the read sweep, 45.0 us after a
+rewriting sweep).
 
 CPU baseline: the reference core (oracle/_ref, built from the reference's own C++ sources) on
 the host's cores, same panel, timed as (fit of 1+C trees) - (fit of 1 tree) over C trees, after

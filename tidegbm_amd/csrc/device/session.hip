@@ -969,14 +969,19 @@ int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other
     });
 }
 
-// Root of every task's tree: quantize, zero the node histograms, build and scan the root.  The
-// first split is selected by the first split step (its partition launch in device-driven mode).
-// Reads and rewrites every word of a buffer (evicts earlier data from the caches in front of HBM).
+// Reads every word of a zero-filled buffer twice the size of the Infinity Cache, so earlier data
+// leaves the caches in front of HBM.  Nothing is written (the buffer stays zero, so the guarded
+// store never runs): the caches keep no dirty lines whose write-back would load the next kernel's
+// reads.
 __global__ __launch_bounds__(256) void k_flush_cache(unsigned long long* buf, size_t words) {
+    unsigned long long acc = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
-        buf[i] = buf[i] + 1ull;
+        acc |= buf[i];
+    if (acc != 0) buf[0] = 0;
 }
 
+// Root of every task's tree: quantize, zero the node histograms, build and scan the root.  The
+// first split is selected by the first split step (its partition launch in device-driven mode).
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     spec_resolve(s);
     const GrowP P = s->growp(p);
