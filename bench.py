@@ -13,9 +13,9 @@ per multi-leaf step, building the smaller child of each of the step's splits).  
 costs m*F bin bytes + m*8 packed-gradient bytes + m*4 row-index bytes read and F*n_bins*24
 histogram bytes written (the algorithmic bytes, with the reference's 24-byte HBin), summed over
 the splits the finished trees keep; HIP events around every launch on the session stream give
-the average duration.  The
-root build (k_hist_root, m = n, no row index) is reported beside it.  `traffic` comes from the
-FETCH_SIZE / WRITE_SIZE counter runs summarized in profiles/hist_traffic.json when present.
+the average duration.  The root build (k_hist_root, m = n, no row index) is reported beside it.
+`traffic` comes from the FETCH_SIZE / WRITE_SIZE counter runs summarized in
+profiles/hist_traffic.json when present.
 
 The root build is timed twice: as the fit runs (its 128 MB of bins may then sit in the 256 MB
 Infinity Cache in front of HBM) and with a 512 MiB buffer read before every root build (profile
@@ -467,7 +467,7 @@ def main():
     ap.add_argument("--rank-trees", type=int, default=100, help="trees of the ranking workload")
     ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the serving request stream")
     ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
-    ap.add_argument("--serve-slots", type=int, default=32, help="concurrent refits when batched")
+    ap.add_argument("--serve-slots", type=int, default=64, help="concurrent refits when batched")
     ap.add_argument("--wf-trees", type=int, default=100, help="trees per walk-forward fit")
     ap.add_argument("--wf-batched-only", action="store_true",
                     help="walk-forward workload: run the batched schedule only (no sequential fits, no speedup)")

@@ -111,6 +111,8 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     // panel bins + refit slots in one session
     std::vector<int> caps(K_, refit_max_rows);
     dev_check(tgbm_session_create(&sess_, n, F, m0.n_bins, K_, caps.data(), cfg_.max_leaves, 1));
+    // refit trees grow several leaves per step, like every device-driven fit
+    dev_check(tgbm_session_set_spec(sess_, cfg_.spec_width > 1 ? cfg_.spec_width : 0));
     if (!cfg_.monotone.empty()) dev_check(tgbm_session_set_monotone(sess_, cfg_.monotone.data()));
     bin_on_device(sess_, X, m0.edges, m0.is_cat, m0.cat_values);
     has_cat_ = m0.has_cat();
