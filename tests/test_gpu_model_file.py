@@ -103,3 +103,40 @@ def test_reference_signature_fit_and_binned_predict():
     finally:
         if h:
             L.tide_free(h)
+
+
+def test_device_forest_rejects_links_that_do_not_point_forward():
+    """The device forest builder refuses a split whose child is itself or an earlier node (the
+    growers append children after their parent, reference tree.cpp:447-453), so no device walk
+    can loop; a well-formed tree of the same shape is accepted."""
+    from pathlib import Path
+    hip = ctypes.CDLL(str(Path(t.__file__).resolve().parent / "lib" / "libtgbm_hip.so"))
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    hip.tgbm_forest_create.argtypes = [P, I, I, I, P, P, P, P, D, P, P, P]
+    hip.tgbm_forest_destroy.argtypes = [P]
+    hip.tgbm_forest_destroy.restype = None
+    hip.tgbm_last_error.restype = ctypes.c_char_p
+    F = 2
+    E = np.zeros((F, 256))
+    E[:, 0] = 0.0
+    ne = np.ones(F, np.int32)
+    vals = np.array([0.0, -1.0, 1.0])
+    lr = np.array([0.1])
+
+    def create(left, right):
+        # {is_leaf, split_type, feature, feature2, thr, left, right, missing_go_left, n_samples, depth}
+        ints = np.array([[0, 0, 0, -1, 1, left, right, 1, 10, 0],
+                         [1, 0, -1, -1, -1, -1, -1, 1, 5, 1],
+                         [1, 0, -1, -1, -1, -1, -1, 1, 5, 1]], np.int32)
+        counts = np.array([3], np.int32)
+        fo = P()
+        rc = hip.tgbm_forest_create(ctypes.byref(fo), F, 3, 1, counts.ctypes.data, ints.ctypes.data, vals.ctypes.data,
+                                    lr.ctypes.data, 0.0, E.ctypes.data, ne.ctypes.data, None)
+        if rc == 0:
+            hip.tgbm_forest_destroy(fo)
+        return rc
+
+    assert create(1, 2) == 0, hip.tgbm_last_error()
+    for left, right in [(0, 2), (1, 0)]:
+        assert create(left, right) != 0
+        assert b"malformed tree node" in hip.tgbm_last_error()

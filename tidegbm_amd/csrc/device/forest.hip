@@ -136,7 +136,9 @@ static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node
             const bool leaf = v[0] != 0;
             FNode nd{0, 0, 0, 0};
             if (!leaf) {
-                if (v[2] < 0 || v[2] >= n_features || v[5] < 0 || v[6] < 0 || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
+                // children come after their parent (every grower appends them, tree.cpp:447-453), so
+                // every walk ends; a link back or to itself would loop the device walks
+                if (v[2] < 0 || v[2] >= n_features || v[5] <= k || v[6] <= k || v[5] >= tree_nodes[t] || v[6] >= tree_nodes[t])
                     throw std::runtime_error("malformed tree node");
                 if (v[1] < 0 || v[1] > 3) throw std::runtime_error("unknown split type");
                 const int f2 = v[1] == 1 ? v[3] : 0;
