@@ -121,6 +121,7 @@ struct tgbm_session {
     std::map<std::vector<long long>, CachedGraph> graphs;
     long long graph_clock = 0;
     bool use_graphs = true;
+    bool spec_graph = false;    // multi-leaf loops from a captured graph instead of direct launches
     // multi-leaf growth (one-task sessions with max_leaves <= 64): leaves per step (0: off), the
     // node capacity and partial slots it was allocated with, the steps of the captured loop (the
     // most a recent tree needed, plus one), whether the current tree grows this way, and a pinned
@@ -396,6 +397,7 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->max_nodes = 2 * max_leaves - 1;
         s->hconst = h_const != 0;
         if (const char* e = std::getenv("TGBM_NO_GRAPH")) s->use_graphs = !(e[0] == '1');
+        if (const char* e = std::getenv("TGBM_SPEC_GRAPH")) s->spec_graph = e[0] == '1';
         init_kernel_attributes();
         {
             int dev = 0, cu = 0;
@@ -1058,12 +1060,13 @@ static bool spec_done(tgbm_session* s, int* steps) {
 static void spec_step(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
     const int ms = s->launch_rows();
     // child builds: a fixed grid of resident blocks takes the step's (row block, feature block)
-    // items; tiles of at least 2,048 rows, sized so the items about fill those blocks
+    // items; tiles of at least 1,024 rows, sized so the items about fill those blocks (1M x 128
+    // fit: 1,290 trees/s at 1,024, 1,280 at 2,048, 1,270 at 512, 1,213 at 4,096)
     int blocks, fbs;
     spec_child_shape(s->hconst, s->n_cu, s->fp, blocks, fbs);
     EventPair e0, e1, e2, e3;
     if (s->timing) s->tic(6, 0.0, e0);   // the step's selection (timer 6)
-    launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 2048, std::max(1, blocks / (fbs * s->an)),
+    launch_spec_select(s->stream, s->S, s->td(), s->an, P, s->spec_width, 1024, std::max(1, blocks / (fbs * s->an)),
                        s->step_parts);
     if (s->timing) s->toc(e0);
     if (s->timing) s->tic(1, 0.0, e1);
@@ -1099,12 +1102,15 @@ static void spec_log_rows(tgbm_session* s) {
     s->log_rows[6].push_back(splits);   // splits the selections made
 }
 
-// Multi-leaf growth of the current tree.  The captured loop holds spec_k steps (the most the last
-// tree needed, plus one) and is only launched here: the tree end is queued behind it and then
-// waits for it (tree_end_enqueue), so the device never waits for the host in between.  A tree
-// that needs more steps continues with single steps (every step makes at least one split the tree
-// keeps, so at most max_leaves - 1 steps end any tree).  Timed and graph-less growth runs step by
-// step, reading the tree's state after each.
+// Multi-leaf growth of the current tree.  The loop of spec_k steps (the most the last 32 trees
+// needed, plus one) is only launched here: the tree end is queued behind it and then waits for it
+// (tree_end_enqueue), so the device never waits for the host in between.  The steps go out as
+// direct launches: a captured graph of them was slower (1M x 128 fit: 1,282 against 1,225
+// trees/s), its launch call holding the host thread for most of the graph's run, with the device
+// idle between some of its kernels; TGBM_SPEC_GRAPH=1 selects the graph.  A tree that needs more
+// steps continues with single steps (every step makes at least one split the tree keeps, so at
+// most max_leaves - 1 steps end any tree).  Timed and graph-less growth runs step by step,
+// reading the tree's state after each.
 static void grow_spec(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p) {
     const int limit = std::max(1, p->max_leaves - 1);
     int steps = 0;
@@ -1117,6 +1123,13 @@ static void grow_spec(tgbm_session* s, const GrowP& P, const tgbm_grow_params* p
         }
     } else {
         const int K = std::min(limit, s->spec_k);
+        if (!s->spec_graph) {
+            for (int k = s->a0; k < s->a0 + s->an; ++k) s->flag_h[2 * k] = 0;
+            for (int k = 0; k < K; ++k) spec_step(s, P, p);
+            s->spec_launched = true;
+            s->spec_k_used = K;
+            return;
+        }
         const std::vector<long long> key{s->launch_rows(), p->max_leaves, p->hist_mode, s->a0, s->an, -K};
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
