@@ -31,6 +31,8 @@ constexpr int SERVE_TREES_PER_CHUNK = 64;   // trees staged in LDS at a time by 
 struct tgbm_forest {
     int F = 0, fp = 0, nb = 255, n_trees = 0, n_nodes = 0;
     int max_chunk_nodes = 1;            // most nodes in any SERVE_TREES_PER_CHUNK consecutive trees
+    int max_depth = 0;                  // deepest node of any tree (root depth 0)
+    bool numeric_only = true;           // every split is a numeric threshold split
     std::vector<int> toff_h;            // host copy of tree_off
     double init = 0.0;
     tgbm::FNode* nodes = nullptr;

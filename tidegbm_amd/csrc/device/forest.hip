@@ -115,6 +115,8 @@ struct HostNodes {
     std::vector<unsigned long long> cmask;   // 4 words per categorical node, in node order
     std::vector<int> tree_cats;    // categorical nodes per tree
     int total = 0;
+    int max_depth = 0;             // deepest node of any tree (the root at depth 0)
+    bool numeric_only = true;      // no pair or categorical node
 };
 
 // cat_masks: 4 words per node (all nodes of all trees), or null when no node is categorical;
@@ -129,8 +131,10 @@ static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node
     h.flags.resize(std::max(h.total, 1));
     h.tree_nodes.assign(tree_nodes, tree_nodes + n_trees);
     int off = 0;
+    std::vector<int> depth;
     for (int t = 0; t < n_trees; ++t) {
         if (tree_nodes[t] < 1 || tree_nodes[t] > 255) throw std::runtime_error("trees must have 1..255 nodes");
+        depth.assign(tree_nodes[t], 0);
         for (int k = 0; k < tree_nodes[t]; ++k) {
             const int* v = node_ints + (size_t)(off + k) * TGBM_NODE_INTS;
             const bool leaf = v[0] != 0;
@@ -154,6 +158,12 @@ static HostNodes build_nodes(int n_trees, const int* tree_nodes, const int* node
                 }
                 nd.left = base + off + v[5];
                 nd.right = base + off + v[6];
+                // children come after their parent, so a node's depth (its longest path from the
+                // root) is final when it is reached
+                depth[v[5]] = std::max(depth[v[5]], depth[k] + 1);
+                depth[v[6]] = std::max(depth[v[6]], depth[k] + 1);
+                h.max_depth = std::max(h.max_depth, depth[k] + 1);
+                if (v[1] != 0) h.numeric_only = false;
             }
             h.nodes[off + k] = nd;
             h.flags[off + k] = (unsigned char)((leaf ? 1 : 0) | ((v[1] & 3) << 1) | ((v[7] ? 1 : 0) << 3));
@@ -210,6 +220,9 @@ static tgbm_forest* assemble(const tgbm_forest* base, int keep_trees, int keep_n
             FHIP(hipMemcpy(f->cmask + (size_t)4 * keep_cat, h.cmask.data(), sizeof(unsigned long long) * h.cmask.size(),
                            hipMemcpyHostToDevice));
     }
+    // kept trees of the base are covered by its whole-forest figures (conservative)
+    f->max_depth = std::max(base && keep_trees > 0 ? base->max_depth : 0, h.max_depth);
+    f->numeric_only = (!base || keep_trees == 0 || base->numeric_only) && h.numeric_only;
     f->max_chunk_nodes = 1;
     for (int t0 = 0; t0 < f->n_trees; t0 += SERVE_TREES_PER_CHUNK) {
         const int t1 = std::min(f->n_trees, t0 + SERVE_TREES_PER_CHUNK);

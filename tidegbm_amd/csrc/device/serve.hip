@@ -11,7 +11,10 @@
 // LDS, then walks the ensemble SERVE_TREES_PER_CHUNK trees at a time: the chunk's packed nodes
 // are copied to LDS, SERVE_LANES threads per row traverse different trees of the chunk
 // (LDS-only dependent loads) and record leaf ids, and one thread per row then adds the chunk's
-// leaf values in tree order.
+// leaf values in tree order.  A row's bins sit at a stride of an odd number of 4-byte words, so
+// the 64 rows of a wave read 64 different LDS banks at every node.  Ensembles of numeric splits
+// only walk SERVE_ILP trees per thread at once, each for the ensemble's depth (a leaf keeps its
+// id), so the dependent LDS loads of those trees overlap.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -42,6 +45,7 @@ namespace {
 
 constexpr int SERVE_ROWS = 64;
 constexpr int SERVE_LANES = 4;   // threads per row (256-thread blocks)
+constexpr int SERVE_ILP = 4;     // trees one thread walks together (numeric ensembles)
 constexpr int SERVE_TC = SERVE_TREES_PER_CHUNK;
 constexpr size_t SERVE_MAX_LDS = 152 * 1024;   // dynamic LDS opt-in (below 160 KiB: the kernel has static LDS too)
 
@@ -55,14 +59,18 @@ struct SReq {
     const int* rows;             // row ids into bins (null: rows 0..n-1)
     double* out;
     double init;
-    int n_trees, nb, fp, n, blk0, pad;
+    int n_trees, nb, fp, n, blk0;
+    int depth;    // > 0: every split numeric, no node deeper than this; 0: general walk
 };
 
 __host__ __device__ constexpr size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// LDS bytes per staged row: fp (a multiple of 16) plus one word, an odd number of words.
+__host__ __device__ constexpr int serve_row_bytes(int fp) { return fp + 4; }
+
 // LDS bytes of k_serve for rows of fp bytes and chunks of at most chunk_nodes nodes.
 size_t serve_lds(int fp, int chunk_nodes) {
-    size_t b = (size_t)SERVE_ROWS * fp;                 // bins
+    size_t b = (size_t)SERVE_ROWS * serve_row_bytes(fp);   // bins
     b += (size_t)SERVE_TC * SERVE_ROWS;                  // leaf ids
     b = align_up(b + sizeof(int) * (SERVE_TC + 1), 8);   // chunk tree offsets
     return b + sizeof(SNode) * (size_t)chunk_nodes;
@@ -84,19 +92,26 @@ __global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, in
     const SReq R = reqs[s_req];
     const int r0 = ((int)blockIdx.x - R.blk0) * SERVE_ROWS;
     const int nr = min(SERVE_ROWS, R.n - r0);
+    const int max_rb = serve_row_bytes(max_fp);
     unsigned char* lbins = smem;
-    unsigned char* lleaf = smem + (size_t)SERVE_ROWS * max_fp;
+    unsigned char* lleaf = smem + (size_t)SERVE_ROWS * max_rb;
     int* loff = (int*)(lleaf + SERVE_TC * SERVE_ROWS);
-    SNode* lnodes = (SNode*)(smem + align_up((size_t)SERVE_ROWS * max_fp + SERVE_TC * SERVE_ROWS +
+    SNode* lnodes = (SNode*)(smem + align_up((size_t)SERVE_ROWS * max_rb + SERVE_TC * SERVE_ROWS +
                                                  sizeof(int) * (SERVE_TC + 1), 8));
-    const int q = R.fp / 16;   // 16-byte words per row
+    const int q = R.fp / 16;                     // 16-byte words per row in HBM
+    const int rw = serve_row_bytes(R.fp) / 4;    // 4-byte words per row in LDS (odd)
     for (int j = threadIdx.x; j < nr * q; j += blockDim.x) {
         const int r = j / q, c = j % q;
         const int row = R.rows ? R.rows[r0 + r] : r0 + r;
-        ((uint4*)lbins)[r * q + c] = ((const uint4*)(R.bins + (size_t)row * R.fp))[c];
+        const uint4 v = ((const uint4*)(R.bins + (size_t)row * R.fp))[c];
+        unsigned int* d = (unsigned int*)lbins + r * rw + 4 * c;
+        d[0] = v.x;
+        d[1] = v.y;
+        d[2] = v.z;
+        d[3] = v.w;
     }
     const int r = threadIdx.x % SERVE_ROWS, g = threadIdx.x / SERVE_ROWS;
-    const unsigned char* row = lbins + r * R.fp;
+    const unsigned char* row = lbins + r * (4 * rw);
     double s = R.init;
     for (int t0 = 0; t0 < R.n_trees; t0 += SERVE_TC) {
         const int nt = min(SERVE_TC, R.n_trees - t0);
@@ -107,7 +122,41 @@ __global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, in
         for (int j = threadIdx.x; j < nn; j += blockDim.x)
             ((uint2*)lnodes)[j] = ((const uint2*)(R.nodes + base))[j];
         __syncthreads();
-        if (r < nr) {
+        if (r < nr && R.depth > 0) {
+            // numeric splits only: SERVE_ILP trees at once, R.depth levels each
+            for (int t = g; t < nt; t += SERVE_LANES * SERVE_ILP) {
+                const uint2* tn[SERVE_ILP];
+                int k[SERVE_ILP];
+#pragma unroll
+                for (int u = 0; u < SERVE_ILP; ++u) {
+                    const int tt = t + u * SERVE_LANES;
+                    tn[u] = (const uint2*)(lnodes + loff[tt < nt ? tt : t]);
+                    k[u] = 0;
+                }
+                for (int d = 0; d < R.depth; ++d) {
+                    uint2 nd[SERVE_ILP];
+#pragma unroll
+                    for (int u = 0; u < SERVE_ILP; ++u) nd[u] = tn[u][k[u]];
+                    bool more = false;
+#pragma unroll
+                    for (int u = 0; u < SERVE_ILP; ++u) {
+                        // x = f | f2 << 16; y = thr | left << 8 | right << 16 | flags << 24
+                        const unsigned int fl = nd[u].y >> 24;
+                        const int b = row[nd[u].x & 0xffff];
+                        const bool left = b == 0 ? ((fl >> 3) & 1) : (b <= (int)(nd[u].y & 0xff));
+                        const int nk = (int)((left ? nd[u].y >> 8 : nd[u].y >> 16) & 0xff);
+                        k[u] = (fl & 1) ? k[u] : nk;
+                        more |= !(fl & 1);
+                    }
+                    if (!__any(more)) break;   // every walk of the wave is at a leaf
+                }
+#pragma unroll
+                for (int u = 0; u < SERVE_ILP; ++u) {
+                    const int tt = t + u * SERVE_LANES;
+                    if (tt < nt) lleaf[tt * SERVE_ROWS + r] = (unsigned char)k[u];
+                }
+            }
+        } else if (r < nr) {
             for (int t = g; t < nt; t += SERVE_LANES) {
                 const SNode* tn = lnodes + loff[t];
                 int k = 0;
@@ -255,7 +304,7 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
             R.fp = f->fp;
             R.n = it.n;
             R.blk0 = blocks;
-            R.pad = 0;
+            R.depth = f->numeric_only ? std::max(f->max_depth, 1) : 0;
             blocks += (it.n + SERVE_ROWS - 1) / SERVE_ROWS;
             R.out = (double*)(p->dev + off[i]);
             size_t o = in_off[i];
@@ -328,7 +377,7 @@ int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f) {
         R.fp = f->fp;
         R.n = n;
         R.blk0 = 0;
-        R.pad = 0;
+        R.depth = f->numeric_only ? std::max(f->max_depth, 1) : 0;
         SHIP(hipMemcpyAsync(scratch, &R, sizeof R, hipMemcpyHostToDevice, st));
         const int blocks = (n + SERVE_ROWS - 1) / SERVE_ROWS;
         if (blocks > 0)
