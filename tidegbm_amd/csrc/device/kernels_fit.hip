@@ -702,10 +702,11 @@ __device__ inline void root_load_bins(uint4 (&b)[FB], const unsigned char* colb,
 }
 
 // The passes of one root block (k_hist_root); RAW: packed words added modulo 2^64 (h == 1, the
-// task's root counts cached).
+// task's root counts cached).  Rows are counted from the 16-row boundary at or below the task's
+// first row: the first `head` of the `count` rows (and any past the last) add zero words.
 template <int K, bool HV, bool RAW>
 __device__ __forceinline__ void hist_root_block(const Task& T, const Shared& S, unsigned long long* lds, int count,
-                                                int bx, int r0, int fr0, int fr1) {
+                                                int head, int bx, int r0, int fr0, int fr1) {
     constexpr int FB = (HV ? 32 : 64) / K;
     constexpr int WORDS = FB * NBIN;
     constexpr int SPT = WORDS * (HV ? 2 : 1) / ROOT_THREADS;
@@ -715,11 +716,11 @@ __device__ __forceinline__ void hist_root_block(const Task& T, const Shared& S, 
     const int lane = threadIdx.x & 63;
     const int rep = lane % K;
     const int row = r0 + threadIdx.x * ROOT_RPL;
-    const bool full = row + ROOT_RPL <= count;
+    const bool full = head == 0 && row + ROOT_RPL <= count;
     // lanes wholly past the end read the last row group's bins and add zeros; a lane across the
     // end reads past it (inside the column padding) and adds zeros for those rows
     const int ld_row = row < count ? row : (count - 1) & ~(ROOT_RPL - 1);
-    const unsigned char* colb = S.bins_col + T.row0;  // row0 is a multiple of 16 here (launch_hist)
+    const unsigned char* colb = S.bins_col + (T.row0 - head);   // 16-byte aligned column loads
     unsigned long long pg[ROOT_RPL], ph[HV ? ROOT_RPL : 1];
     if (full) {
         const uint4* pp = (const uint4*)(T.pk + row);
@@ -741,8 +742,9 @@ __device__ __forceinline__ void hist_root_block(const Task& T, const Shared& S, 
     } else {
 #pragma unroll
         for (int t = 0; t < ROOT_RPL; ++t) {
-            pg[t] = row + t < count ? gbl(T.pk)[row + t] : 0ull;
-            if (HV) ph[t] = row + t < count ? gbl(T.qh)[row + t] : 0ull;
+            const bool in = row + t >= head && row + t < count;
+            pg[t] = in ? gbl(T.pk)[row + t - head] : 0ull;
+            if (HV) ph[t] = in ? gbl(T.qh)[row + t - head] : 0ull;
         }
     }
     uint4 bc[FB], bn[FB];
@@ -885,16 +887,19 @@ __global__ __launch_bounds__(ROOT_THREADS) void k_hist_root(Task* tasks, Shared 
     // A raw pass needs no clearing: each pass stores the difference of its bin's copy sums from
     // the previous pass's.
     const bool raw = !HV && T.cnt_n == count && T.cnt_row0 == T.row0;
+    // rows from the 16-row boundary at or below row0: the column loads stay 16-byte aligned
+    const int head = T.row0 & (ROOT_RPL - 1);
+    const int vcount = count + head;
     if (bx == 0 && by == 0 && threadIdx.x == 0) {
-        tasks[blockIdx.z].hist_parts = (count + ROOT_ROWS - 1) / ROOT_ROWS;
+        tasks[blockIdx.z].hist_parts = (vcount + ROOT_ROWS - 1) / ROOT_ROWS;
         tasks[blockIdx.z].hist_wide = HV ? 1 : raw ? 2 : 3;
     }
     const int r0 = bx * ROOT_ROWS;
     const int fr0 = by * feats_per_block;
     const int fr1 = min(S.n_features, fr0 + feats_per_block);
-    if (r0 >= count || fr0 >= fr1) return;
-    if (raw) hist_root_block<K, false, true>(T, S, lds, count, bx, r0, fr0, fr1);
-    else hist_root_block<K, HV, false>(T, S, lds, count, bx, r0, fr0, fr1);
+    if (r0 >= vcount || fr0 >= fr1) return;
+    if (raw) hist_root_block<K, false, true>(T, S, lds, vcount, head, bx, r0, fr0, fr1);
+    else hist_root_block<K, HV, false>(T, S, lds, vcount, head, bx, r0, fr0, fr1);
 }
 
 // Launch with optional start/stop events recorded at the kernel's own boundaries (AMD extended
@@ -944,12 +949,14 @@ void launch_hist(hipStream_t st, const Shared& S, Task* tasks, int n_tasks, bool
         int xcd = (rv >> 3) & 1;
         if (rv == 0) {
             // smallest power of two in [4, 64] with at most ~one block per CU (256 CUs)
-            const long long want = (long long)((max_count + ROOT_ROWS - 1) / ROOT_ROWS) * n_tasks * S.n_features;
+            const long long want = (long long)((max_count + ROOT_RPL - 1 + ROOT_ROWS - 1) / ROOT_ROWS) * n_tasks * S.n_features;
             fpb = 4;
             while (fpb < 64 && want > 256LL * fpb) fpb *= 2;
             xcd = 1;
         }
-        dim3 grid((max_count + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
+        // a window's rows count from the 16-row boundary at or below its first row (up to 15 more)
+        const int vmax = max_count + ROOT_RPL - 1;
+        dim3 grid((vmax + ROOT_ROWS - 1) / ROOT_ROWS, (S.n_features + fpb - 1) / fpb, n_tasks);
         if (hconst) TIMED_LAUNCH((k_hist_root<16, false>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
         else TIMED_LAUNCH((k_hist_root<16, true>), grid, dim3(ROOT_THREADS), 0, st, e0, e1, tasks, S, fpb, xcd);
         return;

@@ -275,12 +275,6 @@ struct tgbm_session {
             if (th[t].has_sample) return false;
         return true;
     }
-    // the column root kernel loads 16 rows per 128-bit access: every window must start aligned
-    bool all_aligned() const {
-        for (int t = a0; t < a0 + an; ++t)
-            if (th[t].row0 % 16) return false;
-        return true;
-    }
     // Upload task t's TORB chunk list and its entry in the per-task pointer table.
     void upload_chunks(int t) {
         TaskHost& H = th[t];
@@ -1006,7 +1000,7 @@ static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool ene
                                tgbm_session::kFlushBytes / 8);
         s->kernel_events(0, bytes, ep);
     }
-    launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous() && s->all_aligned(), s->hconst, ms,
+    launch_hist(s->stream, s->S, s->td(), s->an, true, s->all_contiguous(), s->hconst, ms,
                 p->hist_mode, s->timing ? ep.a : nullptr, s->timing ? ep.b : nullptr);
     if (s->timing) s->events.push_back(ep);
     launch_scan(s->stream, s->S, s->td(), s->an, P, 0);
