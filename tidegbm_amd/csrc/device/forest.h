@@ -47,6 +47,7 @@ struct tgbm_forest {
     unsigned long long* cmask = nullptr;  // [n_cat][4] bin masks of categorical nodes
     int n_cat = 0;
     std::vector<int> cat_off_h;           // [n_trees + 1] first mask of each tree
+    void* block = nullptr;              // one device allocation holding every array above
     hipStream_t stream = nullptr;
     // staging for the single-call predict entry points
     double* Xd = nullptr;
@@ -56,9 +57,7 @@ struct tgbm_forest {
     int cap = 0;
     ~tgbm_forest() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : {(void*)nodes, (void*)flags, (void*)snodes, (void*)tree_off, (void*)values, (void*)roots,
-                        (void*)lr, (void*)edges, (void*)n_edges, (void*)Xd, (void*)bins, (void*)bins_col_dummy,
-                        (void*)outd, (void*)cmask})
+        for (void* p : {block, (void*)Xd, (void*)bins, (void*)bins_col_dummy, (void*)outd})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }

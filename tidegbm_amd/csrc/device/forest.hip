@@ -212,14 +212,6 @@ static tgbm_forest* assemble(const tgbm_forest* base, int keep_trees, int keep_n
     const int keep_cat = f->cat_off_h.back();
     for (int t = 0; t < n_new; ++t) f->cat_off_h.push_back(f->cat_off_h.back() + h.tree_cats[t]);
     f->n_cat = f->cat_off_h.back();
-    if (f->n_cat) {
-        FHIP(hipMalloc(&f->cmask, sizeof(unsigned long long) * 4 * f->n_cat));
-        if (base && keep_cat)
-            FHIP(hipMemcpy(f->cmask, base->cmask, sizeof(unsigned long long) * 4 * keep_cat, hipMemcpyDeviceToDevice));
-        if (!h.cmask.empty())
-            FHIP(hipMemcpy(f->cmask + (size_t)4 * keep_cat, h.cmask.data(), sizeof(unsigned long long) * h.cmask.size(),
-                           hipMemcpyHostToDevice));
-    }
     // kept trees of the base are covered by its whole-forest figures (conservative)
     f->max_depth = std::max(base && keep_trees > 0 ? base->max_depth : 0, h.max_depth);
     f->numeric_only = (!base || keep_trees == 0 || base->numeric_only) && h.numeric_only;
@@ -228,39 +220,69 @@ static tgbm_forest* assemble(const tgbm_forest* base, int keep_trees, int keep_n
         const int t1 = std::min(f->n_trees, t0 + SERVE_TREES_PER_CHUNK);
         f->max_chunk_nodes = std::max(f->max_chunk_nodes, f->toff_h[t1] - f->toff_h[t0]);
     }
-    const size_t NN = std::max(f->n_nodes, 1), NT = std::max(f->n_trees, 1);
-    FHIP(hipMalloc(&f->nodes, sizeof(FNode) * NN));
-    FHIP(hipMalloc(&f->flags, NN));
-    FHIP(hipMalloc(&f->snodes, sizeof(SNode) * NN));
-    FHIP(hipMalloc(&f->tree_off, sizeof(int) * (f->n_trees + 1)));
-    FHIP(hipMalloc(&f->values, sizeof(double) * NN));
-    FHIP(hipMalloc(&f->roots, sizeof(int) * NT));
-    FHIP(hipMalloc(&f->lr, sizeof(double) * NT));
-    FHIP(hipMalloc(&f->edges, sizeof(double) * (size_t)n_features * NBIN));
-    FHIP(hipMalloc(&f->n_edges, sizeof(int) * n_features));
-    if (base && keep_trees > 0) {
-        FHIP(hipMemcpy(f->nodes, base->nodes, sizeof(FNode) * keep_nodes, hipMemcpyDeviceToDevice));
-        FHIP(hipMemcpy(f->flags, base->flags, keep_nodes, hipMemcpyDeviceToDevice));
-        FHIP(hipMemcpy(f->snodes, base->snodes, sizeof(SNode) * keep_nodes, hipMemcpyDeviceToDevice));
-        FHIP(hipMemcpy(f->values, base->values, sizeof(double) * keep_nodes, hipMemcpyDeviceToDevice));
-        FHIP(hipMemcpy(f->lr, base->lr, sizeof(double) * keep_trees, hipMemcpyDeviceToDevice));
-    }
+    // Every array in one device allocation, filled by one upload of a host image of the same
+    // layout (the new trees' parts and the tables) followed by device copies of the base's kept
+    // parts: a forest costs one allocation and one free, which matters to a server publishing
+    // versions while other threads launch work.
+    const size_t NN = std::max(f->n_nodes, 1), NT = std::max(f->n_trees, 1), NF = (size_t)n_features;
+    size_t bytes = 0;
+    auto carve = [&](size_t b) {
+        const size_t o = bytes;
+        bytes = (bytes + b + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_nodes = carve(sizeof(FNode) * NN), o_snodes = carve(sizeof(SNode) * NN),
+                 o_values = carve(sizeof(double) * NN), o_lr = carve(sizeof(double) * NT),
+                 o_edges = carve(sizeof(double) * NF * NBIN), o_cmask = carve(sizeof(unsigned long long) * 4 * f->n_cat),
+                 o_toff = carve(sizeof(int) * (f->n_trees + 1)), o_roots = carve(sizeof(int) * NT),
+                 o_nedges = carve(sizeof(int) * NF), o_flags = carve(NN);
+    FHIP(hipMalloc(&f->block, bytes));
+    char* const d = (char*)f->block;
+    f->nodes = (FNode*)(d + o_nodes);
+    f->snodes = (SNode*)(d + o_snodes);
+    f->values = (double*)(d + o_values);
+    f->lr = (double*)(d + o_lr);
+    f->edges = (double*)(d + o_edges);
+    f->cmask = f->n_cat ? (unsigned long long*)(d + o_cmask) : nullptr;
+    f->tree_off = (int*)(d + o_toff);
+    f->roots = (int*)(d + o_roots);
+    f->n_edges = (int*)(d + o_nedges);
+    f->flags = (unsigned char*)(d + o_flags);
+    std::vector<char> img(bytes, 0);
+    auto put = [&](size_t o, const void* src, size_t b) {
+        if (b) std::memcpy(img.data() + o, src, b);
+    };
     if (h.total) {
-        FHIP(hipMemcpy(f->nodes + keep_nodes, h.nodes.data(), sizeof(FNode) * h.total, hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->flags + keep_nodes, h.flags.data(), h.total, hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->snodes + keep_nodes, h.snodes.data(), sizeof(SNode) * h.total, hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->values + keep_nodes, values, sizeof(double) * h.total, hipMemcpyHostToDevice));
+        put(o_nodes + sizeof(FNode) * keep_nodes, h.nodes.data(), sizeof(FNode) * h.total);
+        put(o_snodes + sizeof(SNode) * keep_nodes, h.snodes.data(), sizeof(SNode) * h.total);
+        put(o_values + sizeof(double) * keep_nodes, values, sizeof(double) * h.total);
+        put(o_flags + keep_nodes, h.flags.data(), h.total);
     }
-    if (n_new) FHIP(hipMemcpy(f->lr + keep_trees, lr, sizeof(double) * n_new, hipMemcpyHostToDevice));
-    FHIP(hipMemcpy(f->tree_off, f->toff_h.data(), sizeof(int) * f->toff_h.size(), hipMemcpyHostToDevice));
-    FHIP(hipMemcpy(f->roots, f->toff_h.data(), sizeof(int) * std::max(f->n_trees, 1), hipMemcpyHostToDevice));
-    if (base) {
-        FHIP(hipMemcpy(f->edges, base->edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyDeviceToDevice));
-        FHIP(hipMemcpy(f->n_edges, base->n_edges, sizeof(int) * n_features, hipMemcpyDeviceToDevice));
-    } else if (edges) {
-        FHIP(hipMemcpy(f->edges, edges, sizeof(double) * (size_t)n_features * NBIN, hipMemcpyHostToDevice));
-        FHIP(hipMemcpy(f->n_edges, n_edges, sizeof(int) * n_features, hipMemcpyHostToDevice));
+    if (n_new) put(o_lr + sizeof(double) * keep_trees, lr, sizeof(double) * n_new);
+    if (!h.cmask.empty()) put(o_cmask + sizeof(unsigned long long) * 4 * keep_cat, h.cmask.data(), sizeof(unsigned long long) * h.cmask.size());
+    put(o_toff, f->toff_h.data(), sizeof(int) * f->toff_h.size());
+    put(o_roots, f->toff_h.data(), sizeof(int) * std::max(f->n_trees, 1));
+    if (!base && edges) {
+        put(o_edges, edges, sizeof(double) * NF * NBIN);
+        put(o_nedges, n_edges, sizeof(int) * NF);
     }
+    FHIP(hipMemcpyAsync(d, img.data(), bytes, hipMemcpyHostToDevice, nullptr));
+    if (base) {   // the base's kept trees, edges and masks, copied after the image on the same stream
+        auto dd = [&](void* dst, const void* src, size_t b) {
+            if (b) FHIP(hipMemcpyAsync(dst, src, b, hipMemcpyDeviceToDevice, nullptr));
+        };
+        if (keep_trees > 0) {
+            dd(f->nodes, base->nodes, sizeof(FNode) * keep_nodes);
+            dd(f->flags, base->flags, keep_nodes);
+            dd(f->snodes, base->snodes, sizeof(SNode) * keep_nodes);
+            dd(f->values, base->values, sizeof(double) * keep_nodes);
+            dd(f->lr, base->lr, sizeof(double) * keep_trees);
+        }
+        if (keep_cat) dd(f->cmask, base->cmask, sizeof(unsigned long long) * 4 * keep_cat);
+        dd(f->edges, base->edges, sizeof(double) * NF * NBIN);
+        dd(f->n_edges, base->n_edges, sizeof(int) * NF);
+    }
+    FHIP(hipStreamSynchronize(nullptr));
     return guard.release();
 }
 

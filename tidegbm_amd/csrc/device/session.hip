@@ -691,26 +691,50 @@ int tgbm_task_bind(tgbm_session* s, int task, int row0, int n_rows) {
     });
 }
 
+// Exchange two task descriptors and their chunk-table entries on the device (in stream order,
+// behind the work that last wrote them); the slot table pointer and the mapped completion flag
+// belong to the index and stay.
+__global__ __launch_bounds__(64) void k_task_swap(Task* tasks, int a, int b, TorbChunk** chunk_ptrs, int* n_chunks) {
+    static_assert(sizeof(Task) % 4 == 0, "descriptor words");
+    Slot* const sa = tasks[a].slots;
+    Slot* const sb = tasks[b].slots;
+    int* const fa = tasks[a].host_flag;
+    int* const fb = tasks[b].host_flag;
+    __syncthreads();
+    int* const wa = (int*)(tasks + a);
+    int* const wb = (int*)(tasks + b);
+    for (int j = threadIdx.x; j < (int)(sizeof(Task) / 4); j += blockDim.x) {
+        const int x = wa[j], y = wb[j];
+        wa[j] = y;
+        wb[j] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    tasks[a].slots = sa;
+    tasks[b].slots = sb;
+    tasks[a].host_flag = fa;
+    tasks[b].host_flag = fb;
+    TorbChunk* const c = chunk_ptrs[a];
+    chunk_ptrs[a] = chunk_ptrs[b];
+    chunk_ptrs[b] = c;
+    const int n = n_chunks[a];
+    n_chunks[a] = n_chunks[b];
+    n_chunks[b] = n;
+}
+
 int tgbm_task_swap(tgbm_session* s, int a, int b) {
     return dev_guard([&] {
         spec_resolve(s);
         if (a < 0 || a >= s->T || b < 0 || b >= s->T) throw std::runtime_error("task out of range");
         if (a == b) return;
-        HIP_OK(hipStreamSynchronize(s->stream));
-        Task ta = s->read_task(a), tb = s->read_task(b);
-        // the slot table pointer and the mapped completion flag belong to the index, not the task
-        auto keep_index = [](Task& x, Task& y) {
-            std::swap(x.slots, y.slots);
-            std::swap(x.host_flag, y.host_flag);
-        };
-        keep_index(ta, tb);
-        HIP_OK(hipMemcpy(s->tasks_d + a, &tb, sizeof(Task), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(s->tasks_d + b, &ta, sizeof(Task), hipMemcpyHostToDevice));
+        if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
+        hipLaunchKernelGGL(k_task_swap, dim3(1), dim3(64), 0, s->stream, s->tasks_d, a, b, s->chunk_ptrs_d,
+                           s->n_chunks_d);
+        HIP_OK(hipGetLastError());
         std::swap(s->tasks_h[a], s->tasks_h[b]);
-        keep_index(s->tasks_h[a], s->tasks_h[b]);
+        std::swap(s->tasks_h[a].slots, s->tasks_h[b].slots);
+        std::swap(s->tasks_h[a].host_flag, s->tasks_h[b].host_flag);
         std::swap(s->th[a], s->th[b]);
-        s->upload_chunks(a);
-        s->upload_chunks(b);
     });
 }
 

@@ -12,8 +12,9 @@
 //   predictions, then extra_trees trees on all of its rows, one time bucket) take one of K task
 //   slots of a multi-task device session over the same panel.  The refit thread grows one tree
 //   for every slot in use per round (one launch per kernel for all of them); requests join at the
-//   next round when a slot is free and leave when their trees are done, publishing a new version
-//   of their ensemble.  Predict launches run on their own stream and overlap the refit rounds.
+//   next round when a slot is free and leave when their trees are done; a publishing thread then
+//   builds the new version of their ensemble (device forest and host snapshot) while the next
+//   rounds run.  Predict launches run on their own stream and overlap the refit rounds.
 // * Unbatched mode (the baseline) serves the same requests one at a time in arrival order with
 //   the same kernels: one predict per launch, one refit grown alone to completion.
 //
@@ -139,6 +140,7 @@ Server::Server(const std::vector<const Booster*>& models, const double* X, int n
     if (batched_) {
         threads_.emplace_back([this] { predict_loop(); });
         threads_.emplace_back([this] { refit_loop(); });
+        threads_.emplace_back([this] { publish_loop(); });
     } else {
         threads_.emplace_back([this] { unbatched_loop(); });
     }
@@ -150,6 +152,7 @@ Server::~Server() {
         stop_ = true;
     }
     cv_work_.notify_all();
+    cv_pub_.notify_all();
     for (auto& t : threads_) t.join();
     std::lock_guard<std::mutex> lk(mu_);
     for (auto& kv : reqs_) {
@@ -441,17 +444,14 @@ void Server::refit_loop() {
         if (!failed.empty()) finish(failed, err);
         if (n_active == 0) continue;
         std::vector<std::shared_ptr<Req>> done;
-        double t_pub = 0.0;
+        std::vector<Slot> finished;
         auto t_round = Clock::now();
         try {
             refit_round(n_active);
             t_round = Clock::now();
             for (int i = n_active - 1; i >= 0; --i) {   // retire finished refits, keep slots packed
                 if ((int)slots_[i].trees.size() < slots_[i].req->extra_trees) continue;
-                const auto tp = Clock::now();
-                publish_refit(slots_[i]);
-                t_pub += secs(tp, Clock::now());
-                done.push_back(slots_[i].req);
+                finished.push_back(std::move(slots_[i]));
                 const int last = n_active - 1;
                 if (i != last) {
                     dev_check(tgbm_task_swap(sess_, i, last));
@@ -470,13 +470,41 @@ void Server::refit_loop() {
         {
             std::lock_guard<std::mutex> lk(mu_);
             st_.refit_rounds += 1;
-            st_.refit_requests += (long long)done.size();
+            st_.refit_requests += (long long)(done.size() + finished.size());
             st_.refit_busy_s += secs(t, Clock::now());
             st_.refit_admit_s += secs(t, t_admit);
             st_.refit_round_s += secs(t_admit, t_round);
-            st_.refit_publish_s += t_pub;
+            for (auto& sl : finished) pubq_.push_back(std::move(sl));
         }
+        if (!finished.empty()) cv_pub_.notify_one();
         if (!done.empty()) finish(done, err);
+    }
+}
+
+// Publishing thread: finished refits become new versions in the order their trees were done.
+void Server::publish_loop() {
+    for (;;) {
+        std::vector<Slot> batch;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_pub_.wait(lk, [&] { return stop_ || !pubq_.empty(); });
+            if (pubq_.empty()) return;   // stopped, nothing left to publish
+            batch.swap(pubq_);
+        }
+        for (auto& sl : batch) {
+            std::string err;
+            const auto tp = Clock::now();
+            try {
+                publish_refit(sl);
+            } catch (const std::exception& e) {
+                err = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                st_.refit_publish_s += secs(tp, Clock::now());
+            }
+            finish({sl.req}, err);
+        }
     }
 }
 

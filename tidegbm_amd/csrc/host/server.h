@@ -80,6 +80,7 @@ private:
     void refit_round(int n_active);
     void predict_loop();
     void refit_loop();
+    void publish_loop();
     void unbatched_loop();
 
     bool batched_;
@@ -93,7 +94,8 @@ private:
     std::vector<ModelState> models_;
     std::vector<Slot> slots_;
     std::mutex mu_;
-    std::condition_variable cv_work_, cv_done_;
+    std::condition_variable cv_work_, cv_done_, cv_pub_;
+    std::vector<Slot> pubq_;   // finished refits for the publishing thread
     std::deque<std::shared_ptr<Req>> pq_, rq_, uq_;
     std::map<long long, std::shared_ptr<Req>> reqs_;
     long long next_id_ = 1;
