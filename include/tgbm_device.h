@@ -251,6 +251,13 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
  * window (the same sums as tgbm_predict_batch), on the session stream: a refit's starting
  * predictions without a host round trip.  Call after tgbm_task_bind. */
 int tgbm_task_predict_forest(tgbm_session* s, int task, const tgbm_forest* f);
+/* One call for a refit slot (reference Booster::update on a row window, booster.cpp:844-884):
+ * the effect of tgbm_task_bind(task, row0, n_rows), tgbm_task_predict_forest(task, f) and
+ * tgbm_task_set_rows(task, y, NULL, NULL, NULL, NULL, 1, NULL, NULL), all queued on the session
+ * stream with no host wait (y is staged through pinned host memory of the session, so the
+ * caller may reuse it on return).  No reference counterpart: the serving loop's admission path. */
+int tgbm_task_refit_window(tgbm_session* s, int task, int row0, int n_rows, const double* y,
+                           const tgbm_forest* f);
 
 /* ---- validation set scored tree by tree during a fit (reference Booster::fit with X_eval,
  * booster.cpp:459-478 and :707-732) ----

@@ -24,7 +24,7 @@ struct SNode {
 };
 static_assert(sizeof(SNode) == 8, "SNode layout");
 
-constexpr int SERVE_TREES_PER_CHUNK = 64;   // trees staged in LDS at a time by the serving kernel
+constexpr int SERVE_TREES_PER_CHUNK = 32;   // trees staged in LDS at a time by the serving kernel
 
 }  // namespace tgbm
 

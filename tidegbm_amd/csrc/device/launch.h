@@ -14,6 +14,11 @@ void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, i
 // least scratch_bytes owned by the session.
 void session_task_window(tgbm_session* s, int task, double** pred, int* row0, int* n_rows, hipStream_t* stream,
                          void** scratch, size_t scratch_bytes);
+// Rebind a task to a refit window with its targets (one time bucket, no weights, no groups),
+// queued on the session stream without a wait; returns the task's prediction array and the
+// stream for the window's base predictions.
+void session_refit_window(tgbm_session* s, int task, int row0, int n_rows, const double* y, double** pred,
+                          hipStream_t* stream);
 
 void init_kernel_attributes();
 void launch_bin(hipStream_t st, const double* X, int n, int F, const double* edges,

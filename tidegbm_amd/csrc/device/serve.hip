@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -76,21 +77,10 @@ size_t serve_lds(int fp, int chunk_nodes) {
     return b + sizeof(SNode) * (size_t)chunk_nodes;
 }
 
-__global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, int n_req, int max_fp) {
+// Block `blk` of request R: rows blk * SERVE_ROWS .. of the request.
+__device__ __forceinline__ void serve_block(const SReq& R, int blk, int max_fp) {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int s_req;
-    if (threadIdx.x == 0) {
-        int lo = 0, hi = n_req - 1;   // last request whose first block is <= blockIdx.x
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (reqs[mid].blk0 <= (int)blockIdx.x) lo = mid;
-            else hi = mid - 1;
-        }
-        s_req = lo;
-    }
-    __syncthreads();
-    const SReq R = reqs[s_req];
-    const int r0 = ((int)blockIdx.x - R.blk0) * SERVE_ROWS;
+    const int r0 = blk * SERVE_ROWS;
     const int nr = min(SERVE_ROWS, R.n - r0);
     const int max_rb = serve_row_bytes(max_fp);
     unsigned char* lbins = smem;
@@ -187,6 +177,47 @@ __global__ __launch_bounds__(256) void k_serve(const SReq* __restrict__ reqs, in
     if (g == 0 && r < nr) R.out[r0 + r] = s;
 }
 
+// A batch of requests: block b serves the last request whose first block is <= b.
+__global__ __launch_bounds__(256, 8) void k_serve(const SReq* __restrict__ reqs, int n_req, int max_fp) {
+    __shared__ int s_req;
+    if (threadIdx.x == 0) {
+        int lo = 0, hi = n_req - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (reqs[mid].blk0 <= (int)blockIdx.x) lo = mid;
+            else hi = mid - 1;
+        }
+        s_req = lo;
+    }
+    __syncthreads();
+    const SReq R = reqs[s_req];
+    serve_block(R, (int)blockIdx.x - R.blk0, max_fp);
+}
+
+// One request passed by value (no descriptor upload, so the caller need not wait for the launch).
+__global__ __launch_bounds__(256, 8) void k_serve_one(SReq R, int max_fp) { serve_block(R, (int)blockIdx.x, max_fp); }
+
+// A batch of at most SERVE_ARG_REQS requests whose table travels in the kernel arguments and whose
+// row ids and scores live in mapped host memory: the batch is one launch, with no copy before or
+// after it (each copy would wait for GPU resources of its own behind concurrent work).
+constexpr int SERVE_ARG_REQS = 24;
+struct SBatch {
+    SReq r[SERVE_ARG_REQS];
+    int n_req;
+};
+__global__ __launch_bounds__(256, 8) void k_serve_batch(const SBatch B, int max_fp) {
+    // the block's request (the last whose first block is <= blockIdx.x), by constant indices
+    int lo = 0;
+#pragma unroll
+    for (int k = 1; k < SERVE_ARG_REQS; ++k)
+        if (k < B.n_req && B.r[k].blk0 <= (int)blockIdx.x) lo = k;
+    SReq R = B.r[0];
+#pragma unroll
+    for (int k = 1; k < SERVE_ARG_REQS; ++k)
+        if (k == lo) R = B.r[k];
+    serve_block(R, (int)blockIdx.x - R.blk0, max_fp);
+}
+
 template <class Fn>
 int sguard(Fn&& fn) {
     try {
@@ -210,7 +241,8 @@ size_t item_bytes(const tgbm_forest* f, int from_x, int n) {
 struct tgbm_predictor {
     hipStream_t stream = nullptr;
     char* dev = nullptr;
-    char* host = nullptr;   // pinned
+    char* host = nullptr;       // pinned, mapped
+    char* host_dev = nullptr;   // its device address
     size_t cap = 0;
     int lds_ok = 0;
     ~tgbm_predictor() {
@@ -238,8 +270,11 @@ int tgbm_predictor_create(tgbm_predictor** out, size_t staging_bytes) {
         p->cap = std::max<size_t>(staging_bytes, 1 << 20);
         SHIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
         SHIP(hipMalloc(&p->dev, p->cap));
-        SHIP(hipHostMalloc((void**)&p->host, p->cap, hipHostMallocDefault));
+        SHIP(hipHostMalloc((void**)&p->host, p->cap, hipHostMallocMapped));
+        SHIP(hipHostGetDevicePointer((void**)&p->host_dev, p->host, 0));
         SHIP(hipFuncSetAttribute((const void*)k_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int)SERVE_MAX_LDS));
+        SHIP(hipFuncSetAttribute((const void*)k_serve_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)SERVE_MAX_LDS));
         *out = guard.release();
     });
 }
@@ -285,6 +320,44 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
         if (total > p->cap) throw std::runtime_error("predict batch exceeds the predictor's staging memory");
         const size_t lds = serve_lds(max_fp, max_chunk);
         if (lds > SERVE_MAX_LDS) throw std::runtime_error("ensemble chunk too large for LDS");
+        bool all_rows = n_items <= SERVE_ARG_REQS;
+        for (int i = 0; i < n_items && all_rows; ++i) all_rows = items[i].rows != nullptr || items[i].n == 0;
+        if (all_rows) {
+            // row ids in, scores out, both in the mapped staging; the table in the arguments
+            SBatch B;
+            std::memset(&B, 0, sizeof B);
+            B.n_req = n_items;
+            int blocks = 0;
+            for (int i = 0; i < n_items; ++i) {
+                const tgbm_predict_item& it = items[i];
+                const tgbm_forest* f = it.forest;
+                SReq& R = B.r[i];
+                R.nodes = f->snodes;
+                R.cmask = f->cmask;
+                R.tree_off = f->tree_off;
+                R.values = f->values;
+                R.lr = f->lr;
+                R.init = f->init;
+                R.n_trees = f->n_trees;
+                R.nb = f->nb;
+                R.fp = f->fp;
+                R.n = it.n;
+                R.blk0 = blocks;
+                R.depth = f->numeric_only ? std::max(f->max_depth, 1) : 0;
+                blocks += (it.n + SERVE_ROWS - 1) / SERVE_ROWS;
+                R.out = (double*)(p->host_dev + off[i]);
+                R.bins = pbins;
+                R.rows = (const int*)(p->host_dev + in_off[i]);
+                if (it.n) std::memcpy(p->host + in_off[i], it.rows, sizeof(int) * (size_t)it.n);
+            }
+            if (blocks > 0)
+                hipLaunchKernelGGL(k_serve_batch, dim3(blocks), dim3(256), lds, p->stream, B, max_fp);
+            SHIP(hipGetLastError());
+            SHIP(hipStreamSynchronize(p->stream));
+            for (int i = 0; i < n_items; ++i)
+                if (items[i].n > 0) std::memcpy(items[i].out, p->host + off[i], sizeof(double) * (size_t)items[i].n);
+            return;
+        }
         // ---- pack inputs on the host (pinned), one upload
         std::vector<SReq> reqs(n_items);
         int blocks = 0;
@@ -340,6 +413,45 @@ int tgbm_predict_batch(tgbm_predictor* p, const tgbm_session* panel, const tgbm_
         SHIP(hipStreamSynchronize(p->stream));
         for (int i = 0; i < n_items; ++i)
             if (items[i].n > 0) std::memcpy(items[i].out, p->host + off[i], sizeof(double) * (size_t)items[i].n);
+    });
+}
+
+int tgbm_task_refit_window(tgbm_session* s, int task, int row0, int n_rows, const double* y, const tgbm_forest* f) {
+    return sguard([&] {
+        if (!f) throw std::runtime_error("no ensemble");
+        if (!y) throw std::runtime_error("a refit window needs its targets");
+        const unsigned char* pbins = nullptr;
+        int pfp = 0, pn = 0, pF = 0;
+        session_panel_info(s, &pbins, &pfp, &pn, &pF);
+        if (f->F != pF) throw std::runtime_error("ensemble and panel feature counts differ");
+        const size_t lds = serve_lds(f->fp, f->max_chunk_nodes);
+        if (lds > SERVE_MAX_LDS) throw std::runtime_error("ensemble chunk too large for LDS");
+        static std::once_flag attr;
+        std::call_once(attr, [] {
+            SHIP(hipFuncSetAttribute((const void*)k_serve_one, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)SERVE_MAX_LDS));
+        });
+        double* pred = nullptr;
+        hipStream_t st = nullptr;
+        session_refit_window(s, task, row0, n_rows, y, &pred, &st);
+        SReq R;
+        R.nodes = f->snodes;
+        R.cmask = f->cmask;
+        R.tree_off = f->tree_off;
+        R.values = f->values;
+        R.lr = f->lr;
+        R.bins = pbins + (size_t)row0 * pfp;   // the window's rows are contiguous
+        R.rows = nullptr;
+        R.out = pred;
+        R.init = f->init;
+        R.n_trees = f->n_trees;
+        R.nb = f->nb;
+        R.fp = f->fp;
+        R.n = n_rows;
+        R.blk0 = 0;
+        R.depth = f->numeric_only ? std::max(f->max_depth, 1) : 0;
+        hipLaunchKernelGGL(k_serve_one, dim3((n_rows + SERVE_ROWS - 1) / SERVE_ROWS), dim3(256), lds, st, R, f->fp);
+        SHIP(hipGetLastError());
     });
 }
 

@@ -121,6 +121,12 @@ struct tgbm_session {
     std::map<std::vector<long long>, CachedGraph> graphs;
     long long graph_clock = 0;
     bool use_graphs = true;
+    // pinned staging for refit windows' targets: a ring of kStage buffers of the largest task
+    // capacity, allocated together on first use; an event marks the last copy out of each
+    static constexpr int kStage = 4;
+    double* stage[kStage] = {};
+    hipEvent_t stage_ev[kStage] = {};
+    int stage_next = 0;
     bool spec_graph = false;    // multi-leaf loops from a captured graph instead of direct launches
     // multi-leaf growth (one-task sessions with max_leaves <= 64): leaves per step (0: off), the
     // node capacity and partial slots it was allocated with, the steps of the captured loop (the
@@ -203,6 +209,10 @@ struct tgbm_session {
         for (auto& e : events) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
+        }
+        for (int k = 0; k < kStage; ++k) {
+            if (stage[k]) (void)hipHostFree(stage[k]);
+            if (stage_ev[k]) (void)hipEventDestroy(stage_ev[k]);
         }
         for (void* p : allocs) (void)hipFree(p);
         for (int k = 0; k < 2; ++k) {
@@ -326,6 +336,8 @@ struct tgbm_session {
     }
 };
 
+static void spec_resolve(tgbm_session* s);
+
 namespace tgbm {
 void session_panel_info(const tgbm_session* s, const unsigned char** bins_row, int* fp, int* n, int* F) {
     *bins_row = s->bins_row;
@@ -346,6 +358,93 @@ void session_task_window(tgbm_session* s, int task, double** pred, int* row0, in
         s->small_scratch_bytes = scratch_bytes;
     }
     *scratch = s->small_scratch;
+}
+
+// Device side of a refit window (tgbm_task_bind + tgbm_task_set_rows with one time bucket, no
+// weights and no groups): the descriptor's window fields, zero bucket ids and TORB sums, the
+// sweep-chunk list of 2,048-row chunks in segment 1 and its table entry.
+__global__ __launch_bounds__(256) void k_task_window(Task* tasks, int task, int row0, int n_rows, TorbChunk* chunks,
+                                                     int n_chunks, TorbChunk** chunk_ptrs, int* chunk_counts) {
+    Task& T = tasks[task];
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, gs = gridDim.x * blockDim.x;
+    const size_t tw = (size_t)(MAX_BUCKETS + 1) * MAX_NODES;
+    int* const bucket = (int*)T.bucket;
+    for (int i = gt; i < n_rows; i += gs) bucket[i] = 0;
+    for (size_t i = gt; i < tw; i += gs) {
+        T.tg[i] = 0;
+        if (T.th) T.th[i] = 0;
+        T.tc[i] = 0;
+    }
+    for (int k = gt; k < n_chunks; k += gs)
+        chunks[k] = TorbChunk{k * 2048, min(n_rows, (k + 1) * 2048), 1, 0};
+    if (gt == 0) {
+        T.n_rows = n_rows;
+        T.row0 = row0;
+        T.n_sample = n_rows;
+        T.contiguous = 1;
+        T.goss = 0;
+        T.n_buckets = 1;
+        T.has_groups = 0;
+        T.w = nullptr;
+        ((int*)T.cuts)[0] = 0;
+        chunk_ptrs[task] = chunks;
+        chunk_counts[task] = n_chunks;
+    }
+}
+
+// Host side of a refit window: the task's bookkeeping, the device fields (k_task_window) and the
+// targets (through the task's pinned staging), all queued on the session stream without a wait.
+// Returns the task's prediction array and the stream, for the caller to fill the window's base
+// predictions behind them.
+void session_refit_window(tgbm_session* s, int task, int row0, int n_rows, const double* y, double** pred,
+                          hipStream_t* stream) {
+    spec_resolve(s);
+    if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+    TaskHost& H = s->th[task];
+    if (n_rows < 1 || n_rows > H.cap_rows) throw std::runtime_error("task rows exceed the task's capacity");
+    if (row0 < 0 || (long long)row0 + n_rows > s->n) throw std::runtime_error("task row window outside the shared matrix");
+    if (!s->pending.empty()) throw std::runtime_error("collect the pending tree ends first");
+    const int n_chunks = (n_rows + 2047) / 2048;
+    if (n_chunks > H.chunks_cap) {
+        HIP_OK(hipStreamSynchronize(s->stream));   // the old list may still be read
+        H.chunks_d = s->alloc<TorbChunk>(n_chunks);
+        H.chunks_cap = n_chunks;
+    }
+    if (!s->stage[0]) {
+        int cap = 1;
+        for (auto& x : s->th) cap = std::max(cap, x.cap_rows);
+        for (int k = 0; k < tgbm_session::kStage; ++k) {
+            HIP_OK(hipHostMalloc((void**)&s->stage[k], sizeof(double) * cap, hipHostMallocDefault));
+            HIP_OK(hipEventCreateWithFlags(&s->stage_ev[k], hipEventDisableTiming));
+        }
+    }
+    const int sk = s->stage_next;
+    s->stage_next = (sk + 1) % tgbm_session::kStage;
+    double* const ypin = s->stage[sk];
+    HIP_OK(hipEventSynchronize(s->stage_ev[sk]));   // that buffer's last copy has run
+    H.n_rows = n_rows;
+    H.row0 = row0;
+    H.has_sample = false;
+    H.n_sample = n_rows;
+    H.goss = false;
+    H.n_buckets = 1;
+    H.chunks.clear();
+    for (int k = 0; k < n_chunks; ++k) H.chunks.push_back(TorbChunk{k * 2048, std::min(n_rows, (k + 1) * 2048), 1, 0});
+    s->max_chunks = 1;
+    for (auto& x : s->th) s->max_chunks = std::max(s->max_chunks, (int)x.chunks.size());
+    Task& T = s->tasks_h[task];
+    T.n_rows = n_rows;
+    T.row0 = row0;
+    T.w = nullptr;
+    const int blocks = std::min(64, (std::max(n_rows, (MAX_BUCKETS + 1) * MAX_NODES) + 255) / 256);
+    hipLaunchKernelGGL(k_task_window, dim3(blocks), dim3(256), 0, s->stream, s->tasks_d, task, row0, n_rows, H.chunks_d,
+                       n_chunks, s->chunk_ptrs_d, s->n_chunks_d);
+    HIP_OK(hipGetLastError());
+    std::memcpy(ypin, y, sizeof(double) * n_rows);
+    HIP_OK(hipMemcpyAsync((void*)T.y, ypin, sizeof(double) * n_rows, hipMemcpyHostToDevice, s->stream));
+    HIP_OK(hipEventRecord(s->stage_ev[sk], s->stream));
+    *pred = T.pred;
+    *stream = s->stream;
 }
 }  // namespace tgbm
 

@@ -336,9 +336,8 @@ void Server::predict_loop() {
 // Bind a free slot to a refit request: the window's rows, its targets and the base version's
 // predictions (Booster::update: current model on the new rows, booster.cpp:859).
 void Server::start_refit(int slot, const std::shared_ptr<Req>& r) {
-    dev_check(tgbm_task_bind(sess_, slot, r->row0, r->n));   // also clears the slot's buckets
-    dev_check(tgbm_task_predict_forest(sess_, slot, r->base->forest.get()));
-    dev_check(tgbm_task_set_rows(sess_, slot, r->y.data(), nullptr, nullptr, nullptr, nullptr, 1, nullptr, nullptr));
+    // bind + base predictions + targets, queued without a wait (the next round follows them)
+    dev_check(tgbm_task_refit_window(sess_, slot, r->row0, r->n, r->y.data(), r->base->forest.get()));
     slots_[slot].req = r;
     slots_[slot].trees.clear();
     std::vector<double>().swap(r->y);   // inputs are on the device now
