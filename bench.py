@@ -300,10 +300,15 @@ def _serve_stream(p, seconds, rate, n_models, seed):
 
 
 def _serve_run(t, p, models, stream, batched, slots):
+    import gc
     import numpy as np
     srv = t.Server(models, p.X, predict_budget=2 << 30, refit_slots=slots, refit_max_rows=200000,
                    batched=batched)
     tickets = []
+    # the stream is submitted on time: no Python garbage-collection pause (after earlier workloads
+    # the heap is large) may hold requests back and release them in a burst
+    gc.collect()
+    gc.disable()
     t0 = time.perf_counter()
     for (at, kind, m, r0, n) in stream:
         while True:
@@ -327,6 +332,7 @@ def _serve_run(t, p, models, stream, batched, slots):
             if n <= 4096:
                 small.append(l)
     wall = time.perf_counter() - t0
+    gc.enable()
     st = srv.stats()
     srv.close()
     pct = lambda a, q: round(1e3 * float(np.percentile(a, q)), 3) if len(a) else None
