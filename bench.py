@@ -368,9 +368,11 @@ def run_serve(args, p):
     log(f"[serve] {len(stream)} requests over {args.serve_seconds}s: batched")
     b = _serve_run(t, p, models, stream, True, args.serve_slots)
     log(f"[serve] batched: {b}")
-    log("[serve] unbatched")
-    u = _serve_run(t, p, models, stream, False, 1)
-    log(f"[serve] unbatched: {u}")
+    u = None
+    if not args.serve_batched_only:
+        log("[serve] unbatched")
+        u = _serve_run(t, p, models, stream, False, 1)
+        log(f"[serve] unbatched: {u}")
     return {
         "workload": f"8 resident ensembles ({args.serve_trees} trees, depth 8, max_leaves 31, 128 features) over "
                     "the resident 1M x 128 binned panel; Poisson arrivals at 2,000 req/s for "
@@ -475,6 +477,8 @@ def main():
     ap.add_argument("--serve-trees", type=int, default=500, help="trees per served ensemble")
     ap.add_argument("--serve-slots", type=int, default=64, help="concurrent refits when batched")
     ap.add_argument("--wf-trees", type=int, default=100, help="trees per walk-forward fit")
+    ap.add_argument("--serve-batched-only", action="store_true",
+                    help="serving workload: run the batched server only (no unbatched baseline)")
     ap.add_argument("--wf-batched-only", action="store_true",
                     help="walk-forward workload: run the batched schedule only (no sequential fits, no speedup)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -42,7 +42,8 @@ extern thread_local std::string g_dev_err;  // defined in session.hip
                                      " at " #expr);                                          \
     } while (0)
 
-namespace {
+// Named (not anonymous) so that profilers can show the kernels' names.
+namespace tgbm_serve {
 
 constexpr int SERVE_ROWS = 64;
 constexpr int SERVE_LANES = 4;   // threads per row (256-thread blocks)
@@ -236,7 +237,9 @@ size_t item_bytes(const tgbm_forest* f, int from_x, int n) {
     return b;
 }
 
-}  // namespace
+}  // namespace tgbm_serve
+
+using namespace tgbm_serve;
 
 struct tgbm_predictor {
     hipStream_t stream = nullptr;
