@@ -158,6 +158,13 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
                      int n_pairs, const tgbm_grow_params* p, tgbm_cand* out);
 int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d,
                      const tgbm_grow_params* p, int* left_id);
+/* tgbm_apply_split followed by tgbm_leaf_scan_results of both new children, with one wait: the
+ * split of the reference's best-first loop (tree.cpp:445-456) and the candidates of its two
+ * children (tree.cpp:457-461).  gain_f / thr_f / ml_f hold 2F entries (left child's F, then the
+ * right's), energy and n_rows 2.  left_id = -1 when the decision named no splittable leaf. */
+int tgbm_apply_split_fetch(tgbm_session* s, int task, const tgbm_decision* d,
+                           const tgbm_grow_params* p, int* left_id, double* gain_f, int* thr_f,
+                           int* ml_f, double* energy, int* n_rows);
 
 /* Finish every active task's tree: leaf ids for all rows, leaf refit and TORB prefix values
  * (booster.cpp:589-648), pred += lr * prefix (booster.cpp:655-664).  Writes the tree of the

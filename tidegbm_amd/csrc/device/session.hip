@@ -111,6 +111,11 @@ struct tgbm_session {
     int* ph_c = nullptr;
     long long* ph_h = nullptr;
     FCand* pair_out_d = nullptr;
+    // host-driven growth: a mapped pinned buffer the device writes a split's results into (one
+    // wait per split), and the row count of every node grown so far (-1: unknown)
+    char* rb_h = nullptr;
+    char* rb_d = nullptr;
+    std::vector<int> node_rows_h;
     // captured device-driven split loops (one graph launch per tree), keyed by what the capture
     // baked in: largest sample, leaf budget, histogram variant and the active task range
     struct CachedGraph {
@@ -202,6 +207,7 @@ struct tgbm_session {
         if (stream) (void)hipStreamSynchronize(stream);
         if (flush_buf) (void)hipFree(flush_buf);
         if (flag_h) (void)hipHostFree(flag_h);
+        if (rb_h) (void)hipHostFree(rb_h);
         for (auto& kv : graphs) {
             (void)hipGraphExecDestroy(kv.second.ge);
             (void)hipGraphDestroy(kv.second.g);
@@ -1104,6 +1110,10 @@ __global__ __launch_bounds__(256) void k_flush_cache(unsigned long long* buf, si
 static void tree_begin_impl(tgbm_session* s, const tgbm_grow_params* p, bool energy, bool device_mode) {
     spec_resolve(s);
     const GrowP P = s->growp(p);
+    {   // the root's rows (one active task); children are learned from their splits' readbacks
+        const TaskHost& H = s->th[s->a0];
+        s->node_rows_h.assign(1, s->an == 1 ? (H.has_sample ? H.n_sample : H.n_rows) : -1);
+    }
     s->spec_tree = false;
     s->spec_launched = false;
     launch_tree_start(s->stream, s->S, s->td(), s->an, s->max_rows(), s->max_nodes);
@@ -1373,10 +1383,15 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
         HIP_OK(hipMemsetAsync(s->ph_g, 0, sizeof(long long) * n_pairs * NBIN, s->stream));
         HIP_OK(hipMemsetAsync(s->ph_c, 0, sizeof(int) * n_pairs * NBIN, s->stream));
         HIP_OK(hipMemsetAsync(s->ph_h, 0, sizeof(long long) * n_pairs * NBIN, s->stream));
-        DNode nd;
-        HIP_OK(hipStreamSynchronize(s->stream));
-        HIP_OK(hipMemcpy(&nd, s->tasks_h[task].nodes + node, sizeof nd, hipMemcpyDeviceToHost));
-        launch_pair(s->stream, s->S, s->tasks_d, task, node, nd.seg_count, s->pair_f_d, n_pairs, s->ph_g,
+        // the node's row count sizes the launch: known from its split's readback, else read back
+        int count = node < (int)s->node_rows_h.size() ? s->node_rows_h[node] : -1;
+        if (count < 0) {
+            DNode nd;
+            HIP_OK(hipStreamSynchronize(s->stream));
+            HIP_OK(hipMemcpy(&nd, s->tasks_h[task].nodes + node, sizeof nd, hipMemcpyDeviceToHost));
+            count = nd.seg_count;
+        }
+        launch_pair(s->stream, s->S, s->tasks_d, task, node, count, s->pair_f_d, n_pairs, s->ph_g,
                     s->ph_c, s->ph_h, s->hconst, s->growp(p), s->pair_out_d);
         HIP_OK(hipGetLastError());
         std::vector<FCand> r(n_pairs);
@@ -1391,6 +1406,111 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
             out[k].missing_go_left = r[k].ml;
             out[k].valid = r[k].thr >= 0;
         }
+    });
+}
+
+}  // extern "C"
+
+// Host-driven split results: the energy (sum of (g 2^s_e)^2) of the new children L, L + 1
+// (L = n_nodes - 2), then their scan results into the mapped readback buffer.
+__global__ __launch_bounds__(256) void k_child_energy(Task* tasks, int task) {
+    Task& T = tasks[task];
+    if (T.done) return;
+    const int node = T.n_nodes - 2 + (int)blockIdx.y;
+    const DNode& n = T.nodes[node];
+    const int* rows = node_rows(T, n) + n.seg_start;
+    long long acc = 0;
+    const double sc = T.goss_scale;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n.seg_count; k += gridDim.x * blockDim.x) {
+        const int i = rows[k];
+        double g = T.g[i];
+        if (T.goss && T.scaled && T.scaled[i]) g *= sc;
+        acc += __double2ll_rn(ldexp(g * g, T.s_e));
+    }
+    acc = wave_sum_ll(acc);
+    __shared__ long long sacc[4];
+    if ((threadIdx.x & 63) == 0) sacc[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long long sum = sacc[0] + sacc[1] + sacc[2] + sacc[3];
+        if (sum) atomicAdd((unsigned long long*)&T.energy[node], (unsigned long long)sum);
+    }
+}
+
+// Readback layout: int {done, L, s_e, rows_L, rows_R, pad}, long long energy[2], FCand[2][F].
+__global__ __launch_bounds__(256) void k_split_readback(const Task* tasks, int task, int F, char* out) {
+    const Task& T = tasks[task];
+    int* hdr = (int*)out;
+    long long* en = (long long*)(out + 24);
+    FCand* fc = (FCand*)(out + 40);
+    const int done = T.done;
+    const int L = T.n_nodes - 2;
+    if (threadIdx.x == 0) {
+        hdr[0] = done;
+        hdr[1] = L;
+        hdr[2] = T.s_e;
+        hdr[3] = done ? 0 : T.nodes[L].v[N_NS];
+        hdr[4] = done ? 0 : T.nodes[L + 1].v[N_NS];
+        en[0] = done ? 0 : T.energy[L];
+        en[1] = done ? 0 : T.energy[L + 1];
+    }
+    if (done) return;
+    for (int j = threadIdx.x; j < 2 * F; j += blockDim.x) fc[j] = T.fcand[(size_t)L * F + j];
+}
+
+extern "C" {
+
+int tgbm_apply_split_fetch(tgbm_session* s, int task, const tgbm_decision* d, const tgbm_grow_params* p,
+                           int* left_id, double* gain_f, int* thr_f, int* ml_f, double* energy, int* n_rows) {
+    return dev_guard([&] {
+        spec_resolve(s);
+        if (s->an != 1 || task != s->a0)
+            throw std::runtime_error("host-driven growth needs exactly one active task, the one named");
+        const int F = s->F;
+        if (!s->rb_h) {
+            HIP_OK(hipHostMalloc((void**)&s->rb_h, 40 + sizeof(FCand) * 2 * (size_t)F, hipHostMallocMapped));
+            HIP_OK(hipHostGetDevicePointer((void**)&s->rb_d, s->rb_h, 0));
+        }
+        struct {
+            int leaf, type, feat, feat2, thr, mgl;
+            double gain;
+        } cur{d->leaf, d->type, d->feature, d->feature2, d->threshold, d->missing_go_left, d->gain};
+        static_assert(offsetof(Task, cur_gain) - offsetof(Task, cur_leaf) == 24, "layout");
+        HIP_OK(hipMemcpyAsync(reinterpret_cast<char*>(s->tasks_d + task) + offsetof(Task, cur_leaf), &cur,
+                              sizeof cur, hipMemcpyHostToDevice, s->stream));
+        const GrowP P = s->growp(p);
+        split_step(s, P, 1, p);
+        // the split leaf's rows bound the children's (all the task's rows when unknown)
+        int parent_rows = s->th[task].has_sample ? s->th[task].n_sample : s->th[task].n_rows;
+        if (d->leaf >= 0 && d->leaf < (int)s->node_rows_h.size() && s->node_rows_h[d->leaf] >= 0)
+            parent_rows = s->node_rows_h[d->leaf];
+        const int eb = std::max(1, std::min(256, (parent_rows + 1023) / 1024));
+        hipLaunchKernelGGL(k_child_energy, dim3(eb, 2), dim3(256), 0, s->stream, s->tasks_d, task);
+        hipLaunchKernelGGL(k_split_readback, dim3(1), dim3(256), 0, s->stream, (const Task*)s->tasks_d, task, F,
+                           s->rb_d);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(s->stream));
+        const int* hdr = (const int*)s->rb_h;
+        if (hdr[0]) {
+            *left_id = -1;
+            return;
+        }
+        const int L = hdr[1];
+        const long long* en = (const long long*)(s->rb_h + 24);
+        const FCand* fc = (const FCand*)(s->rb_h + 40);
+        for (int j = 0; j < 2 * F; ++j) {
+            gain_f[j] = fc[j].thr >= 0 ? fc[j].gain : -1e300;
+            thr_f[j] = fc[j].thr;
+            ml_f[j] = fc[j].ml;
+        }
+        for (int c = 0; c < 2; ++c) {
+            energy[c] = std::ldexp((double)en[c], -hdr[2]);
+            n_rows[c] = hdr[3 + c];
+        }
+        if ((int)s->node_rows_h.size() < L + 2) s->node_rows_h.resize(L + 2, -1);
+        s->node_rows_h[L] = hdr[3];
+        s->node_rows_h[L + 1] = hdr[4];
+        *left_id = L;
     });
 }
 

@@ -278,6 +278,8 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
     nodes[0].depth = 0;
     std::vector<int> ixg(1, -1);   // interaction group each node's path is bound to (tree.cpp:436-445)
     int pair_used = 0;
+    std::vector<double> ch_gain;    // a split's two children's scan results (left's F, then right's)
+    std::vector<int> ch_thr, ch_ml;
     for (int it = 0; it < c.max_leaves - 1; ++it) {
         int best_leaf = -1;
         tgbm_cand best{};
@@ -308,7 +310,13 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
         tgbm_decision d{best_leaf, best.type, best.feature, best.type == 1 ? best.feature2 : -1,
                         best.threshold, best.missing_go_left, best.gain};
         int left = -1;
-        dev_check(tgbm_apply_split(s, 0, &d, &gp, &left));
+        double ch_energy[2];
+        int ch_rows[2];
+        ch_gain.resize(2 * (size_t)F);
+        ch_thr.resize(2 * (size_t)F);
+        ch_ml.resize(2 * (size_t)F);
+        dev_check(tgbm_apply_split_fetch(s, 0, &d, &gp, &left, ch_gain.data(), ch_thr.data(), ch_ml.data(), ch_energy,
+                                         ch_rows));
         if (left < 0) break;
         nodes.resize(left + 2);
         int child_ixg = ixg[best_leaf];
@@ -319,7 +327,11 @@ void grow_host_driven(tgbm_session* s, const Config& c, int F, const tgbm_grow_p
         ixg[left] = ixg[left + 1] = child_ixg;
         for (int side = 0; side < 2; ++side) {
             LeafInfo& ch = nodes[left + side];
-            fetch_leaf(s, left + side, F, ch);
+            ch.gain.assign(ch_gain.begin() + (size_t)side * F, ch_gain.begin() + (size_t)(side + 1) * F);
+            ch.thr.assign(ch_thr.begin() + (size_t)side * F, ch_thr.begin() + (size_t)(side + 1) * F);
+            ch.ml.assign(ch_ml.begin() + (size_t)side * F, ch_ml.begin() + (size_t)(side + 1) * F);
+            ch.energy = ch_energy[side];
+            ch.rows = ch_rows[side];
             ch.depth = nodes[best_leaf].depth + 1;
             ch.leaf = true;
             ch.cached = false;
