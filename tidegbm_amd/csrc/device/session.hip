@@ -116,6 +116,11 @@ struct tgbm_session {
     char* rb_h = nullptr;
     char* rb_d = nullptr;
     std::vector<int> node_rows_h;
+    void ensure_readback() {
+        if (rb_h) return;
+        HIP_OK(hipHostMalloc((void**)&rb_h, 40 + sizeof(FCand) * 2 * (size_t)F, hipHostMallocMapped));
+        HIP_OK(hipHostGetDevicePointer((void**)&rb_d, rb_h, 0));
+    }
     // captured device-driven split loops (one graph launch per tree), keyed by what the capture
     // baked in: largest sample, leaf budget, histogram variant and the active task range
     struct CachedGraph {
@@ -455,6 +460,25 @@ void session_refit_window(tgbm_session* s, int task, int row0, int n_rows, const
 }  // namespace tgbm
 
 static void spec_resolve(tgbm_session* s);
+
+// One node's readback (layout of k_split_readback, the node in the L slot): its scan results,
+// energy, row count and the task's energy scale.
+__global__ __launch_bounds__(256) void k_node_readback(const Task* tasks, int task, int node, int F, char* out) {
+    const Task& T = tasks[task];
+    int* hdr = (int*)out;
+    long long* en = (long long*)(out + 24);
+    FCand* fc = (FCand*)(out + 40);
+    if (threadIdx.x == 0) {
+        hdr[0] = 0;
+        hdr[1] = node;
+        hdr[2] = T.s_e;
+        hdr[3] = T.nodes[node].v[N_NS];
+        hdr[4] = 0;
+        en[0] = T.energy[node];
+        en[1] = 0;
+    }
+    for (int j = threadIdx.x; j < F; j += blockDim.x) fc[j] = T.fcand[(size_t)node * F + j];
+}
 
 extern "C" {
 
@@ -1347,22 +1371,27 @@ int tgbm_leaf_scan_results(tgbm_session* s, int task, int node, double* gain_f, 
                            double* energy, int* n_rows) {
     return dev_guard([&] {
         spec_resolve(s);
-        const Task& T = s->tasks_h[task];
-        std::vector<FCand> fc(s->F);
+        if (task < 0 || task >= s->T) throw std::runtime_error("task out of range");
+        if (node < 0 || node >= MAX_NODES) throw std::runtime_error("node out of range");
+        const int F = s->F;
+        s->ensure_readback();
+        // the node's scan results, energy, row count and energy scale, written by the device into
+        // mapped host memory (one wait)
+        hipLaunchKernelGGL(k_node_readback, dim3(1), dim3(256), 0, s->stream, (const Task*)s->tasks_d, task, node, F,
+                           s->rb_d);
+        HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(s->stream));
-        HIP_OK(hipMemcpy(fc.data(), T.fcand + (size_t)node * s->F, sizeof(FCand) * s->F, hipMemcpyDeviceToHost));
-        for (int f = 0; f < s->F; ++f) {
+        const int* hdr = (const int*)s->rb_h;
+        const long long* en = (const long long*)(s->rb_h + 24);
+        const FCand* fc = (const FCand*)(s->rb_h + 40);
+        for (int f = 0; f < F; ++f) {
             gain_f[f] = fc[f].thr >= 0 ? fc[f].gain : -1e300;
             thr_f[f] = fc[f].thr;
             ml_f[f] = fc[f].ml;
         }
-        long long e = 0;
-        HIP_OK(hipMemcpy(&e, T.energy + node, sizeof e, hipMemcpyDeviceToHost));
-        const Task d = s->read_task(task);
-        DNode nd;
-        HIP_OK(hipMemcpy(&nd, T.nodes + node, sizeof nd, hipMemcpyDeviceToHost));
-        if (energy) *energy = std::ldexp((double)e, -d.s_e);
-        if (n_rows) *n_rows = nd.v[N_NS];
+        if (energy) *energy = std::ldexp((double)en[0], -hdr[2]);
+        if (n_rows) *n_rows = hdr[3];
+        if (node < (int)s->node_rows_h.size()) s->node_rows_h[node] = hdr[3];
     });
 }
 
@@ -1467,10 +1496,7 @@ int tgbm_apply_split_fetch(tgbm_session* s, int task, const tgbm_decision* d, co
         if (s->an != 1 || task != s->a0)
             throw std::runtime_error("host-driven growth needs exactly one active task, the one named");
         const int F = s->F;
-        if (!s->rb_h) {
-            HIP_OK(hipHostMalloc((void**)&s->rb_h, 40 + sizeof(FCand) * 2 * (size_t)F, hipHostMallocMapped));
-            HIP_OK(hipHostGetDevicePointer((void**)&s->rb_d, s->rb_h, 0));
-        }
+        s->ensure_readback();
         struct {
             int leaf, type, feat, feat2, thr, mgl;
             double gain;
