@@ -110,7 +110,6 @@ struct tgbm_session {
     long long* ph_g = nullptr;
     int* ph_c = nullptr;
     long long* ph_h = nullptr;
-    FCand* pair_out_d = nullptr;
     // host-driven growth: a mapped pinned buffer the device writes a split's results into (one
     // wait per split), and the row count of every node grown so far (-1: unknown)
     char* rb_h = nullptr;
@@ -118,7 +117,9 @@ struct tgbm_session {
     std::vector<int> node_rows_h;
     void ensure_readback() {
         if (rb_h) return;
-        HIP_OK(hipHostMalloc((void**)&rb_h, 40 + sizeof(FCand) * 2 * (size_t)F, hipHostMallocMapped));
+        // room for two nodes' candidates, or a PAIR search's (at most 64 pairs)
+        HIP_OK(hipHostMalloc((void**)&rb_h, 40 + sizeof(FCand) * std::max(2 * (size_t)F, (size_t)64),
+                             hipHostMallocMapped));
         HIP_OK(hipHostGetDevicePointer((void**)&rb_d, rb_h, 0));
     }
     // captured device-driven split loops (one graph launch per tree), keyed by what the capture
@@ -620,7 +621,6 @@ int tgbm_session_create(tgbm_session** out, int n_rows, int n_features, int n_bi
         s->ph_g = s->alloc<long long>(64 * NBIN);
         s->ph_c = s->alloc<int>(64 * NBIN);
         s->ph_h = s->alloc<long long>(64 * NBIN);
-        s->pair_out_d = s->alloc<FCand>(64);
         HIP_OK(hipStreamSynchronize(s->stream));
         *out = guard.release();
     });
@@ -1420,12 +1420,13 @@ int tgbm_pair_search(tgbm_session* s, int task, int node, const int* f1, const i
             HIP_OK(hipMemcpy(&nd, s->tasks_h[task].nodes + node, sizeof nd, hipMemcpyDeviceToHost));
             count = nd.seg_count;
         }
+        // the pair scans write their results into the mapped readback buffer: one wait, no copy
+        s->ensure_readback();
         launch_pair(s->stream, s->S, s->tasks_d, task, node, count, s->pair_f_d, n_pairs, s->ph_g,
-                    s->ph_c, s->ph_h, s->hconst, s->growp(p), s->pair_out_d);
+                    s->ph_c, s->ph_h, s->hconst, s->growp(p), (FCand*)(s->rb_d + 40));
         HIP_OK(hipGetLastError());
-        std::vector<FCand> r(n_pairs);
-        HIP_OK(hipMemcpyAsync(r.data(), s->pair_out_d, sizeof(FCand) * n_pairs, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
+        const FCand* r = (const FCand*)(s->rb_h + 40);
         for (int k = 0; k < n_pairs; ++k) {
             out[k].gain = r[k].thr >= 0 ? r[k].gain : -1e300;
             out[k].feature = f1[k];
