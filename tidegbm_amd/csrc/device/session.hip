@@ -1641,9 +1641,11 @@ static void spec_resolve(tgbm_session* s) {
         HIP_OK(hipStreamSynchronize(s->stream));
         if (!all_finished(steps)) throw std::runtime_error("internal: multi-leaf growth did not finish the tree");
     }
-    // the next loop's steps: the most any of the last 32 trees needed (its splitting steps plus
-    // the selection that finds it finished), so a tree rarely needs the step-by-step path
-    s->spec_hist[s->spec_hist_n++ % 32] = steps + 1;
+    // the next loop's steps: the most splitting steps any of the last 32 trees needed, so a tree
+    // rarely needs the step-by-step path.  No step is added for a selection that finds the tree
+    // finished: the final numbering replays to the end itself (the pending candidates included)
+    // and reports an unfinished tree.
+    s->spec_hist[s->spec_hist_n++ % 32] = std::max(1, steps);
     int kmax = 1;
     for (int k = 0; k < std::min(s->spec_hist_n, 32); ++k) kmax = std::max(kmax, s->spec_hist[k]);
     s->spec_k = std::min(limit, kmax);
