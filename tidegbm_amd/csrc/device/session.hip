@@ -140,8 +140,8 @@ struct tgbm_session {
     int stage_next = 0;
     bool spec_graph = false;    // multi-leaf loops from a captured graph instead of direct launches
     // multi-leaf growth (one-task sessions with max_leaves <= 64): leaves per step (0: off), the
-    // node capacity and partial slots it was allocated with, the steps of the captured loop (the
-    // most a recent tree needed, plus one), whether the current tree grows this way, and a pinned
+    // node capacity and partial slots it was allocated with, the steps of the launched loop (the
+    // most splitting steps a recent tree needed), whether the current tree grows this way, and a pinned
     // copy of the tree's {done, steps}
     int n_cu = 256;         // compute units of the device
     int spec_width = 0;
@@ -1253,8 +1253,8 @@ static void spec_log_rows(tgbm_session* s) {
     s->log_rows[6].push_back(splits);   // splits the selections made
 }
 
-// Multi-leaf growth of the current tree.  The loop of spec_k steps (the most the last 32 trees
-// needed, plus one) is only launched here: the tree end is queued behind it and then waits for it
+// Multi-leaf growth of the current tree.  The loop of spec_k steps (the most splitting steps the
+// last 32 trees needed) is only launched here: the tree end is queued behind it and then waits for it
 // (tree_end_enqueue), so the device never waits for the host in between.  The steps go out as
 // direct launches: a captured graph of them was slower (1M x 128 fit: 1,282 against 1,225
 // trees/s), its launch call holding the host thread for most of the graph's run, with the device
