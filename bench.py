@@ -299,7 +299,10 @@ def _serve_stream(p, seconds, rate, n_models, seed):
     return out
 
 
-def _serve_run(t, p, models, stream, batched, slots):
+def _serve_run(t, p, models, stream, batched, slots, burst=0):
+    """Serves the stream on time and reports latencies.  With burst > 0 it then submits that many
+    refits of the stream's size distribution at once and reports the refits completed per second
+    while they drain (the refit capacity, above any offered rate)."""
     import gc
     import numpy as np
     srv = t.Server(models, p.X, predict_budget=2 << 30, refit_slots=slots, refit_max_rows=200000,
@@ -332,8 +335,20 @@ def _serve_run(t, p, models, stream, batched, slots):
             if n <= 4096:
                 small.append(l)
     wall = time.perf_counter() - t0
-    gc.enable()
     st = srv.stats()
+    cap = None
+    if burst:
+        rng = np.random.default_rng(12345)
+        reqs = []
+        for _ in range(burst):
+            n = int(rng.integers(20000, 200001))
+            reqs.append((int(rng.integers(len(models))), int(rng.integers(0, p.n - n + 1)), n))
+        tb = time.perf_counter()
+        tks = [srv.refit(m, r0, p.y[r0:r0 + n], extra_trees=50) for (m, r0, n) in reqs]
+        for tk in tks:
+            srv.wait(tk)
+        cap = round(burst / (time.perf_counter() - tb), 1)
+    gc.enable()
     srv.close()
     pct = lambda a, q: round(1e3 * float(np.percentile(a, q)), 3) if len(a) else None
     return {
@@ -345,6 +360,7 @@ def _serve_run(t, p, models, stream, batched, slots):
         "predict_rows_per_s": round(rows / wall, 1), "predict_launches": st["predict_launches"],
         "refit_rounds": st["refit_rounds"], "refit_admit_s": round(st["refit_admit_s"], 3),
         "refit_round_s": round(st["refit_round_s"], 3), "refit_publish_s": round(st["refit_publish_s"], 3),
+        "refit_burst": burst or None, "refit_capacity_per_s": cap,
     }
 
 
@@ -366,7 +382,7 @@ def run_serve(args, p):
     t_fit = time.perf_counter() - t_fit
     stream = _serve_stream(p, args.serve_seconds, 2000.0, len(models), seed=7)
     log(f"[serve] {len(stream)} requests over {args.serve_seconds}s: batched")
-    b = _serve_run(t, p, models, stream, True, args.serve_slots)
+    b = _serve_run(t, p, models, stream, True, args.serve_slots, burst=4 * args.serve_slots)
     log(f"[serve] batched: {b}")
     u = None
     if not args.serve_batched_only:
@@ -378,7 +394,8 @@ def run_serve(args, p):
                     "the resident 1M x 128 binned panel; Poisson arrivals at 2,000 req/s for "
                     f"{args.serve_seconds}s; 90% predicts of log-uniform 1..65,536 panel rows, 10% refits of "
                     "20k..200k rows x 50 trees; predict staging budget 2 GiB, "
-                    f"{args.serve_slots} refit slots of up to 200,000 rows",
+                    f"{args.serve_slots} refit slots of up to 200,000 rows; refit capacity: a burst of "
+                    f"{4 * args.serve_slots} such refits submitted at once after the stream (batched)",
         "ensembles_fit_s": round(t_fit, 3), "batched": b, "unbatched": u,
     }
 
