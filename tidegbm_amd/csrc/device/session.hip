@@ -171,8 +171,10 @@ struct tgbm_session {
     char* small_scratch = nullptr;
     size_t small_scratch_bytes = 0;
     // finished-tree staging (device + pinned host), sized for every task at the largest bucket count
-    char* export_d = nullptr;
-    char* export_h[2] = {nullptr, nullptr};   // two host slots: a tree is read while the next runs
+    // two mapped host slots (a tree is read while the next runs), written by k_export directly
+    // through their device addresses export_hd
+    char* export_h[2] = {nullptr, nullptr};
+    char* export_hd[2] = {nullptr, nullptr};
     hipEvent_t export_ev[2] = {nullptr, nullptr};
     size_t export_bytes = 0;
     struct PendingExport {
@@ -1573,7 +1575,7 @@ int tgbm_apply_split(tgbm_session* s, int task, const tgbm_decision* d, const tg
 }
 
 // Enqueue the end of every active task's tree: leaf table, TORB sums, prediction update, loss,
-// export into the next host slot (the copy completes in stream order; an event marks it).
+// export into the next host slot (k_export writes the mapped slot; an event marks its end).
 // The tree end: final numbering (multi-leaf trees), leaf ids, TORB, predictions, loss, export
 // into the tree's host slot.
 static void enqueue_end(tgbm_session* s, const tgbm_session::EndArgs& E) {
@@ -1585,9 +1587,8 @@ static void enqueue_end(tgbm_session* s, const tgbm_session::EndArgs& E) {
                     E.min_cover, E.iid, E.lr, E.pairwise ? 1 : 0);
     if (E.pairwise && E.max_groups > 0) launch_loss(s->stream, s->td(), s->an, E.max_groups);
     if (s->timing) s->toc(ep);
-    launch_export(s->stream, s->td(), s->an, E.L, s->export_d, E.pairwise ? 1 : 0);
+    launch_export(s->stream, s->td(), s->an, E.L, s->export_hd[E.slot], E.pairwise ? 1 : 0);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(s->export_h[E.slot], s->export_d, E.L.stride * s->an, hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipEventRecord(s->export_ev[E.slot], s->stream));
 }
 
@@ -1688,10 +1689,10 @@ static void tree_end_enqueue(tgbm_session* s, double learning_rate, double min_c
         for (int k = 0; k < 2; ++k) {
             if (s->export_h[k]) HIP_OK(hipHostFree(s->export_h[k]));
             s->export_h[k] = nullptr;
-            HIP_OK(hipHostMalloc((void**)&s->export_h[k], need, hipHostMallocDefault));
+            HIP_OK(hipHostMalloc((void**)&s->export_h[k], need, hipHostMallocMapped));
+            HIP_OK(hipHostGetDevicePointer((void**)&s->export_hd[k], s->export_h[k], 0));
             if (!s->export_ev[k]) HIP_OK(hipEventCreateWithFlags(&s->export_ev[k], hipEventDisableTiming));
         }
-        s->export_d = s->alloc<char>(need);
         s->export_bytes = need;
     }
     E.slot = s->next_slot;
