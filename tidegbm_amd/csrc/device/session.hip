@@ -374,6 +374,17 @@ void session_task_window(tgbm_session* s, int task, double** pred, int* row0, in
     *scratch = s->small_scratch;
 }
 
+// A GOSS sample's fields of the task descriptor, set in stream order from the selection's result
+// (gsmall = {rows selected, equal-|g| flag}).  Equal |g| values or an unexpected row count leave
+// them unchanged: the host then orders the rows itself (ties) or reports the error.
+__global__ void k_goss_commit(Task* T, const int* gsmall, int expect, double scale) {
+    if (threadIdx.x != 0 || gsmall[1] != 0 || gsmall[0] != expect) return;
+    T->n_sample = gsmall[0];
+    T->contiguous = 0;
+    T->goss = 1;
+    T->goss_scale = scale;
+}
+
 // Device side of a refit window (tgbm_task_bind + tgbm_task_set_rows with one time bucket, no
 // weights and no groups): the descriptor's window fields, zero bucket ids and TORB sums, the
 // sweep-chunk list of 2,048-row chunks in segment 1 and its table entry.
@@ -1104,6 +1115,11 @@ int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other
         HIP_OK(hipMemsetAsync(H.gsmall, 0, sizeof(int) * 2, s->stream));
         HIP_OK(launch_goss(s->stream, T.g, S, n_top, n_other, H.gperm, H.gk0, H.gk1, H.gv0, H.gv1, H.gkeep, sd,
                            T.idx, H.gsmall, H.gsmall + 1, H.gtmp, H.gtmp_bytes));
+        // the descriptor takes the sample in stream order (no synchronous field writes after the wait)
+        const double scale = (1.0 - goss_top) / std::max(goss_other, 1e-12);
+        hipLaunchKernelGGL(k_goss_commit, dim3(1), dim3(64), 0, s->stream, s->tasks_d + task, H.gsmall,
+                           n_top + n_other, scale);
+        HIP_OK(hipGetLastError());
         int res[2] = {0, 0};
         HIP_OK(hipMemcpyAsync(res, H.gsmall, sizeof res, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
@@ -1113,10 +1129,6 @@ int tgbm_task_goss(tgbm_session* s, int task, double goss_top, double goss_other
         H.has_sample = true;
         H.n_sample = res[0];
         H.goss = true;
-        s->set_field_int(task, offsetof(Task, n_sample), res[0]);
-        s->set_field_int(task, offsetof(Task, contiguous), 0);
-        s->set_field_int(task, offsetof(Task, goss), 1);
-        s->set_field_double(task, offsetof(Task, goss_scale), (1.0 - goss_top) / std::max(goss_other, 1e-12));
     });
 }
 
